@@ -1,0 +1,173 @@
+/* tdyn: MI355X-native batched tree dynamics -- the C ABI.
+ *
+ * Drop-in boundary for Simbody's tree-dynamics operator surface
+ * (SimbodyMatterSubsystem + RungeKuttaMersonIntegrator/TimeStepper) for
+ * batches of independent, unconstrained multibody instances.  The reference
+ * has no FFI: its surface is the C++ handle SimbodyMatterSubsystem used with a
+ * State.  Each entry point below names the reference member it replaces
+ * (paths relative to the Simbody source tree); the argument meaning, the
+ * frames and the q/u orderings are the reference's:
+ *   - bodies are numbered 1..nb in MobilizedBodyIndex order, Ground = 0;
+ *   - q and u slots are assigned in body order (SimbodyMatterSubsystemRep.cpp:
+ *     256-293); Ball/Free use quaternions (w,x,y,z) first, Free then the
+ *     translation (RigidBodyNodeSpec_Free.h:178-186); Ball/Free u are the
+ *     angular velocity in F then (Free) the linear velocity in F; Universal u
+ *     are the Euler-angle rates (RigidBodyNodeSpec_Universal.h:51-53);
+ *   - body forces are SpatialVec (torque, force), applied at the body origin,
+ *     expressed in Ground, one per body including Ground
+ *     (SimbodyMatterSubsystem.h:2210-2216);
+ *   - per-instance arrays are packed instance after instance; use
+ *     tdyn_batch_offsets() for where each instance starts.
+ *
+ * All data pointers are device pointers (hipMalloc'd, or a torch tensor's
+ * data_ptr()); every call takes the HIP stream it enqueues on.  A handle is
+ * not thread-safe: use one host thread per handle (the reference's State must
+ * not be realized concurrently either, SimbodyMatterSubsystem.h:2749-2755).
+ * All functions return TDYN_OK or an error code; tdyn_last_error() gives the
+ * message of the last failure on the calling thread.
+ */
+#ifndef TDYN_H
+#define TDYN_H
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes (the reference throws; SimbodyMatterSubsystem.cpp:247-261,
+ * SmallMatrixMixed.h:878-892, AbstractIntegratorRep.cpp:426-431). */
+#define TDYN_OK             0
+#define TDYN_E_ARG          1   /* bad argument (SimTK_APIARGCHECK) */
+#define TDYN_E_SIZE         2   /* size mismatch (SimTK_ERRCHK2) */
+#define TDYN_E_STAGE        3   /* call order violation (SimTK_STAGECHECK) */
+#define TDYN_E_SINGULAR     4   /* singular articulated inertia D */
+#define TDYN_E_STEP_FAILED  5   /* Integrator::StepFailed for some instance */
+#define TDYN_E_HIP          6   /* HIP runtime error */
+#define TDYN_E_NOMEM        7
+
+/* Mobilizer codes: MobilizedBody::Pin, Slider, Universal, Ball, Free. */
+#define TDYN_PIN       0
+#define TDYN_SLIDER    1
+#define TDYN_UNIVERSAL 2
+#define TDYN_BALL      3
+#define TDYN_FREE      4
+
+/* One tree (what MobilizedBody::<Type>(parent, X_PF, body, X_BM) calls build;
+ * MobilizedBody.cpp:783-808).  Host pointers. */
+typedef struct tdyn_tree_desc {
+    int32_t nb;                /* mobilized bodies (Ground excluded) */
+    const int32_t* parent;     /* [nb] parent body number, 0 = Ground, parent[b-1] < b */
+    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_FREE (Free only on Ground) */
+    const double* mass;        /* [nb] */
+    const double* com_B;       /* [nb][3] mass center in B */
+    const double* inertia_B;   /* [nb][6] inertia about Bo in B: xx yy zz xy xz yz */
+    const double* X_PF;        /* [nb][12] inboard frame on the parent: R row-major, p */
+    const double* X_BM;        /* [nb][12] outboard frame on the body */
+    double gravity[3];         /* Force::Gravity direction*magnitude (Force_Gravity.cpp:210-229) */
+} tdyn_tree_desc;
+
+typedef struct tdyn_batch tdyn_batch;
+
+/* Build a batch: n_instances instances, instance i uses
+ * trees[tree_of_instance[i]] (tree_of_instance NULL: all use trees[0]).
+ * Replaces SimbodyMatterSubsystem construction + realizeTopology
+ * (SimbodyMatterSubsystemRep.cpp:256-330). */
+int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* tree_of_instance,
+                      int n_instances, tdyn_batch** out);
+int tdyn_batch_destroy(tdyn_batch* b);
+
+/* info[0..7] = n_instances, total q, total u, total bodies (incl. Ground),
+ * max nq, max nu, max nb, n_trees. */
+int tdyn_batch_info(const tdyn_batch* b, int64_t* info);
+/* Host arrays of n_instances entries: where each instance starts in packed
+ * q, u and per-body ((nb+1) per instance) arrays. */
+int tdyn_batch_offsets(const tdyn_batch* b, int32_t* q_off, int32_t* u_off, int32_t* body_off);
+
+/* realizePositionKinematics + realizeVelocityKinematics
+ * (SimbodyMatterSubsystem.h:2691, 2705) followed by reads of the body
+ * transforms X_GB [12 per body], spatial velocities V_GB [6 per body], qdot
+ * (calcQDot), and energies [KE, PE] per instance
+ * (SimbodyMatterSubsystem.h:438 calcKineticEnergy, Force_Gravity.cpp:555).
+ * Any output may be NULL. */
+int tdyn_realize_kinematics(tdyn_batch* b, const double* q, const double* u, double* X_GB,
+                            double* V_GB, double* qdot, double* energies, hipStream_t s);
+
+/* realize(state, Stage::Acceleration) of a MultibodySystem with
+ * Force::Gravity and constant applied body forces fapp (NULL: none)
+ * (SimbodyMatterSubsystemRep.cpp:1527-1555): udot, qdot (NULL ok). */
+int tdyn_realize_acceleration(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                              double* udot, double* qdot, hipStream_t s);
+
+/* calcAccelerationIgnoringConstraints(state, fMob, F, udot, A_GB)
+ * (SimbodyMatterSubsystem.h:2171-2176, .cpp:199-235) plus qdotdot
+ * (calcQDotDot).  fmob: packed u layout, F: per-body.  A_GB, qdotdot may be NULL. */
+int tdyn_calc_acceleration_ignoring_constraints(tdyn_batch* b, const double* q, const double* u,
+                                                const double* fmob, const double* F, double* udot,
+                                                double* A_GB, double* qdotdot, hipStream_t s);
+
+/* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
+ * .cpp:431-471). */
+int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV,
+                           hipStream_t s);
+
+/* calcResidualForceIgnoringConstraints(state, fMob, F, udot, resid)
+ * (SimbodyMatterSubsystem.h:2234-2239, .cpp:236-304).  NULL fmob, F or
+ * udot mean all-zero, the reference's zero-length Vectors. */
+int tdyn_calc_residual_force_ignoring_constraints(tdyn_batch* b, const double* q, const double* u,
+                                                  const double* fmob, const double* F,
+                                                  const double* udot, double* resid, hipStream_t s);
+
+/* realizeArticulatedBodyInertias + getArticulatedBodyInertia (and the P+
+ * "as felt through the mobilizer") for every body
+ * (SimbodyMatterSubsystem.h:2730, 2867).  21 doubles per body: mass block
+ * M (xx yy zz xy xz yz), inertia block J (same order), mass-moment block F
+ * (row-major 3x3); P = [J F; F^T M] about Bo in G.  PPlus may be NULL. */
+int tdyn_calc_articulated_body_inertias(tdyn_batch* b, const double* q, double* P, double* PPlus,
+                                        hipStream_t s);
+
+/* calcMobilizerReactionForces(state, FM_G) (SimbodyMatterSubsystemRep.cpp:
+ * 5786-5826) for the given applied forces: reaction at each M frame origin,
+ * expressed in G, 6 per body. */
+int tdyn_calc_mobilizer_reaction_forces(tdyn_batch* b, const double* q, const double* u,
+                                        const double* fmob, const double* F, double* FM,
+                                        hipStream_t s);
+
+/* ---- Integration: RungeKuttaMersonIntegrator + TimeStepper::stepTo ------- */
+typedef struct tdyn_integrator_cfg {
+    double accuracy;             /* Integrator::setAccuracy (<=0: 1e-3 default) */
+    double constraint_tolerance; /* setConstraintTolerance (<=0: accuracy/10) */
+    double fixed_step;           /* setFixedStepSize (<=0: adaptive) */
+    double initial_step, min_step, max_step;   /* <=0: unset */
+    double time_scale;           /* System default time scale (<=0: 0.1) */
+    int32_t max_attempts;        /* per-instance attempt budget (<=0: 1e8) */
+    int32_t precision;           /* 0 = FP64 (the only mode of this build) */
+} tdyn_integrator_cfg;
+
+/* TimeStepper::initialize (Integrator.cpp:333-435): load q,u (packed),
+ * constant applied body forces fapp (NULL: none), start time t0 and
+ * per-instance final times t_end (device, NULL: all t_end_all).  The state
+ * stays resident on the device until tdyn_rollout_get. */
+int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                       double t0, const double* t_end, double t_end_all,
+                       const tdyn_integrator_cfg* cfg, hipStream_t s);
+/* Advance every unfinished instance by at most max_steps accepted steps
+ * (TimeStepper::stepTo(t_end) with Integrator::setFinalTime(t_end),
+ * AbstractIntegratorRep.cpp:216-436).  Resumable: calling again continues. */
+int tdyn_rollout_advance(tdyn_batch* b, int max_steps, hipStream_t s);
+/* Copy out state (packed q, u), per-instance time t, status (0 running,
+ * 1 reached t_end, <0 failed) and stats (attempts, accepted, rejected,
+ * projections; 4 ints per instance).  Any output may be NULL. */
+int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* status,
+                     int32_t* stats, hipStream_t s);
+/* begin + advance until every instance finished + get; synchronizes s.
+ * Returns TDYN_E_STEP_FAILED if any instance failed. */
+int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0,
+                 double t_end, const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s);
+
+const char* tdyn_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDYN_H */

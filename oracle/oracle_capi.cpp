@@ -1,0 +1,261 @@
+// C entry points of the oracle (test infrastructure; see tree_oracle.hpp).
+// Loaded with ctypes by tests/ and by bench.py's cpu_baseline leg only.
+#include "flop_count.hpp"
+#include "tree_oracle.hpp"
+
+#include <atomic>
+#include <chrono>
+#include <memory>
+#include <thread>
+#include <vector>
+
+using namespace oracle;
+
+namespace {
+struct Handle {
+    Model m;
+    std::unique_ptr<Tree<double>> tr;
+};
+inline Handle* H(void* p) { return static_cast<Handle*>(p); }
+
+void loadForces(const Model& M, const double* F, std::vector<SV<double>>& out) {
+    out.assign(M.nb+1, svzero<double>());
+    if (!F) return;
+    for (int b = 0; b <= M.nb; ++b)
+        out[b] = sv(mk3(F[6*b], F[6*b+1], F[6*b+2]), mk3(F[6*b+3], F[6*b+4], F[6*b+5]));
+}
+void storeSV(const SV<double>& v, double* o) { for (int i=0;i<3;++i) { o[i]=v.a[i]; o[3+i]=v.b[i]; } }
+void storeSym(const M33<double>& A, double* o) {   // xx yy zz xy xz yz
+    o[0]=A(0,0); o[1]=A(1,1); o[2]=A(2,2); o[3]=A(1,0); o[4]=A(2,0); o[5]=A(2,1);
+}
+void storeABI(const ABI<double>& P, double* o) {
+    storeSym(P.M, o); storeSym(P.J, o+6); for (int i=0;i<9;++i) o[12+i] = P.F.m[i];
+}
+void realizeAll(Handle* h, const double* q, const double* u) {
+    realizePosition(*h->tr, q);
+    if (u) realizeVelocity(*h->tr, q, u, (double*)nullptr);
+}
+} // namespace
+
+extern "C" {
+
+void* oracle_model_new(const TreeDesc* d) {
+    auto* h = new Handle;
+    if (!h->m.build(*d)) { delete h; return nullptr; }
+    h->tr.reset(new Tree<double>(h->m));
+    return h;
+}
+void oracle_model_free(void* p) { delete H(p); }
+
+int oracle_model_sizes(void* p, int* out /*nb nq nu nquat nlevels*/) {
+    const Model& M = H(p)->m;
+    out[0]=M.nb; out[1]=M.nq; out[2]=M.nu; out[3]=M.nquat; out[4]=(int)M.levels.size();
+    return 0;
+}
+int oracle_model_layout(void* p, int* qIndex, int* uIndex, int* level) {
+    const Model& M = H(p)->m;
+    for (int b=1;b<=M.nb;++b) { qIndex[b-1]=M.qIndex[b]; uIndex[b-1]=M.uIndex[b]; level[b-1]=M.level[b]; }
+    return 0;
+}
+
+// Position+velocity kinematics outputs.  X_GB: (nb+1)*12 (R row-major, p);
+// V_GB: (nb+1)*6; qdot: nq; energies: [ke, pe].  Any output may be null.
+int oracle_kinematics(void* p, const double* q, const double* u, double* X_GB, double* V_GB,
+                      double* qdot, double* energies, double* qerr) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    realizeVelocity(tr, q, u, qdot);
+    for (int b=0;b<=M.nb;++b) {
+        if (X_GB) { for (int i=0;i<9;++i) X_GB[12*b+i]=tr.B[b].X_GB.R.m[i]; for (int i=0;i<3;++i) X_GB[12*b+9+i]=tr.B[b].X_GB.p[i]; }
+        if (V_GB) storeSV(tr.B[b].V_GB, V_GB+6*b);
+    }
+    if (energies) { energies[0] = kineticEnergy(tr); energies[1] = potentialEnergy(tr); }
+    if (qerr) { int k=0; for (size_t L=1; L<M.levels.size(); ++L) for (int b : M.levels[L]) if (M.type[b]==BALL||M.type[b]==FREE) qerr[k++] = tr.B[b].qerr; }
+    return 0;
+}
+
+// Gravity + optional applied body forces, (nb+1)*6.
+int oracle_system_forces(void* p, const double* q, const double* Fapp, double* Fout) {
+    Handle* h = H(p); const Model& M = h->m;
+    realizePosition(*h->tr, q);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    addGravity(*h->tr, F.data());
+    for (int b=0;b<=M.nb;++b) storeSV(F[b], Fout+6*b);
+    return 0;
+}
+
+// calcAccelerationIgnoringConstraints(state, fmob, F, udot, A_GB) + qdotdot.
+int oracle_forward(void* p, const double* q, const double* u, const double* fmob, const double* Fb,
+                   double* udot, double* qdotdot, double* A_GB) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizeAll(h, q, u);
+    const bool ok = realizeABI(tr);
+    realizeABVelocity(tr);
+    std::vector<SV<double>> F; loadForces(M, Fb, F);
+    forwardDynamics(tr, q, u, fmob, F.data(), udot, qdotdot);
+    if (A_GB) for (int b=0;b<=M.nb;++b) storeSV(tr.B[b].A_GB, A_GB+6*b);
+    return ok ? 0 : 1;
+}
+
+// realize(Acceleration): gravity + optional constant applied forces.
+int oracle_realize_acceleration(void* p, const double* q, const double* u, const double* Fapp,
+                                double* udot, double* qdot, double* qdotdot) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    realizeVelocity(tr, q, u, qdot);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    addGravity(tr, F.data());
+    const bool ok = realizeABI(tr);
+    realizeABVelocity(tr);
+    forwardDynamics(tr, q, u, (const double*)nullptr, F.data(), udot, qdotdot);
+    return ok ? 0 : 1;
+}
+
+int oracle_minv(void* p, const double* q, const double* f, double* out) {
+    Handle* h = H(p); Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    const bool ok = realizeABI(tr);
+    multiplyByMInv(tr, f, out);
+    return ok ? 0 : 1;
+}
+
+int oracle_residual(void* p, const double* q, const double* u, const double* fmob, const double* Fb,
+                    const double* udot, double* out) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizeAll(h, q, u);
+    std::vector<SV<double>> F;
+    if (Fb) loadForces(M, Fb, F);
+    residualForces(tr, fmob, Fb ? F.data() : (const SV<double>*)nullptr, udot, out);
+    return 0;
+}
+
+// Articulated-body quantities per body: P, PPlus (21 each: M sym6, J sym6,
+// F 9), D and DI packed by cumulative dof^2.  Any output may be null.
+int oracle_abi(void* p, const double* q, double* P, double* PPlus, double* D, double* DI) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    const bool ok = realizeABI(tr);
+    int off = 0;
+    for (int b=1;b<=M.nb;++b) {
+        if (P) storeABI(tr.B[b].P, P + 21*(b-1));
+        if (PPlus) storeABI(tr.B[b].PPlus, PPlus + 21*(b-1));
+        const int n2 = M.dof[b]*M.dof[b];
+        for (int i=0;i<n2;++i) { if (D) D[off+i]=tr.B[b].D[i]; if (DI) DI[off+i]=tr.B[b].DI[i]; }
+        off += n2;
+    }
+    return ok ? 0 : 1;
+}
+
+int oracle_reactions(void* p, const double* q, const double* u, const double* fmob, const double* Fb,
+                     double* FM) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizeAll(h, q, u);
+    const bool ok = realizeABI(tr);
+    realizeABVelocity(tr);
+    std::vector<SV<double>> F; loadForces(M, Fb, F);
+    std::vector<double> ud(M.nu);
+    forwardDynamics(tr, q, u, fmob, F.data(), ud.data(), (double*)nullptr);
+    std::vector<SV<double>> R(M.nb+1);
+    reactionForces(tr, R.data());
+    for (int b=0;b<=M.nb;++b) storeSV(R[b], FM+6*b);
+    return ok ? 0 : 1;
+}
+
+// TimeStepper(stepTo T) with RungeKuttaMersonIntegrator, finalTime = T.
+// y: [q; u] in/out.  stats: attempts accepted rejected projections evals status
+int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T,
+                     double accuracy, double fixed_h, long max_attempts, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    if (max_attempts > 0) cfg.max_attempts = max_attempts;
+    RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
+// Same integration with the counting scalar.  tallies: addsub mul div sqrt trig.
+int oracle_count_flops_integrate(void* p, double* y, const double* Fapp, double t0, double T,
+                                 double accuracy, double fixed_h, unsigned long long* tallies, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<CReal> rk(M);
+    const int ny = M.nq + M.nu;
+    std::vector<CReal> yc(ny); for (int i=0;i<ny;++i) yc[i] = CReal(y[i]);
+    std::vector<SV<CReal>> F(M.nb+1, svzero<CReal>());
+    if (Fapp) for (int b=0;b<=M.nb;++b)
+        F[b] = sv(mk3<CReal>(Fapp[6*b],Fapp[6*b+1],Fapp[6*b+2]), mk3<CReal>(Fapp[6*b+3],Fapp[6*b+4],Fapp[6*b+5]));
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    tally() = FlopTally();
+    RkmStats st = rk.integrate(yc.data(), t0, T, cfg, Fapp ? F.data() : (const SV<CReal>*)nullptr);
+    const FlopTally t = tally();
+    tallies[0]=t.addsub; tallies[1]=t.mul; tallies[2]=t.div; tallies[3]=t.sqrt; tallies[4]=t.trig;
+    for (int i=0;i<ny;++i) y[i] = yc[i].v;
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
+// Flop count of one realize(Acceleration) (forward dynamics incl. gravity)
+// and of one inverse-dynamics call on the given state.
+int oracle_count_flops_ops(void* p, const double* q, const double* u, unsigned long long* fwd, unsigned long long* inv) {
+    Handle* h = H(p); const Model& M = h->m;
+    Tree<CReal> tr(M);
+    Workspace<CReal> ws;
+    const int ny = M.nq + M.nu;
+    std::vector<CReal> y(ny), yd(ny), r(M.nu);
+    for (int i=0;i<M.nq;++i) y[i] = q[i];
+    for (int i=0;i<M.nu;++i) y[M.nq+i] = u[i];
+    tally() = FlopTally();
+    derivs(tr, ws, y.data(), (const SV<CReal>*)nullptr, yd.data());
+    FlopTally t = tally();
+    fwd[0]=t.addsub; fwd[1]=t.mul; fwd[2]=t.div; fwd[3]=t.sqrt; fwd[4]=t.trig;
+    tally() = FlopTally();
+    realizePosition(tr, y.data());
+    realizeVelocity(tr, y.data(), y.data()+M.nq, (CReal*)nullptr);
+    residualForces(tr, (const CReal*)nullptr, (const SV<CReal>*)nullptr, yd.data()+M.nq, r.data());
+    t = tally();
+    inv[0]=t.addsub; inv[1]=t.mul; inv[2]=t.div; inv[3]=t.sqrt; inv[4]=t.trig;
+    return 0;
+}
+
+// CPU baseline: integrate n_inst instances (one instance per thread at a
+// time, std::thread pool) and return wall seconds.  Instance i uses
+// models[model_of_inst[i]]; Y is instance-major with stride ystride, and
+// Fapp (nullable) is instance-major with stride fstride.  out_body_steps =
+// sum over instances of accepted steps * nb.
+double oracle_time_integrate_batch(void** models, const int* model_of_inst, int n_inst,
+                                   double* Y, int ystride, const double* Fapp, int fstride,
+                                   double t0, double T, double accuracy, double fixed_h,
+                                   int n_threads, long long* out_body_steps, long long* out_attempts) {
+    std::atomic<int> next{0};
+    std::atomic<long long> bodySteps{0}, attempts{0};
+    auto worker = [&]() {
+        std::vector<std::unique_ptr<Rkm<double>>> cache;
+        std::vector<void*> cacheKey;
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n_inst) break;
+            Handle* h = H(models[model_of_inst[i]]);
+            const Model& M = h->m;
+            Rkm<double>* rk = nullptr;
+            for (size_t k=0;k<cacheKey.size();++k) if (cacheKey[k]==(void*)h) rk = cache[k].get();
+            if (!rk) { cache.emplace_back(new Rkm<double>(M)); cacheKey.push_back(h); rk = cache.back().get(); }
+            std::vector<SV<double>> F;
+            if (Fapp) loadForces(M, Fapp + (size_t)i*fstride, F);
+            RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+            RkmStats st = rk->integrate(Y + (size_t)i*ystride, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+            bodySteps += (long long)st.accepted * M.nb;
+            attempts += st.attempts;
+        }
+    };
+    const auto t_start = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (int k=0;k<n_threads;++k) th.emplace_back(worker);
+    for (auto& x : th) x.join();
+    const auto t_end = std::chrono::steady_clock::now();
+    if (out_body_steps) *out_body_steps = bodySteps.load();
+    if (out_attempts) *out_attempts = attempts.load();
+    return std::chrono::duration<double>(t_end - t_start).count();
+}
+
+} // extern "C"

@@ -1,0 +1,179 @@
+"""ctypes binding of the oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+The oracle is the FP64 CPU restatement of Simbody's tree-dynamics path
+(tree_oracle.hpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may import this module.  It is the checker, never the thing
+measured or shipped.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int32)
+
+
+class TreeDesc(ctypes.Structure):
+    """Same layout as tdyn_tree_desc in include/tdyn.h."""
+    _fields_ = [("nb", ctypes.c_int32), ("parent", _i), ("mobilizer", _i),
+                ("mass", _d), ("com_B", _d), ("inertia_B", _d),
+                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.oracle_model_new.restype = vp
+        L.oracle_model_new.argtypes = [ctypes.POINTER(TreeDesc)]
+        L.oracle_model_free.argtypes = [vp]
+        L.oracle_time_integrate_batch.restype = ctypes.c_double
+        for name in ["oracle_model_sizes", "oracle_model_layout", "oracle_kinematics",
+                     "oracle_system_forces", "oracle_forward", "oracle_realize_acceleration",
+                     "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
+                     "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops"]:
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_d)
+
+
+def _pi(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_i)
+
+
+class OracleModel:
+    """One tree.  `spec` is any object with numpy fields parent, mobilizer,
+    mass, com_B, inertia_B, X_PF, X_BM and gravity (see simbody_amd.models)."""
+
+    def __init__(self, spec):
+        self._keep = [np.ascontiguousarray(spec.parent, np.int32),
+                      np.ascontiguousarray(spec.mobilizer, np.int32),
+                      np.ascontiguousarray(spec.mass, np.float64),
+                      np.ascontiguousarray(spec.com_B, np.float64),
+                      np.ascontiguousarray(spec.inertia_B, np.float64),
+                      np.ascontiguousarray(spec.X_PF, np.float64),
+                      np.ascontiguousarray(spec.X_BM, np.float64)]
+        k = self._keep
+        d = TreeDesc()
+        d.nb = len(k[0])
+        d.parent, d.mobilizer = _pi(k[0]), _pi(k[1])
+        d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (_p(x) for x in k[2:])
+        for j in range(3):
+            d.gravity[j] = float(spec.gravity[j])
+        self._desc = d
+        h = lib().oracle_model_new(ctypes.byref(d))
+        if not h:
+            raise ValueError("invalid tree description")
+        self.h = ctypes.c_void_p(h)
+        s = (ctypes.c_int * 5)()
+        lib().oracle_model_sizes(self.h, s)
+        self.nb, self.nq, self.nu, self.nquat, self.nlevels = list(s)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.oracle_model_free(self.h)
+            self.h = None
+
+    def layout(self):
+        qi = (ctypes.c_int * self.nb)(); ui = (ctypes.c_int * self.nb)(); lv = (ctypes.c_int * self.nb)()
+        lib().oracle_model_layout(self.h, qi, ui, lv)
+        return np.array(qi), np.array(ui), np.array(lv)
+
+    def kinematics(self, q, u):
+        X = np.zeros((self.nb + 1, 12)); V = np.zeros((self.nb + 1, 6))
+        qd = np.zeros(self.nq); e = np.zeros(2); qe = np.zeros(max(self.nquat, 1))
+        lib().oracle_kinematics(self.h, _p(q), _p(u), _p(X), _p(V), _p(qd), _p(e), _p(qe))
+        return dict(X_GB=X, V_GB=V, qdot=qd, ke=e[0], pe=e[1], qerr=qe[:self.nquat])
+
+    def system_forces(self, q, Fapp=None):
+        F = np.zeros((self.nb + 1, 6))
+        lib().oracle_system_forces(self.h, _p(q), _p(Fapp), _p(F))
+        return F
+
+    def forward(self, q, u, fmob=None, F=None):
+        ud = np.zeros(self.nu); qdd = np.zeros(self.nq); A = np.zeros((self.nb + 1, 6))
+        rc = lib().oracle_forward(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(ud), _p(qdd), _p(A))
+        return ud, qdd, A, rc
+
+    def realize_acceleration(self, q, u, Fapp=None):
+        ud = np.zeros(self.nu); qd = np.zeros(self.nq); qdd = np.zeros(self.nq)
+        rc = lib().oracle_realize_acceleration(self.h, _p(q), _p(u), _p(Fapp), _p(ud), _p(qd), _p(qdd))
+        return ud, qd, qdd, rc
+
+    def minv(self, q, f):
+        out = np.zeros(self.nu)
+        lib().oracle_minv(self.h, _p(q), _p(np.ascontiguousarray(f, np.float64)), _p(out))
+        return out
+
+    def residual(self, q, u, fmob=None, F=None, udot=None):
+        out = np.zeros(self.nu)
+        lib().oracle_residual(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(udot), _p(out))
+        return out
+
+    def abi(self, q, dofs):
+        n2 = int(sum(d * d for d in dofs))
+        P = np.zeros((self.nb, 21)); PP = np.zeros((self.nb, 21)); D = np.zeros(n2); DI = np.zeros(n2)
+        rc = lib().oracle_abi(self.h, _p(q), _p(P), _p(PP), _p(D), _p(DI))
+        return P, PP, D, DI, rc
+
+    def reactions(self, q, u, fmob=None, F=None):
+        out = np.zeros((self.nb + 1, 6))
+        lib().oracle_reactions(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(out))
+        return out
+
+    def integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None, t0=0.0, max_attempts=0):
+        y = np.array(y, dtype=np.float64, copy=True)
+        st = (ctypes.c_long * 6)()
+        lib().oracle_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+                               ctypes.c_double(accuracy), ctypes.c_double(fixed_h),
+                               ctypes.c_long(max_attempts), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return y, dict(zip(keys, list(st)))
+
+    def count_flops_integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None):
+        y = np.array(y, dtype=np.float64, copy=True)
+        t = (ctypes.c_ulonglong * 5)(); st = (ctypes.c_long * 6)()
+        lib().oracle_count_flops_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(0.0),
+                                           ctypes.c_double(T), ctypes.c_double(accuracy),
+                                           ctypes.c_double(fixed_h), t, st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return dict(addsub=t[0], mul=t[1], div=t[2], sqrt=t[3], trig=t[4]), dict(zip(keys, list(st)))
+
+    def count_flops_ops(self, q, u):
+        f = (ctypes.c_ulonglong * 5)(); g = (ctypes.c_ulonglong * 5)()
+        lib().oracle_count_flops_ops(self.h, _p(q), _p(u), f, g)
+        k = ["addsub", "mul", "div", "sqrt", "trig"]
+        return dict(zip(k, list(f))), dict(zip(k, list(g)))
+
+
+def time_integrate_batch(models, model_of_inst, Y, T, accuracy, fixed_h, n_threads, Fapp=None):
+    """CPU baseline: one instance per thread, std::thread pool.  Y is
+    (n_inst, ny) and is integrated in place.  Returns (seconds, body_steps, attempts)."""
+    arr = (ctypes.c_void_p * len(models))(*[m.h.value for m in models])
+    moi = np.ascontiguousarray(model_of_inst, np.int32)
+    bs = ctypes.c_longlong(0); at = ctypes.c_longlong(0)
+    fstride = 0 if Fapp is None else Fapp.shape[1]
+    sec = lib().oracle_time_integrate_batch(
+        arr, moi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(len(moi)),
+        _p(Y), ctypes.c_int(Y.shape[1]), _p(Fapp), ctypes.c_int(fstride), ctypes.c_double(0.0),
+        ctypes.c_double(T), ctypes.c_double(accuracy), ctypes.c_double(fixed_h),
+        ctypes.c_int(n_threads), ctypes.byref(bs), ctypes.byref(at))
+    return sec, bs.value, at.value

@@ -1,0 +1,941 @@
+// Oracle: scalar FP64 CPU restatement of Simbody's tree-dynamics path.
+//
+// THIS IS TEST INFRASTRUCTURE. It is the checker that the HIP path of
+// simbody_amd is compared against, and the timed CPU baseline.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+// product path never calls into it.
+//
+// Every routine restates the algorithm of the Simbody 3.9 source file cited
+// next to it (paths relative to the Simbody source tree).  It is written from
+// the mathematics of those routines, in this file's own data structures; the
+// operation order follows the reference where that is cheap to do so that the
+// FP64 results agree to rounding.
+//
+// The code is templated on the scalar type T so that a counting scalar
+// (flop_count.hpp) can report the algorithmic flop count of exactly this
+// algorithm on exactly the benchmark inputs.
+//
+// Pinning (what checks this restatement; see tests/test_oracle_*.py):
+//   * SD/FAST reaction-force golden vectors of
+//     Simbody/tests/TestMobilizerReactionForces.cpp:405-435 (Pin + Slider,
+//     gravity) pin position kinematics, spatial inertias, gravity, the
+//     articulated-body inertias, the ABA passes and A_GB.
+//   * Closed forms: single and double planar pendulum equations of motion,
+//     a torque-free body, a Slider under gravity.
+//   * Invariants of Simbody/tests/TestMassMatrix.cpp:658-887: M^-1(M v)=v,
+//     residual(forward(f))=0, forward(u=0) = M^-1 f, symmetric positive M.
+//   * Finite differences of the position kinematics pin V_GB and H; finite
+//     differences of V_GB pin A_GB (coriolis and HDot terms).
+//   * Energy conservation (TestMobilizedBody.cpp:615-662 style) and momentum
+//     conservation of free-floating trees pin the velocity-dependent terms and
+//     the 6x6 Free-joint D inverse.
+//   * RK-Merson: the linear-test-equation stability polynomial and the
+//     embedded error estimate (RungeKuttaMersonIntegrator.cpp:60-80 tableau).
+//   Parity unpinned (no reference-held vector, reference not buildable here):
+//   the exact adaptive step-size sequence of AbstractIntegratorRep
+//   (:448-502) and the quaternion projection cadence are restated from the
+//   source but only pinned by the accuracy/energy properties above.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <limits>
+
+namespace oracle {
+
+using std::sqrt; using std::sin; using std::cos; using std::fabs; using std::pow;
+
+// ---------------------------------------------------------------------------
+// Small fixed-size algebra.
+// ---------------------------------------------------------------------------
+template <class T> struct V3 {
+    T x[3];
+    T& operator[](int i) { return x[i]; }
+    const T& operator[](int i) const { return x[i]; }
+};
+template <class T> inline V3<T> mk3(T a, T b, T c) { V3<T> r; r[0]=a; r[1]=b; r[2]=c; return r; }
+template <class T> inline V3<T> zero3() { return mk3<T>(T(0),T(0),T(0)); }
+template <class T> inline V3<T> operator+(const V3<T>& a, const V3<T>& b) { return mk3<T>(a[0]+b[0], a[1]+b[1], a[2]+b[2]); }
+template <class T> inline V3<T> operator-(const V3<T>& a, const V3<T>& b) { return mk3<T>(a[0]-b[0], a[1]-b[1], a[2]-b[2]); }
+template <class T> inline V3<T> operator-(const V3<T>& a) { return mk3<T>(-a[0], -a[1], -a[2]); }
+template <class T> inline V3<T> operator*(const T& s, const V3<T>& a) { return mk3<T>(s*a[0], s*a[1], s*a[2]); }
+template <class T> inline T dot(const V3<T>& a, const V3<T>& b) { return a[0]*b[0] + a[1]*b[1] + a[2]*b[2]; }
+template <class T> inline V3<T> cross(const V3<T>& a, const V3<T>& b) {
+    return mk3<T>(a[1]*b[2] - a[2]*b[1], a[2]*b[0] - a[0]*b[2], a[0]*b[1] - a[1]*b[0]);
+}
+
+// Row-major 3x3.
+template <class T> struct M33 {
+    T m[9];
+    T& operator()(int r, int c) { return m[r*3+c]; }
+    const T& operator()(int r, int c) const { return m[r*3+c]; }
+};
+template <class T> inline M33<T> ident33() { M33<T> r; for (int i=0;i<9;++i) r.m[i]=T(0); r.m[0]=r.m[4]=r.m[8]=T(1); return r; }
+template <class T> inline M33<T> zero33() { M33<T> r; for (int i=0;i<9;++i) r.m[i]=T(0); return r; }
+template <class T> inline V3<T> operator*(const M33<T>& A, const V3<T>& v) {
+    return mk3<T>(A(0,0)*v[0] + A(0,1)*v[1] + A(0,2)*v[2],
+                  A(1,0)*v[0] + A(1,1)*v[1] + A(1,2)*v[2],
+                  A(2,0)*v[0] + A(2,1)*v[1] + A(2,2)*v[2]);
+}
+template <class T> inline V3<T> mulT(const M33<T>& A, const V3<T>& v) {   // A^T v
+    return mk3<T>(A(0,0)*v[0] + A(1,0)*v[1] + A(2,0)*v[2],
+                  A(0,1)*v[0] + A(1,1)*v[1] + A(2,1)*v[2],
+                  A(0,2)*v[0] + A(1,2)*v[1] + A(2,2)*v[2]);
+}
+template <class T> inline M33<T> operator*(const M33<T>& A, const M33<T>& B) {
+    M33<T> C;
+    for (int r=0;r<3;++r) for (int c=0;c<3;++c)
+        C(r,c) = A(r,0)*B(0,c) + A(r,1)*B(1,c) + A(r,2)*B(2,c);
+    return C;
+}
+template <class T> inline M33<T> mulABt(const M33<T>& A, const M33<T>& B) {  // A B^T
+    M33<T> C;
+    for (int r=0;r<3;++r) for (int c=0;c<3;++c)
+        C(r,c) = A(r,0)*B(c,0) + A(r,1)*B(c,1) + A(r,2)*B(c,2);
+    return C;
+}
+template <class T> inline M33<T> mulAtB(const M33<T>& A, const M33<T>& B) {  // A^T B
+    M33<T> C;
+    for (int r=0;r<3;++r) for (int c=0;c<3;++c)
+        C(r,c) = A(0,r)*B(0,c) + A(1,r)*B(1,c) + A(2,r)*B(2,c);
+    return C;
+}
+template <class T> inline M33<T> operator+(const M33<T>& A, const M33<T>& B) { M33<T> C; for (int i=0;i<9;++i) C.m[i]=A.m[i]+B.m[i]; return C; }
+template <class T> inline M33<T> operator-(const M33<T>& A, const M33<T>& B) { M33<T> C; for (int i=0;i<9;++i) C.m[i]=A.m[i]-B.m[i]; return C; }
+template <class T> inline M33<T> transpose(const M33<T>& A) { M33<T> C; for (int r=0;r<3;++r) for (int c=0;c<3;++c) C(r,c)=A(c,r); return C; }
+template <class T> inline M33<T> crossMat(const V3<T>& v) {
+    M33<T> C;
+    C(0,0)=T(0);  C(0,1)=-v[2]; C(0,2)= v[1];
+    C(1,0)= v[2]; C(1,1)=T(0);  C(1,2)=-v[0];
+    C(2,0)=-v[1]; C(2,1)= v[0]; C(2,2)=T(0);
+    return C;
+}
+// Keep the lower triangle and mirror it: the reference stores SymMat33 as its
+// lower triangle only (SimTKcommon SymMat<3>).
+template <class T> inline M33<T> symFromLower(const M33<T>& A) {
+    M33<T> C = A; C(0,1)=A(1,0); C(0,2)=A(2,0); C(1,2)=A(2,1); return C;
+}
+
+template <class T> struct SV { V3<T> a, b; };   // SpatialVec: (angular, linear)
+template <class T> inline SV<T> sv(const V3<T>& a, const V3<T>& b) { SV<T> r; r.a=a; r.b=b; return r; }
+template <class T> inline SV<T> svzero() { return sv(zero3<T>(), zero3<T>()); }
+template <class T> inline SV<T> operator+(const SV<T>& x, const SV<T>& y) { return sv(x.a+y.a, x.b+y.b); }
+template <class T> inline SV<T> operator-(const SV<T>& x, const SV<T>& y) { return sv(x.a-y.a, x.b-y.b); }
+template <class T> inline T dot(const SV<T>& x, const SV<T>& y) { return dot(x.a,y.a) + dot(x.b,y.b); }
+
+template <class T> struct Xform { M33<T> R; V3<T> p; };
+template <class T> inline Xform<T> compose(const Xform<T>& A, const Xform<T>& B) {
+    Xform<T> C; C.R = A.R*B.R; C.p = A.p + A.R*B.p; return C;
+}
+template <class T> inline Xform<T> invert(const Xform<T>& A) {
+    Xform<T> C; C.R = transpose(A.R); C.p = -(C.R*A.p); return C;
+}
+
+// PhiMatrix action (SimTKcommon/Mechanics/.../SpatialAlgebra.h:685-788):
+//   Phi  * F = (F.a + l x F.b, F.b)      (child-to-parent force shift)
+//   Phi^T* V = (V.a, V.b + V.a x l)      (parent-to-child motion shift)
+template <class T> inline SV<T> phiMul(const V3<T>& l, const SV<T>& F) { return sv(F.a + cross(l, F.b), F.b); }
+template <class T> inline SV<T> phiTMul(const V3<T>& l, const SV<T>& V) { return sv(V.a, V.b + cross(V.a, l)); }
+
+// Rigid-body spatial inertia about the body origin (MassProperties.h:992-1116):
+// mass m, mass-center offset p, unit inertia G about the origin.
+template <class T> struct SpatialInertia { T m; V3<T> p; M33<T> G; };
+template <class T> inline SV<T> siMul(const SpatialInertia<T>& I, const SV<T>& v) {
+    const V3<T> mom = I.G*v.a + cross(I.p, v.b);
+    const V3<T> frc = v.b - cross(I.p, v.a);
+    return sv(I.m*mom, I.m*frc);
+}
+
+// Articulated-body inertia P = [J F; F^T M] (MassProperties.h:1231-1331).
+template <class T> struct ABI { M33<T> M, J, F; };
+template <class T> inline ABI<T> abiFromRigid(const SpatialInertia<T>& I) {
+    ABI<T> P;
+    P.M = zero33<T>(); P.M(0,0)=P.M(1,1)=P.M(2,2)=I.m;
+    for (int i=0;i<9;++i) P.J.m[i] = I.m*I.G.m[i];
+    P.F = crossMat(I.m*I.p);
+    return P;
+}
+template <class T> inline SV<T> abiMul(const ABI<T>& P, const SV<T>& v) {
+    return sv(P.J*v.a + P.F*v.b, mulT(P.F, v.a) + P.M*v.b);
+}
+// Articulated shift across Phi(s): P' = Phi P Phi^T
+// (SimTKcommon/Mechanics/src/MassProperties.cpp:100-127):
+//   F' = F + [s]x M,  J' = J + [s]x F^T - F' [s]x  (symmetric), M' = M.
+template <class T> inline ABI<T> abiShift(const ABI<T>& P, const V3<T>& s) {
+    const M33<T> sx = crossMat(s);
+    ABI<T> Q;
+    Q.M = P.M;
+    Q.F = P.F + sx*P.M;
+    const M33<T> X = mulABt(sx, P.F) - Q.F*sx;   // [s]x F^T - F' [s]x
+    Q.J = P.J + symFromLower(X);
+    return Q;
+}
+template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
+    P.M = P.M + Q.M; P.J = P.J + Q.J; P.F = P.F + Q.F;
+}
+
+// ---------------------------------------------------------------------------
+// Model.
+// ---------------------------------------------------------------------------
+enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4 };
+inline int mobDof(int t) { return t==PIN||t==SLIDER ? 1 : t==UNIVERSAL ? 2 : t==BALL ? 3 : 6; }
+inline int mobNq (int t) { return t==PIN||t==SLIDER ? 1 : t==UNIVERSAL ? 2 : t==BALL ? 4 : 7; }
+
+// Plain-array tree description; identical to tdyn_tree_desc of include/tdyn.h.
+struct TreeDesc {
+    int32_t nb;
+    const int32_t* parent;      // [nb], body numbers 1..nb, 0 = Ground, parent < body
+    const int32_t* mobilizer;   // [nb]
+    const double*  mass;        // [nb]
+    const double*  com_B;       // [nb][3]
+    const double*  inertia_B;   // [nb][6] about Bo in B: xx yy zz xy xz yz
+    const double*  X_PF;        // [nb][12] row-major R then p
+    const double*  X_BM;        // [nb][12]
+    double gravity[3];
+};
+
+struct Model {
+    int nb = 0, nq = 0, nu = 0, nquat = 0;
+    std::vector<int> parent, type, dof, qIndex, uIndex, level;
+    std::vector<std::vector<int>> children;   // in body-number order
+    std::vector<std::vector<int>> levels;     // rbNodeLevels, body numbers per level
+    std::vector<double> mass;
+    std::vector<V3<double>> com;
+    std::vector<M33<double>> Gunit;           // unit inertia about Bo in B
+    std::vector<Xform<double>> X_PF, X_MB;
+    std::vector<V3<double>> p_BM;
+    V3<double> g;
+
+    // Build from the plain description.  Body 0 is Ground.
+    // Slot assignment follows SimbodyMatterSubsystemRep.cpp:256-293 (q and u
+    // slots in body-number order, maxNQ per mobilizer).
+    bool build(const TreeDesc& d) {
+        nb = d.nb;
+        if (nb <= 0) return false;
+        const int n = nb + 1;
+        parent.assign(n, -1); type.assign(n, -1); dof.assign(n, 0);
+        qIndex.assign(n, 0); uIndex.assign(n, 0); level.assign(n, 0);
+        children.assign(n, {}); mass.assign(n, 0.0);
+        com.assign(n, zero3<double>()); Gunit.assign(n, zero33<double>());
+        X_PF.assign(n, Xform<double>{ident33<double>(), zero3<double>()});
+        X_MB = X_PF; p_BM.assign(n, zero3<double>());
+        nq = nu = nquat = 0;
+        for (int b = 1; b <= nb; ++b) {
+            const int i = b-1;
+            const int p = d.parent[i];
+            if (p < 0 || p >= b) return false;
+            const int t = d.mobilizer[i];
+            if (t < PIN || t > FREE) return false;
+            if (t == FREE && p != 0) return false;
+            parent[b] = p; type[b] = t; dof[b] = mobDof(t);
+            level[b] = level[p] + 1;
+            children[p].push_back(b);
+            qIndex[b] = nq; uIndex[b] = nu;
+            nq += mobNq(t); nu += dof[b];
+            if (t == BALL || t == FREE) ++nquat;
+            mass[b] = d.mass[i];
+            if (!(mass[b] > 0)) return false;
+            com[b] = mk3(d.com_B[3*i], d.com_B[3*i+1], d.com_B[3*i+2]);
+            const double* I = d.inertia_B + 6*i;
+            M33<double> Io;
+            Io(0,0)=I[0]; Io(1,1)=I[1]; Io(2,2)=I[2];
+            Io(0,1)=Io(1,0)=I[3]; Io(0,2)=Io(2,0)=I[4]; Io(1,2)=Io(2,1)=I[5];
+            for (int k=0;k<9;++k) Gunit[b].m[k] = Io.m[k]/mass[b];
+            for (int k=0;k<9;++k) X_PF[b].R.m[k] = d.X_PF[12*i+k];
+            X_PF[b].p = mk3(d.X_PF[12*i+9], d.X_PF[12*i+10], d.X_PF[12*i+11]);
+            Xform<double> X_BM;
+            for (int k=0;k<9;++k) X_BM.R.m[k] = d.X_BM[12*i+k];
+            X_BM.p = mk3(d.X_BM[12*i+9], d.X_BM[12*i+10], d.X_BM[12*i+11]);
+            p_BM[b] = X_BM.p;
+            X_MB[b] = invert(X_BM);      // RigidBodyNode.h:1012 X_MB(~X_BM)
+        }
+        int maxLevel = 0;
+        for (int b = 1; b <= nb; ++b) maxLevel = std::max(maxLevel, level[b]);
+        levels.assign(maxLevel+1, {});
+        for (int b = 0; b <= nb; ++b) levels[level[b]].push_back(b);
+        g = mk3(d.gravity[0], d.gravity[1], d.gravity[2]);
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Per-body cache: the union of SBTreePositionCache, SBTreeVelocityCache,
+// SBArticulatedBodyInertiaCache and SBTreeAccelerationCache entries
+// (Simbody/src/SimbodyTreeState.h:648-1162) that the tree path needs.
+// ---------------------------------------------------------------------------
+template <class T> struct Body {
+    // position
+    T cq[2], sq[2];            // mobilizer q pool (sin/cos)
+    T qerr;                    // |quaternion| - 1
+    Xform<T> X_FM, X_PB, X_GB;
+    V3<T> phi;                 // p_PB_G
+    SV<T> H_FM[6], H[6];       // columns, H = H_PB_G
+    SpatialInertia<T> Mk;      // about Bo, in G
+    V3<T> com_G;
+    // velocity
+    SV<T> V_FM, V_PB_G, V_GB, VD_PB_G;
+    SV<T> HDot_FM[6], HDot[6];
+    SV<T> b, a, atot, MAb;     // gyroscopic, mobilizer coriolis, total coriolis, Mk*atot+b
+    // articulated
+    ABI<T> P, PPlus;
+    T D[36], DI[36];
+    SV<T> G[6], PAb;           // G = P H DI (columns), PAb = P a + b
+    // acceleration / temporaries
+    SV<T> z, zPlus, A_GB, F;
+    T eps[6];
+};
+
+template <class T> struct Tree {
+    const Model* M = nullptr;
+    std::vector<Body<T>> B;
+    explicit Tree(const Model& m) : M(&m), B(m.nb+1) { resetGround(); }
+    void resetGround() {
+        Body<T>& g = B[0];
+        g.X_GB.R = ident33<T>(); g.X_GB.p = zero3<T>();
+        g.V_GB = svzero<T>(); g.atot = svzero<T>(); g.A_GB = svzero<T>();
+    }
+};
+
+template <class T> inline Xform<T> castX(const Xform<double>& X) {
+    Xform<T> r; for (int i=0;i<9;++i) r.R.m[i]=T(X.R.m[i]); for (int i=0;i<3;++i) r.p[i]=T(X.p[i]); return r;
+}
+template <class T> inline V3<T> cast3(const V3<double>& v) { return mk3<T>(T(v[0]),T(v[1]),T(v[2])); }
+
+// Rotation from a (normalized) quaternion, Rotation.cpp:600-610.
+template <class T> inline M33<T> quatToRot(const T q[4]) {
+    const T q00=q[0]*q[0], q11=q[1]*q[1], q22=q[2]*q[2], q33=q[3]*q[3];
+    const T q01=q[0]*q[1], q02=q[0]*q[2], q03=q[0]*q[3];
+    const T q12=q[1]*q[2], q13=q[1]*q[3], q23=q[2]*q[3];
+    const T a = q00-q11, c = q22-q33;
+    M33<T> R;
+    R(0,0)=q00+q11-q22-q33; R(0,1)=T(2)*(q12-q03); R(0,2)=T(2)*(q13+q02);
+    R(1,0)=T(2)*(q12+q03);  R(1,1)=a+c;            R(1,2)=T(2)*(q23-q01);
+    R(2,0)=T(2)*(q13-q02);  R(2,1)=T(2)*(q23+q01); R(2,2)=a-c;
+    return R;
+}
+// qdot = N(q) w for an unnormalized quaternion q (Rotation.h:712-719, 990-992).
+template <class T> inline void quatDot(const T q[4], const V3<T>& w, T qd[4]) {
+    const T e0=q[0]/T(2), e1=q[1]/T(2), e2=q[2]/T(2), e3=q[3]/T(2);
+    qd[0] = -e1*w[0] - e2*w[1] - e3*w[2];
+    qd[1] =  e0*w[0] + e3*w[1] - e2*w[2];
+    qd[2] = -e3*w[0] + e0*w[1] + e1*w[2];
+    qd[3] =  e2*w[0] - e1*w[1] + e0*w[2];
+}
+// u = NInv(q) qdot, unnormalized form (Rotation.h:742-748).
+template <class T> inline V3<T> quatDotToW(const T q[4], const T qd[4]) {
+    const T e0=T(2)*q[0], e1=T(2)*q[1], e2=T(2)*q[2], e3=T(2)*q[3];
+    return mk3<T>(-e1*qd[0] + e0*qd[1] - e3*qd[2] + e2*qd[3],
+                  -e2*qd[0] + e3*qd[1] + e0*qd[2] - e1*qd[3],
+                  -e3*qd[0] - e2*qd[1] + e1*qd[2] + e0*qd[3]);
+}
+// qdotdot = N b + NDot w = N b - |w|^2/4 q  (Rotation.h:1006-1013).
+template <class T> inline void quatDotDot(const T q[4], const V3<T>& w, const V3<T>& bdot, T qdd[4]) {
+    T Nb[4]; quatDot(q, bdot, Nb);
+    const T s = T(-0.25)*dot(w,w);
+    for (int i=0;i<4;++i) qdd[i] = Nb[i] + s*q[i];
+}
+
+// ---------------------------------------------------------------------------
+// Position kinematics, base to tip.
+// RigidBodyNodeSpec.h:229-288 (realizePosition) with the mobilizer-specific
+// performQPrecalculations/calcX_FM/calcAcrossJointVelocityJacobian of
+// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free}.h, calcBodyTransforms
+// (RigidBodyNodeSpec.h:554-569), calcParentToChildVelocityJacobianInGround
+// (RigidBodyNodeSpec.cpp:44-74) and calcJointIndependentKinematicsPos
+// (RigidBodyNode.cpp:54-84).  Driver: SimbodyMatterSubsystemRep.cpp:1112-1144.
+// ---------------------------------------------------------------------------
+template <class T> void realizePosition(Tree<T>& tr, const T* q) {
+    const Model& M = *tr.M;
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const Body<T>& Pb = tr.B[M.parent[b]];
+        const T* qb = q + M.qIndex[b];
+        const int t = M.type[b], nd = M.dof[b];
+        B.qerr = T(0);
+        // q precalculations and X_FM
+        B.X_FM.p = zero3<T>();
+        switch (t) {
+        case PIN: {
+            B.cq[0] = cos(qb[0]); B.sq[0] = sin(qb[0]);
+            M33<T> R = zero33<T>();
+            R(0,0)=B.cq[0]; R(0,1)=-B.sq[0]; R(1,0)=B.sq[0]; R(1,1)=B.cq[0]; R(2,2)=T(1);
+            B.X_FM.R = R;
+            B.H_FM[0] = sv(mk3<T>(T(0),T(0),T(1)), zero3<T>());
+        } break;
+        case SLIDER: {
+            B.X_FM.R = ident33<T>();
+            B.X_FM.p = mk3<T>(qb[0], T(0), T(0));
+            B.H_FM[0] = sv(zero3<T>(), mk3<T>(T(1),T(0),T(0)));
+        } break;
+        case UNIVERSAL: {
+            // body-fixed X then Y sequence, Rotation.cpp:241-263
+            B.cq[0] = cos(qb[0]); B.cq[1] = cos(qb[1]);
+            B.sq[0] = sin(qb[0]); B.sq[1] = sin(qb[1]);
+            const T c1=B.cq[0], s1=B.sq[0], c2=B.cq[1], s2=B.sq[1];
+            M33<T> R;
+            R(0,0)= c2;    R(0,1)= T(0); R(0,2)= s2;
+            R(1,0)= s2*s1; R(1,1)= c1;   R(1,2)= -s1*c2;
+            R(2,0)=-s2*c1; R(2,1)= s1;   R(2,2)= c1*c2;
+            B.X_FM.R = R;
+            B.H_FM[0] = sv(mk3<T>(T(1),T(0),T(0)), zero3<T>());
+            B.H_FM[1] = sv(mk3<T>(R(0,1),R(1,1),R(2,1)), zero3<T>());   // R_FM.y()
+        } break;
+        case BALL: case FREE: {
+            const T len = sqrt(qb[0]*qb[0] + qb[1]*qb[1] + qb[2]*qb[2] + qb[3]*qb[3]);
+            B.qerr = len - T(1);
+            const T oo = T(1)/len;
+            T qn[4] = { qb[0]*oo, qb[1]*oo, qb[2]*oo, qb[3]*oo };
+            B.X_FM.R = quatToRot(qn);
+            for (int j=0;j<3;++j) {
+                V3<T> e = zero3<T>(); e[j] = T(1);
+                B.H_FM[j] = sv(e, zero3<T>());
+            }
+            if (t == FREE) {
+                B.X_FM.p = mk3<T>(qb[4], qb[5], qb[6]);
+                for (int j=0;j<3;++j) {
+                    V3<T> e = zero3<T>(); e[j] = T(1);
+                    B.H_FM[3+j] = sv(zero3<T>(), e);
+                }
+            }
+        } break;
+        }
+        // X_PB = X_PF X_FM X_MB ; X_GB = X_GP X_PB
+        const Xform<T> X_PF = castX<T>(M.X_PF[b]);
+        const Xform<T> X_MB = castX<T>(M.X_MB[b]);
+        B.X_PB = compose(X_PF, compose(B.X_FM, X_MB));
+        B.X_GB = compose(Pb.X_GB, B.X_PB);
+        // H_PB_G = R_GF (H_FM + H_MB_F), H_MB_F = (0, -r_MB_F x H_FM.a)
+        const M33<T> R_GF = Pb.X_GB.R * X_PF.R;
+        const V3<T> r_MB_F = B.X_FM.R * X_MB.p;
+        for (int j = 0; j < nd; ++j) {
+            const V3<T> lin = B.H_FM[j].b - cross(r_MB_F, B.H_FM[j].a);
+            B.H[j] = sv(R_GF*B.H_FM[j].a, R_GF*lin);
+        }
+        // joint-independent part
+        B.phi = Pb.X_GB.R * B.X_PB.p;
+        const M33<T>& R_GB = B.X_GB.R;
+        M33<T> Gb; for (int k=0;k<9;++k) Gb.m[k] = T(M.Gunit[b].m[k]);
+        B.Mk.m = T(M.mass[b]);
+        B.Mk.G = symFromLower(mulABt(R_GB*Gb, R_GB));    // R G R^T
+        B.Mk.p = R_GB*cast3<T>(M.com[b]);
+        B.com_G = B.X_GB.p + B.Mk.p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Velocity kinematics, base to tip.  RigidBodyNodeSpec.h:305-333,
+// calcAcrossJointVelocityJacobianDot (_Universal.h:173-189; zero otherwise),
+// calcParentToChildVelocityJacobianInGroundDot (RigidBodyNodeSpec.cpp:82-129),
+// calcJointIndependentKinematicsVel (RigidBodyNode.cpp:98-174), calcQDot
+// (RigidBodyNodeSpec.h:129-134, _Ball.h:187-214, _Free.h:226-257).
+// ---------------------------------------------------------------------------
+template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* qdot) {
+    const Model& M = *tr.M;
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const Body<T>& Pb = tr.B[M.parent[b]];
+        const T* ub = u + M.uIndex[b];
+        const T* qb = q + M.qIndex[b];
+        const int t = M.type[b], nd = M.dof[b];
+        if (qdot) {
+            T* qd = qdot + M.qIndex[b];
+            if (t == BALL || t == FREE) {
+                quatDot(qb, mk3<T>(ub[0],ub[1],ub[2]), qd);
+                if (t == FREE) { qd[4]=ub[3]; qd[5]=ub[4]; qd[6]=ub[5]; }
+            } else for (int j=0;j<nd;++j) qd[j] = ub[j];
+        }
+        B.V_FM = svzero<T>(); B.V_PB_G = svzero<T>();
+        for (int j=0;j<nd;++j) {
+            B.V_FM.a = B.V_FM.a + ub[j]*B.H_FM[j].a; B.V_FM.b = B.V_FM.b + ub[j]*B.H_FM[j].b;
+            B.V_PB_G.a = B.V_PB_G.a + ub[j]*B.H[j].a; B.V_PB_G.b = B.V_PB_G.b + ub[j]*B.H[j].b;
+        }
+        for (int j=0;j<nd;++j) B.HDot_FM[j] = svzero<T>();
+        if (t == UNIVERSAL) {
+            const V3<T> y = mk3<T>(B.X_FM.R(0,1), B.X_FM.R(1,1), B.X_FM.R(2,1));
+            B.HDot_FM[1] = sv(cross(B.V_FM.a, y), zero3<T>());
+        }
+        const Xform<T> X_PF = castX<T>(M.X_PF[b]);
+        const Xform<T> X_MB = castX<T>(M.X_MB[b]);
+        const M33<T> R_GF = Pb.X_GB.R * X_PF.R;
+        const V3<T>& w_GF = Pb.V_GB.a;
+        const V3<T> r_MB_F = B.X_FM.R * X_MB.p;
+        const V3<T>& w_FM = B.V_FM.a;
+        const V3<T> wxr = cross(w_FM, r_MB_F);
+        B.VD_PB_G = svzero<T>();
+        for (int j=0;j<nd;++j) {
+            const V3<T> lin = B.HDot_FM[j].b - cross(r_MB_F, B.HDot_FM[j].a) - cross(wxr, B.H_FM[j].a);
+            B.HDot[j] = sv(R_GF*B.HDot_FM[j].a + cross(w_GF, B.H[j].a),
+                           R_GF*lin            + cross(w_GF, B.H[j].b));
+            B.VD_PB_G.a = B.VD_PB_G.a + ub[j]*B.HDot[j].a;
+            B.VD_PB_G.b = B.VD_PB_G.b + ub[j]*B.HDot[j].b;
+        }
+        // joint-independent
+        const SV<T>& V_GP = Pb.V_GB;
+        B.V_GB = phiTMul(B.phi, V_GP) + B.V_PB_G;
+        const V3<T>& w = B.V_GB.a;
+        B.b = sv(B.Mk.m*cross(w, B.Mk.G*w), B.Mk.m*cross(w, cross(w, B.Mk.p)));
+        B.a = sv(B.VD_PB_G.a, B.VD_PB_G.b + cross(V_GP.a, B.V_GB.b - V_GP.b));
+        B.atot = phiTMul(B.phi, Pb.atot) + B.a;
+        B.MAb = siMul(B.Mk, B.atot) + B.b;
+    }
+}
+
+// Gravity body forces (Force_Gravity.cpp:514-578): F = (p_CB_G x m g, m g).
+template <class T> void addGravity(const Tree<T>& tr, SV<T>* F) {
+    const Model& M = *tr.M;
+    const V3<T> g = cast3<T>(M.g);
+    for (int b = 1; b <= M.nb; ++b) {
+        const Body<T>& B = tr.B[b];
+        const V3<T> f = B.Mk.m*g;
+        F[b] = F[b] + sv(cross(B.Mk.p, f), f);
+    }
+}
+template <class T> T potentialEnergy(const Tree<T>& tr) {   // Force_Gravity.cpp:555
+    const Model& M = *tr.M;
+    const V3<T> g = cast3<T>(M.g);
+    T pe = T(0);
+    for (int b = 1; b <= M.nb; ++b) pe = pe - tr.B[b].Mk.m*dot(g, tr.B[b].com_G);
+    return pe;
+}
+template <class T> T kineticEnergy(const Tree<T>& tr) {      // RigidBodyNode.cpp:182-188
+    T ke = T(0);
+    for (int b = 1; b <= tr.M->nb; ++b) {
+        const Body<T>& B = tr.B[b];
+        ke = ke + dot(B.V_GB, siMul(B.Mk, B.V_GB))/T(2);
+    }
+    return ke;
+}
+
+// Small dense inverse.  dof<=3 use the cofactor forms of
+// SmallMatrixMixed.h:895-960; dof=6 (Free) uses LU with partial pivoting and
+// column solves, the algorithm of LAPACK dgetrf/dgetri that the reference
+// calls (SmallMatrixMixed.h:858-894); LAPACK itself is not vendored.
+template <class T> bool invertSmall(int n, const T* A, T* Ai) {
+    if (n == 1) { Ai[0] = T(1)/A[0]; return true; }
+    if (n == 2) {
+        const T d = A[0]*A[3] - A[1]*A[2];
+        const T o = T(1)/d;
+        Ai[0] = o*A[3]; Ai[1] = -(o*A[1]); Ai[2] = -(o*A[2]); Ai[3] = o*A[0];
+        return true;
+    }
+    if (n == 3) {
+        auto m = [&](int r, int c) -> const T& { return A[r*3+c]; };
+        const T d00 = m(1,1)*m(2,2)-m(1,2)*m(2,1), nd01 = m(1,2)*m(2,0)-m(1,0)*m(2,2),
+                d02 = m(1,0)*m(2,1)-m(1,1)*m(2,0);
+        const T d = m(0,0)*d00 + m(0,1)*nd01 + m(0,2)*d02;
+        const T o = T(1)/d;
+        const T nd10 = m(0,2)*m(2,1)-m(0,1)*m(2,2), d11 = m(0,0)*m(2,2)-m(0,2)*m(2,0),
+                nd12 = m(0,1)*m(2,0)-m(0,0)*m(2,1), d20 = m(0,1)*m(1,2)-m(0,2)*m(1,1),
+                nd21 = m(0,2)*m(1,0)-m(0,0)*m(1,2), d22 = m(0,0)*m(1,1)-m(0,1)*m(1,0);
+        Ai[0]=o*d00;  Ai[1]=o*nd10; Ai[2]=o*d20;
+        Ai[3]=o*nd01; Ai[4]=o*d11;  Ai[5]=o*nd21;
+        Ai[6]=o*d02;  Ai[7]=o*nd12; Ai[8]=o*d22;
+        return true;
+    }
+    // n x n, LU with partial pivoting: P A = L U
+    T LU[36]; int piv[6];
+    for (int i=0;i<n*n;++i) LU[i]=A[i];
+    for (int k=0;k<n;++k) {
+        int p = k; double best = fabs(double(LU[k*n+k]));
+        for (int r=k+1;r<n;++r) { const double v = fabs(double(LU[r*n+k])); if (v > best) { best=v; p=r; } }
+        if (best == 0.0) return false;
+        piv[k]=p;
+        if (p != k) for (int c=0;c<n;++c) std::swap(LU[k*n+c], LU[p*n+c]);
+        const T inv = T(1)/LU[k*n+k];
+        for (int r=k+1;r<n;++r) {
+            LU[r*n+k] = LU[r*n+k]*inv;
+            const T l = LU[r*n+k];
+            for (int c=k+1;c<n;++c) LU[r*n+c] = LU[r*n+c] - l*LU[k*n+c];
+        }
+    }
+    for (int col=0; col<n; ++col) {
+        T x[6];
+        for (int i=0;i<n;++i) x[i] = T(0);
+        // e_col permuted: apply the recorded row interchanges in order
+        int target = col;
+        T e[6]; for (int i=0;i<n;++i) e[i]=T(i==col?1:0);
+        for (int k=0;k<n;++k) if (piv[k]!=k) std::swap(e[k], e[piv[k]]);
+        (void)target;
+        for (int i=0;i<n;++i) { T s = e[i]; for (int j=0;j<i;++j) s = s - LU[i*n+j]*x[j]; x[i]=s; }
+        for (int i=n-1;i>=0;--i) { T s = x[i]; for (int j=i+1;j<n;++j) s = s - LU[i*n+j]*x[j]; x[i] = s/LU[i*n+i]; }
+        for (int i=0;i<n;++i) Ai[i*n+col] = x[i];
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Articulated-body inertias, tip to base.  RigidBodyNodeSpec.cpp:249-325;
+// driver SimbodyMatterSubsystemRep.cpp:1217-1246.  Returns false if a D is
+// singular.
+// ---------------------------------------------------------------------------
+template <class T> bool realizeABI(Tree<T>& tr) {
+    const Model& M = *tr.M;
+    bool ok = true;
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        B.P = abiFromRigid(B.Mk);
+        for (int c : M.children[b]) abiAdd(B.P, abiShift(tr.B[c].PPlus, tr.B[c].phi));
+        SV<T> PH[6];
+        for (int j=0;j<nd;++j) PH[j] = abiMul(B.P, B.H[j]);
+        for (int r=0;r<nd;++r) for (int c=0;c<nd;++c) B.D[r*nd+c] = dot(B.H[r], PH[c]);
+        if (!invertSmall(nd, B.D, B.DI)) ok = false;
+        for (int c=0;c<nd;++c) {
+            SV<T> acc = svzero<T>();
+            for (int k=0;k<nd;++k) { acc.a = acc.a + B.DI[k*nd+c]*PH[k].a; acc.b = acc.b + B.DI[k*nd+c]*PH[k].b; }
+            B.G[c] = acc;
+        }
+        M33<T> massMoment, mass, inertia;
+        for (int r=0;r<3;++r) for (int c=0;c<3;++c) {
+            T mm = T(0), ms = T(0), in = T(0);
+            for (int j=0;j<nd;++j) {
+                mm = mm + B.G[j].a[r]*PH[j].b[c];
+                ms = ms + B.G[j].b[r]*PH[j].b[c];
+                in = in + B.G[j].a[r]*PH[j].a[c];
+            }
+            massMoment(r,c)=mm; mass(r,c)=ms; inertia(r,c)=in;
+        }
+        M33<T> symMass, symIn;
+        for (int r=0;r<3;++r) for (int c=0;c<3;++c) {
+            if (r==c) { symMass(r,c)=mass(r,c); symIn(r,c)=inertia(r,c); }
+            else { symMass(r,c)=(mass(r,c)+mass(c,r))/T(2); symIn(r,c)=(inertia(r,c)+inertia(c,r))/T(2); }
+        }
+        B.PPlus.M = B.P.M - symMass;
+        B.PPlus.F = B.P.F - massMoment;
+        B.PPlus.J = B.P.J - symIn;
+    }
+    return ok;
+}
+// P a + b, RigidBodyNode.cpp:202-213.
+template <class T> void realizeABVelocity(Tree<T>& tr) {
+    for (int b = 1; b <= tr.M->nb; ++b) tr.B[b].PAb = abiMul(tr.B[b].P, tr.B[b].a) + tr.B[b].b;
+}
+
+template <class T> inline SV<T> Hmul(const SV<T>* H, int nd, const T* x) {
+    SV<T> r = svzero<T>();
+    for (int j=0;j<nd;++j) { r.a = r.a + x[j]*H[j].a; r.b = r.b + x[j]*H[j].b; }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Forward dynamics: calcTreeAccelerations (SimbodyMatterSubsystemRep.cpp:
+// 5258-5343) with calcUDotPass1Inward / calcUDotPass2Outward
+// (RigidBodyNodeSpec.cpp:355-446) and calcQDotDot.  fmob may be null (zero);
+// F is [nb+1] body forces in G (index 0 = Ground, ignored).
+// ---------------------------------------------------------------------------
+template <class T> void forwardDynamics(Tree<T>& tr, const T* q, const T* u, const T* fmob,
+                                        const SV<T>* F, T* udot, T* qdotdot) {
+    const Model& M = *tr.M;
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        B.z = B.PAb - F[b];
+        for (int c : M.children[b]) B.z = B.z + phiMul(tr.B[c].phi, tr.B[c].zPlus);
+        for (int j=0;j<nd;++j) {
+            const T f = fmob ? fmob[M.uIndex[b]+j] : T(0);
+            B.eps[j] = f - dot(B.H[j], B.z);
+        }
+        B.zPlus = B.z + Hmul(B.G, nd, B.eps);
+    }
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b], t = M.type[b];
+        const SV<T> Ap = phiTMul(B.phi, tr.B[M.parent[b]].A_GB);
+        T* ud = udot + M.uIndex[b];
+        for (int r=0;r<nd;++r) {
+            T s = T(0);
+            for (int c=0;c<nd;++c) s = s + B.DI[r*nd+c]*B.eps[c];
+            ud[r] = s - dot(B.G[r], Ap);
+        }
+        B.A_GB = Ap + Hmul(B.H, nd, ud) + B.a;
+        if (qdotdot) {
+            T* qdd = qdotdot + M.qIndex[b];
+            const T* qb = q + M.qIndex[b];
+            const T* ub = u + M.uIndex[b];
+            if (t == BALL || t == FREE) {
+                quatDotDot(qb, mk3<T>(ub[0],ub[1],ub[2]), mk3<T>(ud[0],ud[1],ud[2]), qdd);
+                if (t == FREE) { qdd[4]=ud[3]; qdd[5]=ud[4]; qdd[6]=ud[5]; }
+            } else for (int j=0;j<nd;++j) qdd[j] = ud[j];
+        }
+    }
+}
+
+// udot = M^-1 f, RigidBodyNodeSpec.cpp:483-552, SimbodyMatterSubsystemRep.cpp:5357-5400.
+template <class T> void multiplyByMInv(Tree<T>& tr, const T* f, T* out) {
+    const Model& M = *tr.M;
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        B.z = svzero<T>();
+        for (int c : M.children[b]) B.z = B.z + phiMul(tr.B[c].phi, tr.B[c].zPlus);
+        for (int j=0;j<nd;++j) B.eps[j] = f[M.uIndex[b]+j] - dot(B.H[j], B.z);
+        B.zPlus = B.z + Hmul(B.G, nd, B.eps);
+    }
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        const SV<T> Ap = phiTMul(B.phi, tr.B[M.parent[b]].A_GB);
+        T* ud = out + M.uIndex[b];
+        for (int r=0;r<nd;++r) {
+            T s = T(0);
+            for (int c=0;c<nd;++c) s = s + B.DI[r*nd+c]*B.eps[c];
+            ud[r] = s - dot(B.G[r], Ap);
+        }
+        B.A_GB = Ap + Hmul(B.H, nd, ud);
+    }
+}
+
+// Inverse dynamics: calcTreeResidualForces (SimbodyMatterSubsystemRep.cpp:
+// 5532-5616) with calcBodyAccelerationsFromUdotOutward / calcInverseDynamicsPass2Inward
+// (RigidBodyNodeSpec.cpp:566-641).  Null fmob/F/udot mean zeros.
+template <class T> void residualForces(Tree<T>& tr, const T* fmob, const SV<T>* F, const T* udot, T* out) {
+    const Model& M = *tr.M;
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        T ud[6];
+        for (int j=0;j<nd;++j) ud[j] = udot ? udot[M.uIndex[b]+j] : T(0);
+        B.A_GB = phiTMul(B.phi, tr.B[M.parent[b]].A_GB) + Hmul(B.H, nd, ud) + B.a;
+    }
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        B.F = siMul(B.Mk, B.A_GB) + B.b;
+        if (F) B.F = B.F - F[b];
+        for (int c : M.children[b]) B.F = B.F + phiMul(tr.B[c].phi, tr.B[c].F);
+        for (int j=0;j<nd;++j) {
+            const T f = fmob ? fmob[M.uIndex[b]+j] : T(0);
+            out[M.uIndex[b]+j] = dot(B.H[j], B.F) - f;
+        }
+    }
+}
+
+// Mobilizer reaction forces at M, expressed in G (SimbodyMatterSubsystemRep.cpp:
+// 5786-5826).  Requires forwardDynamics() to have run.  Checker only.
+template <class T> void reactionForces(const Tree<T>& tr, SV<T>* FM) {
+    const Model& M = *tr.M;
+    FM[0] = svzero<T>();
+    for (int b = 1; b <= M.nb; ++b) {
+        const Body<T>& B = tr.B[b];
+        const Body<T>& Pb = tr.B[M.parent[b]];
+        const V3<T> p_PB_G = B.X_GB.p - Pb.X_GB.p;
+        const SV<T> Ap = sv(Pb.A_GB.a, Pb.A_GB.b + cross(Pb.A_GB.a, p_PB_G));
+        const SV<T> FB = B.zPlus + abiMul(B.PPlus, Ap);
+        const V3<T> p_BM_G = B.X_GB.R * cast3<T>(M.p_BM[b]);
+        FM[b] = sv(FB.a - cross(p_BM_G, FB.b), FB.b);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// System derivative: the realize(Acceleration) of MultibodySystem with
+// Force::Gravity plus constant body forces (Force::DiscreteForces).  ydot =
+// [qdot; udot].  Fapp may be null; it has nb+1 entries.
+// ---------------------------------------------------------------------------
+template <class T> struct Workspace {
+    std::vector<SV<T>> F;
+    std::vector<T> tmp;
+};
+
+template <class T> bool derivs(Tree<T>& tr, Workspace<T>& ws, const T* y, const SV<T>* Fapp, T* ydot) {
+    const Model& M = *tr.M;
+    const T* q = y; const T* u = y + M.nq;
+    realizePosition(tr, q);
+    realizeVelocity(tr, q, u, ydot);
+    ws.F.assign(M.nb+1, svzero<T>());
+    addGravity(tr, ws.F.data());
+    if (Fapp) for (int b=1;b<=M.nb;++b) ws.F[b] = ws.F[b] + Fapp[b];
+    const bool ok = realizeABI(tr);
+    realizeABVelocity(tr);
+    forwardDynamics(tr, q, u, (const T*)nullptr, ws.F.data(), ydot + M.nq, (T*)nullptr);
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Runge-Kutta-Merson with Simbody's step control.
+//   attemptODEStep         RungeKuttaMersonIntegrator.cpp:86-140
+//   attemptDAEStep         AbstractIntegratorRep.cpp:137-208
+//   adjustStepSize         AbstractIntegratorRep.cpp:448-502
+//   takeOneStep            AbstractIntegratorRep.cpp:513-609 (no events)
+//   stepTo main loop       AbstractIntegratorRep.cpp:216-436 with finalTime set
+//   methodInitialize       AbstractIntegratorRep.cpp:40-53 (h0 = timescale/10, timescale 0.1)
+//   calcErrorNorm          IntegratorRep.h:454-488, scaleDQ :500-514, calcRelativeScaling :644-655
+//   accuracy/consTol       IntegratorRep.h:737-740 (defaults 1e-3, acc/10)
+//   projectQ               SimbodyMatterSubsystemRep.cpp:4096-4184 (no holonomic constraints)
+//   normalizeQuaternions   SimbodyMatterSubsystemRep.cpp:4448-4475, _Ball.h:417-434, _Free.h:476-493
+// ---------------------------------------------------------------------------
+struct RkmConfig {
+    double accuracy = 1e-3;     // Integrator::setAccuracy
+    double fixed_h = 0.0;       // >0 : Integrator::setFixedStepSize(h)
+    double init_h = -1.0, min_h = -1.0, max_h = -1.0;
+    double timescale = 0.1;     // System default time scale
+    long   max_attempts = 100000000;
+};
+struct RkmStats { long attempts=0, accepted=0, rejected=0, projections=0, evals=0; int status=0; };
+
+template <class T> struct Rkm {
+    const Model& M;
+    Tree<T> tr;
+    Workspace<T> ws;
+    std::vector<T> y0, f0, fa, fb, ysave, y1, ytmp, yerr, uscale;
+    explicit Rkm(const Model& m) : M(m), tr(m) {
+        const int ny = m.nq + m.nu;
+        y0.resize(ny); f0.resize(ny); fa.resize(ny); fb.resize(ny);
+        ysave.resize(ny); y1.resize(ny); ytmp.resize(ny); yerr.resize(ny); uscale.resize(m.nu);
+    }
+
+    // Quaternion normalization of q (and, if given, removal of the radial
+    // component of the q error estimate).
+    void normalizeQuats(T* q, T* qerrest) {
+        for (int b=1;b<=M.nb;++b) {
+            const int t = M.type[b];
+            if (t != BALL && t != FREE) continue;
+            T* qq = q + M.qIndex[b];
+            const T n = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]);
+            for (int i=0;i<4;++i) qq[i] = qq[i]/n;
+            if (qerrest) {
+                T* e = qerrest + M.qIndex[b];
+                const T d = e[0]*qq[0] + e[1]*qq[1] + e[2]*qq[2] + e[3]*qq[3];
+                for (int i=0;i<4;++i) e[i] = e[i] - d*qq[i];
+            }
+        }
+    }
+    // RMS of (|quat|-1) over the quaternions of q; NaN-safe.
+    T quatErrRMS(const T* q) const {
+        if (M.nquat == 0) return T(0);
+        T s = T(0);
+        for (int b=1;b<=M.nb;++b) {
+            const int t = M.type[b];
+            if (t != BALL && t != FREE) continue;
+            const T* qq = q + M.qIndex[b];
+            const T e = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - T(1);
+            s = s + e*e;
+        }
+        return sqrt(s/T(M.nquat));
+    }
+    // max(RMS(N Wu NInv dq), RMS(uScale*du)), Wu = 1 (SimbodyMatterSubsystemRep.cpp:1630-1634).
+    T errorNorm(const T* q, const T* err) const {
+        T sq = T(0);
+        for (int b=1;b<=M.nb;++b) {
+            const int t = M.type[b];
+            const T* e = err + M.qIndex[b];
+            if (t == BALL || t == FREE) {
+                const T* qq = q + M.qIndex[b];
+                const V3<T> w = quatDotToW(qq, e);
+                T d[4]; quatDot(qq, w, d);
+                for (int i=0;i<4;++i) sq = sq + d[i]*d[i];
+                if (t == FREE) for (int i=4;i<7;++i) sq = sq + e[i]*e[i];
+            } else for (int i=0;i<M.dof[b];++i) sq = sq + e[i]*e[i];
+        }
+        const T qn = M.nq ? sqrt(sq/T(M.nq)) : T(0);
+        T su = T(0);
+        const T* eu = err + M.nq;
+        for (int i=0;i<M.nu;++i) { const T w = uscale[i]*eu[i]; su = su + w*w; }
+        const T un = M.nu ? sqrt(su/T(M.nu)) : T(0);
+        return qn >= un ? qn : un;
+    }
+
+    // One RK-Merson attempt from (t0,y0,f0) to t1.  Fills y1, yerr, ysave.
+    bool attemptODE(double t0, double t1, const SV<T>* Fapp) {
+        const int ny = M.nq + M.nu;
+        const T h = T(t1 - t0);
+        bool ok = true;
+        for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(3))*f0[i];
+        ok &= derivs(tr, ws, ytmp.data(), Fapp, fa.data());
+        for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(6))*(f0[i] + fa[i]);
+        ok &= derivs(tr, ws, ytmp.data(), Fapp, fa.data());
+        for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(8))*(f0[i] + T(3)*fa[i]);
+        ok &= derivs(tr, ws, ytmp.data(), Fapp, fb.data());
+        for (int i=0;i<ny;++i) ysave[i] = y0[i] + (h/T(2))*((f0[i] - T(3)*fa[i]) + T(4)*fb[i]);
+        ok &= derivs(tr, ws, ysave.data(), Fapp, fa.data());
+        for (int i=0;i<ny;++i) y1[i] = y0[i] + (h/T(6))*((f0[i] + T(4)*fb[i]) + fa[i]);
+        for (int i=0;i<ny;++i) yerr[i] = T(0.2)*T(fabs(double(y1[i]-ysave[i])));
+        return ok;
+    }
+
+    static bool isFiniteD(double x) { return std::isfinite(x); }
+
+    // Integrate y (q then u) from t0 to T.  Returns stats.
+    RkmStats integrate(T* y, double t0, double Tend, const RkmConfig& cfg, const SV<T>* Fapp) {
+        RkmStats st;
+        const int ny = M.nq + M.nu;
+        const double acc = cfg.accuracy;
+        const double consTol = acc/10;
+        double hmin = cfg.min_h, hmax = cfg.max_h, hinit = cfg.init_h;
+        if (cfg.fixed_h > 0) hmin = hmax = hinit = cfg.fixed_h;
+        // initialize(): forced projection of the quaternions, then derivatives.
+        normalizeQuats(y, (T*)nullptr);
+        for (int i=0;i<ny;++i) y0[i] = y[i];
+        double t = t0;
+        double hcur = (hinit != -1) ? hinit : cfg.timescale/10;
+        if (hmin != -1) hcur = std::max(hcur, hmin);
+        if (hmax != -1) hcur = std::min(hcur, hmax);
+        const double projectionLimit = std::max(2*consTol, std::sqrt(consTol));
+        for (;;) {
+            if (t >= Tend) break;
+            // saveTimeAndStateAsPrevious + realizeStateDerivatives
+            for (int i=0;i<ny;++i) y0[i] = y[i];
+            for (int i=0;i<M.nu;++i) {
+                const double vi = fabs(double(y0[M.nq+i]));
+                uscale[i] = vi*1.0 > 1.0 ? T(1.0/vi) : T(1.0);
+            }
+            if (!derivs(tr, ws, y0.data(), Fapp, f0.data())) { st.status = 2; break; }
+            ++st.evals;
+            const double tMax = Tend;
+            bool succeeded = false;
+            double t1 = t;
+            do {
+                bool limited = false;
+                if (tMax < t + 0.95*hcur) { limited = true; t1 = tMax; }
+                else if (tMax > t + 1.001*hcur) t1 = t + hcur;
+                else t1 = tMax;
+                if (!(t1 > t)) { st.status = 3; return st; }
+                if (++st.attempts > cfg.max_attempts) { st.status = 4; return st; }
+                bool converged = attemptODE(t, t1, Fapp);
+                st.evals += 4;
+                double errNorm;
+                if (converged) {
+                    const double e1 = double(errorNorm(y1.data(), yerr.data()));
+                    if (!(e1 > 16.0*acc)) {   // NaN falls through to projection like the reference's comparison
+                        const double qe = double(quatErrRMS(y1.data()));
+                        if (qe > projectionLimit) converged = false;
+                        else if (qe > consTol) { normalizeQuats(y1.data(), yerr.data()); ++st.projections; }
+                    }
+                }
+                errNorm = converged ? double(errorNorm(y1.data(), yerr.data()))
+                                    : std::numeric_limits<double>::infinity();
+                // adjustStepSize
+                const double Safety=0.9, MinShrink=0.1, MaxGrow=5, HysLow=0.9, HysHigh=1.2;
+                double hnew;
+                if (!isFiniteD(errNorm)) hnew = MinShrink*hcur;
+                else if (errNorm == 0) hnew = MaxGrow*hcur;
+                else hnew = Safety*hcur*std::pow(acc/errNorm, 1.0/4);
+                if (hnew > hcur) { if (limited || hnew < HysHigh*hcur) hnew = hcur; }
+                if (hnew < hcur) { if (errNorm <= acc) hnew = hcur; else hnew = std::min(hnew, HysLow*hcur); }
+                hnew = std::min(hnew, MaxGrow*hcur);
+                hnew = std::max(hnew, MinShrink*hcur);
+                if (hmin != -1) hnew = std::max(hnew, hmin);
+                if (hmax != -1) hnew = std::min(hnew, hmax);
+                succeeded = (hnew >= hcur);
+                hcur = hnew;
+                if (!succeeded) ++st.rejected;
+            } while (!succeeded);
+            t = t1;
+            for (int i=0;i<ny;++i) y[i] = y1[i];
+            ++st.accepted;
+        }
+        return st;
+    }
+};
+
+} // namespace oracle

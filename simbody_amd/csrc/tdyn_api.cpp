@@ -1,0 +1,383 @@
+// Host side of the tdyn C ABI: validates and packs tree descriptions into
+// device tables, owns the per-lane scratch and the device-resident integrator
+// state, and launches the kernels of tdyn_kernels.hip.
+#include "../../include/tdyn.h"
+#include "tdyn_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+using namespace tdyn;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) { g_err = msg; return code; }
+int hipfail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return TDYN_E_HIP;
+}
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hipfail(e_, #x); } while (0)
+
+template <class T> struct DBuf {
+    T* p = nullptr; size_t n = 0;
+    hipError_t alloc(size_t count) {
+        free();
+        n = count;
+        if (!count) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v) {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+    void free() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    ~DBuf() { free(); }
+};
+} // namespace
+
+struct tdyn_batch {
+    int n_inst = 0, n_trees = 0;
+    int64_t total_q = 0, total_u = 0, total_b = 0;
+    int max_nq = 0, max_nu = 0, max_nb = 0, max_rec = 0;
+    bool uniform = true;
+    std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
+    std::vector<int4> h_tree_a, h_tree_b, h_body_a;
+    // device tables
+    DBuf<int4> body_a, body_b, tree_a, tree_b;
+    DBuf<int> child_idx, tree_of, q_off, u_off, b_off;
+    DBuf<double> prm, gravity, scratch;
+    // rollout state
+    DBuf<double> vec, uscale, t, h, t_end;
+    DBuf<int> status, stats;
+    DBuf<double> fapp;
+    int ny_max = 0;
+    bool rollout_ready = false;
+    RkmCfg cfg{};
+
+    DevTables tables() const {
+        DevTables T;
+        T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
+        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p;
+        return T;
+    }
+    OpArgs op() const {
+        OpArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
+        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        return a;
+    }
+    RkmArgs rk() const {
+        RkmArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
+        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
+        a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = 0;
+        return a;
+    }
+};
+
+namespace {
+bool finite_all(const double* p, int n) { for (int i = 0; i < n; ++i) if (!std::isfinite(p[i])) return false; return true; }
+
+int validate_tree(const tdyn_tree_desc& d, int t) {
+    char buf[256];
+    if (d.nb <= 0) { snprintf(buf, sizeof buf, "tree %d: nb must be positive (got %d)", t, d.nb); return fail(TDYN_E_ARG, buf); }
+    if (!d.parent || !d.mobilizer || !d.mass || !d.com_B || !d.inertia_B || !d.X_PF || !d.X_BM)
+        return fail(TDYN_E_ARG, "tree " + std::to_string(t) + ": null array in tdyn_tree_desc");
+    for (int b = 1; b <= d.nb; ++b) {
+        const int p = d.parent[b-1], ty = d.mobilizer[b-1];
+        if (p < 0 || p >= b) { snprintf(buf, sizeof buf, "tree %d body %d: parent %d must be in [0,%d)", t, b, p, b); return fail(TDYN_E_ARG, buf); }
+        if (ty < TDYN_PIN || ty > TDYN_FREE) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty == TDYN_FREE && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Free mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
+        if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
+        if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
+            !finite_all(d.X_PF + 12*(b-1), 12) || !finite_all(d.X_BM + 12*(b-1), 12)) {
+            snprintf(buf, sizeof buf, "tree %d body %d: non-finite mass properties or frames", t, b); return fail(TDYN_E_ARG, buf);
+        }
+    }
+    if (!finite_all(d.gravity, 3)) return fail(TDYN_E_ARG, "non-finite gravity");
+    return TDYN_OK;
+}
+} // namespace
+
+extern "C" {
+
+const char* tdyn_last_error(void) { return g_err.c_str(); }
+
+int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* tree_of_instance,
+                      int n_instances, tdyn_batch** out) {
+    if (!out) return fail(TDYN_E_ARG, "out is null");
+    *out = nullptr;
+    if (!trees || n_trees <= 0) return fail(TDYN_E_ARG, "need at least one tree");
+    if (n_instances <= 0) return fail(TDYN_E_ARG, "need at least one instance");
+    for (int t = 0; t < n_trees; ++t) { int rc = validate_tree(trees[t], t); if (rc) return rc; }
+    if (tree_of_instance)
+        for (int i = 0; i < n_instances; ++i)
+            if (tree_of_instance[i] < 0 || tree_of_instance[i] >= n_trees)
+                return fail(TDYN_E_ARG, "tree_of_instance[" + std::to_string(i) + "] out of range");
+
+    auto* B = new tdyn_batch;
+    B->n_inst = n_instances; B->n_trees = n_trees;
+    B->uniform = (tree_of_instance == nullptr) || n_trees == 1;
+    std::vector<int4> body_b; std::vector<int> child_idx; std::vector<double> prm, grav;
+    std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
+    int body_off = 0;
+    for (int t = 0; t < n_trees; ++t) {
+        const tdyn_tree_desc& d = trees[t];
+        const int nb = d.nb;
+        std::vector<std::vector<int>> ch(nb + 1);
+        for (int b = 1; b <= nb; ++b) ch[d.parent[b-1]].push_back(b);
+        int q = 0, u = 0, rec = 0, nquat = 0;
+        for (int b = 1; b <= nb; ++b) {
+            const int ty = d.mobilizer[b-1];
+            B->h_body_a.push_back(make_int4(d.parent[b-1], ty, q, u));
+            body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, 0));
+            for (int c : ch[b]) child_idx.push_back(c);
+            q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(mob_dof(ty));
+            if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
+            // constants
+            double P[PRM_STRIDE] = {0};
+            for (int k = 0; k < 9; ++k) P[PRM_R_PF + k] = d.X_PF[12*(b-1) + k];
+            for (int k = 0; k < 3; ++k) P[PRM_P_PF + k] = d.X_PF[12*(b-1) + 9 + k];
+            const double* Rbm = d.X_BM + 12*(b-1);
+            const double* pbm = Rbm + 9;
+            // X_MB = inverse(X_BM): R_MB = R_BM^T, p_MB = -R_BM^T p_BM
+            for (int r = 0; r < 3; ++r) for (int c2 = 0; c2 < 3; ++c2) P[PRM_R_MB + 3*r + c2] = Rbm[3*c2 + r];
+            for (int r = 0; r < 3; ++r)
+                P[PRM_P_MB + r] = -(P[PRM_R_MB + 3*r]*pbm[0] + P[PRM_R_MB + 3*r + 1]*pbm[1] + P[PRM_R_MB + 3*r + 2]*pbm[2]);
+            const double m = d.mass[b-1];
+            P[PRM_MASS] = m;
+            for (int k = 0; k < 3; ++k) P[PRM_COM + k] = d.com_B[3*(b-1) + k];
+            for (int k = 0; k < 6; ++k) P[PRM_G + k] = d.inertia_B[6*(b-1) + k] / m;
+            for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
+            prm.insert(prm.end(), P, P + PRM_STRIDE);
+        }
+        for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
+        grav.push_back(0.0);
+        B->h_tree_a.push_back(make_int4(nb, q, u, body_off));
+        B->h_tree_b.push_back(make_int4(rec, nquat, 0, 0));
+        tq[t] = q; tu[t] = u; tnb[t] = nb;
+        B->max_rec = std::max(B->max_rec, rec);
+        body_off += nb;
+    }
+    B->h_tree_of.resize(n_instances);
+    B->h_qoff.resize(n_instances); B->h_uoff.resize(n_instances); B->h_boff.resize(n_instances);
+    int64_t q = 0, u = 0, bb = 0;
+    for (int i = 0; i < n_instances; ++i) {
+        const int t = tree_of_instance ? tree_of_instance[i] : 0;
+        B->h_tree_of[i] = t;
+        B->h_qoff[i] = (int)q; B->h_uoff[i] = (int)u; B->h_boff[i] = (int)bb;
+        q += tq[t]; u += tu[t]; bb += tnb[t] + 1;
+        B->max_nq = std::max(B->max_nq, tq[t]); B->max_nu = std::max(B->max_nu, tu[t]);
+        B->max_nb = std::max(B->max_nb, tnb[t]);
+    }
+    if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
+    B->total_q = q; B->total_u = u; B->total_b = bb;
+    hipError_t e;
+    if ((e = B->body_a.upload(B->h_body_a)) != hipSuccess ||
+        (e = B->body_b.upload(body_b)) != hipSuccess ||
+        (e = B->child_idx.upload(child_idx.empty() ? std::vector<int>{0} : child_idx)) != hipSuccess ||
+        (e = B->prm.upload(prm)) != hipSuccess ||
+        (e = B->tree_a.upload(B->h_tree_a)) != hipSuccess ||
+        (e = B->tree_b.upload(B->h_tree_b)) != hipSuccess ||
+        (e = B->gravity.upload(grav)) != hipSuccess ||
+        (e = B->tree_of.upload(B->h_tree_of)) != hipSuccess ||
+        (e = B->q_off.upload(B->h_qoff)) != hipSuccess ||
+        (e = B->u_off.upload(B->h_uoff)) != hipSuccess ||
+        (e = B->b_off.upload(B->h_boff)) != hipSuccess ||
+        (e = B->scratch.alloc((size_t)B->max_rec * n_instances)) != hipSuccess) {
+        delete B;
+        return hipfail(e, "tdyn_batch_create: device allocation/upload");
+    }
+    *out = B;
+    return TDYN_OK;
+}
+
+int tdyn_batch_destroy(tdyn_batch* b) { delete b; return TDYN_OK; }
+
+int tdyn_batch_info(const tdyn_batch* b, int64_t* info) {
+    if (!b || !info) return fail(TDYN_E_ARG, "null argument");
+    info[0] = b->n_inst; info[1] = b->total_q; info[2] = b->total_u; info[3] = b->total_b;
+    info[4] = b->max_nq; info[5] = b->max_nu; info[6] = b->max_nb; info[7] = b->n_trees;
+    return TDYN_OK;
+}
+
+int tdyn_batch_offsets(const tdyn_batch* b, int32_t* q_off, int32_t* u_off, int32_t* body_off) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    for (int i = 0; i < b->n_inst; ++i) {
+        if (q_off) q_off[i] = b->h_qoff[i];
+        if (u_off) u_off[i] = b->h_uoff[i];
+        if (body_off) body_off[i] = b->h_boff[i];
+    }
+    return TDYN_OK;
+}
+
+int tdyn_realize_kinematics(tdyn_batch* b, const double* q, const double* u, double* X_GB,
+                            double* V_GB, double* qdot, double* energies, hipStream_t s) {
+    if (!b || !q || !u) return fail(TDYN_E_ARG, "tdyn_realize_kinematics: batch, q and u are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.out_body = X_GB; a.out_body2 = V_GB; a.out_q = qdot; a.out_scalar = energies;
+    HIPCHK(launch_op(OP_KIN, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_realize_acceleration(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                              double* udot, double* qdot, hipStream_t s) {
+    if (!b || !q || !u || !udot) return fail(TDYN_E_ARG, "tdyn_realize_acceleration: batch, q, u and udot are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fbody = fapp; a.out_u = udot; a.out_q = qdot;
+    HIPCHK(launch_op(OP_FWD_SYS, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_acceleration_ignoring_constraints(tdyn_batch* b, const double* q, const double* u,
+                                                const double* fmob, const double* F, double* udot,
+                                                double* A_GB, double* qdotdot, hipStream_t s) {
+    // The reference requires full-length fMob and F (SimbodyMatterSubsystem.cpp:205-216).
+    if (!b || !q || !u || !udot) return fail(TDYN_E_ARG, "calcAccelerationIgnoringConstraints: batch, q, u and udot are required");
+    if (!fmob || !F) return fail(TDYN_E_SIZE, "calcAccelerationIgnoringConstraints: appliedMobilityForces and appliedBodyForces must be full length");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fmob = fmob; a.fbody = F; a.out_u = udot; a.out_body = A_GB; a.out_q = qdotdot;
+    HIPCHK(launch_op(OP_FWD_GIVEN, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV, hipStream_t s) {
+    if (!b || !q) return fail(TDYN_E_ARG, "multiplyByMInv: batch and q are required");
+    if (!v || !MinvV) return fail(TDYN_E_SIZE, "multiplyByMInv: argument 'v' must have one entry per mobility");
+    OpArgs a = b->op();
+    a.q = q; a.fmob = v; a.out_u = MinvV;
+    HIPCHK(launch_op(OP_MINV, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_residual_force_ignoring_constraints(tdyn_batch* b, const double* q, const double* u,
+                                                  const double* fmob, const double* F,
+                                                  const double* udot, double* resid, hipStream_t s) {
+    if (!b || !q || !u || !resid) return fail(TDYN_E_ARG, "calcResidualForceIgnoringConstraints: batch, q, u and resid are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fmob = fmob; a.fbody = F; a.udot_in = udot; a.out_u = resid;
+    HIPCHK(launch_op(OP_RESID, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_articulated_body_inertias(tdyn_batch* b, const double* q, double* P, double* PPlus, hipStream_t s) {
+    if (!b || !q || !P) return fail(TDYN_E_ARG, "realizeArticulatedBodyInertias: batch, q and P are required");
+    OpArgs a = b->op();
+    a.q = q; a.out_abi = P; a.out_abi2 = PPlus;
+    HIPCHK(launch_op(OP_ABI, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_mobilizer_reaction_forces(tdyn_batch* b, const double* q, const double* u,
+                                        const double* fmob, const double* F, double* FM, hipStream_t s) {
+    if (!b || !q || !u || !FM) return fail(TDYN_E_ARG, "calcMobilizerReactionForces: batch, q, u and FM are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fmob = fmob; a.fbody = F; a.out_body = FM;
+    HIPCHK(launch_op(OP_REACT, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                       double t0, const double* t_end, double t_end_all,
+                       const tdyn_integrator_cfg* cfg, hipStream_t s) {
+    if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_rollout_begin: batch, q, u and cfg are required");
+    if (!t_end && !(t_end_all > t0)) return fail(TDYN_E_ARG, "tdyn_rollout_begin: final time must exceed t0");
+    if (cfg->precision != 0) return fail(TDYN_E_ARG, "tdyn_rollout_begin: only precision 0 (FP64) is available");
+    const double acc = cfg->accuracy > 0 ? cfg->accuracy : 1e-3;
+    if (!(acc < 1.0)) return fail(TDYN_E_ARG, "accuracy must be in (0,1)");
+    RkmCfg c;
+    c.accuracy = acc;
+    c.cons_tol = cfg->constraint_tolerance > 0 ? cfg->constraint_tolerance : acc / 10;
+    c.fixed_h = cfg->fixed_step > 0 ? cfg->fixed_step : 0.0;
+    c.init_h = cfg->initial_step > 0 ? cfg->initial_step : -1.0;
+    c.min_h = cfg->min_step > 0 ? cfg->min_step : -1.0;
+    c.max_h = cfg->max_step > 0 ? cfg->max_step : -1.0;
+    c.timescale = cfg->time_scale > 0 ? cfg->time_scale : 0.1;
+    c.max_attempts = cfg->max_attempts > 0 ? cfg->max_attempts : 100000000;
+    c.max_steps = 0;
+    b->cfg = c;
+    const int ny = b->max_nq + b->max_nu;
+    const size_t L = b->n_inst;
+    if (!b->rollout_ready || b->ny_max != ny) {
+        hipError_t e;
+        if ((e = b->vec.alloc((size_t)9 * ny * L)) != hipSuccess ||
+            (e = b->uscale.alloc((size_t)std::max(1, b->max_nu) * L)) != hipSuccess ||
+            (e = b->t.alloc(L)) != hipSuccess || (e = b->h.alloc(L)) != hipSuccess ||
+            (e = b->t_end.alloc(L)) != hipSuccess || (e = b->status.alloc(L)) != hipSuccess ||
+            (e = b->stats.alloc(4 * L)) != hipSuccess)
+            return hipfail(e, "tdyn_rollout_begin: allocation");
+        b->ny_max = ny;
+        b->rollout_ready = true;
+    }
+    if (fapp) {
+        if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));
+        HIPCHK(hipMemcpyAsync(b->fapp.p, fapp, (size_t)b->total_b * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        b->fapp.free();
+    }
+    if (t_end) {
+        HIPCHK(hipMemcpyAsync(b->t_end.p, t_end, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+        std::vector<double> te(L, t_end_all);
+        HIPCHK(hipMemcpyAsync(b->t_end.p, te.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    RkmArgs a = b->rk();
+    HIPCHK(launch_pack_state(a, q, u, 1, nullptr, nullptr, s));
+    HIPCHK(launch_rkm_init(a, t0, s));
+    return TDYN_OK;
+}
+
+int tdyn_rollout_advance(tdyn_batch* b, int max_steps, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_advance: call tdyn_rollout_begin first");
+    if (max_steps <= 0) return fail(TDYN_E_ARG, "max_steps must be positive");
+    RkmArgs a = b->rk();
+    a.cfg.max_steps = max_steps;
+    HIPCHK(launch_rkm(a, s));
+    return TDYN_OK;
+}
+
+int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* status, int32_t* stats, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_get: no rollout");
+    RkmArgs a = b->rk();
+    if (q && u) HIPCHK(launch_pack_state(a, nullptr, nullptr, 0, q, u, s));
+    else if (q || u) return fail(TDYN_E_ARG, "tdyn_rollout_get: q and u go together");
+    const size_t L = b->n_inst;
+    if (t) HIPCHK(hipMemcpyAsync(t, b->t.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (status) HIPCHK(hipMemcpyAsync(status, b->status.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (stats) HIPCHK(hipMemcpyAsync(stats, b->stats.p, 4 * L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    return TDYN_OK;
+}
+
+int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
+                 const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s) {
+    int rc = tdyn_rollout_begin(b, q, u, fapp, t0, nullptr, t_end, cfg, s);
+    if (rc) return rc;
+    const size_t L = b->n_inst;
+    std::vector<int> st(L);
+    for (int round = 0; round < 1000000; ++round) {
+        rc = tdyn_rollout_advance(b, 1 << 20, s);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(st.data(), b->status.p, L * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool running = false, failed = false;
+        for (size_t i = 0; i < L; ++i) { running |= st[i] == 0; failed |= st[i] < 0; }
+        if (!running) {
+            rc = tdyn_rollout_get(b, q, u, nullptr, nullptr, stats, s);
+            if (rc) return rc;
+            HIPCHK(hipStreamSynchronize(s));
+            return failed ? fail(TDYN_E_STEP_FAILED, "Integrator::StepFailed for at least one instance") : TDYN_OK;
+        }
+    }
+    return fail(TDYN_E_STEP_FAILED, "rollout did not finish");
+}
+
+} // extern "C"

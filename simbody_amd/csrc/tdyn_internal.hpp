@@ -1,0 +1,104 @@
+// Internal layouts shared by the host packing code and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tdyn {
+
+enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4 };
+__host__ __device__ inline int mob_dof(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 3 : 6; }
+__host__ __device__ inline int mob_nq(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 4 : 7; }
+
+// Per-body constants, PRM_STRIDE doubles per body (packed over all trees).
+enum {
+    PRM_R_PF = 0, PRM_P_PF = 9, PRM_R_MB = 12, PRM_P_MB = 21, PRM_MASS = 24,
+    PRM_COM = 25, PRM_G = 28 /* unit inertia about Bo in B: xx yy zz xy xz yz */,
+    PRM_P_BM = 34, PRM_STRIDE = 40
+};
+
+// Per-body record in the per-lane scratch (doubles, relative to rec_off).
+// Stored structure-of-arrays across lanes: scratch[(rec_off + k) * L + lane].
+enum {
+    REC_R = 0, REC_P = 9, REC_V = 12, REC_PHI = 18, REC_MKP = 21, REC_MKG = 24,
+    REC_A = 30, REC_B = 36, REC_PPLUS = 42, REC_ZPLUS = 63, REC_AGB = 69,
+    REC_DIEPS = 75 /* dof doubles, then H (6*dof) then G (6*dof) */
+};
+__host__ __device__ inline int rec_size(int dof) { return REC_DIEPS + 13 * dof; }
+
+struct DevTables {
+    const int4* body_a;      // per global body: parent(local), type, qIndex, uIndex
+    const int4* body_b;      // per global body: child_begin, child_count, rec_off, -
+    const int* child_idx;    // local body numbers
+    const double* prm;       // PRM_STRIDE per global body
+    const int4* tree_a;      // per tree: nb, nq, nu, body_off
+    const int4* tree_b;      // per tree: rec_size, nquat, -, -
+    const double* gravity;   // per tree: 4 doubles
+};
+
+// Boundary layout offsets of each instance (instance-major packing).
+struct InstOffsets {
+    const int* q_off;   // start of the instance's q in packed q arrays
+    const int* u_off;   // start of its u in packed u arrays
+    const int* b_off;   // start of its (nb+1) bodies in packed per-body arrays
+};
+
+struct RkmCfg {
+    double accuracy;    // Integrator::setAccuracy
+    double cons_tol;    // accuracy/10 unless set
+    double fixed_h;     // >0: setFixedStepSize
+    double init_h, min_h, max_h;   // -1 = unset
+    double timescale;   // System default time scale (0.1)
+    int max_steps;      // accepted-step budget for this call (per lane)
+    int max_attempts;   // attempt budget per lane for the whole rollout
+};
+
+// Launch entry points (tdyn_kernels.hip).
+enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6 };
+
+struct OpArgs {
+    DevTables T;
+    const int* tree_of;      // null: every instance uses tree 0
+    int n_inst;
+    int L;                   // lane pitch of the scratch
+    double* scratch;
+    InstOffsets off;
+    const double* q;         // packed boundary layout
+    const double* u;
+    const double* fmob;      // nullable
+    const double* fbody;     // nullable ((nb+1)*6 per instance)
+    const double* udot_in;   // residual input, nullable
+    double* out_u;           // udot / M^-1 f / residual   (packed u layout)
+    double* out_q;           // qdot or qdotdot (packed q layout), nullable
+    double* out_body;        // A_GB / X_GB / reactions (packed (nb+1)*6 or *12), nullable
+    double* out_body2;       // V_GB (kinematics), nullable
+    double* out_abi;         // P per body (21), nullable
+    double* out_abi2;        // PPlus per body (21), nullable
+    double* out_scalar;      // kinematics: ke, pe per instance (2), nullable
+    int* status;             // per instance, nullable
+};
+hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s);
+
+struct RkmArgs {
+    DevTables T;
+    const int* tree_of;
+    int n_inst, L;
+    double* scratch;
+    InstOffsets off;
+    const double* fapp;      // constant applied body forces, packed, nullable
+    double* vec;             // 9 SoA vectors of ny_max*L doubles: y y0 f0 fa fb ysave y1 ytmp yerr
+    int ny_max;
+    double* t;               // per lane
+    double* h;               // per lane: current step size
+    const double* t_end;     // per lane
+    int* status;             // per lane: 0 running, 1 done, <0 failed
+    int* stats;              // per lane: attempts accepted rejected projections
+    double* uscale;          // SoA nu_max*L
+    RkmCfg cfg;
+    int precision;           // 0 = FP64, 1 = mixed
+};
+hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
+hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
+hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
+                             double* q_out, double* u_out, hipStream_t s);
+
+} // namespace tdyn

@@ -1,0 +1,822 @@
+// Tree-dynamics kernels for gfx950 (MI355X), FP64.
+//
+// Mapping: one lane (work-item) per multibody instance.  Each lane walks its
+// own tree in body-number order (a valid base-to-tip order, parents are
+// numbered before children) and in reverse (tip-to-base).  Per-body
+// intermediates that one sweep hands to the next live in a per-lane record
+// in HBM, stored structure-of-arrays across lanes so that lanes running the
+// same tree issue coalesced loads and stores.  The body loops are the
+// level sweeps of Simbody's SimbodyMatterSubsystemRep (see the per-routine
+// citations below); instances are independent, which is the parallelism the
+// reference never exploits.
+#include "tdyn_internal.hpp"
+#include "tdyn_math.hpp"
+
+namespace tdyn {
+
+using d3 = v3<double>;
+using dm = m33<double>;
+using ds = s33<double>;
+using dv = sv<double>;
+using da = abi<double>;
+
+struct Lane {
+    DevTables T;
+    int lane, L, nb, nq, nu, body_off;
+    double* scr;
+    __device__ double& s(int off) const { return scr[(size_t)off * L + lane]; }
+    __device__ int4 ba(int b) const { return T.body_a[body_off + b - 1]; }
+    __device__ int4 bb(int b) const { return T.body_b[body_off + b - 1]; }
+    __device__ const double* prm(int b) const { return T.prm + (size_t)(body_off + b - 1) * PRM_STRIDE; }
+    __device__ int rec(int b) const { return T.body_b[body_off + b - 1].z; }
+};
+
+__device__ __forceinline__ Lane make_lane(const DevTables& T, const int* tree_of, int lane, int L, double* scr) {
+    Lane c;
+    c.T = T; c.lane = lane; c.L = L; c.scr = scr;
+    const int tr = tree_of ? tree_of[lane] : 0;
+    const int4 ta = T.tree_a[tr];
+    c.nb = ta.x; c.nq = ta.y; c.nu = ta.z; c.body_off = ta.w;
+    return c;
+}
+
+// ---- record load/store --------------------------------------------------
+__device__ __forceinline__ d3 ld3(const Lane& c, int o) { return {c.s(o), c.s(o+1), c.s(o+2)}; }
+__device__ __forceinline__ void st3(const Lane& c, int o, d3 v) { c.s(o) = v.x; c.s(o+1) = v.y; c.s(o+2) = v.z; }
+__device__ __forceinline__ dv ldsv(const Lane& c, int o) { return {ld3(c, o), ld3(c, o+3)}; }
+__device__ __forceinline__ void stsv(const Lane& c, int o, const dv& v) { st3(c, o, v.w); st3(c, o+3, v.v); }
+__device__ __forceinline__ dm ldm(const Lane& c, int o) { dm r; for (int i=0;i<9;++i) r.a[i] = c.s(o+i); return r; }
+__device__ __forceinline__ void stm(const Lane& c, int o, const dm& r) { for (int i=0;i<9;++i) c.s(o+i) = r.a[i]; }
+__device__ __forceinline__ ds lds(const Lane& c, int o) { return {c.s(o), c.s(o+1), c.s(o+2), c.s(o+3), c.s(o+4), c.s(o+5)}; }
+__device__ __forceinline__ void sts(const Lane& c, int o, const ds& v) {
+    c.s(o) = v.xx; c.s(o+1) = v.yy; c.s(o+2) = v.zz; c.s(o+3) = v.xy; c.s(o+4) = v.xz; c.s(o+5) = v.yz;
+}
+__device__ __forceinline__ da lda(const Lane& c, int o) { da P; P.M = lds(c, o); P.J = lds(c, o+6); P.F = ldm(c, o+12); return P; }
+__device__ __forceinline__ void sta(const Lane& c, int o, const da& P) { sts(c, o, P.M); sts(c, o+6, P.J); stm(c, o+12, P.F); }
+__device__ __forceinline__ void sta_global(double* o, const da& P) {
+    o[0]=P.M.xx; o[1]=P.M.yy; o[2]=P.M.zz; o[3]=P.M.xy; o[4]=P.M.xz; o[5]=P.M.yz;
+    o[6]=P.J.xx; o[7]=P.J.yy; o[8]=P.J.zz; o[9]=P.J.xy; o[10]=P.J.xz; o[11]=P.J.yz;
+    for (int i=0;i<9;++i) o[12+i] = P.F.a[i];
+}
+
+__device__ __forceinline__ dm prm_m(const double* p) { dm r; for (int i=0;i<9;++i) r.a[i] = p[i]; return r; }
+__device__ __forceinline__ d3 prm_3(const double* p) { return {p[0], p[1], p[2]}; }
+
+template <int TYPE> struct Mob {
+    static constexpr int D = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 3 : 6;
+    static constexpr int NQ = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 4 : 7;
+};
+
+// ---------------------------------------------------------------------------
+// Base-to-tip position (+ velocity) kinematics of one body.
+// Restates RigidBodyNodeSpec<dof>::realizePosition/realizeVelocity
+// (RigidBodyNodeSpec.h:229-333) with the mobilizer parts of
+// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free}.h, calcBodyTransforms
+// (RigidBodyNodeSpec.h:554-569), H_PB_G and its derivative
+// (RigidBodyNodeSpec.cpp:44-129) and the joint-independent parts
+// (RigidBodyNode.cpp:54-174).
+// q(k), u(k): this instance's k-th q/u.  qd(k, v): store qdot.
+// ---------------------------------------------------------------------------
+template <int TYPE, bool VEL, class QG, class UG, class QD>
+__device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, const QG& q, const UG& u, const QD& qd) {
+    constexpr int D = Mob<TYPE>::D, NQ = Mob<TYPE>::NQ;
+    const int p = ba.x, qi = ba.z, ui = ba.w;
+    const double* P = c.prm(b);
+    const int rb = c.rec(b);
+    dm R_GP; d3 p_GP;
+    int rp = 0;
+    if (p == 0) { R_GP = ident<double>(); p_GP = zero3<double>(); }
+    else { rp = c.rec(p); R_GP = ldm(c, rp + REC_R); p_GP = ld3(c, rp + REC_P); }
+
+    double qv[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) qv[k] = q(qi + k);
+    dm R_FM; d3 p_FM = zero3<double>();
+    dv H_FM[D];
+    if constexpr (TYPE == PIN) {
+        double sn, cs; sincos(qv[0], &sn, &cs);
+        R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=0; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=0;
+        R_FM.a[6]=0;  R_FM.a[7]=0;   R_FM.a[8]=1;
+        H_FM[0] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+    } else if constexpr (TYPE == SLIDER) {
+        R_FM = ident<double>(); p_FM = V3(qv[0], 0.0, 0.0);
+        H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+    } else if constexpr (TYPE == UNIVERSAL) {
+        double s1, c1, s2, c2; sincos(qv[0], &s1, &c1); sincos(qv[1], &s2, &c2);
+        R_FM.a[0]= c2;    R_FM.a[1]= 0.0; R_FM.a[2]= s2;
+        R_FM.a[3]= s2*s1; R_FM.a[4]= c1;  R_FM.a[5]= -s1*c2;
+        R_FM.a[6]=-s2*c1; R_FM.a[7]= s1;  R_FM.a[8]= c1*c2;
+        H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
+        H_FM[1] = {col(R_FM, 1), zero3<double>()};
+    } else {   // BALL, FREE
+        const double len = sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]);
+        const double oo = 1.0 / len;
+        R_FM = quat_rot(qv[0]*oo, qv[1]*oo, qv[2]*oo, qv[3]*oo);
+        H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
+        H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
+        H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+        if constexpr (TYPE == FREE) {
+            p_FM = V3(qv[4], qv[5], qv[6]);
+            H_FM[3] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+            H_FM[4] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
+            H_FM[5] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
+        }
+    }
+    const dm R_PF = prm_m(P + PRM_R_PF), R_MB = prm_m(P + PRM_R_MB);
+    const d3 p_PF = prm_3(P + PRM_P_PF), p_MB = prm_3(P + PRM_P_MB);
+    // X_PB = X_PF X_FM X_MB, X_GB = X_GP X_PB
+    const dm R_FB = mul(R_FM, R_MB);
+    const d3 p_FB = p_FM + mul(R_FM, p_MB);
+    const dm R_PB = mul(R_PF, R_FB);
+    const d3 p_PB = p_PF + mul(R_PF, p_FB);
+    const dm R_GB = mul(R_GP, R_PB);
+    const d3 p_GB = p_GP + mul(R_GP, p_PB);
+    const dm R_GF = mul(R_GP, R_PF);
+    const d3 r_MB_F = mul(R_FM, p_MB);
+    dv H[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        H[j] = {mul(R_GF, H_FM[j].w), mul(R_GF, H_FM[j].v - cross(r_MB_F, H_FM[j].w))};
+    const d3 phi = mul(R_GP, p_PB);
+    const double m = P[PRM_MASS];
+    const d3 Mk_p = mul(R_GB, prm_3(P + PRM_COM));
+    const ds Gu = {P[PRM_G], P[PRM_G+1], P[PRM_G+2], P[PRM_G+3], P[PRM_G+4], P[PRM_G+5]};
+    const ds Mk_G = rotate_sym(R_GB, Gu);
+    stm(c, rb + REC_R, R_GB); st3(c, rb + REC_P, p_GB); st3(c, rb + REC_PHI, phi);
+    st3(c, rb + REC_MKP, Mk_p); sts(c, rb + REC_MKG, Mk_G);
+    const int rH = rb + REC_DIEPS + D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) stsv(c, rH + 6*j, H[j]);
+
+    if constexpr (VEL) {
+        const dv V_GP = p == 0 ? svzero<double>() : ldsv(c, rp + REC_V);
+        double uv[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) uv[k] = u(ui + k);
+        if constexpr (TYPE == BALL || TYPE == FREE) {
+            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qd(qi + k, d4[k]);
+            if constexpr (TYPE == FREE) { qd(qi+4, uv[3]); qd(qi+5, uv[4]); qd(qi+6, uv[5]); }
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) qd(qi + k, uv[k]);
+        }
+        dv V_FM = svzero<double>(), V_PB_G = svzero<double>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) { V_FM = V_FM + uv[j]*H_FM[j]; V_PB_G = V_PB_G + uv[j]*H[j]; }
+        const d3 w_GF = V_GP.w, w_FM = V_FM.w;
+        const d3 wxr = cross(w_FM, r_MB_F);
+        dv VD = svzero<double>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            d3 hdw = zero3<double>();
+            if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(R_FM, 1)); }
+            const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
+            const dv HD = {mul(R_GF, hdw) + cross(w_GF, H[j].w), mul(R_GF, lin) + cross(w_GF, H[j].v)};
+            VD = VD + uv[j]*HD;
+        }
+        const dv V_GB = phiT_mul(phi, V_GP) + V_PB_G;
+        const d3 w = V_GB.w;
+        const dv bg = {m*cross(w, mul(Mk_G, w)), m*cross(w, cross(w, Mk_p))};
+        const dv a = {VD.w, VD.v + cross(V_GP.w, V_GB.v - V_GP.v)};
+        stsv(c, rb + REC_V, V_GB); stsv(c, rb + REC_A, a); stsv(c, rb + REC_B, bg);
+    }
+}
+
+template <bool VEL, class QG, class UG, class QD>
+__device__ void sweep_kin(const Lane& c, const QG& q, const UG& u, const QD& qd) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        switch (ba.y) {
+        case PIN:       kin_body<PIN, VEL>(c, b, ba, q, u, qd); break;
+        case SLIDER:    kin_body<SLIDER, VEL>(c, b, ba, q, u, qd); break;
+        case UNIVERSAL: kin_body<UNIVERSAL, VEL>(c, b, ba, q, u, qd); break;
+        case BALL:      kin_body<BALL, VEL>(c, b, ba, q, u, qd); break;
+        default:        kin_body<FREE, VEL>(c, b, ba, q, u, qd); break;
+        }
+    }
+}
+
+// Small dense inverse of D (row-major).  dof<=3: cofactor forms (the
+// reference's SmallMatrixMixed.h:895-960); dof=6: Gauss-Jordan elimination
+// without pivoting (D = H^T P H is symmetric positive definite for positive
+// mass; the reference uses LAPACK LU with partial pivoting, agreement is at
+// rounding level).
+template <int D> __device__ __forceinline__ void inv_small(const double* A, double* Ai) {
+    if constexpr (D == 1) { Ai[0] = 1.0 / A[0]; }
+    else if constexpr (D == 2) {
+        const double o = 1.0 / (A[0]*A[3] - A[1]*A[2]);
+        Ai[0] = o*A[3]; Ai[1] = -(o*A[1]); Ai[2] = -(o*A[2]); Ai[3] = o*A[0];
+    } else if constexpr (D == 3) {
+        const double d00 = A[4]*A[8]-A[5]*A[7], nd01 = A[5]*A[6]-A[3]*A[8], d02 = A[3]*A[7]-A[4]*A[6];
+        const double o = 1.0 / (A[0]*d00 + A[1]*nd01 + A[2]*d02);
+        const double nd10 = A[2]*A[7]-A[1]*A[8], d11 = A[0]*A[8]-A[2]*A[6], nd12 = A[1]*A[6]-A[0]*A[7],
+                     d20 = A[1]*A[5]-A[2]*A[4], nd21 = A[2]*A[3]-A[0]*A[5], d22 = A[0]*A[4]-A[1]*A[3];
+        Ai[0]=o*d00;  Ai[1]=o*nd10; Ai[2]=o*d20;
+        Ai[3]=o*nd01; Ai[4]=o*d11;  Ai[5]=o*nd21;
+        Ai[6]=o*d02;  Ai[7]=o*nd12; Ai[8]=o*d22;
+    } else {
+        double M[D*D];
+#pragma unroll
+        for (int i = 0; i < D*D; ++i) { M[i] = A[i]; Ai[i] = (i % (D+1) == 0) ? 1.0 : 0.0; }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double ip = 1.0 / M[k*D+k];
+#pragma unroll
+            for (int j = 0; j < D; ++j) { M[k*D+j] *= ip; Ai[k*D+j] *= ip; }
+#pragma unroll
+            for (int r = 0; r < D; ++r) {
+                if (r == k) continue;
+                const double f = M[r*D+k];
+#pragma unroll
+                for (int j = 0; j < D; ++j) { M[r*D+j] -= f*M[k*D+j]; Ai[r*D+j] -= f*Ai[k*D+j]; }
+            }
+        }
+    }
+}
+
+enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8 };
+
+// ---------------------------------------------------------------------------
+// Tip-to-base: articulated-body inertias (RigidBodyNodeSpec.cpp:249-325) fused
+// with the first ABA pass (calcUDotPass1Inward :355-400) or the first pass of
+// M^-1 f (multiplyByMInvPass1Inward :483-515).  F(b) -> body force, f(k) ->
+// mobility force.
+// ---------------------------------------------------------------------------
+template <int TYPE, int MODE, class FB, class FM>
+__device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
+                                            const FB& fbody, const FM& fmob,
+                                            double* outP, double* outPP, const double* grav) {
+    constexpr int D = Mob<TYPE>::D;
+    const int rb = bb.z;
+    const double* P0 = c.prm(b);
+    const double m = P0[PRM_MASS];
+    const d3 Mk_p = ld3(c, rb + REC_MKP);
+    const ds Mk_G = lds(c, rb + REC_MKG);
+    da P = abi_from_rbi(m, Mk_p, Mk_G);
+    for (int k = 0; k < bb.y; ++k) {
+        const int ch = c.T.child_idx[bb.x + k];
+        const int rc = c.rec(ch);
+        abi_add_shifted(P, lda(c, rc + REC_PPLUS), ld3(c, rc + REC_PHI));
+    }
+    const int rH = rb + REC_DIEPS + D, rG = rb + REC_DIEPS + 7*D;
+    dv H[D], PH[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) { H[j] = ldsv(c, rH + 6*j); PH[j] = abi_mul(P, H[j]); }
+    double Dm[D*D], DI[D*D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int k = 0; k < D; ++k) Dm[r*D+k] = dot(H[r], PH[k]);
+    inv_small<D>(Dm, DI);
+    dv G[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        dv acc = svzero<double>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) acc = acc + DI[j*D+k]*PH[j];
+        G[k] = acc;
+    }
+    // P+ = P - G PH^T with the mass and inertia blocks symmetrized.
+    double mm[9], ms[9], in[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j) {
+                const double gw = r == 0 ? G[j].w.x : r == 1 ? G[j].w.y : G[j].w.z;
+                const double gv = r == 0 ? G[j].v.x : r == 1 ? G[j].v.y : G[j].v.z;
+                const double pv = k == 0 ? PH[j].v.x : k == 1 ? PH[j].v.y : PH[j].v.z;
+                const double pw = k == 0 ? PH[j].w.x : k == 1 ? PH[j].w.y : PH[j].w.z;
+                a0 += gw*pv; a1 += gv*pv; a2 += gw*pw;
+            }
+            mm[r*3+k] = a0; ms[r*3+k] = a1; in[r*3+k] = a2;
+        }
+    da PP;
+    PP.M = P.M - ds{ms[0], ms[4], ms[8], (ms[3]+ms[1])*0.5, (ms[6]+ms[2])*0.5, (ms[7]+ms[5])*0.5};
+    PP.J = P.J - ds{in[0], in[4], in[8], (in[3]+in[1])*0.5, (in[6]+in[2])*0.5, (in[7]+in[5])*0.5};
+#pragma unroll
+    for (int i = 0; i < 9; ++i) PP.F.a[i] = P.F.a[i] - mm[i];
+    sta(c, rb + REC_PPLUS, PP);
+#pragma unroll
+    for (int j = 0; j < D; ++j) stsv(c, rG + 6*j, G[j]);
+    if constexpr (MODE & IN_ABI_OUT) {
+        sta_global(outP + 21*b, P);
+        if (outPP) sta_global(outPP + 21*b, PP);
+    }
+    if constexpr (MODE & (IN_FWD | IN_MINV)) {
+        dv z;
+        if constexpr (MODE & IN_FWD) {
+            const dv a = ldsv(c, rb + REC_A), bg = ldsv(c, rb + REC_B);
+            dv F = svzero<double>();
+            if constexpr (MODE & IN_SYSF) {
+                const d3 g = {grav[0], grav[1], grav[2]};
+                const d3 f = m*g;
+                F = F + dv{cross(Mk_p, f), f};
+            }
+            F = F + fbody(b);
+            z = (abi_mul(P, a) + bg) - F;
+        } else {
+            z = svzero<double>();
+        }
+        for (int k = 0; k < bb.y; ++k) {
+            const int rc = c.rec(c.T.child_idx[bb.x + k]);
+            z = z + phi_mul(ld3(c, rc + REC_PHI), ldsv(c, rc + REC_ZPLUS));
+        }
+        double eps[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) eps[j] = fmob(ba.w + j) - dot(H[j], z);
+        dv zp = z;
+#pragma unroll
+        for (int j = 0; j < D; ++j) zp = zp + eps[j]*G[j];
+        stsv(c, rb + REC_ZPLUS, zp);
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            double s = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) s += DI[r*D+k]*eps[k];
+            c.s(rb + REC_DIEPS + r) = s;
+        }
+    }
+}
+
+template <int MODE, class FB, class FM>
+__device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, double* outP, double* outPP,
+                             const double* grav) {
+    for (int b = c.nb; b >= 1; --b) {
+        const int4 ba = c.ba(b), bb = c.bb(b);
+        switch (ba.y) {
+        case PIN: case SLIDER: inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
+        case UNIVERSAL:        inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
+        case BALL:             inward_body<BALL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
+        default:               inward_body<FREE, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Base-to-tip second pass: calcUDotPass2Outward (RigidBodyNodeSpec.cpp:408-446,
+// WITH_COR) or multiplyByMInvPass2Outward (:521-552), plus calcQDotDot for
+// quaternions (Rotation.h:1006-1013).
+// ---------------------------------------------------------------------------
+template <int TYPE, bool WITH_COR, class QG, class UG, class UO, class QO>
+__device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba, const QG& q, const UG& u,
+                                             const UO& uo, const QO& qdd) {
+    constexpr int D = Mob<TYPE>::D;
+    const int rb = c.rec(b), p = ba.x;
+    const dv A_GP = p == 0 ? svzero<double>() : ldsv(c, c.rec(p) + REC_AGB);
+    const dv Ap = phiT_mul(ld3(c, rb + REC_PHI), A_GP);
+    const int rH = rb + REC_DIEPS + D, rG = rb + REC_DIEPS + 7*D;
+    double ud[D];
+    dv A = Ap;
+#pragma unroll
+    for (int r = 0; r < D; ++r) ud[r] = c.s(rb + REC_DIEPS + r) - dot(ldsv(c, rG + 6*r), Ap);
+    dv Hu = svzero<double>();
+#pragma unroll
+    for (int j = 0; j < D; ++j) Hu = Hu + ud[j]*ldsv(c, rH + 6*j);
+    A = A + Hu;
+    if constexpr (WITH_COR) A = A + ldsv(c, rb + REC_A);
+    stsv(c, rb + REC_AGB, A);
+#pragma unroll
+    for (int j = 0; j < D; ++j) uo(ba.w + j, ud[j]);
+    if constexpr (WITH_COR) {
+        if constexpr (TYPE == BALL || TYPE == FREE) {
+            double qq[4], nb4[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qq[k] = q(ba.z + k);
+            const d3 w = {u(ba.w), u(ba.w+1), u(ba.w+2)};
+            quat_dot(qq, V3(ud[0], ud[1], ud[2]), nb4);
+            const double sc = -0.25 * dot(w, w);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qdd(ba.z + k, nb4[k] + sc*qq[k]);
+            if constexpr (TYPE == FREE) { qdd(ba.z+4, ud[3]); qdd(ba.z+5, ud[4]); qdd(ba.z+6, ud[5]); }
+        } else {
+#pragma unroll
+            for (int j = 0; j < D; ++j) qdd(ba.z + j, ud[j]);
+        }
+    }
+}
+
+template <bool WITH_COR, class QG, class UG, class UO, class QO>
+__device__ void sweep_outward2(const Lane& c, const QG& q, const UG& u, const UO& uo, const QO& qdd) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        switch (ba.y) {
+        case PIN: case SLIDER: outward_body<PIN, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
+        case UNIVERSAL:        outward_body<UNIVERSAL, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
+        case BALL:             outward_body<BALL, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
+        default:               outward_body<FREE, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse dynamics: calcBodyAccelerationsFromUdotOutward (RigidBodyNodeSpec.cpp:
+// 566-582) and calcInverseDynamicsPass2Inward (:609-641).
+// ---------------------------------------------------------------------------
+template <int D, class UD>
+__device__ __forceinline__ void resid_out_body(const Lane& c, int b, const int4 ba, const UD& udot) {
+    const int rb = c.rec(b), p = ba.x;
+    const dv A_GP = p == 0 ? svzero<double>() : ldsv(c, c.rec(p) + REC_AGB);
+    dv A = phiT_mul(ld3(c, rb + REC_PHI), A_GP);
+    const int rH = rb + REC_DIEPS + D;
+    dv Hu = svzero<double>();
+#pragma unroll
+    for (int j = 0; j < D; ++j) Hu = Hu + udot(ba.w + j)*ldsv(c, rH + 6*j);
+    A = (A + Hu) + ldsv(c, rb + REC_A);
+    stsv(c, rb + REC_AGB, A);
+}
+template <int D, class FB, class FM, class TO>
+__device__ __forceinline__ void resid_in_body(const Lane& c, int b, const int4 ba, const int4 bb,
+                                              const FB& fbody, const FM& fmob, const TO& tau) {
+    const int rb = bb.z;
+    const double m = c.prm(b)[PRM_MASS];
+    dv F = rbi_mul(m, ld3(c, rb + REC_MKP), lds(c, rb + REC_MKG), ldsv(c, rb + REC_AGB)) + ldsv(c, rb + REC_B);
+    F = F - fbody(b);
+    for (int k = 0; k < bb.y; ++k) {
+        const int rc = c.rec(c.T.child_idx[bb.x + k]);
+        F = F + phi_mul(ld3(c, rc + REC_PHI), ldsv(c, rc + REC_ZPLUS));
+    }
+    stsv(c, rb + REC_ZPLUS, F);
+    const int rH = rb + REC_DIEPS + D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) tau(ba.w + j, dot(ldsv(c, rH + 6*j), F) - fmob(ba.w + j));
+}
+
+// ---------------------------------------------------------------------------
+// Accessors.
+// ---------------------------------------------------------------------------
+struct PackedGet {   // packed boundary layout, already offset to the instance
+    const double* p;
+    __device__ double operator()(int k) const { return p[k]; }
+};
+struct PackedGetOpt {
+    const double* p;
+    __device__ double operator()(int k) const { return p ? p[k] : 0.0; }
+};
+struct PackedPut {
+    double* p;
+    __device__ void operator()(int k, double v) const { if (p) p[k] = v; }
+};
+struct SoAGet {
+    const double* p; int L, lane;
+    __device__ double operator()(int k) const { return p[(size_t)k * L + lane]; }
+};
+struct SoAPut {
+    double* p; int L, lane;
+    __device__ void operator()(int k, double v) const { p[(size_t)k * L + lane] = v; }
+};
+struct NoPut { __device__ void operator()(int, double) const {} };
+struct Zero { __device__ double operator()(int) const { return 0.0; } };
+struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
+    const double* p;
+    __device__ dv operator()(int b) const {
+        if (!p) return svzero<double>();
+        const double* f = p + 6*b;
+        return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Operator kernels (the batched SimbodyMatterSubsystem operators).
+// ---------------------------------------------------------------------------
+template <int OP>
+__global__ void __launch_bounds__(64) k_op(OpArgs a) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
+    const int tr = a.tree_of ? a.tree_of[lane] : 0;
+    const double* grav = a.T.gravity + 4*tr;
+    const int qo = a.off.q_off[lane], uo = a.off.u_off[lane], bo = a.off.b_off[lane];
+    const PackedGet q{a.q + qo};
+    const PackedGet u{a.u ? a.u + uo : nullptr};
+    if constexpr (OP == OP_KIN) {
+        sweep_kin<true>(c, q, u, PackedPut{a.out_q ? a.out_q + qo : nullptr});
+        double ke = 0, pe = 0;
+        const d3 g = {grav[0], grav[1], grav[2]};
+        for (int b = 1; b <= c.nb; ++b) {
+            const int rb = c.rec(b);
+            const double m = c.prm(b)[PRM_MASS];
+            const dv V = ldsv(c, rb + REC_V);
+            const d3 Mk_p = ld3(c, rb + REC_MKP);
+            ke += dot(V, rbi_mul(m, Mk_p, lds(c, rb + REC_MKG), V)) * 0.5;
+            pe -= m * dot(g, ld3(c, rb + REC_P) + Mk_p);
+            if (a.out_body) {
+                double* X = a.out_body + 12*(size_t)(bo + b);
+                const dm R = ldm(c, rb + REC_R); const d3 pp = ld3(c, rb + REC_P);
+                for (int i = 0; i < 9; ++i) X[i] = R.a[i];
+                X[9] = pp.x; X[10] = pp.y; X[11] = pp.z;
+            }
+            if (a.out_body2) {
+                double* Vo = a.out_body2 + 6*(size_t)(bo + b);
+                Vo[0]=V.w.x; Vo[1]=V.w.y; Vo[2]=V.w.z; Vo[3]=V.v.x; Vo[4]=V.v.y; Vo[5]=V.v.z;
+            }
+        }
+        if (a.out_body) { double* X = a.out_body + 12*(size_t)bo; for (int i = 0; i < 12; ++i) X[i] = (i == 0 || i == 4 || i == 8) ? 1.0 : 0.0; }
+        if (a.out_body2) { double* Vo = a.out_body2 + 6*(size_t)bo; for (int i = 0; i < 6; ++i) Vo[i] = 0.0; }
+        if (a.out_scalar) { a.out_scalar[2*lane] = ke; a.out_scalar[2*lane+1] = pe; }
+    } else if constexpr (OP == OP_FWD_SYS || OP == OP_FWD_GIVEN || OP == OP_REACT) {
+        sweep_kin<true>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
+        const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
+        const PackedGetOpt fm{(OP != OP_FWD_SYS && a.fmob) ? a.fmob + uo : nullptr};
+        if constexpr (OP == OP_FWD_SYS) sweep_inward<IN_FWD | IN_SYSF>(c, fb, fm, nullptr, nullptr, grav);
+        else sweep_inward<IN_FWD>(c, fb, fm, nullptr, nullptr, grav);
+        const PackedPut uout{a.out_u ? a.out_u + uo : nullptr};
+        if constexpr (OP == OP_FWD_GIVEN) {
+            sweep_outward2<true>(c, q, u, uout, PackedPut{a.out_q ? a.out_q + qo : nullptr});
+        } else {
+            sweep_outward2<true>(c, q, u, uout, NoPut{});
+        }
+        if constexpr (OP == OP_REACT) {
+            // Mobilizer reaction forces at M, in G (SimbodyMatterSubsystemRep.cpp:5786-5826).
+            for (int b = 0; b <= c.nb; ++b) {
+                double* o = a.out_body + 6*(size_t)(bo + b);
+                if (b == 0) { for (int i = 0; i < 6; ++i) o[i] = 0.0; continue; }
+                const int4 ba = c.ba(b);
+                const int rb = c.rec(b);
+                const d3 pB = ld3(c, rb + REC_P);
+                dv A_GP = svzero<double>(); d3 pP = zero3<double>();
+                if (ba.x != 0) { const int rp = c.rec(ba.x); A_GP = ldsv(c, rp + REC_AGB); pP = ld3(c, rp + REC_P); }
+                const d3 l = pB - pP;
+                const dv Ap = {A_GP.w, A_GP.v + cross(A_GP.w, l)};
+                const dv FB = ldsv(c, rb + REC_ZPLUS) + abi_mul(lda(c, rb + REC_PPLUS), Ap);
+                const d3 pBM = mul(ldm(c, rb + REC_R), prm_3(c.prm(b) + PRM_P_BM));
+                const d3 mo = FB.w - cross(pBM, FB.v);
+                o[0]=mo.x; o[1]=mo.y; o[2]=mo.z; o[3]=FB.v.x; o[4]=FB.v.y; o[5]=FB.v.z;
+            }
+        } else if (OP == OP_FWD_GIVEN && a.out_body) {
+            for (int b = 0; b <= c.nb; ++b) {
+                const dv A = b == 0 ? svzero<double>() : ldsv(c, c.rec(b) + REC_AGB);
+                double* o = a.out_body + 6*(size_t)(bo + b);
+                o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
+            }
+        }
+    } else if constexpr (OP == OP_MINV) {
+        sweep_kin<false>(c, q, u, NoPut{});
+        const PackedGet f{a.fmob + uo};
+        sweep_inward<IN_MINV>(c, BodyForce{nullptr}, f, nullptr, nullptr, grav);
+        sweep_outward2<false>(c, q, u, PackedPut{a.out_u + uo}, NoPut{});
+    } else if constexpr (OP == OP_ABI) {
+        sweep_kin<false>(c, q, u, NoPut{});
+        sweep_inward<IN_ABI_OUT>(c, BodyForce{nullptr}, Zero{}, a.out_abi + 21*(size_t)bo,
+                                 a.out_abi2 ? a.out_abi2 + 21*(size_t)bo : nullptr, grav);
+        for (int i = 0; i < 21; ++i) {
+            a.out_abi[21*(size_t)bo + i] = 0.0;
+            if (a.out_abi2) a.out_abi2[21*(size_t)bo + i] = 0.0;
+        }
+    } else if constexpr (OP == OP_RESID) {
+        sweep_kin<true>(c, q, u, NoPut{});
+        const PackedGetOpt ud{a.udot_in ? a.udot_in + uo : nullptr};
+        const PackedGetOpt fm{a.fmob ? a.fmob + uo : nullptr};
+        const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
+        const PackedPut tau{a.out_u + uo};
+        for (int b = 1; b <= c.nb; ++b) {
+            const int4 ba = c.ba(b);
+            switch (mob_dof(ba.y)) {
+            case 1: resid_out_body<1>(c, b, ba, ud); break;
+            case 2: resid_out_body<2>(c, b, ba, ud); break;
+            case 3: resid_out_body<3>(c, b, ba, ud); break;
+            default: resid_out_body<6>(c, b, ba, ud); break;
+            }
+        }
+        for (int b = c.nb; b >= 1; --b) {
+            const int4 ba = c.ba(b), bb = c.bb(b);
+            switch (mob_dof(ba.y)) {
+            case 1: resid_in_body<1>(c, b, ba, bb, fb, fm, tau); break;
+            case 2: resid_in_body<2>(c, b, ba, bb, fb, fm, tau); break;
+            case 3: resid_in_body<3>(c, b, ba, bb, fb, fm, tau); break;
+            default: resid_in_body<6>(c, b, ba, bb, fb, fm, tau); break;
+            }
+        }
+    }
+}
+
+hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    const dim3 blk(64), grd((a.n_inst + 63) / 64);
+    switch (k) {
+    case OP_KIN:       k_op<OP_KIN><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_SYS:   k_op<OP_FWD_SYS><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_GIVEN: k_op<OP_FWD_GIVEN><<<grd, blk, 0, s>>>(a); break;
+    case OP_MINV:      k_op<OP_MINV><<<grd, blk, 0, s>>>(a); break;
+    case OP_RESID:     k_op<OP_RESID><<<grd, blk, 0, s>>>(a); break;
+    case OP_ABI:       k_op<OP_ABI><<<grd, blk, 0, s>>>(a); break;
+    case OP_REACT:     k_op<OP_REACT><<<grd, blk, 0, s>>>(a); break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Runge-Kutta-Merson rollouts with per-instance step control.  One lane runs
+// one instance's TimeStepper loop: RungeKuttaMersonIntegrator.cpp:86-140
+// (stages), AbstractIntegratorRep.cpp:137-208 (projection gate), :448-502
+// (adjustStepSize), :513-609 (takeOneStep), IntegratorRep.h:454-523, 644-655
+// (error norm), SimbodyMatterSubsystemRep.cpp:4096-4184, 4448-4475 (quaternion
+// projection).  Lanes stop after cfg.max_steps accepted steps so a rollout can
+// be resumed by the next launch (the serving loop relies on this).
+// ---------------------------------------------------------------------------
+enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, NVEC = 9 };
+
+__device__ void rkm_derivs(const Lane& c, const double* grav, const double* fapp, const SoAGet& y, const SoAPut& f) {
+    const SoAGet u{y.p + (size_t)c.nq * y.L, y.L, y.lane};
+    const SoAPut fu{f.p + (size_t)c.nq * f.L, f.L, f.lane};
+    sweep_kin<true>(c, y, u, f);
+    sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+    sweep_outward2<true>(c, y, u, fu, NoPut{});
+}
+
+__global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    if (a.status[lane] != 0) return;
+    const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
+    const int tr = a.tree_of ? a.tree_of[lane] : 0;
+    const double* grav = a.T.gravity + 4*tr;
+    const double* fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
+    const int nq = c.nq, nu = c.nu, ny = nq + nu;
+    const int nquat = a.T.tree_b[tr].y;
+    const size_t NV = (size_t)a.ny_max * a.L;
+    auto V = [&](int v, int k) -> double& { return a.vec[v*NV + (size_t)k * a.L + lane]; };
+    auto US = [&](int k) -> double& { return a.uscale[(size_t)k * a.L + lane]; };
+    const SoAGet gY0{a.vec + VY0*NV, a.L, lane}, gYT{a.vec + VYT*NV, a.L, lane}, gYS{a.vec + VYS*NV, a.L, lane};
+    const SoAPut pF0{a.vec + VF0*NV, a.L, lane}, pFA{a.vec + VFA*NV, a.L, lane}, pFB{a.vec + VFB*NV, a.L, lane};
+
+    const RkmCfg& cfg = a.cfg;
+    const double acc = cfg.accuracy, consTol = cfg.cons_tol;
+    double hmin = cfg.min_h, hmax = cfg.max_h;
+    if (cfg.fixed_h > 0) { hmin = hmax = cfg.fixed_h; }
+    const double projLimit = fmax(2*consTol, sqrt(consTol));
+    double t = a.t[lane], hcur = a.h[lane];
+    const double tEnd = a.t_end[lane];
+    int attempts = a.stats[4*lane], accepted = a.stats[4*lane+1], rejected = a.stats[4*lane+2],
+        projections = a.stats[4*lane+3];
+    int status = 0, steps = 0;
+
+    while (steps < cfg.max_steps) {
+        if (t >= tEnd) { status = 1; break; }
+        for (int k = 0; k < ny; ++k) V(VY0, k) = V(VY, k);
+        for (int k = 0; k < nu; ++k) { const double vi = fabs(V(VY0, nq + k)); US(k) = vi > 1.0 ? 1.0/vi : 1.0; }
+        rkm_derivs(c, grav, fapp, gY0, pF0);
+        bool ok = false;
+        double t1 = t;
+        do {
+            bool limited = false;
+            if (tEnd < t + 0.95*hcur) { limited = true; t1 = tEnd; }
+            else if (tEnd > t + 1.001*hcur) t1 = t + hcur;
+            else t1 = tEnd;
+            if (!(t1 > t)) { status = -3; break; }
+            if (++attempts > cfg.max_attempts) { status = -2; break; }
+            const double h = t1 - t;
+            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/3)*V(VF0, k);
+            rkm_derivs(c, grav, fapp, gYT, pFA);
+            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/6)*(V(VF0, k) + V(VFA, k));
+            rkm_derivs(c, grav, fapp, gYT, pFA);
+            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/8)*(V(VF0, k) + 3*V(VFA, k));
+            rkm_derivs(c, grav, fapp, gYT, pFB);
+            for (int k = 0; k < ny; ++k) V(VYS, k) = V(VY0, k) + (h/2)*((V(VF0, k) - 3*V(VFA, k)) + 4*V(VFB, k));
+            rkm_derivs(c, grav, fapp, gYS, pFA);
+            for (int k = 0; k < ny; ++k) {
+                const double y1 = V(VY0, k) + (h/6)*((V(VF0, k) + 4*V(VFB, k)) + V(VFA, k));
+                V(VY1, k) = y1;
+                V(VYE, k) = 0.2 * fabs(y1 - V(VYS, k));
+            }
+            // error norm at y1 (q part weighted by N NInv for quaternions)
+            auto errNorm = [&]() -> double {
+                double sq = 0;
+                for (int b = 1; b <= c.nb; ++b) {
+                    const int4 ba = c.ba(b);
+                    const int qi = ba.z;
+                    if (ba.y == BALL || ba.y == FREE) {
+                        double qq[4], e[4], d4[4];
+                        for (int k = 0; k < 4; ++k) { qq[k] = V(VY1, qi + k); e[k] = V(VYE, qi + k); }
+                        const d3 w = quat_dot_to_w(qq, e);
+                        quat_dot(qq, w, d4);
+                        sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                        if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = V(VYE, qi + k); sq += x*x; }
+                    } else {
+                        for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = V(VYE, qi + k); sq += x*x; }
+                    }
+                }
+                const double qn = nq ? sqrt(sq / nq) : 0.0;
+                double su = 0;
+                for (int k = 0; k < nu; ++k) { const double x = US(k) * V(VYE, nq + k); su += x*x; }
+                const double un = nu ? sqrt(su / nu) : 0.0;
+                return qn >= un ? qn : un;
+            };
+            bool converged = true;
+            const double e1 = errNorm();
+            if (!(e1 > 16.0*acc) && nquat > 0) {
+                double s2 = 0;
+                for (int b = 1; b <= c.nb; ++b) {
+                    const int4 ba = c.ba(b);
+                    if (ba.y != BALL && ba.y != FREE) continue;
+                    double n2 = 0;
+                    for (int k = 0; k < 4; ++k) { const double x = V(VY1, ba.z + k); n2 += x*x; }
+                    const double e = sqrt(n2) - 1.0;
+                    s2 += e*e;
+                }
+                const double qe = sqrt(s2 / nquat);
+                if (qe > projLimit) converged = false;
+                else if (qe > consTol) {
+                    ++projections;
+                    for (int b = 1; b <= c.nb; ++b) {
+                        const int4 ba = c.ba(b);
+                        if (ba.y != BALL && ba.y != FREE) continue;
+                        double qq[4], n2 = 0;
+                        for (int k = 0; k < 4; ++k) { qq[k] = V(VY1, ba.z + k); n2 += qq[k]*qq[k]; }
+                        const double n = sqrt(n2);
+                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; V(VY1, ba.z + k) = qq[k]; }
+                        double d = 0;
+                        for (int k = 0; k < 4; ++k) d += V(VYE, ba.z + k) * qq[k];
+                        for (int k = 0; k < 4; ++k) V(VYE, ba.z + k) = V(VYE, ba.z + k) - d*qq[k];
+                    }
+                }
+            }
+            const double err = converged ? errNorm() : __builtin_inf();
+            double hnew;
+            if (!isfinite(err)) hnew = 0.1*hcur;
+            else if (err == 0) hnew = 5.0*hcur;
+            else hnew = 0.9*hcur*pow(acc/err, 0.25);
+            if (hnew > hcur) { if (limited || hnew < 1.2*hcur) hnew = hcur; }
+            if (hnew < hcur) { if (err <= acc) hnew = hcur; else hnew = fmin(hnew, 0.9*hcur); }
+            hnew = fmin(hnew, 5.0*hcur);
+            hnew = fmax(hnew, 0.1*hcur);
+            if (hmin != -1) hnew = fmax(hnew, hmin);
+            if (hmax != -1) hnew = fmin(hnew, hmax);
+            ok = hnew >= hcur;
+            hcur = hnew;
+            if (!ok) ++rejected;
+        } while (!ok);
+        if (status < 0) break;
+        t = t1;
+        for (int k = 0; k < ny; ++k) V(VY, k) = V(VY1, k);
+        ++accepted; ++steps;
+    }
+    if (status == 0 && t >= tEnd) status = 1;
+    a.t[lane] = t; a.h[lane] = hcur; a.status[lane] = status;
+    a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
+}
+
+// initialize(): forced quaternion normalization, t and h set.
+__global__ void __launch_bounds__(64) k_rkm_init(RkmArgs a, double t0) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
+    const size_t NV = (size_t)a.ny_max * a.L;
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        if (ba.y != BALL && ba.y != FREE) continue;
+        double qq[4], n2 = 0;
+        for (int k = 0; k < 4; ++k) { qq[k] = a.vec[VY*NV + (size_t)(ba.z + k)*a.L + lane]; n2 += qq[k]*qq[k]; }
+        const double n = sqrt(n2);
+        for (int k = 0; k < 4; ++k) a.vec[VY*NV + (size_t)(ba.z + k)*a.L + lane] = qq[k] / n;
+    }
+    const RkmCfg& cfg = a.cfg;
+    double hmin = cfg.min_h, hmax = cfg.max_h, hinit = cfg.init_h;
+    if (cfg.fixed_h > 0) hmin = hmax = hinit = cfg.fixed_h;
+    double h = hinit != -1 ? hinit : cfg.timescale / 10;
+    if (hmin != -1) h = fmax(h, hmin);
+    if (hmax != -1) h = fmin(h, hmax);
+    a.t[lane] = t0; a.h[lane] = h; a.status[lane] = 0;
+    for (int k = 0; k < 4; ++k) a.stats[4*lane + k] = 0;
+}
+
+hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    k_rkm_init<<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a, t0);
+    return hipGetLastError();
+}
+hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    k_rkm<<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// Packed boundary layout <-> SoA state vector VY.
+__global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, double* q_out, double* u_out) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    const int tr = a.tree_of ? a.tree_of[lane] : 0;
+    const int4 ta = a.T.tree_a[tr];
+    const int nq = ta.y, nu = ta.z;
+    const int qo = a.off.q_off[lane], uo = a.off.u_off[lane];
+    double* Y = a.vec;   // VY
+    if (to_soa) {
+        for (int k = 0; k < nq; ++k) Y[(size_t)k*a.L + lane] = q[qo + k];
+        for (int k = 0; k < nu; ++k) Y[(size_t)(nq + k)*a.L + lane] = u[uo + k];
+    } else {
+        for (int k = 0; k < nq; ++k) q_out[qo + k] = Y[(size_t)k*a.L + lane];
+        for (int k = 0; k < nu; ++k) u_out[uo + k] = Y[(size_t)(nq + k)*a.L + lane];
+    }
+}
+hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
+                             double* q_out, double* u_out, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    k_pack<<<dim3((a.n_inst + 255)/256), dim3(256), 0, s>>>(a, q, u, to_soa, q_out, u_out);
+    return hipGetLastError();
+}
+
+} // namespace tdyn

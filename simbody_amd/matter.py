@@ -1,0 +1,284 @@
+"""Host-side Python mirror of the tdyn C ABI (include/tdyn.h).
+
+`MatterBatch` offers Simbody's SimbodyMatterSubsystem operator surface for a
+batch of independent tree instances, under the reference's method names
+(SimbodyMatterSubsystem.h): calcAccelerationIgnoringConstraints,
+calcResidualForceIgnoringConstraints, multiplyByMInv,
+realizeArticulatedBodyInertias/getArticulatedBodyInertia, calcKineticEnergy,
+calcMobilizerReactionForces, plus realizeAcceleration of a MultibodySystem with
+gravity, and `RungeKuttaMerson` / `TimeStepper.stepTo` for integration.
+
+Arrays are torch float64 CUDA tensors in the packed instance-major layout of
+the C ABI; the library's HIP kernels do all arithmetic.  There is no CPU
+fallback: without the built libtdyn.so every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtdyn.so")
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i32 = ctypes.POINTER(ctypes.c_int32)
+
+TDYN_ERRORS = {1: "TDYN_E_ARG", 2: "TDYN_E_SIZE", 3: "TDYN_E_STAGE", 4: "TDYN_E_SINGULAR",
+               5: "TDYN_E_STEP_FAILED", 6: "TDYN_E_HIP", 7: "TDYN_E_NOMEM"}
+
+EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_info", "tdyn_batch_offsets",
+           "tdyn_realize_kinematics", "tdyn_realize_acceleration",
+           "tdyn_calc_acceleration_ignoring_constraints", "tdyn_multiply_by_m_inv",
+           "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
+           "tdyn_calc_mobilizer_reaction_forces", "tdyn_rollout_begin", "tdyn_rollout_advance",
+           "tdyn_rollout_get", "tdyn_rollout", "tdyn_last_error"]
+
+
+class TdynError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{TDYN_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class TreeDescC(ctypes.Structure):
+    _fields_ = [("nb", ctypes.c_int32), ("parent", _i32), ("mobilizer", _i32),
+                ("mass", _d), ("com_B", _d), ("inertia_B", _d),
+                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3)]
+
+
+class IntegratorCfgC(ctypes.Structure):
+    _fields_ = [("accuracy", ctypes.c_double), ("constraint_tolerance", ctypes.c_double),
+                ("fixed_step", ctypes.c_double), ("initial_step", ctypes.c_double),
+                ("min_step", ctypes.c_double), ("max_step", ctypes.c_double),
+                ("time_scale", ctypes.c_double), ("max_attempts", ctypes.c_int32),
+                ("precision", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libtdyn.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library not built: {LIB_PATH} missing "
+                               "(run __graft_entry__.build() or make -C simbody_amd/csrc)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = ctypes.c_int
+        L.tdyn_last_error.restype = ctypes.c_char_p
+        vp = ctypes.c_void_p
+        L.tdyn_batch_create.argtypes = [ctypes.POINTER(TreeDescC), ctypes.c_int, _i32, ctypes.c_int,
+                                        ctypes.POINTER(vp)]
+        L.tdyn_batch_destroy.argtypes = [vp]
+        L.tdyn_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.tdyn_batch_offsets.argtypes = [vp, _i32, _i32, _i32]
+        L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
+        L.tdyn_realize_acceleration.argtypes = [vp] + [vp] * 5 + [vp]
+        L.tdyn_calc_acceleration_ignoring_constraints.argtypes = [vp] + [vp] * 7 + [vp]
+        L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
+        L.tdyn_calc_residual_force_ignoring_constraints.argtypes = [vp] + [vp] * 6 + [vp]
+        L.tdyn_calc_articulated_body_inertias.argtypes = [vp] + [vp] * 3 + [vp]
+        L.tdyn_calc_mobilizer_reaction_forces.argtypes = [vp] + [vp] * 5 + [vp]
+        L.tdyn_rollout_begin.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, ctypes.c_double,
+                                         ctypes.POINTER(IntegratorCfgC), vp]
+        L.tdyn_rollout_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_rollout_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.tdyn_rollout.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(IntegratorCfgC), vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TdynError(rc, lib().tdyn_last_error().decode())
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    import torch
+    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+    assert t.dtype in (torch.float64, torch.int32)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class RungeKuttaMerson:
+    """RungeKuttaMersonIntegrator settings (Integrator.cpp:243-270)."""
+
+    def __init__(self, accuracy=1e-3, fixed_step=0.0, constraint_tolerance=0.0, initial_step=0.0,
+                 min_step=0.0, max_step=0.0, time_scale=0.1, max_attempts=0):
+        self.accuracy, self.fixed_step = accuracy, fixed_step
+        self.constraint_tolerance, self.initial_step = constraint_tolerance, initial_step
+        self.min_step, self.max_step, self.time_scale = min_step, max_step, time_scale
+        self.max_attempts = max_attempts
+
+    def c(self):
+        return IntegratorCfgC(self.accuracy, self.constraint_tolerance, self.fixed_step,
+                              self.initial_step, self.min_step, self.max_step, self.time_scale,
+                              self.max_attempts, 0)
+
+
+class MatterBatch:
+    """A batch of tree instances on one GPU (instance i uses trees[tree_of[i]])."""
+
+    def __init__(self, trees, tree_of=None, n_instances=None, device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.device = torch.device(device)
+        if not isinstance(trees, (list, tuple)):
+            trees = [trees]
+        self.trees = list(trees)
+        if tree_of is not None:
+            tree_of = np.ascontiguousarray(tree_of, np.int32)
+            n_instances = len(tree_of)
+        elif n_instances is None:
+            n_instances = 1
+        self.tree_of = tree_of
+        self._keep = []
+        descs = (TreeDescC * len(trees))()
+        for k, s in enumerate(trees):
+            arrs = [np.ascontiguousarray(s.parent, np.int32), np.ascontiguousarray(s.mobilizer, np.int32)] + \
+                   [np.ascontiguousarray(x, np.float64) for x in (s.mass, s.com_B, s.inertia_B, s.X_PF, s.X_BM)]
+            self._keep.append(arrs)
+            d = descs[k]
+            d.nb = len(arrs[0])
+            d.parent = arrs[0].ctypes.data_as(_i32); d.mobilizer = arrs[1].ctypes.data_as(_i32)
+            d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (a.ctypes.data_as(_d) for a in arrs[2:])
+            for j in range(3):
+                d.gravity[j] = float(s.gravity[j])
+        with torch.cuda.device(self.device):
+            h = ctypes.c_void_p()
+            _check(lib().tdyn_batch_create(descs, len(trees),
+                                           tree_of.ctypes.data_as(_i32) if tree_of is not None else None,
+                                           n_instances, ctypes.byref(h)))
+        self.h = h
+        info = (ctypes.c_int64 * 8)()
+        _check(lib().tdyn_batch_info(self.h, info))
+        (self.n_instances, self.total_q, self.total_u, self.total_b,
+         self.max_nq, self.max_nu, self.max_nb, self.n_trees) = list(info)
+        qo = np.zeros(self.n_instances, np.int32); uo = np.zeros_like(qo); bo = np.zeros_like(qo)
+        _check(lib().tdyn_batch_offsets(self.h, qo.ctypes.data_as(_i32), uo.ctypes.data_as(_i32),
+                                        bo.ctypes.data_as(_i32)))
+        self.q_off, self.u_off, self.b_off = qo, uo, bo
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.tdyn_batch_destroy(self.h)
+            self.h = None
+
+    # ---- helpers ---------------------------------------------------------
+    def _new(self, n):
+        return self.torch.zeros(int(n), dtype=self.torch.float64, device=self.device)
+
+    def to_device(self, a):
+        return self.torch.as_tensor(np.ascontiguousarray(a, np.float64).reshape(-1), device=self.device)
+
+    def pack(self, per_instance):
+        """Concatenate per-instance arrays into the packed layout (device)."""
+        return self.to_device(np.concatenate([np.asarray(x, np.float64).reshape(-1) for x in per_instance]))
+
+    def split_q(self, packed):
+        a = packed.detach().cpu().numpy() if hasattr(packed, "detach") else packed
+        ends = list(self.q_off[1:]) + [self.total_q]
+        return [a[s:e] for s, e in zip(self.q_off, ends)]
+
+    def split_u(self, packed):
+        a = packed.detach().cpu().numpy() if hasattr(packed, "detach") else packed
+        ends = list(self.u_off[1:]) + [self.total_u]
+        return [a[s:e] for s, e in zip(self.u_off, ends)]
+
+    def split_b(self, packed, width):
+        a = packed.detach().cpu().numpy() if hasattr(packed, "detach") else packed
+        a = a.reshape(-1, width)
+        ends = list(self.b_off[1:]) + [self.total_b]
+        return [a[s:e] for s, e in zip(self.b_off, ends)]
+
+    # ---- SimbodyMatterSubsystem surface -----------------------------------
+    def realizeKinematics(self, q, u, want_X=True, want_V=True):
+        X = self._new(12 * self.total_b) if want_X else None
+        V = self._new(6 * self.total_b) if want_V else None
+        qd = self._new(self.total_q); E = self._new(2 * self.n_instances)
+        _check(lib().tdyn_realize_kinematics(self.h, _ptr(q), _ptr(u), _ptr(X), _ptr(V), _ptr(qd),
+                                             _ptr(E), _stream()))
+        return X, V, qd, E
+
+    def calcKineticEnergy(self, q, u):
+        _, _, _, E = self.realizeKinematics(q, u, False, False)
+        return E[0::2]
+
+    def realizeAcceleration(self, q, u, fapp=None):
+        ud = self._new(self.total_u); qd = self._new(self.total_q)
+        _check(lib().tdyn_realize_acceleration(self.h, _ptr(q), _ptr(u), _ptr(fapp), _ptr(ud), _ptr(qd),
+                                               _stream()))
+        return ud, qd
+
+    def calcAccelerationIgnoringConstraints(self, q, u, fmob, F, want_A=True, want_qdotdot=True):
+        ud = self._new(self.total_u)
+        A = self._new(6 * self.total_b) if want_A else None
+        qdd = self._new(self.total_q) if want_qdotdot else None
+        _check(lib().tdyn_calc_acceleration_ignoring_constraints(
+            self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(ud), _ptr(A), _ptr(qdd), _stream()))
+        return ud, A, qdd
+
+    def multiplyByMInv(self, q, v):
+        out = self._new(self.total_u)
+        _check(lib().tdyn_multiply_by_m_inv(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
+        return out
+
+    def calcResidualForceIgnoringConstraints(self, q, u, fmob=None, F=None, udot=None):
+        out = self._new(self.total_u)
+        _check(lib().tdyn_calc_residual_force_ignoring_constraints(
+            self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(udot), _ptr(out), _stream()))
+        return out
+
+    def realizeArticulatedBodyInertias(self, q, want_plus=True):
+        P = self._new(21 * self.total_b)
+        PP = self._new(21 * self.total_b) if want_plus else None
+        _check(lib().tdyn_calc_articulated_body_inertias(self.h, _ptr(q), _ptr(P), _ptr(PP), _stream()))
+        return P, PP
+
+    def calcMobilizerReactionForces(self, q, u, fmob, F):
+        out = self._new(6 * self.total_b)
+        _check(lib().tdyn_calc_mobilizer_reaction_forces(self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F),
+                                                         _ptr(out), _stream()))
+        return out
+
+    # ---- RungeKuttaMersonIntegrator + TimeStepper ------------------------
+    def rollout_begin(self, q, u, integ, t_end, fapp=None, t0=0.0):
+        cfg = integ.c()
+        if hasattr(t_end, "is_cuda"):
+            _check(lib().tdyn_rollout_begin(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.c_double(t0),
+                                            _ptr(t_end), ctypes.c_double(0.0), ctypes.byref(cfg), _stream()))
+        else:
+            _check(lib().tdyn_rollout_begin(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.c_double(t0),
+                                            None, ctypes.c_double(float(t_end)), ctypes.byref(cfg), _stream()))
+
+    def rollout_advance(self, max_steps):
+        _check(lib().tdyn_rollout_advance(self.h, int(max_steps), _stream()))
+
+    def rollout_get(self):
+        torch = self.torch
+        q = self._new(self.total_q); u = self._new(self.total_u); t = self._new(self.n_instances)
+        st = torch.zeros(self.n_instances, dtype=torch.int32, device=self.device)
+        stats = torch.zeros(4 * self.n_instances, dtype=torch.int32, device=self.device)
+        _check(lib().tdyn_rollout_get(self.h, _ptr(q), _ptr(u), _ptr(t), _ptr(st), _ptr(stats), _stream()))
+        return q, u, t, st, stats.view(-1, 4)
+
+    def stepTo(self, q, u, integ, t_end, fapp=None, t0=0.0):
+        """TimeStepper(system, integ).initialize(state); stepTo(t_end), with
+        Integrator::setFinalTime(t_end).  Returns (q, u, stats[n,4])."""
+        torch = self.torch
+        q = q.clone(); u = u.clone()
+        stats = torch.zeros(4 * self.n_instances, dtype=torch.int32, device=self.device)
+        cfg = integ.c()
+        _check(lib().tdyn_rollout(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.c_double(t0),
+                                  ctypes.c_double(t_end), ctypes.byref(cfg), _ptr(stats), _stream()))
+        return q, u, stats.view(-1, 4)
