@@ -77,6 +77,12 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
                       int n_instances, tdyn_batch** out);
 int tdyn_batch_destroy(tdyn_batch* b);
 
+/* Derivative evaluator of the rollouts (no reference counterpart; a tuning
+ * knob): 0 = automatic (the register-resident serial-chain evaluator when
+ * every instance shares one chain model, else the tree sweeps), 1 = tree
+ * sweeps, 2 = chain evaluator (TDYN_E_ARG unless the batch is one chain model). */
+int tdyn_batch_set_evaluator(tdyn_batch* b, int mode);
+
 /* info[0..7] = n_instances, total q, total u, total bodies (incl. Ground),
  * max nq, max nu, max nb, n_trees. */
 int tdyn_batch_info(const tdyn_batch* b, int64_t* info);

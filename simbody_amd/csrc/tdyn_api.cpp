@@ -57,6 +57,8 @@ struct tdyn_batch {
     DBuf<double> fapp;
     int ny_max = 0;
     bool rollout_ready = false;
+    bool is_chain = false;       // uniform batch whose tree is a serial chain
+    int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
     RkmCfg cfg{};
 
     DevTables tables() const {
@@ -79,6 +81,7 @@ struct tdyn_batch {
         a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = 0;
+        a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         return a;
     }
 };
@@ -134,13 +137,15 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         const int nb = d.nb;
         std::vector<std::vector<int>> ch(nb + 1);
         for (int b = 1; b <= nb; ++b) ch[d.parent[b-1]].push_back(b);
-        int q = 0, u = 0, rec = 0, nquat = 0;
+        int q = 0, u = 0, rec = 0, nquat = 0, gslot = 0;
+        bool chain = true;
         for (int b = 1; b <= nb; ++b) {
             const int ty = d.mobilizer[b-1];
+            chain = chain && d.parent[b-1] == b - 1;
             B->h_body_a.push_back(make_int4(d.parent[b-1], ty, q, u));
-            body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, 0));
+            body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, gslot));
             for (int c : ch[b]) child_idx.push_back(c);
-            q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(mob_dof(ty));
+            q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(mob_dof(ty)); gslot += 7 * mob_dof(ty);
             if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
             // constants
             double P[PRM_STRIDE] = {0};
@@ -164,6 +169,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         B->h_tree_a.push_back(make_int4(nb, q, u, body_off));
         B->h_tree_b.push_back(make_int4(rec, nquat, 0, 0));
         tq[t] = q; tu[t] = u; tnb[t] = nb;
+        if (t == 0) B->is_chain = chain;
         B->max_rec = std::max(B->max_rec, rec);
         body_off += nb;
     }
@@ -180,6 +186,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     }
     if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
     B->total_q = q; B->total_u = u; B->total_b = bb;
+    B->is_chain = B->is_chain && B->uniform;
     hipError_t e;
     if ((e = B->body_a.upload(B->h_body_a)) != hipSuccess ||
         (e = B->body_b.upload(body_b)) != hipSuccess ||
@@ -201,6 +208,14 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
 }
 
 int tdyn_batch_destroy(tdyn_batch* b) { delete b; return TDYN_OK; }
+
+int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (mode < 0 || mode > 2) return fail(TDYN_E_ARG, "evaluator mode must be 0, 1 or 2");
+    if (mode == 2 && !b->is_chain) return fail(TDYN_E_ARG, "chain evaluator needs one serial-chain model for the whole batch");
+    b->evaluator = mode;
+    return TDYN_OK;
+}
 
 int tdyn_batch_info(const tdyn_batch* b, int64_t* info) {
     if (!b || !info) return fail(TDYN_E_ARG, "null argument");

@@ -95,6 +95,7 @@ struct RkmArgs {
     double* uscale;          // SoA nu_max*L
     RkmCfg cfg;
     int precision;           // 0 = FP64, 1 = mixed
+    int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
 };
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);

@@ -11,6 +11,7 @@
 // reference never exploits.
 #include "tdyn_internal.hpp"
 #include "tdyn_math.hpp"
+#include "tdyn_chain.hpp"
 
 namespace tdyn {
 
@@ -620,22 +621,94 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, NVEC = 9 };
 
-__device__ void rkm_derivs(const Lane& c, const double* grav, const double* fapp, const SoAGet& y, const SoAPut& f) {
-    const SoAGet u{y.p + (size_t)c.nq * y.L, y.L, y.lane};
-    const SoAPut fu{f.p + (size_t)c.nq * f.L, f.L, f.lane};
-    sweep_kin<true>(c, y, u, f);
-    sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
-    sweep_outward2<true>(c, y, u, fu, NoPut{});
-}
+// Derivative evaluators: y (SoA q then u) -> f (SoA qdot then udot).
+struct TreeEval {
+    Lane c; const double* grav; const double* fapp;
+    __device__ TreeEval(const RkmArgs& a, int lane, int tr)
+        : c(make_lane(a.T, a.tree_of, lane, a.L, a.scratch)), grav(a.T.gravity + 4*tr),
+          fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr) {}
+    __device__ __noinline__ void derivs(const SoAGet& y, const SoAPut& f) const {
+        const SoAGet u{y.p + (size_t)c.nq * y.L, y.L, y.lane};
+        const SoAPut fu{f.p + (size_t)c.nq * f.L, f.L, f.lane};
+        sweep_kin<true>(c, y, u, f);
+        sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+        sweep_outward2<true>(c, y, u, fu, NoPut{});
+    }
+};
 
+struct ChainEval {
+    const double* prm; const int4* body_a; const int4* body_b;
+    int nb, nq;
+    chain::GRec gr; d3 grav; const double* fapp;
+    __device__ ChainEval(const RkmArgs& a, int lane, int tr) {
+        const int4 ta = a.T.tree_a[tr];
+        nb = ta.x; nq = ta.y;
+        prm = a.T.prm + (size_t)ta.w * PRM_STRIDE; body_a = a.T.body_a + ta.w; body_b = a.T.body_b + ta.w;
+        gr = {a.scratch, a.L, lane};
+        grav = V3(a.T.gravity[4*tr], a.T.gravity[4*tr+1], a.T.gravity[4*tr+2]);
+        fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
+    }
+    __device__ dv F(int b) const {
+        if (!fapp) return svzero<double>();
+        const double* f = fapp + 6*b;
+        return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
+    }
+    __device__ __noinline__ void derivs(const SoAGet& y, const SoAPut& f) const {
+        using namespace chain;
+        const SoAGet u{y.p + (size_t)nq * y.L, y.L, y.lane};
+        const SoAPut fu{f.p + (size_t)nq * f.L, f.L, f.lane};
+        dm R = ident<double>(); d3 p = zero3<double>(); dv V = svzero<double>();
+        for (int b = 1; b <= nb; ++b) {
+            const int4 ba = body_a[b-1];
+            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
+            switch (ba.y) {
+            case PIN:       pass1_body<PIN>(P, ba.z, ba.w, y, u, f, R, p, V); break;
+            case SLIDER:    pass1_body<SLIDER>(P, ba.z, ba.w, y, u, f, R, p, V); break;
+            case UNIVERSAL: pass1_body<UNIVERSAL>(P, ba.z, ba.w, y, u, f, R, p, V); break;
+            case BALL:      pass1_body<BALL>(P, ba.z, ba.w, y, u, f, R, p, V); break;
+            default:        pass1_body<FREE>(P, ba.z, ba.w, y, u, f, R, p, V); break;
+            }
+        }
+        da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
+        for (int b = nb; b >= 1; --b) {
+            const int4 ba = body_a[b-1];
+            const int gs = body_b[b-1].w;
+            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
+            const bool hc = b < nb;
+            const dv Fb = F(b);
+            switch (ba.y) {
+            case PIN:       pass2_body<PIN>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
+            case SLIDER:    pass2_body<SLIDER>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
+            case UNIVERSAL: pass2_body<UNIVERSAL>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
+            case BALL:      pass2_body<BALL>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
+            default:        pass2_body<FREE>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
+            }
+        }
+        R = ident<double>(); p = zero3<double>(); V = svzero<double>();
+        dv A = svzero<double>();
+        for (int b = 1; b <= nb; ++b) {
+            const int4 ba = body_a[b-1];
+            const int gs = body_b[b-1].w;
+            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
+            switch (ba.y) {
+            case PIN:       pass3_body<PIN>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
+            case SLIDER:    pass3_body<SLIDER>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
+            case UNIVERSAL: pass3_body<UNIVERSAL>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
+            case BALL:      pass3_body<BALL>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
+            default:        pass3_body<FREE>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
+            }
+        }
+    }
+};
+
+template <class EV>
 __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     if (a.status[lane] != 0) return;
     const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
     const int tr = a.tree_of ? a.tree_of[lane] : 0;
-    const double* grav = a.T.gravity + 4*tr;
-    const double* fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
+    const EV ev(a, lane, tr);
     const int nq = c.nq, nu = c.nu, ny = nq + nu;
     const int nquat = a.T.tree_b[tr].y;
     const size_t NV = (size_t)a.ny_max * a.L;
@@ -659,7 +732,7 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         if (t >= tEnd) { status = 1; break; }
         for (int k = 0; k < ny; ++k) V(VY0, k) = V(VY, k);
         for (int k = 0; k < nu; ++k) { const double vi = fabs(V(VY0, nq + k)); US(k) = vi > 1.0 ? 1.0/vi : 1.0; }
-        rkm_derivs(c, grav, fapp, gY0, pF0);
+        ev.derivs(gY0, pF0);
         bool ok = false;
         double t1 = t;
         do {
@@ -671,13 +744,13 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             if (++attempts > cfg.max_attempts) { status = -2; break; }
             const double h = t1 - t;
             for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/3)*V(VF0, k);
-            rkm_derivs(c, grav, fapp, gYT, pFA);
+            ev.derivs(gYT, pFA);
             for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/6)*(V(VF0, k) + V(VFA, k));
-            rkm_derivs(c, grav, fapp, gYT, pFA);
+            ev.derivs(gYT, pFA);
             for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/8)*(V(VF0, k) + 3*V(VFA, k));
-            rkm_derivs(c, grav, fapp, gYT, pFB);
+            ev.derivs(gYT, pFB);
             for (int k = 0; k < ny; ++k) V(VYS, k) = V(VY0, k) + (h/2)*((V(VF0, k) - 3*V(VFA, k)) + 4*V(VFB, k));
-            rkm_derivs(c, grav, fapp, gYS, pFA);
+            ev.derivs(gYS, pFA);
             for (int k = 0; k < ny; ++k) {
                 const double y1 = V(VY0, k) + (h/6)*((V(VF0, k) + 4*V(VFB, k)) + V(VFA, k));
                 V(VY1, k) = y1;
@@ -791,7 +864,8 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 }
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
-    k_rkm<<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
+    if (a.chain) k_rkm<ChainEval><<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
+    else k_rkm<TreeEval><<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
     return hipGetLastError();
 }
 

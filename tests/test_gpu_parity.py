@@ -184,3 +184,23 @@ def test_adaptive_tree_rollout_with_forces():
         y, s = OracleModel(t).integrate(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, Fapp=fs[i])
         assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
         assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, i
+
+
+def test_chain_and_tree_evaluators_agree():
+    """The register-resident chain evaluator and the tree sweeps give the same
+    rollout (same accept counts, states within rounding growth)."""
+    from simbody_amd.matter import RungeKuttaMerson
+    spec = M.spatial_chain(32, seed=5)
+    n = 128
+    Y = M.chain_batch_states(spec, n, seed=23)
+    outs = []
+    for mode in (1, 2):
+        B = _batch([spec], n=n)
+        B.set_evaluator(mode)
+        q = B.to_device(Y[:, :spec.nq]); u = B.to_device(Y[:, spec.nq:])
+        q1, u1, st = B.stepTo(q, u, RungeKuttaMerson(accuracy=1e-6), 0.05)
+        outs.append((np.concatenate([q1.cpu().numpy().reshape(n, -1), u1.cpu().numpy().reshape(n, -1)], 1),
+                     st.cpu().numpy()))
+    assert (outs[0][1] == outs[1][1]).all()
+    for i in range(n):
+        assert rel_inf(outs[1][0][i], outs[0][0][i]) < 1e-9, i
