@@ -167,7 +167,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
         grav.push_back(0.0);
         B->h_tree_a.push_back(make_int4(nb, q, u, body_off));
-        B->h_tree_b.push_back(make_int4(rec, nquat, 0, 0));
+        B->h_tree_b.push_back(make_int4(rec, nquat, gslot, 0));
         tq[t] = q; tu[t] = u; tnb[t] = nb;
         if (t == 0) B->is_chain = chain;
         B->max_rec = std::max(B->max_rec, rec);
@@ -199,7 +199,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->q_off.upload(B->h_qoff)) != hipSuccess ||
         (e = B->u_off.upload(B->h_uoff)) != hipSuccess ||
         (e = B->b_off.upload(B->h_boff)) != hipSuccess ||
-        (e = B->scratch.alloc((size_t)B->max_rec * n_instances)) != hipSuccess) {
+        (e = B->scratch.alloc((size_t)(B->max_rec + 24) * n_instances)) != hipSuccess) {
         delete B;
         return hipfail(e, "tdyn_batch_create: device allocation/upload");
     }
@@ -323,7 +323,8 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     const size_t L = b->n_inst;
     if (!b->rollout_ready || b->ny_max != ny) {
         hipError_t e;
-        if ((e = b->vec.alloc((size_t)9 * ny * L)) != hipSuccess ||
+        // 8 extra rows: the chain evaluator prefetches fixed-size q/u windows
+        if ((e = b->vec.alloc((size_t)9 * ny * L + 8 * L)) != hipSuccess ||
             (e = b->uscale.alloc((size_t)std::max(1, b->max_nu) * L)) != hipSuccess ||
             (e = b->t.alloc(L)) != hipSuccess || (e = b->h.alloc(L)) != hipSuccess ||
             (e = b->t_end.alloc(L)) != hipSuccess || (e = b->status.alloc(L)) != hipSuccess ||

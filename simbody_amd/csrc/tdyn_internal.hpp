@@ -5,6 +5,20 @@
 
 namespace tdyn {
 
+// Model tables are read-only for the lifetime of a kernel: reading them
+// through the constant address space lets wave-uniform loads become scalar
+// (s_load) loads into SGPRs.
+#define TD_CONST __attribute__((address_space(4)))
+template <class T> __device__ __forceinline__ const TD_CONST T* cptr(const T* p) {
+    return (const TD_CONST T*)p;
+}
+using cdptr = const TD_CONST double*;
+// int4 table entry read through the constant address space.
+__device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
+    const TD_CONST int* p = (const TD_CONST int*)(base + idx);
+    return make_int4(p[0], p[1], p[2], p[3]);
+}
+
 enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4 };
 __host__ __device__ inline int mob_dof(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 3 : 6; }
 __host__ __device__ inline int mob_nq(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 4 : 7; }

@@ -26,12 +26,19 @@ struct Lane {
     int lane, L, nb, nq, nu, body_off;
     double* scr;
     __device__ double& s(int off) const { return scr[(size_t)off * L + lane]; }
-    __device__ int4 ba(int b) const { return T.body_a[body_off + b - 1]; }
-    __device__ int4 bb(int b) const { return T.body_b[body_off + b - 1]; }
-    __device__ const double* prm(int b) const { return T.prm + (size_t)(body_off + b - 1) * PRM_STRIDE; }
-    __device__ int rec(int b) const { return T.body_b[body_off + b - 1].z; }
+    __device__ int4 ba(int b) const { return ldi4(T.body_a, body_off + b - 1); }
+    __device__ int4 bb(int b) const { return ldi4(T.body_b, body_off + b - 1); }
+    __device__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
+    __device__ int rec(int b) const { return ldi4(T.body_b, body_off + b - 1).z; }
 };
 
+__device__ __forceinline__ Lane make_lane_t(const DevTables& T, int tr, int lane, int L, double* scr) {
+    Lane c;
+    c.T = T; c.lane = lane; c.L = L; c.scr = scr;
+    const int4 ta = ldi4(T.tree_a, tr);
+    c.nb = ta.x; c.nq = ta.y; c.nu = ta.z; c.body_off = ta.w;
+    return c;
+}
 __device__ __forceinline__ Lane make_lane(const DevTables& T, const int* tree_of, int lane, int L, double* scr) {
     Lane c;
     c.T = T; c.lane = lane; c.L = L; c.scr = scr;
@@ -60,8 +67,8 @@ __device__ __forceinline__ void sta_global(double* o, const da& P) {
     for (int i=0;i<9;++i) o[12+i] = P.F.a[i];
 }
 
-__device__ __forceinline__ dm prm_m(const double* p) { dm r; for (int i=0;i<9;++i) r.a[i] = p[i]; return r; }
-__device__ __forceinline__ d3 prm_3(const double* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ dm prm_m(cdptr p) { dm r; for (int i=0;i<9;++i) r.a[i] = p[i]; return r; }
+__device__ __forceinline__ d3 prm_3(cdptr p) { return {p[0], p[1], p[2]}; }
 
 template <int TYPE> struct Mob {
     static constexpr int D = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 3 : 6;
@@ -82,7 +89,7 @@ template <int TYPE, bool VEL, class QG, class UG, class QD>
 __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, const QG& q, const UG& u, const QD& qd) {
     constexpr int D = Mob<TYPE>::D, NQ = Mob<TYPE>::NQ;
     const int p = ba.x, qi = ba.z, ui = ba.w;
-    const double* P = c.prm(b);
+    const cdptr P = c.prm(b);
     const int rb = c.rec(b);
     dm R_GP; d3 p_GP;
     int rp = 0;
@@ -251,13 +258,13 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
                                             double* outP, double* outPP, const double* grav) {
     constexpr int D = Mob<TYPE>::D;
     const int rb = bb.z;
-    const double* P0 = c.prm(b);
+    const cdptr P0 = c.prm(b);
     const double m = P0[PRM_MASS];
     const d3 Mk_p = ld3(c, rb + REC_MKP);
     const ds Mk_G = lds(c, rb + REC_MKG);
     da P = abi_from_rbi(m, Mk_p, Mk_G);
     for (int k = 0; k < bb.y; ++k) {
-        const int ch = c.T.child_idx[bb.x + k];
+        const int ch = cptr(c.T.child_idx)[bb.x + k];
         const int rc = c.rec(ch);
         abi_add_shifted(P, lda(c, rc + REC_PPLUS), ld3(c, rc + REC_PHI));
     }
@@ -324,7 +331,7 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
             z = svzero<double>();
         }
         for (int k = 0; k < bb.y; ++k) {
-            const int rc = c.rec(c.T.child_idx[bb.x + k]);
+            const int rc = c.rec(cptr(c.T.child_idx)[bb.x + k]);
             z = z + phi_mul(ld3(c, rc + REC_PHI), ldsv(c, rc + REC_ZPLUS));
         }
         double eps[D];
@@ -438,7 +445,7 @@ __device__ __forceinline__ void resid_in_body(const Lane& c, int b, const int4 b
     dv F = rbi_mul(m, ld3(c, rb + REC_MKP), lds(c, rb + REC_MKG), ldsv(c, rb + REC_AGB)) + ldsv(c, rb + REC_B);
     F = F - fbody(b);
     for (int k = 0; k < bb.y; ++k) {
-        const int rc = c.rec(c.T.child_idx[bb.x + k]);
+        const int rc = c.rec(cptr(c.T.child_idx)[bb.x + k]);
         F = F + phi_mul(ld3(c, rc + REC_PHI), ldsv(c, rc + REC_ZPLUS));
     }
     stsv(c, rb + REC_ZPLUS, F);
@@ -484,12 +491,12 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 // ---------------------------------------------------------------------------
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
-template <int OP>
+template <int OP, bool UNI>
 __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
-    const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
-    const int tr = a.tree_of ? a.tree_of[lane] : 0;
+    const int tr = UNI ? 0 : a.tree_of[lane];
+    const Lane c = make_lane_t(a.T, tr, lane, a.L, a.scratch);
     const double* grav = a.T.gravity + 4*tr;
     const int qo = a.off.q_off[lane], uo = a.off.u_off[lane], bo = a.off.b_off[lane];
     const PackedGet q{a.q + qo};
@@ -595,18 +602,22 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     }
 }
 
-hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
-    if (a.n_inst <= 0) return hipSuccess;
+template <bool UNI>
+static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     const dim3 blk(64), grd((a.n_inst + 63) / 64);
     switch (k) {
-    case OP_KIN:       k_op<OP_KIN><<<grd, blk, 0, s>>>(a); break;
-    case OP_FWD_SYS:   k_op<OP_FWD_SYS><<<grd, blk, 0, s>>>(a); break;
-    case OP_FWD_GIVEN: k_op<OP_FWD_GIVEN><<<grd, blk, 0, s>>>(a); break;
-    case OP_MINV:      k_op<OP_MINV><<<grd, blk, 0, s>>>(a); break;
-    case OP_RESID:     k_op<OP_RESID><<<grd, blk, 0, s>>>(a); break;
-    case OP_ABI:       k_op<OP_ABI><<<grd, blk, 0, s>>>(a); break;
-    case OP_REACT:     k_op<OP_REACT><<<grd, blk, 0, s>>>(a); break;
+    case OP_KIN:       k_op<OP_KIN, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_SYS:   k_op<OP_FWD_SYS, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_GIVEN: k_op<OP_FWD_GIVEN, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_MINV:      k_op<OP_MINV, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_RESID:     k_op<OP_RESID, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_ABI:       k_op<OP_ABI, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_REACT:     k_op<OP_REACT, UNI><<<grd, blk, 0, s>>>(a); break;
     }
+}
+hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    if (a.tree_of) launch_op_t<false>(k, a, s); else launch_op_t<true>(k, a, s);
     return hipGetLastError();
 }
 
@@ -621,101 +632,44 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, NVEC = 9 };
 
-// Derivative evaluators: y (SoA q then u) -> f (SoA qdot then udot).
+// Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 struct TreeEval {
     Lane c; const double* grav; const double* fapp;
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
-        : c(make_lane(a.T, a.tree_of, lane, a.L, a.scratch)), grav(a.T.gravity + 4*tr),
+        : c(make_lane_t(a.T, tr, lane, a.L, a.scratch)), grav(a.T.gravity + 4*tr),
           fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr) {}
-    __device__ __noinline__ void derivs(const SoAGet& y, const SoAPut& f) const {
-        const SoAGet u{y.p + (size_t)c.nq * y.L, y.L, y.lane};
-        const SoAPut fu{f.p + (size_t)c.nq * f.L, f.L, f.lane};
+    __device__ __forceinline__ void derivs(const chain::Rows& yr, const chain::Rows& fr) const {
+        const SoAGet y{yr.p, yr.L, 0}, u{yr.p + (size_t)c.nq * yr.L, yr.L, 0};
+        const SoAPut f{fr.p, fr.L, 0}, fu{fr.p + (size_t)c.nq * fr.L, fr.L, 0};
         sweep_kin<true>(c, y, u, f);
         sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
         sweep_outward2<true>(c, y, u, fu, NoPut{});
     }
 };
+using chain::ChainEval;
 
-struct ChainEval {
-    const double* prm; const int4* body_a; const int4* body_b;
-    int nb, nq;
-    chain::GRec gr; d3 grav; const double* fapp;
-    __device__ ChainEval(const RkmArgs& a, int lane, int tr) {
-        const int4 ta = a.T.tree_a[tr];
-        nb = ta.x; nq = ta.y;
-        prm = a.T.prm + (size_t)ta.w * PRM_STRIDE; body_a = a.T.body_a + ta.w; body_b = a.T.body_b + ta.w;
-        gr = {a.scratch, a.L, lane};
-        grav = V3(a.T.gravity[4*tr], a.T.gravity[4*tr+1], a.T.gravity[4*tr+2]);
-        fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
-    }
-    __device__ dv F(int b) const {
-        if (!fapp) return svzero<double>();
-        const double* f = fapp + 6*b;
-        return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
-    }
-    __device__ __noinline__ void derivs(const SoAGet& y, const SoAPut& f) const {
-        using namespace chain;
-        const SoAGet u{y.p + (size_t)nq * y.L, y.L, y.lane};
-        const SoAPut fu{f.p + (size_t)nq * f.L, f.L, f.lane};
-        dm R = ident<double>(); d3 p = zero3<double>(); dv V = svzero<double>();
-        for (int b = 1; b <= nb; ++b) {
-            const int4 ba = body_a[b-1];
-            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
-            switch (ba.y) {
-            case PIN:       pass1_body<PIN>(P, ba.z, ba.w, y, u, f, R, p, V); break;
-            case SLIDER:    pass1_body<SLIDER>(P, ba.z, ba.w, y, u, f, R, p, V); break;
-            case UNIVERSAL: pass1_body<UNIVERSAL>(P, ba.z, ba.w, y, u, f, R, p, V); break;
-            case BALL:      pass1_body<BALL>(P, ba.z, ba.w, y, u, f, R, p, V); break;
-            default:        pass1_body<FREE>(P, ba.z, ba.w, y, u, f, R, p, V); break;
-            }
-        }
-        da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
-        for (int b = nb; b >= 1; --b) {
-            const int4 ba = body_a[b-1];
-            const int gs = body_b[b-1].w;
-            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
-            const bool hc = b < nb;
-            const dv Fb = F(b);
-            switch (ba.y) {
-            case PIN:       pass2_body<PIN>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
-            case SLIDER:    pass2_body<SLIDER>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
-            case UNIVERSAL: pass2_body<UNIVERSAL>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
-            case BALL:      pass2_body<BALL>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
-            default:        pass2_body<FREE>(P, ba.z, ba.w, y, u, grav, Fb, gr, gs, R, p, V, hc, PP, zp, phic); break;
-            }
-        }
-        R = ident<double>(); p = zero3<double>(); V = svzero<double>();
-        dv A = svzero<double>();
-        for (int b = 1; b <= nb; ++b) {
-            const int4 ba = body_a[b-1];
-            const int gs = body_b[b-1].w;
-            const double* P = prm + (size_t)(b-1) * PRM_STRIDE;
-            switch (ba.y) {
-            case PIN:       pass3_body<PIN>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
-            case SLIDER:    pass3_body<SLIDER>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
-            case UNIVERSAL: pass3_body<UNIVERSAL>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
-            case BALL:      pass3_body<BALL>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
-            default:        pass3_body<FREE>(P, ba.z, ba.w, y, u, fu, gr, gs, R, p, V, A); break;
-            }
-        }
-    }
-};
-
-template <class EV>
+// One lane runs one instance's TimeStepper loop.  The state y0 lives in
+// vector VY; a step evaluates f0 = f(y0) once and then, per attempt, the four
+// Merson stages (RungeKuttaMersonIntegrator.cpp:86-140) through ONE inlined
+// evaluator inside a stage loop.
+// UNI: every instance uses tree 0, so the tree tables are read with
+// wave-uniform (scalar) loads.
+template <class EV, bool UNI>
 __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     if (a.status[lane] != 0) return;
-    const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
-    const int tr = a.tree_of ? a.tree_of[lane] : 0;
+    const int tr = UNI ? 0 : a.tree_of[lane];
     const EV ev(a, lane, tr);
-    const int nq = c.nq, nu = c.nu, ny = nq + nu;
-    const int nquat = a.T.tree_b[tr].y;
+    const int4 ta = ldi4(a.T.tree_a, tr);
+    const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
+    const int4* body_a = a.T.body_a + ta.w;
+    const int nquat = ldi4(a.T.tree_b, tr).y;
     const size_t NV = (size_t)a.ny_max * a.L;
-    auto V = [&](int v, int k) -> double& { return a.vec[v*NV + (size_t)k * a.L + lane]; };
-    auto US = [&](int k) -> double& { return a.uscale[(size_t)k * a.L + lane]; };
-    const SoAGet gY0{a.vec + VY0*NV, a.L, lane}, gYT{a.vec + VYT*NV, a.L, lane}, gYS{a.vec + VYS*NV, a.L, lane};
-    const SoAPut pF0{a.vec + VF0*NV, a.L, lane}, pFA{a.vec + VFA*NV, a.L, lane}, pFB{a.vec + VFB*NV, a.L, lane};
+    const int L = a.L;
+    auto rows = [&](int v) { return chain::Rows{a.vec + v*NV + lane, L}; };
+    const chain::Rows Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
+                      Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
 
     const RkmCfg& cfg = a.cfg;
     const double acc = cfg.accuracy, consTol = cfg.cons_tol;
@@ -728,13 +682,42 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         projections = a.stats[4*lane+3];
     int status = 0, steps = 0;
 
+    // error norm at y1: q part weighted by N NInv for quaternions (IntegratorRep.h:
+    // 454-523), u part relative to y0 (calcRelativeScaling, :644-655)
+    auto errNorm = [&]() -> double {
+        double sq = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int4 ba = ldi4(body_a, b);
+            const int qi = ba.z;
+            if (ba.y == BALL || ba.y == FREE) {
+                double qq[4], e[4], d4[4];
+                for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
+                const d3 w = quat_dot_to_w(qq, e);
+                quat_dot(qq, w, d4);
+                sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = YE(qi + k); sq += x*x; }
+            } else {
+                for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = YE(qi + k); sq += x*x; }
+            }
+        }
+        const double qn = nq ? sqrt(sq / nq) : 0.0;
+        double su = 0;
+#pragma unroll 8
+        for (int k = 0; k < nu; ++k) {
+            const double vi = fabs(Y(nq + k));
+            const double us = vi > 1.0 ? 1.0/vi : 1.0;
+            const double x = us * YE(nq + k);
+            su += x*x;
+        }
+        const double un = nu ? sqrt(su / nu) : 0.0;
+        return qn >= un ? qn : un;
+    };
+
     while (steps < cfg.max_steps) {
         if (t >= tEnd) { status = 1; break; }
-        for (int k = 0; k < ny; ++k) V(VY0, k) = V(VY, k);
-        for (int k = 0; k < nu; ++k) { const double vi = fabs(V(VY0, nq + k)); US(k) = vi > 1.0 ? 1.0/vi : 1.0; }
-        ev.derivs(gY0, pF0);
         bool ok = false;
         double t1 = t;
+        int s0 = 0;          // f0 = f(y0) once per step
         do {
             bool limited = false;
             if (tEnd < t + 0.95*hcur) { limited = true; t1 = tEnd; }
@@ -743,51 +726,43 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             if (!(t1 > t)) { status = -3; break; }
             if (++attempts > cfg.max_attempts) { status = -2; break; }
             const double h = t1 - t;
-            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/3)*V(VF0, k);
-            ev.derivs(gYT, pFA);
-            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/6)*(V(VF0, k) + V(VFA, k));
-            ev.derivs(gYT, pFA);
-            for (int k = 0; k < ny; ++k) V(VYT, k) = V(VY0, k) + (h/8)*(V(VF0, k) + 3*V(VFA, k));
-            ev.derivs(gYT, pFB);
-            for (int k = 0; k < ny; ++k) V(VYS, k) = V(VY0, k) + (h/2)*((V(VF0, k) - 3*V(VFA, k)) + 4*V(VFB, k));
-            ev.derivs(gYS, pFA);
-            for (int k = 0; k < ny; ++k) {
-                const double y1 = V(VY0, k) + (h/6)*((V(VF0, k) + 4*V(VFB, k)) + V(VFA, k));
-                V(VY1, k) = y1;
-                V(VYE, k) = 0.2 * fabs(y1 - V(VYS, k));
-            }
-            // error norm at y1 (q part weighted by N NInv for quaternions)
-            auto errNorm = [&]() -> double {
-                double sq = 0;
-                for (int b = 1; b <= c.nb; ++b) {
-                    const int4 ba = c.ba(b);
-                    const int qi = ba.z;
-                    if (ba.y == BALL || ba.y == FREE) {
-                        double qq[4], e[4], d4[4];
-                        for (int k = 0; k < 4; ++k) { qq[k] = V(VY1, qi + k); e[k] = V(VYE, qi + k); }
-                        const d3 w = quat_dot_to_w(qq, e);
-                        quat_dot(qq, w, d4);
-                        sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
-                        if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = V(VYE, qi + k); sq += x*x; }
-                    } else {
-                        for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = V(VYE, qi + k); sq += x*x; }
-                    }
+            for (int st = s0; st < 5; ++st) {
+                chain::Rows in = Y, out = F0;
+                if (st == 1) {
+#pragma unroll 8
+                    for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/3)*F0(k);
+                    in = YT; out = FA;
+                } else if (st == 2) {
+#pragma unroll 8
+                    for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/6)*(F0(k) + FA(k));
+                    in = YT; out = FA;
+                } else if (st == 3) {
+#pragma unroll 8
+                    for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/8)*(F0(k) + 3*FA(k));
+                    in = YT; out = FB;
+                } else if (st == 4) {
+#pragma unroll 8
+                    for (int k = 0; k < ny; ++k) YS(k) = Y(k) + (h/2)*((F0(k) - 3*FA(k)) + 4*FB(k));
+                    in = YS; out = FA;
                 }
-                const double qn = nq ? sqrt(sq / nq) : 0.0;
-                double su = 0;
-                for (int k = 0; k < nu; ++k) { const double x = US(k) * V(VYE, nq + k); su += x*x; }
-                const double un = nu ? sqrt(su / nu) : 0.0;
-                return qn >= un ? qn : un;
-            };
+                ev.derivs(in, out);
+            }
+            s0 = 1;
+#pragma unroll 8
+            for (int k = 0; k < ny; ++k) {
+                const double y1 = Y(k) + (h/6)*((F0(k) + 4*FB(k)) + FA(k));
+                Y1(k) = y1;
+                YE(k) = 0.2 * fabs(y1 - YS(k));
+            }
             bool converged = true;
             const double e1 = errNorm();
             if (!(e1 > 16.0*acc) && nquat > 0) {
                 double s2 = 0;
-                for (int b = 1; b <= c.nb; ++b) {
-                    const int4 ba = c.ba(b);
+                for (int b = 0; b < nb; ++b) {
+                    const int4 ba = ldi4(body_a, b);
                     if (ba.y != BALL && ba.y != FREE) continue;
                     double n2 = 0;
-                    for (int k = 0; k < 4; ++k) { const double x = V(VY1, ba.z + k); n2 += x*x; }
+                    for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
                     const double e = sqrt(n2) - 1.0;
                     s2 += e*e;
                 }
@@ -795,20 +770,21 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                 if (qe > projLimit) converged = false;
                 else if (qe > consTol) {
                     ++projections;
-                    for (int b = 1; b <= c.nb; ++b) {
-                        const int4 ba = c.ba(b);
+                    for (int b = 0; b < nb; ++b) {
+                        const int4 ba = ldi4(body_a, b);
                         if (ba.y != BALL && ba.y != FREE) continue;
                         double qq[4], n2 = 0;
-                        for (int k = 0; k < 4; ++k) { qq[k] = V(VY1, ba.z + k); n2 += qq[k]*qq[k]; }
+                        for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
                         const double n = sqrt(n2);
-                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; V(VY1, ba.z + k) = qq[k]; }
+                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; Y1(ba.z + k) = qq[k]; }
                         double d = 0;
-                        for (int k = 0; k < 4; ++k) d += V(VYE, ba.z + k) * qq[k];
-                        for (int k = 0; k < 4; ++k) V(VYE, ba.z + k) = V(VYE, ba.z + k) - d*qq[k];
+                        for (int k = 0; k < 4; ++k) d += YE(ba.z + k) * qq[k];
+                        for (int k = 0; k < 4; ++k) YE(ba.z + k) = YE(ba.z + k) - d*qq[k];
                     }
                 }
             }
             const double err = converged ? errNorm() : __builtin_inf();
+            // adjustStepSize, AbstractIntegratorRep.cpp:448-502
             double hnew;
             if (!isfinite(err)) hnew = 0.1*hcur;
             else if (err == 0) hnew = 5.0*hcur;
@@ -825,8 +801,129 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         } while (!ok);
         if (status < 0) break;
         t = t1;
-        for (int k = 0; k < ny; ++k) V(VY, k) = V(VY1, k);
+#pragma unroll 8
+        for (int k = 0; k < ny; ++k) Y(k) = Y1(k);
         ++accepted; ++steps;
+    }
+    if (status == 0 && t >= tEnd) status = 1;
+    a.t[lane] = t; a.h[lane] = hcur; a.status[lane] = status;
+    a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
+}
+
+// adjustStepSize (AbstractIntegratorRep.cpp:448-502).  Returns success.
+__device__ __forceinline__ bool adjust_step(double err, double acc, bool limited, double hmin, double hmax,
+                                            double& hcur) {
+    double hnew;
+    if (!isfinite(err)) hnew = 0.1*hcur;
+    else if (err == 0) hnew = 5.0*hcur;
+    else hnew = 0.9*hcur*pow(acc/err, 0.25);
+    if (hnew > hcur) { if (limited || hnew < 1.2*hcur) hnew = hcur; }
+    if (hnew < hcur) { if (err <= acc) hnew = hcur; else hnew = fmin(hnew, 0.9*hcur); }
+    hnew = fmin(hnew, 5.0*hcur);
+    hnew = fmax(hnew, 0.1*hcur);
+    if (hmin != -1) hnew = fmax(hnew, hmin);
+    if (hmax != -1) hnew = fmin(hnew, hmax);
+    const bool ok = hnew >= hcur;
+    hcur = hnew;
+    return ok;
+}
+
+// Rollouts of a uniform serial-chain batch with the Merson stages fused into
+// the chain evaluator's sweeps (no separate vector passes in the common
+// path).  The state ping-pongs between VY and VY1; a lane whose current state
+// sits in VY1 when its launch ends copies it back to VY.
+__global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
+    using namespace chain;
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    if (a.status[lane] != 0) return;
+    const ChainEval ev(a, lane, 0);
+    const int4 ta = ldi4(a.T.tree_a, 0);
+    const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
+    const int4* body_a = a.T.body_a + ta.w;
+    const int nquat = ldi4(a.T.tree_b, 0).y;
+    const size_t NV = (size_t)a.ny_max * a.L;
+    const int L = a.L;
+    auto rows = [&](int v) { return Rows{a.vec + v*NV + lane, L}; };
+    const Rows S0 = rows(VY), S1 = rows(VY1), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
+               YT = rows(VYT), YE = rows(VYE);
+
+    const RkmCfg& cfg = a.cfg;
+    const double acc = cfg.accuracy, consTol = cfg.cons_tol;
+    double hmin = cfg.min_h, hmax = cfg.max_h;
+    if (cfg.fixed_h > 0) { hmin = hmax = cfg.fixed_h; }
+    const double projLimit = fmax(2*consTol, sqrt(consTol));
+    double t = a.t[lane], hcur = a.h[lane];
+    const double tEnd = a.t_end[lane];
+    int attempts = a.stats[4*lane], accepted = a.stats[4*lane+1], rejected = a.stats[4*lane+2],
+        projections = a.stats[4*lane+3];
+    int status = 0, steps = 0;
+    bool flip = false;     // current state in S1
+    NormSums ns0;
+
+    while (steps < cfg.max_steps) {
+        if (t >= tEnd) { status = 1; break; }
+        const Rows Y = flip ? S1 : S0, Y1 = flip ? S0 : S1;
+        RkRows r{Y, F0, FA, FB, YT, F0, Y1, YE};
+        ev.derivs_rk<0>(r, 0.0, ns0);
+        bool ok = false;
+        double t1 = t;
+        do {
+            bool limited = false;
+            if (tEnd < t + 0.95*hcur) { limited = true; t1 = tEnd; }
+            else if (tEnd > t + 1.001*hcur) t1 = t + hcur;
+            else t1 = tEnd;
+            if (!(t1 > t)) { status = -3; break; }
+            if (++attempts > cfg.max_attempts) { status = -2; break; }
+            const double h = t1 - t;
+            r.st = YT; r.out = FA; ev.derivs_rk<1>(r, h, ns0);
+            ev.derivs_rk<2>(r, h, ns0);
+            r.out = FB; ev.derivs_rk<3>(r, h, ns0);
+            NormSums ns{0.0, 0.0, 0.0, 0.0};
+            r.st = YS; r.out = FA; ev.derivs_rk<4>(r, h, ns);
+            const double qn = nq ? sqrt((ns.sq_other + ns.sq_quat) / nq) : 0.0;
+            const double un = nu ? sqrt(ns.su / nu) : 0.0;
+            const double e1 = qn >= un ? qn : un;
+            double err = e1;
+            bool converged = true;
+            if (!(e1 > 16.0*acc) && nquat > 0) {
+                const double qe = sqrt(ns.s2 / nquat);
+                if (qe > projLimit) converged = false;
+                else if (qe > consTol) {
+                    // normalizeQuaternions (SimbodyMatterSubsystemRep.cpp:4448-4475) and
+                    // the re-evaluated error norm of the projected estimate
+                    ++projections;
+                    double sqq = 0;
+                    for (int b = 0; b < nb; ++b) {
+                        const int4 ba = ldi4(body_a, b);
+                        if (ba.y != BALL && ba.y != FREE) continue;
+                        double qq[4], n2 = 0, e[4], d4[4];
+                        for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
+                        const double n = sqrt(n2);
+                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; Y1(ba.z + k) = qq[k]; }
+                        double d = 0;
+                        for (int k = 0; k < 4; ++k) d += YE(ba.z + k) * qq[k];
+                        for (int k = 0; k < 4; ++k) { e[k] = YE(ba.z + k) - d*qq[k]; YE(ba.z + k) = e[k]; }
+                        const d3 w = quat_dot_to_w(qq, e);
+                        quat_dot(qq, w, d4);
+                        sqq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                    }
+                    const double qn2 = nq ? sqrt((ns.sq_other + sqq) / nq) : 0.0;
+                    err = qn2 >= un ? qn2 : un;
+                }
+            }
+            if (!converged) err = __builtin_inf();
+            ok = adjust_step(err, acc, limited, hmin, hmax, hcur);
+            if (!ok) ++rejected;
+        } while (!ok);
+        if (status < 0) break;
+        t = t1;
+        flip = !flip;
+        ++accepted; ++steps;
+    }
+    if (flip) {
+#pragma unroll 8
+        for (int k = 0; k < ny; ++k) S0(k) = S1(k);
     }
     if (status == 0 && t >= tEnd) status = 1;
     a.t[lane] = t; a.h[lane] = hcur; a.status[lane] = status;
@@ -864,8 +961,10 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 }
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
-    if (a.chain) k_rkm<ChainEval><<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
-    else k_rkm<TreeEval><<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a);
+    const dim3 grd((a.n_inst + 63)/64), blk(64);
+    if (a.chain) k_rkm_chain<<<grd, blk, 0, s>>>(a);
+    else if (a.tree_of) k_rkm<TreeEval, false><<<grd, blk, 0, s>>>(a);
+    else k_rkm<TreeEval, true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -186,11 +186,15 @@ def test_adaptive_tree_rollout_with_forces():
         assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, i
 
 
-def test_chain_and_tree_evaluators_agree():
+@pytest.mark.parametrize("free_root", [False, True])
+def test_chain_and_tree_evaluators_agree(free_root):
     """The register-resident chain evaluator and the tree sweeps give the same
-    rollout (same accept counts, states within rounding growth)."""
+    rollout (same accept counts, states within rounding growth); with a Free
+    root (a floating-base chain) as well."""
     from simbody_amd.matter import RungeKuttaMerson
     spec = M.spatial_chain(32, seed=5)
+    if free_root:
+        spec.mobilizer[0] = M.FREE
     n = 128
     Y = M.chain_batch_states(spec, n, seed=23)
     outs = []
@@ -204,3 +208,20 @@ def test_chain_and_tree_evaluators_agree():
     assert (outs[0][1] == outs[1][1]).all()
     for i in range(n):
         assert rel_inf(outs[1][0][i], outs[0][0][i]) < 1e-9, i
+
+
+def test_chain_evaluator_free_root_matches_oracle():
+    from simbody_amd.matter import RungeKuttaMerson
+    spec = M.spatial_chain(12, seed=8)
+    spec.mobilizer[0] = M.FREE
+    n = 16
+    B = _batch([spec], n=n)
+    Y = M.chain_batch_states(spec, n, seed=24)
+    q = B.to_device(Y[:, :spec.nq]); u = B.to_device(Y[:, spec.nq:])
+    q1, u1, st = B.stepTo(q, u, RungeKuttaMerson(accuracy=1e-6), 0.05)
+    q1 = q1.cpu().numpy().reshape(n, -1); u1 = u1.cpu().numpy().reshape(n, -1); st = st.cpu().numpy()
+    o = OracleModel(spec)
+    for i in range(n):
+        y, s = o.integrate(Y[i], 0.05, 1e-6)
+        assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
+        assert rel_inf(np.concatenate([q1[i], u1[i]]), y) < 1e-8, i
