@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 # Algorithmic FP64 flops per body-step of this workload (+,-,*,/ of the
 # restated Simbody algorithm, counted by the oracle's counting scalar over
 # fixed steps of the 32-body chain; tests/test_flop_count.py re-derives it).
-FLOPS_PER_BODY_STEP = 9034.9375
+FLOPS_PER_BODY_STEP = 9034.59375
 FLOPS_RESID_PER_BODY = 1055.40625
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector = matrix) peak
 PEAK_HBM_GBS = 8000.0

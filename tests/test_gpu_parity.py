@@ -225,3 +225,20 @@ def test_chain_evaluator_free_root_matches_oracle():
         y, s = o.integrate(Y[i], 0.05, 1e-6)
         assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
         assert rel_inf(np.concatenate([q1[i], u1[i]]), y) < 1e-8, i
+
+
+def test_pendulum10_fixture_on_gpu():
+    """Config 1 fixture (tests/golden/pendulum10.json): udot at 1e-12, the
+    adaptive RK-Merson run to t=1 with identical step counts and q(1) at 1e-9."""
+    import json, os
+    from simbody_amd.matter import RungeKuttaMerson
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pendulum10.json")))
+    spec = M.pendulum_chain(10)
+    B = _batch([spec], n=1)
+    q = B.to_device(np.array(g["q"])); u = B.to_device(np.array(g["u"]))
+    ud, _ = B.realizeAcceleration(q, u)
+    assert rel_inf(ud.cpu().numpy(), np.array(g["udot"])) < TOL
+    q1, u1, st = B.stepTo(q, u, RungeKuttaMerson(accuracy=1e-8), 1.0)
+    st = st.cpu().numpy()[0]
+    assert tuple(st[:3]) == (g["attempts"], g["accepted"], g["rejected"])
+    assert rel_inf(np.concatenate([q1.cpu().numpy(), u1.cpu().numpy()]), np.array(g["q1"] + g["u1"])) < 1e-9

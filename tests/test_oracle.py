@@ -1,0 +1,281 @@
+"""Pinning the oracle (CPU, no GPU needed).
+
+The oracle restates Simbody's tree path; these tests check it against what
+the reference's own tests hold (SD/FAST golden reaction forces), against
+closed forms (single and double pendulum), against the invariants of
+Simbody/tests/TestMassMatrix.cpp:658-887, against finite differences of its
+own kinematics, against conservation laws and against the order of the
+Runge-Kutta-Merson method -- chosen so that a dropped term, a wrong sign or a
+transposed operand fails at least one of them.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from common import mixed_trees, rel_inf, simtk_eq
+from simbody_amd import models as M
+from oracle_ctypes import OracleModel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _xf(p, R=None):
+    o = np.zeros(12)
+    o[:9] = (np.eye(3) if R is None else R).reshape(9)
+    o[9:] = p
+    return o
+
+
+# ---------------------------------------------------------------------------
+# Reference-held golden vectors
+# ---------------------------------------------------------------------------
+def test_sdfast_reaction_forces():
+    g = json.load(open(os.path.join(HERE, "golden", "sdfast_reactions.json")))
+    code = {"Pin": M.PIN, "Slider": M.SLIDER}
+    bs = g["bodies"]
+    spec = M.TreeSpec(parent=np.array([b["parent"] for b in bs], np.int32),
+                      mobilizer=np.array([code[b["mobilizer"]] for b in bs], np.int32),
+                      mass=np.array([b["mass"] for b in bs]), com_B=np.array([b["com"] for b in bs], float),
+                      inertia_B=np.array([b["inertia"] for b in bs], float),
+                      X_PF=np.array([_xf(b["p_PF"]) for b in bs]), X_BM=np.array([_xf(b["p_BM"]) for b in bs]),
+                      gravity=np.array(g["gravity"]))
+    o = OracleModel(spec)
+    for case in g["cases"]:
+        q = np.array(case["q"]); u = np.array(case["u"])
+        F = o.system_forces(q)
+        R = o.reactions(q, u, None, F)
+        X = o.kinematics(q, u)["X_GB"]
+        for b, exp in enumerate(case["reaction_in_B"], start=1):
+            Rgb = X[b, :9].reshape(3, 3)
+            for half in (slice(0, 3), slice(3, 6)):
+                got = Rgb.T @ R[b, half]
+                want = np.array(exp[half])
+                assert np.all(np.abs(got - want) < 1e-5 * max(np.linalg.norm(got), 1.0)), (b, got, want)
+
+
+# ---------------------------------------------------------------------------
+# Closed forms
+# ---------------------------------------------------------------------------
+def test_single_pendulum_closed_form():
+    """One Pin link of config 1: theta'' = -m g L sin(theta) / (I + m L^2) = -4.9 sin(theta)."""
+    o = OracleModel(M.pendulum_chain(1))
+    for th in np.linspace(-3.0, 3.0, 13):
+        for w in (-2.0, 0.0, 1.5):
+            ud, _, _, _ = o.realize_acceleration(np.array([th]), np.array([w]))
+            assert abs(ud[0] - (-4.9 * np.sin(th))) < 1e-13
+
+
+def test_double_pendulum_closed_form():
+    """Two Pin links (m = 1, I = 1 about the mass center, L = 1): Lagrange's
+    equations in absolute angles t1 = q1, t2 = q1 + q2:
+      3 t1'' + c t2'' + s t2'^2 + 2 g sin t1 = 0
+      c t1'' + 2 t2'' - s t1'^2 +   g sin t2 = 0,  c = cos(t1-t2), s = sin(t1-t2)."""
+    o = OracleModel(M.pendulum_chain(2))
+    rng = np.random.default_rng(3)
+    g = 9.8
+    for _ in range(20):
+        q = rng.uniform(-np.pi, np.pi, 2); u = rng.normal(size=2)
+        t1, t2 = q[0], q[0] + q[1]; w1, w2 = u[0], u[0] + u[1]
+        c, s = np.cos(t1 - t2), np.sin(t1 - t2)
+        A = np.array([[3.0, c], [c, 2.0]])
+        rhs = -np.array([s * w2**2 + 2 * g * np.sin(t1), -s * w1**2 + g * np.sin(t2)])
+        tdd = np.linalg.solve(A, rhs)
+        ud, _, _, _ = o.realize_acceleration(q, u)
+        assert np.allclose(ud, [tdd[0], tdd[1] - tdd[0]], rtol=0, atol=1e-12)
+
+
+def test_slider_under_gravity_fixed_step_is_exact():
+    """A Slider along a tilted axis falls with constant acceleration; RK-Merson
+    (4th order) integrates the quadratic exactly."""
+    R = M.quat_to_rot([0.9, 0.2, -0.3, 0.1])
+    spec = M.TreeSpec(parent=np.array([0], np.int32), mobilizer=np.array([M.SLIDER], np.int32),
+                      mass=np.array([2.0]), com_B=np.array([[0.1, 0.2, 0.0]]),
+                      inertia_B=np.array([[1.0, 2.0, 1.5, 0.1, 0.0, 0.0]]),
+                      X_PF=np.array([_xf([0, 0, 0], R)]), X_BM=np.array([_xf([0, 0.5, 0])]))
+    o = OracleModel(spec)
+    a = (R.T @ spec.gravity)[0]
+    ud, _, _, _ = o.realize_acceleration(np.array([0.3]), np.array([-0.7]))
+    assert abs(ud[0] - a) < 1e-14
+    y, st = o.integrate(np.array([0.3, -0.7]), 0.37, 1e-3, fixed_h=0.01)
+    T = 0.37
+    assert abs(y[0] - (0.3 - 0.7 * T + 0.5 * a * T * T)) < 1e-13
+    assert abs(y[1] - (-0.7 + a * T)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# Invariants (TestMassMatrix.cpp:658-887)
+# ---------------------------------------------------------------------------
+def _dense_M(o, q, nu):
+    Mm = np.zeros((nu, nu)); z = np.zeros(nu)
+    for i in range(nu):
+        e = np.zeros(nu); e[i] = 1
+        Mm[:, i] = o.residual(q, z, None, None, e)
+    return Mm
+
+
+@pytest.mark.parametrize("idx", range(6))
+def test_mass_matrix_invariants(idx):
+    spec = mixed_trees(11, 6, nmin=5, nmax=14)[idx]
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 11, idx)
+    nu = o.nu
+    Mm = _dense_M(o, q, nu)
+    assert np.abs(Mm - Mm.T).max() <= 1e-13 * np.abs(Mm).max()
+    assert np.linalg.eigvalsh(0.5 * (Mm + Mm.T)).min() > 0
+    v = np.random.default_rng(idx).normal(size=nu)
+    assert simtk_eq(o.minv(q, Mm @ v), v, n=nu)                 # M^-1 (M v) = v
+    f = np.random.default_rng(idx + 50).normal(size=nu)
+    ud0, _, _, _ = o.forward(q, np.zeros(nu), f, None)
+    assert simtk_eq(ud0, o.minv(q, f), n=nu)                    # forward(u=0) = M^-1 f
+    F = o.system_forces(q) + M.random_body_forces(spec, 3, idx)
+    ud, _, _, _ = o.forward(q, u, f, F)
+    res = o.residual(q, u, f, F, ud)
+    assert np.abs(res).max() < 1e-11 * max(1.0, np.abs(f).max())  # residual(forward(f)) = 0
+    C = o.residual(q, u, f, F, None)                              # M udot + C - f = 0 (dense check)
+    assert rel_inf(ud, np.linalg.solve(Mm, -C)) < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# Finite differences of the kinematics
+# ---------------------------------------------------------------------------
+def _skew_inv(W):
+    return np.array([W[2, 1] - W[1, 2], W[0, 2] - W[2, 0], W[1, 0] - W[0, 1]]) / 2
+
+
+@pytest.mark.parametrize("idx", range(3))
+def test_velocity_and_acceleration_match_finite_differences(idx):
+    spec = mixed_trees(13, 3, nmin=4, nmax=10)[idx]
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 13, idx)
+    K = o.kinematics(q, u)
+    qd = K["qdot"]
+    ud, _, A, _ = o.forward(q, u, np.zeros(o.nu), o.system_forces(q))
+    dt = 1e-6
+    Kp = o.kinematics(q + dt * qd, u + dt * ud)
+    Km = o.kinematics(q - dt * qd, u - dt * ud)
+    for b in range(1, o.nb + 1):
+        R = K["X_GB"][b, :9].reshape(3, 3)
+        dR = (Kp["X_GB"][b, :9] - Km["X_GB"][b, :9]).reshape(3, 3) / (2 * dt)
+        w_fd = _skew_inv(dR @ R.T)
+        v_fd = (Kp["X_GB"][b, 9:] - Km["X_GB"][b, 9:]) / (2 * dt)
+        assert np.allclose(K["V_GB"][b], np.concatenate([w_fd, v_fd]), atol=1e-6 * max(1, np.abs(K["V_GB"][b]).max()))
+        A_fd = (Kp["V_GB"][b] - Km["V_GB"][b]) / (2 * dt)
+        assert np.allclose(A[b], A_fd, atol=2e-5 * max(1, np.abs(A[b]).max())), b
+
+
+# ---------------------------------------------------------------------------
+# Conservation laws along RK-Merson trajectories
+# ---------------------------------------------------------------------------
+def test_energy_conservation():
+    """TestMobilizedBody.cpp:615-662 style: RKM at accuracy 1e-10 conserves
+    KE + PE to 1e-8 (relative) with gravity as the only force."""
+    spec = mixed_trees(17, 1, nmin=8, nmax=12)[0]
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 17, 0)
+    q = q * 1.0
+    K0 = o.kinematics(q, u)
+    y, st = o.integrate(np.concatenate([q, u]), 1.0, 1e-10)
+    assert st["status"] == 0 and st["accepted"] > 10
+    K1 = o.kinematics(y[:o.nq], y[o.nq:])
+    E0, E1 = K0["ke"] + K0["pe"], K1["ke"] + K1["pe"]
+    assert abs(E1 - E0) < 1e-8 * max(abs(E0), K0["ke"])
+
+
+def _momenta(spec, K):
+    """Linear momentum and angular momentum about the ground origin."""
+    Pm = np.zeros(3); Lm = np.zeros(3)
+    for b in range(1, spec.nb + 1):
+        R = K["X_GB"][b, :9].reshape(3, 3); p = K["X_GB"][b, 9:]
+        w = K["V_GB"][b, :3]; v = K["V_GB"][b, 3:]
+        m = spec.mass[b - 1]; c = R @ spec.com_B[b - 1]
+        I6 = spec.inertia_B[b - 1]
+        Io = np.array([[I6[0], I6[3], I6[4]], [I6[3], I6[1], I6[5]], [I6[4], I6[5], I6[2]]])
+        Ic = Io - m * (np.dot(spec.com_B[b - 1], spec.com_B[b - 1]) * np.eye(3) - np.outer(spec.com_B[b - 1], spec.com_B[b - 1]))
+        vc = v + np.cross(w, c)
+        rc = p + c
+        Pm += m * vc
+        Lm += R @ Ic @ R.T @ w + m * np.cross(rc, vc)
+    return Pm, Lm
+
+
+def test_momentum_conservation_free_floating_tree():
+    """Zero gravity, Free root (exercises the 6x6 D inverse): linear and
+    angular momentum are conserved."""
+    spec = mixed_trees(19, 1, nmin=6, nmax=9)[0]
+    spec.mobilizer[0] = M.FREE
+    spec.gravity = np.zeros(3)
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 19, 0)
+    P0, L0 = _momenta(spec, o.kinematics(q, u))
+    y, st = o.integrate(np.concatenate([q, u]), 0.5, 1e-10)
+    P1, L1 = _momenta(spec, o.kinematics(y[:o.nq], y[o.nq:]))
+    assert np.abs(P1 - P0).max() < 1e-8 * max(1, np.abs(P0).max())
+    assert np.abs(L1 - L0).max() < 1e-8 * max(1, np.abs(L0).max())
+
+
+# ---------------------------------------------------------------------------
+# Runge-Kutta-Merson
+# ---------------------------------------------------------------------------
+def test_merson_is_fourth_order():
+    """One fixed step of the propagated solution has local error O(h^5): the
+    error ratio between h and h/2 is ~32 (a wrong tableau entry drops the
+    order; RungeKuttaMersonIntegrator.cpp:60-80)."""
+    spec = M.pendulum_chain(3)
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 5, 0, qscale=1.0)
+    y0 = np.concatenate([q, u])
+    ref, _ = o.integrate(y0, 0.04, 1e-13)
+    errs = []
+    for h in (0.04, 0.02):
+        y = y0
+        y, _ = o.integrate(y, 0.04, 1e-3, fixed_h=h)
+        errs.append(np.abs(y - ref).max())
+    # two steps of h/2: global error over the interval scales ~ h^4 -> ratio ~16
+    assert 10 < errs[0] / errs[1] < 24, errs
+
+
+def test_adaptive_accuracy_controls_error():
+    """Tighter accuracy gives a proportionally closer trajectory (step
+    control, AbstractIntegratorRep.cpp:448-502)."""
+    spec = M.pendulum_chain(4)
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 6, 0)
+    y0 = np.concatenate([q, u])
+    ref, _ = o.integrate(y0, 0.5, 1e-12)
+    e = {}
+    for acc in (1e-4, 1e-6, 1e-8):
+        y, st = o.integrate(y0, 0.5, acc)
+        e[acc] = np.abs(y - ref).max() / max(1.0, np.abs(ref).max())
+        assert st["status"] == 0
+    assert e[1e-8] < e[1e-6] < e[1e-4]
+    assert e[1e-8] < 1e-6
+
+
+def test_quaternion_projection_cadence():
+    """Quaternion normalization happens only when the RMS of |q|-1 exceeds
+    the constraint tolerance (accuracy/10; SimbodyMatterSubsystemRep.cpp:
+    4096-4184): at loose accuracy with large steps it occurs, and the
+    integrated quaternions stay normalized to within the tolerance."""
+    spec = M.spatial_chain(6, seed=3, types=(M.BALL,))
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 3, 0, uscale=10.0)
+    for acc in (1e-1, 3e-2, 1e-2):
+        y, st = o.integrate(np.concatenate([q, u]), 2.0, acc)
+        assert st["projections"] > 0
+        err = [np.linalg.norm(y[4 * b:4 * b + 4]) - 1 for b in range(6)]
+        # every accepted adaptive step leaves RMS(|q|-1) <= consTol = accuracy/10
+        assert np.sqrt(np.mean(np.square(err))) <= acc / 10
+
+
+def test_pendulum10_regression_fixture():
+    """The oracle still reproduces the committed config-1 fixture."""
+    g = json.load(open(os.path.join(HERE, "golden", "pendulum10.json")))
+    o = OracleModel(M.pendulum_chain(10))
+    q, u = np.array(g["q"]), np.array(g["u"])
+    ud, _, _, _ = o.realize_acceleration(q, u)
+    assert np.array_equal(ud, np.array(g["udot"]))
+    y, st = o.integrate(np.concatenate([q, u]), 1.0, 1e-8)
+    assert (st["attempts"], st["accepted"], st["rejected"]) == (g["attempts"], g["accepted"], g["rejected"])
+    assert np.array_equal(y, np.array(g["q1"] + g["u1"]))
