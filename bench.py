@@ -30,6 +30,12 @@ FLOPS_PER_BODY_STEP = 9034.59375
 FLOPS_RESID_PER_BODY = 1055.40625
 PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 (vector = matrix) peak
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per body-step of k_rkm_chain on this workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE per dispatch (difference of a 10-step and a 1-step
+# launch, over 9 steps x 2,097,152 bodies), each scaled by what the same
+# counter reads for a plain 2 GiB device copy (FETCH x2.0, WRITE x1.0).
+TRAFFIC_FETCH_PER_BODY_STEP = 1615.9
+TRAFFIC_WRITE_PER_BODY_STEP = 1078.7
 
 
 def log(msg):
@@ -68,7 +74,19 @@ def main():
     ap.add_argument("--bodies", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28                         # 2 GiB of doubles read + 2 GiB written
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the instance batch would shard "
                                    "across GPUs by instance (see DESIGN.md), no multi-GPU run is provided"}))
@@ -118,6 +136,7 @@ def main():
     torch.cuda.synchronize()
     id_ms = e0.elapsed_time(e1)
     achieved = FLOPS_PER_BODY_STEP * body_steps / (ms / 1e3) / 1e12
+    traffic = (TRAFFIC_FETCH_PER_BODY_STEP + TRAFFIC_WRITE_PER_BODY_STEP) * body_steps
     out = {
         "metric": "body-steps/s on 1 MI355X, plain vs mixed/batched at stated err; p50/p99 latency",
         "value": value, "unit": "body-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -128,8 +147,11 @@ def main():
                    "integrator": "RungeKuttaMerson fixed step", "precision": "fp64"},
         "roofline": {"bound": "mfma", "compute": "fp64 vector ALU (no MFMA is used)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "flops_per_body_step": FLOPS_PER_BODY_STEP, "kernel": "k_rkm"},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_note": "HBM bytes of the timed launch (rocprofv3 FETCH_SIZE+WRITE_SIZE per "
+                                     "body-step, copy-calibrated, x body-steps)",
+                     "hbm_gbs": traffic / (ms / 1e3) / 1e9, "hbm_frac": traffic / (ms / 1e3) / 1e9 / PEAK_HBM_GBS,
+                     "flops_per_body_step": FLOPS_PER_BODY_STEP, "kernel": "k_rkm_chain"},
         "inverse_dynamics": {"ms": id_ms, "bodies_per_s": n * spec.nb / (id_ms / 1e3),
                              "tflops": FLOPS_RESID_PER_BODY * n * spec.nb / (id_ms / 1e3) / 1e12},
         "wall_s_timed": wall,

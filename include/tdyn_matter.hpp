@@ -1,0 +1,116 @@
+// tdyn_matter.hpp -- C++ drop-in over the tdyn C ABI (include/tdyn.h).
+//
+// `tdyn::MatterBatch` carries the operator names and argument meaning of
+// Simbody's SimbodyMatterSubsystem (Simbody/include/simbody/internal/
+// SimbodyMatterSubsystem.h) for a batch of independent tree instances; the
+// integrator half mirrors RungeKuttaMersonIntegrator + TimeStepper.  Errors
+// are thrown as tdyn::Error (the reference throws SimTK exceptions from the
+// same places, SimbodyMatterSubsystem.cpp:247-261, 438-442).  All array
+// arguments are device pointers in the packed layout of tdyn.h.
+#pragma once
+#include "tdyn.h"
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace tdyn {
+
+class Error : public std::runtime_error {
+public:
+    Error(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
+    int code() const { return code_; }
+private:
+    int code_;
+};
+
+inline void check(int rc) {
+    if (rc != TDYN_OK) throw Error(rc, tdyn_last_error());
+}
+
+// RungeKuttaMersonIntegrator settings (Integrator.cpp:243-270).
+struct RungeKuttaMerson {
+    tdyn_integrator_cfg cfg{1e-3, 0, 0, 0, 0, 0, 0.1, 0, 0};
+    RungeKuttaMerson& setAccuracy(double a) { cfg.accuracy = a; return *this; }
+    RungeKuttaMerson& setConstraintTolerance(double c) { cfg.constraint_tolerance = c; return *this; }
+    RungeKuttaMerson& setFixedStepSize(double h) { cfg.fixed_step = h; return *this; }
+    RungeKuttaMerson& setInitialStepSize(double h) { cfg.initial_step = h; return *this; }
+    RungeKuttaMerson& setMinimumStepSize(double h) { cfg.min_step = h; return *this; }
+    RungeKuttaMerson& setMaximumStepSize(double h) { cfg.max_step = h; return *this; }
+};
+
+class MatterBatch {
+public:
+    MatterBatch(const std::vector<tdyn_tree_desc>& trees, const std::vector<int32_t>& treeOfInstance) {
+        check(tdyn_batch_create(trees.data(), (int)trees.size(),
+                                treeOfInstance.empty() ? nullptr : treeOfInstance.data(),
+                                (int)treeOfInstance.size(), &h_));
+    }
+    MatterBatch(const tdyn_tree_desc& tree, int nInstances) {
+        check(tdyn_batch_create(&tree, 1, nullptr, nInstances, &h_));
+    }
+    ~MatterBatch() { if (h_) tdyn_batch_destroy(h_); }
+    MatterBatch(const MatterBatch&) = delete;
+    MatterBatch& operator=(const MatterBatch&) = delete;
+
+    tdyn_batch* handle() const { return h_; }
+    void setStream(hipStream_t s) { s_ = s; }
+
+    // sizes: instances, total q, total u, total bodies incl. Ground, max nq, max nu, max nb, trees
+    std::vector<int64_t> info() const { std::vector<int64_t> v(8); check(tdyn_batch_info(h_, v.data())); return v; }
+
+    // realizePositionKinematics + realizeVelocityKinematics, then X_GB, V_GB, qdot, [KE, PE]
+    void realizeKinematics(const double* q, const double* u, double* X_GB, double* V_GB, double* qdot,
+                           double* energies) const {
+        check(tdyn_realize_kinematics(h_, q, u, X_GB, V_GB, qdot, energies, s_));
+    }
+    // System::realize(state, Stage::Acceleration) with gravity (+ constant applied forces)
+    void realizeAcceleration(const double* q, const double* u, const double* appliedBodyForces, double* udot,
+                             double* qdot = nullptr) const {
+        check(tdyn_realize_acceleration(h_, q, u, appliedBodyForces, udot, qdot, s_));
+    }
+    void calcAccelerationIgnoringConstraints(const double* q, const double* u, const double* appliedMobilityForces,
+                                             const double* appliedBodyForces, double* udot, double* A_GB,
+                                             double* qdotdot = nullptr) const {
+        check(tdyn_calc_acceleration_ignoring_constraints(h_, q, u, appliedMobilityForces, appliedBodyForces,
+                                                          udot, A_GB, qdotdot, s_));
+    }
+    void multiplyByMInv(const double* q, const double* v, double* MinvV) const {
+        check(tdyn_multiply_by_m_inv(h_, q, v, MinvV, s_));
+    }
+    void calcResidualForceIgnoringConstraints(const double* q, const double* u, const double* appliedMobilityForces,
+                                              const double* appliedBodyForces, const double* knownUdot,
+                                              double* residualMobilityForces) const {
+        check(tdyn_calc_residual_force_ignoring_constraints(h_, q, u, appliedMobilityForces, appliedBodyForces,
+                                                            knownUdot, residualMobilityForces, s_));
+    }
+    // realizeArticulatedBodyInertias + getArticulatedBodyInertia for every body
+    void realizeArticulatedBodyInertias(const double* q, double* P, double* PPlus = nullptr) const {
+        check(tdyn_calc_articulated_body_inertias(h_, q, P, PPlus, s_));
+    }
+    void calcMobilizerReactionForces(const double* q, const double* u, const double* appliedMobilityForces,
+                                     const double* appliedBodyForces, double* FM_G) const {
+        check(tdyn_calc_mobilizer_reaction_forces(h_, q, u, appliedMobilityForces, appliedBodyForces, FM_G, s_));
+    }
+
+    // TimeStepper(system, integ).initialize(state); stepTo(tEnd) for every instance.
+    void stepTo(double* q, double* u, const double* appliedBodyForces, double t0, double tEnd,
+                const RungeKuttaMerson& integ, int32_t* stats = nullptr) const {
+        check(tdyn_rollout(h_, q, u, appliedBodyForces, t0, tEnd, &integ.cfg, stats, s_));
+    }
+    // resumable form
+    void initialize(const double* q, const double* u, const double* appliedBodyForces, double t0,
+                    const double* tEnd, double tEndAll, const RungeKuttaMerson& integ) const {
+        check(tdyn_rollout_begin(h_, q, u, appliedBodyForces, t0, tEnd, tEndAll, &integ.cfg, s_));
+    }
+    void advance(int maxSteps) const { check(tdyn_rollout_advance(h_, maxSteps, s_)); }
+    void getState(double* q, double* u, double* t, int32_t* status, int32_t* stats) const {
+        check(tdyn_rollout_get(h_, q, u, t, status, stats, s_));
+    }
+
+private:
+    tdyn_batch* h_ = nullptr;
+    hipStream_t s_ = nullptr;
+};
+
+} // namespace tdyn

@@ -1,0 +1,56 @@
+// Example caller of the C++ drop-in (include/tdyn_matter.hpp): the README
+// pendulum of the reference extended to 10 Pin links (config 1), written the
+// way a Simbody user writes it -- build the tree, realize(Acceleration), then
+// TimeStepper::stepTo with RungeKuttaMerson at accuracy 1e-8.
+// Usage: dropin_example q0..q9 u0..u9  -> prints one JSON line.
+#include "../../include/tdyn_matter.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+int main(int argc, char** argv) {
+    const int n = 10;
+    if (argc != 1 + 2 * n) { std::fprintf(stderr, "usage: %s q[10] u[10]\n", argv[0]); return 2; }
+    std::vector<int32_t> parent(n), mob(n, TDYN_PIN);
+    std::vector<double> mass(n, 1.0), com(3 * n, 0.0), inertia(6 * n, 0.0), X_PF(12 * n, 0.0), X_BM(12 * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        parent[i] = i;                                           // chain: body i+1 hangs from body i
+        inertia[6*i] = inertia[6*i+1] = inertia[6*i+2] = 1.0;    // UnitInertia(1), mass 1
+        X_PF[12*i] = X_PF[12*i+4] = X_PF[12*i+8] = 1.0;          // joint at the parent's origin
+        X_BM[12*i] = X_BM[12*i+4] = X_BM[12*i+8] = 1.0; X_BM[12*i+10] = 1.0;   // (0,1,0) in B
+    }
+    tdyn_tree_desc d{n, parent.data(), mob.data(), mass.data(), com.data(), inertia.data(), X_PF.data(), X_BM.data(),
+                     {0.0, -9.8, 0.0}};
+    std::vector<double> hq(n), hu(n);
+    for (int i = 0; i < n; ++i) { hq[i] = std::atof(argv[1 + i]); hu[i] = std::atof(argv[1 + n + i]); }
+    double *q, *u, *udot; int32_t* stats;
+    if (hipMalloc((void**)&q, n * 8) || hipMalloc((void**)&u, n * 8) || hipMalloc((void**)&udot, n * 8) ||
+        hipMalloc((void**)&stats, 16)) { std::fprintf(stderr, "hipMalloc failed\n"); return 1; }
+    (void)hipMemcpy(q, hq.data(), n * 8, hipMemcpyHostToDevice);
+    (void)hipMemcpy(u, hu.data(), n * 8, hipMemcpyHostToDevice);
+    try {
+        tdyn::MatterBatch matter(d, 1);
+        matter.realizeAcceleration(q, u, nullptr, udot);
+        std::vector<double> hud(n);
+        (void)hipMemcpy(hud.data(), udot, n * 8, hipMemcpyDeviceToHost);
+        tdyn::RungeKuttaMerson integ; integ.setAccuracy(1e-8);
+        matter.stepTo(q, u, nullptr, 0.0, 1.0, integ, stats);
+        std::vector<int32_t> hs(4);
+        (void)hipMemcpy(hq.data(), q, n * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(hu.data(), u, n * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(hs.data(), stats, 16, hipMemcpyDeviceToHost);
+        auto arr = [](const std::vector<double>& v) {
+            std::string s = "[";
+            for (size_t i = 0; i < v.size(); ++i) { char b[40]; std::snprintf(b, sizeof b, "%s%.17g", i ? "," : "", v[i]); s += b; }
+            return s + "]";
+        };
+        std::printf("{\"udot\": %s, \"q1\": %s, \"u1\": %s, \"attempts\": %d, \"accepted\": %d, \"rejected\": %d}\n",
+                    arr(hud).c_str(), arr(hq).c_str(), arr(hu).c_str(), hs[0], hs[1], hs[2]);
+    } catch (const tdyn::Error& e) {
+        std::fprintf(stderr, "tdyn error %d: %s\n", e.code(), e.what());
+        return 1;
+    }
+    return 0;
+}

@@ -46,7 +46,6 @@ def test_invalid_trees_are_rejected_on_the_host():
     bad_free = M.TreeSpec(**{**spec.__dict__, "mobilizer": np.array([0, 4, 0, 0], np.int32)})
     for s, code in [(bad_parent, 1), (bad_mass, 1), (bad_free, 1)]:
         with pytest.raises(matter.TdynError) as e:
-            matter.MatterBatch.__init__.__wrapped__ if False else None
             _create(L, s)
         assert e.value.code == code
 

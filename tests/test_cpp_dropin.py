@@ -1,0 +1,33 @@
+"""The C++ drop-in header (include/tdyn_matter.hpp) used as a Simbody user
+would: the 10-link pendulum of config 1, realize(Acceleration) and
+TimeStepper::stepTo with RK-Merson at accuracy 1e-8, compared with the
+committed fixture (oracle results)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from common import rel_inf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "simbody_amd", "dropin_example")
+
+
+def test_example_is_built():
+    if not os.path.exists(os.path.join(ROOT, "simbody_amd", "libtdyn.so")):
+        pytest.skip("library not built")
+    assert os.path.exists(EXE), "build() must build simbody_amd/dropin_example"
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_pendulum10():
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "pendulum10.json")))
+    args = [EXE] + ["%.17g" % x for x in g["q"]] + ["%.17g" % x for x in g["u"]]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    r = json.loads(out.stdout)
+    assert rel_inf(r["udot"], g["udot"]) < 1e-12
+    assert (r["attempts"], r["accepted"], r["rejected"]) == (g["attempts"], g["accepted"], g["rejected"])
+    assert rel_inf(r["q1"] + r["u1"], g["q1"] + g["u1"]) < 1e-9
