@@ -65,6 +65,195 @@ def cpu_baseline(spec, Y, steps, h, budget_s):
             "sample": f"{n_inst} chains x {steps} fixed steps (of the same workload), {threads} threads, {sec:.2f} s"}
 
 
+def _cpu_threads():
+    try:
+        n_cpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n_cpu = os.cpu_count() or 1
+    return max(1, min(16, n_cpu))
+
+
+def row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=True):
+    """10-link planar Pin pendulum, one instance: realize(Acceleration), then
+    RK-Merson accuracy 1e-8 to t = 1 s.  Latency-bound: wall times."""
+    spec = M.pendulum_chain(10)
+    rng = np.random.default_rng(20261202)
+    q0 = rng.uniform(-np.pi, np.pi, 10); u0 = rng.normal(size=10)
+    B = MatterBatch([spec], n_instances=1, device="cuda:0")
+    q = B.to_device(q0); u = B.to_device(u0)
+    integ = RungeKuttaMerson(accuracy=1e-8)
+    B.realizeAcceleration(q, u); B.stepTo(q, u, integ, 1.0)       # warm (module load, allocation)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ud, _ = B.realizeAcceleration(q, u)
+    torch.cuda.synchronize()
+    t_acc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, _, st = B.stepTo(q, u, integ, 1.0)
+    torch.cuda.synchronize()
+    t_roll = time.perf_counter() - t0
+    st = st.cpu().numpy()[0]
+    out = {"workload": "1 x 10-link planar Pin pendulum, FP64, RK-Merson accuracy 1e-8 to t=1 s",
+           "gpu_realize_acceleration_ms": t_acc * 1e3, "gpu_rollout_ms": t_roll * 1e3,
+           "accepted_steps": int(st[1]), "rejected_steps": int(st[2]),
+           "gpu_body_steps_per_s": 10 * int(st[1]) / t_roll}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_ctypes import OracleModel, time_integrate_batch
+        om = OracleModel(spec)
+        Y = np.concatenate([q0, u0])[None, :].copy()
+        sec, bs, _ = time_integrate_batch([om], np.zeros(1, np.int32), Y, 1.0, 1e-8, 0.0, 1)
+        out["cpu_rollout_ms"] = sec * 1e3
+        out["cpu_body_steps_per_s"] = bs / sec
+        out["cpu_cores"] = 1
+    return out
+
+
+def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0):
+    """Uneven branched trees (nb ~ U{8..128}, depth <= 20, Pin/Slider/
+    Universal/Ball + Free roots), constant N(0,1) body forces and torques,
+    per-instance RK-Merson step control at accuracy 1e-6 to t = 0.5 s, in
+    one launch."""
+    log(f"[bench] trees: generating {n}")
+    trees = [M.random_tree(3, i) for i in range(n)]
+    Ys = [M.random_states(t, 3, [i])[0] for i, t in enumerate(trees)]
+    Fs = [M.random_body_forces(t, 3, i) for i, t in enumerate(trees)]
+    nbs = np.array([t.nb for t in trees])
+    B = MatterBatch(trees, tree_of=np.arange(n), device="cuda:0")
+    q = B.pack([y[:t.nq] for y, t in zip(Ys, trees)]); u = B.pack([y[t.nq:] for y, t in zip(Ys, trees)])
+    F = B.pack(Fs)
+    integ = RungeKuttaMerson(accuracy=1e-6)
+    torch.cuda.synchronize()
+    log("[bench] trees: rollout")
+    t0 = time.perf_counter()
+    _, _, st = B.stepTo(q, u, integ, 0.5, fapp=F)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    st = st.cpu().numpy()
+    body_steps = int((nbs * st[:, 1]).sum())
+    out = {"workload": f"{n} uneven trees, nb~U{{8..128}} (mean {nbs.mean():.1f}), FP64, RK-Merson "
+                       "accuracy 1e-6 to t=0.5 s, N(0,1) body forces",
+           "value": body_steps / sec, "unit": "body-steps/s", "seconds": sec,
+           "body_steps": body_steps, "accepted_mean": float(st[:, 1].mean()),
+           "rejected_total": int(st[:, 2].sum()), "evaluator": "tree sweeps (k_rkm)"}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_ctypes import OracleModel, time_integrate_batch
+        threads = _cpu_threads()
+        # per-instance cost scales with nb x steps: take instances in order until the budget estimate
+        k = threads
+        while True:
+            oms = [OracleModel(t) for t in trees[:k]]
+            ny = max(t.nq + t.nu for t in trees[:k]); nbm = max(t.nb for t in trees[:k])
+            Y = np.zeros((k, ny)); Fa = np.zeros((k, 6 * (nbm + 1)))
+            for i in range(k):
+                Y[i, :len(Ys[i])] = Ys[i]; Fa[i, :Fs[i].size] = Fs[i].reshape(-1)
+            sec_c, bs, _ = time_integrate_batch(oms, np.arange(k, dtype=np.int32), Y, 0.5, 1e-6, 0.0, threads, Fapp=Fa)
+            if sec_c > cpu_budget / 4 or k >= n:
+                break
+            k = min(n, int(k * max(2.0, cpu_budget / max(sec_c, 1e-3) * 0.8)))
+        out["cpu_baseline"] = {"value": bs / sec_c, "unit": "body-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"first {k} of the {n} trees, {threads} threads, {sec_c:.2f} s"}
+    return out
+
+
+def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 5000, 20000, 50000), window=1.0, n_cap=25000,
+                seed=5, budget_gib=24, steps_per_advance=4):
+    """Serving loop: Poisson arrivals at each rate; every request is its own
+    32-body chain drawn from the workload's distributions (Pin/Universal/Ball
+    cycling, own masses, inertias and offsets), a seeded initial state,
+    horizon ~ U(0.1, 2) s, RK-Merson accuracy 1e-6.  Slots come from an HBM
+    budget; latency = arrival -> result on the host."""
+    from simbody_amd.serving import Requests, Server, serve
+    template = M.spatial_chain(32, seed=2)
+    n_max = min(n_cap, int(max(rates) * window))
+    log(f"[bench] serving: generating {n_max} requests")
+    params = M.chain_params(seed, np.arange(n_max), nb=32)
+    Y = M.random_states(template, seed, np.arange(n_max))
+    horizon = M.Rng(seed, 91).uniform(n_max, 0.1, 2.0)
+    out = {"workload": "Poisson arrivals; each request one 32-body chain with its own constants, horizon "
+                       "U(0.1,2) s, RK-Merson accuracy 1e-6; FP64 and mixed precision",
+           "hbm_budget_bytes": budget_gib << 30, "steps_per_advance": steps_per_advance, "sweep": []}
+    for prec, rate in [(p, r) for p in (0, 1) for r in rates]:
+        sv = Server(template, RungeKuttaMerson(accuracy=1e-6, precision=prec), budget_gib << 30, max_slots=65536)
+        out["slots_fp64" if prec == 0 else "slots_mixed"] = sv.slots
+        n = min(n_cap, int(rate * window))
+        req = Requests.from_chain_params(template, tuple(p[:n] for p in params), Y[:n, :template.nq],
+                                         Y[:n, template.nq:], horizon[:n])
+        arrivals = np.cumsum(np.random.default_rng(seed + rate).exponential(1.0 / rate, size=n))
+        log(f"[bench] serving: precision {prec}, {rate} req/s, {n} requests")
+        r = serve(sv, req, arrivals, steps_per_advance=steps_per_advance, time_limit=180)
+        ok = np.isfinite(r["latency"])
+        lat = r["latency"][ok]
+        body_steps = int((r["stats"][ok, 1].astype(np.int64) * template.nb).sum())
+        out["sweep"].append({"precision": "fp64" if prec == 0 else "mixed", "rate_req_s": rate, "requests": n,
+                             "completed": int(r["completed"]),
+                             "failed": int((r["status"][ok] < 0).sum()), "wall_s": r["wall"],
+                             "throughput_body_steps_s": body_steps / r["wall"],
+                             "completed_req_s": int(r["completed"]) / r["wall"],
+                             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+                             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+                             "accepted_steps_mean": float(r["stats"][ok, 1].mean()) if len(lat) else None,
+                             "max_occupied_slots": int(r["max_occupied"]), "launches": int(r["launches"])})
+        log(f"[bench] serving: {out['sweep'][-1]}")
+        del sv
+    return out
+
+
+def row_mixed(torch, M, MatterBatch, RungeKuttaMerson, n=65536, T=0.5, h=1e-3):
+    """Mixed precision vs FP64 on the 32-body chains and on a stiff variant
+    (neighbouring-link mass ratios up to 1e4): Merson fixed step h to T.
+    Errors of the mixed path are measured against the FP64 GPU path (which
+    matches the FP64 oracle to 1e-12 on udot): udot at t=0, q and u at T,
+    and the energy drift E(T)-E(0) of both paths."""
+    out = {}
+    steps = int(round(T / h))
+    for variant, spec in (("chains", M.spatial_chain(32, seed=2)), ("stiff", M.stiff_chain(32, seed=2))):
+        Y = M.chain_batch_states(spec, n, seed=2026)
+        res = {}
+        for prec in (0, 1):
+            B = MatterBatch([spec], n_instances=n, device="cuda:0")
+            q = B.to_device(Y[:, :spec.nq]); u = B.to_device(Y[:, spec.nq:])
+            B.rollout_begin(q, u, RungeKuttaMerson(fixed_step=h, precision=prec), t_end=T)
+            _, ud = B.rollout_derivs()
+            _, _, _, E0 = B.realizeKinematics(q, u, False, False)
+            B.rollout_advance(2)                       # warm-up steps (counted in the state, not timed)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(); B.rollout_advance(steps); e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            qT, uT, tT, st, stats = B.rollout_get()
+            _, _, _, E1 = B.realizeKinematics(qT, uT, False, False)
+            torch.cuda.synchronize()
+            assert (st.cpu().numpy() == 1).all()
+            E0 = E0.cpu().numpy().reshape(n, 2).sum(1); E1 = E1.cpu().numpy().reshape(n, 2).sum(1)
+            res[prec] = dict(ms=ms, ud=ud.cpu().numpy().reshape(n, -1),
+                             y=np.concatenate([qT.cpu().numpy().reshape(n, -1), uT.cpu().numpy().reshape(n, -1)], 1),
+                             drift=np.abs(E1 - E0) / np.maximum(np.abs(E0), 1e-300),
+                             accepted=int(stats.cpu().numpy()[:, 1].min()))
+            del B
+        def rel(a, b):
+            return np.abs(a - b).max(1) / np.maximum(np.abs(b).max(1), 1e-300)
+        eu = rel(res[1]["ud"], res[0]["ud"]); ey = rel(res[1]["y"], res[0]["y"])
+        timed_steps = res[0]["accepted"] - 2
+        out[variant] = {
+            "fp64_body_steps_s": n * spec.nb * timed_steps / (res[0]["ms"] / 1e3),
+            "mixed_body_steps_s": n * spec.nb * timed_steps / (res[1]["ms"] / 1e3),
+            "speedup_mixed_over_fp64": res[0]["ms"] / res[1]["ms"],
+            "udot_rel_err_max": float(eu.max()), "udot_rel_err_median": float(np.median(eu)),
+            "state_T_rel_err_max": float(ey.max()), "state_T_rel_err_median": float(np.median(ey)),
+            "energy_drift_fp64_median": float(np.median(res[0]["drift"])),
+            "energy_drift_mixed_median": float(np.median(res[1]["drift"])),
+            "energy_drift_fp64_max": float(res[0]["drift"].max()),
+            "energy_drift_mixed_max": float(res[1]["drift"].max())}
+        log(f"[bench] mixed {variant}: {out[variant]}")
+    out["workload"] = (f"{n} x 32-body chains and a stiff variant (neighbouring mass ratios up to 1e4), Merson "
+                       f"fixed step h={h} to T={T}; errors of the mixed path relative to the FP64 path "
+                       "(per instance ||d||_inf/||ref||_inf)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +263,10 @@ def main():
     ap.add_argument("--bodies", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the secondary workloads (pendulum latency, uneven trees)")
+    ap.add_argument("--only", choices=["pendulum", "trees", "serving", "mixed"], default=None,
+                    help="run only one secondary workload and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
@@ -95,6 +288,18 @@ def main():
     import torch
     from simbody_amd import models as M
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    if args.only == "pendulum":
+        print(json.dumps(row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "trees":
+        print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)), flush=True)
+        return
+    if args.only == "mixed":
+        print(json.dumps(row_mixed(torch, M, MatterBatch, RungeKuttaMerson)), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(row_serving(torch, M, RungeKuttaMerson)), flush=True)
+        return
 
     h = 1e-3
     spec = M.spatial_chain(args.bodies, seed=2)
@@ -159,6 +364,15 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(spec, Y, args.steps, h, args.cpu_budget)
+    if not args.no_rows:
+        del B
+        rows = {}
+        log("[bench] pendulum latency")
+        rows["pendulum10"] = row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)
+        rows["uneven_trees"] = row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)
+        rows["mixed_precision"] = row_mixed(torch, M, MatterBatch, RungeKuttaMerson)
+        rows["serving"] = row_serving(torch, M, RungeKuttaMerson)
+        out["rows"] = rows
     print(json.dumps(out), flush=True)
 
 

@@ -147,7 +147,10 @@ typedef struct tdyn_integrator_cfg {
     double initial_step, min_step, max_step;   /* <=0: unset */
     double time_scale;           /* System default time scale (<=0: 0.1) */
     int32_t max_attempts;        /* per-instance attempt budget (<=0: 1e8) */
-    int32_t precision;           /* 0 = FP64 (the only mode of this build) */
+    int32_t precision;           /* 0 = FP64; 1 = mixed: FP32 kinematics, bias forces and
+                                    articulated inertias, FP64 D = H^T P H and its inverse,
+                                    quaternion normalization and Kahan-compensated state
+                                    (chain evaluator only; TDYN_E_ARG otherwise) */
 } tdyn_integrator_cfg;
 
 /* TimeStepper::initialize (Integrator.cpp:333-435): load q,u (packed),
@@ -166,10 +169,50 @@ int tdyn_rollout_advance(tdyn_batch* b, int max_steps, hipStream_t s);
  * projections; 4 ints per instance).  Any output may be NULL. */
 int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* status,
                      int32_t* stats, hipStream_t s);
+/* The state derivative (qdot, udot; packed) of the current rollout state,
+ * evaluated by the rollout's evaluator at the rollout's precision: the f0
+ * the integrator uses (realize(state, Stage::Acceleration) + getQDot/getUDot,
+ * SimbodyMatterSubsystemRep.cpp:1527-1555).  Overwrites the integrator's
+ * stage storage; call between advances. */
+int tdyn_rollout_derivs(tdyn_batch* b, double* qdot, double* udot, hipStream_t s);
 /* begin + advance until every instance finished + get; synchronizes s.
  * Returns TDYN_E_STEP_FAILED if any instance failed. */
 int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0,
                  double t_end, const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s);
+
+/* ---- Serving: continuous batching of rollout requests ------------------
+ * No reference counterpart: Simbody runs one State per caller.  A server
+ * owns device slots for trees of ONE topology (the parent and mobilizer
+ * arrays of `topology`, a serial chain); every request brings its own
+ * masses, frames, gravity, initial state, final time and (optionally)
+ * applied body forces, and keeps its State on the device until collected.
+ * The slot count follows from an HBM budget.  Requests admitted between
+ * advances join the running ones; each slot runs the same arithmetic as a
+ * one-instance tdyn_rollout of that request, so results do not depend on
+ * what else is in the server.  Not thread-safe (one host thread). */
+typedef struct tdyn_server tdyn_server;
+
+/* slots = min(max_slots, hbm_budget_bytes / bytes per slot), rounded down to
+ * a multiple of 64 (TDYN_E_NOMEM if that is 0). */
+int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg* cfg,
+                       size_t hbm_budget_bytes, int max_slots, tdyn_server** out);
+int tdyn_server_destroy(tdyn_server* sv);
+/* info[0..5] = slots, free slots, occupied slots, bytes per slot, nq, nu. */
+int tdyn_server_info(const tdyn_server* sv, int64_t* info);
+/* Admit the first min(n, free slots) requests; host arrays: trees[n] (same
+ * topology), q [n][nq], u [n][nu], t_end[n], fapp [n][(nb+1)*6] or NULL.
+ * Writes the slot of each admitted request and their count.  All n requests
+ * are validated before any is admitted.  Synchronizes s. */
+int tdyn_server_admit(tdyn_server* sv, int n, const tdyn_tree_desc* trees, const double* q, const double* u,
+                      const double* t_end, const double* fapp, int32_t* slots, int* n_admitted, hipStream_t s);
+/* Advance every running slot by at most max_steps accepted steps (one launch, async). */
+int tdyn_server_advance(tdyn_server* sv, int max_steps, hipStream_t s);
+/* Retire up to max_out finished slots (reached t_end or failed): host
+ * outputs slot ids, q [n][nq], u [n][nu], t, status (1 done, <0 failed) and
+ * stats [n][4]; any output but slots may be NULL.  The slots become free.
+ * Synchronizes s. */
+int tdyn_server_collect(tdyn_server* sv, int max_out, int32_t* slots, double* q, double* u, double* t,
+                        int32_t* status, int32_t* stats, int* n_out, hipStream_t s);
 
 const char* tdyn_last_error(void);
 

@@ -1,8 +1,7 @@
 // Host side of the tdyn C ABI: validates and packs tree descriptions into
 // device tables, owns the per-lane scratch and the device-resident integrator
 // state, and launches the kernels of tdyn_kernels.hip.
-#include "../../include/tdyn.h"
-#include "tdyn_internal.hpp"
+#include "tdyn_host.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -12,83 +11,20 @@
 #include <algorithm>
 
 using namespace tdyn;
+using namespace tdyn::host;
 
-namespace {
-thread_local std::string g_err;
-int fail(int code, const std::string& msg) { g_err = msg; return code; }
-int hipfail(hipError_t e, const char* what) {
-    g_err = std::string(what) + ": " + hipGetErrorString(e);
-    return TDYN_E_HIP;
-}
-#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return hipfail(e_, #x); } while (0)
-
-template <class T> struct DBuf {
-    T* p = nullptr; size_t n = 0;
-    hipError_t alloc(size_t count) {
-        free();
-        n = count;
-        if (!count) return hipSuccess;
-        return hipMalloc((void**)&p, count * sizeof(T));
-    }
-    hipError_t upload(const std::vector<T>& v) {
-        hipError_t e = alloc(v.size());
-        if (e != hipSuccess || v.empty()) return e;
-        return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-    }
-    void free() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
-    ~DBuf() { free(); }
-};
-} // namespace
-
-struct tdyn_batch {
-    int n_inst = 0, n_trees = 0;
-    int64_t total_q = 0, total_u = 0, total_b = 0;
-    int max_nq = 0, max_nu = 0, max_nb = 0, max_rec = 0;
-    bool uniform = true;
-    std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
-    std::vector<int4> h_tree_a, h_tree_b, h_body_a;
-    // device tables
-    DBuf<int4> body_a, body_b, tree_a, tree_b;
-    DBuf<int> child_idx, tree_of, q_off, u_off, b_off;
-    DBuf<double> prm, gravity, scratch;
-    // rollout state
-    DBuf<double> vec, uscale, t, h, t_end;
-    DBuf<int> status, stats;
-    DBuf<double> fapp;
-    int ny_max = 0;
-    bool rollout_ready = false;
-    bool is_chain = false;       // uniform batch whose tree is a serial chain
-    int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
-    RkmCfg cfg{};
-
-    DevTables tables() const {
-        DevTables T;
-        T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
-        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p;
-        return T;
-    }
-    OpArgs op() const {
-        OpArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
-        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
-        return a;
-    }
-    RkmArgs rk() const {
-        RkmArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
-        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
-        a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
-        a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = 0;
-        a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
-        return a;
-    }
-};
+namespace tdyn {
+namespace host {
+std::string& err_slot() { static thread_local std::string e; return e; }
+} // namespace host
+} // namespace tdyn
 
 namespace {
 bool finite_all(const double* p, int n) { for (int i = 0; i < n; ++i) if (!std::isfinite(p[i])) return false; return true; }
+} // namespace
 
+namespace tdyn {
+namespace host {
 int validate_tree(const tdyn_tree_desc& d, int t) {
     char buf[256];
     if (d.nb <= 0) { snprintf(buf, sizeof buf, "tree %d: nb must be positive (got %d)", t, d.nb); return fail(TDYN_E_ARG, buf); }
@@ -108,11 +44,69 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     if (!finite_all(d.gravity, 3)) return fail(TDYN_E_ARG, "non-finite gravity");
     return TDYN_OK;
 }
-} // namespace
+
+void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
+    for (int k = 0; k < PRM_STRIDE; ++k) P[k] = 0.0;
+    for (int k = 0; k < 9; ++k) P[PRM_R_PF + k] = d.X_PF[12*(b-1) + k];
+    for (int k = 0; k < 3; ++k) P[PRM_P_PF + k] = d.X_PF[12*(b-1) + 9 + k];
+    const double* Rbm = d.X_BM + 12*(b-1);
+    const double* pbm = Rbm + 9;
+    // X_MB = inverse(X_BM): R_MB = R_BM^T, p_MB = -R_BM^T p_BM
+    for (int r = 0; r < 3; ++r) for (int c2 = 0; c2 < 3; ++c2) P[PRM_R_MB + 3*r + c2] = Rbm[3*c2 + r];
+    for (int r = 0; r < 3; ++r)
+        P[PRM_P_MB + r] = -(P[PRM_R_MB + 3*r]*pbm[0] + P[PRM_R_MB + 3*r + 1]*pbm[1] + P[PRM_R_MB + 3*r + 2]*pbm[2]);
+    const double m = d.mass[b-1];
+    P[PRM_MASS] = m;
+    for (int k = 0; k < 3; ++k) P[PRM_COM + k] = d.com_B[3*(b-1) + k];
+    for (int k = 0; k < 6; ++k) P[PRM_G + k] = d.inertia_B[6*(b-1) + k] / m;
+    for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
+}
+
+int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
+    if (cfg->precision != 0 && cfg->precision != 1)
+        return fail(TDYN_E_ARG, "integrator precision must be 0 (FP64) or 1 (mixed)");
+    precision = cfg->precision;
+    const double acc = cfg->accuracy > 0 ? cfg->accuracy : 1e-3;
+    if (!(acc < 1.0)) return fail(TDYN_E_ARG, "accuracy must be in (0,1)");
+    c.accuracy = acc;
+    c.cons_tol = cfg->constraint_tolerance > 0 ? cfg->constraint_tolerance : acc / 10;
+    c.fixed_h = cfg->fixed_step > 0 ? cfg->fixed_step : 0.0;
+    c.init_h = cfg->initial_step > 0 ? cfg->initial_step : -1.0;
+    c.min_h = cfg->min_step > 0 ? cfg->min_step : -1.0;
+    c.max_h = cfg->max_step > 0 ? cfg->max_step : -1.0;
+    c.timescale = cfg->time_scale > 0 ? cfg->time_scale : 0.1;
+    c.max_attempts = cfg->max_attempts > 0 ? cfg->max_attempts : 100000000;
+    c.max_steps = 0;
+    return TDYN_OK;
+}
+
+int ensure_rollout_buffers(tdyn_batch* b) {
+    const int ny = b->max_nq + b->max_nu;
+    const size_t L = b->n_inst;
+    hipError_t e;
+    if (b->precision == 1) {
+        const size_t n32 = (size_t)(6 * b->max_nu + 4 * b->max_nb) * L;
+        if (b->scratch32.n < n32 && (e = b->scratch32.alloc(n32)) != hipSuccess)
+            return hipfail(e, "mixed-precision scratch allocation");
+    }
+    if (b->rollout_ready && b->ny_max == ny) return TDYN_OK;
+    // 8 extra rows: the chain evaluator prefetches fixed-size q/u windows
+    if ((e = b->vec.alloc((size_t)11 * ny * L + 8 * L)) != hipSuccess ||
+        (e = b->uscale.alloc((size_t)std::max(1, b->max_nu) * L)) != hipSuccess ||
+        (e = b->t.alloc(L)) != hipSuccess || (e = b->h.alloc(L)) != hipSuccess ||
+        (e = b->t_end.alloc(L)) != hipSuccess || (e = b->status.alloc(L)) != hipSuccess ||
+        (e = b->stats.alloc(4 * L)) != hipSuccess)
+        return hipfail(e, "rollout state allocation");
+    b->ny_max = ny;
+    b->rollout_ready = true;
+    return TDYN_OK;
+}
+} // namespace host
+} // namespace tdyn
 
 extern "C" {
 
-const char* tdyn_last_error(void) { return g_err.c_str(); }
+const char* tdyn_last_error(void) { return err_slot().c_str(); }
 
 int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* tree_of_instance,
                       int n_instances, tdyn_batch** out) {
@@ -147,21 +141,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             for (int c : ch[b]) child_idx.push_back(c);
             q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(mob_dof(ty)); gslot += 7 * mob_dof(ty);
             if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
-            // constants
-            double P[PRM_STRIDE] = {0};
-            for (int k = 0; k < 9; ++k) P[PRM_R_PF + k] = d.X_PF[12*(b-1) + k];
-            for (int k = 0; k < 3; ++k) P[PRM_P_PF + k] = d.X_PF[12*(b-1) + 9 + k];
-            const double* Rbm = d.X_BM + 12*(b-1);
-            const double* pbm = Rbm + 9;
-            // X_MB = inverse(X_BM): R_MB = R_BM^T, p_MB = -R_BM^T p_BM
-            for (int r = 0; r < 3; ++r) for (int c2 = 0; c2 < 3; ++c2) P[PRM_R_MB + 3*r + c2] = Rbm[3*c2 + r];
-            for (int r = 0; r < 3; ++r)
-                P[PRM_P_MB + r] = -(P[PRM_R_MB + 3*r]*pbm[0] + P[PRM_R_MB + 3*r + 1]*pbm[1] + P[PRM_R_MB + 3*r + 2]*pbm[2]);
-            const double m = d.mass[b-1];
-            P[PRM_MASS] = m;
-            for (int k = 0; k < 3; ++k) P[PRM_COM + k] = d.com_B[3*(b-1) + k];
-            for (int k = 0; k < 6; ++k) P[PRM_G + k] = d.inertia_B[6*(b-1) + k] / m;
-            for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
+            double P[PRM_STRIDE];
+            pack_body_params(d, b, P);
             prm.insert(prm.end(), P, P + PRM_STRIDE);
         }
         for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
@@ -305,34 +286,11 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
                        const tdyn_integrator_cfg* cfg, hipStream_t s) {
     if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_rollout_begin: batch, q, u and cfg are required");
     if (!t_end && !(t_end_all > t0)) return fail(TDYN_E_ARG, "tdyn_rollout_begin: final time must exceed t0");
-    if (cfg->precision != 0) return fail(TDYN_E_ARG, "tdyn_rollout_begin: only precision 0 (FP64) is available");
-    const double acc = cfg->accuracy > 0 ? cfg->accuracy : 1e-3;
-    if (!(acc < 1.0)) return fail(TDYN_E_ARG, "accuracy must be in (0,1)");
-    RkmCfg c;
-    c.accuracy = acc;
-    c.cons_tol = cfg->constraint_tolerance > 0 ? cfg->constraint_tolerance : acc / 10;
-    c.fixed_h = cfg->fixed_step > 0 ? cfg->fixed_step : 0.0;
-    c.init_h = cfg->initial_step > 0 ? cfg->initial_step : -1.0;
-    c.min_h = cfg->min_step > 0 ? cfg->min_step : -1.0;
-    c.max_h = cfg->max_step > 0 ? cfg->max_step : -1.0;
-    c.timescale = cfg->time_scale > 0 ? cfg->time_scale : 0.1;
-    c.max_attempts = cfg->max_attempts > 0 ? cfg->max_attempts : 100000000;
-    c.max_steps = 0;
-    b->cfg = c;
-    const int ny = b->max_nq + b->max_nu;
+    { int prec = 0; const int rc = make_rkm_cfg(cfg, b->cfg, prec); if (rc) return rc; b->precision = prec; }
+    if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
     const size_t L = b->n_inst;
-    if (!b->rollout_ready || b->ny_max != ny) {
-        hipError_t e;
-        // 8 extra rows: the chain evaluator prefetches fixed-size q/u windows
-        if ((e = b->vec.alloc((size_t)9 * ny * L + 8 * L)) != hipSuccess ||
-            (e = b->uscale.alloc((size_t)std::max(1, b->max_nu) * L)) != hipSuccess ||
-            (e = b->t.alloc(L)) != hipSuccess || (e = b->h.alloc(L)) != hipSuccess ||
-            (e = b->t_end.alloc(L)) != hipSuccess || (e = b->status.alloc(L)) != hipSuccess ||
-            (e = b->stats.alloc(4 * L)) != hipSuccess)
-            return hipfail(e, "tdyn_rollout_begin: allocation");
-        b->ny_max = ny;
-        b->rollout_ready = true;
-    }
+    { const int rc = ensure_rollout_buffers(b); if (rc) return rc; }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));
         HIPCHK(hipMemcpyAsync(b->fapp.p, fapp, (size_t)b->total_b * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -370,6 +328,15 @@ int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* st
     if (t) HIPCHK(hipMemcpyAsync(t, b->t.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (status) HIPCHK(hipMemcpyAsync(status, b->status.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
     if (stats) HIPCHK(hipMemcpyAsync(stats, b->stats.p, 4 * L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    return TDYN_OK;
+}
+
+int tdyn_rollout_derivs(tdyn_batch* b, double* qdot, double* udot, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_derivs: no rollout");
+    if (!qdot || !udot) return fail(TDYN_E_ARG, "tdyn_rollout_derivs: qdot and udot are required");
+    RkmArgs a = b->rk();
+    HIPCHK(launch_rollout_f0(a, s));
+    HIPCHK(launch_pack_state(a, nullptr, nullptr, 0, qdot, udot, s, 2 /* VF0 */));
     return TDYN_OK;
 }
 

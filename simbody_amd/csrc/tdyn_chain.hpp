@@ -1,5 +1,5 @@
 // Serial-chain evaluator of the state derivative (realize(Acceleration)) for
-// batches whose instances all share one chain model (parent(b) = b-1).
+// batches whose instances all share one chain topology (parent(b) = b-1).
 //
 // Same algorithm as the tree sweeps (Simbody's ABA, RigidBodyNodeSpec.cpp:
 // 249-446 and RigidBodyNode.cpp:54-213), organized so that a lane keeps the
@@ -11,12 +11,20 @@
 //          H u - ...), then H, Phi, coriolis a, gyroscopic b, M_k, gravity,
 //          the articulated inertia P, D^-1, G, P+, z, eps, z+ -- the child's
 //          P+, z+ and Phi arrive in registers from the previous iteration.
-//          Only G and DI*eps (7 dof doubles) are written per body.
+//          Only G and DI*eps (7 dof values) are written per body.
 //   pass 3 (base->tip): kinematics recomputed forward, udot = DI eps - G^T A+,
 //          A_GB = A+ + H udot + a.
-// The model is uniform across the batch, so its constants are read with
-// wave-uniform (scalar) loads and the mobilizer switch is a uniform branch.
+//
+// Precision.  K is the scalar of the kinematic recursion (X_GB, V_GB, H,
+// HDot terms), the bias forces (coriolis a, gyroscopic b, gravity, z, z+)
+// and the articulated inertias (P, P+, G).  K = double is the FP64 path.
+// K = float is the mixed path; in it D = H^T P H is formed with FP64
+// products and sums and inverted in FP64, G = P H D^-1 is formed in FP64
+// and rounded once, DI*eps and udot are FP64, quaternion normalization is
+// FP64, and the state and its Runge-Kutta combinations stay FP64 (the step
+// update y1 = y0 + dy is Kahan-compensated across steps).
 #pragma once
+#include <type_traits>
 #include "tdyn_internal.hpp"
 #include "tdyn_math.hpp"
 
@@ -34,10 +42,37 @@ template <int TYPE> struct MobC {
     static constexpr int NQ = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 4 : 7;
 };
 
-__device__ __forceinline__ dm ldm(cdptr p) { dm r; for (int i = 0; i < 9; ++i) r.a[i] = p[i]; return r; }
-__device__ __forceinline__ d3 ld3(cdptr p) { return {p[0], p[1], p[2]}; }
-__device__ __forceinline__ dm mulABt(const dm& A, const dm& B) {   // A B^T
-    dm C;
+#define TD_GLOBAL __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ TD_GLOBAL T* uniform_ptr(T* p);
+
+// Conversions between the two scalars (identity when K = double).
+template <class K> __device__ __forceinline__ v3<K> tok(const d3& a) { return {K(a.x), K(a.y), K(a.z)}; }
+template <class K> __device__ __forceinline__ sv<K> tok(const dv& a) { return {tok<K>(a.w), tok<K>(a.v)}; }
+template <class K> __device__ __forceinline__ d3 tod(const v3<K>& a) { return {double(a.x), double(a.y), double(a.z)}; }
+template <class K> __device__ __forceinline__ dv tod(const sv<K>& a) { return {tod(a.w), tod(a.v)}; }
+// FP64 dot product of two K spatial vectors (the plain dot when K = double).
+template <class K> __device__ __forceinline__ double dotd(const sv<K>& a, const sv<K>& b) {
+    if constexpr (std::is_same<K, double>::value) return dot(a, b);
+    else return dot(tod(a), tod(b));
+}
+
+// Model constants of the chain, PRM_STRIDE doubles per body.  Two access
+// forms: one model for the whole batch (a constant-address-space pointer, so
+// loads are wave-uniform scalar loads), or one model per lane (serving slots:
+// structure-of-arrays rows, row k of this lane at p[k * L]).
+struct PrmL {
+    const double* p; int L; unsigned lane;     // p: row 0 (the same in every lane)
+    __device__ __forceinline__ double operator[](int k) const {
+        return *(const TD_GLOBAL double*)((const TD_GLOBAL char*)uniform_ptr(p + (size_t)k * L) + lane * 8u);
+    }
+    __device__ __forceinline__ PrmL operator+(size_t k) const { return {p + k * (size_t)L, L, lane}; }
+};
+template <class K, class PA> __device__ __forceinline__ m33<K> ldm(const PA& p) {
+    m33<K> r; for (int i = 0; i < 9; ++i) r.a[i] = K(p[i]); return r;
+}
+template <class K, class PA> __device__ __forceinline__ v3<K> ld3(const PA& p) { return {K(p[0]), K(p[1]), K(p[2])}; }
+template <class K> __device__ __forceinline__ m33<K> mulABt(const m33<K>& A, const m33<K>& B) {   // A B^T
+    m33<K> C;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -45,68 +80,81 @@ __device__ __forceinline__ dm mulABt(const dm& A, const dm& B) {   // A B^T
     return C;
 }
 
+// Angle sin/cos: FP64 for the FP64 path; for K = float the FP64 angle is
+// first reduced to [-pi, pi] in FP64 and the FP32 functions are applied.
+template <class K> __device__ __forceinline__ void sincosk(double q, K* s, K* c) {
+    if constexpr (std::is_same<K, double>::value) sincos(q, s, c);
+    else {
+        const double r = q - 6.283185307179586 * rint(q * 0.15915494309189535);
+        sincosf(float(r), s, c);
+    }
+}
+
 // Mobilizer part: X_FM(q) and H_FM.
-template <int TYPE> struct Joint {
+template <int TYPE, class K> struct Joint {
     static constexpr int D = MobC<TYPE>::D;
-    dm R_FM; d3 p_FM; dv H_FM[D];
+    m33<K> R_FM; v3<K> p_FM; sv<K> H_FM[D];
     // tc: cached trig / normalization values of this q (written by the first
     // sweep with CACHED = false): Pin (cos,sin), Universal (c1,s1,c2,s2),
-    // Ball/Free 1/|quaternion|.
+    // Ball/Free 1/|quaternion| (computed in FP64).
     template <bool CACHED>
-    __device__ __forceinline__ void set(const double* qv, double* tc) {
-        p_FM = zero3<double>();
+    __device__ __forceinline__ void set(const double* qv, K* tc) {
+        const K z = K(0), o = K(1);
+        p_FM = zero3<K>();
         if constexpr (TYPE == PIN) {
-            double sn, cs;
-            if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincos(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
-            R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=0; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=0;
-            R_FM.a[6]=0;  R_FM.a[7]=0;   R_FM.a[8]=1;
-            H_FM[0] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+            K sn, cs;
+            if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
+            R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
+            R_FM.a[6]=z;  R_FM.a[7]=z;   R_FM.a[8]=o;
+            H_FM[0] = {V3(z, z, o), zero3<K>()};
         } else if constexpr (TYPE == SLIDER) {
-            R_FM = ident<double>(); p_FM = V3(qv[0], 0.0, 0.0);
-            H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+            R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
+            H_FM[0] = {zero3<K>(), V3(o, z, z)};
         } else if constexpr (TYPE == UNIVERSAL) {
-            double s1, c1, s2, c2;
+            K s1, c1, s2, c2;
             if constexpr (CACHED) { c1 = tc[0]; s1 = tc[1]; c2 = tc[2]; s2 = tc[3]; }
-            else { sincos(qv[0], &s1, &c1); sincos(qv[1], &s2, &c2); tc[0] = c1; tc[1] = s1; tc[2] = c2; tc[3] = s2; }
-            R_FM.a[0]= c2;    R_FM.a[1]= 0.0; R_FM.a[2]= s2;
-            R_FM.a[3]= s2*s1; R_FM.a[4]= c1;  R_FM.a[5]= -s1*c2;
-            R_FM.a[6]=-s2*c1; R_FM.a[7]= s1;  R_FM.a[8]= c1*c2;
-            H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
-            H_FM[1] = {col(R_FM, 1), zero3<double>()};
+            else { sincosk<K>(qv[0], &s1, &c1); sincosk<K>(qv[1], &s2, &c2); tc[0] = c1; tc[1] = s1; tc[2] = c2; tc[3] = s2; }
+            R_FM.a[0]= c2;    R_FM.a[1]= z;  R_FM.a[2]= s2;
+            R_FM.a[3]= s2*s1; R_FM.a[4]= c1; R_FM.a[5]= -s1*c2;
+            R_FM.a[6]=-s2*c1; R_FM.a[7]= s1; R_FM.a[8]= c1*c2;
+            H_FM[0] = {V3(o, z, z), zero3<K>()};
+            H_FM[1] = {col(R_FM, 1), zero3<K>()};
         } else {
             double oo;
-            if constexpr (CACHED) oo = tc[0];
-            else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = oo; }
-            R_FM = quat_rot(qv[0]*oo, qv[1]*oo, qv[2]*oo, qv[3]*oo);
-            H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
-            H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
-            H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+            if constexpr (CACHED) oo = double(tc[0]);
+            else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = K(oo); }
+            R_FM = quat_rot(K(qv[0]*oo), K(qv[1]*oo), K(qv[2]*oo), K(qv[3]*oo));
+            H_FM[0] = {V3(o, z, z), zero3<K>()};
+            H_FM[1] = {V3(z, o, z), zero3<K>()};
+            H_FM[2] = {V3(z, z, o), zero3<K>()};
             if constexpr (TYPE == FREE) {
-                p_FM = V3(qv[4], qv[5], qv[6]);
-                H_FM[3] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
-                H_FM[4] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
-                H_FM[5] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
+                p_FM = V3(K(qv[4]), K(qv[5]), K(qv[6]));
+                H_FM[3] = {zero3<K>(), V3(o, z, z)};
+                H_FM[4] = {zero3<K>(), V3(z, o, z)};
+                H_FM[5] = {zero3<K>(), V3(z, z, o)};
             }
         }
     }
 };
 
 // X_PB = X_PF X_FM X_MB (RigidBodyNodeSpec.h:554-569)
-__device__ __forceinline__ void body_in_parent(cdptr P, const dm& R_FM, const d3& p_FM, dm& R_PB, d3& p_PB) {
-    const dm R_PF = ldm(P + PRM_R_PF), R_MB = ldm(P + PRM_R_MB);
-    const d3 p_PF = ld3(P + PRM_P_PF), p_MB = ld3(P + PRM_P_MB);
-    const dm R_FB = mul(R_FM, R_MB);
-    const d3 p_FB = p_FM + mul(R_FM, p_MB);
+template <class K, class PA>
+__device__ __forceinline__ void body_in_parent(const PA& P, const m33<K>& R_FM, const v3<K>& p_FM, m33<K>& R_PB,
+                                               v3<K>& p_PB) {
+    const m33<K> R_PF = ldm<K>(P + PRM_R_PF), R_MB = ldm<K>(P + PRM_R_MB);
+    const v3<K> p_PF = ld3<K>(P + PRM_P_PF), p_MB = ld3<K>(P + PRM_P_MB);
+    const m33<K> R_FB = mul(R_FM, R_MB);
+    const v3<K> p_FB = p_FM + mul(R_FM, p_MB);
     R_PB = mul(R_PF, R_FB);
     p_PB = p_PF + mul(R_PF, p_FB);
 }
 
 // H_PB_G = R_GF (H_FM + H_MB_F)  (RigidBodyNodeSpec.cpp:44-74)
-template <int TYPE>
-__device__ __forceinline__ void hinge(cdptr P, const Joint<TYPE>& J, const dm& R_GP, dm& R_GF, d3& r_MB_F,
-                                      dv* H) {
-    R_GF = mul(R_GP, ldm(P + PRM_R_PF));
-    r_MB_F = mul(J.R_FM, ld3(P + PRM_P_MB));
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ void hinge(const PA& P, const Joint<TYPE, K>& J, const m33<K>& R_GP, m33<K>& R_GF,
+                                      v3<K>& r_MB_F, sv<K>* H) {
+    R_GF = mul(R_GP, ldm<K>(P + PRM_R_PF));
+    r_MB_F = mul(J.R_FM, ld3<K>(P + PRM_P_MB));
 #pragma unroll
     for (int j = 0; j < MobC<TYPE>::D; ++j)
         H[j] = {mul(R_GF, J.H_FM[j].w), mul(R_GF, J.H_FM[j].v - cross(r_MB_F, J.H_FM[j].w))};
@@ -114,37 +162,72 @@ __device__ __forceinline__ void hinge(cdptr P, const Joint<TYPE>& J, const dm& R
 
 // Mobilizer coriolis acceleration a = (HDot u) + (0, w_GP x (v_GB - v_GP))
 // (RigidBodyNodeSpec.cpp:82-129, RigidBodyNode.cpp:98-174).
-template <int TYPE>
-__device__ __forceinline__ dv coriolis(const Joint<TYPE>& J, const dm& R_GF, const d3& r_MB_F, const dv* H,
-                                       const double* uv, const dv& V_GP, const dv& V_GB) {
+template <int TYPE, class K>
+__device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>& R_GF, const v3<K>& r_MB_F,
+                                          const sv<K>* H, const double* uv, const sv<K>& V_GP, const sv<K>& V_GB) {
     constexpr int D = MobC<TYPE>::D;
-    dv V_FM = svzero<double>();
+    sv<K> V_FM = svzero<K>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) V_FM = V_FM + uv[j]*J.H_FM[j];
-    const d3 w_FM = V_FM.w;
-    const d3 wxr = cross(w_FM, r_MB_F);
-    dv VD = svzero<double>();
+    for (int j = 0; j < D; ++j) V_FM = V_FM + K(uv[j])*J.H_FM[j];
+    const v3<K> w_FM = V_FM.w;
+    const v3<K> wxr = cross(w_FM, r_MB_F);
+    sv<K> VD = svzero<K>();
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        d3 hdw = zero3<double>();
+        v3<K> hdw = zero3<K>();
         if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(J.R_FM, 1)); }
-        const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
-        const dv HD = {mul(R_GF, hdw) + cross(V_GP.w, H[j].w), mul(R_GF, lin) + cross(V_GP.w, H[j].v)};
-        VD = VD + uv[j]*HD;
+        const v3<K> lin = (zero3<K>() - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
+        const sv<K> HD = {mul(R_GF, hdw) + cross(V_GP.w, H[j].w), mul(R_GF, lin) + cross(V_GP.w, H[j].v)};
+        VD = VD + K(uv[j])*HD;
     }
     return {VD.w, VD.v + cross(V_GP.w, V_GB.v - V_GP.v)};
 }
 
-// Per-lane SoA rows: row k of this lane at base[k * L] (base = array + lane).
-struct Rows {
-    double* p; int L;
-    __device__ __forceinline__ double& operator()(int k) const { return p[(size_t)k * L]; }
+// A pointer that is the same in every active lane, marked as such: the
+// compiler then keeps it in scalar registers and addresses per-lane
+// elements with a 32-bit lane offset.
+template <class T> __device__ __forceinline__ TD_GLOBAL T* uniform_ptr(T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (TD_GLOBAL T*)(((uint64_t)hi << 32) | lo);
+}
+// Per-lane SoA rows: row k of this lane at base[k * L + lane].  U: base and
+// k are the same in every active lane (true for all rows of the chain
+// evaluator, whose topology is shared by the batch), so the row address
+// base + k*L is a scalar and the lane part a 32-bit offset.
+template <class T, bool U = true> struct RowsT {
+    T* base; int L; unsigned lane;
+    __device__ __forceinline__ TD_GLOBAL T& operator()(int k) const {
+        T* rb = base + (size_t)k * L;
+        if constexpr (U) {
+            // 32-bit byte offset: the access becomes scalar base + lane offset
+            const uint32_t off = lane * (uint32_t)sizeof(T);
+            return *(TD_GLOBAL T*)((TD_GLOBAL char*)uniform_ptr(rb) + off);
+        } else {
+            return ((TD_GLOBAL T*)rb)[lane];
+        }
+    }
+    __device__ __forceinline__ RowsT shifted(int k) const { return {base + (size_t)k * L, L, lane}; }
+};
+using Rows = RowsT<double>;
+
+// Where one body's sweep hand-over lives: G (6 dof values of type K), the
+// trig cache (K) and DI*eps (dof doubles).  FP64 path: G and DI*eps at the
+// body's G slot of the double scratch, trig cache at its own double rows.
+// Mixed path: G and the trig cache in float rows (G at 6*uIndex, trig cache
+// after all G rows), DI*eps at the double G slot.
+template <class K> struct KStore {
+    RowsT<K> k; Rows d; int g, tc, de;
+    __device__ __forceinline__ TD_GLOBAL K& G(int i) const { return k(g + i); }
+    __device__ __forceinline__ TD_GLOBAL K& T(int i) const { return k(tc + i); }
+    __device__ __forceinline__ TD_GLOBAL double& DE(int i) const { return d(de + i); }
 };
 
 // Everything one body's step needs from memory, loaded one body ahead so the
 // loads are in flight while the previous body computes.
-struct Pre {
-    double q[7], u[6], tc[4], g[21];
+template <class K> struct Pre {
+    double q[7], u[6]; K tc[4]; K g[18]; double de[3];
 };
 // Raw Runge-Kutta sources of one body's stage input: y0, f0, fa, fb per
 // q then u element.
@@ -157,16 +240,17 @@ struct RkRows {
     Rows st;               // stage input (ytmp for stages 1-3, ysave for stage 4)
     Rows out;              // derivative output of this stage
     Rows y1, ye;           // stage 4: y1 and its error estimate
+    Rows c0, c1;           // mixed path: Kahan compensation of y0 and of y1
 };
 // Error-norm partial sums (IntegratorRep.h:454-523): weighted q part of the
 // quaternions and of the other q, relative u part, quaternion norm errors.
 struct NormSums { double sq_other, sq_quat, su, s2; };
 // Type-exact prefetch: NQ q, D u, the trig cache entries the mobilizer uses
-// and (pass 3) its 7*D G/DI*eps doubles.
+// and (pass 3) its G and DI*eps values.
 template <int TYPE> struct TcN { static constexpr int n = TYPE == PIN ? 2 : TYPE == UNIVERSAL ? 4 : TYPE == SLIDER ? 0 : 1; };
-template <int TYPE, bool G>
-__device__ __forceinline__ void prefetch_t(Pre& x, const Rows& y, int nq, int qi, int ui, const Rows& sc, int tcoff,
-                                           int gs, bool tc) {
+template <int TYPE, bool G, class K>
+__device__ __forceinline__ void prefetch_t(Pre<K>& x, const Rows& y, int nq, int qi, int ui, const KStore<K>& st,
+                                           bool tc) {
     constexpr int NQ = MobC<TYPE>::NQ, D = MobC<TYPE>::D;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) x.q[k] = y(qi + k);
@@ -174,31 +258,36 @@ __device__ __forceinline__ void prefetch_t(Pre& x, const Rows& y, int nq, int qi
     for (int k = 0; k < D; ++k) x.u[k] = y(nq + ui + k);
     if (tc) {
 #pragma unroll
-        for (int k = 0; k < TcN<TYPE>::n; ++k) x.tc[k] = sc(tcoff + k);
+        for (int k = 0; k < TcN<TYPE>::n; ++k) x.tc[k] = st.T(k);
     }
     if constexpr (G && D <= 3) {
 #pragma unroll
-        for (int k = 0; k < 7*D; ++k) x.g[k] = sc(gs + k);
+        for (int k = 0; k < 6*D; ++k) x.g[k] = st.G(k);
+#pragma unroll
+        for (int k = 0; k < D; ++k) x.de[k] = st.DE(k);
     }
 }
-template <bool G>
-__device__ __forceinline__ void prefetch(Pre& x, const Rows& y, int nq, int4 ba, const Rows& sc, int tcoff, int gs,
-                                         bool tc) {
+// AF: the body may be a Free root.  Only body 1 can be Free (the host rejects
+// Free elsewhere), and it is peeled out of every loop, so the loops prefetch
+// with AF = false and never hold a 7-q/6-u body's registers.
+template <bool G, bool AF, class K>
+__device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 ba, const KStore<K>& st, bool tc) {
     switch (ba.y) {
-    case PIN:       prefetch_t<PIN, G>(x, y, nq, ba.z, ba.w, sc, tcoff, gs, tc); break;
-    case SLIDER:    prefetch_t<SLIDER, G>(x, y, nq, ba.z, ba.w, sc, tcoff, gs, tc); break;
-    case UNIVERSAL: prefetch_t<UNIVERSAL, G>(x, y, nq, ba.z, ba.w, sc, tcoff, gs, tc); break;
-    case BALL:      prefetch_t<BALL, G>(x, y, nq, ba.z, ba.w, sc, tcoff, gs, tc); break;
-    default:        prefetch_t<FREE, G>(x, y, nq, ba.z, ba.w, sc, tcoff, gs, tc); break;
+    case PIN:       prefetch_t<PIN, G>(x, y, nq, ba.z, ba.w, st, tc); break;
+    case SLIDER:    prefetch_t<SLIDER, G>(x, y, nq, ba.z, ba.w, st, tc); break;
+    case UNIVERSAL: prefetch_t<UNIVERSAL, G>(x, y, nq, ba.z, ba.w, st, tc); break;
+    default:
+        if constexpr (AF) { if (ba.y == FREE) { prefetch_t<FREE, G>(x, y, nq, ba.z, ba.w, st, tc); break; } }
+        prefetch_t<BALL, G>(x, y, nq, ba.z, ba.w, st, tc); break;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Pass 1 body: forward transforms and velocities, qdot out, trig cache out.
 // ---------------------------------------------------------------------------
-template <int TYPE>
-__device__ __forceinline__ void pass1_body(cdptr P, const Pre& x, int qi, const Rows& f, const Rows& sc,
-                                           int tcoff, dm& R, d3& p, dv& V) {
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi, const Rows& f, const KStore<K>& st,
+                                           m33<K>& R, v3<K>& p, sv<K>& V) {
     constexpr int D = MobC<TYPE>::D;
     const double* qv = x.q; const double* uv = x.u;
     if constexpr (TYPE == BALL || TYPE == FREE) {
@@ -210,27 +299,27 @@ __device__ __forceinline__ void pass1_body(cdptr P, const Pre& x, int qi, const 
 #pragma unroll
         for (int k = 0; k < D; ++k) f(qi + k) = uv[k];
     }
-    Joint<TYPE> J;
-    double tc[4];
+    Joint<TYPE, K> J;
+    K tc[4];
     J.template set<false>(qv, tc);
-    if constexpr (TYPE == PIN) { sc(tcoff) = tc[0]; sc(tcoff+1) = tc[1]; }
-    else if constexpr (TYPE == UNIVERSAL) { sc(tcoff) = tc[0]; sc(tcoff+1) = tc[1]; sc(tcoff+2) = tc[2]; sc(tcoff+3) = tc[3]; }
-    else if constexpr (TYPE == BALL || TYPE == FREE) { sc(tcoff) = tc[0]; }
-    dm R_PB; d3 p_PB;
-    body_in_parent(P, J.R_FM, J.p_FM, R_PB, p_PB);
-    dm R_GF; d3 r_MB_F; dv H[D];
-    hinge<TYPE>(P, J, R, R_GF, r_MB_F, H);
-    const d3 phi = mul(R, p_PB);
-    dv Vpb = svzero<double>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) Vpb = Vpb + uv[j]*H[j];
+    for (int k = 0; k < TcN<TYPE>::n; ++k) st.T(k) = tc[k];
+    m33<K> R_PB; v3<K> p_PB;
+    body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
+    m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];
+    hinge<TYPE, K>(P, J, R, R_GF, r_MB_F, H);
+    const v3<K> phi = mul(R, p_PB);
+    sv<K> Vpb = svzero<K>();
+#pragma unroll
+    for (int j = 0; j < D; ++j) Vpb = Vpb + K(uv[j])*H[j];
     V = phiT_mul(phi, V) + Vpb;
     p = p + mul(R, p_PB);
     R = mul(R, R_PB);
 }
 
-// Small dense inverse of D: cofactor forms for dof<=3 (SmallMatrixMixed.h:
-// 895-960), Gauss-Jordan without pivoting for dof=6 (D is SPD).
+// Small dense inverse of D (FP64 on both paths): cofactor forms for dof<=3
+// (SmallMatrixMixed.h:895-960), Gauss-Jordan without pivoting for dof=6 (D is
+// symmetric positive definite).
 template <int D> __device__ __forceinline__ void dinv(const double* Dm, double* DI) {
     if constexpr (D == 1) DI[0] = 1.0 / Dm[0];
     else if constexpr (D == 2) {
@@ -270,97 +359,98 @@ template <int D> __device__ __forceinline__ void dinv(const double* Dm, double* 
 // In: R, p, V of this body; child's PP, zp, phic (hasChild).  Out: R, p, V
 // of the parent; PP, zp, phic of this body for the parent.
 // ---------------------------------------------------------------------------
-template <int TYPE>
-__device__ __forceinline__ void pass2_body(cdptr P, const Pre& x, const d3& grav, const dv& Fapp,
-                                           const Rows& sc, int gs, dm& R, d3& p, dv& V, bool hasChild,
-                                           da& PP, dv& zp, d3& phic) {
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v3<K>& grav, const sv<K>& Fapp,
+                                           const KStore<K>& st, m33<K>& R, v3<K>& p, sv<K>& V, bool hasChild,
+                                           abi<K>& PP, sv<K>& zp, v3<K>& phic) {
     constexpr int D = MobC<TYPE>::D;
     const double* uv = x.u;
-    Joint<TYPE> J;
-    double tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
+    Joint<TYPE, K> J;
+    K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true>(x.q, tc);
-    dm R_PB; d3 p_PB;
-    body_in_parent(P, J.R_FM, J.p_FM, R_PB, p_PB);
-    const dm R_GB = R; const d3 p_GB = p; const dv V_GB = V;
-    const dm R_GP = mulABt(R_GB, R_PB);
-    const d3 phi = mul(R_GP, p_PB);
-    const d3 p_GP = p_GB - phi;
-    dm R_GF; d3 r_MB_F; dv H[D];
-    hinge<TYPE>(P, J, R_GP, R_GF, r_MB_F, H);
-    dv Vpb = svzero<double>();
+    m33<K> R_PB; v3<K> p_PB;
+    body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
+    const m33<K> R_GB = R; const v3<K> p_GB = p; const sv<K> V_GB = V;
+    const m33<K> R_GP = mulABt(R_GB, R_PB);
+    const v3<K> phi = mul(R_GP, p_PB);
+    const v3<K> p_GP = p_GB - phi;
+    m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];
+    hinge<TYPE, K>(P, J, R_GP, R_GF, r_MB_F, H);
+    sv<K> Vpb = svzero<K>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) Vpb = Vpb + uv[j]*H[j];
-    const d3 w_GP = V_GB.w - Vpb.w;
-    const d3 v_GP = (V_GB.v - Vpb.v) - cross(w_GP, phi);
-    const dv V_GP = {w_GP, v_GP};
-    const dv a = coriolis<TYPE>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
-    const double m = P[PRM_MASS];
-    const d3 Mk_p = mul(R_GB, ld3(P + PRM_COM));
-    const ds Gu = {P[PRM_G], P[PRM_G+1], P[PRM_G+2], P[PRM_G+3], P[PRM_G+4], P[PRM_G+5]};
-    const ds Mk_G = rotate_sym(R_GB, Gu);
-    const d3 w = V_GB.w;
-    const dv bg = {m*cross(w, mul(Mk_G, w)), m*cross(w, cross(w, Mk_p))};
-    const d3 fg = m*grav;
-    const dv F = dv{cross(Mk_p, fg), fg} + Fapp;
-    da Pb = abi_from_rbi(m, Mk_p, Mk_G);
+    for (int j = 0; j < D; ++j) Vpb = Vpb + K(uv[j])*H[j];
+    const v3<K> w_GP = V_GB.w - Vpb.w;
+    const v3<K> v_GP = (V_GB.v - Vpb.v) - cross(w_GP, phi);
+    const sv<K> V_GP = {w_GP, v_GP};
+    const sv<K> a = coriolis<TYPE, K>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const K m = K(P[PRM_MASS]);
+    const v3<K> Mk_p = mul(R_GB, ld3<K>(P + PRM_COM));
+    const s33<K> Gu = {K(P[PRM_G]), K(P[PRM_G+1]), K(P[PRM_G+2]), K(P[PRM_G+3]), K(P[PRM_G+4]), K(P[PRM_G+5])};
+    const s33<K> Mk_G = rotate_sym(R_GB, Gu);
+    const v3<K> w = V_GB.w;
+    const sv<K> bg = {m*cross(w, mul(Mk_G, w)), m*cross(w, cross(w, Mk_p))};
+    const v3<K> fg = m*grav;
+    const sv<K> F = sv<K>{cross(Mk_p, fg), fg} + Fapp;
+    abi<K> Pb = abi_from_rbi(m, Mk_p, Mk_G);
     if (hasChild) abi_add_shifted(Pb, PP, phic);
-    dv PH[D];
+    sv<K> PH[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) PH[j] = abi_mul(Pb, H[j]);
     double Dm[D*D], DI[D*D];
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-        for (int k = 0; k < D; ++k) Dm[r*D+k] = dot(H[r], PH[k]);
+        for (int k = 0; k < D; ++k) Dm[r*D+k] = dotd(H[r], PH[k]);
     dinv<D>(Dm, DI);
-    dv G[D];
+    sv<K> G[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         dv acc = svzero<double>();
 #pragma unroll
-        for (int j = 0; j < D; ++j) acc = acc + DI[j*D+k]*PH[j];
-        G[k] = acc;
+        for (int j = 0; j < D; ++j) acc = acc + DI[j*D+k]*tod(PH[j]);
+        G[k] = tok<K>(acc);
     }
-    dv z = (abi_mul(Pb, a) + bg) - F;
+    sv<K> z = (abi_mul(Pb, a) + bg) - F;
     if (hasChild) z = z + phi_mul(phic, zp);
     double eps[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) eps[j] = 0.0 - dot(H[j], z);
-    dv zn = z;
+    for (int j = 0; j < D; ++j) eps[j] = 0.0 - dotd(H[j], z);
+    sv<K> zn = z;
 #pragma unroll
-    for (int j = 0; j < D; ++j) zn = zn + eps[j]*G[j];
-    double mm[9], ms[9], in[9];
+    for (int j = 0; j < D; ++j) zn = zn + K(eps[j])*G[j];
+    K mm[9], ms[9], in[9];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            double a0 = 0, a1 = 0, a2 = 0;
+            K a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
             for (int j = 0; j < D; ++j) {
-                const double gw = r == 0 ? G[j].w.x : r == 1 ? G[j].w.y : G[j].w.z;
-                const double gv = r == 0 ? G[j].v.x : r == 1 ? G[j].v.y : G[j].v.z;
-                const double pv = k == 0 ? PH[j].v.x : k == 1 ? PH[j].v.y : PH[j].v.z;
-                const double pw = k == 0 ? PH[j].w.x : k == 1 ? PH[j].w.y : PH[j].w.z;
+                const K gw = r == 0 ? G[j].w.x : r == 1 ? G[j].w.y : G[j].w.z;
+                const K gv = r == 0 ? G[j].v.x : r == 1 ? G[j].v.y : G[j].v.z;
+                const K pv = k == 0 ? PH[j].v.x : k == 1 ? PH[j].v.y : PH[j].v.z;
+                const K pw = k == 0 ? PH[j].w.x : k == 1 ? PH[j].w.y : PH[j].w.z;
                 a0 += gw*pv; a1 += gv*pv; a2 += gw*pw;
             }
             mm[r*3+k] = a0; ms[r*3+k] = a1; in[r*3+k] = a2;
         }
-    PP.M = Pb.M - ds{ms[0], ms[4], ms[8], (ms[3]+ms[1])*0.5, (ms[6]+ms[2])*0.5, (ms[7]+ms[5])*0.5};
-    PP.J = Pb.J - ds{in[0], in[4], in[8], (in[3]+in[1])*0.5, (in[6]+in[2])*0.5, (in[7]+in[5])*0.5};
+    const K hf = K(0.5);
+    PP.M = Pb.M - s33<K>{ms[0], ms[4], ms[8], (ms[3]+ms[1])*hf, (ms[6]+ms[2])*hf, (ms[7]+ms[5])*hf};
+    PP.J = Pb.J - s33<K>{in[0], in[4], in[8], (in[3]+in[1])*hf, (in[6]+in[2])*hf, (in[7]+in[5])*hf};
 #pragma unroll
     for (int i = 0; i < 9; ++i) PP.F.a[i] = Pb.F.a[i] - mm[i];
     zp = zn; phic = phi;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-        sc(gs + 6*j + 0) = G[j].w.x; sc(gs + 6*j + 1) = G[j].w.y; sc(gs + 6*j + 2) = G[j].w.z;
-        sc(gs + 6*j + 3) = G[j].v.x; sc(gs + 6*j + 4) = G[j].v.y; sc(gs + 6*j + 5) = G[j].v.z;
+        st.G(6*j + 0) = G[j].w.x; st.G(6*j + 1) = G[j].w.y; st.G(6*j + 2) = G[j].w.z;
+        st.G(6*j + 3) = G[j].v.x; st.G(6*j + 4) = G[j].v.y; st.G(6*j + 5) = G[j].v.z;
     }
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         double sacc = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) sacc += DI[r*D+k]*eps[k];
-        sc(gs + 6*D + r) = sacc;
+        st.DE(r) = sacc;
     }
     R = R_GP; p = p_GP; V = V_GP;
 }
@@ -368,42 +458,41 @@ __device__ __forceinline__ void pass2_body(cdptr P, const Pre& x, const d3& grav
 // ---------------------------------------------------------------------------
 // Pass 3 body: forward kinematics again, then udot and A_GB.
 // ---------------------------------------------------------------------------
-template <int TYPE>
-__device__ __forceinline__ void pass3_body(cdptr P, const Pre& x, int ui, const Rows& fu, const Rows& sc,
-                                           int gs, dm& R, d3& p, dv& V, dv& A, double* ud_out = nullptr) {
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui, const Rows& fu, const KStore<K>& st,
+                                           m33<K>& R, v3<K>& p, sv<K>& V, sv<K>& A, double* ud_out = nullptr) {
     constexpr int D = MobC<TYPE>::D;
     const double* uv = x.u;
-    Joint<TYPE> J;
-    double tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
+    Joint<TYPE, K> J;
+    K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true>(x.q, tc);
-    dm R_PB; d3 p_PB;
-    body_in_parent(P, J.R_FM, J.p_FM, R_PB, p_PB);
-    dm R_GF; d3 r_MB_F; dv H[D];
-    hinge<TYPE>(P, J, R, R_GF, r_MB_F, H);
-    const d3 phi = mul(R, p_PB);
-    dv Vpb = svzero<double>();
+    m33<K> R_PB; v3<K> p_PB;
+    body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
+    m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];
+    hinge<TYPE, K>(P, J, R, R_GF, r_MB_F, H);
+    const v3<K> phi = mul(R, p_PB);
+    sv<K> Vpb = svzero<K>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) Vpb = Vpb + uv[j]*H[j];
-    const dv V_GP = V;
-    const dv V_GB = phiT_mul(phi, V_GP) + Vpb;
-    const dv a = coriolis<TYPE>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
-    const dv Ap = phiT_mul(phi, A);
+    for (int j = 0; j < D; ++j) Vpb = Vpb + K(uv[j])*H[j];
+    const sv<K> V_GP = V;
+    const sv<K> V_GB = phiT_mul(phi, V_GP) + Vpb;
+    const sv<K> a = coriolis<TYPE, K>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const sv<K> Ap = phiT_mul(phi, A);
     double ud[D];
 #pragma unroll
     for (int r = 0; r < D; ++r) {
-        dv G;
+        sv<K> G;
         if constexpr (D <= 3) {
             G = {V3(x.g[6*r], x.g[6*r+1], x.g[6*r+2]), V3(x.g[6*r+3], x.g[6*r+4], x.g[6*r+5])};
-            ud[r] = x.g[6*D + r] - dot(G, Ap);
+            ud[r] = x.de[r] - dotd(G, Ap);
         } else {
-            G = {V3(sc(gs + 6*r), sc(gs + 6*r + 1), sc(gs + 6*r + 2)),
-                 V3(sc(gs + 6*r + 3), sc(gs + 6*r + 4), sc(gs + 6*r + 5))};
-            ud[r] = sc(gs + 6*D + r) - dot(G, Ap);
+            G = {V3(st.G(6*r), st.G(6*r + 1), st.G(6*r + 2)), V3(st.G(6*r + 3), st.G(6*r + 4), st.G(6*r + 5))};
+            ud[r] = st.DE(r) - dotd(G, Ap);
         }
     }
-    dv Hu = svzero<double>();
+    sv<K> Hu = svzero<K>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) Hu = Hu + ud[j]*H[j];
+    for (int j = 0; j < D; ++j) Hu = Hu + K(ud[j])*H[j];
     A = (Ap + Hu) + a;
 #pragma unroll
     for (int j = 0; j < D; ++j) { fu(ui + j) = ud[j]; if (ud_out) ud_out[j] = ud[j]; }
@@ -413,105 +502,47 @@ __device__ __forceinline__ void pass3_body(cdptr P, const Pre& x, int ui, const 
 }
 
 // ---------------------------------------------------------------------------
-// The evaluator.  y, f: this lane's rows of the stage input and output
-// vectors (q rows then u rows); sc: this lane's scratch rows (G records at
-// body_b.w, trig cache at tree_b.z + 4(b-1)).
+// The evaluator.  sc: this lane's double scratch rows (G records at
+// body_b.w, FP64 trig cache at tree_b.z + 4(b-1)); sk: the rows of the K
+// hand-over values (the double scratch for K = double, the float scratch of
+// the mixed path otherwise).
 // ---------------------------------------------------------------------------
+template <class PA, class K>
 struct ChainEval {
-    cdptr prm; const int4* body_a; const int4* body_b;
-    int nb, nq, tcbase;
-    Rows sc; d3 grav; const double* fapp;
+    static constexpr bool MIXED = !std::is_same<K, double>::value;
+    PA prm; const int4* body_a; const int4* body_b;
+    int nb, nq, nu, tcbase;
+    Rows sc; RowsT<K> sk; v3<K> grav; const double* fapp;
 
     __device__ ChainEval(const RkmArgs& a, int lane, int tr) {
         const int4 ta = ldi4(a.T.tree_a, tr);
-        nb = ta.x; nq = ta.y;
-        prm = cptr(a.T.prm) + (size_t)ta.w * PRM_STRIDE;
+        nb = ta.x; nq = ta.y; nu = ta.z;
         body_a = a.T.body_a + ta.w; body_b = a.T.body_b + ta.w;
         tcbase = ldi4(a.T.tree_b, tr).z;
-        sc = {a.scratch + lane, a.L};
-        const cdptr g = cptr(a.T.gravity) + 4*tr;
-        grav = V3(g[0], g[1], g[2]);
+        sc = {a.scratch, a.L, (unsigned)lane};
+        if constexpr (MIXED) sk = {a.scratch32, a.L, (unsigned)lane};
+        else sk = sc;
+        if constexpr (std::is_same<PA, PrmL>::value) {
+            // per-lane model: nb*PRM_STRIDE rows, then 3 gravity rows
+            prm = PrmL{a.prm_lane, a.L, (unsigned)lane};
+            const PrmL g = prm + (size_t)nb * PRM_STRIDE;
+            grav = V3(K(g[0]), K(g[1]), K(g[2]));
+        } else {
+            prm = cptr(a.T.prm) + (size_t)ta.w * PRM_STRIDE;
+            const cdptr g = cptr(a.T.gravity) + 4*tr;
+            grav = V3(K(g[0]), K(g[1]), K(g[2]));
+        }
         fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
     }
-    __device__ __forceinline__ dv F(int b) const {
-        if (!fapp) return svzero<double>();
-        const double* fb = fapp + 6*b;
-        return {V3(fb[0], fb[1], fb[2]), V3(fb[3], fb[4], fb[5])};
+    // Hand-over rows of body b (1-based, table entry ba, G slot gs).
+    __device__ __forceinline__ KStore<K> store(int b, int4 ba, int gs) const {
+        if constexpr (MIXED) return {sk, sc, 6*ba.w, 6*nu + 4*(b-1), gs + 6*mob_dof(ba.y)};
+        else return {sk, sc, gs, tcbase + 4*(b-1), gs + 6*mob_dof(ba.y)};
     }
-    // A Free mobilizer can only connect body 1 to Ground (the host rejects it
-    // elsewhere), so body 1 is peeled out of the loops when it is Free: the
-    // 6-dof body's register demand then never overlaps the loop's.
-    __device__ __forceinline__ void derivs(const Rows& y, const Rows& f) const {
-        const Rows fu{f.p + (size_t)nq * f.L, f.L};
-        Pre cur, nxt;
-        const int4 a0 = ldi4(body_a, 0);
-        const bool freeRoot = a0.y == FREE;
-        // pass 1: base to tip
-        dm R = ident<double>(); d3 p = zero3<double>(); dv V = svzero<double>();
-        prefetch<false>(cur, y, nq, a0, sc, tcbase, 0, false);
-        int bstart = 1;
-        if (freeRoot) {
-            pass1_body<FREE>(prm, cur, a0.z, f, sc, tcbase, R, p, V);
-            bstart = 2;
-            if (nb >= 2) { const int4 bn = ldi4(body_a, 1); prefetch<false>(cur, y, nq, bn, sc, tcbase + 4, 0, false); }
-        }
-        int4 ba = ldi4(body_a, bstart-1);
-        for (int b = bstart; b <= nb; ++b) {
-            int4 bn = ba;
-            if (b < nb) { bn = ldi4(body_a, b); prefetch<false>(nxt, y, nq, bn, sc, tcbase + 4*b, 0, false); }
-            const cdptr P = prm + (size_t)(b-1) * PRM_STRIDE;
-            const int tco = tcbase + 4*(b-1);
-            switch (ba.y) {
-            case PIN:       pass1_body<PIN>(P, cur, ba.z, f, sc, tco, R, p, V); break;
-            case SLIDER:    pass1_body<SLIDER>(P, cur, ba.z, f, sc, tco, R, p, V); break;
-            case UNIVERSAL: pass1_body<UNIVERSAL>(P, cur, ba.z, f, sc, tco, R, p, V); break;
-            default:        pass1_body<BALL>(P, cur, ba.z, f, sc, tco, R, p, V); break;
-            }
-            cur = nxt; ba = bn;
-        }
-        // pass 2: tip to base
-        da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
-        ba = ldi4(body_a, nb-1);
-        prefetch<false>(cur, y, nq, ba, sc, tcbase + 4*(nb-1), 0, true);
-        const int bstop = freeRoot ? 2 : 1;
-        for (int b = nb; b >= bstop; --b) {
-            int4 bn = ba;
-            if (b > 1) { bn = ldi4(body_a, b-2); prefetch<false>(nxt, y, nq, bn, sc, tcbase + 4*(b-2), 0, true); }
-            const int gs = ldi4(body_b, b-1).w;
-            const cdptr P = prm + (size_t)(b-1) * PRM_STRIDE;
-            const bool hc = b < nb;
-            const dv Fb = F(b);
-            switch (ba.y) {
-            case PIN:       pass2_body<PIN>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            case SLIDER:    pass2_body<SLIDER>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            case UNIVERSAL: pass2_body<UNIVERSAL>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            default:        pass2_body<BALL>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            }
-            cur = nxt; ba = bn;
-        }
-        if (freeRoot) pass2_body<FREE>(prm, cur, grav, F(1), sc, ldi4(body_b, 0).w, R, p, V, nb > 1, PP, zp, phic);
-        // pass 3: base to tip
-        R = ident<double>(); p = zero3<double>(); V = svzero<double>();
-        dv A = svzero<double>();
-        prefetch<true>(cur, y, nq, a0, sc, tcbase, ldi4(body_b, 0).w, true);
-        if (freeRoot) {
-            pass3_body<FREE>(prm, cur, a0.w, fu, sc, ldi4(body_b, 0).w, R, p, V, A);
-            if (nb >= 2) { const int4 bn = ldi4(body_a, 1); prefetch<true>(cur, y, nq, bn, sc, tcbase + 4, ldi4(body_b, 1).w, true); }
-        }
-        ba = ldi4(body_a, bstart-1);
-        for (int b = bstart; b <= nb; ++b) {
-            int4 bn = ba;
-            if (b < nb) { bn = ldi4(body_a, b); prefetch<true>(nxt, y, nq, bn, sc, tcbase + 4*b, ldi4(body_b, b).w, true); }
-            const int gs = ldi4(body_b, b-1).w;
-            const cdptr P = prm + (size_t)(b-1) * PRM_STRIDE;
-            switch (ba.y) {
-            case PIN:       pass3_body<PIN>(P, cur, ba.w, fu, sc, gs, R, p, V, A); break;
-            case SLIDER:    pass3_body<SLIDER>(P, cur, ba.w, fu, sc, gs, R, p, V, A); break;
-            case UNIVERSAL: pass3_body<UNIVERSAL>(P, cur, ba.w, fu, sc, gs, R, p, V, A); break;
-            default:        pass3_body<BALL>(P, cur, ba.w, fu, sc, gs, R, p, V, A); break;
-            }
-            cur = nxt; ba = bn;
-        }
+    __device__ __forceinline__ sv<K> F(int b) const {
+        if (!fapp) return svzero<K>();
+        const double* fb = fapp + 6*b;
+        return {V3(K(fb[0]), K(fb[1]), K(fb[2])), V3(K(fb[3]), K(fb[4]), K(fb[5]))};
     }
 
     // ---- Runge-Kutta-Merson stages fused into the sweeps -------------------
@@ -532,19 +563,20 @@ struct ChainEval {
             if constexpr (STAGE >= 4) x.s[3][k] = r.fb(idx);
         }
     }
-    template <int STAGE>
+    template <int STAGE, bool AF>
     __device__ __forceinline__ void pre1_any(Pre1& x, const RkRows& r, int4 ba) const {
         switch (ba.y) {
         case PIN:       pre1<PIN, STAGE>(x, r, ba.z, ba.w); break;
         case SLIDER:    pre1<SLIDER, STAGE>(x, r, ba.z, ba.w); break;
         case UNIVERSAL: pre1<UNIVERSAL, STAGE>(x, r, ba.z, ba.w); break;
-        case BALL:      pre1<BALL, STAGE>(x, r, ba.z, ba.w); break;
-        default:        pre1<FREE, STAGE>(x, r, ba.z, ba.w); break;
+        default:
+            if constexpr (AF) { if (ba.y == FREE) { pre1<FREE, STAGE>(x, r, ba.z, ba.w); break; } }
+            pre1<BALL, STAGE>(x, r, ba.z, ba.w); break;
         }
     }
     // stage input from the raw sources; stored to r.st (stages 1-4)
     template <int TYPE, int STAGE>
-    __device__ __forceinline__ void form(const Pre1& x, Pre& o, const RkRows& r, int qi, int ui, double h) const {
+    __device__ __forceinline__ void form(const Pre1& x, Pre<K>& o, const RkRows& r, int qi, int ui, double h) const {
         constexpr int NQ = MobC<TYPE>::NQ, D = MobC<TYPE>::D;
 #pragma unroll
         for (int k = 0; k < NQ + D; ++k) {
@@ -560,9 +592,9 @@ struct ChainEval {
     }
     // STAGE 4, pass 3 epilogue of one body: y1, yerr and the norm sums.
     template <int TYPE>
-    __device__ __forceinline__ void combine(const Pre& x, const double* y0, const double* f0, const double* fb,
-                                            const double* ud, const RkRows& r, int qi, int ui, double h,
-                                            NormSums& ns) const {
+    __device__ __forceinline__ void combine(const Pre<K>& x, const double* y0, const double* f0, const double* fb,
+                                            const double* c0, const double* ud, const RkRows& r, int qi, int ui,
+                                            double h, NormSums& ns) const {
         constexpr int NQ = MobC<TYPE>::NQ, D = MobC<TYPE>::D;
         double f4[NQ + D];
         if constexpr (TYPE == BALL || TYPE == FREE) {
@@ -578,9 +610,17 @@ struct ChainEval {
 #pragma unroll
         for (int k = 0; k < NQ + D; ++k) {
             const double ys = k < NQ ? x.q[k] : x.u[k - NQ];
-            y1[k] = y0[k] + (h/6)*((f0[k] + 4*fb[k]) + f4[k]);
-            e[k] = 0.2 * fabs(y1[k] - ys);
             const int idx = k < NQ ? qi + k : nq + ui + (k - NQ);
+            if constexpr (MIXED) {
+                // Kahan: the step's increment carries the rounding lost when
+                // the previous increment was added to the state
+                const double dy = (h/6)*((f0[k] + 4*fb[k]) + f4[k]) - c0[k];
+                y1[k] = y0[k] + dy;
+                r.c1(idx) = (y1[k] - y0[k]) - dy;
+            } else {
+                y1[k] = y0[k] + (h/6)*((f0[k] + 4*fb[k]) + f4[k]);
+            }
+            e[k] = 0.2 * fabs(y1[k] - ys);
             r.y1(idx) = y1[k]; r.ye(idx) = e[k];
         }
         if constexpr (TYPE == BALL || TYPE == FREE) {
@@ -605,49 +645,52 @@ struct ChainEval {
 
     template <int TYPE, int STAGE>
     __device__ __forceinline__ void p1(const Pre1& raw, const RkRows& r, const int4 ba, int b, double h,
-                                       dm& R, d3& p, dv& V) const {
-        Pre x;
+                                       m33<K>& R, v3<K>& p, sv<K>& V) const {
+        Pre<K> x;
         form<TYPE, STAGE>(raw, x, r, ba.z, ba.w, h);
-        pass1_body<TYPE>(prm + (size_t)(b-1) * PRM_STRIDE, x, ba.z, r.out, sc, tcbase + 4*(b-1), R, p, V);
+        pass1_body<TYPE, K>(prm + (size_t)(b-1) * PRM_STRIDE, x, ba.z, r.out, store(b, ba, 0), R, p, V);
     }
     template <int TYPE, int STAGE>
-    __device__ __forceinline__ void p3(const Pre& x, const RkRows& r, const Rows& fu, const int4 ba, int b, double h,
-                                       dm& R, d3& p, dv& V, dv& A, NormSums& ns) const {
+    __device__ __forceinline__ void p3(const Pre<K>& x, const RkRows& r, const Rows& fu, const int4 ba, int b, double h,
+                                       m33<K>& R, v3<K>& p, sv<K>& V, sv<K>& A, NormSums& ns) const {
         constexpr int NQ = MobC<TYPE>::NQ, D = MobC<TYPE>::D;
-        double y0[NQ + D], f0[NQ + D], fb[NQ + D];
+        double y0[NQ + D], f0[NQ + D], fb[NQ + D], c0[NQ + D];
         if constexpr (STAGE == 4) {
 #pragma unroll
             for (int k = 0; k < NQ + D; ++k) {
                 const int idx = k < NQ ? ba.z + k : nq + ba.w + (k - NQ);
                 y0[k] = r.y0(idx); f0[k] = r.f0(idx); fb[k] = r.fb(idx);
+                if constexpr (MIXED) c0[k] = r.c0(idx);
             }
         }
         double ud[D];
-        pass3_body<TYPE>(prm + (size_t)(b-1) * PRM_STRIDE, x, ba.w, fu, sc, ldi4(body_b, b-1).w, R, p, V, A, ud);
-        if constexpr (STAGE == 4) combine<TYPE>(x, y0, f0, fb, ud, r, ba.z, ba.w, h, ns);
+        pass3_body<TYPE, K>(prm + (size_t)(b-1) * PRM_STRIDE, x, ba.w, fu, store(b, ba, ldi4(body_b, b-1).w),
+                            R, p, V, A, ud);
+        if constexpr (STAGE == 4) combine<TYPE>(x, y0, f0, fb, c0, ud, r, ba.z, ba.w, h, ns);
     }
 
     template <int STAGE>
     __device__ __forceinline__ void derivs_rk(const RkRows& r, double h, NormSums& ns) const {
         const Rows in = STAGE == 0 ? r.y0 : r.st;
-        const Rows fu{r.out.p + (size_t)nq * r.out.L, r.out.L};
+        const Rows fu = r.out.shifted(nq);
         const int4 a0 = ldi4(body_a, 0);
         const bool freeRoot = a0.y == FREE;
         // pass 1: base to tip (stage input formed here)
-        dm R = ident<double>(); d3 p = zero3<double>(); dv V = svzero<double>();
+        m33<K> R = ident<K>(); v3<K> p = zero3<K>(); sv<K> V = svzero<K>();
         {
             Pre1 c1, n1;
-            pre1_any<STAGE>(c1, r, a0);
             int bstart = 1;
             if (freeRoot) {
-                p1<FREE, STAGE>(c1, r, a0, 1, h, R, p, V);
+                Pre1 c0;
+                pre1_any<STAGE, true>(c0, r, a0);
+                p1<FREE, STAGE>(c0, r, a0, 1, h, R, p, V);
                 bstart = 2;
-                if (nb >= 2) pre1_any<STAGE>(c1, r, ldi4(body_a, 1));
             }
+            if (bstart <= nb) pre1_any<STAGE, false>(c1, r, ldi4(body_a, bstart-1));
             int4 ba = ldi4(body_a, bstart-1);
             for (int b = bstart; b <= nb; ++b) {
                 int4 bn = ba;
-                if (b < nb) { bn = ldi4(body_a, b); pre1_any<STAGE>(n1, r, bn); }
+                if (b < nb) { bn = ldi4(body_a, b); pre1_any<STAGE, false>(n1, r, bn); }
                 switch (ba.y) {
                 case PIN:       p1<PIN, STAGE>(c1, r, ba, b, h, R, p, V); break;
                 case SLIDER:    p1<SLIDER, STAGE>(c1, r, ba, b, h, R, p, V); break;
@@ -658,41 +701,50 @@ struct ChainEval {
             }
         }
         // pass 2: tip to base
-        Pre cur, nxt;
-        da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
+        Pre<K> cur, nxt;
+        abi<K> PP; sv<K> zp = svzero<K>(); v3<K> phic = zero3<K>();
         int4 ba = ldi4(body_a, nb-1);
-        prefetch<false>(cur, in, nq, ba, sc, tcbase + 4*(nb-1), 0, true);
         const int bstop = freeRoot ? 2 : 1;
+        if (nb >= bstop) prefetch<false, false>(cur, in, nq, ba, store(nb, ba, 0), true);
         for (int b = nb; b >= bstop; --b) {
             int4 bn = ba;
-            if (b > 1) { bn = ldi4(body_a, b-2); prefetch<false>(nxt, in, nq, bn, sc, tcbase + 4*(b-2), 0, true); }
-            const int gs = ldi4(body_b, b-1).w;
-            const cdptr P = prm + (size_t)(b-1) * PRM_STRIDE;
+            if (b > bstop) { bn = ldi4(body_a, b-2); prefetch<false, false>(nxt, in, nq, bn, store(b-1, bn, 0), true); }
+            const KStore<K> st = store(b, ba, ldi4(body_b, b-1).w);
+            const PA P = prm + (size_t)(b-1) * PRM_STRIDE;
             const bool hc = b < nb;
-            const dv Fb = F(b);
+            const sv<K> Fb = F(b);
             switch (ba.y) {
-            case PIN:       pass2_body<PIN>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            case SLIDER:    pass2_body<SLIDER>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            case UNIVERSAL: pass2_body<UNIVERSAL>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
-            default:        pass2_body<BALL>(P, cur, grav, Fb, sc, gs, R, p, V, hc, PP, zp, phic); break;
+            case PIN:       pass2_body<PIN, K>(P, cur, grav, Fb, st, R, p, V, hc, PP, zp, phic); break;
+            case SLIDER:    pass2_body<SLIDER, K>(P, cur, grav, Fb, st, R, p, V, hc, PP, zp, phic); break;
+            case UNIVERSAL: pass2_body<UNIVERSAL, K>(P, cur, grav, Fb, st, R, p, V, hc, PP, zp, phic); break;
+            default:        pass2_body<BALL, K>(P, cur, grav, Fb, st, R, p, V, hc, PP, zp, phic); break;
             }
             cur = nxt; ba = bn;
         }
-        if (freeRoot) pass2_body<FREE>(prm, cur, grav, F(1), sc, ldi4(body_b, 0).w, R, p, V, nb > 1, PP, zp, phic);
+        if (freeRoot) {
+            Pre<K> c0;
+            const KStore<K> st = store(1, a0, ldi4(body_b, 0).w);
+            prefetch<false, true>(c0, in, nq, a0, st, true);
+            pass2_body<FREE, K>(prm, c0, grav, F(1), st, R, p, V, nb > 1, PP, zp, phic);
+        }
         // pass 3: base to tip
-        R = ident<double>(); p = zero3<double>(); V = svzero<double>();
-        dv A = svzero<double>();
-        prefetch<true>(cur, in, nq, a0, sc, tcbase, ldi4(body_b, 0).w, true);
+        R = ident<K>(); p = zero3<K>(); V = svzero<K>();
+        sv<K> A = svzero<K>();
         int bstart = 1;
         if (freeRoot) {
-            p3<FREE, STAGE>(cur, r, fu, a0, 1, h, R, p, V, A, ns);
+            Pre<K> c0;
+            prefetch<true, true>(c0, in, nq, a0, store(1, a0, ldi4(body_b, 0).w), true);
+            p3<FREE, STAGE>(c0, r, fu, a0, 1, h, R, p, V, A, ns);
             bstart = 2;
-            if (nb >= 2) prefetch<true>(cur, in, nq, ldi4(body_a, 1), sc, tcbase + 4, ldi4(body_b, 1).w, true);
+        }
+        if (bstart <= nb) {
+            const int4 bs = ldi4(body_a, bstart-1);
+            prefetch<true, false>(cur, in, nq, bs, store(bstart, bs, ldi4(body_b, bstart-1).w), true);
         }
         ba = ldi4(body_a, bstart-1);
         for (int b = bstart; b <= nb; ++b) {
             int4 bn = ba;
-            if (b < nb) { bn = ldi4(body_a, b); prefetch<true>(nxt, in, nq, bn, sc, tcbase + 4*b, ldi4(body_b, b).w, true); }
+            if (b < nb) { bn = ldi4(body_a, b); prefetch<true, false>(nxt, in, nq, bn, store(b+1, bn, ldi4(body_b, b).w), true); }
             switch (ba.y) {
             case PIN:       p3<PIN, STAGE>(cur, r, fu, ba, b, h, R, p, V, A, ns); break;
             case SLIDER:    p3<SLIDER, STAGE>(cur, r, fu, ba, b, h, R, p, V, A, ns); break;

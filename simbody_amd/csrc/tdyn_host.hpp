@@ -1,0 +1,108 @@
+// Host-side state shared by the C ABI implementation files: the batch
+// handle, device buffers, error reporting and model packing.
+#pragma once
+#include "../../include/tdyn.h"
+#include "tdyn_internal.hpp"
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace tdyn {
+namespace host {
+std::string& err_slot();     // this thread's last error message
+inline int fail(int code, const std::string& msg) { err_slot() = msg; return code; }
+inline int hipfail(hipError_t e, const char* what) {
+    err_slot() = std::string(what) + ": " + hipGetErrorString(e);
+    return TDYN_E_HIP;
+}
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return ::tdyn::host::hipfail(e_, #x); } while (0)
+
+template <class T> struct DBuf {
+    T* p = nullptr; size_t n = 0;
+    hipError_t alloc(size_t count) {
+        free();
+        n = count;
+        if (!count) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    hipError_t upload(const std::vector<T>& v) {
+        hipError_t e = alloc(v.size());
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
+    void free() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    ~DBuf() { free(); }
+};
+
+// The PRM_STRIDE model constants of body b (1-based) of tree d: X_PF, X_MB =
+// X_BM^-1, mass, mass center, unit inertia G = I/m about Bo, p_BM.
+void pack_body_params(const tdyn_tree_desc& d, int b, double* P);
+// Validates one tree description (parents before children, known mobilizers,
+// Free only on Ground, positive finite masses, finite frames).
+int validate_tree(const tdyn_tree_desc& d, int t);
+} // namespace host
+} // namespace tdyn
+
+struct tdyn_batch {
+    using RkmCfg = tdyn::RkmCfg; using DevTables = tdyn::DevTables;
+    using OpArgs = tdyn::OpArgs; using RkmArgs = tdyn::RkmArgs;
+    int n_inst = 0, n_trees = 0;
+    int64_t total_q = 0, total_u = 0, total_b = 0;
+    int max_nq = 0, max_nu = 0, max_nb = 0, max_rec = 0;
+    bool uniform = true;
+    std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
+    std::vector<int4> h_tree_a, h_tree_b, h_body_a;
+    // device tables
+    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b;
+    tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;
+    tdyn::host::DBuf<double> prm, gravity, scratch;
+    // rollout state
+    tdyn::host::DBuf<double> vec, uscale, t, h, t_end;
+    tdyn::host::DBuf<int> status, stats;
+    tdyn::host::DBuf<double> fapp;
+    tdyn::host::DBuf<float> scratch32;   // mixed-precision hand-over rows
+    int precision = 0;                   // integrator precision of the current rollout (0 FP64, 1 mixed)
+    int ny_max = 0;
+    bool rollout_ready = false;
+    bool is_chain = false;       // uniform batch whose tree is a serial chain
+    int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
+    RkmCfg cfg{};
+
+    DevTables tables() const {
+        DevTables T;
+        T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
+        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p;
+        return T;
+    }
+    OpArgs op() const {
+        OpArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
+        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        return a;
+    }
+    RkmArgs rk() const {
+        RkmArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
+        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
+        a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
+        a.scratch32 = scratch32.p;
+        a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
+        return a;
+    }
+};
+
+
+namespace tdyn {
+namespace host {
+// RkmCfg from the public integrator settings (defaults of
+// IntegratorRep.h:737-740 and the System time scale).
+int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision);
+// Allocates the device-resident integrator state of a batch (state vectors,
+// per-lane time, step, status and statistics) for its current sizes.
+int ensure_rollout_buffers(tdyn_batch* b);
+} // namespace host
+} // namespace tdyn

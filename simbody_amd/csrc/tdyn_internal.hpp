@@ -97,23 +97,32 @@ struct RkmArgs {
     const int* tree_of;
     int n_inst, L;
     double* scratch;
+    float* scratch32;        // mixed path: float hand-over rows (6*nu G rows, then 4*nb trig rows) per lane
     InstOffsets off;
     const double* fapp;      // constant applied body forces, packed, nullable
-    double* vec;             // 9 SoA vectors of ny_max*L doubles: y y0 f0 fa fb ysave y1 ytmp yerr
+    double* vec;             // 11 SoA vectors of ny_max*L doubles: y y0 f0 fa fb ysave y1 ytmp yerr c0 c1
     int ny_max;
     double* t;               // per lane
     double* h;               // per lane: current step size
     const double* t_end;     // per lane
-    int* status;             // per lane: 0 running, 1 done, <0 failed
+    int* status;             // per lane: 0 running, 1 done, 2 empty serving slot, <0 failed
     int* stats;              // per lane: attempts accepted rejected projections
     double* uscale;          // SoA nu_max*L
     RkmCfg cfg;
     int precision;           // 0 = FP64, 1 = mixed
     int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
+    const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
+                             // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
 };
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
+hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
+                            double t0, hipStream_t s);
+hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double* out, int out_stride, int* outi,
+                              hipStream_t s);
+// vec: which state vector to read when unpacking (0 = the state y, 2 = f0)
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
-                             double* q_out, double* u_out, hipStream_t s);
+                             double* q_out, double* u_out, hipStream_t s, int vec = 0);
+hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s);
 
 } // namespace tdyn

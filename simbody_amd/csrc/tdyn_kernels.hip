@@ -25,7 +25,7 @@ struct Lane {
     DevTables T;
     int lane, L, nb, nq, nu, body_off;
     double* scr;
-    __device__ double& s(int off) const { return scr[(size_t)off * L + lane]; }
+    __device__ double& s(int off) const { return (scr + (size_t)off * L)[(unsigned)lane]; }
     __device__ int4 ba(int b) const { return ldi4(T.body_a, body_off + b - 1); }
     __device__ int4 bb(int b) const { return ldi4(T.body_b, body_off + b - 1); }
     __device__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
@@ -470,12 +470,12 @@ struct PackedPut {
     __device__ void operator()(int k, double v) const { if (p) p[k] = v; }
 };
 struct SoAGet {
-    const double* p; int L, lane;
-    __device__ double operator()(int k) const { return p[(size_t)k * L + lane]; }
+    const double* p; int L; unsigned lane;
+    __device__ double operator()(int k) const { return (p + (size_t)k * L)[lane]; }
 };
 struct SoAPut {
-    double* p; int L, lane;
-    __device__ void operator()(int k, double v) const { p[(size_t)k * L + lane] = v; }
+    double* p; int L; unsigned lane;
+    __device__ void operator()(int k, double v) const { (p + (size_t)k * L)[lane] = v; }
 };
 struct NoPut { __device__ void operator()(int, double) const {} };
 struct Zero { __device__ double operator()(int) const { return 0.0; } };
@@ -630,7 +630,7 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // projection).  Lanes stop after cfg.max_steps accepted steps so a rollout can
 // be resumed by the next launch (the serving loop relies on this).
 // ---------------------------------------------------------------------------
-enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, NVEC = 9 };
+enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
 
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 struct TreeEval {
@@ -638,15 +638,14 @@ struct TreeEval {
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
         : c(make_lane_t(a.T, tr, lane, a.L, a.scratch)), grav(a.T.gravity + 4*tr),
           fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr) {}
-    __device__ __forceinline__ void derivs(const chain::Rows& yr, const chain::Rows& fr) const {
-        const SoAGet y{yr.p, yr.L, 0}, u{yr.p + (size_t)c.nq * yr.L, yr.L, 0};
-        const SoAPut f{fr.p, fr.L, 0}, fu{fr.p + (size_t)c.nq * fr.L, fr.L, 0};
+    __device__ __forceinline__ void derivs(const chain::RowsT<double, false>& yr, const chain::RowsT<double, false>& fr) const {
+        const SoAGet y{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
+        const SoAPut f{fr.base, fr.L, fr.lane}, fu{fr.base + (size_t)c.nq * fr.L, fr.L, fr.lane};
         sweep_kin<true>(c, y, u, f);
         sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
         sweep_outward2<true>(c, y, u, fu, NoPut{});
     }
 };
-using chain::ChainEval;
 
 // One lane runs one instance's TimeStepper loop.  The state y0 lives in
 // vector VY; a step evaluates f0 = f(y0) once and then, per attempt, the four
@@ -667,8 +666,9 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int nquat = ldi4(a.T.tree_b, tr).y;
     const size_t NV = (size_t)a.ny_max * a.L;
     const int L = a.L;
-    auto rows = [&](int v) { return chain::Rows{a.vec + v*NV + lane, L}; };
-    const chain::Rows Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
+    using TRows = chain::RowsT<double, false>;
+    auto rows = [&](int v) { return TRows{a.vec + v*NV, L, (unsigned)lane}; };
+    const TRows Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
                       Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
 
     const RkmCfg& cfg = a.cfg;
@@ -727,7 +727,7 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             if (++attempts > cfg.max_attempts) { status = -2; break; }
             const double h = t1 - t;
             for (int st = s0; st < 5; ++st) {
-                chain::Rows in = Y, out = F0;
+                TRows in = Y, out = F0;
                 if (st == 1) {
 #pragma unroll 8
                     for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/3)*F0(k);
@@ -832,19 +832,21 @@ __device__ __forceinline__ bool adjust_step(double err, double acc, bool limited
 // the chain evaluator's sweeps (no separate vector passes in the common
 // path).  The state ping-pongs between VY and VY1; a lane whose current state
 // sits in VY1 when its launch ends copies it back to VY.
+template <class PA, class K>
 __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
     using namespace chain;
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     if (a.status[lane] != 0) return;
-    const ChainEval ev(a, lane, 0);
+    const ChainEval<PA, K> ev(a, lane, 0);
     const int4 ta = ldi4(a.T.tree_a, 0);
     const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
     const int4* body_a = a.T.body_a + ta.w;
     const int nquat = ldi4(a.T.tree_b, 0).y;
     const size_t NV = (size_t)a.ny_max * a.L;
     const int L = a.L;
-    auto rows = [&](int v) { return Rows{a.vec + v*NV + lane, L}; };
+    auto rows = [&](int v) { return Rows{a.vec + v*NV, L, (unsigned)lane}; };
+    const Rows C0 = rows(VC0), C1 = rows(VC1);
     const Rows S0 = rows(VY), S1 = rows(VY1), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
                YT = rows(VYT), YE = rows(VYE);
 
@@ -863,9 +865,13 @@ __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
 
     while (steps < cfg.max_steps) {
         if (t >= tEnd) { status = 1; break; }
-        const Rows Y = flip ? S1 : S0, Y1 = flip ? S0 : S1;
-        RkRows r{Y, F0, FA, FB, YT, F0, Y1, YE};
-        ev.derivs_rk<0>(r, 0.0, ns0);
+        // Every active lane has taken the same number of accepted steps in
+        // this launch, so flip is wave-uniform; saying so keeps the row
+        // bases in scalar registers.
+        const bool fl = __builtin_amdgcn_readfirstlane(flip ? 1 : 0) != 0;
+        const Rows Y = fl ? S1 : S0, Y1 = fl ? S0 : S1;
+        RkRows r{Y, F0, FA, FB, YT, F0, Y1, YE, fl ? C1 : C0, fl ? C0 : C1};
+        ev.template derivs_rk<0>(r, 0.0, ns0);
         bool ok = false;
         double t1 = t;
         do {
@@ -876,11 +882,11 @@ __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
             if (!(t1 > t)) { status = -3; break; }
             if (++attempts > cfg.max_attempts) { status = -2; break; }
             const double h = t1 - t;
-            r.st = YT; r.out = FA; ev.derivs_rk<1>(r, h, ns0);
-            ev.derivs_rk<2>(r, h, ns0);
-            r.out = FB; ev.derivs_rk<3>(r, h, ns0);
+            r.st = YT; r.out = FA; ev.template derivs_rk<1>(r, h, ns0);
+            ev.template derivs_rk<2>(r, h, ns0);
+            r.out = FB; ev.template derivs_rk<3>(r, h, ns0);
             NormSums ns{0.0, 0.0, 0.0, 0.0};
-            r.st = YS; r.out = FA; ev.derivs_rk<4>(r, h, ns);
+            r.st = YS; r.out = FA; ev.template derivs_rk<4>(r, h, ns);
             const double qn = nq ? sqrt((ns.sq_other + ns.sq_quat) / nq) : 0.0;
             const double un = nu ? sqrt(ns.su / nu) : 0.0;
             const double e1 = qn >= un ? qn : un;
@@ -922,18 +928,56 @@ __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
         ++accepted; ++steps;
     }
     if (flip) {
-#pragma unroll 8
         for (int k = 0; k < ny; ++k) S0(k) = S1(k);
+        if constexpr (!std::is_same<K, double>::value)
+            for (int k = 0; k < ny; ++k) C0(k) = C1(k);
     }
     if (status == 0 && t >= tEnd) status = 1;
     a.t[lane] = t; a.h[lane] = hcur; a.status[lane] = status;
     a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
 }
 
-// initialize(): forced quaternion normalization, t and h set.
-__global__ void __launch_bounds__(64) k_rkm_init(RkmArgs a, double t0) {
+// f = f(y) of the current rollout state (vector VY) into VF0, by the
+// rollout's evaluator at its precision: the derivative the integrator uses.
+template <class PA, class K>
+__global__ void __launch_bounds__(64) k_chain_f0(RkmArgs a) {
+    using namespace chain;
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
+    const ChainEval<PA, K> ev(a, lane, 0);
+    const size_t NV = (size_t)a.ny_max * a.L;
+    auto rows = [&](int v) { return Rows{a.vec + v*NV, a.L, (unsigned)lane}; };
+    const RkRows r{rows(VY), rows(VF0), rows(VFA), rows(VFB), rows(VYT), rows(VF0), rows(VY1), rows(VYE),
+                   rows(VC0), rows(VC1)};
+    NormSums ns{0.0, 0.0, 0.0, 0.0};
+    ev.template derivs_rk<0>(r, 0.0, ns);
+}
+template <bool UNI>
+__global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    const int tr = UNI ? 0 : a.tree_of[lane];
+    const TreeEval ev(a, lane, tr);
+    const size_t NV = (size_t)a.ny_max * a.L;
+    using TRows = chain::RowsT<double, false>;
+    ev.derivs(TRows{a.vec + VY*NV, a.L, (unsigned)lane}, TRows{a.vec + VF0*NV, a.L, (unsigned)lane});
+}
+hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
+    if (a.n_inst <= 0) return hipSuccess;
+    const dim3 grd((a.n_inst + 63)/64), blk(64);
+    if (a.chain && a.precision == 1) {
+        if (a.prm_lane) k_chain_f0<chain::PrmL, float><<<grd, blk, 0, s>>>(a);
+        else k_chain_f0<cdptr, float><<<grd, blk, 0, s>>>(a);
+    } else if (a.chain) {
+        if (a.prm_lane) k_chain_f0<chain::PrmL, double><<<grd, blk, 0, s>>>(a);
+        else k_chain_f0<cdptr, double><<<grd, blk, 0, s>>>(a);
+    } else if (a.tree_of) k_tree_f0<false><<<grd, blk, 0, s>>>(a);
+    else k_tree_f0<true><<<grd, blk, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+// initialize(): forced quaternion normalization, t and h set.
+__device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
     const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
     const size_t NV = (size_t)a.ny_max * a.L;
     for (int b = 1; b <= c.nb; ++b) {
@@ -952,6 +996,68 @@ __global__ void __launch_bounds__(64) k_rkm_init(RkmArgs a, double t0) {
     if (hmax != -1) h = fmin(h, hmax);
     a.t[lane] = t0; a.h[lane] = h; a.status[lane] = 0;
     for (int k = 0; k < 4; ++k) a.stats[4*lane + k] = 0;
+    // Kahan compensation of the mixed path starts at zero
+    for (int k = 0; k < c.nq + c.nu; ++k) a.vec[VC0*NV + (size_t)k*a.L + lane] = 0.0;
+}
+__global__ void __launch_bounds__(64) k_rkm_init(RkmArgs a, double t0) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    rkm_init_lane(a, lane, t0);
+}
+
+// Serving slots.  Admission record of one request (doubles): nb*PRM_STRIDE
+// model constants and 3 gravity, then q (nq), u (nu), t_end, and (nb+1)*6
+// applied body forces.  One 64-thread block per admitted request scatters
+// its record into the slot's structure-of-arrays rows, then initializes the
+// slot's integrator state.
+__global__ void __launch_bounds__(64) k_srv_admit(RkmArgs a, const double* rec, const int* slots, int n,
+                                                  int rec_stride, double t0) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int slot = slots[i];
+    const int4 ta = a.T.tree_a[0];
+    const int nb = ta.x, nq = ta.y, nu = ta.z;
+    const double* r = rec + (size_t)i * rec_stride;
+    const int np = nb * PRM_STRIDE + 3;
+    const size_t L = a.L;
+    double* prm_lane = const_cast<double*>(a.prm_lane);
+    for (int k = threadIdx.x; k < np; k += blockDim.x) prm_lane[(size_t)k * L + slot] = r[k];
+    for (int k = threadIdx.x; k < nq + nu; k += blockDim.x) a.vec[(size_t)k * L + slot] = r[np + k];   // VY
+    double* fa = const_cast<double*>(a.fapp) + 6 * (size_t)a.off.b_off[slot];
+    for (int k = threadIdx.x; k < 6 * (nb + 1); k += blockDim.x) fa[k] = r[np + nq + nu + 1 + k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const_cast<double*>(a.t_end)[slot] = r[np + nq + nu];
+        rkm_init_lane(a, slot, t0);
+    }
+}
+// Copies the finished slots' state out (q then u, t) and their status and
+// statistics, and marks the slots empty (status 2).
+__global__ void k_srv_collect(RkmArgs a, const int* slots, int n, double* out, int out_stride, int* outi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int slot = slots[i];
+    const int4 ta = a.T.tree_a[0];
+    const int ny = ta.y + ta.z;
+    const size_t L = a.L;
+    double* o = out + (size_t)i * out_stride;
+    for (int k = 0; k < ny; ++k) o[k] = a.vec[(size_t)k * L + slot];
+    o[ny] = a.t[slot];
+    outi[5*i] = a.status[slot];
+    for (int k = 0; k < 4; ++k) outi[5*i + 1 + k] = a.stats[4*slot + k];
+    a.status[slot] = 2;
+}
+hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
+                            double t0, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_srv_admit<<<dim3(n), dim3(64), 0, s>>>(a, rec, slots, n, rec_stride, t0);
+    return hipGetLastError();
+}
+hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double* out, int out_stride, int* outi,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_srv_collect<<<dim3((n + 63)/64), dim3(64), 0, s>>>(a, slots, n, out, out_stride, outi);
+    return hipGetLastError();
 }
 
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
@@ -962,21 +1068,27 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
-    if (a.chain) k_rkm_chain<<<grd, blk, 0, s>>>(a);
+    if (a.chain && a.precision == 1) {
+        if (a.prm_lane) k_rkm_chain<chain::PrmL, float><<<grd, blk, 0, s>>>(a);
+        else k_rkm_chain<cdptr, float><<<grd, blk, 0, s>>>(a);
+    } else if (a.chain) {
+        if (a.prm_lane) k_rkm_chain<chain::PrmL, double><<<grd, blk, 0, s>>>(a);
+        else k_rkm_chain<cdptr, double><<<grd, blk, 0, s>>>(a);
+    }
     else if (a.tree_of) k_rkm<TreeEval, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval, true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
 
-// Packed boundary layout <-> SoA state vector VY.
-__global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, double* q_out, double* u_out) {
+// Packed boundary layout <-> SoA state vector VY (or, out only, vector v).
+__global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, double* q_out, double* u_out, int v) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     const int tr = a.tree_of ? a.tree_of[lane] : 0;
     const int4 ta = a.T.tree_a[tr];
     const int nq = ta.y, nu = ta.z;
     const int qo = a.off.q_off[lane], uo = a.off.u_off[lane];
-    double* Y = a.vec;   // VY
+    double* Y = a.vec + (size_t)v * a.ny_max * a.L;
     if (to_soa) {
         for (int k = 0; k < nq; ++k) Y[(size_t)k*a.L + lane] = q[qo + k];
         for (int k = 0; k < nu; ++k) Y[(size_t)(nq + k)*a.L + lane] = u[uo + k];
@@ -986,9 +1098,9 @@ __global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, 
     }
 }
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
-                             double* q_out, double* u_out, hipStream_t s) {
+                             double* q_out, double* u_out, hipStream_t s, int vec) {
     if (a.n_inst <= 0) return hipSuccess;
-    k_pack<<<dim3((a.n_inst + 255)/256), dim3(256), 0, s>>>(a, q, u, to_soa, q_out, u_out);
+    k_pack<<<dim3((a.n_inst + 255)/256), dim3(256), 0, s>>>(a, q, u, to_soa, q_out, u_out, to_soa ? 0 : vec);
     return hipGetLastError();
 }
 

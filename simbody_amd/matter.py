@@ -32,7 +32,9 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_rollout_begin", "tdyn_rollout_advance",
-           "tdyn_rollout_get", "tdyn_rollout", "tdyn_last_error"]
+           "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_last_error",
+           "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
+           "tdyn_server_advance", "tdyn_server_collect"]
 
 
 class TdynError(RuntimeError):
@@ -87,10 +89,34 @@ def lib():
                                          ctypes.POINTER(IntegratorCfgC), vp]
         L.tdyn_rollout_advance.argtypes = [vp, ctypes.c_int, vp]
         L.tdyn_rollout_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.tdyn_rollout_derivs.argtypes = [vp, vp, vp, vp]
         L.tdyn_rollout.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                    ctypes.POINTER(IntegratorCfgC), vp, vp]
+        L.tdyn_server_create.argtypes = [ctypes.POINTER(TreeDescC), ctypes.POINTER(IntegratorCfgC), ctypes.c_size_t,
+                                         ctypes.c_int, ctypes.POINTER(vp)]
+        L.tdyn_server_destroy.argtypes = [vp]
+        L.tdyn_server_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.tdyn_server_admit.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TreeDescC), _d, _d, _d, _d, _i32,
+                                        ctypes.POINTER(ctypes.c_int), vp]
+        L.tdyn_server_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_server_collect.argtypes = [vp, ctypes.c_int, _i32, _d, _d, _d, _i32, _i32,
+                                          ctypes.POINTER(ctypes.c_int), vp]
         _lib = L
     return _lib
+
+
+def fill_tree_desc(d, spec, keep):
+    """Fill a TreeDescC from a TreeSpec-like object; the arrays it points to
+    are appended to `keep`, which must outlive every use of d."""
+    arrs = [np.ascontiguousarray(spec.parent, np.int32), np.ascontiguousarray(spec.mobilizer, np.int32)] + \
+           [np.ascontiguousarray(x, np.float64) for x in (spec.mass, spec.com_B, spec.inertia_B, spec.X_PF, spec.X_BM)]
+    keep.append(arrs)
+    d.nb = len(arrs[0])
+    d.parent = arrs[0].ctypes.data_as(_i32); d.mobilizer = arrs[1].ctypes.data_as(_i32)
+    d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (a.ctypes.data_as(_d) for a in arrs[2:])
+    for j in range(3):
+        d.gravity[j] = float(spec.gravity[j])
+    return d
 
 
 def _check(rc):
@@ -115,17 +141,20 @@ def _stream():
 class RungeKuttaMerson:
     """RungeKuttaMersonIntegrator settings (Integrator.cpp:243-270)."""
 
+    FP64, MIXED = 0, 1
+
     def __init__(self, accuracy=1e-3, fixed_step=0.0, constraint_tolerance=0.0, initial_step=0.0,
-                 min_step=0.0, max_step=0.0, time_scale=0.1, max_attempts=0):
+                 min_step=0.0, max_step=0.0, time_scale=0.1, max_attempts=0, precision=0):
         self.accuracy, self.fixed_step = accuracy, fixed_step
         self.constraint_tolerance, self.initial_step = constraint_tolerance, initial_step
         self.min_step, self.max_step, self.time_scale = min_step, max_step, time_scale
         self.max_attempts = max_attempts
+        self.precision = precision   # 0 FP64, 1 mixed (chain evaluator)
 
     def c(self):
         return IntegratorCfgC(self.accuracy, self.constraint_tolerance, self.fixed_step,
                               self.initial_step, self.min_step, self.max_step, self.time_scale,
-                              self.max_attempts, 0)
+                              self.max_attempts, self.precision)
 
 
 class MatterBatch:
@@ -147,15 +176,7 @@ class MatterBatch:
         self._keep = []
         descs = (TreeDescC * len(trees))()
         for k, s in enumerate(trees):
-            arrs = [np.ascontiguousarray(s.parent, np.int32), np.ascontiguousarray(s.mobilizer, np.int32)] + \
-                   [np.ascontiguousarray(x, np.float64) for x in (s.mass, s.com_B, s.inertia_B, s.X_PF, s.X_BM)]
-            self._keep.append(arrs)
-            d = descs[k]
-            d.nb = len(arrs[0])
-            d.parent = arrs[0].ctypes.data_as(_i32); d.mobilizer = arrs[1].ctypes.data_as(_i32)
-            d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (a.ctypes.data_as(_d) for a in arrs[2:])
-            for j in range(3):
-                d.gravity[j] = float(s.gravity[j])
+            fill_tree_desc(descs[k], s, self._keep)
         with torch.cuda.device(self.device):
             h = ctypes.c_void_p()
             _check(lib().tdyn_batch_create(descs, len(trees),
@@ -277,6 +298,13 @@ class MatterBatch:
         stats = torch.zeros(4 * self.n_instances, dtype=torch.int32, device=self.device)
         _check(lib().tdyn_rollout_get(self.h, _ptr(q), _ptr(u), _ptr(t), _ptr(st), _ptr(stats), _stream()))
         return q, u, t, st, stats.view(-1, 4)
+
+    def rollout_derivs(self):
+        """(qdot, udot) of the current rollout state by the rollout's
+        evaluator and precision (the integrator's f0)."""
+        qd = self._new(self.total_q); ud = self._new(self.total_u)
+        _check(lib().tdyn_rollout_derivs(self.h, _ptr(qd), _ptr(ud), _stream()))
+        return qd, ud
 
     def stepTo(self, q, u, integ, t_end, fapp=None, t0=0.0):
         """TimeStepper(system, integ).initialize(state); stepTo(t_end), with

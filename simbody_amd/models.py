@@ -184,44 +184,134 @@ def spatial_chain(nb=32, seed=2, types=(PIN, UNIVERSAL, BALL)):
                     X_PF=np.tile(_xform(), (nb, 1)), X_BM=X_BM)
 
 
+def stiff_chain(nb=32, seed=2, max_ratio=1e4, types=(PIN, UNIVERSAL, BALL)):
+    """spatial_chain(nb, seed, types) with link masses (and, with them, the
+    inertias about each body origin) scaled by max_ratio**u_b, u_b ~ U(0,1)
+    per link, so neighbouring links differ in mass by ratios up to
+    max_ratio."""
+    spec = spatial_chain(nb, seed, types)
+    f = max_ratio ** Rng(seed, 3).uniform(nb)
+    spec.mass = spec.mass * f
+    spec.inertia_B = spec.inertia_B * f[:, None]
+    return spec
+
+
 _TREE_TYPES = (PIN, SLIDER, UNIVERSAL, BALL)
 _TREE_P = np.array([0.40, 0.10, 0.20, 0.25])
 
 
+def _normals(U):
+    """Box-Muller normals from pairs of uniform columns: U[..., 2k] and
+    U[..., 2k+1] give normal k."""
+    u1 = U[..., 0::2]; u2 = U[..., 1::2]
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def _quat_rows_to_rot(q):
+    """Rotation matrices [n,3,3] of (not necessarily unit) quaternions [n,4]."""
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([np.stack([w*w + x*x - y*y - z*z, 2*(x*y - w*z), 2*(x*z + w*y)], -1),
+                     np.stack([2*(x*y + w*z), w*w - x*x + y*y - z*z, 2*(y*z - w*x)], -1),
+                     np.stack([2*(x*z - w*y), 2*(y*z + w*x), w*w - x*x - y*y + z*z], -1)], 1)
+
+
+# Per-body counter layout of the bulk generators (uniforms per body):
+# 0 parent, 1 mobilizer, 2-3 mass normal, 4 link length, 5-10 link direction
+# (3 normals), 11-13 principal-moment draws, 14-21 principal axes (quaternion
+# of 4 normals), 22-29 X_BM rotation, 30-37 X_PF rotation, 38-43 X_PF offset.
+_BODY_DRAWS = 44
+
+
+def _links_from_uniforms(U):
+    """Mass properties and joint offsets of the links whose draws are the rows
+    of U [n, _BODY_DRAWS]: mass ~ LogNormal(0, 0.5); central inertia with
+    moments (y+z, x+z, x+y), x,y,z ~ U(0.02, 0.2) times m, on random principal
+    axes; inboard joint at L*d in B with L ~ U(0.2, 1) and d a random unit
+    vector; mass center at L*d/2.  Returns mass [n], com [n,3], inertia about
+    Bo [n,6], p_BM [n,3]."""
+    n = U.shape[0]
+    m = np.exp(0.5 * _normals(U[:, 2:4])[:, 0])
+    L = 0.2 + 0.8 * U[:, 4]
+    d = _normals(U[:, 5:11]); d /= np.linalg.norm(d, axis=1, keepdims=True)
+    p = L[:, None] * d
+    c = 0.5 * p
+    xyz = 0.02 + 0.18 * U[:, 11:14]
+    mom = np.stack([xyz[:, 1] + xyz[:, 2], xyz[:, 0] + xyz[:, 2], xyz[:, 0] + xyz[:, 1]], 1)
+    R = _quat_rows_to_rot(_normals(U[:, 14:22]))
+    Ic = m[:, None, None] * np.einsum("nij,nj,nkj->nik", R, mom, R)
+    cc = np.einsum("ni,ni->n", c, c)
+    Io = Ic + m[:, None, None] * (cc[:, None, None] * np.eye(3)[None] - np.einsum("ni,nj->nij", c, c))
+    I6 = np.stack([Io[:, 0, 0], Io[:, 1, 1], Io[:, 2, 2], Io[:, 0, 1], Io[:, 0, 2], Io[:, 1, 2]], 1)
+    return m, c, I6, p
+
+
 def random_tree(seed, index, nmin=8, nmax=128, max_depth=20, general_frames=False):
-    """Uneven branched tree: nb ~ U{nmin..nmax}; body 1 is the root on Ground;
-    each later body's parent is uniform among earlier bodies of depth <
-    max_depth.  The root draws Pin .40 / Slider .10 / Universal .20 / Ball .25 /
-    Free .05; other bodies draw from the first four renormalized.  Mass,
-    inertia and offsets as in spatial_chain.  With general_frames the X_PF and
-    X_BM carry random rotations as well (parity tests)."""
+    """Uneven branched tree: nb ~ U{nmin..nmax}; body 1 is the root on Ground
+    (depth 1); each later body's parent is uniform among the earlier bodies of
+    depth < max_depth, so no body is deeper than max_depth.  The root draws
+    Pin .40 / Slider .10 / Universal .20 / Ball .25 / Free .05; other bodies
+    draw from the first four renormalized.  Mass, inertia and offsets as in
+    spatial_chain.  With general_frames the X_PF and X_BM carry random
+    rotations as well (parity tests).  All draws of a tree come from one
+    counter stream in a fixed per-body layout."""
     rng = Rng(seed, 1000003 + index)
-    nb = int(rng.integers(1, nmin, nmax)[0])
-    parent = np.zeros(nb, np.int32); depth = np.zeros(nb + 1, np.int64)
+    head = rng.uniform(2)
+    nb = min(nmax, nmin + int(head[0] * (nmax - nmin + 1)))
+    U = rng.uniform(nb * _BODY_DRAWS).reshape(nb, _BODY_DRAWS)
+    parent = np.zeros(nb, np.int32)
+    depth = [0, 1]
+    cand = [1] if max_depth > 1 else []
     for b in range(2, nb + 1):
-        cand = np.nonzero(depth[1:b] < max_depth)[0] + 1
-        parent[b - 1] = cand[int(rng.uniform(1)[0] * len(cand)) % len(cand)]
-        depth[b] = depth[parent[b - 1]] + 1
-    depth[1] = 1
+        pb = cand[min(len(cand) - 1, int(U[b - 1, 0] * len(cand)))]
+        parent[b - 1] = pb
+        depth.append(depth[pb] + 1)
+        if depth[b] < max_depth:
+            cand.append(b)
     mob = np.zeros(nb, np.int32)
-    r = rng.uniform(1)[0]
     cum = np.cumsum(np.append(_TREE_P, 0.05))
-    mob[0] = [PIN, SLIDER, UNIVERSAL, BALL, FREE][int(np.searchsorted(cum, r * cum[-1], side="right").clip(0, 4))]
+    mob[0] = [PIN, SLIDER, UNIVERSAL, BALL, FREE][int(np.searchsorted(cum, head[1] * cum[-1], side="right").clip(0, 4))]
     pn = np.cumsum(_TREE_P / _TREE_P.sum())
-    for b in range(1, nb):
-        mob[b] = _TREE_TYPES[int(np.searchsorted(pn, rng.uniform(1)[0], side="right").clip(0, 3))]
-    mass = np.exp(0.5 * rng.normal(nb))
-    com = np.zeros((nb, 3)); inert = np.zeros((nb, 6)); X_BM = np.zeros((nb, 12)); X_PF = np.zeros((nb, 12))
-    for i in range(nb):
-        p, c, Io = _link(rng, mass[i])
-        com[i] = c; inert[i] = _sym6(Io)
-        if general_frames:
-            X_BM[i] = _xform(_random_rotation(rng), p)
-            X_PF[i] = _xform(_random_rotation(rng), 0.1 * rng.normal(3))
-        else:
-            X_BM[i] = _xform(p=p); X_PF[i] = _xform()
+    if nb > 1:
+        mob[1:] = np.array(_TREE_TYPES)[np.searchsorted(pn, U[1:, 1], side="right").clip(0, 3)]
+    mass, com, inert, p = _links_from_uniforms(U)
+    X_BM = np.zeros((nb, 12)); X_PF = np.zeros((nb, 12))
+    X_BM[:, 9:] = p
+    if general_frames:
+        X_BM[:, :9] = _quat_rows_to_rot(_normals(U[:, 22:30])).reshape(nb, 9)
+        X_PF[:, :9] = _quat_rows_to_rot(_normals(U[:, 30:38])).reshape(nb, 9)
+        X_PF[:, 9:] = 0.1 * _normals(U[:, 38:44])
+    else:
+        X_BM[:, :9] = np.eye(3).reshape(9); X_PF[:, :9] = np.eye(3).reshape(9)
     return TreeSpec(parent=parent, mobilizer=mob, mass=mass, com_B=com, inertia_B=inert,
                     X_PF=X_PF, X_BM=X_BM)
+
+
+def chain_params(seed, indices, nb=32, types=(PIN, UNIVERSAL, BALL)):
+    """Per-instance parameters of chains with the topology of
+    spatial_chain(nb, types=types) and the same distributions, one
+    independent draw per instance index (the "each request carries its own
+    tree" serving workload).  Returns mass [n,nb], com [n,nb,3], inertia
+    [n,nb,6], X_PF [n,nb,12], X_BM [n,nb,12]; row i depends only on
+    (seed, indices[i])."""
+    idx = np.asarray(indices)
+    U = _uniform_matrix(seed + 0x5EED, idx, nb * _BODY_DRAWS).reshape(len(idx) * nb, _BODY_DRAWS)
+    mass, com, inert, p = _links_from_uniforms(U)
+    n = len(idx)
+    X_BM = np.zeros((n * nb, 12)); X_BM[:, :9] = np.eye(3).reshape(9); X_BM[:, 9:] = p
+    X_PF = np.tile(_xform(), (n * nb, 1))
+    return (mass.reshape(n, nb), com.reshape(n, nb, 3), inert.reshape(n, nb, 6),
+            X_PF.reshape(n, nb, 12), X_BM.reshape(n, nb, 12))
+
+
+def chain_spec(params, i, types=(PIN, UNIVERSAL, BALL)):
+    """TreeSpec of instance i of chain_params output."""
+    mass, com, inert, X_PF, X_BM = params
+    nb = mass.shape[1]
+    return TreeSpec(parent=np.arange(nb, dtype=np.int32),
+                    mobilizer=np.array([types[k % len(types)] for k in range(nb)], np.int32),
+                    mass=mass[i].copy(), com_B=com[i].copy(), inertia_B=inert[i].copy(),
+                    X_PF=X_PF[i].copy(), X_BM=X_BM[i].copy())
 
 
 def _state_plan(spec):

@@ -1,0 +1,202 @@
+"""Continuous batching of rollout requests on one GPU.
+
+`Server` wraps the tdyn_server_* C ABI (include/tdyn.h): a fixed set of
+device slots, sized from an HBM budget, for trees of one chain topology.
+Each request brings its own model constants, initial state and final time;
+its State stays on the device from admission until it is collected.
+`serve()` is the request loop: requests arriving (by their arrival times)
+are admitted whenever slots are free, all occupied slots advance together
+by a few integrator steps per launch, and finished ones are retired.
+Latency is measured from a request's arrival to the moment its result is
+on the host.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from .matter import IntegratorCfgC, TreeDescC, _check, _d, _i32, _stream, fill_tree_desc, lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_d)
+
+
+class Server:
+    def __init__(self, topology, integ, hbm_budget_bytes, max_slots=0, device="cuda:0"):
+        import torch
+        self.torch = torch
+        self.device = torch.device(device)
+        keep = []
+        d = fill_tree_desc(TreeDescC(), topology, keep)
+        cfg = integ.c()
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _check(lib().tdyn_server_create(ctypes.byref(d), ctypes.byref(cfg), ctypes.c_size_t(int(hbm_budget_bytes)),
+                                            int(max_slots), ctypes.byref(h)))
+        self.h = h
+        self.nb = len(topology.parent)
+        inf = self.info()
+        self.slots, self.nq, self.nu, self.bytes_per_slot = inf["slots"], inf["nq"], inf["nu"], inf["bytes_per_slot"]
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None:
+            try:
+                from . import matter
+                if matter._lib is not None:
+                    matter._lib.tdyn_server_destroy(self.h)
+            except ImportError:      # interpreter shutdown: the process exit frees the device memory
+                pass
+            self.h = None
+
+    def info(self):
+        a = (ctypes.c_int64 * 6)()
+        _check(lib().tdyn_server_info(self.h, a))
+        return dict(zip(["slots", "free", "occupied", "bytes_per_slot", "nq", "nu"], list(a)))
+
+    def admit(self, specs, q, u, t_end, fapp=None):
+        """Admit as many of the requests as there are free slots (in order);
+        returns the slots of the admitted ones (len <= len(specs))."""
+        n = len(specs)
+        if n == 0:
+            return np.zeros(0, np.int32)
+        keep = []
+        descs = (TreeDescC * n)()
+        for i, s in enumerate(specs):
+            fill_tree_desc(descs[i], s, keep)
+        q = np.ascontiguousarray(q, np.float64).reshape(n, self.nq)
+        u = np.ascontiguousarray(u, np.float64).reshape(n, self.nu)
+        t_end = np.ascontiguousarray(t_end, np.float64).reshape(n)
+        fa = None if fapp is None else np.ascontiguousarray(fapp, np.float64).reshape(n, -1)
+        slots = np.zeros(n, np.int32)
+        k = ctypes.c_int(0)
+        _check(lib().tdyn_server_admit(self.h, n, descs, _dp(q), _dp(u), _dp(t_end), _dp(fa),
+                                       slots.ctypes.data_as(_i32), ctypes.byref(k), _stream()))
+        return slots[:k.value]
+
+    def advance(self, max_steps):
+        _check(lib().tdyn_server_advance(self.h, int(max_steps), _stream()))
+
+    def collect(self, max_out=None):
+        m = self.slots if max_out is None else int(max_out)
+        slots = np.zeros(m, np.int32); q = np.zeros((m, self.nq)); u = np.zeros((m, self.nu))
+        t = np.zeros(m); st = np.zeros(m, np.int32); stats = np.zeros((m, 4), np.int32)
+        k = ctypes.c_int(0)
+        _check(lib().tdyn_server_collect(self.h, m, slots.ctypes.data_as(_i32), _dp(q), _dp(u), _dp(t),
+                                         st.ctypes.data_as(_i32), stats.ctypes.data_as(_i32), ctypes.byref(k),
+                                         _stream()))
+        k = k.value
+        return dict(slots=slots[:k], q=q[:k], u=u[:k], t=t[:k], status=st[:k], stats=stats[:k])
+
+
+class Requests:
+    """n rollout requests with their tree descriptions prebuilt, so that the
+    request loop admits a contiguous run of them with one C call and no
+    per-request host work."""
+
+    def __init__(self, specs, q, u, t_end, fapp=None):
+        n = len(specs)
+        self.n = n
+        self._keep = []
+        self.descs = (TreeDescC * n)()
+        for i, s in enumerate(specs):
+            fill_tree_desc(self.descs[i], s, self._keep)
+        self._init_state(q, u, t_end, fapp)
+
+    @classmethod
+    def from_chain_params(cls, template, params, q, u, t_end, fapp=None):
+        """Requests whose trees share `template`'s topology and take their
+        constants from models.chain_params output (no per-request arrays)."""
+        self = cls.__new__(cls)
+        mass, com, inert, X_PF, X_BM = (np.ascontiguousarray(a, np.float64) for a in params)
+        n, nb = mass.shape
+        self.n = n
+        par = np.ascontiguousarray(template.parent, np.int32); mob = np.ascontiguousarray(template.mobilizer, np.int32)
+        self._keep = [par, mob, mass, com, inert, X_PF, X_BM]
+        self.descs = (TreeDescC * n)()
+        addr = [a.ctypes.data for a in (mass, com, inert, X_PF, X_BM)]
+        step = [8 * nb, 24 * nb, 48 * nb, 96 * nb, 96 * nb]
+        pp, pm = par.ctypes.data_as(_i32), mob.ctypes.data_as(_i32)
+        g = [float(x) for x in template.gravity]
+        for i in range(n):
+            d = self.descs[i]
+            d.nb = nb; d.parent = pp; d.mobilizer = pm
+            d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (ctypes.cast(a + i * st, _d) for a, st in zip(addr, step))
+            d.gravity[0], d.gravity[1], d.gravity[2] = g
+        self._init_state(q, u, t_end, fapp)
+        return self
+
+    def _init_state(self, q, u, t_end, fapp):
+        n = self.n
+        self.q = np.ascontiguousarray(q, np.float64).reshape(n, -1)
+        self.u = np.ascontiguousarray(u, np.float64).reshape(n, -1)
+        self.t_end = np.ascontiguousarray(t_end, np.float64).reshape(n)
+        self.fapp = None if fapp is None else np.ascontiguousarray(fapp, np.float64).reshape(n, -1)
+
+    def admit_range(self, server, i0, k):
+        """Admit requests i0..i0+k-1 (as many as there are free slots)."""
+        if k <= 0:
+            return np.zeros(0, np.int32)
+        sz = ctypes.sizeof(TreeDescC)
+        descs = ctypes.cast(ctypes.addressof(self.descs) + i0 * sz, ctypes.POINTER(TreeDescC))
+        qp = ctypes.cast(self.q.ctypes.data + i0 * self.q.shape[1] * 8, _d)
+        up = ctypes.cast(self.u.ctypes.data + i0 * self.u.shape[1] * 8, _d)
+        tp = ctypes.cast(self.t_end.ctypes.data + i0 * 8, _d)
+        fp = None if self.fapp is None else ctypes.cast(self.fapp.ctypes.data + i0 * self.fapp.shape[1] * 8, _d)
+        slots = np.zeros(k, np.int32)
+        kk = ctypes.c_int(0)
+        _check(lib().tdyn_server_admit(server.h, int(k), descs, qp, up, tp, fp, slots.ctypes.data_as(_i32),
+                                       ctypes.byref(kk), _stream()))
+        return slots[:kk.value]
+
+
+def serve(server, req, arrivals, steps_per_advance=4, time_limit=None):
+    """Run the request loop over `req` (a Requests), whose arrival times
+    (seconds from the start, sorted) are `arrivals`.  Requests are admitted
+    first come, first served.  Returns per-request results (q, u, status,
+    stats, latency) and loop figures."""
+    n = req.n
+    arrivals = np.asarray(arrivals, np.float64)
+    assert np.all(np.diff(arrivals) >= 0), "arrivals must be sorted"
+    out_q = np.zeros((n, server.nq)); out_u = np.zeros((n, server.nu))
+    status = np.zeros(n, np.int32); stats = np.zeros((n, 4), np.int32)
+    latency = np.full(n, np.nan); admitted_at = np.full(n, np.nan)
+    slot_req = np.full(server.slots, -1, np.int64)
+    head = 0            # next request to admit (FIFO); requests head..nxt-1 are queued
+    nxt = 0; n_done = 0; occupied = 0; free = server.slots
+    launches = 0; max_occupied = 0
+    t0 = time.perf_counter()
+    while n_done < n:
+        now = time.perf_counter() - t0
+        if time_limit is not None and now > time_limit:
+            break
+        nxt = int(np.searchsorted(arrivals, now, side="right"))
+        if nxt > head and free > 0:
+            k = min(free, nxt - head)
+            slots = req.admit_range(server, head, k)
+            assert len(slots) == k
+            slot_req[slots] = np.arange(head, head + k)
+            admitted_at[head:head + k] = time.perf_counter() - t0
+            head += k
+            occupied += k; free -= k
+            max_occupied = max(max_occupied, occupied)
+        if occupied == 0:
+            if nxt < n:
+                wait = arrivals[min(nxt, n - 1)] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+            continue
+        server.advance(steps_per_advance)
+        launches += 1
+        res = server.collect()
+        tdone = time.perf_counter() - t0
+        ids = slot_req[res["slots"]]
+        slot_req[res["slots"]] = -1
+        out_q[ids] = res["q"]; out_u[ids] = res["u"]
+        status[ids] = res["status"]; stats[ids] = res["stats"]
+        latency[ids] = tdone - arrivals[ids]
+        k = len(ids)
+        n_done += k; occupied -= k; free += k
+    wall = time.perf_counter() - t0
+    return dict(q=out_q, u=out_u, status=status, stats=stats, latency=latency, admitted_at=admitted_at,
+                wall=wall, launches=launches, max_occupied=max_occupied, completed=n_done)
