@@ -139,7 +139,9 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             B->h_body_a.push_back(make_int4(d.parent[b-1], ty, q, u));
             body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, gslot));
             for (int c : ch[b]) child_idx.push_back(c);
-            q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(mob_dof(ty)); gslot += 7 * mob_dof(ty);
+            // batches of different trees carry non-Free bodies in padded 3-dof records
+            const int rd = (!B->uniform && ty != TDYN_FREE) ? 3 : mob_dof(ty);
+            q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(rd); gslot += 7 * mob_dof(ty);
             if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
             double P[PRM_STRIDE];
             pack_body_params(d, b, P);
@@ -168,6 +170,28 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
     B->total_q = q; B->total_u = u; B->total_b = bb;
     B->is_chain = B->is_chain && B->uniform;
+    // Lane order.  A wave runs as long as its slowest lane, so instances of
+    // different trees are assigned to lanes in order of decreasing work
+    // (bodies weighted by their mobilizer's cost): the 64 lanes of a wave
+    // then walk trees of similar size.  Per-lane tables are permuted;
+    // per-instance results are mapped back through inst_of_lane.
+    std::vector<int> lane_tree(n_instances), lane_q(n_instances), lane_u(n_instances), lane_b(n_instances);
+    std::vector<int> inst_of_lane(n_instances);
+    for (int i = 0; i < n_instances; ++i) inst_of_lane[i] = i;
+    if (!B->uniform) {
+        static const double cost[5] = {1.0, 0.9, 1.4, 1.8, 3.3};
+        std::vector<double> tw(n_trees, 0.0);
+        for (int t = 0; t < n_trees; ++t)
+            for (int bdy = 0; bdy < trees[t].nb; ++bdy) tw[t] += cost[trees[t].mobilizer[bdy]];
+        std::stable_sort(inst_of_lane.begin(), inst_of_lane.end(), [&](int x, int y) {
+            return tw[B->h_tree_of[x]] > tw[B->h_tree_of[y]];
+        });
+        B->permuted = true;
+    }
+    for (int l = 0; l < n_instances; ++l) {
+        const int i = inst_of_lane[l];
+        lane_tree[l] = B->h_tree_of[i]; lane_q[l] = B->h_qoff[i]; lane_u[l] = B->h_uoff[i]; lane_b[l] = B->h_boff[i];
+    }
     hipError_t e;
     if ((e = B->body_a.upload(B->h_body_a)) != hipSuccess ||
         (e = B->body_b.upload(body_b)) != hipSuccess ||
@@ -176,10 +200,11 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->tree_a.upload(B->h_tree_a)) != hipSuccess ||
         (e = B->tree_b.upload(B->h_tree_b)) != hipSuccess ||
         (e = B->gravity.upload(grav)) != hipSuccess ||
-        (e = B->tree_of.upload(B->h_tree_of)) != hipSuccess ||
-        (e = B->q_off.upload(B->h_qoff)) != hipSuccess ||
-        (e = B->u_off.upload(B->h_uoff)) != hipSuccess ||
-        (e = B->b_off.upload(B->h_boff)) != hipSuccess ||
+        (e = B->tree_of.upload(lane_tree)) != hipSuccess ||
+        (e = B->q_off.upload(lane_q)) != hipSuccess ||
+        (e = B->u_off.upload(lane_u)) != hipSuccess ||
+        (e = B->b_off.upload(lane_b)) != hipSuccess ||
+        (e = B->inst.upload(inst_of_lane)) != hipSuccess ||
         (e = B->scratch.alloc((size_t)(B->max_rec + 24) * n_instances)) != hipSuccess) {
         delete B;
         return hipfail(e, "tdyn_batch_create: device allocation/upload");
@@ -298,7 +323,8 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         b->fapp.free();
     }
     if (t_end) {
-        HIPCHK(hipMemcpyAsync(b->t_end.p, t_end, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (b->permuted) HIPCHK(launch_lane_gather(b->inst.p, (int)L, t_end, b->t_end.p, s));
+        else HIPCHK(hipMemcpyAsync(b->t_end.p, t_end, L * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
         std::vector<double> te(L, t_end_all);
         HIPCHK(hipMemcpyAsync(b->t_end.p, te.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
@@ -325,9 +351,13 @@ int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* st
     if (q && u) HIPCHK(launch_pack_state(a, nullptr, nullptr, 0, q, u, s));
     else if (q || u) return fail(TDYN_E_ARG, "tdyn_rollout_get: q and u go together");
     const size_t L = b->n_inst;
-    if (t) HIPCHK(hipMemcpyAsync(t, b->t.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
-    if (status) HIPCHK(hipMemcpyAsync(status, b->status.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
-    if (stats) HIPCHK(hipMemcpyAsync(stats, b->stats.p, 4 * L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (b->permuted) {
+        HIPCHK(launch_lane_scatter(b->inst.p, (int)L, b->t.p, t, b->status.p, status, b->stats.p, stats, s));
+    } else {
+        if (t) HIPCHK(hipMemcpyAsync(t, b->t.p, L * sizeof(double), hipMemcpyDeviceToDevice, s));
+        if (status) HIPCHK(hipMemcpyAsync(status, b->status.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
+        if (stats) HIPCHK(hipMemcpyAsync(stats, b->stats.p, 4 * L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    }
     return TDYN_OK;
 }
 

@@ -55,7 +55,9 @@ struct tdyn_batch {
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
     tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b;
-    tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;
+    tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;   // per LANE
+    tdyn::host::DBuf<int> inst;          // instance of each lane
+    bool permuted = false;               // lanes are not in instance order
     tdyn::host::DBuf<double> prm, gravity, scratch;
     // rollout state
     tdyn::host::DBuf<double> vec, uscale, t, h, t_end;
@@ -79,7 +81,7 @@ struct tdyn_batch {
         OpArgs a;
         std::memset(&a, 0, sizeof(a));
         a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
-        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p}; a.inst = permuted ? inst.p : nullptr;
         return a;
     }
     RkmArgs rk() const {

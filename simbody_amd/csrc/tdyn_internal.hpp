@@ -50,7 +50,7 @@ struct DevTables {
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
-struct InstOffsets {
+struct InstOffsets {   // indexed by lane
     const int* q_off;   // start of the instance's q in packed q arrays
     const int* u_off;   // start of its u in packed u arrays
     const int* b_off;   // start of its (nb+1) bodies in packed per-body arrays
@@ -89,6 +89,7 @@ struct OpArgs {
     double* out_abi2;        // PPlus per body (21), nullable
     double* out_scalar;      // kinematics: ke, pe per instance (2), nullable
     int* status;             // per instance, nullable
+    const int* inst;         // instance of each lane (nullable: lane = instance)
 };
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s);
 
@@ -124,5 +125,9 @@ hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double*
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
                              double* q_out, double* u_out, hipStream_t s, int vec = 0);
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s);
+// per-instance <-> per-lane copies for batches whose lanes are permuted
+hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s);
+hipError_t launch_lane_scatter(const int* inst, int n, const double* t_lane, double* t_inst, const int* st_lane,
+                               int* st_inst, const int* stats_lane, int* stats_inst, hipStream_t s);
 
 } // namespace tdyn

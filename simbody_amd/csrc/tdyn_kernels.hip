@@ -192,10 +192,121 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
     }
 }
 
+// Padded generic body (batches of different trees): any non-Free mobilizer
+// is carried as a 3-dof body whose unused H columns, u entries and D rows are
+// zero (D padded with ones on the diagonal, so D^-1 is the mobilizer's own
+// inverse bordered by an identity block and every padded term adds an exact
+// zero).  Only the joint's own X_FM, H_FM and qdot branch on the type, so
+// lanes walking different trees stay on one instruction stream.
 template <bool VEL, class QG, class UG, class QD>
+__device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba, const QG& q, const UG& u,
+                                             const QD& qd) {
+    constexpr int D = 3;
+    const int ty = ba.y, p = ba.x, qi = ba.z, ui = ba.w;
+    const int d = mob_dof(ty);
+    const cdptr P = c.prm(b);
+    const int rb = c.rec(b);
+    dm R_GP; d3 p_GP;
+    int rp = 0;
+    if (p == 0) { R_GP = ident<double>(); p_GP = zero3<double>(); }
+    else { rp = c.rec(p); R_GP = ldm(c, rp + REC_R); p_GP = ld3(c, rp + REC_P); }
+    double qv[4] = {0.0, 0.0, 0.0, 0.0};
+    dm R_FM; d3 p_FM = zero3<double>();
+    dv H_FM[D] = {svzero<double>(), svzero<double>(), svzero<double>()};
+    if (ty == PIN) {
+        qv[0] = q(qi);
+        double sn, cs; sincos(qv[0], &sn, &cs);
+        R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=0; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=0;
+        R_FM.a[6]=0;  R_FM.a[7]=0;   R_FM.a[8]=1;
+        H_FM[0] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+    } else if (ty == SLIDER) {
+        qv[0] = q(qi);
+        R_FM = ident<double>(); p_FM = V3(qv[0], 0.0, 0.0);
+        H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+    } else if (ty == UNIVERSAL) {
+        qv[0] = q(qi); qv[1] = q(qi + 1);
+        double s1, c1, s2, c2; sincos(qv[0], &s1, &c1); sincos(qv[1], &s2, &c2);
+        R_FM.a[0]= c2;    R_FM.a[1]= 0.0; R_FM.a[2]= s2;
+        R_FM.a[3]= s2*s1; R_FM.a[4]= c1;  R_FM.a[5]= -s1*c2;
+        R_FM.a[6]=-s2*c1; R_FM.a[7]= s1;  R_FM.a[8]= c1*c2;
+        H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
+        H_FM[1] = {col(R_FM, 1), zero3<double>()};
+    } else {   // BALL
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qv[k] = q(qi + k);
+        const double len = sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]);
+        const double oo = 1.0 / len;
+        R_FM = quat_rot(qv[0]*oo, qv[1]*oo, qv[2]*oo, qv[3]*oo);
+        H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
+        H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
+        H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+    }
+    const dm R_PF = prm_m(P + PRM_R_PF), R_MB = prm_m(P + PRM_R_MB);
+    const d3 p_PF = prm_3(P + PRM_P_PF), p_MB = prm_3(P + PRM_P_MB);
+    const dm R_FB = mul(R_FM, R_MB);
+    const d3 p_FB = p_FM + mul(R_FM, p_MB);
+    const dm R_PB = mul(R_PF, R_FB);
+    const d3 p_PB = p_PF + mul(R_PF, p_FB);
+    const dm R_GB = mul(R_GP, R_PB);
+    const d3 p_GB = p_GP + mul(R_GP, p_PB);
+    const dm R_GF = mul(R_GP, R_PF);
+    const d3 r_MB_F = mul(R_FM, p_MB);
+    dv H[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        H[j] = {mul(R_GF, H_FM[j].w), mul(R_GF, H_FM[j].v - cross(r_MB_F, H_FM[j].w))};
+    const d3 phi = mul(R_GP, p_PB);
+    const double m = P[PRM_MASS];
+    const d3 Mk_p = mul(R_GB, prm_3(P + PRM_COM));
+    const ds Gu = {P[PRM_G], P[PRM_G+1], P[PRM_G+2], P[PRM_G+3], P[PRM_G+4], P[PRM_G+5]};
+    const ds Mk_G = rotate_sym(R_GB, Gu);
+    stm(c, rb + REC_R, R_GB); st3(c, rb + REC_P, p_GB); st3(c, rb + REC_PHI, phi);
+    st3(c, rb + REC_MKP, Mk_p); sts(c, rb + REC_MKG, Mk_G);
+    const int rH = rb + REC_DIEPS + D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) stsv(c, rH + 6*j, H[j]);
+
+    if constexpr (VEL) {
+        const dv V_GP = p == 0 ? svzero<double>() : ldsv(c, rp + REC_V);
+        double uv[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) uv[k] = k < d ? u(ui + k) : 0.0;
+        if (ty == BALL) {
+            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) qd(qi + k, d4[k]);
+        } else {
+            for (int k = 0; k < d; ++k) qd(qi + k, uv[k]);
+        }
+        dv V_FM = svzero<double>(), V_PB_G = svzero<double>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) { V_FM = V_FM + uv[j]*H_FM[j]; V_PB_G = V_PB_G + uv[j]*H[j]; }
+        const d3 w_GF = V_GP.w, w_FM = V_FM.w;
+        const d3 wxr = cross(w_FM, r_MB_F);
+        dv VD = svzero<double>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            d3 hdw = zero3<double>();
+            if (j == 1 && ty == UNIVERSAL) hdw = cross(w_FM, col(R_FM, 1));
+            const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
+            const dv HD = {mul(R_GF, hdw) + cross(w_GF, H[j].w), mul(R_GF, lin) + cross(w_GF, H[j].v)};
+            VD = VD + uv[j]*HD;
+        }
+        const dv V_GB = phiT_mul(phi, V_GP) + V_PB_G;
+        const d3 w = V_GB.w;
+        const dv bg = {m*cross(w, mul(Mk_G, w)), m*cross(w, cross(w, Mk_p))};
+        const dv a = {VD.w, VD.v + cross(V_GP.w, V_GB.v - V_GP.v)};
+        stsv(c, rb + REC_V, V_GB); stsv(c, rb + REC_A, a); stsv(c, rb + REC_B, bg);
+    }
+}
+
+template <bool VEL, bool PAD = false, class QG, class UG, class QD>
 __device__ void sweep_kin(const Lane& c, const QG& q, const UG& u, const QD& qd) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
+        if constexpr (PAD) {
+            if (ba.y != FREE) { kin_body_gen<VEL>(c, b, ba, q, u, qd); continue; }
+        }
         switch (ba.y) {
         case PIN:       kin_body<PIN, VEL>(c, b, ba, q, u, qd); break;
         case SLIDER:    kin_body<SLIDER, VEL>(c, b, ba, q, u, qd); break;
@@ -252,11 +363,15 @@ enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8 };
 // M^-1 f (multiplyByMInvPass1Inward :483-515).  F(b) -> body force, f(k) ->
 // mobility force.
 // ---------------------------------------------------------------------------
+// TYPE = GEN: the padded generic body (see kin_body_gen); dof from ba.y.
+enum { GEN = 5 };
+template <int TYPE> struct MobD { static constexpr int D = TYPE == GEN ? 3 : Mob<TYPE>::D; };
 template <int TYPE, int MODE, class FB, class FM>
 __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
                                             const FB& fbody, const FM& fmob,
                                             double* outP, double* outPP, const double* grav) {
-    constexpr int D = Mob<TYPE>::D;
+    constexpr int D = MobD<TYPE>::D;
+    const int d = TYPE == GEN ? mob_dof(ba.y) : D;
     const int rb = bb.z;
     const cdptr P0 = c.prm(b);
     const double m = P0[PRM_MASS];
@@ -277,6 +392,10 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
     for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int k = 0; k < D; ++k) Dm[r*D+k] = dot(H[r], PH[k]);
+    if constexpr (TYPE == GEN) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) if (j >= d) Dm[j*D+j] = 1.0;
+    }
     inv_small<D>(Dm, DI);
     dv G[D];
 #pragma unroll
@@ -336,7 +455,7 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
         }
         double eps[D];
 #pragma unroll
-        for (int j = 0; j < D; ++j) eps[j] = fmob(ba.w + j) - dot(H[j], z);
+        for (int j = 0; j < D; ++j) eps[j] = (j < d ? fmob(ba.w + j) : 0.0) - dot(H[j], z);
         dv zp = z;
 #pragma unroll
         for (int j = 0; j < D; ++j) zp = zp + eps[j]*G[j];
@@ -351,11 +470,14 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
     }
 }
 
-template <int MODE, class FB, class FM>
+template <int MODE, bool PAD = false, class FB, class FM>
 __device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, double* outP, double* outPP,
                              const double* grav) {
     for (int b = c.nb; b >= 1; --b) {
         const int4 ba = c.ba(b), bb = c.bb(b);
+        if constexpr (PAD) {
+            if (ba.y != FREE) { inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); continue; }
+        }
         switch (ba.y) {
         case PIN: case SLIDER: inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
         case UNIVERSAL:        inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
@@ -373,7 +495,8 @@ __device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, dou
 template <int TYPE, bool WITH_COR, class QG, class UG, class UO, class QO>
 __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba, const QG& q, const UG& u,
                                              const UO& uo, const QO& qdd) {
-    constexpr int D = Mob<TYPE>::D;
+    constexpr int D = MobD<TYPE>::D;
+    const int d = TYPE == GEN ? mob_dof(ba.y) : D;
     const int rb = c.rec(b), p = ba.x;
     const dv A_GP = p == 0 ? svzero<double>() : ldsv(c, c.rec(p) + REC_AGB);
     const dv Ap = phiT_mul(ld3(c, rb + REC_PHI), A_GP);
@@ -389,29 +512,36 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
     if constexpr (WITH_COR) A = A + ldsv(c, rb + REC_A);
     stsv(c, rb + REC_AGB, A);
 #pragma unroll
-    for (int j = 0; j < D; ++j) uo(ba.w + j, ud[j]);
+    for (int j = 0; j < D; ++j) if (j < d) uo(ba.w + j, ud[j]);
     if constexpr (WITH_COR) {
-        if constexpr (TYPE == BALL || TYPE == FREE) {
-            double qq[4], nb4[4];
+        bool quat = TYPE == BALL || TYPE == FREE;
+        if constexpr (TYPE == GEN) quat = ba.y == BALL;
+        if constexpr (D >= 3) {
+            if (quat) {
+                double qq[4], nb4[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) qq[k] = q(ba.z + k);
-            const d3 w = {u(ba.w), u(ba.w+1), u(ba.w+2)};
-            quat_dot(qq, V3(ud[0], ud[1], ud[2]), nb4);
-            const double sc = -0.25 * dot(w, w);
+                for (int k = 0; k < 4; ++k) qq[k] = q(ba.z + k);
+                const d3 w = {u(ba.w), u(ba.w+1), u(ba.w+2)};
+                quat_dot(qq, V3(ud[0], ud[1], ud[2]), nb4);
+                const double sc = -0.25 * dot(w, w);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) qdd(ba.z + k, nb4[k] + sc*qq[k]);
-            if constexpr (TYPE == FREE) { qdd(ba.z+4, ud[3]); qdd(ba.z+5, ud[4]); qdd(ba.z+6, ud[5]); }
-        } else {
-#pragma unroll
-            for (int j = 0; j < D; ++j) qdd(ba.z + j, ud[j]);
+                for (int k = 0; k < 4; ++k) qdd(ba.z + k, nb4[k] + sc*qq[k]);
+                if constexpr (TYPE == FREE) { qdd(ba.z+4, ud[3]); qdd(ba.z+5, ud[4]); qdd(ba.z+6, ud[5]); }
+                return;
+            }
         }
+#pragma unroll
+        for (int j = 0; j < D; ++j) if (j < d) qdd(ba.z + j, ud[j]);
     }
 }
 
-template <bool WITH_COR, class QG, class UG, class UO, class QO>
+template <bool WITH_COR, bool PAD = false, class QG, class UG, class UO, class QO>
 __device__ void sweep_outward2(const Lane& c, const QG& q, const UG& u, const UO& uo, const QO& qdd) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
+        if constexpr (PAD) {
+            if (ba.y != FREE) { outward_body<GEN, WITH_COR>(c, b, ba, q, u, uo, qdd); continue; }
+        }
         switch (ba.y) {
         case PIN: case SLIDER: outward_body<PIN, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
         case UNIVERSAL:        outward_body<UNIVERSAL, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
@@ -431,9 +561,10 @@ __device__ __forceinline__ void resid_out_body(const Lane& c, int b, const int4 
     const dv A_GP = p == 0 ? svzero<double>() : ldsv(c, c.rec(p) + REC_AGB);
     dv A = phiT_mul(ld3(c, rb + REC_PHI), A_GP);
     const int rH = rb + REC_DIEPS + D;
+    const int d = mob_dof(ba.y);     // < D only for a padded generic body
     dv Hu = svzero<double>();
 #pragma unroll
-    for (int j = 0; j < D; ++j) Hu = Hu + udot(ba.w + j)*ldsv(c, rH + 6*j);
+    for (int j = 0; j < D; ++j) if (j < d) Hu = Hu + udot(ba.w + j)*ldsv(c, rH + 6*j);
     A = (A + Hu) + ldsv(c, rb + REC_A);
     stsv(c, rb + REC_AGB, A);
 }
@@ -450,8 +581,9 @@ __device__ __forceinline__ void resid_in_body(const Lane& c, int b, const int4 b
     }
     stsv(c, rb + REC_ZPLUS, F);
     const int rH = rb + REC_DIEPS + D;
+    const int d = mob_dof(ba.y);
 #pragma unroll
-    for (int j = 0; j < D; ++j) tau(ba.w + j, dot(ldsv(c, rH + 6*j), F) - fmob(ba.w + j));
+    for (int j = 0; j < D; ++j) if (j < d) tau(ba.w + j, dot(ldsv(c, rH + 6*j), F) - fmob(ba.w + j));
 }
 
 // ---------------------------------------------------------------------------
@@ -493,6 +625,7 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 // ---------------------------------------------------------------------------
 template <int OP, bool UNI>
 __global__ void __launch_bounds__(64) k_op(OpArgs a) {
+    constexpr bool PAD = !UNI;      // batches of different trees use padded records
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     const int tr = UNI ? 0 : a.tree_of[lane];
@@ -502,7 +635,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     const PackedGet q{a.q + qo};
     const PackedGet u{a.u ? a.u + uo : nullptr};
     if constexpr (OP == OP_KIN) {
-        sweep_kin<true>(c, q, u, PackedPut{a.out_q ? a.out_q + qo : nullptr});
+        sweep_kin<true, PAD>(c, q, u, PackedPut{a.out_q ? a.out_q + qo : nullptr});
         double ke = 0, pe = 0;
         const d3 g = {grav[0], grav[1], grav[2]};
         for (int b = 1; b <= c.nb; ++b) {
@@ -525,18 +658,21 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         }
         if (a.out_body) { double* X = a.out_body + 12*(size_t)bo; for (int i = 0; i < 12; ++i) X[i] = (i == 0 || i == 4 || i == 8) ? 1.0 : 0.0; }
         if (a.out_body2) { double* Vo = a.out_body2 + 6*(size_t)bo; for (int i = 0; i < 6; ++i) Vo[i] = 0.0; }
-        if (a.out_scalar) { a.out_scalar[2*lane] = ke; a.out_scalar[2*lane+1] = pe; }
+        if (a.out_scalar) {
+            const int inst = a.inst ? a.inst[lane] : lane;
+            a.out_scalar[2*inst] = ke; a.out_scalar[2*inst+1] = pe;
+        }
     } else if constexpr (OP == OP_FWD_SYS || OP == OP_FWD_GIVEN || OP == OP_REACT) {
-        sweep_kin<true>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
+        sweep_kin<true, PAD>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
         const PackedGetOpt fm{(OP != OP_FWD_SYS && a.fmob) ? a.fmob + uo : nullptr};
-        if constexpr (OP == OP_FWD_SYS) sweep_inward<IN_FWD | IN_SYSF>(c, fb, fm, nullptr, nullptr, grav);
-        else sweep_inward<IN_FWD>(c, fb, fm, nullptr, nullptr, grav);
+        if constexpr (OP == OP_FWD_SYS) sweep_inward<IN_FWD | IN_SYSF, PAD>(c, fb, fm, nullptr, nullptr, grav);
+        else sweep_inward<IN_FWD, PAD>(c, fb, fm, nullptr, nullptr, grav);
         const PackedPut uout{a.out_u ? a.out_u + uo : nullptr};
         if constexpr (OP == OP_FWD_GIVEN) {
-            sweep_outward2<true>(c, q, u, uout, PackedPut{a.out_q ? a.out_q + qo : nullptr});
+            sweep_outward2<true, PAD>(c, q, u, uout, PackedPut{a.out_q ? a.out_q + qo : nullptr});
         } else {
-            sweep_outward2<true>(c, q, u, uout, NoPut{});
+            sweep_outward2<true, PAD>(c, q, u, uout, NoPut{});
         }
         if constexpr (OP == OP_REACT) {
             // Mobilizer reaction forces at M, in G (SimbodyMatterSubsystemRep.cpp:5786-5826).
@@ -563,27 +699,27 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
             }
         }
     } else if constexpr (OP == OP_MINV) {
-        sweep_kin<false>(c, q, u, NoPut{});
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
         const PackedGet f{a.fmob + uo};
-        sweep_inward<IN_MINV>(c, BodyForce{nullptr}, f, nullptr, nullptr, grav);
-        sweep_outward2<false>(c, q, u, PackedPut{a.out_u + uo}, NoPut{});
+        sweep_inward<IN_MINV, PAD>(c, BodyForce{nullptr}, f, nullptr, nullptr, grav);
+        sweep_outward2<false, PAD>(c, q, u, PackedPut{a.out_u + uo}, NoPut{});
     } else if constexpr (OP == OP_ABI) {
-        sweep_kin<false>(c, q, u, NoPut{});
-        sweep_inward<IN_ABI_OUT>(c, BodyForce{nullptr}, Zero{}, a.out_abi + 21*(size_t)bo,
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
+        sweep_inward<IN_ABI_OUT, PAD>(c, BodyForce{nullptr}, Zero{}, a.out_abi + 21*(size_t)bo,
                                  a.out_abi2 ? a.out_abi2 + 21*(size_t)bo : nullptr, grav);
         for (int i = 0; i < 21; ++i) {
             a.out_abi[21*(size_t)bo + i] = 0.0;
             if (a.out_abi2) a.out_abi2[21*(size_t)bo + i] = 0.0;
         }
     } else if constexpr (OP == OP_RESID) {
-        sweep_kin<true>(c, q, u, NoPut{});
+        sweep_kin<true, PAD>(c, q, u, NoPut{});
         const PackedGetOpt ud{a.udot_in ? a.udot_in + uo : nullptr};
         const PackedGetOpt fm{a.fmob ? a.fmob + uo : nullptr};
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
         const PackedPut tau{a.out_u + uo};
         for (int b = 1; b <= c.nb; ++b) {
             const int4 ba = c.ba(b);
-            switch (mob_dof(ba.y)) {
+            switch ((PAD && ba.y != FREE) ? 3 : mob_dof(ba.y)) {
             case 1: resid_out_body<1>(c, b, ba, ud); break;
             case 2: resid_out_body<2>(c, b, ba, ud); break;
             case 3: resid_out_body<3>(c, b, ba, ud); break;
@@ -592,7 +728,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         }
         for (int b = c.nb; b >= 1; --b) {
             const int4 ba = c.ba(b), bb = c.bb(b);
-            switch (mob_dof(ba.y)) {
+            switch ((PAD && ba.y != FREE) ? 3 : mob_dof(ba.y)) {
             case 1: resid_in_body<1>(c, b, ba, bb, fb, fm, tau); break;
             case 2: resid_in_body<2>(c, b, ba, bb, fb, fm, tau); break;
             case 3: resid_in_body<3>(c, b, ba, bb, fb, fm, tau); break;
@@ -633,6 +769,7 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
 
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
+template <bool PAD>
 struct TreeEval {
     Lane c; const double* grav; const double* fapp;
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
@@ -641,9 +778,9 @@ struct TreeEval {
     __device__ __forceinline__ void derivs(const chain::RowsT<double, false>& yr, const chain::RowsT<double, false>& fr) const {
         const SoAGet y{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
         const SoAPut f{fr.base, fr.L, fr.lane}, fu{fr.base + (size_t)c.nq * fr.L, fr.L, fr.lane};
-        sweep_kin<true>(c, y, u, f);
-        sweep_inward<IN_FWD | IN_SYSF>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
-        sweep_outward2<true>(c, y, u, fu, NoPut{});
+        sweep_kin<true, PAD>(c, y, u, f);
+        sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+        sweep_outward2<true, PAD>(c, y, u, fu, NoPut{});
     }
 };
 
@@ -957,7 +1094,7 @@ __global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     const int tr = UNI ? 0 : a.tree_of[lane];
-    const TreeEval ev(a, lane, tr);
+    const TreeEval<!UNI> ev(a, lane, tr);
     const size_t NV = (size_t)a.ny_max * a.L;
     using TRows = chain::RowsT<double, false>;
     ev.derivs(TRows{a.vec + VY*NV, a.L, (unsigned)lane}, TRows{a.vec + VF0*NV, a.L, (unsigned)lane});
@@ -1075,8 +1212,34 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
         if (a.prm_lane) k_rkm_chain<chain::PrmL, double><<<grd, blk, 0, s>>>(a);
         else k_rkm_chain<cdptr, double><<<grd, blk, 0, s>>>(a);
     }
-    else if (a.tree_of) k_rkm<TreeEval, false><<<grd, blk, 0, s>>>(a);
-    else k_rkm<TreeEval, true><<<grd, blk, 0, s>>>(a);
+    else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
+    else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+__global__ void k_lane_gather(const int* inst, int n, const double* src, double* dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) dst[l] = src[inst[l]];
+}
+__global__ void k_lane_scatter(const int* inst, int n, const double* tl, double* ti, const int* sl, int* si,
+                               const int* xl, int* xi) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int i = inst[l];
+    if (ti) ti[i] = tl[l];
+    if (si) si[i] = sl[l];
+    if (xi) for (int k = 0; k < 4; ++k) xi[4*i + k] = xl[4*l + k];
+}
+hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_lane_gather<<<dim3((n + 255)/256), dim3(256), 0, s>>>(inst, n, src_inst, dst_lane);
+    return hipGetLastError();
+}
+hipError_t launch_lane_scatter(const int* inst, int n, const double* t_lane, double* t_inst, const int* st_lane,
+                               int* st_inst, const int* stats_lane, int* stats_inst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_lane_scatter<<<dim3((n + 255)/256), dim3(256), 0, s>>>(inst, n, t_lane, t_inst, st_lane, st_inst,
+                                                              stats_lane, stats_inst);
     return hipGetLastError();
 }
 
