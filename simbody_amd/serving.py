@@ -38,6 +38,10 @@ class Server:
         self.nb = len(topology.parent)
         inf = self.info()
         self.slots, self.nq, self.nu, self.bytes_per_slot = inf["slots"], inf["nq"], inf["nu"], inf["bytes_per_slot"]
+        # host result buffers, allocated once (collect returns views of them)
+        m = self.slots
+        self._out = dict(slots=np.zeros(m, np.int32), q=np.zeros((m, self.nq)), u=np.zeros((m, self.nu)),
+                         t=np.zeros(m), status=np.zeros(m, np.int32), stats=np.zeros((m, 4), np.int32))
 
     def __del__(self):
         if getattr(self, "h", None) is not None:
@@ -78,15 +82,16 @@ class Server:
         _check(lib().tdyn_server_advance(self.h, int(max_steps), _stream()))
 
     def collect(self, max_out=None):
-        m = self.slots if max_out is None else int(max_out)
-        slots = np.zeros(m, np.int32); q = np.zeros((m, self.nq)); u = np.zeros((m, self.nu))
-        t = np.zeros(m); st = np.zeros(m, np.int32); stats = np.zeros((m, 4), np.int32)
+        """Retire finished requests.  The returned arrays are views of buffers
+        that the next collect() overwrites: copy what must be kept."""
+        m = self.slots if max_out is None else min(int(max_out), self.slots)
+        o = self._out
         k = ctypes.c_int(0)
-        _check(lib().tdyn_server_collect(self.h, m, slots.ctypes.data_as(_i32), _dp(q), _dp(u), _dp(t),
-                                         st.ctypes.data_as(_i32), stats.ctypes.data_as(_i32), ctypes.byref(k),
-                                         _stream()))
+        _check(lib().tdyn_server_collect(self.h, m, o["slots"].ctypes.data_as(_i32), _dp(o["q"]), _dp(o["u"]),
+                                         _dp(o["t"]), o["status"].ctypes.data_as(_i32),
+                                         o["stats"].ctypes.data_as(_i32), ctypes.byref(k), _stream()))
         k = k.value
-        return dict(slots=slots[:k], q=q[:k], u=u[:k], t=t[:k], status=st[:k], stats=stats[:k])
+        return {key: v[:k] for key, v in o.items()}
 
 
 class Requests:
@@ -150,7 +155,7 @@ class Requests:
         return slots[:kk.value]
 
 
-def serve(server, req, arrivals, steps_per_advance=4, time_limit=None):
+def serve(server, req, arrivals, steps_per_advance=16, time_limit=None):
     """Run the request loop over `req` (a Requests), whose arrival times
     (seconds from the start, sorted) are `arrivals`.  Requests are admitted
     first come, first served.  Returns per-request results (q, u, status,

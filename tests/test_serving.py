@@ -55,7 +55,8 @@ def _run_all(sv, specs, q, u, t_end, F=None, steps=3):
         sv.advance(steps)
         r = sv.collect()
         for j, s in enumerate(r["slots"]):
-            out[slot_req.pop(int(s))] = (r["q"][j], r["u"][j], r["t"][j], r["status"][j], r["stats"][j])
+            out[slot_req.pop(int(s))] = (r["q"][j].copy(), r["u"][j].copy(), r["t"][j], r["status"][j],
+                                         r["stats"][j].copy())
     return out
 
 
