@@ -45,6 +45,17 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     return TDYN_OK;
 }
 
+bool identity_frames(const tdyn_tree_desc& d) {
+    static const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    for (int b = 0; b < d.nb; ++b) {
+        for (int k = 0; k < 9; ++k)
+            if (d.X_PF[12*b + k] != I[k] || d.X_BM[12*b + k] != I[k]) return false;
+        for (int k = 9; k < 12; ++k)
+            if (d.X_PF[12*b + k] != 0.0) return false;
+    }
+    return true;
+}
+
 void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
     for (int k = 0; k < PRM_STRIDE; ++k) P[k] = 0.0;
     for (int k = 0; k < 9; ++k) P[PRM_R_PF + k] = d.X_PF[12*(b-1) + k];
@@ -170,6 +181,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
     B->total_q = q; B->total_u = u; B->total_b = bb;
     B->is_chain = B->is_chain && B->uniform;
+    B->idf = B->is_chain && identity_frames(trees[0]);
     // Lane order.  A wave runs as long as its slowest lane, so instances of
     // different trees are assigned to lanes in order of decreasing work
     // (bodies weighted by their mobilizer's cost): the 64 lanes of a wave

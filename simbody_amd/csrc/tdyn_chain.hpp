@@ -67,6 +67,22 @@ struct PrmL {
     }
     __device__ __forceinline__ PrmL operator+(size_t k) const { return {p + k * (size_t)L, L, lane}; }
 };
+// A model whose frames are all identity rotations at the joints (R_PF = I,
+// p_PF = 0, R_MB = I; X_BM may translate), Simbody's noR_PF/noX_MB node
+// specializations: the products with those frames are skipped.  They are
+// exact in floating point (x*1 + y*0 + z*0 = x), so results are bitwise
+// those of the general form.
+template <class PA> struct IdF {
+    PA p;
+    __device__ __forceinline__ IdF() = default;
+    __device__ __forceinline__ explicit IdF(PA q) : p(q) {}
+    __device__ __forceinline__ double operator[](int k) const { return p[k]; }
+    __device__ __forceinline__ IdF operator+(size_t k) const { return IdF(p + k); }
+};
+template <class PA> struct PrmTraits { static constexpr bool lane = false, idf = false; };
+template <> struct PrmTraits<PrmL> { static constexpr bool lane = true, idf = false; };
+template <class T> struct PrmTraits<IdF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = true; };
+
 template <class K, class PA> __device__ __forceinline__ m33<K> ldm(const PA& p) {
     m33<K> r; for (int i = 0; i < 9; ++i) r.a[i] = K(p[i]); return r;
 }
@@ -141,19 +157,26 @@ template <int TYPE, class K> struct Joint {
 template <class K, class PA>
 __device__ __forceinline__ void body_in_parent(const PA& P, const m33<K>& R_FM, const v3<K>& p_FM, m33<K>& R_PB,
                                                v3<K>& p_PB) {
-    const m33<K> R_PF = ldm<K>(P + PRM_R_PF), R_MB = ldm<K>(P + PRM_R_MB);
-    const v3<K> p_PF = ld3<K>(P + PRM_P_PF), p_MB = ld3<K>(P + PRM_P_MB);
-    const m33<K> R_FB = mul(R_FM, R_MB);
-    const v3<K> p_FB = p_FM + mul(R_FM, p_MB);
-    R_PB = mul(R_PF, R_FB);
-    p_PB = p_PF + mul(R_PF, p_FB);
+    const v3<K> p_MB = ld3<K>(P + PRM_P_MB);
+    if constexpr (PrmTraits<PA>::idf) {
+        R_PB = R_FM;
+        p_PB = p_FM + mul(R_FM, p_MB);
+    } else {
+        const m33<K> R_PF = ldm<K>(P + PRM_R_PF), R_MB = ldm<K>(P + PRM_R_MB);
+        const v3<K> p_PF = ld3<K>(P + PRM_P_PF);
+        const m33<K> R_FB = mul(R_FM, R_MB);
+        const v3<K> p_FB = p_FM + mul(R_FM, p_MB);
+        R_PB = mul(R_PF, R_FB);
+        p_PB = p_PF + mul(R_PF, p_FB);
+    }
 }
 
 // H_PB_G = R_GF (H_FM + H_MB_F)  (RigidBodyNodeSpec.cpp:44-74)
 template <int TYPE, class K, class PA>
 __device__ __forceinline__ void hinge(const PA& P, const Joint<TYPE, K>& J, const m33<K>& R_GP, m33<K>& R_GF,
                                       v3<K>& r_MB_F, sv<K>* H) {
-    R_GF = mul(R_GP, ldm<K>(P + PRM_R_PF));
+    if constexpr (PrmTraits<PA>::idf) R_GF = R_GP;
+    else R_GF = mul(R_GP, ldm<K>(P + PRM_R_PF));
     r_MB_F = mul(J.R_FM, ld3<K>(P + PRM_P_MB));
 #pragma unroll
     for (int j = 0; j < MobC<TYPE>::D; ++j)
@@ -522,13 +545,14 @@ struct ChainEval {
         sc = {a.scratch, a.L, (unsigned)lane};
         if constexpr (MIXED) sk = {a.scratch32, a.L, (unsigned)lane};
         else sk = sc;
-        if constexpr (std::is_same<PA, PrmL>::value) {
+        if constexpr (PrmTraits<PA>::lane) {
             // per-lane model: nb*PRM_STRIDE rows, then 3 gravity rows
-            prm = PrmL{a.prm_lane, a.L, (unsigned)lane};
-            const PrmL g = prm + (size_t)nb * PRM_STRIDE;
+            const PrmL pl{a.prm_lane, a.L, (unsigned)lane};
+            prm = PA(pl);
+            const PrmL g = pl + (size_t)nb * PRM_STRIDE;
             grav = V3(K(g[0]), K(g[1]), K(g[2]));
         } else {
-            prm = cptr(a.T.prm) + (size_t)ta.w * PRM_STRIDE;
+            prm = PA(cptr(a.T.prm) + (size_t)ta.w * PRM_STRIDE);
             const cdptr g = cptr(a.T.gravity) + 4*tr;
             grav = V3(K(g[0]), K(g[1]), K(g[2]));
         }

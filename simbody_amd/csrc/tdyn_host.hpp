@@ -38,6 +38,8 @@ template <class T> struct DBuf {
 // The PRM_STRIDE model constants of body b (1-based) of tree d: X_PF, X_MB =
 // X_BM^-1, mass, mass center, unit inertia G = I/m about Bo, p_BM.
 void pack_body_params(const tdyn_tree_desc& d, int b, double* P);
+// True when every body's X_PF is the identity and X_BM has no rotation.
+bool identity_frames(const tdyn_tree_desc& d);
 // Validates one tree description (parents before children, known mobilizers,
 // Free only on Ground, positive finite masses, finite frames).
 int validate_tree(const tdyn_tree_desc& d, int t);
@@ -68,6 +70,7 @@ struct tdyn_batch {
     int ny_max = 0;
     bool rollout_ready = false;
     bool is_chain = false;       // uniform batch whose tree is a serial chain
+    bool idf = false;            // tree 0's joint frames are identity rotations (R_PF = I, p_PF = 0, R_BM = I)
     int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
     RkmCfg cfg{};
 
@@ -92,6 +95,7 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
+        a.idf = idf ? 1 : 0;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         return a;
     }

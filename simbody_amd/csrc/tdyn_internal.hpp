@@ -112,6 +112,8 @@ struct RkmArgs {
     RkmCfg cfg;
     int precision;           // 0 = FP64, 1 = mixed
     int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
+    int idf;                 // chain evaluator: every frame is an identity rotation at the joints
+                             // (R_PF = I, p_PF = 0, R_MB = I), so the frame products are skipped
     const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
                              // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
 };

@@ -1102,6 +1102,8 @@ __global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
+    // the derivative is the general form's bit for bit, so identity frames
+    // need no separate build here
     if (a.chain && a.precision == 1) {
         if (a.prm_lane) k_chain_f0<chain::PrmL, float><<<grd, blk, 0, s>>>(a);
         else k_chain_f0<cdptr, float><<<grd, blk, 0, s>>>(a);
@@ -1205,12 +1207,19 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
-    if (a.chain && a.precision == 1) {
-        if (a.prm_lane) k_rkm_chain<chain::PrmL, float><<<grd, blk, 0, s>>>(a);
-        else k_rkm_chain<cdptr, float><<<grd, blk, 0, s>>>(a);
-    } else if (a.chain) {
-        if (a.prm_lane) k_rkm_chain<chain::PrmL, double><<<grd, blk, 0, s>>>(a);
-        else k_rkm_chain<cdptr, double><<<grd, blk, 0, s>>>(a);
+    using chain::PrmL; using chain::IdF;
+    if (a.chain) {
+        const int v = (a.prm_lane ? 1 : 0) | (a.precision == 1 ? 2 : 0) | (a.idf ? 4 : 0);
+        switch (v) {
+        case 0: k_rkm_chain<cdptr, double><<<grd, blk, 0, s>>>(a); break;
+        case 1: k_rkm_chain<PrmL, double><<<grd, blk, 0, s>>>(a); break;
+        case 2: k_rkm_chain<cdptr, float><<<grd, blk, 0, s>>>(a); break;
+        case 3: k_rkm_chain<PrmL, float><<<grd, blk, 0, s>>>(a); break;
+        case 4: k_rkm_chain<IdF<cdptr>, double><<<grd, blk, 0, s>>>(a); break;
+        case 5: k_rkm_chain<IdF<PrmL>, double><<<grd, blk, 0, s>>>(a); break;
+        case 6: k_rkm_chain<IdF<cdptr>, float><<<grd, blk, 0, s>>>(a); break;
+        default: k_rkm_chain<IdF<PrmL>, float><<<grd, blk, 0, s>>>(a); break;
+        }
     }
     else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);

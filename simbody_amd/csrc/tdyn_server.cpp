@@ -38,12 +38,14 @@ struct tdyn_server {
     Pinned<int> slot_host, outi_host, status_host;
     std::vector<int> free_list;
     std::vector<char> occupied;
+    bool idf = false;      // every model admitted so far has identity joint frames
     ~tdyn_server() { if (B) tdyn_batch_destroy(B); }
     int rec_stride() const { return nb * PRM_STRIDE + 3 + nq + nu + 1 + 6 * (nb + 1); }
     RkmArgs rk() const {
         RkmArgs a = B->rk();
         a.chain = 1;
         a.prm_lane = prm_lane.p;
+        a.idf = idf ? 1 : 0;
         return a;
     }
 };
@@ -97,6 +99,7 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
         delete S;
         return hipfail(e, "tdyn_server_create: device allocation");
     }
+    S->idf = identity_frames(*topology);
     S->free_list.resize(n);
     for (size_t i = 0; i < n; ++i) S->free_list[i] = (int)(n - 1 - i);    // pop_back gives slot 0 first
     S->occupied.assign(n, 0);
@@ -130,6 +133,9 @@ int tdyn_server_admit(tdyn_server* sv, int n, const tdyn_tree_desc* trees, const
         if (rc) return rc;
         if (!std::isfinite(t_end[i])) return fail(TDYN_E_ARG, "tdyn_server_admit: non-finite t_end");
     }
+    // a model with general frames switches the server to the general kernel for good
+    for (int i = 0; i < std::min<int>(n, (int)sv->free_list.size()); ++i)
+        if (sv->idf && !identity_frames(trees[i])) sv->idf = false;
     const int k = std::min<int>(n, (int)sv->free_list.size());
     if (k == 0) return TDYN_OK;
     const int stride = sv->rec_stride();
