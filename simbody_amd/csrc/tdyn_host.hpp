@@ -84,14 +84,14 @@ struct tdyn_batch {
         OpArgs a;
         std::memset(&a, 0, sizeof(a));
         a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
-        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p}; a.inst = permuted ? inst.p : nullptr;
+        a.scratch = scratch.p; a.rec_stride = max_rec + 24; a.off = {q_off.p, u_off.p, b_off.p}; a.inst = permuted ? inst.p : nullptr;
         return a;
     }
     RkmArgs rk() const {
         RkmArgs a;
         std::memset(&a, 0, sizeof(a));
         a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
-        a.scratch = scratch.p; a.off = {q_off.p, u_off.p, b_off.p};
+        a.scratch = scratch.p; a.rec_stride = max_rec + 24; a.off = {q_off.p, u_off.p, b_off.p};
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;

@@ -75,6 +75,7 @@ struct OpArgs {
     int n_inst;
     int L;                   // lane pitch of the scratch
     double* scratch;
+    int rec_stride;          // doubles of scratch per lane (per-lane record layout of mixed-tree batches)
     InstOffsets off;
     const double* q;         // packed boundary layout
     const double* u;
@@ -98,6 +99,7 @@ struct RkmArgs {
     const int* tree_of;
     int n_inst, L;
     double* scratch;
+    int rec_stride;          // doubles of scratch per lane (per-lane record layout of mixed-tree batches)
     float* scratch32;        // mixed path: float hand-over rows (6*nu G rows, then 4*nb trig rows) per lane
     InstOffsets off;
     const double* fapp;      // constant applied body forces, packed, nullable
@@ -117,7 +119,21 @@ struct RkmArgs {
     const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
                              // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
 };
+// Rollout state vectors (rows of ny_max * L doubles in RkmArgs::vec): y, y0,
+// f0, fa, fb, ysave, y1, ystage, yerr and the mixed path's Kahan rows.
+enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
+
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
+// Chain-evaluator rollouts (f0_only: only the derivative of the current state)
+hipError_t launch_chain_fp64_batch(const RkmArgs& a, hipStream_t s, bool f0_only);
+hipError_t launch_chain_fp64_lane(const RkmArgs& a, hipStream_t s, bool f0_only);
+hipError_t launch_chain_mixed_batch(const RkmArgs& a, hipStream_t s, bool f0_only);
+hipError_t launch_chain_mixed_lane(const RkmArgs& a, hipStream_t s, bool f0_only);
+inline hipError_t launch_chain_variant(const RkmArgs& a, hipStream_t s, bool f0_only) {
+    if (a.precision == 1)
+        return a.prm_lane ? launch_chain_mixed_lane(a, s, f0_only) : launch_chain_mixed_batch(a, s, f0_only);
+    return a.prm_lane ? launch_chain_fp64_lane(a, s, f0_only) : launch_chain_fp64_batch(a, s, f0_only);
+}
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
 hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
                             double t0, hipStream_t s);

@@ -21,27 +21,37 @@ using ds = s33<double>;
 using dv = sv<double>;
 using da = abi<double>;
 
+// One lane's view of its instance.  Record storage: structure-of-arrays
+// (row off of every lane contiguous: scr = scratch + lane, rs = L) when all
+// lanes walk the same tree, so equal offsets coalesce; per-lane records
+// (scr = scratch + lane * record stride, rs = 1) when lanes walk different
+// trees, so that reading a parent's or a child's record, whose offset differs
+// from lane to lane, touches a few cache lines per lane instead of one per
+// value.
 struct Lane {
     DevTables T;
     int lane, L, nb, nq, nu, body_off;
-    double* scr;
-    __device__ double& s(int off) const { return (scr + (size_t)off * L)[(unsigned)lane]; }
+    double* scr; int rs;
+    __device__ double& s(int off) const { return scr[(size_t)off * rs]; }
     __device__ int4 ba(int b) const { return ldi4(T.body_a, body_off + b - 1); }
     __device__ int4 bb(int b) const { return ldi4(T.body_b, body_off + b - 1); }
     __device__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
     __device__ int rec(int b) const { return ldi4(T.body_b, body_off + b - 1).z; }
 };
 
-__device__ __forceinline__ Lane make_lane_t(const DevTables& T, int tr, int lane, int L, double* scr) {
+__device__ __forceinline__ Lane make_lane_t(const DevTables& T, int tr, int lane, int L, double* scr,
+                                           int per_lane_stride = 0) {
     Lane c;
-    c.T = T; c.lane = lane; c.L = L; c.scr = scr;
+    c.T = T; c.lane = lane; c.L = L;
+    if (per_lane_stride > 0) { c.scr = scr + (size_t)lane * per_lane_stride; c.rs = 1; }
+    else { c.scr = scr + lane; c.rs = L; }
     const int4 ta = ldi4(T.tree_a, tr);
     c.nb = ta.x; c.nq = ta.y; c.nu = ta.z; c.body_off = ta.w;
     return c;
 }
 __device__ __forceinline__ Lane make_lane(const DevTables& T, const int* tree_of, int lane, int L, double* scr) {
     Lane c;
-    c.T = T; c.lane = lane; c.L = L; c.scr = scr;
+    c.T = T; c.lane = lane; c.L = L; c.scr = scr + lane; c.rs = L;
     const int tr = tree_of ? tree_of[lane] : 0;
     const int4 ta = T.tree_a[tr];
     c.nb = ta.x; c.nq = ta.y; c.nu = ta.z; c.body_off = ta.w;
@@ -629,7 +639,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     const int tr = UNI ? 0 : a.tree_of[lane];
-    const Lane c = make_lane_t(a.T, tr, lane, a.L, a.scratch);
+    const Lane c = make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0);
     const double* grav = a.T.gravity + 4*tr;
     const int qo = a.off.q_off[lane], uo = a.off.u_off[lane], bo = a.off.b_off[lane];
     const PackedGet q{a.q + qo};
@@ -766,14 +776,13 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // projection).  Lanes stop after cfg.max_steps accepted steps so a rollout can
 // be resumed by the next launch (the serving loop relies on this).
 // ---------------------------------------------------------------------------
-enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
 
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 template <bool PAD>
 struct TreeEval {
     Lane c; const double* grav; const double* fapp;
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
-        : c(make_lane_t(a.T, tr, lane, a.L, a.scratch)), grav(a.T.gravity + 4*tr),
+        : c(make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0)), grav(a.T.gravity + 4*tr),
           fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr) {}
     __device__ __forceinline__ void derivs(const chain::RowsT<double, false>& yr, const chain::RowsT<double, false>& fr) const {
         const SoAGet y{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
@@ -947,148 +956,6 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
 }
 
-// adjustStepSize (AbstractIntegratorRep.cpp:448-502).  Returns success.
-__device__ __forceinline__ bool adjust_step(double err, double acc, bool limited, double hmin, double hmax,
-                                            double& hcur) {
-    double hnew;
-    if (!isfinite(err)) hnew = 0.1*hcur;
-    else if (err == 0) hnew = 5.0*hcur;
-    else hnew = 0.9*hcur*pow(acc/err, 0.25);
-    if (hnew > hcur) { if (limited || hnew < 1.2*hcur) hnew = hcur; }
-    if (hnew < hcur) { if (err <= acc) hnew = hcur; else hnew = fmin(hnew, 0.9*hcur); }
-    hnew = fmin(hnew, 5.0*hcur);
-    hnew = fmax(hnew, 0.1*hcur);
-    if (hmin != -1) hnew = fmax(hnew, hmin);
-    if (hmax != -1) hnew = fmin(hnew, hmax);
-    const bool ok = hnew >= hcur;
-    hcur = hnew;
-    return ok;
-}
-
-// Rollouts of a uniform serial-chain batch with the Merson stages fused into
-// the chain evaluator's sweeps (no separate vector passes in the common
-// path).  The state ping-pongs between VY and VY1; a lane whose current state
-// sits in VY1 when its launch ends copies it back to VY.
-template <class PA, class K>
-__global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
-    using namespace chain;
-    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.n_inst) return;
-    if (a.status[lane] != 0) return;
-    const ChainEval<PA, K> ev(a, lane, 0);
-    const int4 ta = ldi4(a.T.tree_a, 0);
-    const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
-    const int4* body_a = a.T.body_a + ta.w;
-    const int nquat = ldi4(a.T.tree_b, 0).y;
-    const size_t NV = (size_t)a.ny_max * a.L;
-    const int L = a.L;
-    auto rows = [&](int v) { return Rows{a.vec + v*NV, L, (unsigned)lane}; };
-    const Rows C0 = rows(VC0), C1 = rows(VC1);
-    const Rows S0 = rows(VY), S1 = rows(VY1), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
-               YT = rows(VYT), YE = rows(VYE);
-
-    const RkmCfg& cfg = a.cfg;
-    const double acc = cfg.accuracy, consTol = cfg.cons_tol;
-    double hmin = cfg.min_h, hmax = cfg.max_h;
-    if (cfg.fixed_h > 0) { hmin = hmax = cfg.fixed_h; }
-    const double projLimit = fmax(2*consTol, sqrt(consTol));
-    double t = a.t[lane], hcur = a.h[lane];
-    const double tEnd = a.t_end[lane];
-    int attempts = a.stats[4*lane], accepted = a.stats[4*lane+1], rejected = a.stats[4*lane+2],
-        projections = a.stats[4*lane+3];
-    int status = 0, steps = 0;
-    bool flip = false;     // current state in S1
-    NormSums ns0;
-
-    while (steps < cfg.max_steps) {
-        if (t >= tEnd) { status = 1; break; }
-        // Every active lane has taken the same number of accepted steps in
-        // this launch, so flip is wave-uniform; saying so keeps the row
-        // bases in scalar registers.
-        const bool fl = __builtin_amdgcn_readfirstlane(flip ? 1 : 0) != 0;
-        const Rows Y = fl ? S1 : S0, Y1 = fl ? S0 : S1;
-        RkRows r{Y, F0, FA, FB, YT, F0, Y1, YE, fl ? C1 : C0, fl ? C0 : C1};
-        ev.template derivs_rk<0>(r, 0.0, ns0);
-        bool ok = false;
-        double t1 = t;
-        do {
-            bool limited = false;
-            if (tEnd < t + 0.95*hcur) { limited = true; t1 = tEnd; }
-            else if (tEnd > t + 1.001*hcur) t1 = t + hcur;
-            else t1 = tEnd;
-            if (!(t1 > t)) { status = -3; break; }
-            if (++attempts > cfg.max_attempts) { status = -2; break; }
-            const double h = t1 - t;
-            r.st = YT; r.out = FA; ev.template derivs_rk<1>(r, h, ns0);
-            ev.template derivs_rk<2>(r, h, ns0);
-            r.out = FB; ev.template derivs_rk<3>(r, h, ns0);
-            NormSums ns{0.0, 0.0, 0.0, 0.0};
-            r.st = YS; r.out = FA; ev.template derivs_rk<4>(r, h, ns);
-            const double qn = nq ? sqrt((ns.sq_other + ns.sq_quat) / nq) : 0.0;
-            const double un = nu ? sqrt(ns.su / nu) : 0.0;
-            const double e1 = qn >= un ? qn : un;
-            double err = e1;
-            bool converged = true;
-            if (!(e1 > 16.0*acc) && nquat > 0) {
-                const double qe = sqrt(ns.s2 / nquat);
-                if (qe > projLimit) converged = false;
-                else if (qe > consTol) {
-                    // normalizeQuaternions (SimbodyMatterSubsystemRep.cpp:4448-4475) and
-                    // the re-evaluated error norm of the projected estimate
-                    ++projections;
-                    double sqq = 0;
-                    for (int b = 0; b < nb; ++b) {
-                        const int4 ba = ldi4(body_a, b);
-                        if (ba.y != BALL && ba.y != FREE) continue;
-                        double qq[4], n2 = 0, e[4], d4[4];
-                        for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
-                        const double n = sqrt(n2);
-                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; Y1(ba.z + k) = qq[k]; }
-                        double d = 0;
-                        for (int k = 0; k < 4; ++k) d += YE(ba.z + k) * qq[k];
-                        for (int k = 0; k < 4; ++k) { e[k] = YE(ba.z + k) - d*qq[k]; YE(ba.z + k) = e[k]; }
-                        const d3 w = quat_dot_to_w(qq, e);
-                        quat_dot(qq, w, d4);
-                        sqq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
-                    }
-                    const double qn2 = nq ? sqrt((ns.sq_other + sqq) / nq) : 0.0;
-                    err = qn2 >= un ? qn2 : un;
-                }
-            }
-            if (!converged) err = __builtin_inf();
-            ok = adjust_step(err, acc, limited, hmin, hmax, hcur);
-            if (!ok) ++rejected;
-        } while (!ok);
-        if (status < 0) break;
-        t = t1;
-        flip = !flip;
-        ++accepted; ++steps;
-    }
-    if (flip) {
-        for (int k = 0; k < ny; ++k) S0(k) = S1(k);
-        if constexpr (!std::is_same<K, double>::value)
-            for (int k = 0; k < ny; ++k) C0(k) = C1(k);
-    }
-    if (status == 0 && t >= tEnd) status = 1;
-    a.t[lane] = t; a.h[lane] = hcur; a.status[lane] = status;
-    a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
-}
-
-// f = f(y) of the current rollout state (vector VY) into VF0, by the
-// rollout's evaluator at its precision: the derivative the integrator uses.
-template <class PA, class K>
-__global__ void __launch_bounds__(64) k_chain_f0(RkmArgs a) {
-    using namespace chain;
-    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= a.n_inst) return;
-    const ChainEval<PA, K> ev(a, lane, 0);
-    const size_t NV = (size_t)a.ny_max * a.L;
-    auto rows = [&](int v) { return Rows{a.vec + v*NV, a.L, (unsigned)lane}; };
-    const RkRows r{rows(VY), rows(VF0), rows(VFA), rows(VFB), rows(VYT), rows(VF0), rows(VY1), rows(VYE),
-                   rows(VC0), rows(VC1)};
-    NormSums ns{0.0, 0.0, 0.0, 0.0};
-    ev.template derivs_rk<0>(r, 0.0, ns);
-}
 template <bool UNI>
 __global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1104,13 +971,8 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     const dim3 grd((a.n_inst + 63)/64), blk(64);
     // the derivative is the general form's bit for bit, so identity frames
     // need no separate build here
-    if (a.chain && a.precision == 1) {
-        if (a.prm_lane) k_chain_f0<chain::PrmL, float><<<grd, blk, 0, s>>>(a);
-        else k_chain_f0<cdptr, float><<<grd, blk, 0, s>>>(a);
-    } else if (a.chain) {
-        if (a.prm_lane) k_chain_f0<chain::PrmL, double><<<grd, blk, 0, s>>>(a);
-        else k_chain_f0<cdptr, double><<<grd, blk, 0, s>>>(a);
-    } else if (a.tree_of) k_tree_f0<false><<<grd, blk, 0, s>>>(a);
+    if (a.chain) return launch_chain_variant(a, s, true);
+    if (a.tree_of) k_tree_f0<false><<<grd, blk, 0, s>>>(a);
     else k_tree_f0<true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
@@ -1207,21 +1069,8 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
-    using chain::PrmL; using chain::IdF;
-    if (a.chain) {
-        const int v = (a.prm_lane ? 1 : 0) | (a.precision == 1 ? 2 : 0) | (a.idf ? 4 : 0);
-        switch (v) {
-        case 0: k_rkm_chain<cdptr, double><<<grd, blk, 0, s>>>(a); break;
-        case 1: k_rkm_chain<PrmL, double><<<grd, blk, 0, s>>>(a); break;
-        case 2: k_rkm_chain<cdptr, float><<<grd, blk, 0, s>>>(a); break;
-        case 3: k_rkm_chain<PrmL, float><<<grd, blk, 0, s>>>(a); break;
-        case 4: k_rkm_chain<IdF<cdptr>, double><<<grd, blk, 0, s>>>(a); break;
-        case 5: k_rkm_chain<IdF<PrmL>, double><<<grd, blk, 0, s>>>(a); break;
-        case 6: k_rkm_chain<IdF<cdptr>, float><<<grd, blk, 0, s>>>(a); break;
-        default: k_rkm_chain<IdF<PrmL>, float><<<grd, blk, 0, s>>>(a); break;
-        }
-    }
-    else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
+    if (a.chain) return launch_chain_variant(a, s, false);
+    if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
