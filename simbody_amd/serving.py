@@ -78,6 +78,11 @@ class Server:
                                        slots.ctypes.data_as(_i32), ctypes.byref(k), _stream()))
         return slots[:k.value]
 
+    def admit_requests(self, req, i0, k):
+        """Admit requests i0..i0+k-1 of a Requests (as many as there are free
+        slots); returns their slots."""
+        return req.admit_range(self, i0, k)
+
     def advance(self, max_steps):
         _check(lib().tdyn_server_advance(self.h, int(max_steps), _stream()))
 
@@ -178,7 +183,7 @@ def serve(server, req, arrivals, steps_per_advance=16, time_limit=None):
         nxt = int(np.searchsorted(arrivals, now, side="right"))
         if nxt > head and free > 0:
             k = min(free, nxt - head)
-            slots = req.admit_range(server, head, k)
+            slots = server.admit_requests(req, head, k)
             assert len(slots) == k
             slot_req[slots] = np.arange(head, head + k)
             admitted_at[head:head + k] = time.perf_counter() - t0
