@@ -135,7 +135,8 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
                        "accuracy 1e-6 to t=0.5 s, N(0,1) body forces",
            "value": body_steps / sec, "unit": "body-steps/s", "seconds": sec,
            "body_steps": body_steps, "accepted_mean": float(st[:, 1].mean()),
-           "rejected_total": int(st[:, 2].sum()), "evaluator": "tree sweeps (k_rkm)"}
+           "rejected_total": int(st[:, 2].sum()),
+           "evaluator": {1: "tree sweeps", 2: "chain", 3: "streaming tree evaluator"}.get(B.active_evaluator(), "?")}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle_ctypes import OracleModel, time_integrate_batch

@@ -79,9 +79,15 @@ int tdyn_batch_destroy(tdyn_batch* b);
 
 /* Derivative evaluator of the rollouts (no reference counterpart; a tuning
  * knob): 0 = automatic (the register-resident serial-chain evaluator when
- * every instance shares one chain model, else the tree sweeps), 1 = tree
- * sweeps, 2 = chain evaluator (TDYN_E_ARG unless the batch is one chain model). */
+ * every instance shares one chain model; the streaming tree evaluator for a
+ * batch of different trees whose only Free body, if any, is body 1; else the
+ * tree sweeps), 1 = tree sweeps, 2 = chain evaluator (TDYN_E_ARG unless the
+ * batch is one chain model).  Takes effect at the next tdyn_rollout_begin. */
 int tdyn_batch_set_evaluator(tdyn_batch* b, int mode);
+/* The evaluator of the current rollout (no reference counterpart; for tests
+ * and tuning): *active = 0 before the first tdyn_rollout_begin, else 1 tree
+ * sweeps, 2 chain evaluator, 3 streaming tree evaluator. */
+int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active);
 
 /* info[0..7] = n_instances, total q, total u, total bodies (incl. Ground),
  * max nq, max nu, max nb, n_trees. */

@@ -134,7 +134,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     auto* B = new tdyn_batch;
     B->n_inst = n_instances; B->n_trees = n_trees;
     B->uniform = (tree_of_instance == nullptr) || n_trees == 1;
-    std::vector<int4> body_b; std::vector<int> child_idx; std::vector<double> prm, grav;
+    std::vector<int4> body_b, pre; std::vector<int> child_idx; std::vector<double> prm, grav;
+    B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
     int body_off = 0;
     for (int t = 0; t < n_trees; ++t) {
@@ -157,6 +158,19 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             double P[PRM_STRIDE];
             pack_body_params(d, b, P);
             prm.insert(prm.end(), P, P + PRM_STRIDE);
+        }
+        // depth-first preorder (children in increasing order): body, depth, child count
+        {
+            std::vector<int> depth(nb + 1, 0), stack(ch[0].rbegin(), ch[0].rend());
+            while (!stack.empty()) {
+                const int b = stack.back(); stack.pop_back();
+                depth[b] = depth[d.parent[b-1]] + 1;
+                B->dmax = std::max(B->dmax, depth[b]);
+                pre.push_back(make_int4(b, depth[b], (int)ch[b].size(), 0));
+                stack.insert(stack.end(), ch[b].rbegin(), ch[b].rend());
+            }
+            for (int b = 2; b <= nb; ++b)
+                if (d.mobilizer[b-1] == TDYN_FREE) B->stream_ok = false;   // a Free body must come first
         }
         for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
         grav.push_back(0.0);
@@ -181,6 +195,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
     B->total_q = q; B->total_u = u; B->total_b = bb;
     B->is_chain = B->is_chain && B->uniform;
+    B->stream_stride = tdyn::tstream::scratch_doubles(B->dmax, B->max_nb);
     B->idf = B->is_chain && identity_frames(trees[0]);
     // Lane order.  A wave runs as long as its slowest lane, so instances of
     // different trees are assigned to lanes in order of decreasing work
@@ -217,7 +232,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->u_off.upload(lane_u)) != hipSuccess ||
         (e = B->b_off.upload(lane_b)) != hipSuccess ||
         (e = B->inst.upload(inst_of_lane)) != hipSuccess ||
-        (e = B->scratch.alloc((size_t)(B->max_rec + 24) * n_instances)) != hipSuccess) {
+        (e = B->pre.upload(pre)) != hipSuccess ||
+        (e = B->scratch.alloc((size_t)std::max(B->max_rec + 24, B->stream_ok ? B->stream_stride : 0) * n_instances)) != hipSuccess) {
         delete B;
         return hipfail(e, "tdyn_batch_create: device allocation/upload");
     }
@@ -232,6 +248,14 @@ int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
     if (mode < 0 || mode > 2) return fail(TDYN_E_ARG, "evaluator mode must be 0, 1 or 2");
     if (mode == 2 && !b->is_chain) return fail(TDYN_E_ARG, "chain evaluator needs one serial-chain model for the whole batch");
     b->evaluator = mode;
+    return TDYN_OK;
+}
+
+int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active) {
+    if (!b || !active) return fail(TDYN_E_ARG, "null argument");
+    if (!b->rollout_ready) { *active = 0; return TDYN_OK; }
+    const RkmArgs a = b->rk();
+    *active = a.chain ? 2 : a.vec_aos ? 3 : 1;
     return TDYN_OK;
 }
 
@@ -327,6 +351,7 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
     const size_t L = b->n_inst;
+    b->rollout_aos = b->stream_ok && b->evaluator == 0;
     { const int rc = ensure_rollout_buffers(b); if (rc) return rc; }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));

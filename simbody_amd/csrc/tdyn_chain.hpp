@@ -37,9 +37,13 @@ using ds = s33<double>;
 using dv = sv<double>;
 using da = abi<double>;
 
+// TYPE = GEN: a padded generic body (trees of different shapes in one wave):
+// any non-Free mobilizer carried as 3 dof, the type chosen at run time; its
+// unused H columns, u entries and D rows are zero and D is padded with ones
+// on the diagonal, so every padded term adds an exact zero.
 template <int TYPE> struct MobC {
-    static constexpr int D = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 3 : 6;
-    static constexpr int NQ = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : TYPE == BALL ? 4 : 7;
+    static constexpr int D = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : (TYPE == BALL || TYPE == GEN) ? 3 : 6;
+    static constexpr int NQ = TYPE <= SLIDER ? 1 : TYPE == UNIVERSAL ? 2 : (TYPE == BALL || TYPE == GEN) ? 4 : 7;
 };
 
 #define TD_GLOBAL __attribute__((address_space(1)))
@@ -110,14 +114,45 @@ template <class K> __device__ __forceinline__ void sincosk(double q, K* s, K* c)
 template <int TYPE, class K> struct Joint {
     static constexpr int D = MobC<TYPE>::D;
     m33<K> R_FM; v3<K> p_FM; sv<K> H_FM[D];
+    int ty = TYPE, d = D;       // run-time type and dof of a GEN body
     // tc: cached trig / normalization values of this q (written by the first
     // sweep with CACHED = false): Pin (cos,sin), Universal (c1,s1,c2,s2),
     // Ball/Free 1/|quaternion| (computed in FP64).
     template <bool CACHED>
-    __device__ __forceinline__ void set(const double* qv, K* tc) {
+    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE) {
         const K z = K(0), o = K(1);
         p_FM = zero3<K>();
-        if constexpr (TYPE == PIN) {
+        if constexpr (TYPE == GEN) {
+            ty = rty; d = mob_dof(rty);
+            H_FM[0] = H_FM[1] = H_FM[2] = svzero<K>();
+            if (rty == PIN) {
+                K sn, cs;
+                if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
+                R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
+                R_FM.a[6]=z;  R_FM.a[7]=z;   R_FM.a[8]=o;
+                H_FM[0] = {V3(z, z, o), zero3<K>()};
+            } else if (rty == SLIDER) {
+                R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
+                H_FM[0] = {zero3<K>(), V3(o, z, z)};
+            } else if (rty == UNIVERSAL) {
+                K s1, c1, s2, c2;
+                if constexpr (CACHED) { c1 = tc[0]; s1 = tc[1]; c2 = tc[2]; s2 = tc[3]; }
+                else { sincosk<K>(qv[0], &s1, &c1); sincosk<K>(qv[1], &s2, &c2); tc[0] = c1; tc[1] = s1; tc[2] = c2; tc[3] = s2; }
+                R_FM.a[0]= c2;    R_FM.a[1]= z;  R_FM.a[2]= s2;
+                R_FM.a[3]= s2*s1; R_FM.a[4]= c1; R_FM.a[5]= -s1*c2;
+                R_FM.a[6]=-s2*c1; R_FM.a[7]= s1; R_FM.a[8]= c1*c2;
+                H_FM[0] = {V3(o, z, z), zero3<K>()};
+                H_FM[1] = {col(R_FM, 1), zero3<K>()};
+            } else {
+                double oo;
+                if constexpr (CACHED) oo = double(tc[0]);
+                else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = K(oo); }
+                R_FM = quat_rot(K(qv[0]*oo), K(qv[1]*oo), K(qv[2]*oo), K(qv[3]*oo));
+                H_FM[0] = {V3(o, z, z), zero3<K>()};
+                H_FM[1] = {V3(z, o, z), zero3<K>()};
+                H_FM[2] = {V3(z, z, o), zero3<K>()};
+            }
+        } else if constexpr (TYPE == PIN) {
             K sn, cs;
             if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
             R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
@@ -199,6 +234,7 @@ __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>&
     for (int j = 0; j < D; ++j) {
         v3<K> hdw = zero3<K>();
         if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(J.R_FM, 1)); }
+        else if constexpr (TYPE == GEN) { if (j == 1 && J.ty == UNIVERSAL) hdw = cross(w_FM, col(J.R_FM, 1)); }
         const v3<K> lin = (zero3<K>() - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
         const sv<K> HD = {mul(R_GF, hdw) + cross(V_GP.w, H[j].w), mul(R_GF, lin) + cross(V_GP.w, H[j].v)};
         VD = VD + K(uv[j])*HD;
@@ -251,6 +287,7 @@ template <class K> struct KStore {
 // loads are in flight while the previous body computes.
 template <class K> struct Pre {
     double q[7], u[6]; K tc[4]; K g[18]; double de[3];
+    int ty;                 // GEN: the body's mobilizer
 };
 // Raw Runge-Kutta sources of one body's stage input: y0, f0, fa, fb per
 // q then u element.
@@ -270,9 +307,9 @@ struct RkRows {
 struct NormSums { double sq_other, sq_quat, su, s2; };
 // Type-exact prefetch: NQ q, D u, the trig cache entries the mobilizer uses
 // and (pass 3) its G and DI*eps values.
-template <int TYPE> struct TcN { static constexpr int n = TYPE == PIN ? 2 : TYPE == UNIVERSAL ? 4 : TYPE == SLIDER ? 0 : 1; };
-template <int TYPE, bool G, class K>
-__device__ __forceinline__ void prefetch_t(Pre<K>& x, const Rows& y, int nq, int qi, int ui, const KStore<K>& st,
+template <int TYPE> struct TcN { static constexpr int n = TYPE == PIN ? 2 : (TYPE == UNIVERSAL || TYPE == GEN) ? 4 : TYPE == SLIDER ? 0 : 1; };
+template <int TYPE, bool G, class K, class YR, class ST>
+__device__ __forceinline__ void prefetch_t(Pre<K>& x, const YR& y, int nq, int qi, int ui, const ST& st,
                                            bool tc) {
     constexpr int NQ = MobC<TYPE>::NQ, D = MobC<TYPE>::D;
 #pragma unroll
@@ -308,12 +345,22 @@ __device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 
 // ---------------------------------------------------------------------------
 // Pass 1 body: forward transforms and velocities, qdot out, trig cache out.
 // ---------------------------------------------------------------------------
-template <int TYPE, class K, class PA>
-__device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi, const Rows& f, const KStore<K>& st,
+template <int TYPE, class K, class PA, class YR, class ST>
+__device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi, const YR& f, const ST& st,
                                            m33<K>& R, v3<K>& p, sv<K>& V) {
     constexpr int D = MobC<TYPE>::D;
     const double* qv = x.q; const double* uv = x.u;
-    if constexpr (TYPE == BALL || TYPE == FREE) {
+    if constexpr (TYPE == GEN) {
+        if (x.ty == BALL) {
+            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
+        } else {
+            const int d = mob_dof(x.ty);
+#pragma unroll
+            for (int k = 0; k < D; ++k) if (k < d) f(qi + k) = uv[k];
+        }
+    } else if constexpr (TYPE == BALL || TYPE == FREE) {
         double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
 #pragma unroll
         for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
@@ -323,8 +370,8 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
         for (int k = 0; k < D; ++k) f(qi + k) = uv[k];
     }
     Joint<TYPE, K> J;
-    K tc[4];
-    J.template set<false>(qv, tc);
+    K tc[4] = {K(0), K(0), K(0), K(0)};
+    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE);
 #pragma unroll
     for (int k = 0; k < TcN<TYPE>::n; ++k) st.T(k) = tc[k];
     m33<K> R_PB; v3<K> p_PB;
@@ -382,15 +429,18 @@ template <int D> __device__ __forceinline__ void dinv(const double* Dm, double* 
 // In: R, p, V of this body; child's PP, zp, phic (hasChild).  Out: R, p, V
 // of the parent; PP, zp, phic of this body for the parent.
 // ---------------------------------------------------------------------------
-template <int TYPE, class K, class PA>
+// accP, accZ (nullable): contributions of further children, already shifted
+// to this body (trees); the child in PP/zp/phic is added by this routine.
+template <int TYPE, class K, class PA, class ST>
 __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v3<K>& grav, const sv<K>& Fapp,
-                                           const KStore<K>& st, m33<K>& R, v3<K>& p, sv<K>& V, bool hasChild,
-                                           abi<K>& PP, sv<K>& zp, v3<K>& phic) {
+                                           const ST& st, m33<K>& R, v3<K>& p, sv<K>& V, bool hasChild,
+                                           abi<K>& PP, sv<K>& zp, v3<K>& phic,
+                                           const abi<K>* accP = nullptr, const sv<K>* accZ = nullptr) {
     constexpr int D = MobC<TYPE>::D;
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc);
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     const m33<K> R_GB = R; const v3<K> p_GB = p; const sv<K> V_GB = V;
@@ -415,6 +465,11 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const v3<K> fg = m*grav;
     const sv<K> F = sv<K>{cross(Mk_p, fg), fg} + Fapp;
     abi<K> Pb = abi_from_rbi(m, Mk_p, Mk_G);
+    if (accP) {
+        Pb.M = Pb.M + accP->M; Pb.J = Pb.J + accP->J;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Pb.F.a[i] = Pb.F.a[i] + accP->F.a[i];
+    }
     if (hasChild) abi_add_shifted(Pb, PP, phic);
     sv<K> PH[D];
 #pragma unroll
@@ -424,6 +479,10 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     for (int r = 0; r < D; ++r)
 #pragma unroll
         for (int k = 0; k < D; ++k) Dm[r*D+k] = dotd(H[r], PH[k]);
+    if constexpr (TYPE == GEN) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) if (j >= J.d) Dm[j*D+j] = 1.0;
+    }
     dinv<D>(Dm, DI);
     sv<K> G[D];
 #pragma unroll
@@ -434,6 +493,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
         G[k] = tok<K>(acc);
     }
     sv<K> z = (abi_mul(Pb, a) + bg) - F;
+    if (accZ) z = z + *accZ;
     if (hasChild) z = z + phi_mul(phic, zp);
     double eps[D];
 #pragma unroll
@@ -481,14 +541,14 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
 // ---------------------------------------------------------------------------
 // Pass 3 body: forward kinematics again, then udot and A_GB.
 // ---------------------------------------------------------------------------
-template <int TYPE, class K, class PA>
-__device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui, const Rows& fu, const KStore<K>& st,
+template <int TYPE, class K, class PA, class YR, class ST>
+__device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui, const YR& fu, const ST& st,
                                            m33<K>& R, v3<K>& p, sv<K>& V, sv<K>& A, double* ud_out = nullptr) {
     constexpr int D = MobC<TYPE>::D;
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc);
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];
@@ -518,7 +578,10 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     for (int j = 0; j < D; ++j) Hu = Hu + K(ud[j])*H[j];
     A = (Ap + Hu) + a;
 #pragma unroll
-    for (int j = 0; j < D; ++j) { fu(ui + j) = ud[j]; if (ud_out) ud_out[j] = ud[j]; }
+    for (int j = 0; j < D; ++j) {
+        if (TYPE == GEN && j >= J.d) continue;
+        fu(ui + j) = ud[j]; if (ud_out) ud_out[j] = ud[j];
+    }
     p = p + mul(R, p_PB);
     R = mul(R, R_PB);
     V = V_GB;

@@ -56,7 +56,7 @@ struct tdyn_batch {
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
-    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b;
+    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b, pre;
     tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;   // per LANE
     tdyn::host::DBuf<int> inst;          // instance of each lane
     bool permuted = false;               // lanes are not in instance order
@@ -72,12 +72,16 @@ struct tdyn_batch {
     bool is_chain = false;       // uniform batch whose tree is a serial chain
     bool idf = false;            // tree 0's joint frames are identity rotations (R_PF = I, p_PF = 0, R_BM = I)
     int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
+    bool stream_ok = false;      // different trees, Free only at body 1: the streaming tree evaluator applies
+    int dmax = 0;                // deepest body of any tree
+    int stream_stride = 0;       // scratch doubles per lane of the streaming tree evaluator
+    bool rollout_aos = false;    // the current rollout keeps lane-contiguous vectors (streaming tree evaluator)
     RkmCfg cfg{};
 
     DevTables tables() const {
         DevTables T;
         T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
-        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p;
+        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p;
         return T;
     }
     OpArgs op() const {
@@ -97,6 +101,9 @@ struct tdyn_batch {
         a.scratch32 = scratch32.p;
         a.idf = idf ? 1 : 0;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
+        a.vec_aos = rollout_aos ? 1 : 0;
+        a.stream_dmax = dmax;
+        if (rollout_aos) a.rec_stride = stream_stride;
         return a;
     }
 };

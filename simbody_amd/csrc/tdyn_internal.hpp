@@ -20,6 +20,8 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
 }
 
 enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4 };
+// Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
+enum { GEN = 5 };
 __host__ __device__ inline int mob_dof(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 3 : 6; }
 __host__ __device__ inline int mob_nq(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 4 : 7; }
 
@@ -47,6 +49,7 @@ struct DevTables {
     const int4* tree_a;      // per tree: nb, nq, nu, body_off
     const int4* tree_b;      // per tree: rec_size, nquat, -, -
     const double* gravity;   // per tree: 4 doubles
+    const int4* pre;         // per global body: depth-first preorder of its tree: body, depth, child count, -
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
@@ -118,7 +121,26 @@ struct RkmArgs {
                              // (R_PF = I, p_PF = 0, R_MB = I), so the frame products are skipped
     const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
                              // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
+    int vec_aos;             // 1: vectors lane-contiguous (entry k of lane l at ((v*L + l)*ny_max + k)) and
+                             // the streaming tree evaluator (batches of different trees)
+    int stream_dmax;         // streaming tree evaluator: deepest body of any tree
 };
+// Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
+// depth: R p V A of the last body with two or more children seen at that
+// depth), ACC (27 per depth: P and z sums of a body's children after the
+// first), then one hand-over record per preorder position: pose R p V (18),
+// G (6 per dof), DI*eps (dof), trig cache (4).  Position 0 (body 1, 6 dof if
+// Free) has 64, the others (3 dof, padded) 43.
+namespace tstream {
+enum { ST_W = 24, ACC_W = 27, POS0_W = 64, POS_W = 43 };
+__host__ __device__ inline int scratch_doubles(int dmax, int nbmax) {
+    return (ST_W + ACC_W) * (dmax + 1) + POS0_W + POS_W * (nbmax > 1 ? nbmax - 1 : 0);
+}
+} // namespace tstream
+// Entry k of state vector v of a lane, in either layout.
+__device__ __forceinline__ double& vec_at(const RkmArgs& a, int v, int lane, int k) {
+    return a.vec_aos ? a.vec[((size_t)v * a.L + lane) * a.ny_max + k] : a.vec[((size_t)v * a.ny_max + k) * a.L + lane];
+}
 // Rollout state vectors (rows of ny_max * L doubles in RkmArgs::vec): y, y0,
 // f0, fa, fb, ysave, y1, ystage, yerr and the mixed path's Kahan rows.
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };

@@ -374,7 +374,6 @@ enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8 };
 // mobility force.
 // ---------------------------------------------------------------------------
 // TYPE = GEN: the padded generic body (see kin_body_gen); dof from ba.y.
-enum { GEN = 5 };
 template <int TYPE> struct MobD { static constexpr int D = TYPE == GEN ? 3 : Mob<TYPE>::D; };
 template <int TYPE, int MODE, class FB, class FM>
 __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
@@ -780,6 +779,10 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 template <bool PAD>
 struct TreeEval {
+    using VR = chain::RowsT<double, false>;
+    static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
+        return VR{a.vec + (size_t)v * a.ny_max * a.L, a.L, (unsigned)lane};
+    }
     Lane c; const double* grav; const double* fapp;
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
         : c(make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0)), grav(a.T.gravity + 4*tr),
@@ -790,6 +793,222 @@ struct TreeEval {
         sweep_kin<true, PAD>(c, y, u, f);
         sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
         sweep_outward2<true, PAD>(c, y, u, fu, NoPut{});
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Streaming tree evaluator (rollouts of batches of different trees).  Each
+// lane walks its tree in depth-first preorder (parents first, every subtree
+// contiguous) with the chain evaluator's register-resident pass bodies on
+// padded generic bodies:
+//   pass 1 (preorder): the parent's pose comes from registers when the
+//          parent is the previous body, else from a per-depth stack; the
+//          body's pose is written to its position's hand-over row;
+//   pass 2 (reverse preorder): the body's pose is read back; the children's
+//          articulated inertias and bias forces arrive in registers from the
+//          previous body (the first child) plus a per-depth accumulator of the
+//          other children; the body's own P+, z+ go to the parent in
+//          registers (when the parent is next) or into its accumulator;
+//   pass 3 (preorder): as pass 1, with the parent's acceleration.
+// All per-lane data is lane-contiguous (the lanes of a wave walk different
+// trees, so no access pattern would coalesce across lanes).
+// ---------------------------------------------------------------------------
+namespace tstream {
+struct LaneVec {
+    double* p;
+    __device__ __forceinline__ double& operator()(int k) const { return p[k]; }
+};
+struct LaneStore {
+    double* p; int g, tc, de;
+    __device__ __forceinline__ double& G(int i) const { return p[g + i]; }
+    __device__ __forceinline__ double& T(int i) const { return p[tc + i]; }
+    __device__ __forceinline__ double& DE(int i) const { return p[de + i]; }
+};
+} // namespace tstream
+
+struct TreeStreamEval {
+    using VR = tstream::LaneVec;
+    DevTables T;
+    int nb, nq, body_off, acc0, pos0;
+    const int4* pre;
+    double* S;
+    d3 grav; const double* fapp;
+    static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
+        return VR{a.vec + ((size_t)v * a.L + lane) * a.ny_max};
+    }
+    __device__ TreeStreamEval(const RkmArgs& a, int lane, int tr) : T(a.T) {
+        const int4 ta = ldi4(a.T.tree_a, tr);
+        nb = ta.x; nq = ta.y; body_off = ta.w;
+        pre = a.T.pre + body_off;
+        S = a.scratch + (size_t)lane * a.rec_stride;
+        acc0 = tstream::ST_W * (a.stream_dmax + 1);
+        pos0 = acc0 + tstream::ACC_W * (a.stream_dmax + 1);
+        const double* g = a.T.gravity + 4*tr;
+        grav = V3(g[0], g[1], g[2]);
+        fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
+        // accumulators start at zero; every reader clears what it read
+        for (int k = 0; k < tstream::ACC_W * (a.stream_dmax + 1); ++k) S[acc0 + k] = 0.0;
+    }
+    __device__ __forceinline__ int pos(int i) const {
+        return pos0 + (i == 0 ? 0 : tstream::POS0_W + (i - 1) * tstream::POS_W);
+    }
+    __device__ __forceinline__ tstream::LaneStore store(int i) const {
+        const int o = pos(i);
+        return i == 0 ? tstream::LaneStore{S, o + 18, o + 60, o + 54} : tstream::LaneStore{S, o + 18, o + 39, o + 36};
+    }
+    __device__ __forceinline__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
+    __device__ __forceinline__ dv F(int b) const {
+        if (!fapp) return svzero<double>();
+        const double* f = fapp + 6*b;
+        return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
+    }
+    // q, u (zero-padded), trig cache and (G_ROWS) G, DI*eps of position i
+    template <int TYPE, bool G_ROWS>
+    __device__ __forceinline__ void load(chain::Pre<double>& x, const VR& y, int4 ba, const tstream::LaneStore& st) const {
+        if constexpr (TYPE == FREE) {
+            x.ty = FREE;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) x.q[k] = y(ba.z + k);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) x.u[k] = y(nq + ba.w + k);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x.tc[k] = st.T(k);
+        } else {
+            const int ty = ba.y, d = mob_dof(ty), nqb = mob_nq(ty);
+            x.ty = ty;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x.q[k] = k < nqb ? y(ba.z + k) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x.u[k] = k < d ? y(nq + ba.w + k) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x.tc[k] = st.T(k);
+            if constexpr (G_ROWS) {
+#pragma unroll
+                for (int k = 0; k < 18; ++k) x.g[k] = st.G(k);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x.de[k] = st.DE(k);
+            }
+        }
+    }
+    __device__ __forceinline__ void ld_pose(const double* s, dm& R, d3& p, dv& V) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R.a[k] = s[k];
+        p = V3(s[9], s[10], s[11]);
+        V = {V3(s[12], s[13], s[14]), V3(s[15], s[16], s[17])};
+    }
+    __device__ __forceinline__ void st_pose(double* s, const dm& R, const d3& p, const dv& V) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s[k] = R.a[k];
+        s[9] = p.x; s[10] = p.y; s[11] = p.z;
+        s[12] = V.w.x; s[13] = V.w.y; s[14] = V.w.z; s[15] = V.v.x; s[16] = V.v.y; s[17] = V.v.z;
+    }
+    // Pass 1 at position i: the parent's pose (ground, registers or stack) to
+    // this body's; qdot and trig cache out; pose to the position record and
+    // to the depth stack.
+    template <int TYPE>
+    __device__ __forceinline__ void p1(int i, int& prev_d, const VR& y, const VR& f, dm& R, d3& p, dv& V) const {
+        const int4 pe = ldi4(pre, i);
+        const int b = pe.x, d = pe.y;
+        const int4 ba = ldi4(T.body_a, body_off + b - 1);
+        if (d == 1) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); }
+        else if (prev_d != d - 1) ld_pose(S + tstream::ST_W * (d - 1), R, p, V);
+        const tstream::LaneStore st = store(i);
+        chain::Pre<double> x;
+        load<TYPE, false>(x, y, ba, st);
+        chain::pass1_body<TYPE, double>(prm(b), x, ba.z, f, st, R, p, V);
+        st_pose(S + pos(i), R, p, V);
+        if (pe.z > 1) st_pose(S + tstream::ST_W * d, R, p, V);   // later children need it
+        prev_d = d;
+    }
+    // Pass 2 at position i (reverse order): P, D, G, z of this body from its
+    // first child (registers, hc) and its other children (accumulator);
+    // P+, z+ to the parent in registers or added into the parent's
+    // accumulator.
+    template <int TYPE>
+    __device__ __forceinline__ void p2(int i, int& prev_d, const VR& y, chain::da& PP, chain::dv& zp, d3& phic) const {
+        const int4 pe = ldi4(pre, i);
+        const int b = pe.x, d = pe.y;
+        const int4 ba = ldi4(T.body_a, body_off + b - 1);
+        const bool hc = i + 1 < nb && prev_d == d + 1;
+        dm R; d3 p; dv V;
+        ld_pose(S + pos(i), R, p, V);
+        const tstream::LaneStore st = store(i);
+        chain::Pre<double> x;
+        load<TYPE, false>(x, y, ba, st);
+        if (pe.z > 1) {
+            double* ac = S + acc0 + tstream::ACC_W * d;
+            chain::da aP;
+            aP.M = {ac[0], ac[1], ac[2], ac[3], ac[4], ac[5]};
+            aP.J = {ac[6], ac[7], ac[8], ac[9], ac[10], ac[11]};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) aP.F.a[k] = ac[12 + k];
+            const dv aZ = {V3(ac[21], ac[22], ac[23]), V3(ac[24], ac[25], ac[26])};
+#pragma unroll
+            for (int k = 0; k < tstream::ACC_W; ++k) ac[k] = 0.0;
+            chain::pass2_body<TYPE, double>(prm(b), x, grav, F(b), st, R, p, V, hc, PP, zp, phic, &aP, &aZ);
+        } else {
+            chain::pass2_body<TYPE, double>(prm(b), x, grav, F(b), st, R, p, V, hc, PP, zp, phic);
+        }
+        // PP, zp, phic: this body's P+, z+ and offset from its parent
+        if (d > 1 && !(i >= 1 && ldi4(pre, i - 1).y == d - 1)) {
+            double* ap = S + acc0 + tstream::ACC_W * (d - 1);
+            chain::da Q;
+            Q.M = {ap[0], ap[1], ap[2], ap[3], ap[4], ap[5]};
+            Q.J = {ap[6], ap[7], ap[8], ap[9], ap[10], ap[11]};
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Q.F.a[k] = ap[12 + k];
+            abi_add_shifted(Q, PP, phic);
+            const dv z = dv{V3(ap[21], ap[22], ap[23]), V3(ap[24], ap[25], ap[26])} + phi_mul(phic, zp);
+            ap[0] = Q.M.xx; ap[1] = Q.M.yy; ap[2] = Q.M.zz; ap[3] = Q.M.xy; ap[4] = Q.M.xz; ap[5] = Q.M.yz;
+            ap[6] = Q.J.xx; ap[7] = Q.J.yy; ap[8] = Q.J.zz; ap[9] = Q.J.xy; ap[10] = Q.J.xz; ap[11] = Q.J.yz;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) ap[12 + k] = Q.F.a[k];
+            ap[21] = z.w.x; ap[22] = z.w.y; ap[23] = z.w.z; ap[24] = z.v.x; ap[25] = z.v.y; ap[26] = z.v.z;
+        }
+        prev_d = d;
+    }
+    // Pass 3 at position i: udot = DI eps - G^T A+ and A_GB.
+    template <int TYPE>
+    __device__ __forceinline__ void p3(int i, int& prev_d, const VR& y, const VR& fu, dm& R, d3& p, dv& V, dv& A) const {
+        const int4 pe = ldi4(pre, i);
+        const int b = pe.x, d = pe.y;
+        const int4 ba = ldi4(T.body_a, body_off + b - 1);
+        if (d == 1) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); A = svzero<double>(); }
+        else if (prev_d != d - 1) {
+            const double* s = S + tstream::ST_W * (d - 1);
+            ld_pose(s, R, p, V);
+            A = {V3(s[18], s[19], s[20]), V3(s[21], s[22], s[23])};
+        }
+        const tstream::LaneStore st = store(i);
+        chain::Pre<double> x;
+        load<TYPE, true>(x, y, ba, st);
+        chain::pass3_body<TYPE, double>(prm(b), x, ba.w, fu, st, R, p, V, A);
+        if (pe.z > 1) {
+            double* s = S + tstream::ST_W * d;
+            st_pose(s, R, p, V);
+            s[18] = A.w.x; s[19] = A.w.y; s[20] = A.w.z; s[21] = A.v.x; s[22] = A.v.y; s[23] = A.v.z;
+        }
+        prev_d = d;
+    }
+    __device__ __forceinline__ void derivs(const VR& y, const VR& f) const {
+        const VR fu{f.p + nq};
+        // position 0 is body 1, the only body that may be Free
+        const bool free0 = ldi4(T.body_a, body_off).y == FREE;
+        {
+            dm R; d3 p; dv V; int pd = 0;
+            if (free0) p1<FREE>(0, pd, y, f, R, p, V); else p1<GEN>(0, pd, y, f, R, p, V);
+            for (int i = 1; i < nb; ++i) p1<GEN>(i, pd, y, f, R, p, V);
+        }
+        {
+            chain::da PP; dv zp = svzero<double>(); d3 phic = zero3<double>(); int pd = 0;
+            for (int i = nb - 1; i >= 1; --i) p2<GEN>(i, pd, y, PP, zp, phic);
+            if (free0) p2<FREE>(0, pd, y, PP, zp, phic); else p2<GEN>(0, pd, y, PP, zp, phic);
+        }
+        {
+            dm R; d3 p; dv V, A; int pd = 0;
+            if (free0) p3<FREE>(0, pd, y, fu, R, p, V, A); else p3<GEN>(0, pd, y, fu, R, p, V, A);
+            for (int i = 1; i < nb; ++i) p3<GEN>(i, pd, y, fu, R, p, V, A);
+        }
     }
 };
 
@@ -812,8 +1031,8 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int nquat = ldi4(a.T.tree_b, tr).y;
     const size_t NV = (size_t)a.ny_max * a.L;
     const int L = a.L;
-    using TRows = chain::RowsT<double, false>;
-    auto rows = [&](int v) { return TRows{a.vec + v*NV, L, (unsigned)lane}; };
+    using TRows = typename EV::VR;
+    auto rows = [&](int v) { return EV::vrow(a, v, lane); };
     const TRows Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
                       Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
 
@@ -956,15 +1175,13 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     a.stats[4*lane] = attempts; a.stats[4*lane+1] = accepted; a.stats[4*lane+2] = rejected; a.stats[4*lane+3] = projections;
 }
 
-template <bool UNI>
+template <class EV, bool UNI>
 __global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     const int tr = UNI ? 0 : a.tree_of[lane];
-    const TreeEval<!UNI> ev(a, lane, tr);
-    const size_t NV = (size_t)a.ny_max * a.L;
-    using TRows = chain::RowsT<double, false>;
-    ev.derivs(TRows{a.vec + VY*NV, a.L, (unsigned)lane}, TRows{a.vec + VF0*NV, a.L, (unsigned)lane});
+    const EV ev(a, lane, tr);
+    ev.derivs(EV::vrow(a, VY, lane), EV::vrow(a, VF0, lane));
 }
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
@@ -972,22 +1189,22 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     // the derivative is the general form's bit for bit, so identity frames
     // need no separate build here
     if (a.chain) return launch_chain_variant(a, s, true);
-    if (a.tree_of) k_tree_f0<false><<<grd, blk, 0, s>>>(a);
-    else k_tree_f0<true><<<grd, blk, 0, s>>>(a);
+    if (a.vec_aos) k_tree_f0<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
+    else if (a.tree_of) k_tree_f0<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
+    else k_tree_f0<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
 
 // initialize(): forced quaternion normalization, t and h set.
 __device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
     const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
-    const size_t NV = (size_t)a.ny_max * a.L;
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
         if (ba.y != BALL && ba.y != FREE) continue;
         double qq[4], n2 = 0;
-        for (int k = 0; k < 4; ++k) { qq[k] = a.vec[VY*NV + (size_t)(ba.z + k)*a.L + lane]; n2 += qq[k]*qq[k]; }
+        for (int k = 0; k < 4; ++k) { qq[k] = vec_at(a, VY, lane, ba.z + k); n2 += qq[k]*qq[k]; }
         const double n = sqrt(n2);
-        for (int k = 0; k < 4; ++k) a.vec[VY*NV + (size_t)(ba.z + k)*a.L + lane] = qq[k] / n;
+        for (int k = 0; k < 4; ++k) vec_at(a, VY, lane, ba.z + k) = qq[k] / n;
     }
     const RkmCfg& cfg = a.cfg;
     double hmin = cfg.min_h, hmax = cfg.max_h, hinit = cfg.init_h;
@@ -998,7 +1215,7 @@ __device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
     a.t[lane] = t0; a.h[lane] = h; a.status[lane] = 0;
     for (int k = 0; k < 4; ++k) a.stats[4*lane + k] = 0;
     // Kahan compensation of the mixed path starts at zero
-    for (int k = 0; k < c.nq + c.nu; ++k) a.vec[VC0*NV + (size_t)k*a.L + lane] = 0.0;
+    for (int k = 0; k < c.nq + c.nu; ++k) vec_at(a, VC0, lane, k) = 0.0;
 }
 __global__ void __launch_bounds__(64) k_rkm_init(RkmArgs a, double t0) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1070,7 +1287,8 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
     if (a.chain) return launch_chain_variant(a, s, false);
-    if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
+    if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
+    else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
@@ -1109,13 +1327,12 @@ __global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, 
     const int4 ta = a.T.tree_a[tr];
     const int nq = ta.y, nu = ta.z;
     const int qo = a.off.q_off[lane], uo = a.off.u_off[lane];
-    double* Y = a.vec + (size_t)v * a.ny_max * a.L;
     if (to_soa) {
-        for (int k = 0; k < nq; ++k) Y[(size_t)k*a.L + lane] = q[qo + k];
-        for (int k = 0; k < nu; ++k) Y[(size_t)(nq + k)*a.L + lane] = u[uo + k];
+        for (int k = 0; k < nq; ++k) vec_at(a, v, lane, k) = q[qo + k];
+        for (int k = 0; k < nu; ++k) vec_at(a, v, lane, nq + k) = u[uo + k];
     } else {
-        for (int k = 0; k < nq; ++k) q_out[qo + k] = Y[(size_t)k*a.L + lane];
-        for (int k = 0; k < nu; ++k) u_out[uo + k] = Y[(size_t)(nq + k)*a.L + lane];
+        for (int k = 0; k < nq; ++k) q_out[qo + k] = vec_at(a, v, lane, k);
+        for (int k = 0; k < nu; ++k) u_out[uo + k] = vec_at(a, v, lane, nq + k);
     }
 }
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,

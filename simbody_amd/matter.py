@@ -26,7 +26,8 @@ _i32 = ctypes.POINTER(ctypes.c_int32)
 TDYN_ERRORS = {1: "TDYN_E_ARG", 2: "TDYN_E_SIZE", 3: "TDYN_E_STAGE", 4: "TDYN_E_SINGULAR",
                5: "TDYN_E_STEP_FAILED", 6: "TDYN_E_HIP", 7: "TDYN_E_NOMEM"}
 
-EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator", "tdyn_batch_info",
+EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator", "tdyn_batch_get_evaluator",
+           "tdyn_batch_info",
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_multiply_by_m_inv",
@@ -76,6 +77,7 @@ def lib():
                                         ctypes.POINTER(vp)]
         L.tdyn_batch_destroy.argtypes = [vp]
         L.tdyn_batch_set_evaluator.argtypes = [vp, ctypes.c_int]
+        L.tdyn_batch_get_evaluator.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.tdyn_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.tdyn_batch_offsets.argtypes = [vp, _i32, _i32, _i32]
         L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
@@ -200,6 +202,16 @@ class MatterBatch:
     def set_evaluator(self, mode):
         """0 automatic, 1 tree sweeps, 2 register-resident serial-chain evaluator."""
         _check(lib().tdyn_batch_set_evaluator(self.h, int(mode)))
+
+    def active_evaluator(self):
+        """Evaluator of the current rollout: 0 none yet, 1 tree sweeps, 2 chain,
+        3 streaming tree evaluator."""
+        v = ctypes.c_int(0)
+        _check(lib().tdyn_batch_get_evaluator(self.h, ctypes.byref(v)))
+        return v.value
+
+    def uses_tree_stream(self):
+        return self.active_evaluator() == 3
 
     # ---- helpers ---------------------------------------------------------
     def _new(self, n):
