@@ -109,7 +109,14 @@ def row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=True):
     return out
 
 
-def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0):
+def _evaluator_name(B):
+    ev, lanes = B.active_evaluator()
+    name = {1: "tree sweeps", 2: "chain", 3: "streaming tree evaluator",
+            4: "level-parallel tree evaluator"}.get(ev, "?")
+    return f"{name}, {lanes} lane(s) per instance"
+
+
+def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0):
     """Uneven branched trees (nb ~ U{8..128}, depth <= 20, Pin/Slider/
     Universal/Ball + Free roots), constant N(0,1) body forces and torques,
     per-instance RK-Merson step control at accuracy 1e-6 to t = 0.5 s, in
@@ -120,6 +127,7 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     Fs = [M.random_body_forces(t, 3, i) for i, t in enumerate(trees)]
     nbs = np.array([t.nb for t in trees])
     B = MatterBatch(trees, tree_of=np.arange(n), device="cuda:0")
+    B.set_tree_lanes(lanes)
     q = B.pack([y[:t.nq] for y, t in zip(Ys, trees)]); u = B.pack([y[t.nq:] for y, t in zip(Ys, trees)])
     F = B.pack(Fs)
     integ = RungeKuttaMerson(accuracy=1e-6)
@@ -136,7 +144,7 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
            "value": body_steps / sec, "unit": "body-steps/s", "seconds": sec,
            "body_steps": body_steps, "accepted_mean": float(st[:, 1].mean()),
            "rejected_total": int(st[:, 2].sum()),
-           "evaluator": {1: "tree sweeps", 2: "chain", 3: "streaming tree evaluator"}.get(B.active_evaluator(), "?")}
+           "evaluator": _evaluator_name(B)}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle_ctypes import OracleModel, time_integrate_batch
@@ -273,6 +281,8 @@ def main():
                     help="skip the secondary workloads (pendulum latency, uneven trees)")
     ap.add_argument("--only", choices=["pendulum", "trees", "serving", "mixed"], default=None,
                     help="run only one secondary workload and print its JSON")
+    ap.add_argument("--tree-lanes", type=int, default=0,
+                    help="lanes per instance of the tree evaluator (0 automatic, 1, 2, 4, 8, 16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
@@ -298,7 +308,8 @@ def main():
         print(json.dumps(row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)), flush=True)
         return
     if args.only == "trees":
-        print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu,
+                                   lanes=args.tree_lanes)), flush=True)
         return
     if args.only == "mixed":
         print(json.dumps(row_mixed(torch, M, MatterBatch, RungeKuttaMerson)), flush=True)

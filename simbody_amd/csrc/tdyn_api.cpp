@@ -91,6 +91,23 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
     return TDYN_OK;
 }
 
+// Lanes per instance of the level-parallel tree evaluator for the next
+// rollout (0: the one-lane streaming evaluator).  Automatic: the most lanes
+// (up to 16) that keep the launch within one wavefront per SIMD, since a
+// rollout lasts as long as its slowest instance and the kernel's register
+// use allows one wavefront per SIMD.
+int tree_group_lanes(const tdyn_batch* b) {
+    if (b->tree_lanes > 0) return b->tree_lanes == 1 ? 0 : b->tree_lanes;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int64_t simds = 4 * (int64_t)cus;
+    int g = 1;
+    while (g < 16 && ((int64_t)b->n_inst * 2 * g + 63) / 64 <= simds) g *= 2;
+    return g == 1 ? 0 : g;
+}
+
 int ensure_rollout_buffers(tdyn_batch* b) {
     const int ny = b->max_nq + b->max_nu;
     const size_t L = b->n_inst;
@@ -134,7 +151,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     auto* B = new tdyn_batch;
     B->n_inst = n_instances; B->n_trees = n_trees;
     B->uniform = (tree_of_instance == nullptr) || n_trees == 1;
-    std::vector<int4> body_b, pre; std::vector<int> child_idx; std::vector<double> prm, grav;
+    std::vector<int4> body_b, pre, bfs; std::vector<int> child_idx, lvl; std::vector<double> prm, grav;
     B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
     int body_off = 0;
@@ -175,7 +192,30 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
         grav.push_back(0.0);
         B->h_tree_a.push_back(make_int4(nb, q, u, body_off));
-        B->h_tree_b.push_back(make_int4(rec, nquat, gslot, 0));
+        // breadth-first order (roots, then each level's children in order):
+        // body, parent's position, first child's position, child count; and
+        // the start of every level
+        const int lvl_off = (int)lvl.size();
+        {
+            std::vector<int> order(ch[0]), pos(nb + 1, -1), depth(nb + 1, 0);
+            std::vector<int> starts{0};
+            for (size_t k = 0; k < order.size(); ++k) depth[order[k]] = 1;
+            for (size_t k = 0; k < order.size(); ++k) {
+                const int b = order[k];
+                pos[b] = (int)k;
+                if (k > 0 && depth[b] != depth[order[k - 1]]) starts.push_back((int)k);
+                for (int c : ch[b]) { depth[c] = depth[b] + 1; order.push_back(c); }
+            }
+            starts.push_back(nb);
+            for (int k = 0; k < nb; ++k) {
+                const int b = order[k], p = d.parent[b - 1];
+                const int first = ch[b].empty() ? 0 : pos[ch[b][0]];
+                bfs.push_back(make_int4(b, p == 0 ? -1 : pos[p], first, (int)ch[b].size()));
+            }
+            lvl.push_back((int)starts.size() - 1);
+            lvl.insert(lvl.end(), starts.begin(), starts.end());
+        }
+        B->h_tree_b.push_back(make_int4(rec, nquat, gslot, lvl_off));
         tq[t] = q; tu[t] = u; tnb[t] = nb;
         if (t == 0) B->is_chain = chain;
         B->max_rec = std::max(B->max_rec, rec);
@@ -196,6 +236,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     B->total_q = q; B->total_u = u; B->total_b = bb;
     B->is_chain = B->is_chain && B->uniform;
     B->stream_stride = tdyn::tstream::scratch_doubles(B->dmax, B->max_nb);
+    B->group_stride = tdyn::tgroup::scratch_doubles(B->max_nb);
     B->idf = B->is_chain && identity_frames(trees[0]);
     // Lane order.  A wave runs as long as its slowest lane, so instances of
     // different trees are assigned to lanes in order of decreasing work
@@ -233,7 +274,10 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->b_off.upload(lane_b)) != hipSuccess ||
         (e = B->inst.upload(inst_of_lane)) != hipSuccess ||
         (e = B->pre.upload(pre)) != hipSuccess ||
-        (e = B->scratch.alloc((size_t)std::max(B->max_rec + 24, B->stream_ok ? B->stream_stride : 0) * n_instances)) != hipSuccess) {
+        (e = B->bfs.upload(bfs)) != hipSuccess ||
+        (e = B->lvl.upload(lvl)) != hipSuccess ||
+        (e = B->scratch.alloc((size_t)std::max({B->max_rec + 24, B->stream_ok ? B->stream_stride : 0,
+                                                B->stream_ok ? B->group_stride : 0}) * n_instances)) != hipSuccess) {
         delete B;
         return hipfail(e, "tdyn_batch_create: device allocation/upload");
     }
@@ -251,11 +295,24 @@ int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
     return TDYN_OK;
 }
 
-int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active) {
+int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(TDYN_E_ARG, "lanes per instance must be 0 (automatic), 1, 2, 4, 8 or 16");
+    b->tree_lanes = lanes;
+    return TDYN_OK;
+}
+
+int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes) {
     if (!b || !active) return fail(TDYN_E_ARG, "null argument");
-    if (!b->rollout_ready) { *active = 0; return TDYN_OK; }
-    const RkmArgs a = b->rk();
-    *active = a.chain ? 2 : a.vec_aos ? 3 : 1;
+    int act = 0, ln = 0;
+    if (b->rollout_ready) {
+        const RkmArgs a = b->rk();
+        act = a.chain ? 2 : a.grp > 1 ? 4 : a.vec_aos ? 3 : 1;
+        ln = act == 4 ? a.grp : 1;
+    }
+    *active = act;
+    if (lanes) *lanes = ln;
     return TDYN_OK;
 }
 
@@ -352,6 +409,7 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
     const size_t L = b->n_inst;
     b->rollout_aos = b->stream_ok && b->evaluator == 0;
+    b->rollout_grp = b->rollout_aos ? tree_group_lanes(b) : 0;
     { const int rc = ensure_rollout_buffers(b); if (rc) return rc; }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));

@@ -56,7 +56,8 @@ struct tdyn_batch {
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
-    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b, pre;
+    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b, pre, bfs;
+    tdyn::host::DBuf<int> lvl;
     tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;   // per LANE
     tdyn::host::DBuf<int> inst;          // instance of each lane
     bool permuted = false;               // lanes are not in instance order
@@ -76,12 +77,15 @@ struct tdyn_batch {
     int dmax = 0;                // deepest body of any tree
     int stream_stride = 0;       // scratch doubles per lane of the streaming tree evaluator
     bool rollout_aos = false;    // the current rollout keeps lane-contiguous vectors (streaming tree evaluator)
+    int tree_lanes = 0;          // lanes per instance of the tree evaluators: 0 automatic, 1 streaming, 2-16 level-parallel
+    int rollout_grp = 0;         // lanes per instance of the current rollout's level-parallel evaluator (0: not used)
+    int group_stride = 0;        // scratch doubles per instance of the level-parallel evaluator
     RkmCfg cfg{};
 
     DevTables tables() const {
         DevTables T;
         T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
-        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p;
+        T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p; T.bfs = bfs.p; T.lvl = lvl.p;
         return T;
     }
     OpArgs op() const {
@@ -103,7 +107,8 @@ struct tdyn_batch {
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
-        if (rollout_aos) a.rec_stride = stream_stride;
+        a.grp = rollout_aos ? rollout_grp : 0;
+        if (rollout_aos) a.rec_stride = a.grp > 1 ? group_stride : stream_stride;
         return a;
     }
 };

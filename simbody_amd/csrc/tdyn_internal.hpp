@@ -47,9 +47,12 @@ struct DevTables {
     const int* child_idx;    // local body numbers
     const double* prm;       // PRM_STRIDE per global body
     const int4* tree_a;      // per tree: nb, nq, nu, body_off
-    const int4* tree_b;      // per tree: rec_size, nquat, -, -
+    const int4* tree_b;      // per tree: rec_size, nquat, hand-over base of the chain evaluator, lvl offset
     const double* gravity;   // per tree: 4 doubles
     const int4* pre;         // per global body: depth-first preorder of its tree: body, depth, child count, -
+    const int4* bfs;         // per global body: breadth-first order of its tree: body, parent's position
+                             // (-1 = Ground), first child's position, child count
+    const int* lvl;          // per tree at tree_b.w: level count n, then the n+1 level starts in bfs order
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
@@ -124,6 +127,7 @@ struct RkmArgs {
     int vec_aos;             // 1: vectors lane-contiguous (entry k of lane l at ((v*L + l)*ny_max + k)) and
                              // the streaming tree evaluator (batches of different trees)
     int stream_dmax;         // streaming tree evaluator: deepest body of any tree
+    int grp;                 // >1: level-parallel tree evaluator with grp lanes per instance (vec_aos layout)
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that
@@ -136,7 +140,29 @@ enum { ST_W = 24, ACC_W = 27, POS0_W = 64, POS_W = 43 };
 __host__ __device__ inline int scratch_doubles(int dmax, int nbmax) {
     return (ST_W + ACC_W) * (dmax + 1) + POS0_W + POS_W * (nbmax > 1 ? nbmax - 1 : 0);
 }
+// A lane-contiguous state vector and a body's hand-over values (G, DI*eps,
+// trig cache) in a per-lane record.
+struct LaneVec {
+    double* p;
+    __device__ __forceinline__ double& operator()(int k) const { return p[k]; }
+};
+struct LaneStore {
+    double* p; int g, tc, de;
+    __device__ __forceinline__ double& G(int i) const { return p[g + i]; }
+    __device__ __forceinline__ double& T(int i) const { return p[tc + i]; }
+    __device__ __forceinline__ double& DE(int i) const { return p[de + i]; }
+};
 } // namespace tstream
+// Per-instance records of the level-parallel tree evaluator (doubles), one
+// per breadth-first position j: pose R p V (0..17), trig cache (18..21),
+// P+ z+ phi for the parent (22..51), A_GB (52..57), G (58.., 6 per dof) and
+// DI*eps (dof).  Position 0 (body 1, 6 dof if Free) has 100, the others
+// (3 dof, padded) 79.
+namespace tgroup {
+enum { R_POSE = 0, R_TC = 18, R_PZ = 22, R_A = 52, R_G = 58, POS0_W = 100, POS_W = 79 };
+__host__ __device__ inline int rec(int j) { return j == 0 ? 0 : POS0_W + POS_W * (j - 1); }
+__host__ __device__ inline int scratch_doubles(int nbmax) { return rec(nbmax); }
+} // namespace tgroup
 // Entry k of state vector v of a lane, in either layout.
 __device__ __forceinline__ double& vec_at(const RkmArgs& a, int v, int lane, int k) {
     return a.vec_aos ? a.vec[((size_t)v * a.L + lane) * a.ny_max + k] : a.vec[((size_t)v * a.ny_max + k) * a.L + lane];
@@ -157,6 +183,9 @@ inline hipError_t launch_chain_variant(const RkmArgs& a, hipStream_t s, bool f0_
     return a.prm_lane ? launch_chain_fp64_lane(a, s, f0_only) : launch_chain_fp64_batch(a, s, f0_only);
 }
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
+// Level-parallel tree evaluator (a.grp lanes per instance): rollout steps,
+// or (f0_only) the derivative of the current state.
+hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only);
 hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
                             double t0, hipStream_t s);
 hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double* out, int out_stride, int* outi,

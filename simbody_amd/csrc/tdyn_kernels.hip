@@ -813,18 +813,6 @@ struct TreeEval {
 // All per-lane data is lane-contiguous (the lanes of a wave walk different
 // trees, so no access pattern would coalesce across lanes).
 // ---------------------------------------------------------------------------
-namespace tstream {
-struct LaneVec {
-    double* p;
-    __device__ __forceinline__ double& operator()(int k) const { return p[k]; }
-};
-struct LaneStore {
-    double* p; int g, tc, de;
-    __device__ __forceinline__ double& G(int i) const { return p[g + i]; }
-    __device__ __forceinline__ double& T(int i) const { return p[tc + i]; }
-    __device__ __forceinline__ double& DE(int i) const { return p[de + i]; }
-};
-} // namespace tstream
 
 struct TreeStreamEval {
     using VR = tstream::LaneVec;
@@ -1189,6 +1177,7 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     // the derivative is the general form's bit for bit, so identity frames
     // need no separate build here
     if (a.chain) return launch_chain_variant(a, s, true);
+    if (a.grp > 1) return launch_tree_group(a, s, true);
     if (a.vec_aos) k_tree_f0<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
     else if (a.tree_of) k_tree_f0<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_tree_f0<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
@@ -1287,6 +1276,7 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
     if (a.chain) return launch_chain_variant(a, s, false);
+    if (a.grp > 1) return launch_tree_group(a, s, false);
     if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
     else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);

@@ -1,0 +1,413 @@
+// Level-parallel tree evaluator: G lanes (G = 2, 4, 8 or 16, a divisor of the
+// 64-lane wavefront) cooperate on one instance of a batch of different
+// trees.  A batch of a few thousand instances gives a one-lane-per-instance
+// launch a fraction of the chip's 1,024 SIMDs, and a rollout lasts as long
+// as its slowest instance (the largest tree with the most step attempts);
+// splitting each instance over G lanes cuts that instance's time.
+//
+// Each instance's bodies are visited in breadth-first order, one tree level
+// at a time (Simbody's level sweeps, SimbodyMatterSubsystemRep.cpp:
+// realizeArticulatedBodyInertias / calcTreeForwardDynamicsOperator run the
+// same recursions level by level).  The bodies of a level are dealt to the G
+// lanes round-robin (position j to lane j mod G); the levels are separated
+// by a fence, and the lanes of a group are lanes of one wavefront, so no
+// barrier is needed.  The per-body arithmetic is the chain evaluator's pass
+// bodies on padded 3-dof bodies (body 1 may be Free):
+//   pass 1 (top-down): parent pose from the parent's record, this body's
+//          pose, qdot and trig cache out;
+//   pass 2 (bottom-up): the children's P+, z+ pulled from their records and
+//          summed, then P, D, D^-1, G, z, P+, z+ of this body;
+//   pass 3 (top-down): udot = DI eps - G^T A+ and A_GB from the parent's A.
+// The Merson stage combinations, the error norm and the projection are split
+// over the group's lanes by state entry or body, and the norms are summed
+// across the group with butterfly shuffles, so every lane of a group holds
+// the bitwise-same step size and takes the same branches.
+#include <type_traits>
+#include "tdyn_internal.hpp"
+#include "tdyn_math.hpp"
+#include "tdyn_chain.hpp"
+#include "tdyn_rkm_chain.hpp"
+
+namespace tdyn {
+namespace {
+
+using d3 = v3<double>;
+using dm = m33<double>;
+using dv = sv<double>;
+using da = abi<double>;
+
+// Orders the group's memory accesses: what one lane stored before is seen by
+// every lane of the workgroup (here: the wavefront) after.
+__device__ __forceinline__ void grp_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+// Sum over the G lanes of a group (aligned lane blocks of G).  The butterfly
+// adds the same pairs in every lane, so all lanes get bitwise the same sum.
+template <int G> __device__ __forceinline__ double grp_sum(double x) {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) x += __shfl_xor(x, m, 64);
+    return x;
+}
+
+template <int V> using TyTag = std::integral_constant<int, V>;
+
+template <int G>
+struct GroupEval {
+    using VR = tstream::LaneVec;
+    DevTables T;
+    int nb, nq, body_off, nlev, l;
+    const int4* bfs; const int* ls;
+    double* S;
+    d3 grav; const double* fapp;
+    bool free0;
+    static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
+        return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
+    }
+    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group) : T(a.T), l(lane_in_group) {
+        const int4 ta = ldi4(a.T.tree_a, tr);
+        nb = ta.x; nq = ta.y; body_off = ta.w;
+        bfs = a.T.bfs + body_off;
+        const int* lv = a.T.lvl + ldi4(a.T.tree_b, tr).w;
+        nlev = lv[0]; ls = lv + 1;
+        S = a.scratch + (size_t)inst * a.rec_stride;
+        const double* g = a.T.gravity + 4*tr;
+        grav = V3(g[0], g[1], g[2]);
+        fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[inst] : nullptr;
+        free0 = ldi4(a.T.body_a, body_off).y == FREE;
+    }
+    __device__ __forceinline__ tstream::LaneStore store(int j) const {
+        const int o = tgroup::rec(j);
+        return tstream::LaneStore{S, o + tgroup::R_G, o + tgroup::R_TC, o + tgroup::R_G + (j == 0 ? 36 : 18)};
+    }
+    __device__ __forceinline__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
+    __device__ __forceinline__ dv F(int b) const {
+        if (!fapp) return svzero<double>();
+        const double* f = fapp + 6*b;
+        return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
+    }
+    template <int TYPE, bool G_ROWS>
+    __device__ __forceinline__ void load(chain::Pre<double>& x, const VR& y, int4 ba, const tstream::LaneStore& st) const {
+        if constexpr (TYPE == FREE) {
+            x.ty = FREE;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) x.q[k] = y(ba.z + k);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) x.u[k] = y(nq + ba.w + k);
+            x.tc[0] = st.T(0);
+        } else {
+            const int ty = ba.y, d = mob_dof(ty), nqb = mob_nq(ty);
+            x.ty = ty;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x.q[k] = k < nqb ? y(ba.z + k) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x.u[k] = k < d ? y(nq + ba.w + k) : 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x.tc[k] = st.T(k);
+            if constexpr (G_ROWS) {
+#pragma unroll
+                for (int k = 0; k < 18; ++k) x.g[k] = st.G(k);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) x.de[k] = st.DE(k);
+            }
+        }
+    }
+    __device__ __forceinline__ void ld_pose(const double* s, dm& R, d3& p, dv& V) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) R.a[k] = s[k];
+        p = V3(s[9], s[10], s[11]);
+        V = {V3(s[12], s[13], s[14]), V3(s[15], s[16], s[17])};
+    }
+    __device__ __forceinline__ void st_pose(double* s, const dm& R, const d3& p, const dv& V) const {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) s[k] = R.a[k];
+        s[9] = p.x; s[10] = p.y; s[11] = p.z;
+        s[12] = V.w.x; s[13] = V.w.y; s[14] = V.w.z; s[15] = V.v.x; s[16] = V.v.y; s[17] = V.v.z;
+    }
+    // Runs f(TyTag<TYPE>, j) for this lane's positions of level lev:
+    // j = ls[lev] + ... with j = l (mod G); position 0 (body 1) may be Free.
+    template <class Fn> __device__ __forceinline__ void each(int lev, Fn&& f) const {
+        const int s0 = ls[lev], e = ls[lev + 1];
+        int j = s0 + ((l - s0) & (G - 1));
+        if (j == 0) {
+            if (free0) f(TyTag<FREE>{}, 0); else f(TyTag<GEN>{}, 0);
+            j = G;
+        }
+        for (; j < e; j += G) f(TyTag<GEN>{}, j);
+    }
+    __device__ __forceinline__ void derivs(const VR& y, const VR& f) const {
+        using namespace chain;
+        const VR fu{f.p + nq};
+        for (int lev = 0; lev < nlev; ++lev) {
+            each(lev, [&](auto tag, int j) {
+                constexpr int TY = decltype(tag)::value;
+                const int4 e = ldi4(bfs, j);
+                const int b = e.x;
+                const int4 ba = ldi4(T.body_a, body_off + b - 1);
+                dm R; d3 p; dv V;
+                if (e.y < 0) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); }
+                else ld_pose(S + tgroup::rec(e.y), R, p, V);
+                const tstream::LaneStore st = store(j);
+                Pre<double> x;
+                load<TY, false>(x, y, ba, st);
+                pass1_body<TY, double>(prm(b), x, ba.z, f, st, R, p, V);
+                st_pose(S + tgroup::rec(j), R, p, V);
+            });
+            grp_sync();
+        }
+        for (int lev = nlev - 1; lev >= 0; --lev) {
+            each(lev, [&](auto tag, int j) {
+                constexpr int TY = decltype(tag)::value;
+                const int4 e = ldi4(bfs, j);
+                const int b = e.x;
+                const int4 ba = ldi4(T.body_a, body_off + b - 1);
+                double* r = S + tgroup::rec(j);
+                dm R; d3 p; dv V;
+                ld_pose(r, R, p, V);
+                const tstream::LaneStore st = store(j);
+                Pre<double> x;
+                load<TY, false>(x, y, ba, st);
+                da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
+                if (e.w > 0) {
+                    // children's P+ and z+, shifted to this body, in child order
+                    da aP;
+                    aP.M = {0, 0, 0, 0, 0, 0}; aP.J = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) aP.F.a[k] = 0.0;
+                    dv aZ = svzero<double>();
+                    for (int c = e.z; c < e.z + e.w; ++c) {
+                        const double* q = S + tgroup::rec(c) + tgroup::R_PZ;
+                        da Q;
+                        Q.M = {q[0], q[1], q[2], q[3], q[4], q[5]};
+                        Q.J = {q[6], q[7], q[8], q[9], q[10], q[11]};
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) Q.F.a[k] = q[12 + k];
+                        const dv zc = {V3(q[21], q[22], q[23]), V3(q[24], q[25], q[26])};
+                        const d3 ph = V3(q[27], q[28], q[29]);
+                        abi_add_shifted(aP, Q, ph);
+                        aZ = aZ + phi_mul(ph, zc);
+                    }
+                    pass2_body<TY, double>(prm(b), x, grav, F(b), st, R, p, V, false, PP, zp, phic, &aP, &aZ);
+                } else {
+                    pass2_body<TY, double>(prm(b), x, grav, F(b), st, R, p, V, false, PP, zp, phic);
+                }
+                if (e.y >= 0) {
+                    double* q = r + tgroup::R_PZ;
+                    q[0] = PP.M.xx; q[1] = PP.M.yy; q[2] = PP.M.zz; q[3] = PP.M.xy; q[4] = PP.M.xz; q[5] = PP.M.yz;
+                    q[6] = PP.J.xx; q[7] = PP.J.yy; q[8] = PP.J.zz; q[9] = PP.J.xy; q[10] = PP.J.xz; q[11] = PP.J.yz;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) q[12 + k] = PP.F.a[k];
+                    q[21] = zp.w.x; q[22] = zp.w.y; q[23] = zp.w.z; q[24] = zp.v.x; q[25] = zp.v.y; q[26] = zp.v.z;
+                    q[27] = phic.x; q[28] = phic.y; q[29] = phic.z;
+                }
+            });
+            grp_sync();
+        }
+        for (int lev = 0; lev < nlev; ++lev) {
+            each(lev, [&](auto tag, int j) {
+                constexpr int TY = decltype(tag)::value;
+                const int4 e = ldi4(bfs, j);
+                const int b = e.x;
+                const int4 ba = ldi4(T.body_a, body_off + b - 1);
+                dm R; d3 p; dv V, A;
+                if (e.y < 0) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); A = svzero<double>(); }
+                else {
+                    const double* s = S + tgroup::rec(e.y);
+                    ld_pose(s, R, p, V);
+                    A = {V3(s[tgroup::R_A], s[tgroup::R_A + 1], s[tgroup::R_A + 2]),
+                         V3(s[tgroup::R_A + 3], s[tgroup::R_A + 4], s[tgroup::R_A + 5])};
+                }
+                const tstream::LaneStore st = store(j);
+                Pre<double> x;
+                load<TY, true>(x, y, ba, st);
+                pass3_body<TY, double>(prm(b), x, ba.w, fu, st, R, p, V, A);
+                double* s = S + tgroup::rec(j) + tgroup::R_A;
+                s[0] = A.w.x; s[1] = A.w.y; s[2] = A.w.z; s[3] = A.v.x; s[4] = A.v.y; s[5] = A.v.z;
+            });
+            grp_sync();
+        }
+    }
+};
+
+// One group of G lanes runs one instance's TimeStepper loop (the one-lane
+// loop of the other tree evaluators, RungeKuttaMersonIntegratorRep.cpp:
+// 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
+// the group.
+template <int G>
+__global__ void __launch_bounds__(64) k_rkm_grp(RkmArgs a) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int inst = gt / G, l = gt & (G - 1);
+    if (inst >= a.n_inst) return;                // whole groups: G divides the block
+    if (a.status[inst] != 0) return;
+    const int tr = a.tree_of[inst];
+    const GroupEval<G> ev(a, inst, tr, l);
+    const int4 ta = ldi4(a.T.tree_a, tr);
+    const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
+    const int4* body_a = a.T.body_a + ta.w;
+    const int nquat = ldi4(a.T.tree_b, tr).y;
+    using VR = tstream::LaneVec;
+    auto rows = [&](int v) { return GroupEval<G>::vrow(a, v, inst); };
+    const VR Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
+             Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
+
+    const RkmCfg& cfg = a.cfg;
+    const double acc = cfg.accuracy, consTol = cfg.cons_tol;
+    double hmin = cfg.min_h, hmax = cfg.max_h;
+    if (cfg.fixed_h > 0) { hmin = hmax = cfg.fixed_h; }
+    const double projLimit = fmax(2*consTol, sqrt(consTol));
+    double t = a.t[inst], hcur = a.h[inst];
+    const double tEnd = a.t_end[inst];
+    int attempts = a.stats[4*inst], accepted = a.stats[4*inst+1], rejected = a.stats[4*inst+2],
+        projections = a.stats[4*inst+3];
+    int status = 0, steps = 0;
+
+    // error norm at y1 (IntegratorRep.h:454-523, calcRelativeScaling :644-655)
+    auto errNorm = [&]() -> double {
+        double sq = 0;
+        for (int b = l; b < nb; b += G) {
+            const int4 ba = ldi4(body_a, b);
+            const int qi = ba.z;
+            if (ba.y == BALL || ba.y == FREE) {
+                double qq[4], e[4], d4[4];
+                for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
+                const d3 w = quat_dot_to_w(qq, e);
+                quat_dot(qq, w, d4);
+                sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = YE(qi + k); sq += x*x; }
+            } else {
+                for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = YE(qi + k); sq += x*x; }
+            }
+        }
+        double su = 0;
+        for (int k = l; k < nu; k += G) {
+            const double vi = fabs(Y(nq + k));
+            const double us = vi > 1.0 ? 1.0/vi : 1.0;
+            const double x = us * YE(nq + k);
+            su += x*x;
+        }
+        sq = grp_sum<G>(sq); su = grp_sum<G>(su);
+        const double qn = nq ? sqrt(sq / nq) : 0.0;
+        const double un = nu ? sqrt(su / nu) : 0.0;
+        return qn >= un ? qn : un;
+    };
+
+    while (steps < cfg.max_steps) {
+        if (t >= tEnd) { status = 1; break; }
+        bool ok = false;
+        double t1 = t;
+        int s0 = 0;          // f0 = f(y0) once per step
+        do {
+            bool limited = false;
+            if (tEnd < t + 0.95*hcur) { limited = true; t1 = tEnd; }
+            else if (tEnd > t + 1.001*hcur) t1 = t + hcur;
+            else t1 = tEnd;
+            if (!(t1 > t)) { status = -3; break; }
+            if (++attempts > cfg.max_attempts) { status = -2; break; }
+            const double h = t1 - t;
+            for (int st = s0; st < 5; ++st) {
+                VR in = Y, out = F0;
+                if (st == 1) {
+                    for (int k = l; k < ny; k += G) YT(k) = Y(k) + (h/3)*F0(k);
+                    in = YT; out = FA;
+                } else if (st == 2) {
+                    for (int k = l; k < ny; k += G) YT(k) = Y(k) + (h/6)*(F0(k) + FA(k));
+                    in = YT; out = FA;
+                } else if (st == 3) {
+                    for (int k = l; k < ny; k += G) YT(k) = Y(k) + (h/8)*(F0(k) + 3*FA(k));
+                    in = YT; out = FB;
+                } else if (st == 4) {
+                    for (int k = l; k < ny; k += G) YS(k) = Y(k) + (h/2)*((F0(k) - 3*FA(k)) + 4*FB(k));
+                    in = YS; out = FA;
+                }
+                grp_sync();
+                ev.derivs(in, out);
+            }
+            s0 = 1;
+            for (int k = l; k < ny; k += G) {
+                const double y1 = Y(k) + (h/6)*((F0(k) + 4*FB(k)) + FA(k));
+                Y1(k) = y1;
+                YE(k) = 0.2 * fabs(y1 - YS(k));
+            }
+            grp_sync();
+            bool converged = true;
+            const double e1 = errNorm();
+            if (!(e1 > 16.0*acc) && nquat > 0) {
+                double s2 = 0;
+                for (int b = l; b < nb; b += G) {
+                    const int4 ba = ldi4(body_a, b);
+                    if (ba.y != BALL && ba.y != FREE) continue;
+                    double n2 = 0;
+                    for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
+                    const double e = sqrt(n2) - 1.0;
+                    s2 += e*e;
+                }
+                s2 = grp_sum<G>(s2);
+                const double qe = sqrt(s2 / nquat);
+                if (qe > projLimit) converged = false;
+                else if (qe > consTol) {
+                    ++projections;
+                    for (int b = l; b < nb; b += G) {
+                        const int4 ba = ldi4(body_a, b);
+                        if (ba.y != BALL && ba.y != FREE) continue;
+                        double qq[4], n2 = 0;
+                        for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
+                        const double n = sqrt(n2);
+                        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; Y1(ba.z + k) = qq[k]; }
+                        double d = 0;
+                        for (int k = 0; k < 4; ++k) d += YE(ba.z + k) * qq[k];
+                        for (int k = 0; k < 4; ++k) YE(ba.z + k) = YE(ba.z + k) - d*qq[k];
+                    }
+                    grp_sync();
+                }
+            }
+            const double err = converged ? errNorm() : __builtin_inf();
+            ok = adjust_step(err, acc, limited, hmin, hmax, hcur);
+            if (!ok) ++rejected;
+        } while (!ok);
+        if (status < 0) break;
+        t = t1;
+        for (int k = l; k < ny; k += G) Y(k) = Y1(k);
+        grp_sync();
+        ++accepted; ++steps;
+    }
+    if (status == 0 && t >= tEnd) status = 1;
+    if (l == 0) {
+        a.t[inst] = t; a.h[inst] = hcur; a.status[inst] = status;
+        a.stats[4*inst] = attempts; a.stats[4*inst+1] = accepted; a.stats[4*inst+2] = rejected;
+        a.stats[4*inst+3] = projections;
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(64) k_grp_f0(RkmArgs a) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int inst = gt / G, l = gt & (G - 1);
+    if (inst >= a.n_inst) return;
+    const GroupEval<G> ev(a, inst, a.tree_of[inst], l);
+    ev.derivs(GroupEval<G>::vrow(a, VY, inst), GroupEval<G>::vrow(a, VF0, inst));
+}
+
+template <int G>
+hipError_t launch_g(const RkmArgs& a, hipStream_t s, bool f0_only) {
+    const size_t threads = (size_t)a.n_inst * G;
+    const dim3 grd((unsigned)((threads + 63) / 64)), blk(64);
+    if (f0_only) k_grp_f0<G><<<grd, blk, 0, s>>>(a);
+    else k_rkm_grp<G><<<grd, blk, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+} // namespace
+
+hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
+    if (a.n_inst <= 0) return hipSuccess;
+    switch (a.grp) {
+    case 2:  return launch_g<2>(a, s, f0_only);
+    case 4:  return launch_g<4>(a, s, f0_only);
+    case 8:  return launch_g<8>(a, s, f0_only);
+    case 16: return launch_g<16>(a, s, f0_only);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace tdyn

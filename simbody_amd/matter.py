@@ -27,6 +27,7 @@ TDYN_ERRORS = {1: "TDYN_E_ARG", 2: "TDYN_E_SIZE", 3: "TDYN_E_STAGE", 4: "TDYN_E_
                5: "TDYN_E_STEP_FAILED", 6: "TDYN_E_HIP", 7: "TDYN_E_NOMEM"}
 
 EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator", "tdyn_batch_get_evaluator",
+           "tdyn_batch_set_tree_lanes",
            "tdyn_batch_info",
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
@@ -77,7 +78,8 @@ def lib():
                                         ctypes.POINTER(vp)]
         L.tdyn_batch_destroy.argtypes = [vp]
         L.tdyn_batch_set_evaluator.argtypes = [vp, ctypes.c_int]
-        L.tdyn_batch_get_evaluator.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.tdyn_batch_get_evaluator.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.tdyn_batch_set_tree_lanes.argtypes = [vp, ctypes.c_int]
         L.tdyn_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.tdyn_batch_offsets.argtypes = [vp, _i32, _i32, _i32]
         L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
@@ -203,15 +205,23 @@ class MatterBatch:
         """0 automatic, 1 tree sweeps, 2 register-resident serial-chain evaluator."""
         _check(lib().tdyn_batch_set_evaluator(self.h, int(mode)))
 
+    def set_tree_lanes(self, lanes):
+        """Lanes per instance of the tree evaluators: 0 automatic, 1 streaming
+        (one lane per instance), 2/4/8/16 level-parallel."""
+        _check(lib().tdyn_batch_set_tree_lanes(self.h, int(lanes)))
+
     def active_evaluator(self):
-        """Evaluator of the current rollout: 0 none yet, 1 tree sweeps, 2 chain,
-        3 streaming tree evaluator."""
-        v = ctypes.c_int(0)
-        _check(lib().tdyn_batch_get_evaluator(self.h, ctypes.byref(v)))
-        return v.value
+        """(evaluator, lanes per instance) of the current rollout: evaluator 0
+        none yet, 1 tree sweeps, 2 chain, 3 streaming tree evaluator, 4
+        level-parallel tree evaluator."""
+        v = ctypes.c_int(0); n = ctypes.c_int(0)
+        _check(lib().tdyn_batch_get_evaluator(self.h, ctypes.byref(v), ctypes.byref(n)))
+        return v.value, n.value
 
     def uses_tree_stream(self):
-        return self.active_evaluator() == 3
+        """True when the current rollout runs on one of the tree evaluators for
+        batches of different trees (streaming or level-parallel)."""
+        return self.active_evaluator()[0] in (3, 4)
 
     # ---- helpers ---------------------------------------------------------
     def _new(self, n):

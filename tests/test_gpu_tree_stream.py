@@ -1,6 +1,8 @@
-"""Rollouts of batches of different trees run on the streaming tree evaluator
-(each lane walks its own tree in depth-first preorder; lane-contiguous
-state).  Held to the parity bar of test_gpu_parity: the integrator's
+"""Rollouts of batches of different trees run on the tree evaluators for
+such batches: the streaming evaluator (one lane walks its tree depth-first)
+and the level-parallel evaluator (2, 4, 8 or 16 lanes share one instance,
+one tree level at a time); instance-contiguous state.  Held to the parity
+bar of test_gpu_parity: the integrator's
 derivative at the rollout's initial state within 1e-12 relative of the
 oracle's udot and qdot (FP64), and whole rollouts with the oracle's
 accept/reject counts.  Shapes: the largest trees of the workloads (128
@@ -47,14 +49,20 @@ def _states(trees, seed, forces):
     return qs, us, fs
 
 
-def _check_derivs(trees, seed, forces=True, expect_stream=True):
+LANES = [1, 2, 4, 16]
+
+
+def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=0):
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson
     n = len(trees)
     B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
+    B.set_tree_lanes(lanes)
     qs, us, fs = _states(trees, seed, forces)
     B.rollout_begin(B.pack(qs), B.pack(us), RungeKuttaMerson(fixed_step=1e-3), t_end=1.0,
                     fapp=B.pack(fs) if forces else None)
     assert B.uses_tree_stream() == expect_stream
+    if expect_stream and lanes:
+        assert B.active_evaluator() == ((3, 1) if lanes == 1 else (4, lanes))
     q0, u0 = B.rollout_get()[:2]             # quaternions normalized by the integrator's initialize()
     qd, ud = B.rollout_derivs()
     q0s, u0s, qds, uds = B.split_q(q0), B.split_u(u0), B.split_q(qd), B.split_u(ud)
@@ -69,7 +77,8 @@ def _check_derivs(trees, seed, forces=True, expect_stream=True):
     return worst
 
 
-def test_largest_trees_ragged_wave():
+@pytest.mark.parametrize("lanes", LANES)
+def test_largest_trees_ragged_wave(lanes):
     """67 trees of 100..128 bodies (a wave plus 3 lanes), depth up to 20, Free
     roots, all five mobilizers, general joint frames."""
     trees = [M.random_tree(91, i, nmin=100, nmax=128, general_frames=True) for i in range(67)]
@@ -77,26 +86,28 @@ def test_largest_trees_ragged_wave():
     trees[1] = M.random_tree(92, 1, nmin=128, nmax=128, max_depth=20, general_frames=True)
     for k in range(0, 67, 5):
         trees[k].mobilizer[0] = M.FREE
-    _check_derivs(trees, 3)
+    _check_derivs(trees, 3, lanes=lanes)
 
 
-def test_mixed_small_trees_and_one_body_trees():
+@pytest.mark.parametrize("lanes", LANES)
+def test_mixed_small_trees_and_one_body_trees(lanes):
     trees = mixed_trees(17, 40, nmin=1, nmax=24)
     for k, m in enumerate([M.PIN, M.SLIDER, M.UNIVERSAL, M.BALL, M.FREE]):
         trees[k] = M.random_tree(18, k, nmin=1, nmax=1, general_frames=True)
         trees[k].mobilizer[0] = m
-    _check_derivs(trees, 4)
+    _check_derivs(trees, 4, lanes=lanes)
 
 
-def test_binary_trees_and_forests():
+@pytest.mark.parametrize("lanes", LANES)
+def test_binary_trees_and_forests(lanes):
     """Bodies with two children at every depth (the accumulators of every
     level in use), several roots on Ground, a chain, without applied forces."""
     trees = [_binary_tree(7, 5), _binary_tree(5, 6), _binary_tree(1, 7),
              _forest(8, 40, [5, 9, 23]), _forest(9, 64, [2, 3, 4, 40]),
              M.spatial_chain(20, seed=3)]
     trees[3].mobilizer[0] = M.FREE
-    _check_derivs(trees, 5, forces=False)
-    _check_derivs(trees, 6, forces=True)
+    _check_derivs(trees, 5, forces=False, lanes=lanes)
+    _check_derivs(trees, 6, forces=True, lanes=lanes)
 
 
 def test_free_body_not_first_falls_back():
@@ -108,25 +119,28 @@ def test_free_body_not_first_falls_back():
     _check_derivs(trees, 7, expect_stream=False)
 
 
-def test_stream_and_sweeps_rollouts_agree_and_match_oracle():
-    """Adaptive rollouts (accuracy 1e-6, applied forces): the streaming
-    evaluator and the tree sweeps take the same steps; both match the
-    oracle's trajectory."""
+def test_tree_evaluators_rollouts_agree_and_match_oracle():
+    """Adaptive rollouts (accuracy 1e-6, applied forces): the streaming and
+    level-parallel evaluators and the tree sweeps take the same steps; all
+    match the oracle's trajectory."""
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson
     trees = mixed_trees(41, 10, nmin=8, nmax=40) + [_binary_tree(4, 12), _forest(13, 20, [7, 11])]
     n = len(trees)
     qs, us, fs = _states(trees, 41, True)
     outs = []
-    for mode in (0, 1):
+    for mode, lanes in ((1, 0), (0, 1), (0, 2), (0, 8)):
         B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
         B.set_evaluator(mode)
+        B.set_tree_lanes(lanes)
         q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
         outs.append((B.split_q(q1), B.split_u(u1), st.cpu().numpy()))
-    assert (outs[0][2] == outs[1][2]).all()
+    for o in outs[1:]:
+        assert (o[2] == outs[0][2]).all()
     for i, t in enumerate(trees):
-        y_s = np.concatenate([outs[0][0][i], outs[0][1][i]])
-        y_w = np.concatenate([outs[1][0][i], outs[1][1][i]])
-        assert rel_inf(y_s, y_w) < 1e-9, i
         y, s = OracleModel(t).integrate(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, Fapp=fs[i])
         assert (s["accepted"], s["rejected"]) == tuple(outs[0][2][i, 1:3]), (i, s)
-        assert rel_inf(y_s, y) < 1e-8, i
+        y_w = np.concatenate([outs[0][0][i], outs[0][1][i]])
+        for o in outs[1:]:
+            y_e = np.concatenate([o[0][i], o[1][i]])
+            assert rel_inf(y_e, y_w) < 1e-9, i
+            assert rel_inf(y_e, y) < 1e-8, i
