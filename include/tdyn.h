@@ -86,10 +86,11 @@ int tdyn_batch_destroy(tdyn_batch* b);
 int tdyn_batch_set_evaluator(tdyn_batch* b, int mode);
 /* Lanes per instance of the tree evaluators of a batch of different trees
  * (no reference counterpart; a tuning knob): 0 = automatic (the most lanes,
- * up to 16, that keep the launch within one wavefront per SIMD), 1 = the
- * streaming evaluator (one lane walks the tree depth-first), 2, 4, 8, 16 =
- * the level-parallel evaluator (that many lanes share one instance, one tree
- * level at a time).  Takes effect at the next tdyn_rollout_begin. */
+ * up to 16, that keep the launch within about four wavefronts per SIMD),
+ * 1 = the streaming evaluator (one lane walks the tree depth-first), 2, 4, 8,
+ * 16, 32, 64 = the level-parallel evaluator (that many lanes share one
+ * instance, one tree level at a time).  Takes effect at the next
+ * tdyn_rollout_begin. */
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes);
 /* The evaluator of the current rollout (no reference counterpart; for tests
  * and tuning): *active = 0 before the first tdyn_rollout_begin, else 1 tree

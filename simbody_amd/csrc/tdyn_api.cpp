@@ -92,19 +92,22 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
 }
 
 // Lanes per instance of the level-parallel tree evaluator for the next
-// rollout (0: the one-lane streaming evaluator).  Automatic: the most lanes
-// (up to 16) that keep the launch within one wavefront per SIMD, since a
-// rollout lasts as long as its slowest instance and the kernel's register
-// use allows one wavefront per SIMD.
+// rollout (0: the one-lane streaming evaluator).  A rollout lasts as long as
+// its slowest instance, so lanes per instance pay until the launch holds
+// about four wavefronts per SIMD (the kernel's registers allow one resident
+// wavefront per SIMD; the heaviest instances run first, the rest fill in).
+// Measured on 16,384 trees of 8..128 bodies: 1, 2, 4, 8, 16, 32, 64 lanes
+// give 2.6, 3.2, 5.7, 10.0, 12.0-12.5, 11.8, 9.5 x 1e7 body-steps/s.  Above
+// 16 lanes most tree levels are narrower than the group.
 int tree_group_lanes(const tdyn_batch* b) {
     if (b->tree_lanes > 0) return b->tree_lanes == 1 ? 0 : b->tree_lanes;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    const int64_t simds = 4 * (int64_t)cus;
+    const int64_t lanes_target = 4 * 64 * 4 * (int64_t)cus;     // four wavefronts per SIMD
     int g = 1;
-    while (g < 16 && ((int64_t)b->n_inst * 2 * g + 63) / 64 <= simds) g *= 2;
+    while (g < 16 && (int64_t)b->n_inst * 2 * g <= lanes_target) g *= 2;
     return g == 1 ? 0 : g;
 }
 
@@ -151,7 +154,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     auto* B = new tdyn_batch;
     B->n_inst = n_instances; B->n_trees = n_trees;
     B->uniform = (tree_of_instance == nullptr) || n_trees == 1;
-    std::vector<int4> body_b, pre, bfs; std::vector<int> child_idx, lvl; std::vector<double> prm, grav;
+    std::vector<int4> body_b, pre, bfs, bfs_b; std::vector<int> child_idx, lvl; std::vector<double> prm, grav, prm_bfs;
     B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
     int body_off = 0;
@@ -196,6 +199,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         // body, parent's position, first child's position, child count; and
         // the start of every level
         const int lvl_off = (int)lvl.size();
+        size_t prm_base = 0;
         {
             std::vector<int> order(ch[0]), pos(nb + 1, -1), depth(nb + 1, 0);
             std::vector<int> starts{0};
@@ -211,8 +215,21 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
                 const int b = order[k], p = d.parent[b - 1];
                 const int first = ch[b].empty() ? 0 : pos[ch[b][0]];
                 bfs.push_back(make_int4(b, p == 0 ? -1 : pos[p], first, (int)ch[b].size()));
+                const int4 ba = B->h_body_a[body_off + b - 1];
+                bfs_b.push_back(make_int4(ba.y, ba.z, ba.w, b));
             }
+            // model constants in breadth-first order, in blocks of BW positions
+            // stored constant-major
+            const size_t base = prm_bfs.size();
+            const int BW = tdyn::tgroup::BW;
+            prm_bfs.resize(base + (size_t)tdyn::tgroup::blocks(nb) * BW * PRM_STRIDE, 0.0);
+            for (int k = 0; k < nb; ++k)
+                for (int c = 0; c < PRM_STRIDE; ++c)
+                    prm_bfs[base + (size_t)(k / BW) * PRM_STRIDE * BW + (size_t)c * BW + k % BW] =
+                        prm[(size_t)(body_off + order[k] - 1) * PRM_STRIDE + c];
+            prm_base = base;
             lvl.push_back((int)starts.size() - 1);
+            lvl.push_back((int)prm_base);
             lvl.insert(lvl.end(), starts.begin(), starts.end());
         }
         B->h_tree_b.push_back(make_int4(rec, nquat, gslot, lvl_off));
@@ -232,12 +249,14 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         B->max_nq = std::max(B->max_nq, tq[t]); B->max_nu = std::max(B->max_nu, tu[t]);
         B->max_nb = std::max(B->max_nb, tnb[t]);
     }
-    if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
+    if (q > INT32_MAX || u > INT32_MAX || bb * 21 > INT32_MAX || prm_bfs.size() > (size_t)INT32_MAX) { delete B; return fail(TDYN_E_SIZE, "batch too large for 32-bit offsets"); }
     B->total_q = q; B->total_u = u; B->total_b = bb;
     B->is_chain = B->is_chain && B->uniform;
     B->stream_stride = tdyn::tstream::scratch_doubles(B->dmax, B->max_nb);
     B->group_stride = tdyn::tgroup::scratch_doubles(B->max_nb);
     B->idf = B->is_chain && identity_frames(trees[0]);
+    B->idf_all = true;
+    for (int t = 0; t < n_trees && B->idf_all; ++t) B->idf_all = identity_frames(trees[t]);
     // Lane order.  A wave runs as long as its slowest lane, so instances of
     // different trees are assigned to lanes in order of decreasing work
     // (bodies weighted by their mobilizer's cost): the 64 lanes of a wave
@@ -275,6 +294,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->inst.upload(inst_of_lane)) != hipSuccess ||
         (e = B->pre.upload(pre)) != hipSuccess ||
         (e = B->bfs.upload(bfs)) != hipSuccess ||
+        (e = B->bfs_b.upload(bfs_b)) != hipSuccess ||
+        (e = B->prm_bfs.upload(prm_bfs)) != hipSuccess ||
         (e = B->lvl.upload(lvl)) != hipSuccess ||
         (e = B->scratch.alloc((size_t)std::max({B->max_rec + 24, B->stream_ok ? B->stream_stride : 0,
                                                 B->stream_ok ? B->group_stride : 0}) * n_instances)) != hipSuccess) {
@@ -297,8 +318,8 @@ int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
 
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
-    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-        return fail(TDYN_E_ARG, "lanes per instance must be 0 (automatic), 1, 2, 4, 8 or 16");
+    if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+        return fail(TDYN_E_ARG, "lanes per instance must be 0 (automatic) or a power of two up to 64");
     b->tree_lanes = lanes;
     return TDYN_OK;
 }

@@ -56,12 +56,12 @@ struct tdyn_batch {
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
-    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b, pre, bfs;
+    tdyn::host::DBuf<int4> body_a, body_b, tree_a, tree_b, pre, bfs, bfs_b;
     tdyn::host::DBuf<int> lvl;
     tdyn::host::DBuf<int> child_idx, tree_of, q_off, u_off, b_off;   // per LANE
     tdyn::host::DBuf<int> inst;          // instance of each lane
     bool permuted = false;               // lanes are not in instance order
-    tdyn::host::DBuf<double> prm, gravity, scratch;
+    tdyn::host::DBuf<double> prm, gravity, scratch, prm_bfs;
     // rollout state
     tdyn::host::DBuf<double> vec, uscale, t, h, t_end;
     tdyn::host::DBuf<int> status, stats;
@@ -72,6 +72,7 @@ struct tdyn_batch {
     bool rollout_ready = false;
     bool is_chain = false;       // uniform batch whose tree is a serial chain
     bool idf = false;            // tree 0's joint frames are identity rotations (R_PF = I, p_PF = 0, R_BM = I)
+    bool idf_all = false;        // every tree's joint frames are
     int evaluator = 0;           // 0 auto, 1 tree sweeps, 2 chain (register-resident)
     bool stream_ok = false;      // different trees, Free only at body 1: the streaming tree evaluator applies
     int dmax = 0;                // deepest body of any tree
@@ -86,6 +87,7 @@ struct tdyn_batch {
         DevTables T;
         T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
         T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p; T.bfs = bfs.p; T.lvl = lvl.p;
+        T.bfs_b = bfs_b.p; T.prm_bfs = prm_bfs.p;
         return T;
     }
     OpArgs op() const {
@@ -103,8 +105,8 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.idf = idf ? 1 : 0;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
+        a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

@@ -52,7 +52,11 @@ struct DevTables {
     const int4* pre;         // per global body: depth-first preorder of its tree: body, depth, child count, -
     const int4* bfs;         // per global body: breadth-first order of its tree: body, parent's position
                              // (-1 = Ground), first child's position, child count
-    const int* lvl;          // per tree at tree_b.w: level count n, then the n+1 level starts in bfs order
+    const int* lvl;          // per tree at tree_b.w: level count n, prm_bfs offset, then the n+1 level
+                             // starts in bfs order
+    const int4* bfs_b;       // per global body, breadth-first order: mobilizer type, qIndex, uIndex, -
+    const double* prm_bfs;   // per tree at lvl[1]: breadth-first positions in blocks of 16, constant k of
+                             // position j at [(j/16)*16*PRM_STRIDE + k*16 + j%16]
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
@@ -153,15 +157,18 @@ struct LaneStore {
     __device__ __forceinline__ double& DE(int i) const { return p[de + i]; }
 };
 } // namespace tstream
-// Per-instance records of the level-parallel tree evaluator (doubles), one
-// per breadth-first position j: pose R p V (0..17), trig cache (18..21),
-// P+ z+ phi for the parent (22..51), A_GB (52..57), G (58.., 6 per dof) and
-// DI*eps (dof).  Position 0 (body 1, 6 dof if Free) has 100, the others
-// (3 dof, padded) 79.
+// Per-instance records of the level-parallel tree evaluator: NF fields per
+// breadth-first position, in blocks of BW positions stored field-major
+// (field f of position j at (j/BW)*NF*BW + f*BW + j%BW), so that the lanes of
+// a group, which work on consecutive positions, touch consecutive doubles,
+// while each lane still addresses a record's fields by immediate offsets.
+// Fields: pose R p V (0..17), trig cache (18..21), P+ z+ phi for the parent
+// (22..51), A_GB (52..57), G (58..93, 6 per dof) and DI*eps (94..99).
 namespace tgroup {
-enum { R_POSE = 0, R_TC = 18, R_PZ = 22, R_A = 52, R_G = 58, POS0_W = 100, POS_W = 79 };
-__host__ __device__ inline int rec(int j) { return j == 0 ? 0 : POS0_W + POS_W * (j - 1); }
-__host__ __device__ inline int scratch_doubles(int nbmax) { return rec(nbmax); }
+enum { R_POSE = 0, R_TC = 18, R_PZ = 22, R_A = 52, R_G = 58, R_DE = 94, NF = 100, BW = 4 };
+__host__ __device__ inline int blocks(int n) { return (n + BW - 1) / BW; }
+__host__ __device__ inline int scratch_doubles(int nbmax) { return NF * BW * blocks(nbmax); }
+__host__ __device__ inline int at(int j) { return (j / BW) * NF * BW + j % BW; }
 } // namespace tgroup
 // Entry k of state vector v of a lane, in either layout.
 __device__ __forceinline__ double& vec_at(const RkmArgs& a, int v, int lane, int k) {

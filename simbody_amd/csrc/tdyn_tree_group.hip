@@ -1,4 +1,4 @@
-// Level-parallel tree evaluator: G lanes (G = 2, 4, 8 or 16, a divisor of the
+// Level-parallel tree evaluator: G lanes (G = 2, 4, ..., 64, a divisor of the
 // 64-lane wavefront) cooperate on one instance of a batch of different
 // trees.  A batch of a few thousand instances gives a one-lane-per-instance
 // launch a fraction of the chip's 1,024 SIMDs, and a rollout lasts as long
@@ -53,56 +53,75 @@ template <int G> __device__ __forceinline__ double grp_sum(double x) {
 
 template <int V> using TyTag = std::integral_constant<int, V>;
 
-template <int G>
+// Blocked views (stride BW between fields): a position's record fields; the
+// hand-over values of the pass bodies; a body's model constants.
+struct RecS {
+    double* p;
+    __device__ __forceinline__ double& operator[](int f) const { return p[f * tgroup::BW]; }
+};
+struct StoreS {
+    double* p;
+    __device__ __forceinline__ double& G(int i) const { return p[(tgroup::R_G + i) * tgroup::BW]; }
+    __device__ __forceinline__ double& T(int i) const { return p[(tgroup::R_TC + i) * tgroup::BW]; }
+    __device__ __forceinline__ double& DE(int i) const { return p[(tgroup::R_DE + i) * tgroup::BW]; }
+};
+struct PrmS {
+    const double* p;
+    __device__ __forceinline__ double operator[](int k) const { return p[k * tgroup::BW]; }
+    __device__ __forceinline__ PrmS operator+(int k) const { return {p + k * tgroup::BW}; }
+};
+
+template <int G, bool IDF>
 struct GroupEval {
+    using PA = std::conditional_t<IDF, chain::IdF<PrmS>, PrmS>;
     using VR = tstream::LaneVec;
-    DevTables T;
-    int nb, nq, body_off, nlev, l;
-    const int4* bfs; const int* ls;
+    int nb, nq, nlev, l;
+    const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
     double* S;
     d3 grav; const double* fapp;
     bool free0;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
         return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
     }
-    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group) : T(a.T), l(lane_in_group) {
+    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group) : l(lane_in_group) {
         const int4 ta = ldi4(a.T.tree_a, tr);
-        nb = ta.x; nq = ta.y; body_off = ta.w;
-        bfs = a.T.bfs + body_off;
+        nb = ta.x; nq = ta.y;
+        const int body_off = ta.w;
+        bfs = a.T.bfs + body_off; bfs_b = a.T.bfs_b + body_off;
         const int* lv = a.T.lvl + ldi4(a.T.tree_b, tr).w;
-        nlev = lv[0]; ls = lv + 1;
+        nlev = lv[0]; prm = a.T.prm_bfs + lv[1]; ls = lv + 2;
         S = a.scratch + (size_t)inst * a.rec_stride;
         const double* g = a.T.gravity + 4*tr;
         grav = V3(g[0], g[1], g[2]);
         fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[inst] : nullptr;
         free0 = ldi4(a.T.body_a, body_off).y == FREE;
     }
-    __device__ __forceinline__ tstream::LaneStore store(int j) const {
-        const int o = tgroup::rec(j);
-        return tstream::LaneStore{S, o + tgroup::R_G, o + tgroup::R_TC, o + tgroup::R_G + (j == 0 ? 36 : 18)};
+    __device__ __forceinline__ RecS rec(int j) const { return {S + tgroup::at(j)}; }
+    __device__ __forceinline__ StoreS store(int j) const { return {S + tgroup::at(j)}; }
+    __device__ __forceinline__ PA prm_of(int j) const {
+        return PA(PrmS{prm + (j / tgroup::BW) * PRM_STRIDE * tgroup::BW + j % tgroup::BW});
     }
-    __device__ __forceinline__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
     __device__ __forceinline__ dv F(int b) const {
         if (!fapp) return svzero<double>();
         const double* f = fapp + 6*b;
         return {V3(f[0], f[1], f[2]), V3(f[3], f[4], f[5])};
     }
     template <int TYPE, bool G_ROWS>
-    __device__ __forceinline__ void load(chain::Pre<double>& x, const VR& y, int4 ba, const tstream::LaneStore& st) const {
+    __device__ __forceinline__ void load(chain::Pre<double>& x, const VR& y, int4 bt, const StoreS& st) const {
         if constexpr (TYPE == FREE) {
             x.ty = FREE;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) x.q[k] = y(ba.z + k);
+            for (int k = 0; k < 7; ++k) x.q[k] = y(bt.y + k);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) x.u[k] = y(nq + ba.w + k);
+            for (int k = 0; k < 6; ++k) x.u[k] = y(nq + bt.z + k);
             x.tc[0] = st.T(0);
         } else {
-            const int ty = ba.y, d = mob_dof(ty), nqb = mob_nq(ty);
+            const int ty = bt.x, d = mob_dof(ty), nqb = mob_nq(ty);
             x.ty = ty;
 #pragma unroll
-            for (int k = 0; k < 4; ++k) x.q[k] = k < nqb ? y(ba.z + k) : 0.0;
+            for (int k = 0; k < 4; ++k) x.q[k] = k < nqb ? y(bt.y + k) : 0.0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) x.u[k] = k < d ? y(nq + ba.w + k) : 0.0;
+            for (int k = 0; k < 3; ++k) x.u[k] = k < d ? y(nq + bt.z + k) : 0.0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) x.tc[k] = st.T(k);
             if constexpr (G_ROWS) {
@@ -113,13 +132,13 @@ struct GroupEval {
             }
         }
     }
-    __device__ __forceinline__ void ld_pose(const double* s, dm& R, d3& p, dv& V) const {
+    __device__ __forceinline__ void ld_pose(const RecS& s, dm& R, d3& p, dv& V) const {
 #pragma unroll
         for (int k = 0; k < 9; ++k) R.a[k] = s[k];
         p = V3(s[9], s[10], s[11]);
         V = {V3(s[12], s[13], s[14]), V3(s[15], s[16], s[17])};
     }
-    __device__ __forceinline__ void st_pose(double* s, const dm& R, const d3& p, const dv& V) const {
+    __device__ __forceinline__ void st_pose(const RecS& s, const dm& R, const d3& p, const dv& V) const {
 #pragma unroll
         for (int k = 0; k < 9; ++k) s[k] = R.a[k];
         s[9] = p.x; s[10] = p.y; s[11] = p.z;
@@ -142,32 +161,28 @@ struct GroupEval {
         for (int lev = 0; lev < nlev; ++lev) {
             each(lev, [&](auto tag, int j) {
                 constexpr int TY = decltype(tag)::value;
-                const int4 e = ldi4(bfs, j);
-                const int b = e.x;
-                const int4 ba = ldi4(T.body_a, body_off + b - 1);
+                const int4 e = ldi4(bfs, j), bt = ldi4(bfs_b, j);
                 dm R; d3 p; dv V;
                 if (e.y < 0) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); }
-                else ld_pose(S + tgroup::rec(e.y), R, p, V);
-                const tstream::LaneStore st = store(j);
+                else ld_pose(rec(e.y), R, p, V);
+                const StoreS st = store(j);
                 Pre<double> x;
-                load<TY, false>(x, y, ba, st);
-                pass1_body<TY, double>(prm(b), x, ba.z, f, st, R, p, V);
-                st_pose(S + tgroup::rec(j), R, p, V);
+                load<TY, false>(x, y, bt, st);
+                pass1_body<TY, double>(prm_of(j), x, bt.y, f, st, R, p, V);
+                st_pose(rec(j), R, p, V);
             });
             grp_sync();
         }
         for (int lev = nlev - 1; lev >= 0; --lev) {
             each(lev, [&](auto tag, int j) {
                 constexpr int TY = decltype(tag)::value;
-                const int4 e = ldi4(bfs, j);
-                const int b = e.x;
-                const int4 ba = ldi4(T.body_a, body_off + b - 1);
-                double* r = S + tgroup::rec(j);
+                const int4 e = ldi4(bfs, j), bt = ldi4(bfs_b, j);
+                const RecS r = rec(j);
                 dm R; d3 p; dv V;
                 ld_pose(r, R, p, V);
-                const tstream::LaneStore st = store(j);
+                const StoreS st = store(j);
                 Pre<double> x;
-                load<TY, false>(x, y, ba, st);
+                load<TY, false>(x, y, bt, st);
                 da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
                 if (e.w > 0) {
                     // children's P+ and z+, shifted to this body, in child order
@@ -177,7 +192,7 @@ struct GroupEval {
                     for (int k = 0; k < 9; ++k) aP.F.a[k] = 0.0;
                     dv aZ = svzero<double>();
                     for (int c = e.z; c < e.z + e.w; ++c) {
-                        const double* q = S + tgroup::rec(c) + tgroup::R_PZ;
+                        const RecS q{S + tgroup::at(c) + tgroup::R_PZ * tgroup::BW};
                         da Q;
                         Q.M = {q[0], q[1], q[2], q[3], q[4], q[5]};
                         Q.J = {q[6], q[7], q[8], q[9], q[10], q[11]};
@@ -188,12 +203,13 @@ struct GroupEval {
                         abi_add_shifted(aP, Q, ph);
                         aZ = aZ + phi_mul(ph, zc);
                     }
-                    pass2_body<TY, double>(prm(b), x, grav, F(b), st, R, p, V, false, PP, zp, phic, &aP, &aZ);
+                    pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic,
+                                           &aP, &aZ);
                 } else {
-                    pass2_body<TY, double>(prm(b), x, grav, F(b), st, R, p, V, false, PP, zp, phic);
+                    pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic);
                 }
                 if (e.y >= 0) {
-                    double* q = r + tgroup::R_PZ;
+                    const RecS q{S + tgroup::at(j) + tgroup::R_PZ * tgroup::BW};
                     q[0] = PP.M.xx; q[1] = PP.M.yy; q[2] = PP.M.zz; q[3] = PP.M.xy; q[4] = PP.M.xz; q[5] = PP.M.yz;
                     q[6] = PP.J.xx; q[7] = PP.J.yy; q[8] = PP.J.zz; q[9] = PP.J.xy; q[10] = PP.J.xz; q[11] = PP.J.yz;
 #pragma unroll
@@ -207,23 +223,22 @@ struct GroupEval {
         for (int lev = 0; lev < nlev; ++lev) {
             each(lev, [&](auto tag, int j) {
                 constexpr int TY = decltype(tag)::value;
-                const int4 e = ldi4(bfs, j);
-                const int b = e.x;
-                const int4 ba = ldi4(T.body_a, body_off + b - 1);
+                const int4 e = ldi4(bfs, j), bt = ldi4(bfs_b, j);
                 dm R; d3 p; dv V, A;
                 if (e.y < 0) { R = ident<double>(); p = zero3<double>(); V = svzero<double>(); A = svzero<double>(); }
                 else {
-                    const double* s = S + tgroup::rec(e.y);
+                    const RecS s = rec(e.y);
                     ld_pose(s, R, p, V);
                     A = {V3(s[tgroup::R_A], s[tgroup::R_A + 1], s[tgroup::R_A + 2]),
                          V3(s[tgroup::R_A + 3], s[tgroup::R_A + 4], s[tgroup::R_A + 5])};
                 }
-                const tstream::LaneStore st = store(j);
+                const StoreS st = store(j);
                 Pre<double> x;
-                load<TY, true>(x, y, ba, st);
-                pass3_body<TY, double>(prm(b), x, ba.w, fu, st, R, p, V, A);
-                double* s = S + tgroup::rec(j) + tgroup::R_A;
-                s[0] = A.w.x; s[1] = A.w.y; s[2] = A.w.z; s[3] = A.v.x; s[4] = A.v.y; s[5] = A.v.z;
+                load<TY, true>(x, y, bt, st);
+                pass3_body<TY, double>(prm_of(j), x, bt.z, fu, st, R, p, V, A);
+                const RecS s = rec(j);
+                s[tgroup::R_A] = A.w.x; s[tgroup::R_A + 1] = A.w.y; s[tgroup::R_A + 2] = A.w.z;
+                s[tgroup::R_A + 3] = A.v.x; s[tgroup::R_A + 4] = A.v.y; s[tgroup::R_A + 5] = A.v.z;
             });
             grp_sync();
         }
@@ -234,20 +249,20 @@ struct GroupEval {
 // loop of the other tree evaluators, RungeKuttaMersonIntegratorRep.cpp:
 // 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
 // the group.
-template <int G>
+template <int G, bool IDF>
 __global__ void __launch_bounds__(64) k_rkm_grp(RkmArgs a) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int inst = gt / G, l = gt & (G - 1);
     if (inst >= a.n_inst) return;                // whole groups: G divides the block
     if (a.status[inst] != 0) return;
     const int tr = a.tree_of[inst];
-    const GroupEval<G> ev(a, inst, tr, l);
+    const GroupEval<G, IDF> ev(a, inst, tr, l);
     const int4 ta = ldi4(a.T.tree_a, tr);
     const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
     const int4* body_a = a.T.body_a + ta.w;
     const int nquat = ldi4(a.T.tree_b, tr).y;
     using VR = tstream::LaneVec;
-    auto rows = [&](int v) { return GroupEval<G>::vrow(a, v, inst); };
+    auto rows = [&](int v) { return GroupEval<G, IDF>::vrow(a, v, inst); };
     const VR Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
              Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
 
@@ -379,21 +394,29 @@ __global__ void __launch_bounds__(64) k_rkm_grp(RkmArgs a) {
     }
 }
 
-template <int G>
+template <int G, bool IDF>
 __global__ void __launch_bounds__(64) k_grp_f0(RkmArgs a) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int inst = gt / G, l = gt & (G - 1);
     if (inst >= a.n_inst) return;
-    const GroupEval<G> ev(a, inst, a.tree_of[inst], l);
-    ev.derivs(GroupEval<G>::vrow(a, VY, inst), GroupEval<G>::vrow(a, VF0, inst));
+    const GroupEval<G, IDF> ev(a, inst, a.tree_of[inst], l);
+    ev.derivs(GroupEval<G, IDF>::vrow(a, VY, inst), GroupEval<G, IDF>::vrow(a, VF0, inst));
 }
 
+// a.idf: every tree's joint frames are identity rotations (Simbody's
+// noR_PF/noX_MB node forms), the frame products are skipped; the results
+// are bitwise those of the general form.
 template <int G>
 hipError_t launch_g(const RkmArgs& a, hipStream_t s, bool f0_only) {
     const size_t threads = (size_t)a.n_inst * G;
     const dim3 grd((unsigned)((threads + 63) / 64)), blk(64);
-    if (f0_only) k_grp_f0<G><<<grd, blk, 0, s>>>(a);
-    else k_rkm_grp<G><<<grd, blk, 0, s>>>(a);
+    if (a.idf) {
+        if (f0_only) k_grp_f0<G, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_grp<G, true><<<grd, blk, 0, s>>>(a);
+    } else {
+        if (f0_only) k_grp_f0<G, false><<<grd, blk, 0, s>>>(a);
+        else k_rkm_grp<G, false><<<grd, blk, 0, s>>>(a);
+    }
     return hipGetLastError();
 }
 
@@ -406,6 +429,8 @@ hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     case 4:  return launch_g<4>(a, s, f0_only);
     case 8:  return launch_g<8>(a, s, f0_only);
     case 16: return launch_g<16>(a, s, f0_only);
+    case 32: return launch_g<32>(a, s, f0_only);
+    case 64: return launch_g<64>(a, s, f0_only);
     default: return hipErrorInvalidValue;
     }
 }

@@ -49,7 +49,7 @@ def _states(trees, seed, forces):
     return qs, us, fs
 
 
-LANES = [1, 2, 4, 16]
+LANES = [1, 2, 4, 16, 64]
 
 
 def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=0):
@@ -144,3 +144,25 @@ def test_tree_evaluators_rollouts_agree_and_match_oracle():
             y_e = np.concatenate([o[0][i], o[1][i]])
             assert rel_inf(y_e, y_w) < 1e-9, i
             assert rel_inf(y_e, y) < 1e-8, i
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16])
+def test_identity_frame_trees(lanes):
+    """Trees of the uneven-trees workload (identity joint frames, so the
+    level-parallel evaluator skips the frame products): derivative and an
+    adaptive rollout against the oracle."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    trees = [M.random_tree(95, i, nmin=8, nmax=128) for i in range(70)]
+    for k in range(0, 70, 9):
+        trees[k].mobilizer[0] = M.FREE
+    _check_derivs(trees, 8, lanes=lanes)
+    sub = trees[:8]
+    qs, us, fs = _states(sub, 9, True)
+    B = MatterBatch(sub, tree_of=np.arange(len(sub), dtype=np.int32))
+    B.set_tree_lanes(lanes)
+    q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.05, fapp=B.pack(fs))
+    q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
+    for i, t in enumerate(sub):
+        y, s = OracleModel(t).integrate(np.concatenate([qs[i], us[i]]), 0.05, 1e-6, Fapp=fs[i])
+        assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
+        assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, i
