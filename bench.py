@@ -116,7 +116,7 @@ def _evaluator_name(B):
     return f"{name}, {lanes} lane(s) per instance"
 
 
-def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0):
+def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0, lane_range=(0, 32)):
     """Uneven branched trees (nb ~ U{8..128}, depth <= 20, Pin/Slider/
     Universal/Ball + Free roots), constant N(0,1) body forces and torques,
     per-instance RK-Merson step control at accuracy 1e-6 to t = 0.5 s, in
@@ -128,6 +128,7 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     nbs = np.array([t.nb for t in trees])
     B = MatterBatch(trees, tree_of=np.arange(n), device="cuda:0")
     B.set_tree_lanes(lanes)
+    B.set_tree_lane_range(*lane_range)
     q = B.pack([y[:t.nq] for y, t in zip(Ys, trees)]); u = B.pack([y[t.nq:] for y, t in zip(Ys, trees)])
     F = B.pack(Fs)
     integ = RungeKuttaMerson(accuracy=1e-6)
@@ -283,6 +284,8 @@ def main():
                     help="run only one secondary workload and print its JSON")
     ap.add_argument("--tree-lanes", type=int, default=0,
                     help="lanes per instance of the tree evaluator (0 automatic, 1, 2, 4, 8, 16)")
+    ap.add_argument("--tree-lane-range", type=int, nargs=2, default=[0, 32],
+                    help="automatic lanes per instance: min (0: from the batch size) and max")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
@@ -309,7 +312,7 @@ def main():
         return
     if args.only == "trees":
         print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu,
-                                   lanes=args.tree_lanes)), flush=True)
+                                   lanes=args.tree_lanes, lane_range=tuple(args.tree_lane_range))), flush=True)
         return
     if args.only == "mixed":
         print(json.dumps(row_mixed(torch, M, MatterBatch, RungeKuttaMerson)), flush=True)

@@ -85,17 +85,22 @@ int tdyn_batch_destroy(tdyn_batch* b);
  * batch is one chain model).  Takes effect at the next tdyn_rollout_begin. */
 int tdyn_batch_set_evaluator(tdyn_batch* b, int mode);
 /* Lanes per instance of the tree evaluators of a batch of different trees
- * (no reference counterpart; a tuning knob): 0 = automatic (the most lanes,
- * up to 16, that keep the launch within about four wavefronts per SIMD),
- * 1 = the streaming evaluator (one lane walks the tree depth-first), 2, 4, 8,
- * 16, 32, 64 = the level-parallel evaluator (that many lanes share one
- * instance, one tree level at a time).  Takes effect at the next
- * tdyn_rollout_begin. */
+ * (no reference counterpart; a tuning knob): 0 = automatic (the level-
+ * parallel evaluator with, per instance, its tree's widest level rounded up
+ * to a power of two, clamped to the lane range below), 1 = the streaming
+ * evaluator (one lane walks the tree depth-first), 4, 8, 16, 32, 64 = the
+ * level-parallel evaluator with that many lanes for every instance.  Takes
+ * effect at the next tdyn_rollout_begin. */
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes);
+/* Lane range of the automatic setting (no reference counterpart; a tuning
+ * knob): lanes_min 0 (default: 4 to 16 from the batch size, about four
+ * wavefronts per SIMD) or 4..64, lanes_max 4..64 (default 32); powers of two. */
+int tdyn_batch_set_tree_lane_range(tdyn_batch* b, int lanes_min, int lanes_max);
 /* The evaluator of the current rollout (no reference counterpart; for tests
  * and tuning): *active = 0 before the first tdyn_rollout_begin, else 1 tree
  * sweeps, 2 chain evaluator, 3 streaming tree evaluator, 4 level-parallel
- * tree evaluator; *lanes (nullable) = lanes per instance. */
+ * tree evaluator; *lanes (nullable) = lanes per instance (0: chosen per
+ * instance). */
 int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes);
 
 /* info[0..7] = n_instances, total q, total u, total bodies (incl. Ground),

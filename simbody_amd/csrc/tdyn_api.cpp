@@ -91,24 +91,50 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
     return TDYN_OK;
 }
 
-// Lanes per instance of the level-parallel tree evaluator for the next
-// rollout (0: the one-lane streaming evaluator).  A rollout lasts as long as
-// its slowest instance, so lanes per instance pay until the launch holds
-// about four wavefronts per SIMD (the kernel's registers allow one resident
-// wavefront per SIMD; the heaviest instances run first, the rest fill in).
-// Measured on 16,384 trees of 8..128 bodies: 1, 2, 4, 8, 16, 32, 64 lanes
-// give 2.6, 3.2, 5.7, 10.0, 12.0-12.5, 11.8, 9.5 x 1e7 body-steps/s.  Above
-// 16 lanes most tree levels are narrower than the group.
-int tree_group_lanes(const tdyn_batch* b) {
-    if (b->tree_lanes > 0) return b->tree_lanes == 1 ? 0 : b->tree_lanes;
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        cus = 256;
-    const int64_t lanes_target = 4 * 64 * 4 * (int64_t)cus;     // four wavefronts per SIMD
-    int g = 1;
-    while (g < 16 && (int64_t)b->n_inst * 2 * g <= lanes_target) g *= 2;
-    return g == 1 ? 0 : g;
+// Wavefront plan of the level-parallel tree evaluator for the next rollout
+// (returns 0 and leaves the plan empty for the one-lane streaming
+// evaluator).  Lanes per instance G: the tree_lanes setting, or automatic:
+// the widest level of the instance's tree rounded up to a power of two,
+// clamped to [lanes_min, lanes_max].  A rollout lasts as long as its slowest
+// instance, and the largest trees are the slowest; wide trees get lanes for
+// all the bodies of a level, and their wavefronts come first in the grid.
+// lanes_min 0: the most lanes (up to 16) that keep the grid within about
+// four wavefronts per SIMD (the kernel's registers allow one resident
+// wavefront per SIMD).  Measured on 16,384 trees of 8..128 bodies (body-
+// steps/s): 4, 8, 16, 32, 64 lanes for every instance 0.56, 1.06, 1.32,
+// 1.27, 0.95 x 1e8; per instance in [16, 32] 1.31e8, [8, 64] 1.23e8.
+int build_tree_plan(tdyn_batch* b, hipStream_t s) {
+    if (b->tree_lanes == 1) return 0;
+    int gmin = b->lanes_min, gmax = b->lanes_max;
+    if (gmin <= 0) {
+        const int64_t lanes_target = 4 * 64 * (int64_t)b->n_simd;
+        gmin = 4;
+        while (gmin < 16 && (int64_t)b->n_inst * 2 * gmin <= lanes_target) gmin *= 2;
+    }
+    gmax = std::max(gmax, gmin);
+    const int L = b->n_inst;
+    std::vector<int> g(L);
+    for (int l = 0; l < L; ++l) {
+        int w = 4;
+        while (w < b->h_tree_maxw[b->h_lane_tree[l]]) w *= 2;
+        g[l] = b->tree_lanes > 1 ? b->tree_lanes : std::min(gmax, std::max(gmin, w));
+    }
+    std::vector<int4> plan; std::vector<int> idx;
+    for (int G = 64; G >= 4; G /= 2) {
+        std::vector<int> mine;
+        for (int l = 0; l < L; ++l) if (g[l] == G) mine.push_back(l);
+        for (size_t k = 0; k < mine.size(); k += 64 / G) {
+            const int cnt = (int)std::min(mine.size() - k, (size_t)(64 / G));
+            plan.push_back(make_int4(G, (int)idx.size(), cnt, 0));
+            idx.insert(idx.end(), mine.begin() + k, mine.begin() + k + cnt);
+        }
+    }
+    hipError_t e;
+    if ((e = b->plan.upload(plan)) != hipSuccess || (e = b->plan_idx.upload(idx)) != hipSuccess)
+        return hipfail(e, "tree evaluator plan upload");
+    b->plan_waves = (int)plan.size();
+    (void)s;
+    return 0;
 }
 
 int ensure_rollout_buffers(tdyn_batch* b) {
@@ -228,6 +254,9 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
                     prm_bfs[base + (size_t)(k / BW) * PRM_STRIDE * BW + (size_t)c * BW + k % BW] =
                         prm[(size_t)(body_off + order[k] - 1) * PRM_STRIDE + c];
             prm_base = base;
+            int maxw = 0;
+            for (size_t k = 0; k + 1 < starts.size(); ++k) maxw = std::max(maxw, starts[k + 1] - starts[k]);
+            B->h_tree_maxw.push_back(maxw);
             lvl.push_back((int)starts.size() - 1);
             lvl.push_back((int)prm_base);
             lvl.insert(lvl.end(), starts.begin(), starts.end());
@@ -255,6 +284,12 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     B->stream_stride = tdyn::tstream::scratch_doubles(B->dmax, B->max_nb);
     B->group_stride = tdyn::tgroup::scratch_doubles(B->max_nb);
     B->idf = B->is_chain && identity_frames(trees[0]);
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            B->n_simd = 4 * cus;
+    }
     B->idf_all = true;
     for (int t = 0; t < n_trees && B->idf_all; ++t) B->idf_all = identity_frames(trees[t]);
     // Lane order.  A wave runs as long as its slowest lane, so instances of
@@ -279,6 +314,9 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         const int i = inst_of_lane[l];
         lane_tree[l] = B->h_tree_of[i]; lane_q[l] = B->h_qoff[i]; lane_u[l] = B->h_uoff[i]; lane_b[l] = B->h_boff[i];
     }
+    B->h_lane_tree = lane_tree;
+    {
+    }
     hipError_t e;
     if ((e = B->body_a.upload(B->h_body_a)) != hipSuccess ||
         (e = B->body_b.upload(body_b)) != hipSuccess ||
@@ -294,6 +332,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->inst.upload(inst_of_lane)) != hipSuccess ||
         (e = B->pre.upload(pre)) != hipSuccess ||
         (e = B->bfs.upload(bfs)) != hipSuccess ||
+
         (e = B->bfs_b.upload(bfs_b)) != hipSuccess ||
         (e = B->prm_bfs.upload(prm_bfs)) != hipSuccess ||
         (e = B->lvl.upload(lvl)) != hipSuccess ||
@@ -318,9 +357,18 @@ int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
 
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
-    if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)) != 0)
-        return fail(TDYN_E_ARG, "lanes per instance must be 0 (automatic) or a power of two up to 64");
+    if (!(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64))
+        return fail(TDYN_E_ARG, "lanes per instance must be 0 (automatic), 1, 4, 8, 16, 32 or 64");
     b->tree_lanes = lanes;
+    return TDYN_OK;
+}
+
+int tdyn_batch_set_tree_lane_range(tdyn_batch* b, int lanes_min, int lanes_max) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    auto ok = [](int v) { return v == 4 || v == 8 || v == 16 || v == 32 || v == 64; };
+    if (!(lanes_min == 0 || ok(lanes_min)) || !ok(lanes_max) || (lanes_min && lanes_min > lanes_max))
+        return fail(TDYN_E_ARG, "lane range: lanes_min 0 or 4..64, lanes_max 4..64, powers of two, min <= max");
+    b->lanes_min = lanes_min; b->lanes_max = lanes_max;
     return TDYN_OK;
 }
 
@@ -329,8 +377,8 @@ int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes) {
     int act = 0, ln = 0;
     if (b->rollout_ready) {
         const RkmArgs a = b->rk();
-        act = a.chain ? 2 : a.grp > 1 ? 4 : a.vec_aos ? 3 : 1;
-        ln = act == 4 ? a.grp : 1;
+        act = a.chain ? 2 : a.grp ? 4 : a.vec_aos ? 3 : 1;
+        ln = act == 4 ? (b->tree_lanes > 1 ? b->tree_lanes : 0) : 1;
     }
     *active = act;
     if (lanes) *lanes = ln;
@@ -430,7 +478,12 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
     const size_t L = b->n_inst;
     b->rollout_aos = b->stream_ok && b->evaluator == 0;
-    b->rollout_grp = b->rollout_aos ? tree_group_lanes(b) : 0;
+    b->rollout_grp = 0;
+    if (b->rollout_aos && b->tree_lanes != 1) {
+        const int rc = build_tree_plan(b, s);
+        if (rc) return rc;
+        b->rollout_grp = 1;
+    }
     { const int rc = ensure_rollout_buffers(b); if (rc) return rc; }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));

@@ -78,9 +78,14 @@ struct tdyn_batch {
     int dmax = 0;                // deepest body of any tree
     int stream_stride = 0;       // scratch doubles per lane of the streaming tree evaluator
     bool rollout_aos = false;    // the current rollout keeps lane-contiguous vectors (streaming tree evaluator)
-    int tree_lanes = 0;          // lanes per instance of the tree evaluators: 0 automatic, 1 streaming, 2-16 level-parallel
-    int rollout_grp = 0;         // lanes per instance of the current rollout's level-parallel evaluator (0: not used)
+    int tree_lanes = 0;          // lanes per instance of the tree evaluators: 0 automatic, 1 streaming, 4-64 level-parallel
+    int lanes_min = 0, lanes_max = 32;   // automatic lanes: range (lanes_min 0: from the batch size)
+    int rollout_grp = 0;         // the current rollout runs the level-parallel evaluator (1) or not (0)
     int group_stride = 0;        // scratch doubles per instance of the level-parallel evaluator
+    int n_simd = 1024;           // SIMDs of the device (4 per compute unit)
+    std::vector<int> h_lane_tree, h_tree_maxw;   // tree of each lane; widest level of each tree
+    tdyn::host::DBuf<int4> plan; tdyn::host::DBuf<int> plan_idx;   // wavefront plan of the level-parallel evaluator
+    int plan_waves = 0;
     RkmCfg cfg{};
 
     DevTables tables() const {
@@ -110,7 +115,8 @@ struct tdyn_batch {
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;
-        if (rollout_aos) a.rec_stride = a.grp > 1 ? group_stride : stream_stride;
+        a.plan = plan.p; a.plan_idx = plan_idx.p; a.plan_waves = plan_waves;
+        if (rollout_aos) a.rec_stride = a.grp ? group_stride : stream_stride;
         return a;
     }
 };

@@ -131,7 +131,11 @@ struct RkmArgs {
     int vec_aos;             // 1: vectors lane-contiguous (entry k of lane l at ((v*L + l)*ny_max + k)) and
                              // the streaming tree evaluator (batches of different trees)
     int stream_dmax;         // streaming tree evaluator: deepest body of any tree
-    int grp;                 // >1: level-parallel tree evaluator with grp lanes per instance (vec_aos layout)
+    int grp;                 // >1: level-parallel tree evaluator (vec_aos layout), run by the plan below
+    const int4* plan;        // level-parallel evaluator, per wavefront: lanes per instance G, first entry of
+                             // plan_idx, instance count (<= 64/G)
+    const int* plan_idx;     // instances (lane slots) in plan order
+    int plan_waves;
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that
@@ -190,8 +194,8 @@ inline hipError_t launch_chain_variant(const RkmArgs& a, hipStream_t s, bool f0_
     return a.prm_lane ? launch_chain_fp64_lane(a, s, f0_only) : launch_chain_fp64_batch(a, s, f0_only);
 }
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
-// Level-parallel tree evaluator (a.grp lanes per instance): rollout steps,
-// or (f0_only) the derivative of the current state.
+// Level-parallel tree evaluator (wavefront plan a.plan): rollout steps, or
+// (f0_only) the derivative of the current state.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only);
 hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
                             double t0, hipStream_t s);

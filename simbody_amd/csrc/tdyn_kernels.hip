@@ -1177,7 +1177,7 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
     // the derivative is the general form's bit for bit, so identity frames
     // need no separate build here
     if (a.chain) return launch_chain_variant(a, s, true);
-    if (a.grp > 1) return launch_tree_group(a, s, true);
+    if (a.grp) return launch_tree_group(a, s, true);
     if (a.vec_aos) k_tree_f0<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
     else if (a.tree_of) k_tree_f0<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_tree_f0<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);
@@ -1276,7 +1276,7 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
     if (a.chain) return launch_chain_variant(a, s, false);
-    if (a.grp > 1) return launch_tree_group(a, s, false);
+    if (a.grp) return launch_tree_group(a, s, false);
     if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
     else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);

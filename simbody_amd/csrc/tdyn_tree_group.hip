@@ -1,6 +1,6 @@
-// Level-parallel tree evaluator: G lanes (G = 2, 4, ..., 64, a divisor of the
-// 64-lane wavefront) cooperate on one instance of a batch of different
-// trees.  A batch of a few thousand instances gives a one-lane-per-instance
+// Level-parallel tree evaluator: G lanes (G = 4, 8, 16, 32 or 64, a divisor
+// of the 64-lane wavefront) cooperate on one instance of a batch of
+// different trees.  A batch of a few thousand instances gives a one-lane-per-instance
 // launch a fraction of the chip's 1,024 SIMDs, and a rollout lasts as long
 // as its slowest instance (the largest tree with the most step attempts);
 // splitting each instance over G lanes cuts that instance's time.
@@ -248,13 +248,9 @@ struct GroupEval {
 // One group of G lanes runs one instance's TimeStepper loop (the one-lane
 // loop of the other tree evaluators, RungeKuttaMersonIntegratorRep.cpp:
 // 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
-// the group.
+// the group.  l: the lane's index in the group.
 template <int G, bool IDF>
-__global__ void __launch_bounds__(64) k_rkm_grp(RkmArgs a) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int inst = gt / G, l = gt & (G - 1);
-    if (inst >= a.n_inst) return;                // whole groups: G divides the block
-    if (a.status[inst] != 0) return;
+__device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) {
     const int tr = a.tree_of[inst];
     const GroupEval<G, IDF> ev(a, inst, tr, l);
     const int4 ta = ldi4(a.T.tree_a, tr);
@@ -394,45 +390,52 @@ __global__ void __launch_bounds__(64) k_rkm_grp(RkmArgs a) {
     }
 }
 
-template <int G, bool IDF>
-__global__ void __launch_bounds__(64) k_grp_f0(RkmArgs a) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int inst = gt / G, l = gt & (G - 1);
-    if (inst >= a.n_inst) return;
-    const GroupEval<G, IDF> ev(a, inst, a.tree_of[inst], l);
-    ev.derivs(GroupEval<G, IDF>::vrow(a, VY, inst), GroupEval<G, IDF>::vrow(a, VF0, inst));
-}
-
-// a.idf: every tree's joint frames are identity rotations (Simbody's
-// noR_PF/noX_MB node forms), the frame products are skipped; the results
-// are bitwise those of the general form.
-template <int G>
-hipError_t launch_g(const RkmArgs& a, hipStream_t s, bool f0_only) {
-    const size_t threads = (size_t)a.n_inst * G;
-    const dim3 grd((unsigned)((threads + 63) / 64)), blk(64);
-    if (a.idf) {
-        if (f0_only) k_grp_f0<G, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_grp<G, true><<<grd, blk, 0, s>>>(a);
+// Work distribution: a plan of wavefronts, each running up to 64/G
+// instances with G lanes each (G chosen per wavefront by the host from the
+// width of the instances' tree levels, so that the largest trees, which
+// bound a rollout's duration, get the most lanes); the wavefronts of the
+// widest trees come first in the grid.
+template <int G, bool IDF, bool F0>
+__device__ __forceinline__ void plan_wave(const RkmArgs& a, int start, int count) {
+    const int grp = threadIdx.x / G, l = threadIdx.x & (G - 1);
+    if (grp >= count) return;
+    const int inst = a.plan_idx[start + grp];
+    if constexpr (F0) {
+        const GroupEval<G, IDF> ev(a, inst, a.tree_of[inst], l);
+        ev.derivs(GroupEval<G, IDF>::vrow(a, VY, inst), GroupEval<G, IDF>::vrow(a, VF0, inst));
     } else {
-        if (f0_only) k_grp_f0<G, false><<<grd, blk, 0, s>>>(a);
-        else k_rkm_grp<G, false><<<grd, blk, 0, s>>>(a);
+        if (a.status[inst] != 0) return;
+        grp_instance<G, IDF>(a, inst, l);
     }
-    return hipGetLastError();
+}
+template <bool IDF, bool F0>
+__global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
+    const int4 w = a.plan[blockIdx.x];
+    switch (w.x) {
+    case 4:  plan_wave<4, IDF, F0>(a, w.y, w.z); break;
+    case 8:  plan_wave<8, IDF, F0>(a, w.y, w.z); break;
+    case 16: plan_wave<16, IDF, F0>(a, w.y, w.z); break;
+    case 32: plan_wave<32, IDF, F0>(a, w.y, w.z); break;
+    default: plan_wave<64, IDF, F0>(a, w.y, w.z); break;
+    }
 }
 
 } // namespace
 
+// a.idf: every tree's joint frames are identity rotations (Simbody's
+// noR_PF/noX_MB node forms), the frame products are skipped; the results
+// are bitwise those of the general form.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
-    if (a.n_inst <= 0) return hipSuccess;
-    switch (a.grp) {
-    case 2:  return launch_g<2>(a, s, f0_only);
-    case 4:  return launch_g<4>(a, s, f0_only);
-    case 8:  return launch_g<8>(a, s, f0_only);
-    case 16: return launch_g<16>(a, s, f0_only);
-    case 32: return launch_g<32>(a, s, f0_only);
-    case 64: return launch_g<64>(a, s, f0_only);
-    default: return hipErrorInvalidValue;
+    if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
+    const dim3 grd((unsigned)a.plan_waves), blk(64);
+    if (a.idf) {
+        if (f0_only) k_rkm_plan<true, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_plan<true, false><<<grd, blk, 0, s>>>(a);
+    } else {
+        if (f0_only) k_rkm_plan<false, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_plan<false, false><<<grd, blk, 0, s>>>(a);
     }
+    return hipGetLastError();
 }
 
 } // namespace tdyn

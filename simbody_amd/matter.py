@@ -27,7 +27,7 @@ TDYN_ERRORS = {1: "TDYN_E_ARG", 2: "TDYN_E_SIZE", 3: "TDYN_E_STAGE", 4: "TDYN_E_
                5: "TDYN_E_STEP_FAILED", 6: "TDYN_E_HIP", 7: "TDYN_E_NOMEM"}
 
 EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator", "tdyn_batch_get_evaluator",
-           "tdyn_batch_set_tree_lanes",
+           "tdyn_batch_set_tree_lanes", "tdyn_batch_set_tree_lane_range",
            "tdyn_batch_info",
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
@@ -80,6 +80,7 @@ def lib():
         L.tdyn_batch_set_evaluator.argtypes = [vp, ctypes.c_int]
         L.tdyn_batch_get_evaluator.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.tdyn_batch_set_tree_lanes.argtypes = [vp, ctypes.c_int]
+        L.tdyn_batch_set_tree_lane_range.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tdyn_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
         L.tdyn_batch_offsets.argtypes = [vp, _i32, _i32, _i32]
         L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
@@ -206,9 +207,15 @@ class MatterBatch:
         _check(lib().tdyn_batch_set_evaluator(self.h, int(mode)))
 
     def set_tree_lanes(self, lanes):
-        """Lanes per instance of the tree evaluators: 0 automatic, 1 streaming
-        (one lane per instance), 2/4/8/16 level-parallel."""
+        """Lanes per instance of the tree evaluators: 0 automatic (per
+        instance), 1 streaming (one lane per instance), 4/8/16/32/64
+        level-parallel."""
         _check(lib().tdyn_batch_set_tree_lanes(self.h, int(lanes)))
+
+    def set_tree_lane_range(self, lanes_min=0, lanes_max=32):
+        """Range of the automatic lanes per instance (lanes_min 0: from the
+        batch size)."""
+        _check(lib().tdyn_batch_set_tree_lane_range(self.h, int(lanes_min), int(lanes_max)))
 
     def active_evaluator(self):
         """(evaluator, lanes per instance) of the current rollout: evaluator 0

@@ -1,7 +1,7 @@
 """Rollouts of batches of different trees run on the tree evaluators for
 such batches: the streaming evaluator (one lane walks its tree depth-first)
-and the level-parallel evaluator (2, 4, 8 or 16 lanes share one instance,
-one tree level at a time); instance-contiguous state.  Held to the parity
+and the level-parallel evaluator (4 to 64 lanes share one instance, one
+tree level at a time); instance-contiguous state.  Held to the parity
 bar of test_gpu_parity: the integrator's
 derivative at the rollout's initial state within 1e-12 relative of the
 oracle's udot and qdot (FP64), and whole rollouts with the oracle's
@@ -49,7 +49,7 @@ def _states(trees, seed, forces):
     return qs, us, fs
 
 
-LANES = [1, 2, 4, 16, 64]
+LANES = [0, 1, 4, 16, 64]
 
 
 def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=0):
@@ -61,7 +61,7 @@ def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=0):
     B.rollout_begin(B.pack(qs), B.pack(us), RungeKuttaMerson(fixed_step=1e-3), t_end=1.0,
                     fapp=B.pack(fs) if forces else None)
     assert B.uses_tree_stream() == expect_stream
-    if expect_stream and lanes:
+    if expect_stream:
         assert B.active_evaluator() == ((3, 1) if lanes == 1 else (4, lanes))
     q0, u0 = B.rollout_get()[:2]             # quaternions normalized by the integrator's initialize()
     qd, ud = B.rollout_derivs()
@@ -121,17 +121,20 @@ def test_free_body_not_first_falls_back():
 
 def test_tree_evaluators_rollouts_agree_and_match_oracle():
     """Adaptive rollouts (accuracy 1e-6, applied forces): the streaming and
-    level-parallel evaluators and the tree sweeps take the same steps; all
-    match the oracle's trajectory."""
+    level-parallel evaluators (the same lanes for every instance, and lanes
+    chosen per instance) and the tree sweeps take the same steps; all match
+    the oracle's trajectory."""
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson
     trees = mixed_trees(41, 10, nmin=8, nmax=40) + [_binary_tree(4, 12), _forest(13, 20, [7, 11])]
     n = len(trees)
     qs, us, fs = _states(trees, 41, True)
     outs = []
-    for mode, lanes in ((1, 0), (0, 1), (0, 2), (0, 8)):
+    for mode, lanes, rng in ((1, 0, None), (0, 1, None), (0, 4, None), (0, 0, (8, 64)), (0, 0, (4, 8)), (0, 64, None)):
         B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
         B.set_evaluator(mode)
         B.set_tree_lanes(lanes)
+        if rng:
+            B.set_tree_lane_range(*rng)
         q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
         outs.append((B.split_q(q1), B.split_u(u1), st.cpu().numpy()))
     for o in outs[1:]:
