@@ -113,10 +113,11 @@ def _evaluator_name(B):
     ev, lanes = B.active_evaluator()
     name = {1: "tree sweeps", 2: "chain", 3: "streaming tree evaluator",
             4: "level-parallel tree evaluator"}.get(ev, "?")
-    return f"{name}, {lanes} lane(s) per instance"
+    return f"{name}, " + (f"{lanes} lane(s) per instance" if lanes else "lanes per instance from each tree")
 
 
-def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0, lane_range=(0, 32)):
+def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0, lane_range=(0, 32),
+              evaluator=0):
     """Uneven branched trees (nb ~ U{8..128}, depth <= 20, Pin/Slider/
     Universal/Ball + Free roots), constant N(0,1) body forces and torques,
     per-instance RK-Merson step control at accuracy 1e-6 to t = 0.5 s, in
@@ -127,6 +128,7 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     Fs = [M.random_body_forces(t, 3, i) for i, t in enumerate(trees)]
     nbs = np.array([t.nb for t in trees])
     B = MatterBatch(trees, tree_of=np.arange(n), device="cuda:0")
+    B.set_evaluator(evaluator)
     B.set_tree_lanes(lanes)
     B.set_tree_lane_range(*lane_range)
     q = B.pack([y[:t.nq] for y, t in zip(Ys, trees)]); u = B.pack([y[t.nq:] for y, t in zip(Ys, trees)])
@@ -284,6 +286,8 @@ def main():
                     help="run only one secondary workload and print its JSON")
     ap.add_argument("--tree-lanes", type=int, default=0,
                     help="lanes per instance of the tree evaluator (0 automatic, 1, 2, 4, 8, 16)")
+    ap.add_argument("--tree-evaluator", type=int, default=0,
+                    help="evaluator of the uneven-trees row: 0 automatic, 1 tree sweeps, 3 level-parallel")
     ap.add_argument("--tree-lane-range", type=int, nargs=2, default=[0, 32],
                     help="automatic lanes per instance: min (0: from the batch size) and max")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -312,7 +316,8 @@ def main():
         return
     if args.only == "trees":
         print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu,
-                                   lanes=args.tree_lanes, lane_range=tuple(args.tree_lane_range))), flush=True)
+                                   lanes=args.tree_lanes, lane_range=tuple(args.tree_lane_range),
+                                   evaluator=args.tree_evaluator)), flush=True)
         return
     if args.only == "mixed":
         print(json.dumps(row_mixed(torch, M, MatterBatch, RungeKuttaMerson)), flush=True)

@@ -79,10 +79,12 @@ int tdyn_batch_destroy(tdyn_batch* b);
 
 /* Derivative evaluator of the rollouts (no reference counterpart; a tuning
  * knob): 0 = automatic (the register-resident serial-chain evaluator when
- * every instance shares one chain model; the streaming tree evaluator for a
- * batch of different trees whose only Free body, if any, is body 1; else the
- * tree sweeps), 1 = tree sweeps, 2 = chain evaluator (TDYN_E_ARG unless the
- * batch is one chain model).  Takes effect at the next tdyn_rollout_begin. */
+ * every instance shares one chain model; the level-parallel tree evaluator
+ * for a batch of different trees whose only Free body, if any, is body 1;
+ * else the tree sweeps), 1 = tree sweeps, 2 = chain evaluator (TDYN_E_ARG
+ * unless the batch is one chain model), 3 = the level-parallel tree
+ * evaluator (falls back to the tree sweeps where the batch does not
+ * qualify).  Takes effect at the next tdyn_rollout_begin. */
 int tdyn_batch_set_evaluator(tdyn_batch* b, int mode);
 /* Lanes per instance of the tree evaluators of a batch of different trees
  * (no reference counterpart; a tuning knob): 0 = automatic (the level-

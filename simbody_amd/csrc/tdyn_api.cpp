@@ -349,7 +349,7 @@ int tdyn_batch_destroy(tdyn_batch* b) { delete b; return TDYN_OK; }
 
 int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
-    if (mode < 0 || mode > 2) return fail(TDYN_E_ARG, "evaluator mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return fail(TDYN_E_ARG, "evaluator mode must be 0, 1, 2 or 3");
     if (mode == 2 && !b->is_chain) return fail(TDYN_E_ARG, "chain evaluator needs one serial-chain model for the whole batch");
     b->evaluator = mode;
     return TDYN_OK;
@@ -378,7 +378,7 @@ int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes) {
     if (b->rollout_ready) {
         const RkmArgs a = b->rk();
         act = a.chain ? 2 : a.grp ? 4 : a.vec_aos ? 3 : 1;
-        ln = act == 4 ? (b->tree_lanes > 1 ? b->tree_lanes : 0) : 1;
+        ln = act >= 4 ? (b->tree_lanes > 1 ? b->tree_lanes : 0) : 1;
     }
     *active = act;
     if (lanes) *lanes = ln;
@@ -477,7 +477,7 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
     const size_t L = b->n_inst;
-    b->rollout_aos = b->stream_ok && b->evaluator == 0;
+    b->rollout_aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3);
     b->rollout_grp = 0;
     if (b->rollout_aos && b->tree_lanes != 1) {
         const int rc = build_tree_plan(b, s);

@@ -420,6 +420,7 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
     }
 }
 
+
 } // namespace
 
 // a.idf: every tree's joint frames are identity rotations (Simbody's

@@ -203,7 +203,8 @@ class MatterBatch:
             self.h = None
 
     def set_evaluator(self, mode):
-        """0 automatic, 1 tree sweeps, 2 register-resident serial-chain evaluator."""
+        """0 automatic, 1 tree sweeps, 2 register-resident serial-chain
+        evaluator, 3 level-parallel tree evaluator."""
         _check(lib().tdyn_batch_set_evaluator(self.h, int(mode)))
 
     def set_tree_lanes(self, lanes):
@@ -220,7 +221,7 @@ class MatterBatch:
     def active_evaluator(self):
         """(evaluator, lanes per instance) of the current rollout: evaluator 0
         none yet, 1 tree sweeps, 2 chain, 3 streaming tree evaluator, 4
-        level-parallel tree evaluator."""
+        level-parallel tree evaluator; lanes 0 when chosen per instance."""
         v = ctypes.c_int(0); n = ctypes.c_int(0)
         _check(lib().tdyn_batch_get_evaluator(self.h, ctypes.byref(v), ctypes.byref(n)))
         return v.value, n.value

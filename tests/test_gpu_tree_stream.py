@@ -1,7 +1,7 @@
 """Rollouts of batches of different trees run on the tree evaluators for
 such batches: the streaming evaluator (one lane walks its tree depth-first)
-and the level-parallel evaluator (4 to 64 lanes share one instance, one
-tree level at a time); instance-contiguous state.  Held to the parity
+the level-parallel evaluator (4 to 64 lanes share one instance, one tree
+level at a time); instance-contiguous state.  Held to the parity
 bar of test_gpu_parity: the integrator's
 derivative at the rollout's initial state within 1e-12 relative of the
 oracle's udot and qdot (FP64), and whole rollouts with the oracle's
@@ -49,20 +49,25 @@ def _states(trees, seed, forces):
     return qs, us, fs
 
 
-LANES = [0, 1, 4, 16, 64]
+# (evaluator mode, lanes per instance): 0 automatic (level-parallel, lanes
+# per instance from the tree) or 3 level-parallel; lanes 1: streaming
+LANES = [(0, 0), (0, 1), (3, 4), (3, 16), (3, 64)]
+KIND = {0: 4, 3: 4}
 
 
-def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=0):
+def _check_derivs(trees, seed, forces=True, expect_stream=True, lanes=(0, 0)):
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson
     n = len(trees)
+    mode, lanes = lanes
     B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
+    B.set_evaluator(mode)
     B.set_tree_lanes(lanes)
     qs, us, fs = _states(trees, seed, forces)
     B.rollout_begin(B.pack(qs), B.pack(us), RungeKuttaMerson(fixed_step=1e-3), t_end=1.0,
                     fapp=B.pack(fs) if forces else None)
     assert B.uses_tree_stream() == expect_stream
     if expect_stream:
-        assert B.active_evaluator() == ((3, 1) if lanes == 1 else (4, lanes))
+        assert B.active_evaluator() == ((3, 1) if lanes == 1 else (KIND[mode], lanes))
     q0, u0 = B.rollout_get()[:2]             # quaternions normalized by the integrator's initialize()
     qd, ud = B.rollout_derivs()
     q0s, u0s, qds, uds = B.split_q(q0), B.split_u(u0), B.split_q(qd), B.split_u(ud)
@@ -129,7 +134,8 @@ def test_tree_evaluators_rollouts_agree_and_match_oracle():
     n = len(trees)
     qs, us, fs = _states(trees, 41, True)
     outs = []
-    for mode, lanes, rng in ((1, 0, None), (0, 1, None), (0, 4, None), (0, 0, (8, 64)), (0, 0, (4, 8)), (0, 64, None)):
+    for mode, lanes, rng in ((1, 0, None), (0, 1, None), (3, 4, None), (3, 0, (8, 64)), (0, 0, (4, 8)),
+                             (3, 64, None), (0, 0, None)):
         B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
         B.set_evaluator(mode)
         B.set_tree_lanes(lanes)
@@ -149,7 +155,7 @@ def test_tree_evaluators_rollouts_agree_and_match_oracle():
             assert rel_inf(y_e, y) < 1e-8, i
 
 
-@pytest.mark.parametrize("lanes", [1, 4, 16])
+@pytest.mark.parametrize("lanes", [(0, 1), (3, 16), (0, 0)])
 def test_identity_frame_trees(lanes):
     """Trees of the uneven-trees workload (identity joint frames, so the
     level-parallel evaluator skips the frame products): derivative and an
@@ -162,7 +168,8 @@ def test_identity_frame_trees(lanes):
     sub = trees[:8]
     qs, us, fs = _states(sub, 9, True)
     B = MatterBatch(sub, tree_of=np.arange(len(sub), dtype=np.int32))
-    B.set_tree_lanes(lanes)
+    B.set_evaluator(lanes[0])
+    B.set_tree_lanes(lanes[1])
     q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.05, fapp=B.pack(fs))
     q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
     for i, t in enumerate(sub):
