@@ -146,6 +146,59 @@ int oracle_abi(void* p, const double* q, double* P, double* PPlus, double* D, do
     return ok ? 0 : 1;
 }
 
+// multiplyByM: out = M v (nu).
+int oracle_multiply_by_m(void* p, const double* q, const double* v, double* out) {
+    Handle* h = H(p); Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    multiplyByM(tr, v, out);
+    return 0;
+}
+
+// multiplyBySystemJacobian: Jv = J v, (nb+1)*6.
+int oracle_jacobian(void* p, const double* q, const double* v, double* Jv) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    std::vector<SV<double>> J(M.nb+1);
+    systemJacobian(tr, v, J.data());
+    for (int b=0;b<=M.nb;++b) storeSV(J[b], Jv+6*b);
+    return 0;
+}
+
+// multiplyBySystemJacobianTranspose: out = J^T F, F (nb+1)*6.
+int oracle_jacobian_transpose(void* p, const double* q, const double* F, double* out) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    std::vector<SV<double>> Fs; loadForces(M, F, Fs);
+    systemJacobianTranspose(tr, Fs.data(), out);
+    return 0;
+}
+
+// calcBodyAccelerationFromUDot: A_GB (nb+1)*6; null udot means zeros.
+int oracle_body_acceleration(void* p, const double* q, const double* u, const double* udot, double* A) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizeAll(h, q, u);
+    std::vector<SV<double>> As(M.nb+1);
+    bodyAccelerationFromUDot(tr, udot, As.data());
+    for (int b=0;b<=M.nb;++b) storeSV(As[b], A+6*b);
+    return 0;
+}
+
+// calcCompositeBodyInertias: R (nb+1)*10 per body: mass, mass center (3),
+// unit inertia about the origin (xx yy zz xy xz yz), all in G.
+int oracle_composite_inertias(void* p, const double* q, double* R) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    std::vector<SpatialInertia<double>> Rs(M.nb+1);
+    compositeBodyInertias(tr, Rs.data());
+    for (int b=0;b<=M.nb;++b) {
+        double* o = R + 10*b;
+        o[0] = Rs[b].m;
+        for (int i=0;i<3;++i) o[1+i] = Rs[b].p[i];
+        storeSym(Rs[b].G, o+4);
+    }
+    return 0;
+}
+
 int oracle_reactions(void* p, const double* q, const double* u, const double* fmob, const double* Fb,
                      double* FM) {
     Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;

@@ -41,7 +41,9 @@ def lib():
         for name in ["oracle_model_sizes", "oracle_model_layout", "oracle_kinematics",
                      "oracle_system_forces", "oracle_forward", "oracle_realize_acceleration",
                      "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
-                     "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops"]:
+                     "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
+                     "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
+                     "oracle_body_acceleration", "oracle_composite_inertias"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -138,6 +140,35 @@ class OracleModel:
         out = np.zeros((self.nb + 1, 6))
         lib().oracle_reactions(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(out))
         return out
+
+    def multiply_by_m(self, q, v):
+        out = np.zeros(self.nu)
+        lib().oracle_multiply_by_m(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(out))
+        return out
+
+    def jacobian(self, q, v):
+        Jv = np.zeros((self.nb + 1, 6))
+        lib().oracle_jacobian(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(Jv))
+        return Jv
+
+    def jacobian_transpose(self, q, F):
+        out = np.zeros(self.nu)
+        lib().oracle_jacobian_transpose(self.h, _p(q), _p(np.ascontiguousarray(F, np.float64).reshape(-1)),
+                                        _p(out))
+        return out
+
+    def body_acceleration(self, q, u, udot=None):
+        A = np.zeros((self.nb + 1, 6))
+        ud = None if udot is None else np.ascontiguousarray(udot, np.float64)
+        lib().oracle_body_acceleration(self.h, _p(q), _p(u), _p(ud), _p(A))
+        return A
+
+    def composite_inertias(self, q):
+        """(nb+1, 10): mass, mass center (3), unit inertia about the body
+        origin (xx yy zz xy xz yz), in G; row 0 is Ground (infinite mass)."""
+        R = np.zeros((self.nb + 1, 10))
+        lib().oracle_composite_inertias(self.h, _p(q), _p(R))
+        return R
 
     def integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None, t0=0.0, max_attempts=0):
         y = np.array(y, dtype=np.float64, copy=True)

@@ -721,6 +721,115 @@ template <class T> void residualForces(Tree<T>& tr, const T* fmob, const SV<T>* 
     }
 }
 
+// ---------------------------------------------------------------------------
+// O(n) operators that reuse the position (and velocity) caches.
+// ---------------------------------------------------------------------------
+// Mass-matrix product f = M v: multiplyByMPass1Outward / multiplyByMPass2Inward
+// (RigidBodyNodeSpec.cpp:657-695), driver SimbodyMatterSubsystemRep.cpp:
+// 5411-5450.  Needs realizePosition().  Uses B.A_GB and B.F as temporaries.
+template <class T> void multiplyByM(Tree<T>& tr, const T* v, T* out) {
+    const Model& M = *tr.M;
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        B.A_GB = phiTMul(B.phi, tr.B[M.parent[b]].A_GB) + Hmul(B.H, M.dof[b], v + M.uIndex[b]);
+    }
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        B.F = siMul(B.Mk, B.A_GB);
+        for (int c : M.children[b]) B.F = B.F + phiMul(tr.B[c].phi, tr.B[c].F);
+        for (int j=0;j<M.dof[b];++j) out[M.uIndex[b]+j] = dot(B.H[j], B.F);
+    }
+}
+
+// Jv = J v, one spatial vector per body, Ground's zero (RigidBodyNodeSpec.cpp:
+// 760-785; driver SimbodyMatterSubsystemRep.cpp:6293).  Needs realizePosition().
+template <class T> void systemJacobian(Tree<T>& tr, const T* v, SV<T>* Jv) {
+    const Model& M = *tr.M;
+    Jv[0] = svzero<T>();
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        const Body<T>& B = tr.B[b];
+        Jv[b] = phiTMul(B.phi, Jv[M.parent[b]]) + Hmul(B.H, M.dof[b], v + M.uIndex[b]);
+    }
+}
+
+// f = J^T F for spatial forces F at the body origins, in G; F[0] (Ground) is
+// ignored (RigidBodyNodeSpec.cpp:795-815).  Needs realizePosition().  Uses
+// B.z as the temporary.
+template <class T> void systemJacobianTranspose(Tree<T>& tr, const SV<T>* F, T* out) {
+    const Model& M = *tr.M;
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        B.z = F[b];
+        for (int c : M.children[b]) B.z = B.z + phiMul(tr.B[c].phi, tr.B[c].z);
+        for (int j=0;j<M.dof[b];++j) out[M.uIndex[b]+j] = dot(B.H[j], B.z);
+    }
+}
+
+// A_GB = J udot + JDot u (calcBodyAccelerationsFromUdotOutward,
+// RigidBodyNodeSpec.cpp:566-582; driver SimbodyMatterSubsystemRep.cpp:
+// 3592-3625).  Needs realizeVelocity().  Null udot means zeros.
+template <class T> void bodyAccelerationFromUDot(Tree<T>& tr, const T* udot, SV<T>* A) {
+    const Model& M = *tr.M;
+    A[0] = svzero<T>();
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        const Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        T ud[6];
+        for (int j=0;j<nd;++j) ud[j] = udot ? udot[M.uIndex[b]+j] : T(0);
+        A[b] = phiTMul(B.phi, A[M.parent[b]]) + Hmul(B.H, nd, ud) + B.a;
+    }
+}
+
+// Unit inertia of a unit point mass at p, taken about the origin:
+// |p|^2 I - p p^T (UnitInertia::pointMassAt, MassProperties.h:866-867).
+template <class T> inline M33<T> pointMassUnitInertia(const V3<T>& p) {
+    const T xx = p[0]*p[0], yy = p[1]*p[1], zz = p[2]*p[2];
+    M33<T> G;
+    G(0,0) = yy + zz; G(1,1) = xx + zz; G(2,2) = xx + yy;
+    G(0,1) = G(1,0) = -(p[0]*p[1]);
+    G(0,2) = G(2,0) = -(p[0]*p[2]);
+    G(1,2) = G(2,1) = -(p[1]*p[2]);
+    return G;
+}
+// SpatialInertia::shift(S): origin moved from O to O+S through the centroid
+// (MassProperties.h:1103-1116, UnitInertia shifts :770-800).
+template <class T> inline SpatialInertia<T> siShift(const SpatialInertia<T>& I, const V3<T>& S) {
+    SpatialInertia<T> R;
+    R.m = I.m;
+    const V3<T> pNew = I.p - S;
+    R.G = (I.G - pointMassUnitInertia(I.p)) + pointMassUnitInertia(pNew);
+    R.p = pNew;
+    return R;
+}
+// SpatialInertia += (MassProperties.h:1037-1045): mass-weighted mass center
+// and unit inertia.
+template <class T> inline void siAdd(SpatialInertia<T>& R, const SpatialInertia<T>& Q) {
+    const T mtot = R.m + Q.m, oomtot = T(1)/mtot;
+    R.p = oomtot*(R.m*R.p + Q.m*Q.p);
+    M33<T> G;
+    for (int i=0;i<9;++i) G.m[i] = oomtot*(R.m*R.G.m[i] + Q.m*Q.G.m[i]);
+    R.G = G;
+    R.m = mtot;
+}
+// Composite body inertias: each body with everything outboard of it welded,
+// about its origin, in G (RigidBodyNode.cpp:232-243, driver
+// SimbodyMatterSubsystemRep.cpp:5196-5205).  R[0] is Ground's infinite mass
+// (RigidBodyNode_Weld.cpp:168-171).  Needs realizePosition().
+template <class T> void compositeBodyInertias(const Tree<T>& tr, SpatialInertia<T>* R) {
+    const Model& M = *tr.M;
+    R[0].m = T(std::numeric_limits<double>::infinity()); R[0].p = zero3<T>(); R[0].G = ident33<T>();
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        R[b] = tr.B[b].Mk;
+        for (int c : M.children[b]) siAdd(R[b], siShift(R[c], -tr.B[c].phi));
+    }
+}
+
 // Mobilizer reaction forces at M, expressed in G (SimbodyMatterSubsystemRep.cpp:
 // 5786-5826).  Requires forwardDynamics() to have run.  Checker only.
 template <class T> void reactionForces(const Tree<T>& tr, SV<T>* FM) {

@@ -1,8 +1,8 @@
 """Rollouts of batches of different trees run on the tree evaluators for
-such batches: the streaming evaluator (one lane walks its tree depth-first)
-the level-parallel evaluator (4 to 64 lanes share one instance, one tree
-level at a time); instance-contiguous state.  Held to the parity
-bar of test_gpu_parity: the integrator's
+such batches: the streaming evaluator (one lane walks its tree
+depth-first) and the level-parallel evaluator (4 to 64 lanes share one
+instance, one tree level at a time); instance-contiguous state.  Held to
+the parity bar of test_gpu_parity: the integrator's
 derivative at the rollout's initial state within 1e-12 relative of the
 oracle's udot and qdot (FP64), and whole rollouts with the oracle's
 accept/reject counts.  Shapes: the largest trees of the workloads (128
