@@ -154,6 +154,35 @@ int tdyn_calc_residual_force_ignoring_constraints(tdyn_batch* b, const double* q
 int tdyn_calc_articulated_body_inertias(tdyn_batch* b, const double* q, double* P, double* PPlus,
                                         hipStream_t s);
 
+/* multiplyByM(state, v, Mv) (SimbodyMatterSubsystem.h:1262): Mv = M v in
+ * O(n), per instance (nu doubles each, packed like u).  Needs q only. */
+int tdyn_multiply_by_m(tdyn_batch* b, const double* q, const double* v, double* Mv, hipStream_t s);
+
+/* multiplyBySystemJacobian(state, v, Jv) (SimbodyMatterSubsystem.h:554):
+ * Jv = J v, the spatial velocity each body would have for speeds v, 6
+ * doubles per body including Ground's zero ((nb+1)*6 per instance). */
+int tdyn_multiply_by_system_jacobian(tdyn_batch* b, const double* q, const double* v, double* Jv,
+                                     hipStream_t s);
+
+/* multiplyBySystemJacobianTranspose(state, F_G, f) (SimbodyMatterSubsystem.h:
+ * 646): f = J^T F_G for spatial forces (torque, force) at the body origins,
+ * in G, (nb+1)*6 per instance with Ground's entry ignored; f packed like u. */
+int tdyn_multiply_by_system_jacobian_transpose(tdyn_batch* b, const double* q, const double* F_G, double* f,
+                                               hipStream_t s);
+
+/* calcBodyAccelerationFromUDot(state, udot, A_GB) (SimbodyMatterSubsystem.h:
+ * 2365): A_GB = J udot + JDot u, (nb+1)*6 per instance with Ground's zero;
+ * udot NULL means zero (only the velocity-dependent part). */
+int tdyn_calc_body_acceleration_from_udot(tdyn_batch* b, const double* q, const double* u, const double* udot,
+                                          double* A_GB, hipStream_t s);
+
+/* calcCompositeBodyInertias(state, R) (SimbodyMatterSubsystem.h:2323): the
+ * spatial inertia of each body with everything outboard of it welded, about
+ * its origin, in G.  10 doubles per body, Ground first ((nb+1)*10 per
+ * instance): mass, mass center (3), unit inertia (xx yy zz xy xz yz);
+ * Ground's entry is infinite mass, zero mass center, unit inertia 1. */
+int tdyn_calc_composite_body_inertias(tdyn_batch* b, const double* q, double* R, hipStream_t s);
+
 /* calcMobilizerReactionForces(state, FM_G) (SimbodyMatterSubsystemRep.cpp:
  * 5786-5826) for the given applied forces: reaction at each M frame origin,
  * expressed in G, 6 per body. */

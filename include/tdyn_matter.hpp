@@ -92,6 +92,22 @@ public:
                                      const double* appliedBodyForces, double* FM_G) const {
         check(tdyn_calc_mobilizer_reaction_forces(h_, q, u, appliedMobilityForces, appliedBodyForces, FM_G, s_));
     }
+    void multiplyByM(const double* q, const double* v, double* Mv) const {
+        check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
+    }
+    void multiplyBySystemJacobian(const double* q, const double* v, double* Jv) const {
+        check(tdyn_multiply_by_system_jacobian(h_, q, v, Jv, s_));
+    }
+    void multiplyBySystemJacobianTranspose(const double* q, const double* F_G, double* f) const {
+        check(tdyn_multiply_by_system_jacobian_transpose(h_, q, F_G, f, s_));
+    }
+    void calcBodyAccelerationFromUDot(const double* q, const double* u, const double* knownUDot,
+                                      double* A_GB) const {
+        check(tdyn_calc_body_acceleration_from_udot(h_, q, u, knownUDot, A_GB, s_));
+    }
+    void calcCompositeBodyInertias(const double* q, double* R) const {
+        check(tdyn_calc_composite_body_inertias(h_, q, R, s_));
+    }
 
     // TimeStepper(system, integ).initialize(state); stepTo(tEnd) for every instance.
     void stepTo(double* q, double* u, const double* appliedBodyForces, double t0, double tEnd,

@@ -459,6 +459,50 @@ int tdyn_calc_articulated_body_inertias(tdyn_batch* b, const double* q, double* 
     return TDYN_OK;
 }
 
+int tdyn_multiply_by_m(tdyn_batch* b, const double* q, const double* v, double* Mv, hipStream_t s) {
+    if (!b || !q || !v || !Mv) return fail(TDYN_E_ARG, "multiplyByM: batch, q, v and Mv are required");
+    OpArgs a = b->op();
+    a.q = q; a.udot_in = v; a.out_u = Mv;
+    HIPCHK(launch_op(OP_MULM, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_multiply_by_system_jacobian(tdyn_batch* b, const double* q, const double* v, double* Jv, hipStream_t s) {
+    if (!b || !q || !v || !Jv) return fail(TDYN_E_ARG, "multiplyBySystemJacobian: batch, q, v and Jv are required");
+    OpArgs a = b->op();
+    a.q = q; a.udot_in = v; a.out_body = Jv;
+    HIPCHK(launch_op(OP_JAC, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_multiply_by_system_jacobian_transpose(tdyn_batch* b, const double* q, const double* F_G, double* f,
+                                               hipStream_t s) {
+    if (!b || !q || !F_G || !f)
+        return fail(TDYN_E_ARG, "multiplyBySystemJacobianTranspose: batch, q, F_G and f are required");
+    OpArgs a = b->op();
+    a.q = q; a.fbody = F_G; a.out_u = f;
+    HIPCHK(launch_op(OP_JACT, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_body_acceleration_from_udot(tdyn_batch* b, const double* q, const double* u, const double* udot,
+                                          double* A_GB, hipStream_t s) {
+    if (!b || !q || !u || !A_GB)
+        return fail(TDYN_E_ARG, "calcBodyAccelerationFromUDot: batch, q, u and A_GB are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.udot_in = udot; a.out_body = A_GB;
+    HIPCHK(launch_op(OP_BACC, a, s));
+    return TDYN_OK;
+}
+
+int tdyn_calc_composite_body_inertias(tdyn_batch* b, const double* q, double* R, hipStream_t s) {
+    if (!b || !q || !R) return fail(TDYN_E_ARG, "calcCompositeBodyInertias: batch, q and R are required");
+    OpArgs a = b->op();
+    a.q = q; a.out_body = R;
+    HIPCHK(launch_op(OP_CBI, a, s));
+    return TDYN_OK;
+}
+
 int tdyn_calc_mobilizer_reaction_forces(tdyn_batch* b, const double* q, const double* u,
                                         const double* fmob, const double* F, double* FM, hipStream_t s) {
     if (!b || !q || !u || !FM) return fail(TDYN_E_ARG, "calcMobilizerReactionForces: batch, q, u and FM are required");

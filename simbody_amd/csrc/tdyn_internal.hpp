@@ -77,7 +77,8 @@ struct RkmCfg {
 };
 
 // Launch entry points (tdyn_kernels.hip).
-enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6 };
+enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6,
+              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11 };
 
 struct OpArgs {
     DevTables T;
@@ -91,10 +92,11 @@ struct OpArgs {
     const double* u;
     const double* fmob;      // nullable
     const double* fbody;     // nullable ((nb+1)*6 per instance)
-    const double* udot_in;   // residual input, nullable
+    const double* udot_in;   // residual / body-acceleration input (nullable); the vector v of M v and J v
     double* out_u;           // udot / M^-1 f / residual   (packed u layout)
     double* out_q;           // qdot or qdotdot (packed q layout), nullable
-    double* out_body;        // A_GB / X_GB / reactions (packed (nb+1)*6 or *12), nullable
+    double* out_body;        // A_GB / X_GB / reactions / J v (packed (nb+1)*6 or *12), composite
+                             // inertias ((nb+1)*10), nullable
     double* out_body2;       // V_GB (kinematics), nullable
     double* out_abi;         // P per body (21), nullable
     double* out_abi2;        // PPlus per body (21), nullable

@@ -596,6 +596,76 @@ __device__ __forceinline__ void resid_in_body(const Lane& c, int b, const int4 b
 }
 
 // ---------------------------------------------------------------------------
+// O(n) operators on the position (and velocity) records.
+// ---------------------------------------------------------------------------
+// Outward: A_GB = Phi^T A_GP + H v -- the system Jacobian product J v
+// (RigidBodyNodeSpec.cpp:760-785) and pass 1 of M v (:657-669).
+template <int D, class VG>
+__device__ __forceinline__ void jac_out_body(const Lane& c, int b, const int4 ba, const VG& v) {
+    const int rb = c.rec(b), p = ba.x;
+    const dv A_GP = p == 0 ? svzero<double>() : ldsv(c, c.rec(p) + REC_AGB);
+    const dv A = phiT_mul(ld3(c, rb + REC_PHI), A_GP);
+    const int rH = rb + REC_DIEPS + D;
+    const int d = mob_dof(ba.y);
+    dv Hv = svzero<double>();
+#pragma unroll
+    for (int j = 0; j < D; ++j) if (j < d) Hv = Hv + v(ba.w + j)*ldsv(c, rH + 6*j);
+    stsv(c, rb + REC_AGB, A + Hv);
+}
+// Inward: z = F_own + sum_children Phi z_child, tau = H^T z.  MK: F_own =
+// Mk_G A_GB (pass 2 of M v, RigidBodyNodeSpec.cpp:672-695); else the given
+// body force (J^T F, :795-815).
+template <int D, bool MK, class FB, class TO>
+__device__ __forceinline__ void jact_in_body(const Lane& c, int b, const int4 ba, const int4 bb,
+                                             const FB& fbody, const TO& tau) {
+    const int rb = bb.z;
+    dv z;
+    if constexpr (MK) z = rbi_mul(c.prm(b)[PRM_MASS], ld3(c, rb + REC_MKP), lds(c, rb + REC_MKG), ldsv(c, rb + REC_AGB));
+    else z = fbody(b);
+    for (int k = 0; k < bb.y; ++k) {
+        const int rc = c.rec(cptr(c.T.child_idx)[bb.x + k]);
+        z = z + phi_mul(ld3(c, rc + REC_PHI), ldsv(c, rc + REC_ZPLUS));
+    }
+    stsv(c, rb + REC_ZPLUS, z);
+    const int rH = rb + REC_DIEPS + D;
+    const int d = mob_dof(ba.y);
+#pragma unroll
+    for (int j = 0; j < D; ++j) if (j < d) tau(ba.w + j, dot(ldsv(c, rH + 6*j), z));
+}
+// Unit inertia of a unit point mass at p about the origin, |p|^2 I - p p^T
+// (UnitInertia::pointMassAt, MassProperties.h:866-867).
+__device__ __forceinline__ ds point_mass(const d3& p) {
+    const double xx = p.x*p.x, yy = p.y*p.y, zz = p.z*p.z;
+    return {yy + zz, xx + zz, xx + yy, -(p.x*p.y), -(p.x*p.z), -(p.y*p.z)};
+}
+// Composite body inertia (RigidBodyNode.cpp:232-243): R = Mk + sum over the
+// children of R_child shifted to this origin (SpatialInertia::shift,
+// MassProperties.h:1103-1116) and added mass-weighted (operator+=, :1037-1045).
+// Kept in the P+ slots of the record: m, mass center (3), unit inertia (6).
+__device__ __forceinline__ void cbi_in_body(const Lane& c, int b, const int4 bb) {
+    const int rb = bb.z;
+    double m = c.prm(b)[PRM_MASS];
+    d3 p = ld3(c, rb + REC_MKP);
+    ds G = lds(c, rb + REC_MKG);
+    for (int k = 0; k < bb.y; ++k) {
+        const int rc = c.rec(cptr(c.T.child_idx)[bb.x + k]);
+        const double mc = c.s(rc + REC_PPLUS);
+        const d3 pc = ld3(c, rc + REC_PPLUS + 1);
+        const ds Gc = lds(c, rc + REC_PPLUS + 4);
+        const d3 pn = pc + ld3(c, rc + REC_PHI);          // shift by -phi: new origin is this body's
+        const ds Gs = (Gc - point_mass(pc)) + point_mass(pn);
+        const double mt = m + mc, oo = 1.0 / mt;
+        p = oo*(m*p + mc*pn);
+        G = {oo*(m*G.xx + mc*Gs.xx), oo*(m*G.yy + mc*Gs.yy), oo*(m*G.zz + mc*Gs.zz),
+             oo*(m*G.xy + mc*Gs.xy), oo*(m*G.xz + mc*Gs.xz), oo*(m*G.yz + mc*Gs.yz)};
+        m = mt;
+    }
+    c.s(rb + REC_PPLUS) = m;
+    st3(c, rb + REC_PPLUS + 1, p);
+    sts(c, rb + REC_PPLUS + 4, G);
+}
+
+// ---------------------------------------------------------------------------
 // Accessors.
 // ---------------------------------------------------------------------------
 struct PackedGet {   // packed boundary layout, already offset to the instance
@@ -632,6 +702,8 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 // ---------------------------------------------------------------------------
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
+// D of a record: padded generic bodies are 3-dof in batches of different trees.
+#define TD_RDOF(ba) ((PAD && (ba).y != FREE) ? 3 : mob_dof((ba).y))
 template <int OP, bool UNI>
 __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     constexpr bool PAD = !UNI;      // batches of different trees use padded records
@@ -720,6 +792,82 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
             a.out_abi[21*(size_t)bo + i] = 0.0;
             if (a.out_abi2) a.out_abi2[21*(size_t)bo + i] = 0.0;
         }
+    } else if constexpr (OP == OP_MULM || OP == OP_JAC) {
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
+        const PackedGet v{a.udot_in + uo};
+        for (int b = 1; b <= c.nb; ++b) {
+            const int4 ba = c.ba(b);
+            switch (TD_RDOF(ba)) {
+            case 1: jac_out_body<1>(c, b, ba, v); break;
+            case 2: jac_out_body<2>(c, b, ba, v); break;
+            case 3: jac_out_body<3>(c, b, ba, v); break;
+            default: jac_out_body<6>(c, b, ba, v); break;
+            }
+        }
+        if constexpr (OP == OP_JAC) {
+            for (int b = 0; b <= c.nb; ++b) {
+                const dv A = b == 0 ? svzero<double>() : ldsv(c, c.rec(b) + REC_AGB);
+                double* o = a.out_body + 6*(size_t)(bo + b);
+                o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
+            }
+        } else {
+            const PackedPut tau{a.out_u + uo};
+            for (int b = c.nb; b >= 1; --b) {
+                const int4 ba = c.ba(b), bb = c.bb(b);
+                switch (TD_RDOF(ba)) {
+                case 1: jact_in_body<1, true>(c, b, ba, bb, BodyForce{nullptr}, tau); break;
+                case 2: jact_in_body<2, true>(c, b, ba, bb, BodyForce{nullptr}, tau); break;
+                case 3: jact_in_body<3, true>(c, b, ba, bb, BodyForce{nullptr}, tau); break;
+                default: jact_in_body<6, true>(c, b, ba, bb, BodyForce{nullptr}, tau); break;
+                }
+            }
+        }
+    } else if constexpr (OP == OP_JACT) {
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
+        const BodyForce fb{a.fbody + 6*(size_t)bo};
+        const PackedPut tau{a.out_u + uo};
+        for (int b = c.nb; b >= 1; --b) {
+            const int4 ba = c.ba(b), bb = c.bb(b);
+            switch (TD_RDOF(ba)) {
+            case 1: jact_in_body<1, false>(c, b, ba, bb, fb, tau); break;
+            case 2: jact_in_body<2, false>(c, b, ba, bb, fb, tau); break;
+            case 3: jact_in_body<3, false>(c, b, ba, bb, fb, tau); break;
+            default: jact_in_body<6, false>(c, b, ba, bb, fb, tau); break;
+            }
+        }
+    } else if constexpr (OP == OP_BACC) {
+        // calcBodyAccelerationFromUDot (SimbodyMatterSubsystemRep.cpp:3592-3625)
+        sweep_kin<true, PAD>(c, q, u, NoPut{});
+        const PackedGetOpt ud{a.udot_in ? a.udot_in + uo : nullptr};
+        for (int b = 1; b <= c.nb; ++b) {
+            const int4 ba = c.ba(b);
+            switch (TD_RDOF(ba)) {
+            case 1: resid_out_body<1>(c, b, ba, ud); break;
+            case 2: resid_out_body<2>(c, b, ba, ud); break;
+            case 3: resid_out_body<3>(c, b, ba, ud); break;
+            default: resid_out_body<6>(c, b, ba, ud); break;
+            }
+        }
+        for (int b = 0; b <= c.nb; ++b) {
+            const dv A = b == 0 ? svzero<double>() : ldsv(c, c.rec(b) + REC_AGB);
+            double* o = a.out_body + 6*(size_t)(bo + b);
+            o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
+        }
+    } else if constexpr (OP == OP_CBI) {
+        // calcCompositeBodyInertias (SimbodyMatterSubsystemRep.cpp:5196-5205);
+        // Ground's entry is its infinite mass (RigidBodyNode_Weld.cpp:168-171)
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
+        for (int b = c.nb; b >= 1; --b) cbi_in_body(c, b, c.bb(b));
+        for (int b = 0; b <= c.nb; ++b) {
+            double* o = a.out_body + 10*(size_t)(bo + b);
+            if (b == 0) {
+                o[0] = __builtin_inf();
+                for (int i = 1; i < 10; ++i) o[i] = (i >= 4 && i <= 6) ? 1.0 : 0.0;
+                continue;
+            }
+            const int rb = c.rec(b);
+            for (int i = 0; i < 10; ++i) o[i] = c.s(rb + REC_PPLUS + i);
+        }
     } else if constexpr (OP == OP_RESID) {
         sweep_kin<true, PAD>(c, q, u, NoPut{});
         const PackedGetOpt ud{a.udot_in ? a.udot_in + uo : nullptr};
@@ -747,6 +895,8 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     }
 }
 
+#undef TD_RDOF
+
 template <bool UNI>
 static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     const dim3 blk(64), grd((a.n_inst + 63) / 64);
@@ -758,6 +908,11 @@ static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     case OP_RESID:     k_op<OP_RESID, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_ABI:       k_op<OP_ABI, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_REACT:     k_op<OP_REACT, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_MULM:      k_op<OP_MULM, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_JAC:       k_op<OP_JAC, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_JACT:      k_op<OP_JACT, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_BACC:      k_op<OP_BACC, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_CBI:       k_op<OP_CBI, UNI><<<grd, blk, 0, s>>>(a); break;
     }
 }
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {

@@ -33,7 +33,9 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
-           "tdyn_calc_mobilizer_reaction_forces", "tdyn_rollout_begin", "tdyn_rollout_advance",
+           "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
+           "tdyn_multiply_by_system_jacobian_transpose", "tdyn_calc_body_acceleration_from_udot",
+           "tdyn_calc_composite_body_inertias", "tdyn_rollout_begin", "tdyn_rollout_advance",
            "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_last_error",
            "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
            "tdyn_server_advance", "tdyn_server_collect"]
@@ -90,6 +92,11 @@ def lib():
         L.tdyn_calc_residual_force_ignoring_constraints.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_calc_articulated_body_inertias.argtypes = [vp] + [vp] * 3 + [vp]
         L.tdyn_calc_mobilizer_reaction_forces.argtypes = [vp] + [vp] * 5 + [vp]
+        L.tdyn_multiply_by_m.argtypes = [vp] * 5
+        L.tdyn_multiply_by_system_jacobian.argtypes = [vp] * 5
+        L.tdyn_multiply_by_system_jacobian_transpose.argtypes = [vp] * 5
+        L.tdyn_calc_body_acceleration_from_udot.argtypes = [vp] * 6
+        L.tdyn_calc_composite_body_inertias.argtypes = [vp] * 4
         L.tdyn_rollout_begin.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, ctypes.c_double,
                                          ctypes.POINTER(IntegratorCfgC), vp]
         L.tdyn_rollout_advance.argtypes = [vp, ctypes.c_int, vp]
@@ -306,6 +313,39 @@ class MatterBatch:
         out = self._new(6 * self.total_b)
         _check(lib().tdyn_calc_mobilizer_reaction_forces(self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F),
                                                          _ptr(out), _stream()))
+        return out
+
+    def multiplyByM(self, q, v):
+        """M v per instance (packed like u)."""
+        out = self._new(self.total_u)
+        _check(lib().tdyn_multiply_by_m(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
+        return out
+
+    def multiplyBySystemJacobian(self, q, v):
+        """J v: (nb+1)*6 per instance, Ground's entry zero."""
+        out = self._new(6 * self.total_b)
+        _check(lib().tdyn_multiply_by_system_jacobian(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
+        return out
+
+    def multiplyBySystemJacobianTranspose(self, q, F_G):
+        """J^T F_G (packed like u); F_G (nb+1)*6 per instance, Ground ignored."""
+        out = self._new(self.total_u)
+        _check(lib().tdyn_multiply_by_system_jacobian_transpose(self.h, _ptr(q), _ptr(F_G), _ptr(out),
+                                                                _stream()))
+        return out
+
+    def calcBodyAccelerationFromUDot(self, q, u, udot=None):
+        """J udot + JDot u: (nb+1)*6 per instance; udot None means zero."""
+        out = self._new(6 * self.total_b)
+        _check(lib().tdyn_calc_body_acceleration_from_udot(self.h, _ptr(q), _ptr(u), _ptr(udot), _ptr(out),
+                                                           _stream()))
+        return out
+
+    def calcCompositeBodyInertias(self, q):
+        """(nb+1)*10 per instance: mass, mass center (3), unit inertia about
+        the body origin (xx yy zz xy xz yz), in G; Ground first."""
+        out = self._new(10 * self.total_b)
+        _check(lib().tdyn_calc_composite_body_inertias(self.h, _ptr(q), _ptr(out), _stream()))
         return out
 
     # ---- RungeKuttaMersonIntegrator + TimeStepper ------------------------
