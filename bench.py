@@ -212,6 +212,60 @@ def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 5000, 20000, 50000), wi
     return out
 
 
+def row_operators(torch, M, MatterBatch, n=65536, reps=5, cpu=True, cpu_sample=2000):
+    """Single-shot operators on the config-2 chains (65,536 x 32 bodies, one
+    model for the batch): kernel time per call (HIP events on the launch
+    stream, mean of `reps` calls after one warm call), bodies/s, and the
+    boundary bytes (inputs and outputs of the call) per second."""
+    spec = M.spatial_chain(32, seed=2)
+    Y = M.chain_batch_states(spec, n, seed=2026)
+    B = MatterBatch([spec], n_instances=n, device="cuda:0")
+    q = B.to_device(Y[:, :spec.nq]); u = B.to_device(Y[:, spec.nq:])
+    rng = np.random.default_rng(7)
+    v = B.to_device(rng.normal(size=(n, spec.nu)))
+    F = B.to_device(rng.normal(size=(n, (spec.nb + 1) * 6)))
+    nq, nu, nbp = spec.nq, spec.nu, spec.nb + 1
+    ou = torch.empty(n * nu, dtype=torch.float64, device="cuda:0")
+    ob = torch.empty(n * nbp * 10, dtype=torch.float64, device="cuda:0")
+    ob6 = ob[:n * nbp * 6]
+    ops = {   # name: (call into preallocated outputs, boundary doubles per instance)
+        "multiplyByM": (lambda: B.multiplyByM(q, v, out=ou), nq + 2 * nu),
+        "multiplyByMInv": (lambda: B.multiplyByMInv(q, v, out=ou), nq + 2 * nu),
+        "multiplyBySystemJacobian": (lambda: B.multiplyBySystemJacobian(q, v, out=ob6), nq + nu + 6 * nbp),
+        "multiplyBySystemJacobianTranspose": (lambda: B.multiplyBySystemJacobianTranspose(q, F, out=ou),
+                                              nq + 6 * nbp + nu),
+        "calcBodyAccelerationFromUDot": (lambda: B.calcBodyAccelerationFromUDot(q, u, v, out=ob6),
+                                         nq + 2 * nu + 6 * nbp),
+        "calcCompositeBodyInertias": (lambda: B.calcCompositeBodyInertias(q, out=ob), nq + 10 * nbp),
+    }
+    out = {"workload": f"{n} chains x {spec.nb} bodies (config-2 model and states), FP64, one call",
+           "ops": {}}
+    for name, (call, dbl) in ops.items():
+        call(); torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out["ops"][name] = {"ms": ms, "bodies_per_s": n * spec.nb / (ms / 1e3),
+                            "boundary_gbs": n * dbl * 8 / (ms / 1e3) / 1e9}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_ctypes import OracleModel
+        o = OracleModel(spec)
+        Yc = Y[:cpu_sample]
+        vc = rng.normal(size=(cpu_sample, nu))
+        t0 = time.perf_counter()
+        for i in range(cpu_sample):
+            o.multiply_by_m(Yc[i, :nq], vc[i])
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = {"op": "multiplyByM", "bodies_per_s": cpu_sample * spec.nb / sec, "cores": 1,
+                               "kind": "port", "sample": f"{cpu_sample} chains, one thread, through ctypes"}
+    return out
+
+
 def row_mixed(torch, M, MatterBatch, RungeKuttaMerson, n=65536, T=0.5, h=1e-3):
     """Mixed precision vs FP64 on the 32-body chains (Merson fixed step h to
     T) and on a stiff variant (neighbouring-link mass ratios up to 1e4;
@@ -282,7 +336,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the secondary workloads (pendulum latency, uneven trees)")
-    ap.add_argument("--only", choices=["pendulum", "trees", "serving", "mixed"], default=None,
+    ap.add_argument("--only", choices=["pendulum", "trees", "serving", "mixed", "operators"], default=None,
                     help="run only one secondary workload and print its JSON")
     ap.add_argument("--tree-lanes", type=int, default=0,
                     help="lanes per instance of the tree evaluator (0 automatic, 1, 2, 4, 8, 16)")
@@ -318,6 +372,9 @@ def main():
         print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu,
                                    lanes=args.tree_lanes, lane_range=tuple(args.tree_lane_range),
                                    evaluator=args.tree_evaluator)), flush=True)
+        return
+    if args.only == "operators":
+        print(json.dumps(row_operators(torch, M, MatterBatch, cpu=not args.no_cpu)), flush=True)
         return
     if args.only == "mixed":
         print(json.dumps(row_mixed(torch, M, MatterBatch, RungeKuttaMerson)), flush=True)
@@ -395,6 +452,7 @@ def main():
         log("[bench] pendulum latency")
         rows["pendulum10"] = row_pendulum(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)
         rows["uneven_trees"] = row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu)
+        rows["operators"] = row_operators(torch, M, MatterBatch, cpu=not args.no_cpu)
         rows["mixed_precision"] = row_mixed(torch, M, MatterBatch, RungeKuttaMerson)
         rows["serving"] = row_serving(torch, M, RungeKuttaMerson)
         out["rows"] = rows

@@ -292,8 +292,8 @@ class MatterBatch:
             self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(ud), _ptr(A), _ptr(qdd), _stream()))
         return ud, A, qdd
 
-    def multiplyByMInv(self, q, v):
-        out = self._new(self.total_u)
+    def multiplyByMInv(self, q, v, out=None):
+        out = self._new(self.total_u) if out is None else out
         _check(lib().tdyn_multiply_by_m_inv(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
         return out
 
@@ -315,36 +315,36 @@ class MatterBatch:
                                                          _ptr(out), _stream()))
         return out
 
-    def multiplyByM(self, q, v):
+    def multiplyByM(self, q, v, out=None):
         """M v per instance (packed like u)."""
-        out = self._new(self.total_u)
+        out = self._new(self.total_u) if out is None else out
         _check(lib().tdyn_multiply_by_m(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
         return out
 
-    def multiplyBySystemJacobian(self, q, v):
+    def multiplyBySystemJacobian(self, q, v, out=None):
         """J v: (nb+1)*6 per instance, Ground's entry zero."""
-        out = self._new(6 * self.total_b)
+        out = self._new(6 * self.total_b) if out is None else out
         _check(lib().tdyn_multiply_by_system_jacobian(self.h, _ptr(q), _ptr(v), _ptr(out), _stream()))
         return out
 
-    def multiplyBySystemJacobianTranspose(self, q, F_G):
+    def multiplyBySystemJacobianTranspose(self, q, F_G, out=None):
         """J^T F_G (packed like u); F_G (nb+1)*6 per instance, Ground ignored."""
-        out = self._new(self.total_u)
+        out = self._new(self.total_u) if out is None else out
         _check(lib().tdyn_multiply_by_system_jacobian_transpose(self.h, _ptr(q), _ptr(F_G), _ptr(out),
                                                                 _stream()))
         return out
 
-    def calcBodyAccelerationFromUDot(self, q, u, udot=None):
+    def calcBodyAccelerationFromUDot(self, q, u, udot=None, out=None):
         """J udot + JDot u: (nb+1)*6 per instance; udot None means zero."""
-        out = self._new(6 * self.total_b)
+        out = self._new(6 * self.total_b) if out is None else out
         _check(lib().tdyn_calc_body_acceleration_from_udot(self.h, _ptr(q), _ptr(u), _ptr(udot), _ptr(out),
                                                            _stream()))
         return out
 
-    def calcCompositeBodyInertias(self, q):
+    def calcCompositeBodyInertias(self, q, out=None):
         """(nb+1)*10 per instance: mass, mass center (3), unit inertia about
         the body origin (xx yy zz xy xz yz), in G; Ground first."""
-        out = self._new(10 * self.total_b)
+        out = self._new(10 * self.total_b) if out is None else out
         _check(lib().tdyn_calc_composite_body_inertias(self.h, _ptr(q), _ptr(out), _stream()))
         return out
 
