@@ -1,7 +1,9 @@
 // Example caller of the C++ drop-in (include/tdyn_matter.hpp): the README
 // pendulum of the reference extended to 10 Pin links (config 1), written the
 // way a Simbody user writes it -- build the tree, realize(Acceleration), then
-// TimeStepper::stepTo with RungeKuttaMerson at accuracy 1e-8.
+// TimeStepper::stepTo with RungeKuttaMerson at accuracy 1e-8.  Also prints
+// M*udot (multiplyByM) and the composite mass of body 1
+// (calcCompositeBodyInertias) at the initial state.
 // Usage: dropin_example q0..q9 u0..u9  -> prints one JSON line.
 #include "../../include/tdyn_matter.hpp"
 
@@ -25,16 +27,21 @@ int main(int argc, char** argv) {
                      {0.0, -9.8, 0.0}};
     std::vector<double> hq(n), hu(n);
     for (int i = 0; i < n; ++i) { hq[i] = std::atof(argv[1 + i]); hu[i] = std::atof(argv[1 + n + i]); }
-    double *q, *u, *udot; int32_t* stats;
+    double *q, *u, *udot, *Mud, *R; int32_t* stats;
     if (hipMalloc((void**)&q, n * 8) || hipMalloc((void**)&u, n * 8) || hipMalloc((void**)&udot, n * 8) ||
+        hipMalloc((void**)&Mud, n * 8) || hipMalloc((void**)&R, 10 * (n + 1) * 8) ||
         hipMalloc((void**)&stats, 16)) { std::fprintf(stderr, "hipMalloc failed\n"); return 1; }
     (void)hipMemcpy(q, hq.data(), n * 8, hipMemcpyHostToDevice);
     (void)hipMemcpy(u, hu.data(), n * 8, hipMemcpyHostToDevice);
     try {
         tdyn::MatterBatch matter(d, 1);
         matter.realizeAcceleration(q, u, nullptr, udot);
-        std::vector<double> hud(n);
+        std::vector<double> hud(n), hMud(n), hR(10 * (n + 1));
         (void)hipMemcpy(hud.data(), udot, n * 8, hipMemcpyDeviceToHost);
+        matter.multiplyByM(q, udot, Mud);
+        matter.calcCompositeBodyInertias(q, R);
+        (void)hipMemcpy(hMud.data(), Mud, n * 8, hipMemcpyDeviceToHost);
+        (void)hipMemcpy(hR.data(), R, 10 * (n + 1) * 8, hipMemcpyDeviceToHost);
         tdyn::RungeKuttaMerson integ; integ.setAccuracy(1e-8);
         matter.stepTo(q, u, nullptr, 0.0, 1.0, integ, stats);
         std::vector<int32_t> hs(4);
@@ -46,8 +53,9 @@ int main(int argc, char** argv) {
             for (size_t i = 0; i < v.size(); ++i) { char b[40]; std::snprintf(b, sizeof b, "%s%.17g", i ? "," : "", v[i]); s += b; }
             return s + "]";
         };
-        std::printf("{\"udot\": %s, \"q1\": %s, \"u1\": %s, \"attempts\": %d, \"accepted\": %d, \"rejected\": %d}\n",
-                    arr(hud).c_str(), arr(hq).c_str(), arr(hu).c_str(), hs[0], hs[1], hs[2]);
+        std::printf("{\"udot\": %s, \"Mudot\": %s, \"mass1\": %.17g, \"q1\": %s, \"u1\": %s, "
+                    "\"attempts\": %d, \"accepted\": %d, \"rejected\": %d}\n",
+                    arr(hud).c_str(), arr(hMud).c_str(), hR[10], arr(hq).c_str(), arr(hu).c_str(), hs[0], hs[1], hs[2]);
     } catch (const tdyn::Error& e) {
         std::fprintf(stderr, "tdyn error %d: %s\n", e.code(), e.what());
         return 1;

@@ -1,7 +1,8 @@
 """The C++ drop-in header (include/tdyn_matter.hpp) used as a Simbody user
 would: the 10-link pendulum of config 1, realize(Acceleration) and
 TimeStepper::stepTo with RK-Merson at accuracy 1e-8, compared with the
-committed fixture (oracle results)."""
+committed fixture (oracle results); multiplyByM and the composite mass of
+body 1 at the initial state."""
 import json
 import os
 import subprocess
@@ -10,6 +11,8 @@ import numpy as np
 import pytest
 
 from common import rel_inf
+from simbody_amd import models as M
+from oracle_ctypes import OracleModel
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "simbody_amd", "dropin_example")
@@ -29,5 +32,10 @@ def test_cpp_dropin_pendulum10():
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout)
     assert rel_inf(r["udot"], g["udot"]) < 1e-12
+    # multiplyByM through the C++ class, against the oracle at the same state
+    spec = M.pendulum_chain(10)
+    o = OracleModel(spec)
+    assert rel_inf(r["Mudot"], o.multiply_by_m(np.array(g["q"]), np.array(r["udot"]))) < 1e-12
+    assert r["mass1"] == 10.0
     assert (r["attempts"], r["accepted"], r["rejected"]) == (g["attempts"], g["accepted"], g["rejected"])
     assert rel_inf(r["q1"] + r["u1"], g["q1"] + g["u1"]) < 1e-9
