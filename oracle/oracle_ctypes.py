@@ -21,7 +21,7 @@ class TreeDesc(ctypes.Structure):
     """Same layout as tdyn_tree_desc in include/tdyn.h."""
     _fields_ = [("nb", ctypes.c_int32), ("parent", _i), ("mobilizer", _i),
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
-                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3)]
+                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("pitch", _d)]
 
 
 _lib = None
@@ -80,6 +80,10 @@ class OracleModel:
         d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (_p(x) for x in k[2:])
         for j in range(3):
             d.gravity[j] = float(spec.gravity[j])
+        pitch = getattr(spec, "pitch", None)
+        if pitch is not None:
+            self._pitch = np.ascontiguousarray(pitch, np.float64)
+            d.pitch = _p(self._pitch)
         self._desc = d
         h = lib().oracle_model_new(ctypes.byref(d))
         if not h:

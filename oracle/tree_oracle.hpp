@@ -179,9 +179,15 @@ template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
 // ---------------------------------------------------------------------------
 // Model.
 // ---------------------------------------------------------------------------
-enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4 };
-inline int mobDof(int t) { return t==PIN||t==SLIDER ? 1 : t==UNIVERSAL ? 2 : t==BALL ? 3 : 6; }
-inline int mobNq (int t) { return t==PIN||t==SLIDER ? 1 : t==UNIVERSAL ? 2 : t==BALL ? 4 : 7; }
+enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8 };
+inline int mobDof(int t) {
+    static const int d[9] = {1, 1, 2, 3, 6, 3, 3, 2, 1};
+    return d[t];
+}
+inline int mobNq(int t) {
+    static const int n[9] = {1, 1, 2, 4, 7, 3, 3, 2, 1};
+    return n[t];
+}
 
 // Plain-array tree description; identical to tdyn_tree_desc of include/tdyn.h.
 struct TreeDesc {
@@ -194,6 +200,7 @@ struct TreeDesc {
     const double*  X_PF;        // [nb][12] row-major R then p
     const double*  X_BM;        // [nb][12]
     double gravity[3];
+    const double*  pitch;       // [nb] Screw pitch (length per radian); nullable
 };
 
 struct Model {
@@ -206,6 +213,7 @@ struct Model {
     std::vector<M33<double>> Gunit;           // unit inertia about Bo in B
     std::vector<Xform<double>> X_PF, X_MB;
     std::vector<V3<double>> p_BM;
+    std::vector<double> pitch;
     V3<double> g;
 
     // Build from the plain description.  Body 0 is Ground.
@@ -220,14 +228,14 @@ struct Model {
         children.assign(n, {}); mass.assign(n, 0.0);
         com.assign(n, zero3<double>()); Gunit.assign(n, zero33<double>());
         X_PF.assign(n, Xform<double>{ident33<double>(), zero3<double>()});
-        X_MB = X_PF; p_BM.assign(n, zero3<double>());
+        X_MB = X_PF; p_BM.assign(n, zero3<double>()); pitch.assign(n, 0.0);
         nq = nu = nquat = 0;
         for (int b = 1; b <= nb; ++b) {
             const int i = b-1;
             const int p = d.parent[i];
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
-            if (t < PIN || t > FREE) return false;
+            if (t < PIN || t > SCREW) return false;
             if (t == FREE && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
@@ -249,6 +257,7 @@ struct Model {
             for (int k=0;k<9;++k) X_BM.R.m[k] = d.X_BM[12*i+k];
             X_BM.p = mk3(d.X_BM[12*i+9], d.X_BM[12*i+10], d.X_BM[12*i+11]);
             p_BM[b] = X_BM.p;
+            pitch[b] = d.pitch ? d.pitch[i] : 0.0;
             X_MB[b] = invert(X_BM);      // RigidBodyNode.h:1012 X_MB(~X_BM)
         }
         int maxLevel = 0;
@@ -382,6 +391,35 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
             B.X_FM.R = R;
             B.H_FM[0] = sv(mk3<T>(T(1),T(0),T(0)), zero3<T>());
             B.H_FM[1] = sv(mk3<T>(R(0,1),R(1,1),R(2,1)), zero3<T>());   // R_FM.y()
+        } break;
+        // Translation, Planar, Cylinder, Screw: H_FM constant in F, so their
+        // HDot_FM is zero (RigidBodyNodeSpec_Translation.h, _Planar.h:115-156,
+        // _Cylinder.h:100-139, _Screw.h:109-148).
+        case TRANSLATION: {
+            B.X_FM.R = ident33<T>();
+            B.X_FM.p = mk3<T>(qb[0], qb[1], qb[2]);
+            for (int j=0;j<3;++j) { V3<T> e = zero3<T>(); e[j] = T(1); B.H_FM[j] = sv(zero3<T>(), e); }
+        } break;
+        case PLANAR: case CYLINDER: case SCREW: {
+            B.cq[0] = cos(qb[0]); B.sq[0] = sin(qb[0]);
+            M33<T> R = zero33<T>();
+            R(0,0)=B.cq[0]; R(0,1)=-B.sq[0]; R(1,0)=B.sq[0]; R(1,1)=B.cq[0]; R(2,2)=T(1);
+            B.X_FM.R = R;
+            const V3<T> ez = mk3<T>(T(0),T(0),T(1));
+            if (t == PLANAR) {
+                B.X_FM.p = mk3<T>(qb[1], qb[2], T(0));
+                B.H_FM[0] = sv(ez, zero3<T>());
+                B.H_FM[1] = sv(zero3<T>(), mk3<T>(T(1),T(0),T(0)));
+                B.H_FM[2] = sv(zero3<T>(), mk3<T>(T(0),T(1),T(0)));
+            } else if (t == CYLINDER) {
+                B.X_FM.p = mk3<T>(T(0), T(0), qb[1]);
+                B.H_FM[0] = sv(ez, zero3<T>());
+                B.H_FM[1] = sv(zero3<T>(), ez);
+            } else {
+                const T pt = T(M.pitch[b]);
+                B.X_FM.p = mk3<T>(T(0), T(0), qb[0]*pt);
+                B.H_FM[0] = sv(ez, mk3<T>(T(0), T(0), pt));
+            }
         } break;
         case BALL: case FREE: {
             const T len = sqrt(qb[0]*qb[0] + qb[1]*qb[1] + qb[2]*qb[2] + qb[3]*qb[3]);

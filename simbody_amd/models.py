@@ -18,8 +18,9 @@ from dataclasses import dataclass, field
 import numpy as np
 
 PIN, SLIDER, UNIVERSAL, BALL, FREE = 0, 1, 2, 3, 4
-DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6}
-NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7}
+TRANSLATION, PLANAR, CYLINDER, SCREW = 5, 6, 7, 8
+DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1}
+NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1}
 NAMES = {PIN: "Pin", SLIDER: "Slider", UNIVERSAL: "Universal", BALL: "Ball", FREE: "Free"}
 
 
@@ -33,6 +34,7 @@ class TreeSpec:
     X_PF: np.ndarray        # [nb,12]
     X_BM: np.ndarray        # [nb,12]
     gravity: np.ndarray = field(default_factory=lambda: np.array([0.0, -9.8, 0.0]))
+    pitch: np.ndarray = None  # [nb] Screw pitch (length per radian), None = zeros
 
     @property
     def nb(self):
@@ -325,6 +327,11 @@ def _state_plan(spec):
                 plan.append(("angle", qi, c)); qi += 1; c += 1
         elif t == SLIDER:
             plan.append(("trans", qi, c)); qi += 1; c += 1
+        elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW):
+            kinds = {TRANSLATION: ("trans",) * 3, PLANAR: ("angle", "trans", "trans"),
+                     CYLINDER: ("angle", "trans"), SCREW: ("angle",)}[t]
+            for k in kinds:
+                plan.append((k, qi, c)); qi += 1; c += 1
         else:
             plan.append(("quat", qi, c)); qi += 4; c += 8
             if t == FREE:

@@ -27,3 +27,21 @@ def mixed_trees(seed, n, nmin=4, nmax=16, free_every=3):
             t.mobilizer[0] = M.FREE
         trees.append(t)
     return trees
+
+
+def ext_trees(seed, n, nmin=4, nmax=16):
+    """Random trees (general joint frames) whose non-root bodies draw all nine
+    mobilizers but Free uniformly -- Pin, Slider, Universal, Ball, Translation,
+    Planar, Cylinder, Screw -- with Screw pitches in [-0.5, 0.5]; every third
+    root is Free."""
+    types = [M.PIN, M.SLIDER, M.UNIVERSAL, M.BALL, M.TRANSLATION, M.PLANAR, M.CYLINDER, M.SCREW]
+    trees = []
+    for i in range(n):
+        t = M.random_tree(seed, i, nmin=nmin, nmax=nmax, general_frames=True)
+        rng = np.random.default_rng(seed * 7919 + i)
+        t.mobilizer[:] = rng.choice(types, size=t.nb)
+        if i % 3 == 0:
+            t.mobilizer[0] = M.FREE
+        t.pitch = rng.uniform(-0.5, 0.5, size=t.nb)
+        trees.append(t)
+    return trees
