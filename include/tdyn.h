@@ -46,25 +46,32 @@ extern "C" {
 #define TDYN_E_HIP          6   /* HIP runtime error */
 #define TDYN_E_NOMEM        7
 
-/* Mobilizer codes: MobilizedBody::Pin, Slider, Universal, Ball, Free. */
-#define TDYN_PIN       0
-#define TDYN_SLIDER    1
-#define TDYN_UNIVERSAL 2
-#define TDYN_BALL      3
-#define TDYN_FREE      4
+/* Mobilizer codes: MobilizedBody::Pin, Slider, Universal, Ball, Free,
+ * Translation, Planar, Cylinder, Screw (RigidBodyNodeSpec_<Type>.h). */
+#define TDYN_PIN         0
+#define TDYN_SLIDER      1
+#define TDYN_UNIVERSAL   2
+#define TDYN_BALL        3
+#define TDYN_FREE        4
+#define TDYN_TRANSLATION 5   /* q = p_FM (3), u = v_FM in F (3) */
+#define TDYN_PLANAR      6   /* q = (angle about z, x, y), u = their rates, translation in F */
+#define TDYN_CYLINDER    7   /* q = (angle about z, z), u = their rates */
+#define TDYN_SCREW       8   /* q = angle about z, translation pitch*q along z */
 
 /* One tree (what MobilizedBody::<Type>(parent, X_PF, body, X_BM) calls build;
  * MobilizedBody.cpp:783-808).  Host pointers. */
 typedef struct tdyn_tree_desc {
     int32_t nb;                /* mobilized bodies (Ground excluded) */
     const int32_t* parent;     /* [nb] parent body number, 0 = Ground, parent[b-1] < b */
-    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_FREE (Free only on Ground) */
+    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_SCREW (Free only on Ground) */
     const double* mass;        /* [nb] */
     const double* com_B;       /* [nb][3] mass center in B */
     const double* inertia_B;   /* [nb][6] inertia about Bo in B: xx yy zz xy xz yz */
     const double* X_PF;        /* [nb][12] inboard frame on the parent: R row-major, p */
     const double* X_BM;        /* [nb][12] outboard frame on the body */
     double gravity[3];         /* Force::Gravity direction*magnitude (Force_Gravity.cpp:210-229) */
+    const double* screw_pitch; /* [nb] MobilizedBody::Screw pitch (length per radian), read for Screw
+                                  bodies only; NULL: no Screw bodies */
 } tdyn_tree_desc;
 
 typedef struct tdyn_batch tdyn_batch;

@@ -33,7 +33,10 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     for (int b = 1; b <= d.nb; ++b) {
         const int p = d.parent[b-1], ty = d.mobilizer[b-1];
         if (p < 0 || p >= b) { snprintf(buf, sizeof buf, "tree %d body %d: parent %d must be in [0,%d)", t, b, p, b); return fail(TDYN_E_ARG, buf); }
-        if (ty < TDYN_PIN || ty > TDYN_FREE) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty < TDYN_PIN || ty > TDYN_SCREW) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty == TDYN_SCREW && (!d.screw_pitch || !std::isfinite(d.screw_pitch[b-1]))) {
+            snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
+        }
         if (ty == TDYN_FREE && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Free mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
         if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
         if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
@@ -71,6 +74,7 @@ void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
     for (int k = 0; k < 3; ++k) P[PRM_COM + k] = d.com_B[3*(b-1) + k];
     for (int k = 0; k < 6; ++k) P[PRM_G + k] = d.inertia_B[6*(b-1) + k] / m;
     for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
+    if (d.mobilizer[b-1] == TDYN_SCREW) P[PRM_PITCH] = d.screw_pitch[b-1];
 }
 
 int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
@@ -193,12 +197,12 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         bool chain = true;
         for (int b = 1; b <= nb; ++b) {
             const int ty = d.mobilizer[b-1];
-            chain = chain && d.parent[b-1] == b - 1;
+            chain = chain && d.parent[b-1] == b - 1 && !mob_generic_only(ty);
             B->h_body_a.push_back(make_int4(d.parent[b-1], ty, q, u));
             body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, gslot));
             for (int c : ch[b]) child_idx.push_back(c);
             // batches of different trees carry non-Free bodies in padded 3-dof records
-            const int rd = (!B->uniform && ty != TDYN_FREE) ? 3 : mob_dof(ty);
+            const int rd = ((!B->uniform || mob_generic_only(ty)) && ty != TDYN_FREE) ? 3 : mob_dof(ty);
             q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(rd); gslot += 7 * mob_dof(ty);
             if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
             double P[PRM_STRIDE];
@@ -301,7 +305,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     std::vector<int> inst_of_lane(n_instances);
     for (int i = 0; i < n_instances; ++i) inst_of_lane[i] = i;
     if (!B->uniform) {
-        static const double cost[5] = {1.0, 0.9, 1.4, 1.8, 3.3};
+        static const double cost[9] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0};
         std::vector<double> tw(n_trees, 0.0);
         for (int t = 0; t < n_trees; ++t)
             for (int bdy = 0; bdy < trees[t].nb; ++bdy) tw[t] += cost[trees[t].mobilizer[bdy]];

@@ -116,16 +116,41 @@ template <int TYPE, class K> struct Joint {
     m33<K> R_FM; v3<K> p_FM; sv<K> H_FM[D];
     int ty = TYPE, d = D;       // run-time type and dof of a GEN body
     // tc: cached trig / normalization values of this q (written by the first
-    // sweep with CACHED = false): Pin (cos,sin), Universal (c1,s1,c2,s2),
-    // Ball/Free 1/|quaternion| (computed in FP64).
+    // sweep with CACHED = false): Pin, Planar, Cylinder, Screw (cos,sin of
+    // q0), Universal (c1,s1,c2,s2), Ball/Free 1/|quaternion| (computed in
+    // FP64).  pitch: the Screw pitch (GEN only).
     template <bool CACHED>
-    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE) {
+    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0)) {
         const K z = K(0), o = K(1);
         p_FM = zero3<K>();
         if constexpr (TYPE == GEN) {
             ty = rty; d = mob_dof(rty);
             H_FM[0] = H_FM[1] = H_FM[2] = svzero<K>();
-            if (rty == PIN) {
+            if (rty == TRANSLATION) {
+                // RigidBodyNodeSpec_Translation.h: X_FM = (I, q), H_FM = (0, e_k)
+                R_FM = ident<K>(); p_FM = V3(K(qv[0]), K(qv[1]), K(qv[2]));
+                H_FM[0] = {zero3<K>(), V3(o, z, z)};
+                H_FM[1] = {zero3<K>(), V3(z, o, z)};
+                H_FM[2] = {zero3<K>(), V3(z, z, o)};
+            } else if (rty == PLANAR || rty == CYLINDER || rty == SCREW) {
+                // _Planar.h:115-156, _Cylinder.h:100-139, _Screw.h:109-148
+                K sn, cs;
+                if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
+                R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
+                R_FM.a[6]=z;  R_FM.a[7]=z;   R_FM.a[8]=o;
+                H_FM[0] = {V3(z, z, o), zero3<K>()};
+                if (rty == PLANAR) {
+                    p_FM = V3(K(qv[1]), K(qv[2]), z);
+                    H_FM[1] = {zero3<K>(), V3(o, z, z)};
+                    H_FM[2] = {zero3<K>(), V3(z, o, z)};
+                } else if (rty == CYLINDER) {
+                    p_FM = V3(z, z, K(qv[1]));
+                    H_FM[1] = {zero3<K>(), V3(z, z, o)};
+                } else {
+                    p_FM = V3(z, z, K(qv[0])*pitch);
+                    H_FM[0] = {V3(z, z, o), V3(z, z, pitch)};
+                }
+            } else if (rty == PIN) {
                 K sn, cs;
                 if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
                 R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
@@ -342,6 +367,13 @@ __device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 
     }
 }
 
+// The pitch of a Screw body (padded generic bodies only), else zero.
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ K screw_pitch(const PA& P, const Pre<K>& x) {
+    if constexpr (TYPE == GEN) { if (x.ty == SCREW) return K(P[PRM_PITCH]); }
+    return K(0);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1 body: forward transforms and velocities, qdot out, trig cache out.
 // ---------------------------------------------------------------------------
@@ -371,7 +403,7 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
     }
     Joint<TYPE, K> J;
     K tc[4] = {K(0), K(0), K(0), K(0)};
-    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE);
+    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
 #pragma unroll
     for (int k = 0; k < TcN<TYPE>::n; ++k) st.T(k) = tc[k];
     m33<K> R_PB; v3<K> p_PB;
@@ -440,7 +472,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE);
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     const m33<K> R_GB = R; const v3<K> p_GB = p; const sv<K> V_GB = V;
@@ -548,7 +580,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE);
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];

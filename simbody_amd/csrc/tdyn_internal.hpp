@@ -19,17 +19,36 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
     return make_int4(p[0], p[1], p[2], p[3]);
 }
 
-enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4 };
+enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8 };
 // Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
-enum { GEN = 5 };
-__host__ __device__ inline int mob_dof(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 3 : 6; }
-__host__ __device__ inline int mob_nq(int t) { return t <= SLIDER ? 1 : t == UNIVERSAL ? 2 : t == BALL ? 4 : 7; }
+enum { GEN = 15 };
+__host__ __device__ inline int mob_dof(int t) {
+    switch (t) {
+    case PIN: case SLIDER: case SCREW: return 1;
+    case UNIVERSAL: case CYLINDER: return 2;
+    case BALL: case TRANSLATION: case PLANAR: return 3;
+    default: return 6;
+    }
+}
+__host__ __device__ inline int mob_nq(int t) {
+    switch (t) {
+    case PIN: case SLIDER: case SCREW: return 1;
+    case UNIVERSAL: case CYLINDER: return 2;
+    case TRANSLATION: case PLANAR: return 3;
+    case BALL: return 4;
+    default: return 7;
+    }
+}
+// Translation, Planar, Cylinder and Screw run only as padded generic bodies
+// (constant H_FM in F, so no HDot_FM term); the typed code paths cover
+// Pin, Slider, Universal, Ball and Free.
+__host__ __device__ inline bool mob_generic_only(int t) { return t >= TRANSLATION; }
 
 // Per-body constants, PRM_STRIDE doubles per body (packed over all trees).
 enum {
     PRM_R_PF = 0, PRM_P_PF = 9, PRM_R_MB = 12, PRM_P_MB = 21, PRM_MASS = 24,
     PRM_COM = 25, PRM_G = 28 /* unit inertia about Bo in B: xx yy zz xy xz yz */,
-    PRM_P_BM = 34, PRM_STRIDE = 40
+    PRM_P_BM = 34, PRM_PITCH = 37 /* Screw */, PRM_STRIDE = 40
 };
 
 // Per-body record in the per-lane scratch (doubles, relative to rec_off).

@@ -241,6 +241,35 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         R_FM.a[6]=-s2*c1; R_FM.a[7]= s1;  R_FM.a[8]= c1*c2;
         H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
         H_FM[1] = {col(R_FM, 1), zero3<double>()};
+    } else if (ty == TRANSLATION) {
+        // RigidBodyNodeSpec_Translation.h: X_FM = (I, q), H_FM = (0, e_k)
+        qv[0] = q(qi); qv[1] = q(qi + 1); qv[2] = q(qi + 2);
+        R_FM = ident<double>(); p_FM = V3(qv[0], qv[1], qv[2]);
+        H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+        H_FM[1] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
+        H_FM[2] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
+    } else if (ty == PLANAR || ty == CYLINDER || ty == SCREW) {
+        // RigidBodyNodeSpec_Planar.h:115-156, _Cylinder.h:100-139,
+        // _Screw.h:109-148: rotation about z by q0, H_FM constant in F
+        qv[0] = q(qi);
+        double sn, cs; sincos(qv[0], &sn, &cs);
+        R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=0; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=0;
+        R_FM.a[6]=0;  R_FM.a[7]=0;   R_FM.a[8]=1;
+        H_FM[0] = {V3(0.0, 0.0, 1.0), zero3<double>()};
+        if (ty == PLANAR) {
+            qv[1] = q(qi + 1); qv[2] = q(qi + 2);
+            p_FM = V3(qv[1], qv[2], 0.0);
+            H_FM[1] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+            H_FM[2] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
+        } else if (ty == CYLINDER) {
+            qv[1] = q(qi + 1);
+            p_FM = V3(0.0, 0.0, qv[1]);
+            H_FM[1] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
+        } else {
+            const double pt = P[PRM_PITCH];
+            p_FM = V3(0.0, 0.0, qv[0]*pt);
+            H_FM[0] = {V3(0.0, 0.0, 1.0), V3(0.0, 0.0, pt)};
+        }
     } else {   // BALL
 #pragma unroll
         for (int k = 0; k < 4; ++k) qv[k] = q(qi + k);
@@ -314,7 +343,7 @@ template <bool VEL, bool PAD = false, class QG, class UG, class QD>
 __device__ void sweep_kin(const Lane& c, const QG& q, const UG& u, const QD& qd) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
-        if constexpr (PAD) {
+        if (PAD || mob_generic_only(ba.y)) {
             if (ba.y != FREE) { kin_body_gen<VEL>(c, b, ba, q, u, qd); continue; }
         }
         switch (ba.y) {
@@ -484,7 +513,7 @@ __device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, dou
                              const double* grav) {
     for (int b = c.nb; b >= 1; --b) {
         const int4 ba = c.ba(b), bb = c.bb(b);
-        if constexpr (PAD) {
+        if (PAD || mob_generic_only(ba.y)) {
             if (ba.y != FREE) { inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); continue; }
         }
         switch (ba.y) {
@@ -548,7 +577,7 @@ template <bool WITH_COR, bool PAD = false, class QG, class UG, class UO, class Q
 __device__ void sweep_outward2(const Lane& c, const QG& q, const UG& u, const UO& uo, const QO& qdd) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
-        if constexpr (PAD) {
+        if (PAD || mob_generic_only(ba.y)) {
             if (ba.y != FREE) { outward_body<GEN, WITH_COR>(c, b, ba, q, u, uo, qdd); continue; }
         }
         switch (ba.y) {
@@ -703,7 +732,7 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
 // D of a record: padded generic bodies are 3-dof in batches of different trees.
-#define TD_RDOF(ba) ((PAD && (ba).y != FREE) ? 3 : mob_dof((ba).y))
+#define TD_RDOF(ba) (((PAD || mob_generic_only((ba).y)) && (ba).y != FREE) ? 3 : mob_dof((ba).y))
 template <int OP, bool UNI>
 __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     constexpr bool PAD = !UNI;      // batches of different trees use padded records
@@ -876,7 +905,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         const PackedPut tau{a.out_u + uo};
         for (int b = 1; b <= c.nb; ++b) {
             const int4 ba = c.ba(b);
-            switch ((PAD && ba.y != FREE) ? 3 : mob_dof(ba.y)) {
+            switch (TD_RDOF(ba)) {
             case 1: resid_out_body<1>(c, b, ba, ud); break;
             case 2: resid_out_body<2>(c, b, ba, ud); break;
             case 3: resid_out_body<3>(c, b, ba, ud); break;
@@ -885,7 +914,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         }
         for (int b = c.nb; b >= 1; --b) {
             const int4 ba = c.ba(b), bb = c.bb(b);
-            switch ((PAD && ba.y != FREE) ? 3 : mob_dof(ba.y)) {
+            switch (TD_RDOF(ba)) {
             case 1: resid_in_body<1>(c, b, ba, bb, fb, fm, tau); break;
             case 2: resid_in_body<2>(c, b, ba, bb, fb, fm, tau); break;
             case 3: resid_in_body<3>(c, b, ba, bb, fb, fm, tau); break;

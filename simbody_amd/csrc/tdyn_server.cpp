@@ -58,9 +58,12 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
     *out = nullptr;
     { const int rc = validate_tree(*topology, 0); if (rc) return rc; }
     const int nb = topology->nb;
-    for (int b = 1; b <= nb; ++b)
+    for (int b = 1; b <= nb; ++b) {
         if (topology->parent[b-1] != b - 1)
             return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs serial-chain topologies (parent[b-1] == b-1)");
+        if (mob_generic_only(topology->mobilizer[b-1]))
+            return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs Pin, Slider, Universal, Ball and Free bodies");
+    }
     RkmCfg c;
     int prec = 0;
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }

@@ -50,7 +50,7 @@ class TdynError(RuntimeError):
 class TreeDescC(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int32), ("parent", _i32), ("mobilizer", _i32),
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
-                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3)]
+                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("screw_pitch", _d)]
 
 
 class IntegratorCfgC(ctypes.Structure):
@@ -128,6 +128,11 @@ def fill_tree_desc(d, spec, keep):
     d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (a.ctypes.data_as(_d) for a in arrs[2:])
     for j in range(3):
         d.gravity[j] = float(spec.gravity[j])
+    pitch = getattr(spec, "pitch", None)
+    if pitch is not None:
+        pa = np.ascontiguousarray(pitch, np.float64)
+        keep.append(pa)
+        d.screw_pitch = pa.ctypes.data_as(_d)
     return d
 
 
