@@ -179,13 +179,14 @@ template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
 // ---------------------------------------------------------------------------
 // Model.
 // ---------------------------------------------------------------------------
-enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8 };
+enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8,
+                 GIMBAL=9 };
 inline int mobDof(int t) {
-    static const int d[9] = {1, 1, 2, 3, 6, 3, 3, 2, 1};
+    static const int d[10] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3};
     return d[t];
 }
 inline int mobNq(int t) {
-    static const int n[9] = {1, 1, 2, 4, 7, 3, 3, 2, 1};
+    static const int n[10] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3};
     return n[t];
 }
 
@@ -235,7 +236,7 @@ struct Model {
             const int p = d.parent[i];
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
-            if (t < PIN || t > SCREW) return false;
+            if (t < PIN || t > GIMBAL) return false;
             if (t == FREE && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
@@ -276,7 +277,7 @@ struct Model {
 // ---------------------------------------------------------------------------
 template <class T> struct Body {
     // position
-    T cq[2], sq[2];            // mobilizer q pool (sin/cos)
+    T cq[3], sq[3];            // mobilizer q pool (sin/cos)
     T qerr;                    // |quaternion| - 1
     Xform<T> X_FM, X_PB, X_GB;
     V3<T> phi;                 // p_PB_G
@@ -350,7 +351,7 @@ template <class T> inline void quatDotDot(const T q[4], const V3<T>& w, const V3
 // Position kinematics, base to tip.
 // RigidBodyNodeSpec.h:229-288 (realizePosition) with the mobilizer-specific
 // performQPrecalculations/calcX_FM/calcAcrossJointVelocityJacobian of
-// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free}.h, calcBodyTransforms
+// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free,Gimbal}.h, calcBodyTransforms
 // (RigidBodyNodeSpec.h:554-569), calcParentToChildVelocityJacobianInGround
 // (RigidBodyNodeSpec.cpp:44-74) and calcJointIndependentKinematicsPos
 // (RigidBodyNode.cpp:54-84).  Driver: SimbodyMatterSubsystemRep.cpp:1112-1144.
@@ -421,6 +422,21 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
                 B.H_FM[0] = sv(ez, mk3<T>(T(0), T(0), pt));
             }
         } break;
+        case GIMBAL: {
+            // body-fixed 1-2-3 angles (RigidBodyNodeSpec_Gimbal.h:141-176,
+            // Rotation.h:342-349); H_FM maps qdot to w_FM.
+            for (int j=0;j<3;++j) { B.cq[j] = cos(qb[j]); B.sq[j] = sin(qb[j]); }
+            const T c0=B.cq[0], c1=B.cq[1], c2=B.cq[2], s0=B.sq[0], s1=B.sq[1], s2=B.sq[2];
+            const T s0s1 = s0*s1, s2c0 = s2*c0, c0c2 = c0*c2, nc1 = -c1;
+            M33<T> R;
+            R(0,0)= c1*c2;              R(0,1)= s2*nc1;             R(0,2)= s1;
+            R(1,0)= s2c0 + s0s1*c2;     R(1,1)= c0c2 - s0s1*s2;     R(1,2)= s0*nc1;
+            R(2,0)= s0*s2 - s1*c0c2;    R(2,1)= s0*c2 + s1*s2c0;    R(2,2)= c0*c1;
+            B.X_FM.R = R;
+            B.H_FM[0] = sv(mk3<T>(T(1), T(0), T(0)), zero3<T>());
+            B.H_FM[1] = sv(mk3<T>(T(0), c0, s0), zero3<T>());
+            B.H_FM[2] = sv(mk3<T>(s1, -s0*c1, c0*c1), zero3<T>());
+        } break;
         case BALL: case FREE: {
             const T len = sqrt(qb[0]*qb[0] + qb[1]*qb[1] + qb[2]*qb[2] + qb[3]*qb[3]);
             B.qerr = len - T(1);
@@ -465,7 +481,8 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
 
 // ---------------------------------------------------------------------------
 // Velocity kinematics, base to tip.  RigidBodyNodeSpec.h:305-333,
-// calcAcrossJointVelocityJacobianDot (_Universal.h:173-189; zero otherwise),
+// calcAcrossJointVelocityJacobianDot (_Universal.h:173-189, _Gimbal.h:180-202;
+// zero otherwise),
 // calcParentToChildVelocityJacobianInGroundDot (RigidBodyNodeSpec.cpp:82-129),
 // calcJointIndependentKinematicsVel (RigidBodyNode.cpp:98-174), calcQDot
 // (RigidBodyNodeSpec.h:129-134, _Ball.h:187-214, _Free.h:226-257).
@@ -495,6 +512,12 @@ template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* 
         if (t == UNIVERSAL) {
             const V3<T> y = mk3<T>(B.X_FM.R(0,1), B.X_FM.R(1,1), B.X_FM.R(2,1));
             B.HDot_FM[1] = sv(cross(B.V_FM.a, y), zero3<T>());
+        } else if (t == GIMBAL) {
+            // d/dt of H_FM with qdot = u (RigidBodyNodeSpec_Gimbal.h:180-202)
+            const T c0=B.cq[0], c1=B.cq[1], s0=B.sq[0], s1=B.sq[1];
+            const T dc0 = -s0*ub[0], dc1 = -s1*ub[1], ds0 = c0*ub[0], ds1 = c1*ub[1];
+            B.HDot_FM[1] = sv(mk3<T>(T(0), dc0, ds0), zero3<T>());
+            B.HDot_FM[2] = sv(mk3<T>(ds1, -ds0*c1 - s0*dc1, dc0*c1 + c0*dc1), zero3<T>());
         }
         const Xform<T> X_PF = castX<T>(M.X_PF[b]);
         const Xform<T> X_MB = castX<T>(M.X_MB[b]);

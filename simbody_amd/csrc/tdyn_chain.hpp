@@ -117,8 +117,10 @@ template <int TYPE, class K> struct Joint {
     int ty = TYPE, d = D;       // run-time type and dof of a GEN body
     // tc: cached trig / normalization values of this q (written by the first
     // sweep with CACHED = false): Pin, Planar, Cylinder, Screw (cos,sin of
-    // q0), Universal (c1,s1,c2,s2), Ball/Free 1/|quaternion| (computed in
-    // FP64).  pitch: the Screw pitch (GEN only).
+    // q0), Universal and Gimbal (cos,sin of q0 then of q1; a Gimbal's third
+    // angle is recomputed, which keeps the cache at four entries),
+    // Ball/Free 1/|quaternion| (computed in FP64).  pitch: the Screw pitch
+    // (GEN only).
     template <bool CACHED>
     __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0)) {
         const K z = K(0), o = K(1);
@@ -126,17 +128,57 @@ template <int TYPE, class K> struct Joint {
         if constexpr (TYPE == GEN) {
             ty = rty; d = mob_dof(rty);
             H_FM[0] = H_FM[1] = H_FM[2] = svzero<K>();
-            if (rty == TRANSLATION) {
+            // Joint angles: q0 about z (Pin, Planar, Cylinder, Screw), body-fixed
+            // x, y (Universal) and x, y, z (Gimbal).  Their sin/cos come from one
+            // sincos in a loop shared by every type; when CACHED the first two
+            // come from the trig cache.
+            const int na = rty == UNIVERSAL ? 2 : rty == GIMBAL ? 3
+                         : (rty == SLIDER || rty == TRANSLATION || rty == BALL) ? 0 : 1;
+            K c0 = o, s0 = z, c1 = o, s1 = z, c2 = o, s2 = z;
+            int k = 0;
+            if constexpr (CACHED) { c0 = tc[0]; s0 = tc[1]; c1 = tc[2]; s1 = tc[3]; k = 2; }
+            double qa = CACHED ? qv[2] : qv[0], qb = qv[1];      // angle k, angle k+1
+#pragma unroll 1
+            for (; k < na; ++k) {
+                K sn, cs;
+                sincosk<K>(qa, &sn, &cs);
+                if (k == 0) { c0 = cs; s0 = sn; } else if (k == 1) { c1 = cs; s1 = sn; } else { c2 = cs; s2 = sn; }
+                qa = qb; qb = qv[2];
+            }
+            if constexpr (!CACHED) { tc[0] = c0; tc[1] = s0; tc[2] = c1; tc[3] = s1; }
+            if (rty == SLIDER) {
+                R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
+                H_FM[0] = {zero3<K>(), V3(o, z, z)};
+            } else if (rty == TRANSLATION) {
                 // RigidBodyNodeSpec_Translation.h: X_FM = (I, q), H_FM = (0, e_k)
                 R_FM = ident<K>(); p_FM = V3(K(qv[0]), K(qv[1]), K(qv[2]));
                 H_FM[0] = {zero3<K>(), V3(o, z, z)};
                 H_FM[1] = {zero3<K>(), V3(z, o, z)};
                 H_FM[2] = {zero3<K>(), V3(z, z, o)};
-            } else if (rty == PLANAR || rty == CYLINDER || rty == SCREW) {
-                // _Planar.h:115-156, _Cylinder.h:100-139, _Screw.h:109-148
-                K sn, cs;
-                if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
-                R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
+            } else if (rty == BALL) {
+                double oo;
+                if constexpr (CACHED) oo = double(tc[0]);
+                else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = K(oo); }
+                R_FM = quat_rot(K(qv[0]*oo), K(qv[1]*oo), K(qv[2]*oo), K(qv[3]*oo));
+                H_FM[0] = {V3(o, z, z), zero3<K>()};
+                H_FM[1] = {V3(z, o, z), zero3<K>()};
+                H_FM[2] = {V3(z, z, o), zero3<K>()};
+            } else if (rty == UNIVERSAL || rty == GIMBAL) {
+                // Body-fixed 1-2-3 rotation (_Gimbal.h:141-176, Rotation.h:
+                // 342-349); a Universal's third angle has (sin, cos) = (0, 1),
+                // which gives its x-then-y rotation entry for entry.
+                const K s0s1 = s0*s1, s2c0 = s2*c0, c0c2 = c0*c2, nc1 = -c1;
+                R_FM.a[0]= c1*c2;             R_FM.a[1]= s2*nc1;            R_FM.a[2]= s1;
+                R_FM.a[3]= s2c0 + s0s1*c2;    R_FM.a[4]= c0c2 - s0s1*s2;    R_FM.a[5]= s0*nc1;
+                R_FM.a[6]= s0*s2 - s1*c0c2;   R_FM.a[7]= s0*c2 + s1*s2c0;   R_FM.a[8]= c0*c1;
+                H_FM[0] = {V3(o, z, z), zero3<K>()};
+                H_FM[1] = {V3(z, c0, s0), zero3<K>()};
+                if (rty == GIMBAL) H_FM[2] = {V3(s1, s0*nc1, c0*c1), zero3<K>()};
+            } else {
+                // Pin (_Pin.h), Planar (_Planar.h:115-156), Cylinder
+                // (_Cylinder.h:100-139), Screw (_Screw.h:109-148): rotation
+                // about z by q0, H_FM constant in F
+                R_FM.a[0]=c0; R_FM.a[1]=-s0; R_FM.a[2]=z; R_FM.a[3]=s0; R_FM.a[4]=c0; R_FM.a[5]=z;
                 R_FM.a[6]=z;  R_FM.a[7]=z;   R_FM.a[8]=o;
                 H_FM[0] = {V3(z, z, o), zero3<K>()};
                 if (rty == PLANAR) {
@@ -146,36 +188,10 @@ template <int TYPE, class K> struct Joint {
                 } else if (rty == CYLINDER) {
                     p_FM = V3(z, z, K(qv[1]));
                     H_FM[1] = {zero3<K>(), V3(z, z, o)};
-                } else {
+                } else if (rty == SCREW) {
                     p_FM = V3(z, z, K(qv[0])*pitch);
                     H_FM[0] = {V3(z, z, o), V3(z, z, pitch)};
                 }
-            } else if (rty == PIN) {
-                K sn, cs;
-                if constexpr (CACHED) { cs = tc[0]; sn = tc[1]; } else { sincosk<K>(qv[0], &sn, &cs); tc[0] = cs; tc[1] = sn; }
-                R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=z; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=z;
-                R_FM.a[6]=z;  R_FM.a[7]=z;   R_FM.a[8]=o;
-                H_FM[0] = {V3(z, z, o), zero3<K>()};
-            } else if (rty == SLIDER) {
-                R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
-                H_FM[0] = {zero3<K>(), V3(o, z, z)};
-            } else if (rty == UNIVERSAL) {
-                K s1, c1, s2, c2;
-                if constexpr (CACHED) { c1 = tc[0]; s1 = tc[1]; c2 = tc[2]; s2 = tc[3]; }
-                else { sincosk<K>(qv[0], &s1, &c1); sincosk<K>(qv[1], &s2, &c2); tc[0] = c1; tc[1] = s1; tc[2] = c2; tc[3] = s2; }
-                R_FM.a[0]= c2;    R_FM.a[1]= z;  R_FM.a[2]= s2;
-                R_FM.a[3]= s2*s1; R_FM.a[4]= c1; R_FM.a[5]= -s1*c2;
-                R_FM.a[6]=-s2*c1; R_FM.a[7]= s1; R_FM.a[8]= c1*c2;
-                H_FM[0] = {V3(o, z, z), zero3<K>()};
-                H_FM[1] = {col(R_FM, 1), zero3<K>()};
-            } else {
-                double oo;
-                if constexpr (CACHED) oo = double(tc[0]);
-                else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = K(oo); }
-                R_FM = quat_rot(K(qv[0]*oo), K(qv[1]*oo), K(qv[2]*oo), K(qv[3]*oo));
-                H_FM[0] = {V3(o, z, z), zero3<K>()};
-                H_FM[1] = {V3(z, o, z), zero3<K>()};
-                H_FM[2] = {V3(z, z, o), zero3<K>()};
             }
         } else if constexpr (TYPE == PIN) {
             K sn, cs;
@@ -244,7 +260,8 @@ __device__ __forceinline__ void hinge(const PA& P, const Joint<TYPE, K>& J, cons
 }
 
 // Mobilizer coriolis acceleration a = (HDot u) + (0, w_GP x (v_GB - v_GP))
-// (RigidBodyNodeSpec.cpp:82-129, RigidBodyNode.cpp:98-174).
+// (RigidBodyNodeSpec.cpp:82-129, RigidBodyNode.cpp:98-174).  HDot_FM is
+// nonzero for Universal and Gimbal only.
 template <int TYPE, class K>
 __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>& R_GF, const v3<K>& r_MB_F,
                                           const sv<K>* H, const double* uv, const sv<K>& V_GP, const sv<K>& V_GB) {
@@ -259,7 +276,16 @@ __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>&
     for (int j = 0; j < D; ++j) {
         v3<K> hdw = zero3<K>();
         if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(J.R_FM, 1)); }
-        else if constexpr (TYPE == GEN) { if (j == 1 && J.ty == UNIVERSAL) hdw = cross(w_FM, col(J.R_FM, 1)); }
+        else if constexpr (TYPE == GEN) {
+            // Universal and Gimbal (_Universal.h:173-189, _Gimbal.h:180-202):
+            // column j turns with the angles before it, d/dt H_FM(j) =
+            // (sum_{k<j} u_k H_FM(k).w) x H_FM(j).w; a Universal's padded
+            // third column is zero.
+            if (j >= 1 && (J.ty == UNIVERSAL || J.ty == GIMBAL)) {
+                const v3<K> wj = j == 1 ? K(uv[0])*J.H_FM[0].w : K(uv[0])*J.H_FM[0].w + K(uv[1])*J.H_FM[1].w;
+                hdw = cross(wj, J.H_FM[j].w);
+            }
+        }
         const v3<K> lin = (zero3<K>() - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
         const sv<K> HD = {mul(R_GF, hdw) + cross(V_GP.w, H[j].w), mul(R_GF, lin) + cross(V_GP.w, H[j].v)};
         VD = VD + K(uv[j])*HD;

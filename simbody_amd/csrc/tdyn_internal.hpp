@@ -19,14 +19,15 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
     return make_int4(p[0], p[1], p[2], p[3]);
 }
 
-enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8 };
+enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8,
+       GIMBAL = 9 };
 // Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
 enum { GEN = 15 };
 __host__ __device__ inline int mob_dof(int t) {
     switch (t) {
     case PIN: case SLIDER: case SCREW: return 1;
     case UNIVERSAL: case CYLINDER: return 2;
-    case BALL: case TRANSLATION: case PLANAR: return 3;
+    case BALL: case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     default: return 6;
     }
 }
@@ -34,14 +35,13 @@ __host__ __device__ inline int mob_nq(int t) {
     switch (t) {
     case PIN: case SLIDER: case SCREW: return 1;
     case UNIVERSAL: case CYLINDER: return 2;
-    case TRANSLATION: case PLANAR: return 3;
+    case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     case BALL: return 4;
     default: return 7;
     }
 }
-// Translation, Planar, Cylinder and Screw run only as padded generic bodies
-// (constant H_FM in F, so no HDot_FM term); the typed code paths cover
-// Pin, Slider, Universal, Ball and Free.
+// Translation, Planar, Cylinder, Screw and Gimbal run only as padded generic
+// bodies; the typed code paths cover Pin, Slider, Universal, Ball and Free.
 __host__ __device__ inline bool mob_generic_only(int t) { return t >= TRANSLATION; }
 
 // Per-body constants, PRM_STRIDE doubles per body (packed over all trees).

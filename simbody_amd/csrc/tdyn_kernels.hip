@@ -233,14 +233,21 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         qv[0] = q(qi);
         R_FM = ident<double>(); p_FM = V3(qv[0], 0.0, 0.0);
         H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
-    } else if (ty == UNIVERSAL) {
+    } else if (ty == UNIVERSAL || ty == GIMBAL) {
+        // Universal: body-fixed x then y; Gimbal: body-fixed x, y, z
+        // (_Gimbal.h:141-176, Rotation.h:342-349).  With the third angle's
+        // (sin, cos) = (0, 1) the 1-2-3 product is the Universal's rotation.
         qv[0] = q(qi); qv[1] = q(qi + 1);
-        double s1, c1, s2, c2; sincos(qv[0], &s1, &c1); sincos(qv[1], &s2, &c2);
-        R_FM.a[0]= c2;    R_FM.a[1]= 0.0; R_FM.a[2]= s2;
-        R_FM.a[3]= s2*s1; R_FM.a[4]= c1;  R_FM.a[5]= -s1*c2;
-        R_FM.a[6]=-s2*c1; R_FM.a[7]= s1;  R_FM.a[8]= c1*c2;
+        double s0, c0, s1, c1, s2 = 0.0, c2 = 1.0;
+        sincos(qv[0], &s0, &c0); sincos(qv[1], &s1, &c1);
+        if (ty == GIMBAL) { qv[2] = q(qi + 2); sincos(qv[2], &s2, &c2); }
+        const double s0s1 = s0*s1, s2c0 = s2*c0, c0c2 = c0*c2, nc1 = -c1;
+        R_FM.a[0]= c1*c2;             R_FM.a[1]= s2*nc1;            R_FM.a[2]= s1;
+        R_FM.a[3]= s2c0 + s0s1*c2;    R_FM.a[4]= c0c2 - s0s1*s2;    R_FM.a[5]= s0*nc1;
+        R_FM.a[6]= s0*s2 - s1*c0c2;   R_FM.a[7]= s0*c2 + s1*s2c0;   R_FM.a[8]= c0*c1;
         H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
-        H_FM[1] = {col(R_FM, 1), zero3<double>()};
+        H_FM[1] = {V3(0.0, c0, s0), zero3<double>()};
+        if (ty == GIMBAL) H_FM[2] = {V3(s1, s0*nc1, c0*c1), zero3<double>()};
     } else if (ty == TRANSLATION) {
         // RigidBodyNodeSpec_Translation.h: X_FM = (I, q), H_FM = (0, e_k)
         qv[0] = q(qi); qv[1] = q(qi + 1); qv[2] = q(qi + 2);
@@ -326,7 +333,10 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
 #pragma unroll
         for (int j = 0; j < D; ++j) {
             d3 hdw = zero3<double>();
-            if (j == 1 && ty == UNIVERSAL) hdw = cross(w_FM, col(R_FM, 1));
+            // Universal and Gimbal (_Universal.h:173-189, _Gimbal.h:180-202):
+            // d/dt H_FM(j) = (sum_{k<j} u_k H_FM(k).w) x H_FM(j).w
+            if (j >= 1 && (ty == UNIVERSAL || ty == GIMBAL))
+                hdw = cross(j == 1 ? uv[0]*H_FM[0].w : uv[0]*H_FM[0].w + uv[1]*H_FM[1].w, H_FM[j].w);
             const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
             const dv HD = {mul(R_GF, hdw) + cross(w_GF, H[j].w), mul(R_GF, lin) + cross(w_GF, H[j].v)};
             VD = VD + uv[j]*HD;

@@ -18,9 +18,10 @@ from dataclasses import dataclass, field
 import numpy as np
 
 PIN, SLIDER, UNIVERSAL, BALL, FREE = 0, 1, 2, 3, 4
-TRANSLATION, PLANAR, CYLINDER, SCREW = 5, 6, 7, 8
-DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1}
-NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1}
+TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL = 5, 6, 7, 8, 9
+DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1, GIMBAL: 3}
+NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1,
+      GIMBAL: 3}
 NAMES = {PIN: "Pin", SLIDER: "Slider", UNIVERSAL: "Universal", BALL: "Ball", FREE: "Free"}
 
 
@@ -327,9 +328,10 @@ def _state_plan(spec):
                 plan.append(("angle", qi, c)); qi += 1; c += 1
         elif t == SLIDER:
             plan.append(("trans", qi, c)); qi += 1; c += 1
-        elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW):
+        elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL):
             kinds = {TRANSLATION: ("trans",) * 3, PLANAR: ("angle", "trans", "trans"),
-                     CYLINDER: ("angle", "trans"), SCREW: ("angle",)}[t]
+                     CYLINDER: ("angle", "trans"), SCREW: ("angle",),
+                     GIMBAL: ("angle", "tilt", "angle")}[t]
             for k in kinds:
                 plan.append((k, qi, c)); qi += 1; c += 1
         else:
@@ -352,8 +354,8 @@ def _uniform_matrix(seed, indices, n):
 
 def random_states(spec, seed, indices, qscale=np.pi, uscale=1.0):
     """Initial states of instances `indices`, instance-major (n, nq + nu):
-    angles U(-qscale, qscale), slider/free translations U(-1,1), random unit
-    quaternions, u ~ N(0, uscale^2).  Row i depends only on (seed, indices[i])."""
+    angles U(-qscale, qscale), slider/free translations and Gimbal middle
+    angles U(-1,1), random unit quaternions, u ~ N(0, uscale^2).  Row i depends only on (seed, indices[i])."""
     plan, nc = _state_plan(spec)
     nq = spec.nq
     U = _uniform_matrix(seed, indices, nc)
@@ -368,6 +370,8 @@ def random_states(spec, seed, indices, qscale=np.pi, uscale=1.0):
         if kind == "angle":
             Y[:, off] = -qscale + 2 * qscale * U[:, c]
         elif kind == "trans":
+            Y[:, off] = -1.0 + 2.0 * U[:, c]
+        elif kind == "tilt":     # Gimbal's middle angle, > 0.5 rad from gimbal lock
             Y[:, off] = -1.0 + 2.0 * U[:, c]
         elif kind == "quat":
             qq = normal(c, 4)

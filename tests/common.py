@@ -30,11 +30,11 @@ def mixed_trees(seed, n, nmin=4, nmax=16, free_every=3):
 
 
 def ext_trees(seed, n, nmin=4, nmax=16):
-    """Random trees (general joint frames) whose non-root bodies draw all nine
+    """Random trees (general joint frames) whose non-root bodies draw all ten
     mobilizers but Free uniformly -- Pin, Slider, Universal, Ball, Translation,
-    Planar, Cylinder, Screw -- with Screw pitches in [-0.5, 0.5]; every third
-    root is Free."""
-    types = [M.PIN, M.SLIDER, M.UNIVERSAL, M.BALL, M.TRANSLATION, M.PLANAR, M.CYLINDER, M.SCREW]
+    Planar, Cylinder, Screw, Gimbal -- with Screw pitches in [-0.5, 0.5];
+    every third root is Free."""
+    types = [M.PIN, M.SLIDER, M.UNIVERSAL, M.BALL, M.TRANSLATION, M.PLANAR, M.CYLINDER, M.SCREW, M.GIMBAL]
     trees = []
     for i in range(n):
         t = M.random_tree(seed, i, nmin=nmin, nmax=nmax, general_frames=True)

@@ -1,4 +1,4 @@
-"""GPU parity for trees with Translation, Planar, Cylinder and Screw
+"""GPU parity for trees with Translation, Planar, Cylinder, Screw and Gimbal
 mobilizers mixed with the other five (FP64, 1e-12 relative per instance
 against the oracle, through the C ABI): every single-shot operator on
 batches of different trees (padded generic records) and on one shared tree
@@ -116,7 +116,7 @@ def test_rollouts_through_every_tree_evaluator():
 def test_chain_with_new_mobilizers_uses_tree_evaluator():
     from simbody_amd.matter import MatterBatch, RungeKuttaMerson, TdynError
     spec = M.spatial_chain(12, seed=4)
-    spec.mobilizer[:] = [M.SCREW, M.PLANAR, M.CYLINDER, M.TRANSLATION, M.PIN, M.BALL] * 2
+    spec.mobilizer[:] = [M.SCREW, M.PLANAR, M.CYLINDER, M.TRANSLATION, M.GIMBAL, M.BALL] * 2
     spec.pitch = np.linspace(-0.3, 0.3, 12)
     n = 64
     B = MatterBatch([spec], n_instances=n)

@@ -1,7 +1,11 @@
-"""Pinning the oracle's Translation, Planar, Cylinder and Screw mobilizers
-(CPU): closed forms of single bodies on Ground, the mass-matrix invariants of
+"""Pinning the oracle's Translation, Planar, Cylinder, Screw and Gimbal
+mobilizers (CPU): closed forms of single bodies on Ground, the Gimbal's
+rotation against a product of axis rotations and its dynamics against the
+same tree with a Ball in its place, the mass-matrix invariants of
 Simbody/tests/TestMassMatrix.cpp:658-887 and finite differences of the
-kinematics on trees that mix all nine mobilizers, and energy conservation."""
+kinematics on trees that mix all ten mobilizers, and energy conservation."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -128,3 +132,112 @@ def test_energy_conservation_all_mobilizers():
     K1 = o.kinematics(y[:o.nq], y[o.nq:])
     E0, E1 = K0["ke"] + K0["pe"], K1["ke"] + K1["pe"]
     assert abs(E1 - E0) < 1e-8 * max(abs(E0), K0["ke"])
+
+
+def _rx(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+
+
+def _ry(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+
+
+def _rz(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+
+
+def _quat(R):
+    """Unit quaternion (w, x, y, z) of a rotation matrix (Shepperd's method)."""
+    t = np.trace(R)
+    k = int(np.argmax([t, R[0, 0], R[1, 1], R[2, 2]]))
+    if k == 0:
+        w = 0.5 * np.sqrt(1 + t); f = 0.25 / w
+        q = [w, (R[2, 1] - R[1, 2]) * f, (R[0, 2] - R[2, 0]) * f, (R[1, 0] - R[0, 1]) * f]
+    else:
+        i, j, l = k - 1, k % 3, (k + 1) % 3
+        x = 0.5 * np.sqrt(1 + R[i, i] - R[j, j] - R[l, l]); f = 0.25 / x
+        q = [0.0] * 4
+        q[0] = (R[l, j] - R[j, l]) * f
+        q[1 + i] = x; q[1 + j] = (R[j, i] + R[i, j]) * f; q[1 + l] = (R[l, i] + R[i, l]) * f
+    q = np.array(q)
+    return q / np.linalg.norm(q)
+
+
+def _gimbal_h(q):
+    c0, s0, c1, s1 = np.cos(q[0]), np.sin(q[0]), np.cos(q[1]), np.sin(q[1])
+    return np.array([[1, 0, s1], [0, c0, -s0 * c1], [0, s0, c0 * c1]])
+
+
+def test_gimbal_rotation_is_body_fixed_xyz():
+    """X_GB of a Gimbal body on Ground (identity frames) is Rx(q0) Ry(q1)
+    Rz(q2), and its angular velocity in Ground is R_x qd0 ex + R_x R_y qd1 ey
+    + R qd2 ez -- the body-fixed 1-2-3 sequence from its definition."""
+    o = OracleModel(_single(M.GIMBAL))
+    q, qd = np.array([0.7, -0.9, 2.1]), np.array([0.4, -1.3, 0.8])
+    K = o.kinematics(q, qd)
+    Rx, Ry, Rz = _rx(q[0]), _ry(q[1]), _rz(q[2])
+    R = Rx @ Ry @ Rz
+    assert np.abs(K["X_GB"][1, :9] - R.ravel()).max() < 1e-15
+    w = Rx[:, 0] * qd[0] + (Rx @ Ry)[:, 1] * qd[1] + R[:, 2] * qd[2]
+    assert np.abs(K["V_GB"][1, :3] - w).max() < 1e-15
+    assert np.array_equal(K["qdot"], qd)                     # u = qdot
+
+
+def test_gimbal_isotropic_body_keeps_its_angular_velocity():
+    """A torque-free body with isotropic inertia about its COM at Bo keeps
+    w_GB constant, whatever the angles: the Euler-angle accelerations that
+    the Gimbal's HDot_FM term gives must cancel exactly."""
+    o = OracleModel(_single(M.GIMBAL, inertia=(0.4, 0.4, 0.4), gravity=(0.0, 0.0, 0.0)))
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        q = rng.uniform(-1.0, 1.0, 3); u = rng.normal(size=3)
+        ud, _, A, _ = o.forward(q, u, np.zeros(3), np.zeros((2, 6)))
+        assert np.abs(A[1]).max() < 1e-14
+        assert np.abs(ud).max() > 1e-3                      # the angle rates do change
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_gimbal_matches_ball_on_the_same_tree(idx):
+    """A tree with Gimbal bodies and the same tree with Ball mobilizers in
+    their place, at the same configuration (quaternion of the Gimbal's
+    rotation) and the same motion (w_FM = H_FM qdot as the Ball's u): every
+    body's X_GB, V_GB and A_GB agree, and so does the Gimbal's angular
+    acceleration w_FM' = H_FM qdd + HDot_FM qd with the Ball's udot."""
+    spec = ext_trees(71, 4, nmin=6, nmax=12)[idx]
+    spec.mobilizer[1:] = np.where(np.arange(1, spec.nb) % 2 == 1, M.GIMBAL, spec.mobilizer[1:])
+    spec.mobilizer[0] = M.GIMBAL if spec.mobilizer[0] != M.FREE else M.FREE
+    ball = copy.deepcopy(spec)
+    ball.mobilizer = np.where(spec.mobilizer == M.GIMBAL, M.BALL, spec.mobilizer).astype(np.int32)
+    q, u = M.random_state(spec, 71, idx)
+    qb, ub = [], []
+    qo = uo = 0
+    gimbals = []
+    for t in spec.mobilizer:
+        t = int(t)
+        if t == M.GIMBAL:
+            a = q[qo:qo + 3]
+            qb.append(_quat(_rx(a[0]) @ _ry(a[1]) @ _rz(a[2])))
+            ub.append(_gimbal_h(a) @ u[uo:uo + 3])
+            gimbals.append((qo, uo, len(np.concatenate(ub)) - 3 if ub else 0))
+        else:
+            qb.append(q[qo:qo + M.NQ[t]]); ub.append(u[uo:uo + M.DOF[t]])
+        qo += M.NQ[t]; uo += M.DOF[t]
+    qb, ub = np.concatenate(qb), np.concatenate(ub)
+    og, ob = OracleModel(spec), OracleModel(ball)
+    Kg, Kb = og.kinematics(q, u), ob.kinematics(qb, ub)
+    assert np.abs(Kg["X_GB"] - Kb["X_GB"]).max() < 1e-12
+    assert np.abs(Kg["V_GB"] - Kb["V_GB"]).max() < 1e-12 * max(1, np.abs(Kb["V_GB"]).max())
+    F = M.random_body_forces(spec, 72, idx)
+    udg, _, Ag, _ = og.forward(q, u, np.zeros(og.nu), og.system_forces(q) + F)
+    udb, _, Ab, _ = ob.forward(qb, ub, np.zeros(ob.nu), ob.system_forces(qb) + F)
+    assert np.abs(Ag - Ab).max() < 1e-10 * max(1, np.abs(Ab).max())
+    for qo, uo, ubo in gimbals:
+        a, ad, add = q[qo:qo + 3], u[uo:uo + 3], udg[uo:uo + 3]
+        c0, s0, c1, s1 = np.cos(a[0]), np.sin(a[0]), np.cos(a[1]), np.sin(a[1])
+        Hd = np.array([[0, 0, c1 * ad[1]],
+                       [0, -s0 * ad[0], -c0 * ad[0] * c1 + s0 * s1 * ad[1]],
+                       [0, c0 * ad[0], -s0 * ad[0] * c1 - c0 * s1 * ad[1]]])
+        assert np.abs(_gimbal_h(a) @ add + Hd @ ad - udb[ubo:ubo + 3]).max() < 1e-10 * max(1, np.abs(udb).max())
