@@ -73,6 +73,8 @@ typedef struct tdyn_tree_desc {
     double gravity[3];         /* Force::Gravity direction*magnitude (Force_Gravity.cpp:210-229) */
     const double* screw_pitch; /* [nb] MobilizedBody::Screw pitch (length per radian), read for Screw
                                   bodies only; NULL: no Screw bodies */
+    const int32_t* reversed;   /* [nb] nonzero: the mobilizer is reversed (MobilizedBody::Reverse,
+                                  RigidBodyNodeSpec.h:262-277); not for Free; NULL: none reversed */
 } tdyn_tree_desc;
 
 typedef struct tdyn_batch tdyn_batch;

@@ -21,7 +21,8 @@ class TreeDesc(ctypes.Structure):
     """Same layout as tdyn_tree_desc in include/tdyn.h."""
     _fields_ = [("nb", ctypes.c_int32), ("parent", _i), ("mobilizer", _i),
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
-                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("pitch", _d)]
+                ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("pitch", _d),
+                ("reversed", _i)]
 
 
 _lib = None
@@ -84,6 +85,10 @@ class OracleModel:
         if pitch is not None:
             self._pitch = np.ascontiguousarray(pitch, np.float64)
             d.pitch = _p(self._pitch)
+        rev = getattr(spec, "reversed", None)
+        if rev is not None:
+            self._rev = np.ascontiguousarray(rev, np.int32)
+            d.reversed = _pi(self._rev)
         self._desc = d
         h = lib().oracle_model_new(ctypes.byref(d))
         if not h:

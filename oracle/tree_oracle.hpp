@@ -29,6 +29,9 @@
 //   * Energy conservation (TestMobilizedBody.cpp:615-662 style) and momentum
 //     conservation of free-floating trees pin the velocity-dependent terms and
 //     the 6x6 Free-joint D inverse.
+//   * Reversed mobilizers: the same physical system built forward and with
+//     the mobilizer reversed (Simbody/tests/TestReverseMobilizers.cpp
+//     style) moves the same way.
 //   * RK-Merson: the linear-test-equation stability polynomial and the
 //     embedded error estimate (RungeKuttaMersonIntegrator.cpp:60-80 tableau).
 //   Parity unpinned (no reference-held vector, reference not buildable here):
@@ -202,6 +205,7 @@ struct TreeDesc {
     const double*  X_BM;        // [nb][12]
     double gravity[3];
     const double*  pitch;       // [nb] Screw pitch (length per radian); nullable
+    const int32_t* reversed;    // [nb] nonzero: reversed mobilizer; nullable
 };
 
 struct Model {
@@ -215,6 +219,7 @@ struct Model {
     std::vector<Xform<double>> X_PF, X_MB;
     std::vector<V3<double>> p_BM;
     std::vector<double> pitch;
+    std::vector<int> reversed;
     V3<double> g;
 
     // Build from the plain description.  Body 0 is Ground.
@@ -229,7 +234,7 @@ struct Model {
         children.assign(n, {}); mass.assign(n, 0.0);
         com.assign(n, zero3<double>()); Gunit.assign(n, zero33<double>());
         X_PF.assign(n, Xform<double>{ident33<double>(), zero3<double>()});
-        X_MB = X_PF; p_BM.assign(n, zero3<double>()); pitch.assign(n, 0.0);
+        X_MB = X_PF; p_BM.assign(n, zero3<double>()); pitch.assign(n, 0.0); reversed.assign(n, 0);
         nq = nu = nquat = 0;
         for (int b = 1; b <= nb; ++b) {
             const int i = b-1;
@@ -259,6 +264,8 @@ struct Model {
             X_BM.p = mk3(d.X_BM[12*i+9], d.X_BM[12*i+10], d.X_BM[12*i+11]);
             p_BM[b] = X_BM.p;
             pitch[b] = d.pitch ? d.pitch[i] : 0.0;
+            reversed[b] = d.reversed ? (d.reversed[i] != 0) : 0;
+            if (reversed[b] && t == FREE) return false;
             X_MB[b] = invert(X_BM);      // RigidBodyNode.h:1012 X_MB(~X_BM)
         }
         int maxLevel = 0;
@@ -282,6 +289,7 @@ template <class T> struct Body {
     Xform<T> X_FM, X_PB, X_GB;
     V3<T> phi;                 // p_PB_G
     SV<T> H_FM[6], H[6];       // columns, H = H_PB_G
+    SV<T> H_MF[6];             // reversed mobilizer: the mobilizer's own H
     SpatialInertia<T> Mk;      // about Bo, in G
     V3<T> com_G;
     // velocity
@@ -456,6 +464,20 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
             }
         } break;
         }
+        if (M.reversed[b]) {
+            // Reversed mobilizer (RigidBodyNodeSpec.h:262-277): the mobilizer's
+            // own transform and H are X_MF and H_MF; X_FM = ~X_MF and
+            // calcReverseMobilizerH_FM (RigidBodyNodeSpec.cpp:131-166):
+            //   H_FM_w = -R_FM H_MF_w,  H_FM_v = ~[p_FM]x H_FM_w - R_FM H_MF_v
+            B.X_FM = invert(B.X_FM);
+            const M33<T>& R = B.X_FM.R;
+            const M33<T> np = transpose(crossMat(B.X_FM.p));
+            for (int j = 0; j < nd; ++j) {
+                B.H_MF[j] = B.H_FM[j];
+                const V3<T> w = -(R*B.H_MF[j].a);
+                B.H_FM[j] = sv(w, np*w - R*B.H_MF[j].b);
+            }
+        }
         // X_PB = X_PF X_FM X_MB ; X_GB = X_GP X_PB
         const Xform<T> X_PF = castX<T>(M.X_PF[b]);
         const Xform<T> X_MB = castX<T>(M.X_MB[b]);
@@ -508,16 +530,36 @@ template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* 
             B.V_FM.a = B.V_FM.a + ub[j]*B.H_FM[j].a; B.V_FM.b = B.V_FM.b + ub[j]*B.H_FM[j].b;
             B.V_PB_G.a = B.V_PB_G.a + ub[j]*B.H[j].a; B.V_PB_G.b = B.V_PB_G.b + ub[j]*B.H[j].b;
         }
+        const bool rev = M.reversed[b] != 0;
         for (int j=0;j<nd;++j) B.HDot_FM[j] = svzero<T>();
         if (t == UNIVERSAL) {
-            const V3<T> y = mk3<T>(B.X_FM.R(0,1), B.X_FM.R(1,1), B.X_FM.R(2,1));
-            B.HDot_FM[1] = sv(cross(B.V_FM.a, y), zero3<T>());
+            // the mobilizer's own frames and velocity (findX_F0M0, find_w_F0M0;
+            // RigidBodyNode.h:738-848): ~X_FM and H_MF u when reversed
+            const M33<T> R0 = rev ? transpose(B.X_FM.R) : B.X_FM.R;
+            V3<T> w0 = B.V_FM.a;
+            if (rev) { w0 = zero3<T>(); for (int j=0;j<nd;++j) w0 = w0 + ub[j]*B.H_MF[j].a; }
+            const V3<T> y = mk3<T>(R0(0,1), R0(1,1), R0(2,1));
+            B.HDot_FM[1] = sv(cross(w0, y), zero3<T>());
         } else if (t == GIMBAL) {
             // d/dt of H_FM with qdot = u (RigidBodyNodeSpec_Gimbal.h:180-202)
             const T c0=B.cq[0], c1=B.cq[1], s0=B.sq[0], s1=B.sq[1];
             const T dc0 = -s0*ub[0], dc1 = -s1*ub[1], ds0 = c0*ub[0], ds1 = c1*ub[1];
             B.HDot_FM[1] = sv(mk3<T>(T(0), dc0, ds0), zero3<T>());
             B.HDot_FM[2] = sv(mk3<T>(ds1, -ds0*c1 - s0*dc1, dc0*c1 + c0*dc1), zero3<T>());
+        }
+        if (rev) {
+            // calcReverseMobilizerHDot_FM (RigidBodyNodeSpec.cpp:168-216):
+            //   HDot_FM = [w_FM]x H_FM - R_FM HDot_MF
+            //   HDot_FM_v -= [p_FM]x HDot_FM_w + ([v_FM]x - [w_FM]x [p_FM]x) H_FM_w
+            const M33<T>& R = B.X_FM.R;
+            const M33<T> px = crossMat(B.X_FM.p), wx = crossMat(B.V_FM.a), vx = crossMat(B.V_FM.b);
+            const M33<T> vwp = vx - wx*px;
+            for (int j=0;j<nd;++j) {
+                const SV<T> HD_MF = B.HDot_FM[j];
+                SV<T> hd = sv(wx*B.H_FM[j].a - R*HD_MF.a, wx*B.H_FM[j].b - R*HD_MF.b);
+                hd.b = hd.b - (px*hd.a + vwp*B.H_FM[j].a);
+                B.HDot_FM[j] = hd;
+            }
         }
         const Xform<T> X_PF = castX<T>(M.X_PF[b]);
         const Xform<T> X_MB = castX<T>(M.X_MB[b]);

@@ -38,6 +38,7 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
             snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
         }
         if (ty == TDYN_FREE && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Free mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
+        if (ty == TDYN_FREE && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed Free mobilizer is not supported in this build", t, b); return fail(TDYN_E_ARG, buf); }
         if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
         if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
             !finite_all(d.X_PF + 12*(b-1), 12) || !finite_all(d.X_BM + 12*(b-1), 12)) {
@@ -75,6 +76,13 @@ void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
     for (int k = 0; k < 6; ++k) P[PRM_G + k] = d.inertia_B[6*(b-1) + k] / m;
     for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
     if (d.mobilizer[b-1] == TDYN_SCREW) P[PRM_PITCH] = d.screw_pitch[b-1];
+    if (d.reversed && d.reversed[b-1]) P[PRM_REV] = 1.0;
+}
+
+bool any_reversed(const tdyn_tree_desc& d) {
+    if (!d.reversed) return false;
+    for (int b = 0; b < d.nb; ++b) if (d.reversed[b]) return true;
+    return false;
 }
 
 int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
@@ -183,7 +191,10 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
 
     auto* B = new tdyn_batch;
     B->n_inst = n_instances; B->n_trees = n_trees;
-    B->uniform = (tree_of_instance == nullptr) || n_trees == 1;
+    // reversed mobilizers run as padded generic bodies, as in batches of
+    // different trees
+    for (int t = 0; t < n_trees; ++t) B->any_rev = B->any_rev || any_reversed(trees[t]);
+    B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev;
     std::vector<int4> body_b, pre, bfs, bfs_b; std::vector<int> child_idx, lvl; std::vector<double> prm, grav, prm_bfs;
     B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);

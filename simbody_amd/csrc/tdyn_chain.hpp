@@ -83,9 +83,19 @@ template <class PA> struct IdF {
     __device__ __forceinline__ double operator[](int k) const { return p[k]; }
     __device__ __forceinline__ IdF operator+(size_t k) const { return IdF(p + k); }
 };
-template <class PA> struct PrmTraits { static constexpr bool lane = false, idf = false; };
-template <> struct PrmTraits<PrmL> { static constexpr bool lane = true, idf = false; };
-template <class T> struct PrmTraits<IdF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = true; };
+// A model that may hold reversed mobilizers (MobilizedBody::Reverse): each
+// generic body's PRM_REV entry says whether its X_FM and H_FM are reversed.
+template <class PA> struct RevF {
+    PA p;
+    __device__ __forceinline__ RevF() = default;
+    __device__ __forceinline__ explicit RevF(PA q) : p(q) {}
+    __device__ __forceinline__ double operator[](int k) const { return p[k]; }
+    __device__ __forceinline__ RevF operator+(size_t k) const { return RevF(p + k); }
+};
+template <class PA> struct PrmTraits { static constexpr bool lane = false, idf = false, rev = false; };
+template <> struct PrmTraits<PrmL> { static constexpr bool lane = true, idf = false, rev = false; };
+template <class T> struct PrmTraits<IdF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = true, rev = false; };
+template <class T> struct PrmTraits<RevF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = false, rev = true; };
 
 template <class K, class PA> __device__ __forceinline__ m33<K> ldm(const PA& p) {
     m33<K> r; for (int i = 0; i < 9; ++i) r.a[i] = K(p[i]); return r;
@@ -115,6 +125,28 @@ template <int TYPE, class K> struct Joint {
     static constexpr int D = MobC<TYPE>::D;
     m33<K> R_FM; v3<K> p_FM; sv<K> H_FM[D];
     int ty = TYPE, d = D;       // run-time type and dof of a GEN body
+    bool rv = false;            // reversed (GEN only); Hm: the mobilizer's own angular H columns
+    v3<K> Hm[3];
+    // Reversed mobilizer (RigidBodyNodeSpec.h:262-277): what set() computed is
+    // the mobilizer's X_MF and H_MF; X_FM = ~X_MF and, by
+    // calcReverseMobilizerH_FM (RigidBodyNodeSpec.cpp:131-166),
+    //   H_FM_w = -R_FM H_MF_w,  H_FM_v = H_FM_w x p_FM - R_FM H_MF_v.
+    __device__ __forceinline__ void reverse() {
+        static_assert(TYPE == GEN, "reversal is carried by generic bodies");
+        rv = true;
+        const m33<K> R_MF = R_FM;
+        p_FM = -mulT(R_MF, p_FM);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R_FM.a[3*r + c] = R_MF.a[3*c + r];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Hm[j] = H_FM[j].w;
+            const v3<K> w = -mulT(R_MF, Hm[j]);
+            H_FM[j] = {w, cross(w, p_FM) - mulT(R_MF, H_FM[j].v)};
+        }
+    }
     // tc: cached trig / normalization values of this q (written by the first
     // sweep with CACHED = false): Pin, Planar, Cylinder, Screw (cos,sin of
     // q0), Universal and Gimbal (cos,sin of q0 then of q1; a Gimbal's third
@@ -262,7 +294,14 @@ __device__ __forceinline__ void hinge(const PA& P, const Joint<TYPE, K>& J, cons
 // Mobilizer coriolis acceleration a = (HDot u) + (0, w_GP x (v_GB - v_GP))
 // (RigidBodyNodeSpec.cpp:82-129, RigidBodyNode.cpp:98-174).  HDot_FM is
 // nonzero for Universal and Gimbal only.
-template <int TYPE, class K>
+// REV: the body may be reversed (J.rv); its HDot_FM then follows
+// calcReverseMobilizerHDot_FM (RigidBodyNodeSpec.cpp:168-216):
+//   HDot_FM_w = w_FM x H_FM_w - R_FM HDot_MF_w
+//   HDot_FM_v = w_FM x H_FM_v - R_FM HDot_MF_v
+//               - (p_FM x HDot_FM_w + v_FM x H_FM_w - w_FM x (p_FM x H_FM_w))
+// with HDot_MF from the mobilizer's own H columns (HDot_MF_v is zero for
+// every mobilizer carried here).
+template <int TYPE, class K, bool REV = false>
 __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>& R_GF, const v3<K>& r_MB_F,
                                           const sv<K>* H, const double* uv, const sv<K>& V_GP, const sv<K>& V_GB) {
     constexpr int D = MobC<TYPE>::D;
@@ -276,17 +315,28 @@ __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>&
     for (int j = 0; j < D; ++j) {
         v3<K> hdw = zero3<K>();
         if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(J.R_FM, 1)); }
-        else if constexpr (TYPE == GEN) {
+        v3<K> hdv = zero3<K>();
+        if constexpr (TYPE == GEN) {
             // Universal and Gimbal (_Universal.h:173-189, _Gimbal.h:180-202):
             // column j turns with the angles before it, d/dt H_FM(j) =
             // (sum_{k<j} u_k H_FM(k).w) x H_FM(j).w; a Universal's padded
-            // third column is zero.
+            // third column is zero.  For a reversed body the same formula in
+            // the mobilizer's own columns gives HDot_MF.
+            v3<K> h0 = J.H_FM[0].w, h1 = J.H_FM[1].w, hj = J.H_FM[j].w;
+            if constexpr (REV) { if (J.rv) { h0 = J.Hm[0]; h1 = J.Hm[1]; hj = J.Hm[j]; } }
             if (j >= 1 && (J.ty == UNIVERSAL || J.ty == GIMBAL)) {
-                const v3<K> wj = j == 1 ? K(uv[0])*J.H_FM[0].w : K(uv[0])*J.H_FM[0].w + K(uv[1])*J.H_FM[1].w;
-                hdw = cross(wj, J.H_FM[j].w);
+                const v3<K> wj = j == 1 ? K(uv[0])*h0 : K(uv[0])*h0 + K(uv[1])*h1;
+                hdw = cross(wj, hj);
+            }
+            if constexpr (REV) {
+                if (J.rv) {
+                    const v3<K> Hw = J.H_FM[j].w, Hv = J.H_FM[j].v;
+                    hdw = cross(w_FM, Hw) - mul(J.R_FM, hdw);
+                    hdv = cross(w_FM, Hv) - ((cross(J.p_FM, hdw) + cross(V_FM.v, Hw)) - cross(w_FM, cross(J.p_FM, Hw)));
+                }
             }
         }
-        const v3<K> lin = (zero3<K>() - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
+        const v3<K> lin = (hdv - cross(r_MB_F, hdw)) - cross(wxr, J.H_FM[j].w);
         const sv<K> HD = {mul(R_GF, hdw) + cross(V_GP.w, H[j].w), mul(R_GF, lin) + cross(V_GP.w, H[j].v)};
         VD = VD + K(uv[j])*HD;
     }
@@ -393,6 +443,13 @@ __device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 
     }
 }
 
+// Reverses a generic body's joint when the model may hold reversed
+// mobilizers and this body's is.
+template <int TYPE, class K, class PA>
+__device__ __forceinline__ void reverse_if(const PA& P, Joint<TYPE, K>& J) {
+    if constexpr (TYPE == GEN && PrmTraits<PA>::rev) { if (P[PRM_REV] != 0.0) J.reverse(); }
+}
+
 // The pitch of a Screw body (padded generic bodies only), else zero.
 template <int TYPE, class K, class PA>
 __device__ __forceinline__ K screw_pitch(const PA& P, const Pre<K>& x) {
@@ -430,6 +487,7 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
     Joint<TYPE, K> J;
     K tc[4] = {K(0), K(0), K(0), K(0)};
     J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    reverse_if<TYPE>(P, J);
 #pragma unroll
     for (int k = 0; k < TcN<TYPE>::n; ++k) st.T(k) = tc[k];
     m33<K> R_PB; v3<K> p_PB;
@@ -499,6 +557,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     const m33<K> R_GB = R; const v3<K> p_GB = p; const sv<K> V_GB = V;
@@ -513,7 +572,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const v3<K> w_GP = V_GB.w - Vpb.w;
     const v3<K> v_GP = (V_GB.v - Vpb.v) - cross(w_GP, phi);
     const sv<K> V_GP = {w_GP, v_GP};
-    const sv<K> a = coriolis<TYPE, K>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::rev>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
     const K m = K(P[PRM_MASS]);
     const v3<K> Mk_p = mul(R_GB, ld3<K>(P + PRM_COM));
     const s33<K> Gu = {K(P[PRM_G]), K(P[PRM_G+1]), K(P[PRM_G+2]), K(P[PRM_G+3]), K(P[PRM_G+4]), K(P[PRM_G+5])};
@@ -607,6 +666,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
     m33<K> R_GF; v3<K> r_MB_F; sv<K> H[D];
@@ -617,7 +677,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     for (int j = 0; j < D; ++j) Vpb = Vpb + K(uv[j])*H[j];
     const sv<K> V_GP = V;
     const sv<K> V_GB = phiT_mul(phi, V_GP) + Vpb;
-    const sv<K> a = coriolis<TYPE, K>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::rev>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
     const sv<K> Ap = phiT_mul(phi, A);
     double ud[D];
 #pragma unroll

@@ -40,6 +40,8 @@ template <class T> struct DBuf {
 void pack_body_params(const tdyn_tree_desc& d, int b, double* P);
 // True when every body's X_PF is the identity and X_BM has no rotation.
 bool identity_frames(const tdyn_tree_desc& d);
+// True when some body's mobilizer is reversed.
+bool any_reversed(const tdyn_tree_desc& d);
 // Validates one tree description (parents before children, known mobilizers,
 // Free only on Ground, positive finite masses, finite frames).
 int validate_tree(const tdyn_tree_desc& d, int t);
@@ -53,6 +55,7 @@ struct tdyn_batch {
     int64_t total_q = 0, total_u = 0, total_b = 0;
     int max_nq = 0, max_nu = 0, max_nb = 0, max_rec = 0;
     bool uniform = true;
+    bool any_rev = false;        // some tree has a reversed mobilizer (the batch runs padded generic bodies)
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
@@ -112,6 +115,7 @@ struct tdyn_batch {
         a.scratch32 = scratch32.p;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
+        a.rev = any_rev ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

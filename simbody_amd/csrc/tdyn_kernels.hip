@@ -287,6 +287,25 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
         H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
     }
+    // Reversed mobilizer (RigidBodyNodeSpec.h:262-277): the above is X_MF and
+    // H_MF; X_FM = ~X_MF and calcReverseMobilizerH_FM (RigidBodyNodeSpec.cpp:
+    // 131-166): H_FM_w = -R_FM H_MF_w, H_FM_v = H_FM_w x p_FM - R_FM H_MF_v.
+    // Hm keeps the mobilizer's own angular columns for HDot.
+    const bool rv = P[PRM_REV] != 0.0;
+    d3 Hm[D] = {H_FM[0].w, H_FM[1].w, H_FM[2].w};
+    if (rv) {
+        const dm R_MF = R_FM;
+        p_FM = zero3<double>() - mulT(R_MF, p_FM);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c2 = 0; c2 < 3; ++c2) R_FM.a[3*r + c2] = R_MF.a[3*c2 + r];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            const d3 w = zero3<double>() - mulT(R_MF, Hm[j]);
+            H_FM[j] = {w, cross(w, p_FM) - mulT(R_MF, H_FM[j].v)};
+        }
+    }
     const dm R_PF = prm_m(P + PRM_R_PF), R_MB = prm_m(P + PRM_R_MB);
     const d3 p_PF = prm_3(P + PRM_P_PF), p_MB = prm_3(P + PRM_P_MB);
     const dm R_FB = mul(R_FM, R_MB);
@@ -334,10 +353,18 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         for (int j = 0; j < D; ++j) {
             d3 hdw = zero3<double>();
             // Universal and Gimbal (_Universal.h:173-189, _Gimbal.h:180-202):
-            // d/dt H_FM(j) = (sum_{k<j} u_k H_FM(k).w) x H_FM(j).w
+            // d/dt H(j) = (sum_{k<j} u_k H(k).w) x H(j).w in the mobilizer's
+            // own columns (HDot_MF for a reversed body)
             if (j >= 1 && (ty == UNIVERSAL || ty == GIMBAL))
-                hdw = cross(j == 1 ? uv[0]*H_FM[0].w : uv[0]*H_FM[0].w + uv[1]*H_FM[1].w, H_FM[j].w);
-            const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
+                hdw = cross(j == 1 ? uv[0]*Hm[0] : uv[0]*Hm[0] + uv[1]*Hm[1], Hm[j]);
+            d3 hdv = zero3<double>();
+            if (rv) {
+                // calcReverseMobilizerHDot_FM (RigidBodyNodeSpec.cpp:168-216)
+                const d3 Hw = H_FM[j].w, Hv = H_FM[j].v;
+                hdw = cross(w_FM, Hw) - mul(R_FM, hdw);
+                hdv = cross(w_FM, Hv) - ((cross(p_FM, hdw) + cross(V_FM.v, Hw)) - cross(w_FM, cross(p_FM, Hw)));
+            }
+            const d3 lin = (hdv - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
             const dv HD = {mul(R_GF, hdw) + cross(w_GF, H[j].w), mul(R_GF, lin) + cross(w_GF, H[j].v)};
             VD = VD + uv[j]*HD;
         }
@@ -1038,7 +1065,10 @@ struct TreeStreamEval {
         const int o = pos(i);
         return i == 0 ? tstream::LaneStore{S, o + 18, o + 60, o + 54} : tstream::LaneStore{S, o + 18, o + 39, o + 36};
     }
-    __device__ __forceinline__ cdptr prm(int b) const { return cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE; }
+    // (every generic body checks its reversed-mobilizer flag)
+    __device__ __forceinline__ chain::RevF<cdptr> prm(int b) const {
+        return chain::RevF<cdptr>(cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE);
+    }
     __device__ __forceinline__ dv F(int b) const {
         if (!fapp) return svzero<double>();
         const double* f = fapp + 6*b;

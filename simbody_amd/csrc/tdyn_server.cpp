@@ -64,6 +64,8 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
         if (mob_generic_only(topology->mobilizer[b-1]))
             return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs Pin, Slider, Universal, Ball and Free bodies");
     }
+    if (any_reversed(*topology))
+        return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs forward mobilizers only");
     RkmCfg c;
     int prec = 0;
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }

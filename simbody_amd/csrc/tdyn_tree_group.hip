@@ -71,9 +71,12 @@ struct PrmS {
     __device__ __forceinline__ PrmS operator+(int k) const { return {p + k * tgroup::BW}; }
 };
 
-template <int G, bool IDF>
+template <int G, int IDF>
 struct GroupEval {
-    using PA = std::conditional_t<IDF, chain::IdF<PrmS>, PrmS>;
+    // IDF: 0 general joint frames, 1 identity frames at the joints, 2 general
+    // frames and reversed mobilizers
+    using PA = std::conditional_t<IDF == 1, chain::IdF<PrmS>,
+                                  std::conditional_t<IDF == 2, chain::RevF<PrmS>, PrmS>>;
     using VR = tstream::LaneVec;
     int nb, nq, nlev, l;
     const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
@@ -249,7 +252,7 @@ struct GroupEval {
 // loop of the other tree evaluators, RungeKuttaMersonIntegratorRep.cpp:
 // 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
 // the group.  l: the lane's index in the group.
-template <int G, bool IDF>
+template <int G, int IDF>
 __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) {
     const int tr = a.tree_of[inst];
     const GroupEval<G, IDF> ev(a, inst, tr, l);
@@ -395,7 +398,7 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) 
 // width of the instances' tree levels, so that the largest trees, which
 // bound a rollout's duration, get the most lanes); the wavefronts of the
 // widest trees come first in the grid.
-template <int G, bool IDF, bool F0>
+template <int G, int IDF, bool F0>
 __device__ __forceinline__ void plan_wave(const RkmArgs& a, int start, int count) {
     const int grp = threadIdx.x / G, l = threadIdx.x & (G - 1);
     if (grp >= count) return;
@@ -408,7 +411,7 @@ __device__ __forceinline__ void plan_wave(const RkmArgs& a, int start, int count
         grp_instance<G, IDF>(a, inst, l);
     }
 }
-template <bool IDF, bool F0>
+template <int IDF, bool F0>
 __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
     const int4 w = a.plan[blockIdx.x];
     switch (w.x) {
@@ -425,16 +428,20 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
 
 // a.idf: every tree's joint frames are identity rotations (Simbody's
 // noR_PF/noX_MB node forms), the frame products are skipped; the results
-// are bitwise those of the general form.
+// are bitwise those of the general form.  a.rev: some mobilizer is reversed;
+// only that build carries the reversal code.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);
-    if (a.idf) {
-        if (f0_only) k_rkm_plan<true, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_plan<true, false><<<grd, blk, 0, s>>>(a);
+    if (a.rev) {
+        if (f0_only) k_rkm_plan<2, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_plan<2, false><<<grd, blk, 0, s>>>(a);
+    } else if (a.idf) {
+        if (f0_only) k_rkm_plan<1, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_plan<1, false><<<grd, blk, 0, s>>>(a);
     } else {
-        if (f0_only) k_rkm_plan<false, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_plan<false, false><<<grd, blk, 0, s>>>(a);
+        if (f0_only) k_rkm_plan<0, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm_plan<0, false><<<grd, blk, 0, s>>>(a);
     }
     return hipGetLastError();
 }

@@ -36,6 +36,7 @@ class TreeSpec:
     X_BM: np.ndarray        # [nb,12]
     gravity: np.ndarray = field(default_factory=lambda: np.array([0.0, -9.8, 0.0]))
     pitch: np.ndarray = None  # [nb] Screw pitch (length per radian), None = zeros
+    reversed: np.ndarray = None  # [nb] int32, nonzero = MobilizedBody::Reverse; None = all forward
 
     @property
     def nb(self):

@@ -45,3 +45,29 @@ def ext_trees(seed, n, nmin=4, nmax=16):
         t.pitch = rng.uniform(-0.5, 0.5, size=t.nb)
         trees.append(t)
     return trees
+
+
+def rev_trees(seed, n, nmin=4, nmax=16, p_rev=0.5):
+    """ext_trees with each non-Free mobilizer reversed with probability p_rev."""
+    trees = ext_trees(seed, n, nmin, nmax)
+    for i, t in enumerate(trees):
+        rng = np.random.default_rng(seed * 104729 + i)
+        t.reversed = ((rng.uniform(size=t.nb) < p_rev) & (t.mobilizer != M.FREE)).astype(np.int32)
+    return trees
+
+
+def rot_to_quat(R):
+    """Unit quaternion (w, x, y, z) of a rotation matrix (Shepperd's method)."""
+    t = np.trace(R)
+    k = int(np.argmax([t, R[0, 0], R[1, 1], R[2, 2]]))
+    if k == 0:
+        w = 0.5 * np.sqrt(1 + t); f = 0.25 / w
+        q = [w, (R[2, 1] - R[1, 2]) * f, (R[0, 2] - R[2, 0]) * f, (R[1, 0] - R[0, 1]) * f]
+    else:
+        i, j, l = k - 1, k % 3, (k + 1) % 3
+        x = 0.5 * np.sqrt(1 + R[i, i] - R[j, j] - R[l, l]); f = 0.25 / x
+        q = [0.0] * 4
+        q[0] = (R[l, j] - R[j, l]) * f
+        q[1 + i] = x; q[1 + j] = (R[j, i] + R[i, j]) * f; q[1 + l] = (R[l, i] + R[i, l]) * f
+    q = np.array(q)
+    return q / np.linalg.norm(q)

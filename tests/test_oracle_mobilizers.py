@@ -9,7 +9,7 @@ import copy
 import numpy as np
 import pytest
 
-from common import ext_trees, rel_inf, simtk_eq
+from common import ext_trees, rel_inf, rot_to_quat, simtk_eq
 from simbody_amd import models as M
 from oracle_ctypes import OracleModel
 
@@ -149,23 +149,6 @@ def _rz(a):
     return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
 
 
-def _quat(R):
-    """Unit quaternion (w, x, y, z) of a rotation matrix (Shepperd's method)."""
-    t = np.trace(R)
-    k = int(np.argmax([t, R[0, 0], R[1, 1], R[2, 2]]))
-    if k == 0:
-        w = 0.5 * np.sqrt(1 + t); f = 0.25 / w
-        q = [w, (R[2, 1] - R[1, 2]) * f, (R[0, 2] - R[2, 0]) * f, (R[1, 0] - R[0, 1]) * f]
-    else:
-        i, j, l = k - 1, k % 3, (k + 1) % 3
-        x = 0.5 * np.sqrt(1 + R[i, i] - R[j, j] - R[l, l]); f = 0.25 / x
-        q = [0.0] * 4
-        q[0] = (R[l, j] - R[j, l]) * f
-        q[1 + i] = x; q[1 + j] = (R[j, i] + R[i, j]) * f; q[1 + l] = (R[l, i] + R[i, l]) * f
-    q = np.array(q)
-    return q / np.linalg.norm(q)
-
-
 def _gimbal_h(q):
     c0, s0, c1, s1 = np.cos(q[0]), np.sin(q[0]), np.cos(q[1]), np.sin(q[1])
     return np.array([[1, 0, s1], [0, c0, -s0 * c1], [0, s0, c0 * c1]])
@@ -219,7 +202,7 @@ def test_gimbal_matches_ball_on_the_same_tree(idx):
         t = int(t)
         if t == M.GIMBAL:
             a = q[qo:qo + 3]
-            qb.append(_quat(_rx(a[0]) @ _ry(a[1]) @ _rz(a[2])))
+            qb.append(rot_to_quat(_rx(a[0]) @ _ry(a[1]) @ _rz(a[2])))
             ub.append(_gimbal_h(a) @ u[uo:uo + 3])
             gimbals.append((qo, uo, len(np.concatenate(ub)) - 3 if ub else 0))
         else:
