@@ -183,13 +183,13 @@ template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
 // Model.
 // ---------------------------------------------------------------------------
 enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8,
-                 GIMBAL=9 };
+                 GIMBAL=9, WELD=10 };
 inline int mobDof(int t) {
-    static const int d[10] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3};
+    static const int d[11] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3, 0};
     return d[t];
 }
 inline int mobNq(int t) {
-    static const int n[10] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3};
+    static const int n[11] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3, 0};
     return n[t];
 }
 
@@ -241,7 +241,7 @@ struct Model {
             const int p = d.parent[i];
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
-            if (t < PIN || t > GIMBAL) return false;
+            if (t < PIN || t > WELD) return false;
             if (t == FREE && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
@@ -359,7 +359,8 @@ template <class T> inline void quatDotDot(const T q[4], const V3<T>& w, const V3
 // Position kinematics, base to tip.
 // RigidBodyNodeSpec.h:229-288 (realizePosition) with the mobilizer-specific
 // performQPrecalculations/calcX_FM/calcAcrossJointVelocityJacobian of
-// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free,Gimbal}.h, calcBodyTransforms
+// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free,Gimbal}.h, RigidBodyNode_Weld.cpp,
+// calcBodyTransforms
 // (RigidBodyNodeSpec.h:554-569), calcParentToChildVelocityJacobianInGround
 // (RigidBodyNodeSpec.cpp:44-74) and calcJointIndependentKinematicsPos
 // (RigidBodyNode.cpp:54-84).  Driver: SimbodyMatterSubsystemRep.cpp:1112-1144.
@@ -430,6 +431,11 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
                 B.H_FM[0] = sv(ez, mk3<T>(T(0), T(0), pt));
             }
         } break;
+        case WELD:
+            // no mobilities: X_FM = identity (RigidBodyNode_Weld.cpp:389-423);
+            // with no H column the ABI passes give P+ = P and z+ = z
+            B.X_FM.R = ident33<T>();
+            break;
         case GIMBAL: {
             // body-fixed 1-2-3 angles (RigidBodyNodeSpec_Gimbal.h:141-176,
             // Rotation.h:342-349); H_FM maps qdot to w_FM.

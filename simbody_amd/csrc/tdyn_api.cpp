@@ -33,7 +33,7 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     for (int b = 1; b <= d.nb; ++b) {
         const int p = d.parent[b-1], ty = d.mobilizer[b-1];
         if (p < 0 || p >= b) { snprintf(buf, sizeof buf, "tree %d body %d: parent %d must be in [0,%d)", t, b, p, b); return fail(TDYN_E_ARG, buf); }
-        if (ty < TDYN_PIN || ty > TDYN_GIMBAL) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty < TDYN_PIN || ty > TDYN_WELD) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
         if (ty == TDYN_SCREW && (!d.screw_pitch || !std::isfinite(d.screw_pitch[b-1]))) {
             snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
         }
@@ -316,7 +316,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     std::vector<int> inst_of_lane(n_instances);
     for (int i = 0; i < n_instances; ++i) inst_of_lane[i] = i;
     if (!B->uniform) {
-        static const double cost[10] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9};
+        static const double cost[11] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9, 0.6};
         std::vector<double> tw(n_trees, 0.0);
         for (int t = 0; t < n_trees; ++t)
             for (int bdy = 0; bdy < trees[t].nb; ++bdy) tw[t] += cost[trees[t].mobilizer[bdy]];

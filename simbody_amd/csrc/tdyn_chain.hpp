@@ -165,7 +165,7 @@ template <int TYPE, class K> struct Joint {
             // sincos in a loop shared by every type; when CACHED the first two
             // come from the trig cache.
             const int na = rty == UNIVERSAL ? 2 : rty == GIMBAL ? 3
-                         : (rty == SLIDER || rty == TRANSLATION || rty == BALL) ? 0 : 1;
+                         : (rty == SLIDER || rty == TRANSLATION || rty == BALL || rty == WELD) ? 0 : 1;
             K c0 = o, s0 = z, c1 = o, s1 = z, c2 = o, s2 = z;
             int k = 0;
             if constexpr (CACHED) { c0 = tc[0]; s0 = tc[1]; c1 = tc[2]; s1 = tc[3]; k = 2; }
@@ -181,6 +181,9 @@ template <int TYPE, class K> struct Joint {
             if (rty == SLIDER) {
                 R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
                 H_FM[0] = {zero3<K>(), V3(o, z, z)};
+            } else if (rty == WELD) {
+                // RigidBodyNode_Weld.cpp:389-423: X_FM = identity, no H
+                R_FM = ident<K>();
             } else if (rty == TRANSLATION) {
                 // RigidBodyNodeSpec_Translation.h: X_FM = (I, q), H_FM = (0, e_k)
                 R_FM = ident<K>(); p_FM = V3(K(qv[0]), K(qv[1]), K(qv[2]));

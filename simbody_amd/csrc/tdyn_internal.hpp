@@ -20,7 +20,7 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
 }
 
 enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8,
-       GIMBAL = 9 };
+       GIMBAL = 9, WELD = 10 };
 // Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
 enum { GEN = 15 };
 __host__ __device__ inline int mob_dof(int t) {
@@ -28,6 +28,7 @@ __host__ __device__ inline int mob_dof(int t) {
     case PIN: case SLIDER: case SCREW: return 1;
     case UNIVERSAL: case CYLINDER: return 2;
     case BALL: case TRANSLATION: case PLANAR: case GIMBAL: return 3;
+    case WELD: return 0;
     default: return 6;
     }
 }
@@ -37,11 +38,13 @@ __host__ __device__ inline int mob_nq(int t) {
     case UNIVERSAL: case CYLINDER: return 2;
     case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     case BALL: return 4;
+    case WELD: return 0;
     default: return 7;
     }
 }
-// Translation, Planar, Cylinder, Screw and Gimbal run only as padded generic
-// bodies; the typed code paths cover Pin, Slider, Universal, Ball and Free.
+// Translation, Planar, Cylinder, Screw, Gimbal and Weld run only as padded
+// generic bodies; the typed code paths cover Pin, Slider, Universal, Ball and
+// Free.  A Weld body (no mobilities) has all three padded columns zero.
 __host__ __device__ inline bool mob_generic_only(int t) { return t >= TRANSLATION; }
 
 // Per-body constants, PRM_STRIDE doubles per body (packed over all trees).

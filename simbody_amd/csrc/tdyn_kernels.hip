@@ -233,6 +233,8 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         qv[0] = q(qi);
         R_FM = ident<double>(); p_FM = V3(qv[0], 0.0, 0.0);
         H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
+    } else if (ty == WELD) {
+        R_FM = ident<double>();     // RigidBodyNode_Weld.cpp:389-423: no H
     } else if (ty == UNIVERSAL || ty == GIMBAL) {
         // Universal: body-fixed x then y; Gimbal: body-fixed x, y, z
         // (_Gimbal.h:141-176, Rotation.h:342-349).  With the third angle's

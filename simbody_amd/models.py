@@ -18,10 +18,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 PIN, SLIDER, UNIVERSAL, BALL, FREE = 0, 1, 2, 3, 4
-TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL = 5, 6, 7, 8, 9
-DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1, GIMBAL: 3}
+TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL, WELD = 5, 6, 7, 8, 9, 10
+DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1, GIMBAL: 3,
+       WELD: 0}
 NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1,
-      GIMBAL: 3}
+      GIMBAL: 3, WELD: 0}
 NAMES = {PIN: "Pin", SLIDER: "Slider", UNIVERSAL: "Universal", BALL: "Ball", FREE: "Free"}
 
 
@@ -329,6 +330,8 @@ def _state_plan(spec):
                 plan.append(("angle", qi, c)); qi += 1; c += 1
         elif t == SLIDER:
             plan.append(("trans", qi, c)); qi += 1; c += 1
+        elif t == WELD:
+            continue
         elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL):
             kinds = {TRANSLATION: ("trans",) * 3, PLANAR: ("angle", "trans", "trans"),
                      CYLINDER: ("angle", "trans"), SCREW: ("angle",),

@@ -71,3 +71,15 @@ def rot_to_quat(R):
         q[1 + i] = x; q[1 + j] = (R[j, i] + R[i, j]) * f; q[1 + l] = (R[l, i] + R[i, l]) * f
     q = np.array(q)
     return q / np.linalg.norm(q)
+
+
+def weld_trees(seed, n, nmin=4, nmax=16, p_weld=0.3):
+    """ext_trees with each non-root body welded to its parent with
+    probability p_weld (the root keeps its mobilizer, so every tree moves)."""
+    trees = ext_trees(seed, n, nmin, nmax)
+    for i, t in enumerate(trees):
+        rng = np.random.default_rng(seed * 15485863 + i)
+        w = rng.uniform(size=t.nb) < p_weld
+        w[0] = False
+        t.mobilizer = np.where(w, M.WELD, t.mobilizer).astype(np.int32)
+    return trees
