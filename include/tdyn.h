@@ -77,6 +77,9 @@ typedef struct tdyn_tree_desc {
                                   bodies only; NULL: no Screw bodies */
     const int32_t* reversed;   /* [nb] nonzero: the mobilizer is reversed (MobilizedBody::Reverse,
                                   RigidBodyNodeSpec.h:262-277); not for Free; NULL: none reversed */
+    int32_t use_euler_angles;  /* nonzero: SimbodyMatterSubsystem::setUseEulerAngles(true) -- Ball and
+                                  Free q slots (4 and 7, unchanged) hold body-fixed x-y-z angles (Free:
+                                  then the translation), the last slot unused; u keeps its meaning */
 } tdyn_tree_desc;
 
 typedef struct tdyn_batch tdyn_batch;

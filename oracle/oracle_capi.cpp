@@ -70,7 +70,7 @@ int oracle_kinematics(void* p, const double* q, const double* u, double* X_GB, d
         if (V_GB) storeSV(tr.B[b].V_GB, V_GB+6*b);
     }
     if (energies) { energies[0] = kineticEnergy(tr); energies[1] = potentialEnergy(tr); }
-    if (qerr) { int k=0; for (size_t L=1; L<M.levels.size(); ++L) for (int b : M.levels[L]) if (M.type[b]==BALL||M.type[b]==FREE) qerr[k++] = tr.B[b].qerr; }
+    if (qerr) { int k=0; for (size_t L=1; L<M.levels.size(); ++L) for (int b : M.levels[L]) if (M.isQuat(b)) qerr[k++] = tr.B[b].qerr; }
     return 0;
 }
 

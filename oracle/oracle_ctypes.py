@@ -22,7 +22,7 @@ class TreeDesc(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int32), ("parent", _i), ("mobilizer", _i),
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
                 ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("pitch", _d),
-                ("reversed", _i)]
+                ("reversed", _i), ("use_euler_angles", ctypes.c_int32)]
 
 
 _lib = None
@@ -85,6 +85,7 @@ class OracleModel:
         if pitch is not None:
             self._pitch = np.ascontiguousarray(pitch, np.float64)
             d.pitch = _p(self._pitch)
+        d.use_euler_angles = 1 if getattr(spec, "use_euler_angles", False) else 0
         rev = getattr(spec, "reversed", None)
         if rev is not None:
             self._rev = np.ascontiguousarray(rev, np.int32)

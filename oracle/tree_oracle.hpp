@@ -206,6 +206,7 @@ struct TreeDesc {
     double gravity[3];
     const double*  pitch;       // [nb] Screw pitch (length per radian); nullable
     const int32_t* reversed;    // [nb] nonzero: reversed mobilizer; nullable
+    int32_t use_euler_angles;   // Ball and Free use body-fixed x-y-z angles, not quaternions
 };
 
 struct Model {
@@ -220,7 +221,12 @@ struct Model {
     std::vector<V3<double>> p_BM;
     std::vector<double> pitch;
     std::vector<int> reversed;
+    bool euler = false;                       // SimbodyMatterSubsystem::setUseEulerAngles
     V3<double> g;
+    // Ball and Free carry a quaternion (four q slots either way; in Euler
+    // mode the last slot of the four is unused: RigidBodyNodeSpec_Ball.h:
+    // 183-214, _Free.h:222-257)
+    bool isQuat(int b) const { return !euler && (type[b] == BALL || type[b] == FREE); }
 
     // Build from the plain description.  Body 0 is Ground.
     // Slot assignment follows SimbodyMatterSubsystemRep.cpp:256-293 (q and u
@@ -236,6 +242,7 @@ struct Model {
         X_PF.assign(n, Xform<double>{ident33<double>(), zero3<double>()});
         X_MB = X_PF; p_BM.assign(n, zero3<double>()); pitch.assign(n, 0.0); reversed.assign(n, 0);
         nq = nu = nquat = 0;
+        euler = d.use_euler_angles != 0;
         for (int b = 1; b <= nb; ++b) {
             const int i = b-1;
             const int p = d.parent[i];
@@ -248,7 +255,7 @@ struct Model {
             children[p].push_back(b);
             qIndex[b] = nq; uIndex[b] = nu;
             nq += mobNq(t); nu += dof[b];
-            if (t == BALL || t == FREE) ++nquat;
+            if ((t == BALL || t == FREE) && !euler) ++nquat;
             mass[b] = d.mass[i];
             if (!(mass[b] > 0)) return false;
             com[b] = mk3(d.com_B[3*i], d.com_B[3*i+1], d.com_B[3*i+2]);
@@ -347,6 +354,20 @@ template <class T> inline V3<T> quatDotToW(const T q[4], const T qd[4]) {
     return mk3<T>(-e1*qd[0] + e0*qd[1] - e3*qd[2] + e2*qd[3],
                   -e2*qd[0] + e3*qd[1] + e0*qd[2] - e1*qd[3],
                   -e3*qd[0] - e2*qd[1] + e1*qd[2] + e0*qd[3]);
+}
+// Body-fixed 1-2-3 angle rates from the angular velocity in the parent,
+// qdot = N_P(q) w (Rotation.h:395-406): c, s = cos, sin of the angles.
+template <class T> inline V3<T> eulerDot(const T* c, const T* s, const V3<T>& w) {
+    const T oocosy = T(1)/c[1];
+    const T t = (s[0]*w[1] - c[0]*w[2])*oocosy;
+    return mk3<T>(w[0] + t*s[1], c[0]*w[1] + s[0]*w[2], -t);
+}
+// qdotdot = N_P b + NDot_P w, written with qdot (Rotation.h:1041-1060).
+template <class T> inline V3<T> eulerDotDot(const T* c, const T* s, const V3<T>& qd, const V3<T>& b) {
+    const T oocosy = T(1)/c[1];
+    const V3<T> Nb = eulerDot(c, s, b);
+    const T q1oc1 = qd[1]*oocosy;
+    return Nb + mk3<T>((qd[0]*s[1] - qd[2])*q1oc1, qd[0]*qd[2]*c[1], (qd[2]*s[1] - qd[0])*q1oc1);
 }
 // qdotdot = N b + NDot w = N b - |w|^2/4 q  (Rotation.h:1006-1013).
 template <class T> inline void quatDotDot(const T q[4], const V3<T>& w, const V3<T>& bdot, T qdd[4]) {
@@ -451,7 +472,24 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
             B.H_FM[1] = sv(mk3<T>(T(0), c0, s0), zero3<T>());
             B.H_FM[2] = sv(mk3<T>(s1, -s0*c1, c0*c1), zero3<T>());
         } break;
-        case BALL: case FREE: {
+        case BALL: case FREE: if (M.euler) {
+            // Euler-angle mode (RigidBodyNodeSpec_Ball.h:113-158, _Free.h:
+            // 139-188): body-fixed 1-2-3 angles in q0..q2, Free translation
+            // in q3..q5
+            for (int j=0;j<3;++j) { B.cq[j] = cos(qb[j]); B.sq[j] = sin(qb[j]); }
+            const T c0=B.cq[0], c1=B.cq[1], c2=B.cq[2], s0=B.sq[0], s1=B.sq[1], s2=B.sq[2];
+            const T s0s1 = s0*s1, s2c0 = s2*c0, c0c2 = c0*c2, nc1 = -c1;
+            M33<T> R;
+            R(0,0)= c1*c2;              R(0,1)= s2*nc1;             R(0,2)= s1;
+            R(1,0)= s2c0 + s0s1*c2;     R(1,1)= c0c2 - s0s1*s2;     R(1,2)= s0*nc1;
+            R(2,0)= s0*s2 - s1*c0c2;    R(2,1)= s0*c2 + s1*s2c0;    R(2,2)= c0*c1;
+            B.X_FM.R = R;
+            for (int j=0;j<3;++j) { V3<T> e = zero3<T>(); e[j] = T(1); B.H_FM[j] = sv(e, zero3<T>()); }
+            if (t == FREE) {
+                B.X_FM.p = mk3<T>(qb[3], qb[4], qb[5]);
+                for (int j=0;j<3;++j) { V3<T> e = zero3<T>(); e[j] = T(1); B.H_FM[3+j] = sv(zero3<T>(), e); }
+            }
+        } else {
             const T len = sqrt(qb[0]*qb[0] + qb[1]*qb[1] + qb[2]*qb[2] + qb[3]*qb[3]);
             B.qerr = len - T(1);
             const T oo = T(1)/len;
@@ -526,9 +564,16 @@ template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* 
         const int t = M.type[b], nd = M.dof[b];
         if (qdot) {
             T* qd = qdot + M.qIndex[b];
-            if (t == BALL || t == FREE) {
+            if (M.isQuat(b)) {
                 quatDot(qb, mk3<T>(ub[0],ub[1],ub[2]), qd);
                 if (t == FREE) { qd[4]=ub[3]; qd[5]=ub[4]; qd[6]=ub[5]; }
+            } else if (t == BALL || t == FREE) {
+                // Euler-angle rates qdot = N(q) w_FM (Rotation.h:395-406,
+                // 1021-1028); the unused fourth slot's rate is zero
+                const V3<T> ea = eulerDot(B.cq, B.sq, mk3<T>(ub[0],ub[1],ub[2]));
+                qd[0]=ea[0]; qd[1]=ea[1]; qd[2]=ea[2];
+                if (t == FREE) { qd[3]=ub[3]; qd[4]=ub[4]; qd[5]=ub[5]; qd[6]=T(0); }
+                else qd[3] = T(0);
             } else for (int j=0;j<nd;++j) qd[j] = ub[j];
         }
         B.V_FM = svzero<T>(); B.V_PB_G = svzero<T>();
@@ -768,9 +813,16 @@ template <class T> void forwardDynamics(Tree<T>& tr, const T* q, const T* u, con
             T* qdd = qdotdot + M.qIndex[b];
             const T* qb = q + M.qIndex[b];
             const T* ub = u + M.uIndex[b];
-            if (t == BALL || t == FREE) {
+            if (M.isQuat(b)) {
                 quatDotDot(qb, mk3<T>(ub[0],ub[1],ub[2]), mk3<T>(ud[0],ud[1],ud[2]), qdd);
                 if (t == FREE) { qdd[4]=ud[3]; qdd[5]=ud[4]; qdd[6]=ud[5]; }
+            } else if (t == BALL || t == FREE) {
+                // Euler-angle mode (RigidBodyNodeSpec_Ball.h:355-392)
+                const V3<T> qd = eulerDot(B.cq, B.sq, mk3<T>(ub[0],ub[1],ub[2]));
+                const V3<T> ea = eulerDotDot(B.cq, B.sq, qd, mk3<T>(ud[0],ud[1],ud[2]));
+                qdd[0]=ea[0]; qdd[1]=ea[1]; qdd[2]=ea[2];
+                if (t == FREE) { qdd[3]=ud[3]; qdd[4]=ud[4]; qdd[5]=ud[5]; qdd[6]=T(0); }
+                else qdd[3] = T(0);
             } else for (int j=0;j<nd;++j) qdd[j] = ud[j];
         }
     }
@@ -1016,8 +1068,7 @@ template <class T> struct Rkm {
     // component of the q error estimate).
     void normalizeQuats(T* q, T* qerrest) {
         for (int b=1;b<=M.nb;++b) {
-            const int t = M.type[b];
-            if (t != BALL && t != FREE) continue;
+            if (!M.isQuat(b)) continue;
             T* qq = q + M.qIndex[b];
             const T n = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]);
             for (int i=0;i<4;++i) qq[i] = qq[i]/n;
@@ -1033,8 +1084,7 @@ template <class T> struct Rkm {
         if (M.nquat == 0) return T(0);
         T s = T(0);
         for (int b=1;b<=M.nb;++b) {
-            const int t = M.type[b];
-            if (t != BALL && t != FREE) continue;
+            if (!M.isQuat(b)) continue;
             const T* qq = q + M.qIndex[b];
             const T e = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - T(1);
             s = s + e*e;
@@ -1047,7 +1097,7 @@ template <class T> struct Rkm {
         for (int b=1;b<=M.nb;++b) {
             const int t = M.type[b];
             const T* e = err + M.qIndex[b];
-            if (t == BALL || t == FREE) {
+            if (M.isQuat(b)) {
                 const T* qq = q + M.qIndex[b];
                 const V3<T> w = quatDotToW(qq, e);
                 T d[4]; quatDot(qq, w, d);

@@ -77,6 +77,7 @@ void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
     for (int k = 0; k < 3; ++k) P[PRM_P_BM + k] = pbm[k];
     if (d.mobilizer[b-1] == TDYN_SCREW) P[PRM_PITCH] = d.screw_pitch[b-1];
     if (d.reversed && d.reversed[b-1]) P[PRM_REV] = 1.0;
+    if (d.use_euler_angles && (d.mobilizer[b-1] == TDYN_BALL || d.mobilizer[b-1] == TDYN_FREE)) P[PRM_EULER] = 1.0;
 }
 
 bool any_reversed(const tdyn_tree_desc& d) {
@@ -191,10 +192,13 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
 
     auto* B = new tdyn_batch;
     B->n_inst = n_instances; B->n_trees = n_trees;
-    // reversed mobilizers run as padded generic bodies, as in batches of
-    // different trees
-    for (int t = 0; t < n_trees; ++t) B->any_rev = B->any_rev || any_reversed(trees[t]);
-    B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev;
+    // reversed mobilizers and Euler-angle Balls run as padded generic
+    // bodies, as in batches of different trees
+    for (int t = 0; t < n_trees; ++t) {
+        B->any_rev = B->any_rev || any_reversed(trees[t]);
+        B->any_euler = B->any_euler || trees[t].use_euler_angles != 0;
+    }
+    B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev && !B->any_euler;
     std::vector<int4> body_b, pre, bfs, bfs_b; std::vector<int> child_idx, lvl; std::vector<double> prm, grav, prm_bfs;
     B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
@@ -215,7 +219,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             // batches of different trees carry non-Free bodies in padded 3-dof records
             const int rd = ((!B->uniform || mob_generic_only(ty)) && ty != TDYN_FREE) ? 3 : mob_dof(ty);
             q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(rd); gslot += 7 * mob_dof(ty);
-            if (ty == TDYN_BALL || ty == TDYN_FREE) ++nquat;
+            if ((ty == TDYN_BALL || ty == TDYN_FREE) && !d.use_euler_angles) ++nquat;
             double P[PRM_STRIDE];
             pack_body_params(d, b, P);
             prm.insert(prm.end(), P, P + PRM_STRIDE);

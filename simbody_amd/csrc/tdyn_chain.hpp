@@ -83,19 +83,21 @@ template <class PA> struct IdF {
     __device__ __forceinline__ double operator[](int k) const { return p[k]; }
     __device__ __forceinline__ IdF operator+(size_t k) const { return IdF(p + k); }
 };
-// A model that may hold reversed mobilizers (MobilizedBody::Reverse): each
-// generic body's PRM_REV entry says whether its X_FM and H_FM are reversed.
-template <class PA> struct RevF {
+// A model that may hold reversed mobilizers (MobilizedBody::Reverse) or use
+// Euler angles: each body's PRM_REV entry says whether its X_FM and H_FM are
+// reversed (generic bodies), PRM_EULER whether its Ball or Free q are
+// body-fixed x-y-z angles.
+template <class PA> struct ExtF {
     PA p;
-    __device__ __forceinline__ RevF() = default;
-    __device__ __forceinline__ explicit RevF(PA q) : p(q) {}
+    __device__ __forceinline__ ExtF() = default;
+    __device__ __forceinline__ explicit ExtF(PA q) : p(q) {}
     __device__ __forceinline__ double operator[](int k) const { return p[k]; }
-    __device__ __forceinline__ RevF operator+(size_t k) const { return RevF(p + k); }
+    __device__ __forceinline__ ExtF operator+(size_t k) const { return ExtF(p + k); }
 };
-template <class PA> struct PrmTraits { static constexpr bool lane = false, idf = false, rev = false; };
-template <> struct PrmTraits<PrmL> { static constexpr bool lane = true, idf = false, rev = false; };
-template <class T> struct PrmTraits<IdF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = true, rev = false; };
-template <class T> struct PrmTraits<RevF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = false, rev = true; };
+template <class PA> struct PrmTraits { static constexpr bool lane = false, idf = false, ext = false; };
+template <> struct PrmTraits<PrmL> { static constexpr bool lane = true, idf = false, ext = false; };
+template <class T> struct PrmTraits<IdF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = true, ext = false; };
+template <class T> struct PrmTraits<ExtF<T>> { static constexpr bool lane = PrmTraits<T>::lane, idf = false, ext = true; };
 
 template <class K, class PA> __device__ __forceinline__ m33<K> ldm(const PA& p) {
     m33<K> r; for (int i = 0; i < 9; ++i) r.a[i] = K(p[i]); return r;
@@ -153,10 +155,31 @@ template <int TYPE, class K> struct Joint {
     // angle is recomputed, which keeps the cache at four entries),
     // Ball/Free 1/|quaternion| (computed in FP64).  pitch: the Screw pitch
     // (GEN only).
+    // eul: Ball and Free hold body-fixed x-y-z angles instead of a quaternion
+    // (Euler-angle mode, RigidBodyNodeSpec_Ball.h:113-158, _Free.h:139-188);
+    // ec keeps (c0, s0, c1, s1) of those angles for their rates.
+    K ec[4];
     template <bool CACHED>
-    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0)) {
+    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0), bool eul = false) {
         const K z = K(0), o = K(1);
         p_FM = zero3<K>();
+        if constexpr (TYPE == FREE) {
+            if (eul) {
+                // the root only: the three sin/cos are recomputed in every sweep
+                K c0, s0, c1, s1, c2, s2;
+                sincosk<K>(qv[0], &s0, &c0); sincosk<K>(qv[1], &s1, &c1); sincosk<K>(qv[2], &s2, &c2);
+                ec[0] = c0; ec[1] = s0; ec[2] = c1; ec[3] = s1;
+                R_FM = rot_xyz(c0, s0, c1, s1, c2, s2);
+                p_FM = V3(K(qv[3]), K(qv[4]), K(qv[5]));
+                H_FM[0] = {V3(o, z, z), zero3<K>()};
+                H_FM[1] = {V3(z, o, z), zero3<K>()};
+                H_FM[2] = {V3(z, z, o), zero3<K>()};
+                H_FM[3] = {zero3<K>(), V3(o, z, z)};
+                H_FM[4] = {zero3<K>(), V3(z, o, z)};
+                H_FM[5] = {zero3<K>(), V3(z, z, o)};
+                return;
+            }
+        }
         if constexpr (TYPE == GEN) {
             ty = rty; d = mob_dof(rty);
             H_FM[0] = H_FM[1] = H_FM[2] = svzero<K>();
@@ -164,7 +187,8 @@ template <int TYPE, class K> struct Joint {
             // x, y (Universal) and x, y, z (Gimbal).  Their sin/cos come from one
             // sincos in a loop shared by every type; when CACHED the first two
             // come from the trig cache.
-            const int na = rty == UNIVERSAL ? 2 : rty == GIMBAL ? 3
+            const bool xyz = rty == UNIVERSAL || rty == GIMBAL || (rty == BALL && eul);
+            const int na = rty == UNIVERSAL ? 2 : xyz ? 3
                          : (rty == SLIDER || rty == TRANSLATION || rty == BALL || rty == WELD) ? 0 : 1;
             K c0 = o, s0 = z, c1 = o, s1 = z, c2 = o, s2 = z;
             int k = 0;
@@ -178,6 +202,7 @@ template <int TYPE, class K> struct Joint {
                 qa = qb; qb = qv[2];
             }
             if constexpr (!CACHED) { tc[0] = c0; tc[1] = s0; tc[2] = c1; tc[3] = s1; }
+            ec[0] = c0; ec[1] = s0; ec[2] = c1; ec[3] = s1;
             if (rty == SLIDER) {
                 R_FM = ident<K>(); p_FM = V3(K(qv[0]), z, z);
                 H_FM[0] = {zero3<K>(), V3(o, z, z)};
@@ -190,7 +215,7 @@ template <int TYPE, class K> struct Joint {
                 H_FM[0] = {zero3<K>(), V3(o, z, z)};
                 H_FM[1] = {zero3<K>(), V3(z, o, z)};
                 H_FM[2] = {zero3<K>(), V3(z, z, o)};
-            } else if (rty == BALL) {
+            } else if (rty == BALL && !eul) {
                 double oo;
                 if constexpr (CACHED) oo = double(tc[0]);
                 else { oo = 1.0 / sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]); tc[0] = K(oo); }
@@ -198,7 +223,7 @@ template <int TYPE, class K> struct Joint {
                 H_FM[0] = {V3(o, z, z), zero3<K>()};
                 H_FM[1] = {V3(z, o, z), zero3<K>()};
                 H_FM[2] = {V3(z, z, o), zero3<K>()};
-            } else if (rty == UNIVERSAL || rty == GIMBAL) {
+            } else if (xyz) {
                 // Body-fixed 1-2-3 rotation (_Gimbal.h:141-176, Rotation.h:
                 // 342-349); a Universal's third angle has (sin, cos) = (0, 1),
                 // which gives its x-then-y rotation entry for entry.
@@ -206,9 +231,16 @@ template <int TYPE, class K> struct Joint {
                 R_FM.a[0]= c1*c2;             R_FM.a[1]= s2*nc1;            R_FM.a[2]= s1;
                 R_FM.a[3]= s2c0 + s0s1*c2;    R_FM.a[4]= c0c2 - s0s1*s2;    R_FM.a[5]= s0*nc1;
                 R_FM.a[6]= s0*s2 - s1*c0c2;   R_FM.a[7]= s0*c2 + s1*s2c0;   R_FM.a[8]= c0*c1;
-                H_FM[0] = {V3(o, z, z), zero3<K>()};
-                H_FM[1] = {V3(z, c0, s0), zero3<K>()};
-                if (rty == GIMBAL) H_FM[2] = {V3(s1, s0*nc1, c0*c1), zero3<K>()};
+                if (rty == BALL) {
+                    // Euler-angle Ball: u is still w_FM in F
+                    H_FM[0] = {V3(o, z, z), zero3<K>()};
+                    H_FM[1] = {V3(z, o, z), zero3<K>()};
+                    H_FM[2] = {V3(z, z, o), zero3<K>()};
+                } else {
+                    H_FM[0] = {V3(o, z, z), zero3<K>()};
+                    H_FM[1] = {V3(z, c0, s0), zero3<K>()};
+                    if (rty == GIMBAL) H_FM[2] = {V3(s1, s0*nc1, c0*c1), zero3<K>()};
+                }
             } else {
                 // Pin (_Pin.h), Planar (_Planar.h:115-156), Cylinder
                 // (_Cylinder.h:100-139), Screw (_Screw.h:109-148): rotation
@@ -446,11 +478,17 @@ __device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 
     }
 }
 
+// Euler-angle mode of a Ball (generic) or Free body, in models that may use it.
+template <int TYPE, class PA>
+__device__ __forceinline__ bool euler_of(const PA& P) {
+    if constexpr ((TYPE == GEN || TYPE == FREE) && PrmTraits<PA>::ext) return P[PRM_EULER] != 0.0;
+    return false;
+}
 // Reverses a generic body's joint when the model may hold reversed
 // mobilizers and this body's is.
 template <int TYPE, class K, class PA>
 __device__ __forceinline__ void reverse_if(const PA& P, Joint<TYPE, K>& J) {
-    if constexpr (TYPE == GEN && PrmTraits<PA>::rev) { if (P[PRM_REV] != 0.0) J.reverse(); }
+    if constexpr (TYPE == GEN && PrmTraits<PA>::ext) { if (P[PRM_REV] != 0.0) J.reverse(); }
 }
 
 // The pitch of a Screw body (padded generic bodies only), else zero.
@@ -468,28 +506,43 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
                                            m33<K>& R, v3<K>& p, sv<K>& V) {
     constexpr int D = MobC<TYPE>::D;
     const double* qv = x.q; const double* uv = x.u;
+    const bool eul = euler_of<TYPE>(P);
     if constexpr (TYPE == GEN) {
         if (x.ty == BALL) {
-            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+            if (!eul) {
+                double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
+                for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
+            }
         } else {
             const int d = mob_dof(x.ty);
 #pragma unroll
             for (int k = 0; k < D; ++k) if (k < d) f(qi + k) = uv[k];
         }
     } else if constexpr (TYPE == BALL || TYPE == FREE) {
-        double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+        if (!eul) {
+            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
-        if constexpr (TYPE == FREE) { f(qi+4) = uv[3]; f(qi+5) = uv[4]; f(qi+6) = uv[5]; }
+            for (int k = 0; k < 4; ++k) f(qi + k) = d4[k];
+            if constexpr (TYPE == FREE) { f(qi+4) = uv[3]; f(qi+5) = uv[4]; f(qi+6) = uv[5]; }
+        }
     } else {
 #pragma unroll
         for (int k = 0; k < D; ++k) f(qi + k) = uv[k];
     }
     Joint<TYPE, K> J;
     K tc[4] = {K(0), K(0), K(0), K(0)};
-    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), eul);
+    if constexpr (TYPE == GEN || TYPE == FREE) {
+        if (eul && (TYPE == FREE || x.ty == BALL)) {
+            // x-y-z angle rates N(q) w (RigidBodyNodeSpec_Ball.h:183-214,
+            // _Free.h:222-257); the fourth q slot is unused, its rate zero
+            const v3<K> ed = euler_dot(J.ec[0], J.ec[1], J.ec[2], J.ec[3], V3(K(uv[0]), K(uv[1]), K(uv[2])));
+            f(qi) = double(ed.x); f(qi + 1) = double(ed.y); f(qi + 2) = double(ed.z);
+            if constexpr (TYPE == FREE) { f(qi+3) = uv[3]; f(qi+4) = uv[4]; f(qi+5) = uv[5]; f(qi+6) = 0.0; }
+            else f(qi + 3) = 0.0;
+        }
+    }
     reverse_if<TYPE>(P, J);
 #pragma unroll
     for (int k = 0; k < TcN<TYPE>::n; ++k) st.T(k) = tc[k];
@@ -559,7 +612,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), euler_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
@@ -575,7 +628,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const v3<K> w_GP = V_GB.w - Vpb.w;
     const v3<K> v_GP = (V_GB.v - Vpb.v) - cross(w_GP, phi);
     const sv<K> V_GP = {w_GP, v_GP};
-    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::rev>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::ext>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
     const K m = K(P[PRM_MASS]);
     const v3<K> Mk_p = mul(R_GB, ld3<K>(P + PRM_COM));
     const s33<K> Gu = {K(P[PRM_G]), K(P[PRM_G+1]), K(P[PRM_G+2]), K(P[PRM_G+3]), K(P[PRM_G+4]), K(P[PRM_G+5])};
@@ -668,7 +721,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x));
+    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), euler_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
@@ -680,7 +733,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     for (int j = 0; j < D; ++j) Vpb = Vpb + K(uv[j])*H[j];
     const sv<K> V_GP = V;
     const sv<K> V_GB = phiT_mul(phi, V_GP) + Vpb;
-    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::rev>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
+    const sv<K> a = coriolis<TYPE, K, PrmTraits<PA>::ext>(J, R_GF, r_MB_F, H, uv, V_GP, V_GB);
     const sv<K> Ap = phiT_mul(phi, A);
     double ud[D];
 #pragma unroll

@@ -56,6 +56,7 @@ struct tdyn_batch {
     int max_nq = 0, max_nu = 0, max_nb = 0, max_rec = 0;
     bool uniform = true;
     bool any_rev = false;        // some tree has a reversed mobilizer (the batch runs padded generic bodies)
+    bool any_euler = false;      // some tree uses Euler angles for Ball and Free (likewise)
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
@@ -115,7 +116,7 @@ struct tdyn_batch {
         a.scratch32 = scratch32.p;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.rev = any_rev ? 1 : 0;
+        a.ext = (any_rev || any_euler) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

@@ -51,7 +51,8 @@ __host__ __device__ inline bool mob_generic_only(int t) { return t >= TRANSLATIO
 enum {
     PRM_R_PF = 0, PRM_P_PF = 9, PRM_R_MB = 12, PRM_P_MB = 21, PRM_MASS = 24,
     PRM_COM = 25, PRM_G = 28 /* unit inertia about Bo in B: xx yy zz xy xz yz */,
-    PRM_P_BM = 34, PRM_PITCH = 37 /* Screw */, PRM_REV = 38 /* 1: reversed mobilizer */, PRM_STRIDE = 40
+    PRM_P_BM = 34, PRM_PITCH = 37 /* Screw */, PRM_REV = 38 /* 1: reversed mobilizer */,
+    PRM_EULER = 39 /* 1: Ball/Free q are x-y-z angles */, PRM_STRIDE = 40
 };
 
 // Per-body record in the per-lane scratch (doubles, relative to rec_off).
@@ -150,7 +151,7 @@ struct RkmArgs {
     int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
     int idf;                 // chain evaluator: every frame is an identity rotation at the joints
                              // (R_PF = I, p_PF = 0, R_MB = I), so the frame products are skipped
-    int rev;                 // some body's mobilizer is reversed (PRM_REV)
+    int ext;                 // some body's mobilizer is reversed (PRM_REV) or uses Euler angles (PRM_EULER)
     const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
                              // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
     int vec_aos;             // 1: vectors lane-contiguous (entry k of lane l at ((v*L + l)*ny_max + k)) and

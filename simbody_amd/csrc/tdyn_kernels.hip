@@ -111,6 +111,9 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
     for (int k = 0; k < NQ; ++k) qv[k] = q(qi + k);
     dm R_FM; d3 p_FM = zero3<double>();
     dv H_FM[D];
+    // Ball/Free q as x-y-z angles (Euler-angle mode); ec: their c0 s0 c1 s1
+    const bool eul = (TYPE == BALL || TYPE == FREE) && P[PRM_EULER] != 0.0;
+    double ec[4] = {1.0, 0.0, 1.0, 0.0};
     if constexpr (TYPE == PIN) {
         double sn, cs; sincos(qv[0], &sn, &cs);
         R_FM.a[0]=cs; R_FM.a[1]=-sn; R_FM.a[2]=0; R_FM.a[3]=sn; R_FM.a[4]=cs; R_FM.a[5]=0;
@@ -129,12 +132,18 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
     } else {   // BALL, FREE
         const double len = sqrt(qv[0]*qv[0] + qv[1]*qv[1] + qv[2]*qv[2] + qv[3]*qv[3]);
         const double oo = 1.0 / len;
+        if (eul) {
+            // x-y-z angles, Free translation in q3..q5 (RigidBodyNodeSpec_Free.h:139-188)
+            double s2, c2;
+            sincos(qv[0], &ec[1], &ec[0]); sincos(qv[1], &ec[3], &ec[2]); sincos(qv[2], &s2, &c2);
+            R_FM = rot_xyz(ec[0], ec[1], ec[2], ec[3], c2, s2);
+        } else
         R_FM = quat_rot(qv[0]*oo, qv[1]*oo, qv[2]*oo, qv[3]*oo);
         H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
         H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
         H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
         if constexpr (TYPE == FREE) {
-            p_FM = V3(qv[4], qv[5], qv[6]);
+            p_FM = eul ? V3(qv[3], qv[4], qv[5]) : V3(qv[4], qv[5], qv[6]);
             H_FM[3] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
             H_FM[4] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
             H_FM[5] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
@@ -172,10 +181,17 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
 #pragma unroll
         for (int k = 0; k < D; ++k) uv[k] = u(ui + k);
         if constexpr (TYPE == BALL || TYPE == FREE) {
-            double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
+            if (eul) {
+                const d3 ed = euler_dot(ec[0], ec[1], ec[2], ec[3], V3(uv[0], uv[1], uv[2]));
+                qd(qi, ed.x); qd(qi + 1, ed.y); qd(qi + 2, ed.z);
+                if constexpr (TYPE == FREE) { qd(qi+3, uv[3]); qd(qi+4, uv[4]); qd(qi+5, uv[5]); qd(qi+6, 0.0); }
+                else qd(qi + 3, 0.0);
+            } else {
+                double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) qd(qi + k, d4[k]);
-            if constexpr (TYPE == FREE) { qd(qi+4, uv[3]); qd(qi+5, uv[4]); qd(qi+6, uv[5]); }
+                for (int k = 0; k < 4; ++k) qd(qi + k, d4[k]);
+                if constexpr (TYPE == FREE) { qd(qi+4, uv[3]); qd(qi+5, uv[4]); qd(qi+6, uv[5]); }
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < D; ++k) qd(qi + k, uv[k]);
@@ -223,6 +239,8 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
     double qv[4] = {0.0, 0.0, 0.0, 0.0};
     dm R_FM; d3 p_FM = zero3<double>();
     dv H_FM[D] = {svzero<double>(), svzero<double>(), svzero<double>()};
+    const bool eul = ty == BALL && P[PRM_EULER] != 0.0;   // x-y-z angles, not a quaternion
+    double ec[4] = {1.0, 0.0, 1.0, 0.0};                   // their c0 s0 c1 s1
     if (ty == PIN) {
         qv[0] = q(qi);
         double sn, cs; sincos(qv[0], &sn, &cs);
@@ -279,6 +297,14 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
             p_FM = V3(0.0, 0.0, qv[0]*pt);
             H_FM[0] = {V3(0.0, 0.0, 1.0), V3(0.0, 0.0, pt)};
         }
+    } else if (eul) {   // BALL with x-y-z angles (RigidBodyNodeSpec_Ball.h:113-158)
+        qv[0] = q(qi); qv[1] = q(qi + 1); qv[2] = q(qi + 2);
+        double s2, c2;
+        sincos(qv[0], &ec[1], &ec[0]); sincos(qv[1], &ec[3], &ec[2]); sincos(qv[2], &s2, &c2);
+        R_FM = rot_xyz(ec[0], ec[1], ec[2], ec[3], c2, s2);
+        H_FM[0] = {V3(1.0, 0.0, 0.0), zero3<double>()};
+        H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
+        H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
     } else {   // BALL
 #pragma unroll
         for (int k = 0; k < 4; ++k) qv[k] = q(qi + k);
@@ -338,7 +364,11 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         double uv[D];
 #pragma unroll
         for (int k = 0; k < D; ++k) uv[k] = k < d ? u(ui + k) : 0.0;
-        if (ty == BALL) {
+        if (ty == BALL && eul) {
+            // angle rates N(q) w; the unused fourth slot's rate is zero
+            const d3 ed = euler_dot(ec[0], ec[1], ec[2], ec[3], V3(uv[0], uv[1], uv[2]));
+            qd(qi, ed.x); qd(qi + 1, ed.y); qd(qi + 2, ed.z); qd(qi + 3, 0.0);
+        } else if (ty == BALL) {
             double d4[4]; quat_dot(qv, V3(uv[0], uv[1], uv[2]), d4);
 #pragma unroll
             for (int k = 0; k < 4; ++k) qd(qi + k, d4[k]);
@@ -594,6 +624,18 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
         bool quat = TYPE == BALL || TYPE == FREE;
         if constexpr (TYPE == GEN) quat = ba.y == BALL;
         if constexpr (D >= 3) {
+            if (quat && c.prm(b)[PRM_EULER] != 0.0) {
+                // x-y-z angles: qdotdot = N b + NDot w (RigidBodyNodeSpec_Ball.h:355-392)
+                double s0, c0, s1, c1;
+                sincos(q(ba.z), &s0, &c0); sincos(q(ba.z + 1), &s1, &c1);
+                const d3 w = {u(ba.w), u(ba.w+1), u(ba.w+2)};
+                const d3 qd = euler_dot(c0, s0, c1, s1, w);
+                const d3 ea = euler_dotdot(c0, s0, c1, s1, qd, V3(ud[0], ud[1], ud[2]));
+                qdd(ba.z, ea.x); qdd(ba.z + 1, ea.y); qdd(ba.z + 2, ea.z);
+                if constexpr (TYPE == FREE) { qdd(ba.z+3, ud[3]); qdd(ba.z+4, ud[4]); qdd(ba.z+5, ud[5]); qdd(ba.z+6, 0.0); }
+                else qdd(ba.z + 3, 0.0);
+                return;
+            }
             if (quat) {
                 double qq[4], nb4[4];
 #pragma unroll
@@ -1068,8 +1110,8 @@ struct TreeStreamEval {
         return i == 0 ? tstream::LaneStore{S, o + 18, o + 60, o + 54} : tstream::LaneStore{S, o + 18, o + 39, o + 36};
     }
     // (every generic body checks its reversed-mobilizer flag)
-    __device__ __forceinline__ chain::RevF<cdptr> prm(int b) const {
-        return chain::RevF<cdptr>(cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE);
+    __device__ __forceinline__ chain::ExtF<cdptr> prm(int b) const {
+        return chain::ExtF<cdptr>(cptr(T.prm) + (size_t)(body_off + b - 1) * PRM_STRIDE);
     }
     __device__ __forceinline__ dv F(int b) const {
         if (!fapp) return svzero<double>();
@@ -1268,7 +1310,10 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         for (int b = 0; b < nb; ++b) {
             const int4 ba = ldi4(body_a, b);
             const int qi = ba.z;
-            if (ba.y == BALL || ba.y == FREE) {
+            // (a Ball or Free in Euler-angle mode holds no quaternion; the
+            // flag is read per body, not tested once per tree, which keeps
+            // the compiler from duplicating the loop for the two modes)
+            if ((ba.y == BALL || ba.y == FREE) && a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] == 0.0) {
                 double qq[4], e[4], d4[4];
                 for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
                 const d3 w = quat_dot_to_w(qq, e);
@@ -1416,6 +1461,7 @@ __device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
         if (ba.y != BALL && ba.y != FREE) continue;
+        if (c.prm(b)[PRM_EULER] != 0.0) continue;      // x-y-z angles, not a quaternion
         double qq[4], n2 = 0;
         for (int k = 0; k < 4; ++k) { qq[k] = vec_at(a, VY, lane, ba.z + k); n2 += qq[k]*qq[k]; }
         const double n = sqrt(n2);

@@ -168,5 +168,31 @@ template <class S> TD v3<S> quat_dot_to_w(const S q[4], const S d[4]) {
             -e3*d[0] - e2*d[1] + e1*d[2] + e0*d[3]};
 }
 
+// Body-fixed x-y-z (1-2-3) rotation from the cosines and sines of its three
+// angles (Rotation.h:342-349).
+template <class S> TD m33<S> rot_xyz(S c0, S s0, S c1, S s1, S c2, S s2) {
+    const S s0s1 = s0*s1, s2c0 = s2*c0, c0c2 = c0*c2, nc1 = -c1;
+    m33<S> R;
+    R.a[0]= c1*c2;             R.a[1]= s2*nc1;            R.a[2]= s1;
+    R.a[3]= s2c0 + s0s1*c2;    R.a[4]= c0c2 - s0s1*s2;    R.a[5]= s0*nc1;
+    R.a[6]= s0*s2 - s1*c0c2;   R.a[7]= s0*c2 + s1*s2c0;   R.a[8]= c0*c1;
+    return R;
+}
+// Rates of x-y-z angles from the angular velocity in the parent frame,
+// qdot = N_P(q) w (Rotation.h:395-406).
+template <class S> TD v3<S> euler_dot(S c0, S s0, S c1, S s1, v3<S> w) {
+    const S oocosy = S(1)/c1;
+    const S t = (s0*w.y - c0*w.z)*oocosy;
+    return {w.x + t*s1, c0*w.y + s0*w.z, -t};
+}
+// Second derivatives of the angles from qdot and the angular acceleration
+// b in the parent frame: N_P b + NDot_P w (Rotation.h:1041-1060).
+template <class S> TD v3<S> euler_dotdot(S c0, S s0, S c1, S s1, v3<S> qd, v3<S> b) {
+    const S oocosy = S(1)/c1;
+    const v3<S> Nb = euler_dot(c0, s0, c1, s1, b);
+    const S q1oc1 = qd.y*oocosy;
+    return Nb + v3<S>{(qd.x*s1 - qd.z)*q1oc1, qd.x*qd.z*c1, (qd.z*s1 - qd.x)*q1oc1};
+}
+
 #undef TD
 } // namespace tdyn

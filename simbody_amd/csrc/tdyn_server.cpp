@@ -66,6 +66,8 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
     }
     if (any_reversed(*topology))
         return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs forward mobilizers only");
+    if (topology->use_euler_angles)
+        return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs quaternion Ball and Free bodies only");
     RkmCfg c;
     int prec = 0;
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }

@@ -74,9 +74,9 @@ struct PrmS {
 template <int G, int IDF>
 struct GroupEval {
     // IDF: 0 general joint frames, 1 identity frames at the joints, 2 general
-    // frames and reversed mobilizers
+    // frames with reversed mobilizers or Euler-angle Balls and Frees
     using PA = std::conditional_t<IDF == 1, chain::IdF<PrmS>,
-                                  std::conditional_t<IDF == 2, chain::RevF<PrmS>, PrmS>>;
+                                  std::conditional_t<IDF == 2, chain::ExtF<PrmS>, PrmS>>;
     using VR = tstream::LaneVec;
     int nb, nq, nlev, l;
     const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
@@ -282,7 +282,10 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) 
         for (int b = l; b < nb; b += G) {
             const int4 ba = ldi4(body_a, b);
             const int qi = ba.z;
-            if (ba.y == BALL || ba.y == FREE) {
+            // (a Ball or Free in Euler-angle mode holds no quaternion; the
+            // flag is read per body, not tested once per tree, which keeps
+            // the compiler from duplicating the loop for the two modes)
+            if ((ba.y == BALL || ba.y == FREE) && a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] == 0.0) {
                 double qq[4], e[4], d4[4];
                 for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
                 const d3 w = quat_dot_to_w(qq, e);
@@ -428,12 +431,12 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
 
 // a.idf: every tree's joint frames are identity rotations (Simbody's
 // noR_PF/noX_MB node forms), the frame products are skipped; the results
-// are bitwise those of the general form.  a.rev: some mobilizer is reversed;
-// only that build carries the reversal code.
+// are bitwise those of the general form.  a.ext: some mobilizer is reversed
+// or uses Euler angles; only that build carries the code for them.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);
-    if (a.rev) {
+    if (a.ext) {
         if (f0_only) k_rkm_plan<2, true><<<grd, blk, 0, s>>>(a);
         else k_rkm_plan<2, false><<<grd, blk, 0, s>>>(a);
     } else if (a.idf) {

@@ -51,7 +51,7 @@ class TreeDescC(ctypes.Structure):
     _fields_ = [("nb", ctypes.c_int32), ("parent", _i32), ("mobilizer", _i32),
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
                 ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("screw_pitch", _d),
-                ("reversed", _i32)]
+                ("reversed", _i32), ("use_euler_angles", ctypes.c_int32)]
 
 
 class IntegratorCfgC(ctypes.Structure):
@@ -134,6 +134,7 @@ def fill_tree_desc(d, spec, keep):
         pa = np.ascontiguousarray(pitch, np.float64)
         keep.append(pa)
         d.screw_pitch = pa.ctypes.data_as(_d)
+    d.use_euler_angles = 1 if getattr(spec, "use_euler_angles", False) else 0
     rev = getattr(spec, "reversed", None)
     if rev is not None:
         ra = np.ascontiguousarray(rev, np.int32)

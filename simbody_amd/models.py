@@ -38,6 +38,7 @@ class TreeSpec:
     gravity: np.ndarray = field(default_factory=lambda: np.array([0.0, -9.8, 0.0]))
     pitch: np.ndarray = None  # [nb] Screw pitch (length per radian), None = zeros
     reversed: np.ndarray = None  # [nb] int32, nonzero = MobilizedBody::Reverse; None = all forward
+    use_euler_angles: bool = False  # Ball/Free q as body-fixed x-y-z angles (4/7 q slots kept)
 
     @property
     def nb(self):
@@ -338,6 +339,11 @@ def _state_plan(spec):
                      GIMBAL: ("angle", "tilt", "angle")}[t]
             for k in kinds:
                 plan.append((k, qi, c)); qi += 1; c += 1
+        elif getattr(spec, "use_euler_angles", False):
+            # x-y-z angles, (Free) translation, one unused q slot left at 0
+            for k in ("angle", "tilt", "angle") + (("trans",) * 3 if t == FREE else ()):
+                plan.append((k, qi, c)); qi += 1; c += 1
+            qi += 1
         else:
             plan.append(("quat", qi, c)); qi += 4; c += 8
             if t == FREE:
