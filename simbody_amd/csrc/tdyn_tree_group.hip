@@ -45,8 +45,8 @@ __device__ __forceinline__ void grp_sync() {
 }
 // Sum over the G lanes of a group (aligned lane blocks of G).  The butterfly
 // adds the same pairs in every lane, so all lanes get bitwise the same sum.
-template <int G> __device__ __forceinline__ double grp_sum(double x) {
-#pragma unroll
+// G is wave-uniform (one G per wavefront).
+__device__ __forceinline__ double grp_sum(int G, double x) {
     for (int m = 1; m < G; m <<= 1) x += __shfl_xor(x, m, 64);
     return x;
 }
@@ -71,14 +71,14 @@ struct PrmS {
     __device__ __forceinline__ PrmS operator+(int k) const { return {p + k * tgroup::BW}; }
 };
 
-template <int G, int IDF>
+template <int IDF>
 struct GroupEval {
     // IDF: 0 general joint frames, 1 identity frames at the joints, 2 general
     // frames with reversed mobilizers or Euler-angle Balls and Frees
     using PA = std::conditional_t<IDF == 1, chain::IdF<PrmS>,
                                   std::conditional_t<IDF == 2, chain::ExtF<PrmS>, PrmS>>;
     using VR = tstream::LaneVec;
-    int nb, nq, nlev, l;
+    int nb, nq, nlev, l, G;
     const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
     double* S;
     d3 grav; const double* fapp;
@@ -86,7 +86,8 @@ struct GroupEval {
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
         return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
     }
-    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group) : l(lane_in_group) {
+    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group, int lanes)
+        : l(lane_in_group), G(lanes) {
         const int4 ta = ldi4(a.T.tree_a, tr);
         nb = ta.x; nq = ta.y;
         const int body_off = ta.w;
@@ -252,16 +253,16 @@ struct GroupEval {
 // loop of the other tree evaluators, RungeKuttaMersonIntegratorRep.cpp:
 // 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
 // the group.  l: the lane's index in the group.
-template <int G, int IDF>
-__device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) {
+template <int IDF>
+__device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, int G) {
     const int tr = a.tree_of[inst];
-    const GroupEval<G, IDF> ev(a, inst, tr, l);
+    const GroupEval<IDF> ev(a, inst, tr, l, G);
     const int4 ta = ldi4(a.T.tree_a, tr);
     const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
     const int4* body_a = a.T.body_a + ta.w;
     const int nquat = ldi4(a.T.tree_b, tr).y;
     using VR = tstream::LaneVec;
-    auto rows = [&](int v) { return GroupEval<G, IDF>::vrow(a, v, inst); };
+    auto rows = [&](int v) { return GroupEval<IDF>::vrow(a, v, inst); };
     const VR Y = rows(VY), F0 = rows(VF0), FA = rows(VFA), FB = rows(VFB), YS = rows(VYS),
              Y1 = rows(VY1), YT = rows(VYT), YE = rows(VYE);
 
@@ -303,7 +304,7 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) 
             const double x = us * YE(nq + k);
             su += x*x;
         }
-        sq = grp_sum<G>(sq); su = grp_sum<G>(su);
+        sq = grp_sum(G, sq); su = grp_sum(G, su);
         const double qn = nq ? sqrt(sq / nq) : 0.0;
         const double un = nu ? sqrt(su / nu) : 0.0;
         return qn >= un ? qn : un;
@@ -359,7 +360,7 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) 
                     const double e = sqrt(n2) - 1.0;
                     s2 += e*e;
                 }
-                s2 = grp_sum<G>(s2);
+                s2 = grp_sum(G, s2);
                 const double qe = sqrt(s2 / nquat);
                 if (qe > projLimit) converged = false;
                 else if (qe > consTol) {
@@ -401,29 +402,27 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l) 
 // width of the instances' tree levels, so that the largest trees, which
 // bound a rollout's duration, get the most lanes); the wavefronts of the
 // widest trees come first in the grid.
-template <int G, int IDF, bool F0>
-__device__ __forceinline__ void plan_wave(const RkmArgs& a, int start, int count) {
-    const int grp = threadIdx.x / G, l = threadIdx.x & (G - 1);
+template <int IDF, bool F0>
+__device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, int count) {
+    const int lg = __builtin_ctz((unsigned)G);
+    const int grp = threadIdx.x >> lg, l = threadIdx.x & (G - 1);
     if (grp >= count) return;
     const int inst = a.plan_idx[start + grp];
     if constexpr (F0) {
-        const GroupEval<G, IDF> ev(a, inst, a.tree_of[inst], l);
-        ev.derivs(GroupEval<G, IDF>::vrow(a, VY, inst), GroupEval<G, IDF>::vrow(a, VF0, inst));
+        const GroupEval<IDF> ev(a, inst, a.tree_of[inst], l, G);
+        ev.derivs(GroupEval<IDF>::vrow(a, VY, inst), GroupEval<IDF>::vrow(a, VF0, inst));
     } else {
         if (a.status[inst] != 0) return;
-        grp_instance<G, IDF>(a, inst, l);
+        grp_instance<IDF>(a, inst, l, G);
     }
 }
+// One build for every G (4 to 64, a power of two, the same in the whole
+// wavefront): one copy of the evaluator's code serves all wavefronts.
 template <int IDF, bool F0>
 __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
     const int4 w = a.plan[blockIdx.x];
-    switch (w.x) {
-    case 4:  plan_wave<4, IDF, F0>(a, w.y, w.z); break;
-    case 8:  plan_wave<8, IDF, F0>(a, w.y, w.z); break;
-    case 16: plan_wave<16, IDF, F0>(a, w.y, w.z); break;
-    case 32: plan_wave<32, IDF, F0>(a, w.y, w.z); break;
-    default: plan_wave<64, IDF, F0>(a, w.y, w.z); break;
-    }
+    const int G = __builtin_amdgcn_readfirstlane(w.x);
+    plan_wave<IDF, F0>(a, G, w.y, w.z);
 }
 
 
