@@ -183,13 +183,13 @@ template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
 // Model.
 // ---------------------------------------------------------------------------
 enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8,
-                 GIMBAL=9, WELD=10 };
+                 GIMBAL=9, WELD=10, BENDSTRETCH=11 };
 inline int mobDof(int t) {
-    static const int d[11] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3, 0};
+    static const int d[12] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3, 0, 2};
     return d[t];
 }
 inline int mobNq(int t) {
-    static const int n[11] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3, 0};
+    static const int n[12] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3, 0, 2};
     return n[t];
 }
 
@@ -248,7 +248,7 @@ struct Model {
             const int p = d.parent[i];
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
-            if (t < PIN || t > WELD) return false;
+            if (t < PIN || t > BENDSTRETCH) return false;
             if (t == FREE && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
@@ -272,7 +272,7 @@ struct Model {
             p_BM[b] = X_BM.p;
             pitch[b] = d.pitch ? d.pitch[i] : 0.0;
             reversed[b] = d.reversed ? (d.reversed[i] != 0) : 0;
-            if (reversed[b] && t == FREE) return false;
+            if (reversed[b] && (t == FREE || t == BENDSTRETCH)) return false;
             X_MB[b] = invert(X_BM);      // RigidBodyNode.h:1012 X_MB(~X_BM)
         }
         int maxLevel = 0;
@@ -380,7 +380,7 @@ template <class T> inline void quatDotDot(const T q[4], const V3<T>& w, const V3
 // Position kinematics, base to tip.
 // RigidBodyNodeSpec.h:229-288 (realizePosition) with the mobilizer-specific
 // performQPrecalculations/calcX_FM/calcAcrossJointVelocityJacobian of
-// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free,Gimbal}.h, RigidBodyNode_Weld.cpp,
+// RigidBodyNodeSpec_{Pin,Slider,Universal,Ball,Free,Gimbal,PolarCoords}.h, RigidBodyNode_Weld.cpp,
 // calcBodyTransforms
 // (RigidBodyNodeSpec.h:554-569), calcParentToChildVelocityJacobianInGround
 // (RigidBodyNodeSpec.cpp:44-74) and calcJointIndependentKinematicsPos
@@ -451,6 +451,19 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
                 B.X_FM.p = mk3<T>(T(0), T(0), qb[0]*pt);
                 B.H_FM[0] = sv(ez, mk3<T>(T(0), T(0), pt));
             }
+        } break;
+        case BENDSTRETCH: {
+            // RigidBodyNodeSpec_PolarCoords.h:128-172: rotation q0 about z,
+            // then a slide q1 along the rotated x; H_FM(0) = (z, z x p_FM),
+            // H_FM(1) = (0, M's x axis in F)
+            B.cq[0] = cos(qb[0]); B.sq[0] = sin(qb[0]);
+            M33<T> R = zero33<T>();
+            R(0,0)=B.cq[0]; R(0,1)=-B.sq[0]; R(1,0)=B.sq[0]; R(1,1)=B.cq[0]; R(2,2)=T(1);
+            B.X_FM.R = R;
+            B.X_FM.p = R*mk3<T>(qb[1], T(0), T(0));
+            const V3<T> ez = mk3<T>(T(0),T(0),T(1));
+            B.H_FM[0] = sv(ez, cross(ez, B.X_FM.p));
+            B.H_FM[1] = sv(zero3<T>(), mk3<T>(R(0,0), R(1,0), R(2,0)));
         } break;
         case WELD:
             // no mobilities: X_FM = identity (RigidBodyNode_Weld.cpp:389-423);
@@ -591,6 +604,13 @@ template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* 
             if (rev) { w0 = zero3<T>(); for (int j=0;j<nd;++j) w0 = w0 + ub[j]*B.H_MF[j].a; }
             const V3<T> y = mk3<T>(R0(0,1), R0(1,1), R0(2,1));
             B.HDot_FM[1] = sv(cross(w0, y), zero3<T>());
+        } else if (t == BENDSTRETCH) {
+            // RigidBodyNodeSpec_PolarCoords.h:174-200: HDot_FM(0) = (0, z x v_FM),
+            // HDot_FM(1) = (0, w_FM x Mx)
+            const V3<T> ez = mk3<T>(T(0),T(0),T(1));
+            const V3<T> mx = mk3<T>(B.X_FM.R(0,0), B.X_FM.R(1,0), B.X_FM.R(2,0));
+            B.HDot_FM[0] = sv(zero3<T>(), cross(ez, B.V_FM.b));
+            B.HDot_FM[1] = sv(zero3<T>(), cross(B.V_FM.a, mx));
         } else if (t == GIMBAL) {
             // d/dt of H_FM with qdot = u (RigidBodyNodeSpec_Gimbal.h:180-202)
             const T c0=B.cq[0], c1=B.cq[1], s0=B.sq[0], s1=B.sq[1];

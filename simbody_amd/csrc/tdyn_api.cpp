@@ -33,12 +33,12 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     for (int b = 1; b <= d.nb; ++b) {
         const int p = d.parent[b-1], ty = d.mobilizer[b-1];
         if (p < 0 || p >= b) { snprintf(buf, sizeof buf, "tree %d body %d: parent %d must be in [0,%d)", t, b, p, b); return fail(TDYN_E_ARG, buf); }
-        if (ty < TDYN_PIN || ty > TDYN_WELD) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty < TDYN_PIN || ty > TDYN_BENDSTRETCH) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
         if (ty == TDYN_SCREW && (!d.screw_pitch || !std::isfinite(d.screw_pitch[b-1]))) {
             snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
         }
         if (ty == TDYN_FREE && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Free mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
-        if (ty == TDYN_FREE && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed Free mobilizer is not supported in this build", t, b); return fail(TDYN_E_ARG, buf); }
+        if ((ty == TDYN_FREE || ty == TDYN_BENDSTRETCH) && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed %s mobilizer is not supported in this build", t, b, ty == TDYN_FREE ? "Free" : "BendStretch"); return fail(TDYN_E_ARG, buf); }
         if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
         if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
             !finite_all(d.X_PF + 12*(b-1), 12) || !finite_all(d.X_BM + 12*(b-1), 12)) {
@@ -197,6 +197,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     for (int t = 0; t < n_trees; ++t) {
         B->any_rev = B->any_rev || any_reversed(trees[t]);
         B->any_euler = B->any_euler || trees[t].use_euler_angles != 0;
+        for (int b = 0; b < trees[t].nb; ++b) B->any_bendstretch = B->any_bendstretch || trees[t].mobilizer[b] == TDYN_BENDSTRETCH;
     }
     B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev && !B->any_euler;
     std::vector<int4> body_b, pre, bfs, bfs_b; std::vector<int> child_idx, lvl; std::vector<double> prm, grav, prm_bfs;
@@ -320,7 +321,7 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     std::vector<int> inst_of_lane(n_instances);
     for (int i = 0; i < n_instances; ++i) inst_of_lane[i] = i;
     if (!B->uniform) {
-        static const double cost[11] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9, 0.6};
+        static const double cost[12] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9, 0.6, 1.3};
         std::vector<double> tw(n_trees, 0.0);
         for (int t = 0; t < n_trees; ++t)
             for (int bdy = 0; bdy < trees[t].nb; ++bdy) tw[t] += cost[trees[t].mobilizer[bdy]];

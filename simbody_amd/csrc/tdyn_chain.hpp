@@ -159,7 +159,8 @@ template <int TYPE, class K> struct Joint {
     // (Euler-angle mode, RigidBodyNodeSpec_Ball.h:113-158, _Free.h:139-188);
     // ec keeps (c0, s0, c1, s1) of those angles for their rates.
     K ec[4];
-    template <bool CACHED>
+    // EXT: the build for extended models (BendStretch bodies possible).
+    template <bool CACHED, bool EXT = false>
     __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0), bool eul = false) {
         const K z = K(0), o = K(1);
         p_FM = zero3<K>();
@@ -258,6 +259,12 @@ template <int TYPE, class K> struct Joint {
                 } else if (rty == SCREW) {
                     p_FM = V3(z, z, K(qv[0])*pitch);
                     H_FM[0] = {V3(z, z, o), V3(z, z, pitch)};
+                } else if (EXT && rty == BENDSTRETCH) {
+                    // RigidBodyNodeSpec_PolarCoords.h:128-172: slide q1 along
+                    // the rotated x; H_FM(0) = (z, z x p_FM), H_FM(1) = (0, Mx)
+                    p_FM = mul(R_FM, V3(K(qv[1]), z, z));
+                    H_FM[0] = {V3(z, z, o), cross(V3(z, z, o), p_FM)};
+                    H_FM[1] = {zero3<K>(), col(R_FM, 0)};
                 }
             }
         } else if constexpr (TYPE == PIN) {
@@ -364,6 +371,10 @@ __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>&
                 hdw = cross(wj, hj);
             }
             if constexpr (REV) {
+                // BendStretch (PolarCoords.h:174-200, never reversed):
+                // HDot_FM(0) = (0, z x v_FM), HDot_FM(1) = (0, w_FM x Mx)
+                if (J.ty == BENDSTRETCH && j < 2)
+                    hdv = j == 0 ? cross(V3(K(0), K(0), K(1)), V_FM.v) : cross(w_FM, col(J.R_FM, 0));
                 if (J.rv) {
                     const v3<K> Hw = J.H_FM[j].w, Hv = J.H_FM[j].v;
                     hdw = cross(w_FM, Hw) - mul(J.R_FM, hdw);
@@ -532,7 +543,7 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
     }
     Joint<TYPE, K> J;
     K tc[4] = {K(0), K(0), K(0), K(0)};
-    J.template set<false>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), eul);
+    J.template set<false, PrmTraits<PA>::ext>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), eul);
     if constexpr (TYPE == GEN || TYPE == FREE) {
         if (eul && (TYPE == FREE || x.ty == BALL)) {
             // x-y-z angle rates N(q) w (RigidBodyNodeSpec_Ball.h:183-214,
@@ -612,7 +623,8 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), euler_of<TYPE>(P));
+    J.template set<true, PrmTraits<PA>::ext>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x),
+                                          euler_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
@@ -721,7 +733,8 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     const double* uv = x.u;
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
-    J.template set<true>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), euler_of<TYPE>(P));
+    J.template set<true, PrmTraits<PA>::ext>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x),
+                                          euler_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);

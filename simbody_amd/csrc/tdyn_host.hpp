@@ -57,6 +57,7 @@ struct tdyn_batch {
     bool uniform = true;
     bool any_rev = false;        // some tree has a reversed mobilizer (the batch runs padded generic bodies)
     bool any_euler = false;      // some tree uses Euler angles for Ball and Free (likewise)
+    bool any_bendstretch = false; // some tree has a BendStretch body
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
@@ -116,7 +117,7 @@ struct tdyn_batch {
         a.scratch32 = scratch32.p;
         a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.ext = (any_rev || any_euler) ? 1 : 0;
+        a.ext = (any_rev || any_euler || any_bendstretch) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

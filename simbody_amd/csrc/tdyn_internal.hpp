@@ -20,13 +20,13 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
 }
 
 enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8,
-       GIMBAL = 9, WELD = 10 };
+       GIMBAL = 9, WELD = 10, BENDSTRETCH = 11 };
 // Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
 enum { GEN = 15 };
 __host__ __device__ inline int mob_dof(int t) {
     switch (t) {
     case PIN: case SLIDER: case SCREW: return 1;
-    case UNIVERSAL: case CYLINDER: return 2;
+    case UNIVERSAL: case CYLINDER: case BENDSTRETCH: return 2;
     case BALL: case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     case WELD: return 0;
     default: return 6;
@@ -35,16 +35,18 @@ __host__ __device__ inline int mob_dof(int t) {
 __host__ __device__ inline int mob_nq(int t) {
     switch (t) {
     case PIN: case SLIDER: case SCREW: return 1;
-    case UNIVERSAL: case CYLINDER: return 2;
+    case UNIVERSAL: case CYLINDER: case BENDSTRETCH: return 2;
     case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     case BALL: return 4;
     case WELD: return 0;
     default: return 7;
     }
 }
-// Translation, Planar, Cylinder, Screw, Gimbal and Weld run only as padded
-// generic bodies; the typed code paths cover Pin, Slider, Universal, Ball and
-// Free.  A Weld body (no mobilities) has all three padded columns zero.
+// Translation, Planar, Cylinder, Screw, Gimbal, Weld and BendStretch run only
+// as padded generic bodies; the typed code paths cover Pin, Slider,
+// Universal, Ball and Free.  A Weld body (no mobilities) has all three padded
+// columns zero.  BendStretch (a linear HDot_FM term) runs in the builds for
+// extended models only (PrmTraits<>::ext, RkmArgs::ext).
 __host__ __device__ inline bool mob_generic_only(int t) { return t >= TRANSLATION; }
 
 // Per-body constants, PRM_STRIDE doubles per body (packed over all trees).
@@ -151,7 +153,8 @@ struct RkmArgs {
     int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
     int idf;                 // chain evaluator: every frame is an identity rotation at the joints
                              // (R_PF = I, p_PF = 0, R_MB = I), so the frame products are skipped
-    int ext;                 // some body's mobilizer is reversed (PRM_REV) or uses Euler angles (PRM_EULER)
+    int ext;                 // some body's mobilizer is reversed (PRM_REV), uses Euler angles (PRM_EULER)
+                             // or is a BendStretch
     const double* prm_lane;  // chain evaluator with one model per lane (serving slots), nullable:
                              // row k of lane l at prm_lane[k*L + l], nb*PRM_STRIDE model rows then gravity (3)
     int vec_aos;             // 1: vectors lane-contiguous (entry k of lane l at ((v*L + l)*ny_max + k)) and

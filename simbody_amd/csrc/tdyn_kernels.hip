@@ -275,7 +275,7 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
         H_FM[0] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
         H_FM[1] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
         H_FM[2] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
-    } else if (ty == PLANAR || ty == CYLINDER || ty == SCREW) {
+    } else if (ty == PLANAR || ty == CYLINDER || ty == SCREW || ty == BENDSTRETCH) {
         // RigidBodyNodeSpec_Planar.h:115-156, _Cylinder.h:100-139,
         // _Screw.h:109-148: rotation about z by q0, H_FM constant in F
         qv[0] = q(qi);
@@ -292,10 +292,16 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
             qv[1] = q(qi + 1);
             p_FM = V3(0.0, 0.0, qv[1]);
             H_FM[1] = {zero3<double>(), V3(0.0, 0.0, 1.0)};
-        } else {
+        } else if (ty == SCREW) {
             const double pt = P[PRM_PITCH];
             p_FM = V3(0.0, 0.0, qv[0]*pt);
             H_FM[0] = {V3(0.0, 0.0, 1.0), V3(0.0, 0.0, pt)};
+        } else {
+            // BendStretch (PolarCoords.h:128-172): slide q1 along the rotated x
+            qv[1] = q(qi + 1);
+            p_FM = mul(R_FM, V3(qv[1], 0.0, 0.0));
+            H_FM[0] = {V3(0.0, 0.0, 1.0), cross(V3(0.0, 0.0, 1.0), p_FM)};
+            H_FM[1] = {zero3<double>(), col(R_FM, 0)};
         }
     } else if (eul) {   // BALL with x-y-z angles (RigidBodyNodeSpec_Ball.h:113-158)
         qv[0] = q(qi); qv[1] = q(qi + 1); qv[2] = q(qi + 2);
@@ -390,6 +396,9 @@ __device__ __forceinline__ void kin_body_gen(const Lane& c, int b, const int4 ba
             if (j >= 1 && (ty == UNIVERSAL || ty == GIMBAL))
                 hdw = cross(j == 1 ? uv[0]*Hm[0] : uv[0]*Hm[0] + uv[1]*Hm[1], Hm[j]);
             d3 hdv = zero3<double>();
+            // BendStretch (PolarCoords.h:174-200, never reversed)
+            if (ty == BENDSTRETCH && j < 2)
+                hdv = j == 0 ? cross(V3(0.0, 0.0, 1.0), V_FM.v) : cross(w_FM, col(R_FM, 0));
             if (rv) {
                 // calcReverseMobilizerHDot_FM (RigidBodyNodeSpec.cpp:168-216)
                 const d3 Hw = H_FM[j].w, Hv = H_FM[j].v;

@@ -83,3 +83,16 @@ def weld_trees(seed, n, nmin=4, nmax=16, p_weld=0.3):
         w[0] = False
         t.mobilizer = np.where(w, M.WELD, t.mobilizer).astype(np.int32)
     return trees
+
+
+def bs_trees(seed, n, nmin=4, nmax=16, p_bs=0.3):
+    """ext_trees with each non-root body a BendStretch with probability p_bs
+    (and body 2 always one, when the tree has it)."""
+    trees = ext_trees(seed, n, nmin, nmax)
+    for i, t in enumerate(trees):
+        rng = np.random.default_rng(seed * 32452843 + i)
+        w = rng.uniform(size=t.nb) < p_bs
+        w[0] = False
+        w[1:2] = True
+        t.mobilizer = np.where(w, M.BENDSTRETCH, t.mobilizer).astype(np.int32)
+    return trees

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from common import ext_trees, rel_inf
+from common import bs_trees, ext_trees, rel_inf
 from simbody_amd import models as M
 from oracle_ctypes import OracleModel
 
@@ -150,3 +150,59 @@ def test_server_rejects_new_mobilizers_and_screw_needs_pitch():
     spec.pitch = None
     with pytest.raises(TdynError, match="screw_pitch"):
         MatterBatch([spec], n_instances=2)
+
+
+@pytest.mark.parametrize("kind", KINDS)
+def test_bendstretch_operators(kind):
+    """BendStretch bodies (linear HDot_FM) among the other mobilizers: every
+    operator against the oracle at the FP64 bar."""
+    from simbody_amd.matter import MatterBatch
+    if kind == "ragged":
+        per = bs_trees(71, 12, nmin=2, nmax=24)
+        B = MatterBatch(per, tree_of=np.arange(len(per), dtype=np.int32))
+    else:
+        spec = bs_trees(71, 1, nmin=20, nmax=20)[0]
+        per = [spec] * 40
+        B = MatterBatch([spec], n_instances=40)
+    qs, us, fs, fm, vs = [], [], [], [], []
+    for i, t in enumerate(per):
+        q, u = M.random_state(t, 71, i); qs.append(q); us.append(u)
+        fs.append(M.random_body_forces(t, 72, i))
+        r = np.random.default_rng(7100 + i)
+        fm.append(r.normal(size=t.nu)); vs.append(r.normal(size=t.nu))
+    q, u = B.pack(qs), B.pack(us)
+    X, V, qd, E = B.realizeKinematics(q, u)
+    ud, A, qdd = B.calcAccelerationIgnoringConstraints(q, u, B.pack(fm), B.pack(fs))
+    res = B.calcResidualForceIgnoringConstraints(q, u, B.pack(fm), B.pack(fs), B.pack(vs))
+    Ab = B.calcBodyAccelerationFromUDot(q, u, B.pack(vs))
+    Xs, Vs, uds, As = B.split_b(X, 12), B.split_b(V, 6), B.split_u(ud), B.split_b(A, 6)
+    ress, Abs = B.split_u(res), B.split_b(Ab, 6)
+    for i, t in enumerate(per):
+        o = OracleModel(t)
+        K = o.kinematics(qs[i], us[i])
+        assert rel_inf(Xs[i], K["X_GB"]) < TOL and rel_inf(Vs[i], K["V_GB"]) < TOL, i
+        f_ud, _, f_A, _ = o.forward(qs[i], us[i], fm[i], fs[i])
+        assert rel_inf(uds[i], f_ud) < TOL and rel_inf(As[i], f_A) < TOL, i
+        assert rel_inf(ress[i], o.residual(qs[i], us[i], fm[i], fs[i], vs[i])) < TOL, i
+        assert rel_inf(Abs[i], o.body_acceleration(qs[i], us[i], vs[i])) < TOL, i
+
+
+def test_bendstretch_rollouts_through_every_tree_evaluator():
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    trees = bs_trees(73, 10, nmin=4, nmax=30)
+    n = len(trees)
+    qs, us, fs = [], [], []
+    for i, t in enumerate(trees):
+        q, u = M.random_state(t, 73, i); qs.append(q); us.append(u)
+        fs.append(M.random_body_forces(t, 74, i))
+    ref = [OracleModel(t).integrate(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, Fapp=fs[i])
+           for i, t in enumerate(trees)]
+    for mode, lanes in ((1, 0), (0, 1), (0, 0), (3, 8)):
+        B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
+        B.set_evaluator(mode); B.set_tree_lanes(lanes)
+        q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
+        q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
+        for i in range(n):
+            y, s = ref[i]
+            assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (mode, lanes, i, s, st[i])
+            assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, (mode, lanes, i)

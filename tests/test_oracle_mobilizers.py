@@ -9,7 +9,7 @@ import copy
 import numpy as np
 import pytest
 
-from common import ext_trees, rel_inf, rot_to_quat, simtk_eq
+from common import bs_trees, ext_trees, rel_inf, rot_to_quat, simtk_eq
 from simbody_amd import models as M
 from oracle_ctypes import OracleModel
 
@@ -224,3 +224,65 @@ def test_gimbal_matches_ball_on_the_same_tree(idx):
                        [0, -s0 * ad[0], -c0 * ad[0] * c1 + s0 * s1 * ad[1]],
                        [0, c0 * ad[0], -s0 * ad[0] * c1 - c0 * s1 * ad[1]]])
         assert np.abs(_gimbal_h(a) @ add + Hd @ ad - udb[ubo:ubo + 3]).max() < 1e-10 * max(1, np.abs(udb).max())
+
+
+def test_bendstretch_kinematics_closed_form():
+    """A BendStretch body on Ground (identity frames): X_GB = (Rz(q0),
+    q1 (cos q0, sin q0, 0)); V_GB = (u0 z, u1 (c, s, 0) + q1 u0 (-s, c, 0))
+    (RigidBodyNodeSpec_PolarCoords.h:128-172)."""
+    o = OracleModel(_single(M.BENDSTRETCH))
+    q, u = np.array([0.8, 1.3]), np.array([-0.6, 0.9])
+    K = o.kinematics(q, u)
+    c, s = np.cos(q[0]), np.sin(q[0])
+    assert np.abs(K["X_GB"][1] - np.array([c, -s, 0, s, c, 0, 0, 0, 1, q[1] * c, q[1] * s, 0])).max() < 1e-15
+    v = u[1] * np.array([c, s, 0]) + q[1] * u[0] * np.array([-s, c, 0])
+    assert np.abs(K["V_GB"][1] - np.concatenate([[0, 0, u[0]], v])).max() < 1e-15
+    assert np.array_equal(K["qdot"], u)
+
+
+BS = bs_trees(151, 4, nmin=5, nmax=14)
+
+
+@pytest.mark.parametrize("idx", range(len(BS)))
+def test_bendstretch_trees_invariants_and_finite_differences(idx):
+    """Trees with BendStretch bodies (nonzero linear HDot_FM): the
+    mass-matrix invariants, and V_GB and A_GB against finite differences."""
+    spec = BS[idx]
+    assert (spec.mobilizer == M.BENDSTRETCH).any()
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 151, idx)
+    nu = o.nu
+    z = np.zeros(nu)
+    Mm = np.column_stack([o.residual(q, z, None, None, np.eye(nu)[i]) for i in range(nu)])
+    assert np.abs(Mm - Mm.T).max() <= 1e-13 * np.abs(Mm).max()
+    v = np.random.default_rng(idx).normal(size=nu)
+    assert simtk_eq(o.minv(q, Mm @ v), v, n=nu)
+    K = o.kinematics(q, u)
+    assert abs(0.5 * u @ Mm @ u - K["ke"]) <= 1e-12 * K["ke"]
+    qd = K["qdot"]
+    ud, _, A, _ = o.forward(q, u, np.zeros(nu), o.system_forces(q))
+    dt = 1e-6
+    Kp = o.kinematics(q + dt * qd, u + dt * ud)
+    Km = o.kinematics(q - dt * qd, u - dt * ud)
+    for b in range(1, o.nb + 1):
+        R = K["X_GB"][b, :9].reshape(3, 3)
+        dR = (Kp["X_GB"][b, :9] - Km["X_GB"][b, :9]).reshape(3, 3) / (2 * dt)
+        v_fd = (Kp["X_GB"][b, 9:] - Km["X_GB"][b, 9:]) / (2 * dt)
+        scale = max(1, np.abs(K["V_GB"][b]).max())
+        assert np.allclose(K["V_GB"][b], np.concatenate([_skew_inv(dR @ R.T), v_fd]), atol=1e-6 * scale), b
+        A_fd = (Kp["V_GB"][b] - Km["V_GB"][b]) / (2 * dt)
+        assert np.allclose(A[b], A_fd, atol=2e-5 * max(1, np.abs(A[b]).max())), b
+
+
+def test_bendstretch_energy_and_reversal_rejected():
+    spec = bs_trees(157, 1, nmin=8, nmax=12)[0]
+    o = OracleModel(spec)
+    q, u = M.random_state(spec, 157, 0)
+    K0 = o.kinematics(q, u)
+    y, st = o.integrate(np.concatenate([q, u]), 1.0, 1e-10)
+    assert st["status"] == 0 and st["accepted"] > 10
+    K1 = o.kinematics(y[:o.nq], y[o.nq:])
+    assert abs((K1["ke"] + K1["pe"]) - (K0["ke"] + K0["pe"])) < 1e-8 * max(abs(K0["ke"] + K0["pe"]), K0["ke"])
+    spec.reversed = (spec.mobilizer == M.BENDSTRETCH).astype(np.int32)
+    with pytest.raises(ValueError):
+        OracleModel(spec)
