@@ -150,6 +150,22 @@ int tdyn_calc_acceleration_ignoring_constraints(tdyn_batch* b, const double* q, 
                                                 const double* fmob, const double* F, double* udot,
                                                 double* A_GB, double* qdotdot, hipStream_t s);
 
+/* Forward dynamics with known udot on some bodies: prescribed motion at the
+ * acceleration level and locks (the isUDotKnown branches of
+ * RigidBodyNodeSpec.cpp:292-296, 355-446; what realize(Acceleration) does for
+ * a MobilizedBody with Motion::Acceleration or lock(state, Motion::Acceleration)).
+ * known: device int32, nb+1 per instance in the body layout of F (Ground
+ * first, ignored), nonzero = that body's udot is known; udot_known (packed u
+ * layout) holds those udots.  Out: udot (the known entries copied, the others
+ * computed), tau (on the known mobilities the generalized force that Simbody
+ * keeps on the left-hand side, M udot + c + tau = fmob + J^T F, i.e. minus
+ * the force the motion needs; zero on free mobilities), A_GB ((nb+1)*6 per
+ * instance, nullable).  fmob and F are nullable (zero). */
+int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const double* u,
+                                           const double* fmob, const double* F, const int32_t* known,
+                                           const double* udot_known, double* udot, double* tau,
+                                           double* A_GB, hipStream_t stream);
+
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
  * .cpp:431-471). */
 int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV,

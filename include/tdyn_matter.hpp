@@ -92,6 +92,15 @@ public:
                                      const double* appliedBodyForces, double* FM_G) const {
         check(tdyn_calc_mobilizer_reaction_forces(h_, q, u, appliedMobilityForces, appliedBodyForces, FM_G, s_));
     }
+    // Forward dynamics with known udot on the bodies flagged in `known`
+    // (prescribed motion at the acceleration level, locks; no single
+    // SimbodyMatterSubsystem call: what realize(Acceleration) does with
+    // Motion::Acceleration mobilizers).  See tdyn_calc_acceleration_with_known_udot.
+    void calcAccelerationWithKnownUDot(const double* q, const double* u, const double* fMob, const double* F,
+                                       const int32_t* known, const double* udotKnown, double* udot, double* tau,
+                                       double* A_GB = nullptr) const {
+        check(tdyn_calc_acceleration_with_known_udot(h_, q, u, fMob, F, known, udotKnown, udot, tau, A_GB, s_));
+    }
     void multiplyByM(const double* q, const double* v, double* Mv) const {
         check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
     }

@@ -97,6 +97,20 @@ int oracle_forward(void* p, const double* q, const double* u, const double* fmob
     return ok ? 0 : 1;
 }
 
+// Forward dynamics with known udot on the bodies marked in presc ([nb+1]);
+// udot in/out, tau out (zero on free mobilities).
+int oracle_forward_prescribed(void* p, const double* q, const double* u, const double* fmob, const double* Fb,
+                              const int* presc, double* udot, double* tau, double* A_GB) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizeAll(h, q, u);
+    const bool ok = realizeABI(tr, presc);
+    realizeABVelocity(tr);
+    std::vector<SV<double>> F; loadForces(M, Fb, F);
+    forwardDynamicsPrescribed(tr, presc, fmob, F.data(), udot, tau);
+    if (A_GB) for (int b=0;b<=M.nb;++b) storeSV(tr.B[b].A_GB, A_GB+6*b);
+    return ok ? 0 : 1;
+}
+
 // realize(Acceleration): gravity + optional constant applied forces.
 int oracle_realize_acceleration(void* p, const double* q, const double* u, const double* Fapp,
                                 double* udot, double* qdot, double* qdotdot) {

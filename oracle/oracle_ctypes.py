@@ -44,7 +44,7 @@ def lib():
                      "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
                      "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
                      "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
-                     "oracle_body_acceleration", "oracle_composite_inertias"]:
+                     "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -124,6 +124,16 @@ class OracleModel:
         ud = np.zeros(self.nu); qdd = np.zeros(self.nq); A = np.zeros((self.nb + 1, 6))
         rc = lib().oracle_forward(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(ud), _p(qdd), _p(A))
         return ud, qdd, A, rc
+
+    def forward_prescribed(self, q, u, fmob, F, presc, udot_known):
+        """Forward dynamics with known udot on the bodies where presc[b] != 0
+        (presc has nb+1 entries, Ground first): returns (udot, tau, A_GB, rc),
+        udot holding udot_known on the prescribed mobilities."""
+        ud = np.array(udot_known, np.float64, copy=True); tau = np.zeros(self.nu)
+        A = np.zeros((self.nb + 1, 6))
+        pr = np.ascontiguousarray(presc, np.int32)
+        rc = lib().oracle_forward_prescribed(self.h, _p(q), _p(u), _p(fmob), _p(F), _pi(pr), _p(ud), _p(tau), _p(A))
+        return ud, tau, A, rc
 
     def realize_acceleration(self, q, u, Fapp=None):
         ud = np.zeros(self.nu); qd = np.zeros(self.nq); qdd = np.zeros(self.nq)

@@ -746,7 +746,9 @@ template <class T> bool invertSmall(int n, const T* A, T* Ai) {
 // driver SimbodyMatterSubsystemRep.cpp:1217-1246.  Returns false if a D is
 // singular.
 // ---------------------------------------------------------------------------
-template <class T> bool realizeABI(Tree<T>& tr) {
+// presc (nullable, [nb+1]): bodies whose udot is known (prescribed motion or
+// a lock); their P+ = P (RigidBodyNodeSpec.cpp:292-296).
+template <class T> bool realizeABI(Tree<T>& tr, const int* presc = nullptr) {
     const Model& M = *tr.M;
     bool ok = true;
     for (int L = (int)M.levels.size()-1; L >= 1; --L)
@@ -755,6 +757,7 @@ template <class T> bool realizeABI(Tree<T>& tr) {
         const int nd = M.dof[b];
         B.P = abiFromRigid(B.Mk);
         for (int c : M.children[b]) abiAdd(B.P, abiShift(tr.B[c].PPlus, tr.B[c].phi));
+        if (presc && presc[b]) { B.PPlus = B.P; continue; }
         SV<T> PH[6];
         for (int j=0;j<nd;++j) PH[j] = abiMul(B.P, B.H[j]);
         for (int r=0;r<nd;++r) for (int c=0;c<nd;++c) B.D[r*nd+c] = dot(B.H[r], PH[c]);
@@ -845,6 +848,51 @@ template <class T> void forwardDynamics(Tree<T>& tr, const T* q, const T* u, con
                 else qdd[3] = T(0);
             } else for (int j=0;j<nd;++j) qdd[j] = ud[j];
         }
+    }
+}
+
+// Forward dynamics with known udot on the bodies marked in presc ([nb+1]):
+// calcUDotPass1Inward / calcUDotPass2Outward with their isUDotKnown branches
+// (RigidBodyNodeSpec.cpp:355-446).  udot holds the known entries on input and
+// receives the others; tau receives, on the prescribed mobilities,
+// eps - H^T P A+ (zero elsewhere): with tau on the left-hand side,
+// M udot + c + tau = f + J^T F, so the force the motion needs is -tau.
+// Needs realizeABI(tr, presc) and realizeABVelocity first.
+template <class T> void forwardDynamicsPrescribed(Tree<T>& tr, const int* presc, const T* fmob,
+                                                  const SV<T>* F, T* udot, T* tau) {
+    const Model& M = *tr.M;
+    for (int L = (int)M.levels.size()-1; L >= 1; --L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        B.z = B.PAb - F[b];
+        if (presc[b]) B.z = B.z + abiMul(B.P, Hmul(B.H, nd, udot + M.uIndex[b]));
+        for (int c : M.children[b]) B.z = B.z + phiMul(tr.B[c].phi, tr.B[c].zPlus);
+        for (int j=0;j<nd;++j) {
+            const T f = fmob ? fmob[M.uIndex[b]+j] : T(0);
+            B.eps[j] = f - dot(B.H[j], B.z);
+        }
+        B.zPlus = presc[b] ? B.z : B.z + Hmul(B.G, nd, B.eps);
+    }
+    for (size_t L = 1; L < M.levels.size(); ++L)
+    for (int b : M.levels[L]) {
+        Body<T>& B = tr.B[b];
+        const int nd = M.dof[b];
+        const SV<T> Ap = phiTMul(B.phi, tr.B[M.parent[b]].A_GB);
+        T* ud = udot + M.uIndex[b];
+        T* tb = tau + M.uIndex[b];
+        if (presc[b]) {
+            const SV<T> PA = abiMul(B.P, Ap);
+            for (int r=0;r<nd;++r) tb[r] = B.eps[r] - dot(B.H[r], PA);
+        } else {
+            for (int r=0;r<nd;++r) {
+                T s = T(0);
+                for (int c=0;c<nd;++c) s = s + B.DI[r*nd+c]*B.eps[c];
+                ud[r] = s - dot(B.G[r], Ap);
+                tb[r] = T(0);
+            }
+        }
+        B.A_GB = Ap + Hmul(B.H, nd, ud) + B.a;
     }
 }
 

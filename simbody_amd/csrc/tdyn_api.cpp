@@ -452,6 +452,19 @@ int tdyn_calc_acceleration_ignoring_constraints(tdyn_batch* b, const double* q, 
     return TDYN_OK;
 }
 
+int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const double* u,
+                                           const double* fmob, const double* F, const int32_t* known,
+                                           const double* udot_known, double* udot, double* tau,
+                                           double* A_GB, hipStream_t s) {
+    if (!b || !q || !u || !udot || !tau) return fail(TDYN_E_ARG, "calcAccelerationWithKnownUDot: batch, q, u, udot and tau are required");
+    if (!known || !udot_known) return fail(TDYN_E_ARG, "calcAccelerationWithKnownUDot: known and udot_known are required");
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fmob = fmob; a.fbody = F; a.presc = known; a.udot_in = udot_known;
+    a.out_u = udot; a.out_tau = tau; a.out_body = A_GB;
+    HIPCHK(launch_op(OP_FWD_PRES, a, s));
+    return TDYN_OK;
+}
+
 int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV, hipStream_t s) {
     if (!b || !q) return fail(TDYN_E_ARG, "multiplyByMInv: batch and q are required");
     if (!v || !MinvV) return fail(TDYN_E_SIZE, "multiplyByMInv: argument 'v' must have one entry per mobility");

@@ -103,7 +103,7 @@ struct RkmCfg {
 
 // Launch entry points (tdyn_kernels.hip).
 enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6,
-              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11 };
+              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11, OP_FWD_PRES = 12 };
 
 struct OpArgs {
     DevTables T;
@@ -128,6 +128,8 @@ struct OpArgs {
     double* out_scalar;      // kinematics: ke, pe per instance (2), nullable
     int* status;             // per instance, nullable
     const int* inst;         // instance of each lane (nullable: lane = instance)
+    const int* presc;        // OP_FWD_PRES: per body (nb+1 per instance) nonzero = udot known (udot_in)
+    double* out_tau;         // OP_FWD_PRES: tau on the known mobilities, 0 elsewhere (packed u layout)
 };
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s);
 

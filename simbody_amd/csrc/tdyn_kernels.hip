@@ -472,7 +472,11 @@ template <int D> __device__ __forceinline__ void inv_small(const double* A, doub
     }
 }
 
-enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8 };
+enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8, IN_PRES = 16 };
+// Known udot (prescribed motion at the acceleration level, locks): per-body
+// flags m (nb+1 per instance, Ground first), the known udot up and the tau
+// output (packed u layout), all offset to the instance.
+struct Pres { const int* m; const double* up; double* tau; };
 
 // ---------------------------------------------------------------------------
 // Tip-to-base: articulated-body inertias (RigidBodyNodeSpec.cpp:249-325) fused
@@ -485,7 +489,13 @@ template <int TYPE> struct MobD { static constexpr int D = TYPE == GEN ? 3 : Mob
 template <int TYPE, int MODE, class FB, class FM>
 __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
                                             const FB& fbody, const FM& fmob,
-                                            double* outP, double* outPP, const double* grav) {
+                                            double* outP, double* outPP, const double* grav, Pres pr = {}) {
+    // (IN_PRES) a body with known udot: P+ = P, z+ = z and z += P H udot; its
+    // G slot keeps P H and its DI*eps slot eps, so the outward pass forms
+    // tau = eps - (P H)^T A+ where it would form udot (RigidBodyNodeSpec.cpp:
+    // 292-296, 380-399, 434-443)
+    bool pres = false;
+    if constexpr ((MODE & IN_PRES) != 0) pres = pr.m[b] != 0;
     constexpr int D = MobD<TYPE>::D;
     const int d = TYPE == GEN ? mob_dof(ba.y) : D;
     const int rb = bb.z;
@@ -543,6 +553,13 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
     PP.J = P.J - ds{in[0], in[4], in[8], (in[3]+in[1])*0.5, (in[6]+in[2])*0.5, (in[7]+in[5])*0.5};
 #pragma unroll
     for (int i = 0; i < 9; ++i) PP.F.a[i] = P.F.a[i] - mm[i];
+    if constexpr ((MODE & IN_PRES) != 0) {
+        if (pres) {
+            PP = P;
+#pragma unroll
+            for (int j = 0; j < D; ++j) G[j] = PH[j];
+        }
+    }
     sta(c, rb + REC_PPLUS, PP);
 #pragma unroll
     for (int j = 0; j < D; ++j) stsv(c, rG + 6*j, G[j]);
@@ -562,6 +579,14 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
             }
             F = F + fbody(b);
             z = (abi_mul(P, a) + bg) - F;
+            if constexpr ((MODE & IN_PRES) != 0) {
+                if (pres) {
+                    dv Hu = svzero<double>();
+#pragma unroll
+                    for (int j = 0; j < D; ++j) if (j < d) Hu = Hu + pr.up[ba.w + j]*H[j];
+                    z = z + abi_mul(P, Hu);
+                }
+            }
         } else {
             z = svzero<double>();
         }
@@ -573,32 +598,34 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
 #pragma unroll
         for (int j = 0; j < D; ++j) eps[j] = (j < d ? fmob(ba.w + j) : 0.0) - dot(H[j], z);
         dv zp = z;
+        if (!pres) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) zp = zp + eps[j]*G[j];
+            for (int j = 0; j < D; ++j) zp = zp + eps[j]*G[j];
+        }
         stsv(c, rb + REC_ZPLUS, zp);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             double s = 0;
 #pragma unroll
             for (int k = 0; k < D; ++k) s += DI[r*D+k]*eps[k];
-            c.s(rb + REC_DIEPS + r) = s;
+            c.s(rb + REC_DIEPS + r) = pres ? eps[r] : s;
         }
     }
 }
 
 template <int MODE, bool PAD = false, class FB, class FM>
 __device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, double* outP, double* outPP,
-                             const double* grav) {
+                             const double* grav, Pres pr = {}) {
     for (int b = c.nb; b >= 1; --b) {
         const int4 ba = c.ba(b), bb = c.bb(b);
         if (PAD || mob_generic_only(ba.y)) {
-            if (ba.y != FREE) { inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); continue; }
+            if (ba.y != FREE) { inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); continue; }
         }
         switch (ba.y) {
-        case PIN: case SLIDER: inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
-        case UNIVERSAL:        inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
-        case BALL:             inward_body<BALL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
-        default:               inward_body<FREE, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav); break;
+        case PIN: case SLIDER: inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        case UNIVERSAL:        inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        case BALL:             inward_body<BALL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        default:               inward_body<FREE, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
         }
     }
 }
@@ -608,9 +635,9 @@ __device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, dou
 // WITH_COR) or multiplyByMInvPass2Outward (:521-552), plus calcQDotDot for
 // quaternions (Rotation.h:1006-1013).
 // ---------------------------------------------------------------------------
-template <int TYPE, bool WITH_COR, class QG, class UG, class UO, class QO>
+template <int TYPE, bool WITH_COR, bool PRES = false, class QG, class UG, class UO, class QO>
 __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba, const QG& q, const UG& u,
-                                             const UO& uo, const QO& qdd) {
+                                             const UO& uo, const QO& qdd, Pres pr = {}) {
     constexpr int D = MobD<TYPE>::D;
     const int d = TYPE == GEN ? mob_dof(ba.y) : D;
     const int rb = c.rec(b), p = ba.x;
@@ -621,6 +648,16 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
     dv A = Ap;
 #pragma unroll
     for (int r = 0; r < D; ++r) ud[r] = c.s(rb + REC_DIEPS + r) - dot(ldsv(c, rG + 6*r), Ap);
+    if constexpr (PRES) {
+        // a body with known udot: what was formed is tau; udot is the known one
+        const bool pres = pr.m[b] != 0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+            if (r >= d) { ud[r] = 0.0; continue; }
+            pr.tau[ba.w + r] = pres ? ud[r] : 0.0;
+            if (pres) ud[r] = pr.up[ba.w + r];
+        }
+    }
     dv Hu = svzero<double>();
 #pragma unroll
     for (int j = 0; j < D; ++j) Hu = Hu + ud[j]*ldsv(c, rH + 6*j);
@@ -663,18 +700,18 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
     }
 }
 
-template <bool WITH_COR, bool PAD = false, class QG, class UG, class UO, class QO>
-__device__ void sweep_outward2(const Lane& c, const QG& q, const UG& u, const UO& uo, const QO& qdd) {
+template <bool WITH_COR, bool PAD = false, bool PRES = false, class QG, class UG, class UO, class QO>
+__device__ void sweep_outward2(const Lane& c, const QG& q, const UG& u, const UO& uo, const QO& qdd, Pres pr = {}) {
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
         if (PAD || mob_generic_only(ba.y)) {
-            if (ba.y != FREE) { outward_body<GEN, WITH_COR>(c, b, ba, q, u, uo, qdd); continue; }
+            if (ba.y != FREE) { outward_body<GEN, WITH_COR, PRES>(c, b, ba, q, u, uo, qdd, pr); continue; }
         }
         switch (ba.y) {
-        case PIN: case SLIDER: outward_body<PIN, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
-        case UNIVERSAL:        outward_body<UNIVERSAL, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
-        case BALL:             outward_body<BALL, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
-        default:               outward_body<FREE, WITH_COR>(c, b, ba, q, u, uo, qdd); break;
+        case PIN: case SLIDER: outward_body<PIN, WITH_COR, PRES>(c, b, ba, q, u, uo, qdd, pr); break;
+        case UNIVERSAL:        outward_body<UNIVERSAL, WITH_COR, PRES>(c, b, ba, q, u, uo, qdd, pr); break;
+        case BALL:             outward_body<BALL, WITH_COR, PRES>(c, b, ba, q, u, uo, qdd, pr); break;
+        default:               outward_body<FREE, WITH_COR, PRES>(c, b, ba, q, u, uo, qdd, pr); break;
         }
     }
 }
@@ -862,6 +899,22 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
             const int inst = a.inst ? a.inst[lane] : lane;
             a.out_scalar[2*inst] = ke; a.out_scalar[2*inst+1] = pe;
         }
+    } else if constexpr (OP == OP_FWD_PRES) {
+        // forward dynamics with known udot on the flagged bodies: udot (known
+        // and computed) to out_u, tau to out_tau, A_GB to out_body
+        sweep_kin<true, PAD>(c, q, u, NoPut{});
+        const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
+        const PackedGetOpt fm{a.fmob ? a.fmob + uo : nullptr};
+        const Pres pr{a.presc + bo, a.udot_in + uo, a.out_tau + uo};
+        sweep_inward<IN_FWD | IN_PRES, PAD>(c, fb, fm, nullptr, nullptr, grav, pr);
+        sweep_outward2<true, PAD, true>(c, q, u, PackedPut{a.out_u + uo}, NoPut{}, pr);
+        if (a.out_body) {
+            for (int b = 0; b <= c.nb; ++b) {
+                const dv A = b == 0 ? svzero<double>() : ldsv(c, c.rec(b) + REC_AGB);
+                double* o = a.out_body + 6*(size_t)(bo + b);
+                o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
+            }
+        }
     } else if constexpr (OP == OP_FWD_SYS || OP == OP_FWD_GIVEN || OP == OP_REACT) {
         sweep_kin<true, PAD>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
@@ -1032,6 +1085,7 @@ static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     case OP_JACT:      k_op<OP_JACT, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_BACC:      k_op<OP_BACC, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_CBI:       k_op<OP_CBI, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_PRES:  k_op<OP_FWD_PRES, UNI><<<grd, blk, 0, s>>>(a); break;
     }
 }
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {

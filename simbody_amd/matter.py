@@ -31,7 +31,8 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_info",
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
-           "tdyn_calc_acceleration_ignoring_constraints", "tdyn_multiply_by_m_inv",
+           "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
+           "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
            "tdyn_multiply_by_system_jacobian_transpose", "tdyn_calc_body_acceleration_from_udot",
@@ -89,6 +90,7 @@ def lib():
         L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_realize_acceleration.argtypes = [vp] + [vp] * 5 + [vp]
         L.tdyn_calc_acceleration_ignoring_constraints.argtypes = [vp] + [vp] * 7 + [vp]
+        L.tdyn_calc_acceleration_with_known_udot.argtypes = [vp] + [vp] * 9 + [vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
         L.tdyn_calc_residual_force_ignoring_constraints.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_calc_articulated_body_inertias.argtypes = [vp] + [vp] * 3 + [vp]
@@ -303,6 +305,26 @@ class MatterBatch:
         _check(lib().tdyn_calc_acceleration_ignoring_constraints(
             self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(ud), _ptr(A), _ptr(qdd), _stream()))
         return ud, A, qdd
+
+    def calcAccelerationWithKnownUDot(self, q, u, fmob, F, known, udot_known, want_A=True):
+        """Forward dynamics with known udot on the bodies flagged in `known`
+        (int32 device tensor, nb+1 per instance in the body layout of F):
+        prescribed motion at the acceleration level and locks.  Returns
+        (udot, tau, A_GB); tau is nonzero on the known mobilities only, with
+        M udot + c + tau = fmob + J^T F."""
+        ud = self._new(self.total_u)
+        tau = self._new(self.total_u)
+        A = self._new(6 * self.total_b) if want_A else None
+        _check(lib().tdyn_calc_acceleration_with_known_udot(
+            self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(known), _ptr(udot_known), _ptr(ud), _ptr(tau),
+            _ptr(A), _stream()))
+        return ud, tau, A
+
+    def pack_body_flags(self, per_instance):
+        """Concatenate per-instance body flags (nb+1 each, Ground first) into
+        the packed int32 device layout of `known`."""
+        return self.torch.as_tensor(np.concatenate([np.asarray(x, np.int32).reshape(-1) for x in per_instance]),
+                                    dtype=self.torch.int32, device=self.device)
 
     def multiplyByMInv(self, q, v, out=None):
         out = self._new(self.total_u) if out is None else out
