@@ -61,3 +61,34 @@ def _create(L, s):
     d.mass, d.com_B, d.inertia_B, d.X_PF, d.X_BM = (a.ctypes.data_as(matter._d) for a in arrs[2:])
     h = ctypes.c_void_p()
     matter._check(L.tdyn_batch_create(ctypes.byref(d), 1, None, 4, ctypes.byref(h)))
+
+
+def test_extended_model_descriptions_are_checked_on_the_host():
+    """Mobilizer codes, Screw pitch, reversal of Free and BendStretch bodies:
+    rejected by tdyn_batch_create before any device work (no GPU needed)."""
+    import numpy as np
+    from simbody_amd import matter, models as M
+    if not os.path.exists(matter.LIB_PATH):
+        pytest.skip("libtdyn.so not built")
+    L = matter.lib()
+
+    def create(spec):
+        keep = []
+        d = matter.fill_tree_desc(matter.TreeDescC(), spec, keep)
+        h = ctypes.c_void_p()
+        matter._check(L.tdyn_batch_create(ctypes.byref(d), 1, None, 2, ctypes.byref(h)))
+
+    spec = M.spatial_chain(4, seed=2)
+    cases = [
+        ({"mobilizer": np.array([0, 12, 0, 0], np.int32)}, "unknown mobilizer"),
+        ({"mobilizer": np.array([0, M.SCREW, 0, 0], np.int32), "pitch": None}, "screw_pitch"),
+        ({"mobilizer": np.array([M.FREE, 0, 0, 0], np.int32), "reversed": np.array([1, 0, 0, 0], np.int32)},
+         "reversed Free"),
+        ({"mobilizer": np.array([0, M.BENDSTRETCH, 0, 0], np.int32), "reversed": np.array([0, 1, 0, 0], np.int32)},
+         "reversed BendStretch"),
+    ]
+    for change, msg in cases:
+        bad = M.TreeSpec(**{**spec.__dict__, **change})
+        with pytest.raises(matter.TdynError, match=msg) as e:
+            create(bad)
+        assert e.value.code == 1
