@@ -166,6 +166,15 @@ int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const
                                            const double* udot_known, double* udot, double* tau,
                                            double* A_GB, hipStream_t stream);
 
+/* Known udot for rollouts (locks: known udot 0 with the locked u at its
+ * value, usually 0; prescribed constant udot): the device arrays of
+ * tdyn_calc_acceleration_with_known_udot, kept by pointer (the caller keeps
+ * them alive) and used by every later tdyn_rollout_begin / advance / derivs
+ * of this batch; NULL, NULL clears.  No Simbody counterpart as a call: what
+ * MobilizedBody::lock(state) and Motion prescriptions do to the State.
+ * Such rollouts run on the tree sweeps (evaluator 0 or 1), in FP64. */
+int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double* udot_known);
+
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
  * .cpp:431-471). */
 int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV,

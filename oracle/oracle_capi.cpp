@@ -242,6 +242,20 @@ int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T
     return st.status;
 }
 
+// Integration with known udot on the bodies flagged in presc ([nb+1]): the
+// values udot_known ([nu]) on their mobilities at every derivative.
+int oracle_integrate_known(void* p, double* y, const double* Fapp, const int* presc, const double* udot_known,
+                           double t0, double T, double accuracy, double fixed_h, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    rk.ws.presc = presc; rk.ws.udotKnown = udot_known;
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
 // Same integration with the counting scalar.  tallies: addsub mul div sqrt trig.
 int oracle_count_flops_integrate(void* p, double* y, const double* Fapp, double t0, double T,
                                  double accuracy, double fixed_h, unsigned long long* tallies, long* stats) {

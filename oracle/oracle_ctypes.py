@@ -44,7 +44,8 @@ def lib():
                      "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
                      "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
                      "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
-                     "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed"]:
+                     "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
+                     "oracle_integrate_known"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -134,6 +135,18 @@ class OracleModel:
         pr = np.ascontiguousarray(presc, np.int32)
         rc = lib().oracle_forward_prescribed(self.h, _p(q), _p(u), _p(fmob), _p(F), _pi(pr), _p(ud), _p(tau), _p(A))
         return ud, tau, A, rc
+
+    def integrate_known(self, y0, T, accuracy, presc, udot_known, Fapp=None, fixed_h=0.0):
+        """integrate() with known udot on the bodies where presc[b] != 0."""
+        y = np.array(y0, np.float64, copy=True)
+        st = (ctypes.c_long * 6)()
+        pr = np.ascontiguousarray(presc, np.int32)
+        uk = np.ascontiguousarray(udot_known, np.float64)
+        Fa = None if Fapp is None else np.ascontiguousarray(Fapp, np.float64)
+        lib().oracle_integrate_known(self.h, _p(y), _p(Fa), _pi(pr), _p(uk), ctypes.c_double(0.0),
+                                     ctypes.c_double(T), ctypes.c_double(accuracy), ctypes.c_double(fixed_h), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return y, dict(zip(keys, list(st)))
 
     def realize_acceleration(self, q, u, Fapp=None):
         ud = np.zeros(self.nu); qd = np.zeros(self.nq); qdd = np.zeros(self.nq)

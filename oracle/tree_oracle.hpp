@@ -1083,6 +1083,10 @@ template <class T> void reactionForces(const Tree<T>& tr, SV<T>* FM) {
 template <class T> struct Workspace {
     std::vector<SV<T>> F;
     std::vector<T> tmp;
+    // known udot (locks, constant prescribed udot): presc [nb+1], udotKnown
+    // [nu]; null when none
+    const int* presc = nullptr;
+    const double* udotKnown = nullptr;
 };
 
 template <class T> bool derivs(Tree<T>& tr, Workspace<T>& ws, const T* y, const SV<T>* Fapp, T* ydot) {
@@ -1093,9 +1097,16 @@ template <class T> bool derivs(Tree<T>& tr, Workspace<T>& ws, const T* y, const 
     ws.F.assign(M.nb+1, svzero<T>());
     addGravity(tr, ws.F.data());
     if (Fapp) for (int b=1;b<=M.nb;++b) ws.F[b] = ws.F[b] + Fapp[b];
-    const bool ok = realizeABI(tr);
+    const bool ok = realizeABI(tr, ws.presc);
     realizeABVelocity(tr);
-    forwardDynamics(tr, q, u, (const T*)nullptr, ws.F.data(), ydot + M.nq, (T*)nullptr);
+    if (ws.presc) {
+        T* ud = ydot + M.nq;
+        for (int i=0;i<M.nu;++i) ud[i] = T(ws.udotKnown[i]);
+        ws.tmp.resize(M.nu);
+        forwardDynamicsPrescribed(tr, ws.presc, (const T*)nullptr, ws.F.data(), ud, ws.tmp.data());
+    } else {
+        forwardDynamics(tr, q, u, (const T*)nullptr, ws.F.data(), ydot + M.nq, (T*)nullptr);
+    }
     return ok;
 }
 

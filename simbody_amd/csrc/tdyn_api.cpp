@@ -375,6 +375,14 @@ int tdyn_batch_set_evaluator(tdyn_batch* b, int mode) {
     return TDYN_OK;
 }
 
+int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double* udot_known) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if ((known == nullptr) != (udot_known == nullptr))
+        return fail(TDYN_E_ARG, "tdyn_batch_set_known_udot: known and udot_known go together");
+    b->known_p = known; b->known_u = udot_known;
+    return TDYN_OK;
+}
+
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
     if (!(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64))
@@ -553,8 +561,10 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     { int prec = 0; const int rc = make_rkm_cfg(cfg, b->cfg, prec); if (rc) return rc; b->precision = prec; }
     if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
+    if (b->known_p && (b->evaluator == 2 || b->evaluator == 3 || b->precision == 1))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps (evaluator 0 or 1, FP64)");
     const size_t L = b->n_inst;
-    b->rollout_aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3);
+    b->rollout_aos = !b->known_p && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3);
     b->rollout_grp = 0;
     if (b->rollout_aos && b->tree_lanes != 1) {
         const int rc = build_tree_plan(b, s);

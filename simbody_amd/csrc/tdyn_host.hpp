@@ -58,6 +58,8 @@ struct tdyn_batch {
     bool any_rev = false;        // some tree has a reversed mobilizer (the batch runs padded generic bodies)
     bool any_euler = false;      // some tree uses Euler angles for Ball and Free (likewise)
     bool any_bendstretch = false; // some tree has a BendStretch body
+    const int* known_p = nullptr;     // tdyn_batch_set_known_udot: caller's device arrays (rollouts)
+    const double* known_u = nullptr;
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
@@ -115,7 +117,8 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (evaluator == 2 || (evaluator == 0 && is_chain)) ? 1 : 0;
+        a.chain = (!known_p && (evaluator == 2 || (evaluator == 0 && is_chain))) ? 1 : 0;
+        a.presc = known_p; a.udot_known = known_u;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
         a.ext = (any_rev || any_euler || any_bendstretch) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;

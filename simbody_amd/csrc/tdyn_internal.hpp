@@ -167,6 +167,8 @@ struct RkmArgs {
                              // plan_idx, instance count (<= 64/G)
     const int* plan_idx;     // instances (lane slots) in plan order
     int plan_waves;
+    const int* presc;        // known udot (tree sweeps only): per body (nb+1 per instance, packed) nonzero = known
+    const double* udot_known; // the known udots (packed u layout)
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that

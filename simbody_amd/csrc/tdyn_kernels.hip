@@ -654,7 +654,7 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
 #pragma unroll
         for (int r = 0; r < D; ++r) {
             if (r >= d) { ud[r] = 0.0; continue; }
-            pr.tau[ba.w + r] = pres ? ud[r] : 0.0;
+            if (pr.tau) pr.tau[ba.w + r] = pres ? ud[r] : 0.0;
             if (pres) ud[r] = pr.up[ba.w + r];
         }
     }
@@ -1112,15 +1112,23 @@ struct TreeEval {
         return VR{a.vec + (size_t)v * a.ny_max * a.L, a.L, (unsigned)lane};
     }
     Lane c; const double* grav; const double* fapp;
+    Pres pr;   // known udot (locks, constant prescribed udot): pr.m null when none
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
         : c(make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0)), grav(a.T.gravity + 4*tr),
-          fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr) {}
+          fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr),
+          pr{a.presc ? a.presc + a.off.b_off[lane] : nullptr, a.presc ? a.udot_known + a.off.u_off[lane] : nullptr,
+             nullptr} {}
     __device__ __forceinline__ void derivs(const chain::RowsT<double, false>& yr, const chain::RowsT<double, false>& fr) const {
         const SoAGet y{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
         const SoAPut f{fr.base, fr.L, fr.lane}, fu{fr.base + (size_t)c.nq * fr.L, fr.L, fr.lane};
         sweep_kin<true, PAD>(c, y, u, f);
-        sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
-        sweep_outward2<true, PAD>(c, y, u, fu, NoPut{});
+        if (pr.m) {
+            sweep_inward<IN_FWD | IN_SYSF | IN_PRES, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav, pr);
+            sweep_outward2<true, PAD, true>(c, y, u, fu, NoPut{}, pr);
+        } else {
+            sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+            sweep_outward2<true, PAD>(c, y, u, fu, NoPut{});
+        }
     }
 };
 

@@ -32,6 +32,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
+           "tdyn_batch_set_known_udot",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -91,6 +92,7 @@ def lib():
         L.tdyn_realize_acceleration.argtypes = [vp] + [vp] * 5 + [vp]
         L.tdyn_calc_acceleration_ignoring_constraints.argtypes = [vp] + [vp] * 7 + [vp]
         L.tdyn_calc_acceleration_with_known_udot.argtypes = [vp] + [vp] * 9 + [vp]
+        L.tdyn_batch_set_known_udot.argtypes = [vp, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
         L.tdyn_calc_residual_force_ignoring_constraints.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_calc_articulated_body_inertias.argtypes = [vp] + [vp] * 3 + [vp]
@@ -319,6 +321,13 @@ class MatterBatch:
             self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(known), _ptr(udot_known), _ptr(ud), _ptr(tau),
             _ptr(A), _stream()))
         return ud, tau, A
+
+    def set_known_udot(self, known=None, udot_known=None):
+        """Known udot for the following rollouts (locks, constant prescribed
+        udot): device tensors as for calcAccelerationWithKnownUDot, kept alive
+        by this object; None, None clears."""
+        self._known = (known, udot_known)
+        _check(lib().tdyn_batch_set_known_udot(self.h, _ptr(known), _ptr(udot_known)))
 
     def pack_body_flags(self, per_instance):
         """Concatenate per-instance body flags (nb+1 each, Ground first) into

@@ -56,3 +56,44 @@ def test_known_udot_matches_oracle(kind, lock):
             assert rel_inf(taus[i], o_tau) < TOL, i
         else:
             assert np.all(np.asarray(taus[i]) == 0.0), i
+
+
+@pytest.mark.parametrize("kind", ["ragged", "shared"])
+def test_rollouts_with_locks_and_known_udot(kind):
+    """Rollouts with known udot (locks on some instances' bodies, constant
+    prescribed udot on others) on the tree sweeps: the oracle's step counts
+    and states within 1e-8; other evaluators refuse such a batch."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson, TdynError
+    if kind == "ragged":
+        per = ext_trees(201, 10, nmin=3, nmax=24)
+        B = MatterBatch(per, tree_of=np.arange(len(per), dtype=np.int32))
+    else:
+        spec = M.spatial_chain(16, seed=11)
+        per = [spec] * 32
+        B = MatterBatch([spec], n_instances=32)
+    qs, us, fs, masks, known = [], [], [], [], []
+    for i, t in enumerate(per):
+        q, u = M.random_state(t, 201, i)
+        r = np.random.default_rng(20100 + i)
+        p = (r.uniform(size=t.nb + 1) < 0.3).astype(np.int32); p[0] = 0
+        m = _umask(t, p)
+        if i % 2 == 0:
+            u = np.where(m, 0.0, u); k = np.zeros(t.nu)          # locks
+        else:
+            k = np.where(m, r.normal(size=t.nu), 0.0)            # constant known udot
+        qs.append(q); us.append(u); masks.append(p); known.append(k)
+        fs.append(M.random_body_forces(t, 202, i))
+    kn, ku = B.pack_body_flags(masks), B.pack(known)
+    B.set_known_udot(kn, ku)
+    B.set_evaluator(3)
+    with pytest.raises(TdynError):
+        B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
+    B.set_evaluator(0)
+    q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
+    assert B.active_evaluator()[0] == 1
+    q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
+    for i, t in enumerate(per):
+        y, s = OracleModel(t).integrate_known(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, masks[i], known[i],
+                                              Fapp=fs[i])
+        assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
+        assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, i
