@@ -117,7 +117,7 @@ def _evaluator_name(B):
 
 
 def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_budget=6.0, lanes=0, lane_range=(0, 32),
-              evaluator=0):
+              evaluator=0, autotune=True):
     """Uneven branched trees (nb ~ U{8..128}, depth <= 20, Pin/Slider/
     Universal/Ball + Free roots), constant N(0,1) body forces and torques,
     per-instance RK-Merson step control at accuracy 1e-6 to t = 0.5 s, in
@@ -134,6 +134,11 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     q = B.pack([y[:t.nq] for y, t in zip(Ys, trees)]); u = B.pack([y[t.nq:] for y, t in zip(Ys, trees)])
     F = B.pack(Fs)
     integ = RungeKuttaMerson(accuracy=1e-6)
+    tuned = None
+    if autotune:
+        # probe rollouts of this state to t = 0.05 s pick the lanes per instance
+        log("[bench] trees: autotune")
+        tuned = B.autotune(q, u, integ, 0.05, fapp=F)
     torch.cuda.synchronize()
     log("[bench] trees: rollout")
     t0 = time.perf_counter()
@@ -148,6 +153,9 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
            "body_steps": body_steps, "accepted_mean": float(st[:, 1].mean()),
            "rejected_total": int(st[:, 2].sum()),
            "evaluator": _evaluator_name(B)}
+    if tuned is not None:
+        out["autotuned"] = {"mode": tuned[0], "lanes": tuned[1], "lanes_range": [tuned[2], tuned[3]],
+                            "note": "tdyn_batch_autotune, probe rollouts to t=0.05 s outside the timed region"}
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle_ctypes import OracleModel, time_integrate_batch
@@ -344,6 +352,8 @@ def main():
                     help="evaluator of the uneven-trees row: 0 automatic, 1 tree sweeps, 3 level-parallel")
     ap.add_argument("--tree-lane-range", type=int, nargs=2, default=[0, 32],
                     help="automatic lanes per instance: min (0: from the batch size) and max")
+    ap.add_argument("--no-tree-autotune", action="store_true",
+                    help="uneven-trees row: keep the given lane settings instead of tdyn_batch_autotune's choice")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
@@ -371,7 +381,10 @@ def main():
     if args.only == "trees":
         print(json.dumps(row_trees(torch, M, MatterBatch, RungeKuttaMerson, cpu=not args.no_cpu,
                                    lanes=args.tree_lanes, lane_range=tuple(args.tree_lane_range),
-                                   evaluator=args.tree_evaluator)), flush=True)
+                                   evaluator=args.tree_evaluator,
+                                   autotune=not args.no_tree_autotune and args.tree_lanes == 0
+                                   and args.tree_evaluator == 0 and tuple(args.tree_lane_range) == (0, 32))),
+              flush=True)
         return
     if args.only == "operators":
         print(json.dumps(row_operators(torch, M, MatterBatch, cpu=not args.no_cpu)), flush=True)

@@ -245,6 +245,22 @@ typedef struct tdyn_integrator_cfg {
                                     (chain evaluator only; TDYN_E_ARG otherwise) */
 } tdyn_integrator_cfg;
 
+/* Autotuner of the tree evaluators' occupancy (no Simbody counterpart: it
+ * picks how the batch maps onto the GPU).  For a batch of different trees it
+ * times short probe rollouts (0 to t_probe, each candidate twice, the faster
+ * kept) of the given state under fixed lanes per instance (8, 16, 32), ranges
+ * of the automatic choice ([4,16], [8,32], [16,32], [16,64]), the one-lane
+ * streaming evaluator and the tree sweeps, applies the fastest, and appends
+ * it to cache_path (nullable) under a key of the batch's shape (instances,
+ * mean bodies, depth, widest level, log2 buckets); a later call with a key
+ * already in the file applies the cached choice without probing.  Batches of
+ * one model, or with known udot, have nothing to choose.  chosen (nullable,
+ * 5 entries): evaluator mode, lanes, lane range min and max, 1 if it came
+ * from the cache.  q, u are not modified; call tdyn_rollout_begin after. */
+int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                        const tdyn_integrator_cfg* cfg, double t_probe, const char* cache_path,
+                        int32_t* chosen, hipStream_t stream);
+
 /* TimeStepper::initialize (Integrator.cpp:333-435): load q,u (packed),
  * constant applied body forces fapp (NULL: none), start time t0 and
  * per-instance final times t_end (device, NULL: all t_end_all).  The state

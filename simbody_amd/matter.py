@@ -32,7 +32,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
-           "tdyn_batch_set_known_udot",
+           "tdyn_batch_set_known_udot", "tdyn_batch_autotune",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -93,6 +93,8 @@ def lib():
         L.tdyn_calc_acceleration_ignoring_constraints.argtypes = [vp] + [vp] * 7 + [vp]
         L.tdyn_calc_acceleration_with_known_udot.argtypes = [vp] + [vp] * 9 + [vp]
         L.tdyn_batch_set_known_udot.argtypes = [vp, vp, vp]
+        L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
+                                          ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
         L.tdyn_calc_residual_force_ignoring_constraints.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_calc_articulated_body_inertias.argtypes = [vp] + [vp] * 3 + [vp]
@@ -321,6 +323,17 @@ class MatterBatch:
             self.h, _ptr(q), _ptr(u), _ptr(fmob), _ptr(F), _ptr(known), _ptr(udot_known), _ptr(ud), _ptr(tau),
             _ptr(A), _stream()))
         return ud, tau, A
+
+    def autotune(self, q, u, integ, t_probe, fapp=None, cache=None):
+        """Pick the tree evaluators' lanes per instance by timing short probe
+        rollouts of this state (tdyn_batch_autotune).  Returns (evaluator
+        mode, lanes, lanes_min, lanes_max, from_cache)."""
+        out = (ctypes.c_int32 * 5)()
+        cfg = integ.c()
+        _check(lib().tdyn_batch_autotune(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.byref(cfg),
+                                         ctypes.c_double(t_probe), cache.encode() if cache else None, out,
+                                         _stream()))
+        return tuple(out)
 
     def set_known_udot(self, known=None, udot_known=None):
         """Known udot for the following rollouts (locks, constant prescribed
