@@ -368,8 +368,8 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the instance batch would shard "
-                                   "across GPUs by instance (see DESIGN.md), no multi-GPU run is provided"}))
+        print(json.dumps({"error": "this build runs on one GPU only; the instances are independent and "
+                                   "would shard across GPUs by instance, but no multi-GPU run is provided"}))
         sys.exit(2)
 
     import torch

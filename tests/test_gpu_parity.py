@@ -1,6 +1,6 @@
 """Parity of the HIP path (libtdyn.so via the C ABI) with the oracle.
 
-FP64 tolerance (north star): 1e-12 relative (||d||_inf / ||ref||_inf per
+FP64 tolerance: 1e-12 relative (||d||_inf / ||ref||_inf per
 instance) on udot and residual forces; the other operator outputs are held to
 the same bound.  Trajectories: 1e-9 relative after short horizons (the GPU
 contracts a*b+c into FMA, the oracle does not, so single-step results differ
