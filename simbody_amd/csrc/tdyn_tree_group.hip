@@ -70,6 +70,21 @@ struct PrmS {
     __device__ __forceinline__ double operator[](int k) const { return p[k * tgroup::BW]; }
     __device__ __forceinline__ PrmS operator+(int k) const { return {p + k * tgroup::BW}; }
 };
+// A body's P+, z+ and phi for its parent (30 doubles), field k at p[k * s]:
+// in the LDS level frontier (s = G) or in the body's record (s = BW).
+struct PZS {
+    double* p; int s;
+    __device__ __forceinline__ double& operator[](int k) const { return p[k * s]; }
+};
+// LDS level frontier of the pass-2 hand-over: per group two level buffers
+// (the level being read and the level being written) of 30 fields x G
+// positions, field-major so the lanes of a group touch consecutive doubles.
+// 64/G groups x 2 x 30 x G doubles = 30 KB per wavefront for every G.  A
+// level wider than G keeps its hand-over in the records.  (The pose and A
+// hand-overs of passes 1 and 3 through the same frontier measured slower:
+// the records those passes write are read back by later passes anyway.)
+constexpr int kFrontierFields = 30;
+constexpr size_t kFrontierBytes = 64 * 2 * kFrontierFields * sizeof(double);
 
 template <int IDF>
 struct GroupEval {
@@ -79,6 +94,7 @@ struct GroupEval {
                                   std::conditional_t<IDF == 2, chain::ExtF<PrmS>, PrmS>>;
     using VR = tstream::LaneVec;
     int nb, nq, nlev, l, G;
+    double* lf;        // this group's LDS frontier (2 x 30 x G doubles)
     const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
     double* S;
     d3 grav; const double* fapp;
@@ -86,8 +102,8 @@ struct GroupEval {
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
         return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
     }
-    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group, int lanes)
-        : l(lane_in_group), G(lanes) {
+    __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group, int lanes, double* frontier)
+        : l(lane_in_group), G(lanes), lf(frontier) {
         const int4 ta = ldi4(a.T.tree_a, tr);
         nb = ta.x; nq = ta.y;
         const int body_off = ta.w;
@@ -101,6 +117,14 @@ struct GroupEval {
         free0 = ldi4(a.T.body_a, body_off).y == FREE;
     }
     __device__ __forceinline__ RecS rec(int j) const { return {S + tgroup::at(j)}; }
+    // P+ z+ phi of position j of level lev: the LDS frontier when the level
+    // fits it (at most G positions, the same answer in every lane of the
+    // group), else the record
+    __device__ __forceinline__ PZS pz(int j, int lev) const {
+        const int s0 = ls[lev];
+        if (ls[lev + 1] - s0 <= G) return {lf + (size_t)((lev & 1) * kFrontierFields) * G + (j - s0), G};
+        return {S + tgroup::at(j) + tgroup::R_PZ * tgroup::BW, tgroup::BW};
+    }
     __device__ __forceinline__ StoreS store(int j) const { return {S + tgroup::at(j)}; }
     __device__ __forceinline__ PA prm_of(int j) const {
         return PA(PrmS{prm + (j / tgroup::BW) * PRM_STRIDE * tgroup::BW + j % tgroup::BW});
@@ -196,7 +220,7 @@ struct GroupEval {
                     for (int k = 0; k < 9; ++k) aP.F.a[k] = 0.0;
                     dv aZ = svzero<double>();
                     for (int c = e.z; c < e.z + e.w; ++c) {
-                        const RecS q{S + tgroup::at(c) + tgroup::R_PZ * tgroup::BW};
+                        const PZS q = pz(c, lev + 1);
                         da Q;
                         Q.M = {q[0], q[1], q[2], q[3], q[4], q[5]};
                         Q.J = {q[6], q[7], q[8], q[9], q[10], q[11]};
@@ -213,7 +237,7 @@ struct GroupEval {
                     pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic);
                 }
                 if (e.y >= 0) {
-                    const RecS q{S + tgroup::at(j) + tgroup::R_PZ * tgroup::BW};
+                    const PZS q = pz(j, lev);
                     q[0] = PP.M.xx; q[1] = PP.M.yy; q[2] = PP.M.zz; q[3] = PP.M.xy; q[4] = PP.M.xz; q[5] = PP.M.yz;
                     q[6] = PP.J.xx; q[7] = PP.J.yy; q[8] = PP.J.zz; q[9] = PP.J.xy; q[10] = PP.J.xz; q[11] = PP.J.yz;
 #pragma unroll
@@ -254,9 +278,9 @@ struct GroupEval {
 // 80-160 and AbstractIntegratorRep.cpp:448-502), its vector work split over
 // the group.  l: the lane's index in the group.
 template <int IDF>
-__device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, int G) {
+__device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, int G, double* frontier) {
     const int tr = a.tree_of[inst];
-    const GroupEval<IDF> ev(a, inst, tr, l, G);
+    const GroupEval<IDF> ev(a, inst, tr, l, G, frontier);
     const int4 ta = ldi4(a.T.tree_a, tr);
     const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
     const int4* body_a = a.T.body_a + ta.w;
@@ -404,16 +428,18 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
 // widest trees come first in the grid.
 template <int IDF, bool F0>
 __device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, int count) {
+    extern __shared__ double lds_frontier[];
     const int lg = __builtin_ctz((unsigned)G);
     const int grp = threadIdx.x >> lg, l = threadIdx.x & (G - 1);
     if (grp >= count) return;
     const int inst = a.plan_idx[start + grp];
+    double* frontier = lds_frontier + (size_t)grp * 2 * kFrontierFields * G;
     if constexpr (F0) {
-        const GroupEval<IDF> ev(a, inst, a.tree_of[inst], l, G);
+        const GroupEval<IDF> ev(a, inst, a.tree_of[inst], l, G, frontier);
         ev.derivs(GroupEval<IDF>::vrow(a, VY, inst), GroupEval<IDF>::vrow(a, VF0, inst));
     } else {
         if (a.status[inst] != 0) return;
-        grp_instance<IDF>(a, inst, l, G);
+        grp_instance<IDF>(a, inst, l, G, frontier);
     }
 }
 // One build for every G (4 to 64, a power of two, the same in the whole
@@ -436,14 +462,14 @@ hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);
     if (a.ext) {
-        if (f0_only) k_rkm_plan<2, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_plan<2, false><<<grd, blk, 0, s>>>(a);
+        if (f0_only) k_rkm_plan<2, true><<<grd, blk, kFrontierBytes, s>>>(a);
+        else k_rkm_plan<2, false><<<grd, blk, kFrontierBytes, s>>>(a);
     } else if (a.idf) {
-        if (f0_only) k_rkm_plan<1, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_plan<1, false><<<grd, blk, 0, s>>>(a);
+        if (f0_only) k_rkm_plan<1, true><<<grd, blk, kFrontierBytes, s>>>(a);
+        else k_rkm_plan<1, false><<<grd, blk, kFrontierBytes, s>>>(a);
     } else {
-        if (f0_only) k_rkm_plan<0, true><<<grd, blk, 0, s>>>(a);
-        else k_rkm_plan<0, false><<<grd, blk, 0, s>>>(a);
+        if (f0_only) k_rkm_plan<0, true><<<grd, blk, kFrontierBytes, s>>>(a);
+        else k_rkm_plan<0, false><<<grd, blk, kFrontierBytes, s>>>(a);
     }
     return hipGetLastError();
 }
