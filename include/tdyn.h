@@ -172,7 +172,9 @@ int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const
  * them alive) and used by every later tdyn_rollout_begin / advance / derivs
  * of this batch; NULL, NULL clears.  No Simbody counterpart as a call: what
  * MobilizedBody::lock(state) and Motion prescriptions do to the State.
- * Such rollouts run on the tree sweeps (evaluator 0 or 1), in FP64. */
+ * Such rollouts run in FP64 on the tree sweeps or, for a batch of different
+ * trees, on the level-parallel tree evaluator (evaluator 0 or 3 with lanes
+ * other than 1). */
 int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double* udot_known);
 
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
@@ -254,7 +256,8 @@ typedef struct tdyn_integrator_cfg {
  * it to cache_path (nullable) under a key of the batch's shape (instances,
  * mean bodies, depth, widest level, log2 buckets); a later call with a key
  * already in the file applies the cached choice without probing.  Batches of
- * one model, or with known udot, have nothing to choose.  chosen (nullable,
+ * one model have nothing to choose; with known udot the one-lane streaming
+ * evaluator is not a candidate.  chosen (nullable,
  * 5 entries): evaluator mode, lanes, lane range min and max, 1 if it came
  * from the cache.  q, u are not modified; call tdyn_rollout_begin after. */
 int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const double* fapp,

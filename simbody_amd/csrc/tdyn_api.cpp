@@ -561,10 +561,11 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     { int prec = 0; const int rc = make_rkm_cfg(cfg, b->cfg, prec); if (rc) return rc; b->precision = prec; }
     if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
-    if (b->known_p && (b->evaluator == 2 || b->evaluator == 3 || b->precision == 1))
-        return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps (evaluator 0 or 1, FP64)");
+    if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || b->precision == 1))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps or the level-parallel "
+                                "tree evaluator (not one lane per instance), in FP64");
     const size_t L = b->n_inst;
-    b->rollout_aos = !b->known_p && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3);
+    b->rollout_aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) && !(b->known_p && b->tree_lanes == 1);
     b->rollout_grp = 0;
     if (b->rollout_aos && b->tree_lanes != 1) {
         const int rc = build_tree_plan(b, s);

@@ -34,7 +34,7 @@ std::string shape_key(const tdyn_batch* b) {
     char buf[160];
     snprintf(buf, sizeof buf, "tree-lanes n%d nb%d depth%d width%d trees%d ext%d", ilog2(b->n_inst),
              ilog2((double)b->total_b / b->n_inst - 1.0), ilog2(b->dmax + 1.0), ilog2(w),
-             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch) ? 1 : 0);
+             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch || b->known_p) ? 1 : 0);
     return buf;
 }
 
@@ -106,13 +106,16 @@ int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const d
     if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_batch_autotune: batch, q, u and cfg are required");
     if (!(t_probe > 0)) return fail(TDYN_E_ARG, "tdyn_batch_autotune: t_probe must be positive");
     std::vector<Candidate> cands;
-    if (!b->stream_ok || b->known_p) {
-        // one model for the batch (chain evaluator or tree sweeps), or known
-        // udot (tree sweeps): nothing to choose
+    if (!b->stream_ok) {
+        // one model for the batch (chain evaluator or tree sweeps): nothing
+        // to choose
         cands.push_back({0, 0, 0, 32});
     } else {
         cands = {{3, 0, 16, 32}, {3, 0, 8, 32}, {3, 0, 16, 64}, {3, 0, 4, 16},
                  {3, 8, 0, 32}, {3, 16, 0, 32}, {3, 32, 0, 32}, {3, 1, 0, 32}, {1, 0, 0, 32}};
+        // known udot: not on the one-lane streaming evaluator
+        if (b->known_p) cands.erase(std::remove_if(cands.begin(), cands.end(),
+                                                   [](const Candidate& c) { return c.lanes == 1; }), cands.end());
     }
     const std::string key = shape_key(b);
     Candidate best = cands[0];

@@ -618,7 +618,13 @@ template <int TYPE, class K, class PA, class ST>
 __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v3<K>& grav, const sv<K>& Fapp,
                                            const ST& st, m33<K>& R, v3<K>& p, sv<K>& V, bool hasChild,
                                            abi<K>& PP, sv<K>& zp, v3<K>& phic,
-                                           const abi<K>* accP = nullptr, const sv<K>* accZ = nullptr) {
+                                           const abi<K>* accP = nullptr, const sv<K>* accZ = nullptr,
+                                           const double* ukn = nullptr) {
+    // ukn: the body's known udot (locks, constant prescribed udot), or null.
+    // Such a body hands its parent P+ = P and z+ = z with z += P H udot
+    // (RigidBodyNodeSpec.cpp:292-296, 380-399) and keeps G = 0 and udot in
+    // its DI*eps slot, so pass 3 forms udot = DI*eps - G^T A+ = udot exactly
+    // (:434-443).
     constexpr int D = MobC<TYPE>::D;
     const double* uv = x.u;
     Joint<TYPE, K> J;
@@ -678,6 +684,12 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
         G[k] = tok<K>(acc);
     }
     sv<K> z = (abi_mul(Pb, a) + bg) - F;
+    if (ukn) {
+        sv<K> Hu = svzero<K>();
+#pragma unroll
+        for (int j = 0; j < D; ++j) if (TYPE != GEN || j < J.d) Hu = Hu + K(ukn[j])*H[j];
+        z = z + abi_mul(Pb, Hu);
+    }
     if (accZ) z = z + *accZ;
     if (hasChild) z = z + phi_mul(phic, zp);
     double eps[D];
@@ -708,6 +720,11 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
 #pragma unroll
     for (int i = 0; i < 9; ++i) PP.F.a[i] = Pb.F.a[i] - mm[i];
     zp = zn; phic = phi;
+    if (ukn) {
+        PP = Pb; zp = z;
+#pragma unroll
+        for (int j = 0; j < D; ++j) G[j] = svzero<K>();
+    }
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         st.G(6*j + 0) = G[j].w.x; st.G(6*j + 1) = G[j].w.y; st.G(6*j + 2) = G[j].w.z;
@@ -718,6 +735,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
         double sacc = 0;
 #pragma unroll
         for (int k = 0; k < D; ++k) sacc += DI[r*D+k]*eps[k];
+        if (ukn) sacc = (TYPE != GEN || r < J.d) ? ukn[r] : 0.0;
         st.DE(r) = sacc;
     }
     R = R_GP; p = p_GP; V = V_GP;

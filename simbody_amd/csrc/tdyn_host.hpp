@@ -120,7 +120,7 @@ struct tdyn_batch {
         a.chain = (!known_p && (evaluator == 2 || (evaluator == 0 && is_chain))) ? 1 : 0;
         a.presc = known_p; a.udot_known = known_u;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.ext = (any_rev || any_euler || any_bendstretch) ? 1 : 0;
+        a.ext = (any_rev || any_euler || any_bendstretch || known_p) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

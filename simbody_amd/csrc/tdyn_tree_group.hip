@@ -98,6 +98,7 @@ struct GroupEval {
     const int4* bfs; const int4* bfs_b; const int* ls; const double* prm;
     double* S;
     d3 grav; const double* fapp;
+    const int* pm; const double* uk;   // known udot: per-body flags (Ground first) and values, or null
     bool free0;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
         return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
@@ -114,6 +115,8 @@ struct GroupEval {
         const double* g = a.T.gravity + 4*tr;
         grav = V3(g[0], g[1], g[2]);
         fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[inst] : nullptr;
+        pm = (IDF == 2 && a.presc) ? a.presc + a.off.b_off[inst] : nullptr;
+        uk = pm ? a.udot_known + a.off.u_off[inst] : nullptr;
         free0 = ldi4(a.T.body_a, body_off).y == FREE;
     }
     __device__ __forceinline__ RecS rec(int j) const { return {S + tgroup::at(j)}; }
@@ -211,6 +214,8 @@ struct GroupEval {
                 const StoreS st = store(j);
                 Pre<double> x;
                 load<TY, false>(x, y, bt, st);
+                const double* ukn = nullptr;      // known udot (extended build only)
+                if constexpr (IDF == 2) { if (pm && pm[bt.w]) ukn = uk + bt.z; }
                 da PP; dv zp = svzero<double>(); d3 phic = zero3<double>();
                 if (e.w > 0) {
                     // children's P+ and z+, shifted to this body, in child order
@@ -232,9 +237,10 @@ struct GroupEval {
                         aZ = aZ + phi_mul(ph, zc);
                     }
                     pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic,
-                                           &aP, &aZ);
+                                           &aP, &aZ, ukn);
                 } else {
-                    pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic);
+                    pass2_body<TY, double>(prm_of(j), x, grav, F(bt.w), st, R, p, V, false, PP, zp, phic,
+                                           nullptr, nullptr, ukn);
                 }
                 if (e.y >= 0) {
                     const PZS q = pz(j, lev);
@@ -457,7 +463,8 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
 // a.idf: every tree's joint frames are identity rotations (Simbody's
 // noR_PF/noX_MB node forms), the frame products are skipped; the results
 // are bitwise those of the general form.  a.ext: some mobilizer is reversed
-// or uses Euler angles; only that build carries the code for them.
+// or uses Euler angles, or some bodies have known udot; only that build
+// carries the code for them.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);

@@ -58,12 +58,8 @@ def test_known_udot_matches_oracle(kind, lock):
             assert np.all(np.asarray(taus[i]) == 0.0), i
 
 
-@pytest.mark.parametrize("kind", ["ragged", "shared"])
-def test_rollouts_with_locks_and_known_udot(kind):
-    """Rollouts with known udot (locks on some instances' bodies, constant
-    prescribed udot on others) on the tree sweeps: the oracle's step counts
-    and states within 1e-8; other evaluators refuse such a batch."""
-    from simbody_amd.matter import MatterBatch, RungeKuttaMerson, TdynError
+def _known_case(kind):
+    from simbody_amd.matter import MatterBatch
     if kind == "ragged":
         per = ext_trees(201, 10, nmin=3, nmax=24)
         B = MatterBatch(per, tree_of=np.arange(len(per), dtype=np.int32))
@@ -83,14 +79,55 @@ def test_rollouts_with_locks_and_known_udot(kind):
             k = np.where(m, r.normal(size=t.nu), 0.0)            # constant known udot
         qs.append(q); us.append(u); masks.append(p); known.append(k)
         fs.append(M.random_body_forces(t, 202, i))
+    return per, B, qs, us, fs, masks, known
+
+
+@pytest.mark.parametrize("kind", ["ragged", "shared"])
+def test_rollouts_with_locks_and_known_udot(kind):
+    """Rollouts with known udot (locks on some instances' bodies, constant
+    prescribed udot on others) on every evaluator that takes them: the tree
+    sweeps and, for different trees, the level-parallel evaluator at fixed
+    and automatic lanes.  The oracle's step counts and states within 1e-8;
+    the chain evaluator and the one-lane streaming evaluator refuse such a
+    batch."""
+    from simbody_amd.matter import RungeKuttaMerson, TdynError
+    per, B, qs, us, fs, masks, known = _known_case(kind)
     kn, ku = B.pack_body_flags(masks), B.pack(known)
     B.set_known_udot(kn, ku)
-    B.set_evaluator(3)
-    with pytest.raises(TdynError):
-        B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
-    B.set_evaluator(0)
-    q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6), 0.1, fapp=B.pack(fs))
-    assert B.active_evaluator()[0] == 1
+    rkm = RungeKuttaMerson(accuracy=1e-6)
+    for mode, lanes in ((3, 1), (2, 0)):
+        with pytest.raises(TdynError):       # (a batch of different trees refuses mode 2 at once)
+            B.set_tree_lanes(lanes); B.set_evaluator(mode)
+            B.stepTo(B.pack(qs), B.pack(us), rkm, 0.1, fapp=B.pack(fs))
+    ref = [OracleModel(t).integrate_known(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, masks[i], known[i],
+                                          Fapp=fs[i]) for i, t in enumerate(per)]
+    runs = ((1, 0, 1), (0, 0, 4), (3, 8, 4), (3, 0, 4)) if kind == "ragged" else ((1, 0, 1), (0, 0, 1), (3, 0, 1))
+    for mode, lanes, active in runs:
+        B.set_evaluator(mode); B.set_tree_lanes(lanes)
+        q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), rkm, 0.1, fapp=B.pack(fs))
+        assert B.active_evaluator()[0] == active, (mode, lanes)
+        q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
+        for i, t in enumerate(per):
+            y, s = ref[i]
+            assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (mode, lanes, i, s, st[i])
+            assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, (mode, lanes, i)
+            lk = _umask(t, masks[i])
+            if i % 2 == 0:
+                assert np.array_equal(u1s[i][lk], us[i][lk]), (mode, lanes, i)     # locked u stay put
+
+
+def test_autotune_with_known_udot():
+    """The autotuner probes the level-parallel settings and the tree sweeps
+    for a batch with known udot (not the one-lane streaming evaluator), and
+    the rollout it sets up still matches the oracle."""
+    from simbody_amd.matter import RungeKuttaMerson
+    per, B, qs, us, fs, masks, known = _known_case("ragged")
+    B.set_known_udot(B.pack_body_flags(masks), B.pack(known))
+    rkm = RungeKuttaMerson(accuracy=1e-6)
+    chosen = B.autotune(B.pack(qs), B.pack(us), rkm, 0.02, fapp=B.pack(fs))
+    assert chosen[1] != 1 and chosen[4] == 0, chosen
+    q1, u1, st = B.stepTo(B.pack(qs), B.pack(us), rkm, 0.1, fapp=B.pack(fs))
+    assert B.active_evaluator()[0] in (1, 4)
     q1s, u1s, st = B.split_q(q1), B.split_u(u1), st.cpu().numpy()
     for i, t in enumerate(per):
         y, s = OracleModel(t).integrate_known(np.concatenate([qs[i], us[i]]), 0.1, 1e-6, masks[i], known[i],
