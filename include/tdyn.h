@@ -48,7 +48,8 @@ extern "C" {
 
 /* Mobilizer codes: MobilizedBody::Pin, Slider, Universal, Ball, Free,
  * Translation, Planar, Cylinder, Screw, Gimbal (RigidBodyNodeSpec_<Type>.h), Weld
- * (RigidBodyNode_Weld.cpp), BendStretch (RigidBodyNodeSpec_PolarCoords.h). */
+ * (RigidBodyNode_Weld.cpp), BendStretch (RigidBodyNodeSpec_PolarCoords.h),
+ * Bushing (RigidBodyNodeSpec_Bushing.h). */
 #define TDYN_PIN         0
 #define TDYN_SLIDER      1
 #define TDYN_UNIVERSAL   2
@@ -61,13 +62,15 @@ extern "C" {
 #define TDYN_GIMBAL      9   /* q = body-fixed x-y-z angles, u = qdot (singular at q1 = +-pi/2) */
 #define TDYN_WELD       10   /* no q, no u: the body is fixed to its parent */
 #define TDYN_BENDSTRETCH 11  /* q = (angle about z, slide along the rotated x), u = qdot; not reversible */
+#define TDYN_BUSHING    12   /* q = body-fixed x-y-z angles then p_FM, u = qdot (6); on Ground only, not
+                                reversible, not in the serving loop */
 
 /* One tree (what MobilizedBody::<Type>(parent, X_PF, body, X_BM) calls build;
  * MobilizedBody.cpp:783-808).  Host pointers. */
 typedef struct tdyn_tree_desc {
     int32_t nb;                /* mobilized bodies (Ground excluded) */
     const int32_t* parent;     /* [nb] parent body number, 0 = Ground, parent[b-1] < b */
-    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_BENDSTRETCH (Free only on Ground) */
+    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_BUSHING (Free and Bushing only on Ground) */
     const double* mass;        /* [nb] */
     const double* com_B;       /* [nb][3] mass center in B */
     const double* inertia_B;   /* [nb][6] inertia about Bo in B: xx yy zz xy xz yz */

@@ -183,13 +183,13 @@ template <class T> inline void abiAdd(ABI<T>& P, const ABI<T>& Q) {
 // Model.
 // ---------------------------------------------------------------------------
 enum Mobilizer { PIN=0, SLIDER=1, UNIVERSAL=2, BALL=3, FREE=4, TRANSLATION=5, PLANAR=6, CYLINDER=7, SCREW=8,
-                 GIMBAL=9, WELD=10, BENDSTRETCH=11 };
+                 GIMBAL=9, WELD=10, BENDSTRETCH=11, BUSHING=12 };
 inline int mobDof(int t) {
-    static const int d[12] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3, 0, 2};
+    static const int d[13] = {1, 1, 2, 3, 6, 3, 3, 2, 1, 3, 0, 2, 6};
     return d[t];
 }
 inline int mobNq(int t) {
-    static const int n[12] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3, 0, 2};
+    static const int n[13] = {1, 1, 2, 4, 7, 3, 3, 2, 1, 3, 0, 2, 6};
     return n[t];
 }
 
@@ -248,8 +248,8 @@ struct Model {
             const int p = d.parent[i];
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
-            if (t < PIN || t > BENDSTRETCH) return false;
-            if (t == FREE && p != 0) return false;
+            if (t < PIN || t > BUSHING) return false;
+            if ((t == FREE || t == BUSHING) && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
             children[p].push_back(b);
@@ -272,7 +272,7 @@ struct Model {
             p_BM[b] = X_BM.p;
             pitch[b] = d.pitch ? d.pitch[i] : 0.0;
             reversed[b] = d.reversed ? (d.reversed[i] != 0) : 0;
-            if (reversed[b] && (t == FREE || t == BENDSTRETCH)) return false;
+            if (reversed[b] && (t == FREE || t == BENDSTRETCH || t == BUSHING)) return false;
             X_MB[b] = invert(X_BM);      // RigidBodyNode.h:1012 X_MB(~X_BM)
         }
         int maxLevel = 0;
@@ -470,7 +470,7 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
             // with no H column the ABI passes give P+ = P and z+ = z
             B.X_FM.R = ident33<T>();
             break;
-        case GIMBAL: {
+        case GIMBAL: case BUSHING: {
             // body-fixed 1-2-3 angles (RigidBodyNodeSpec_Gimbal.h:141-176,
             // Rotation.h:342-349); H_FM maps qdot to w_FM.
             for (int j=0;j<3;++j) { B.cq[j] = cos(qb[j]); B.sq[j] = sin(qb[j]); }
@@ -484,6 +484,13 @@ template <class T> void realizePosition(Tree<T>& tr, const T* q) {
             B.H_FM[0] = sv(mk3<T>(T(1), T(0), T(0)), zero3<T>());
             B.H_FM[1] = sv(mk3<T>(T(0), c0, s0), zero3<T>());
             B.H_FM[2] = sv(mk3<T>(s1, -s0*c1, c0*c1), zero3<T>());
+            if (t == BUSHING) {
+                // Bushing (RigidBodyNodeSpec_Bushing.h:152-191): the Gimbal's
+                // rotation and angle columns, then p_FM = (q3, q4, q5) with
+                // identity translation columns; u = qdot
+                B.X_FM.p = mk3<T>(qb[3], qb[4], qb[5]);
+                for (int j=0;j<3;++j) { V3<T> e = zero3<T>(); e[j] = T(1); B.H_FM[3+j] = sv(zero3<T>(), e); }
+            }
         } break;
         case BALL: case FREE: if (M.euler) {
             // Euler-angle mode (RigidBodyNodeSpec_Ball.h:113-158, _Free.h:
@@ -611,8 +618,9 @@ template <class T> void realizeVelocity(Tree<T>& tr, const T* q, const T* u, T* 
             const V3<T> mx = mk3<T>(B.X_FM.R(0,0), B.X_FM.R(1,0), B.X_FM.R(2,0));
             B.HDot_FM[0] = sv(zero3<T>(), cross(ez, B.V_FM.b));
             B.HDot_FM[1] = sv(zero3<T>(), cross(B.V_FM.a, mx));
-        } else if (t == GIMBAL) {
-            // d/dt of H_FM with qdot = u (RigidBodyNodeSpec_Gimbal.h:180-202)
+        } else if (t == GIMBAL || t == BUSHING) {
+            // d/dt of H_FM with qdot = u (RigidBodyNodeSpec_Gimbal.h:180-202,
+            // _Bushing.h:195-220; a Bushing's translation columns are constant)
             const T c0=B.cq[0], c1=B.cq[1], s0=B.sq[0], s1=B.sq[1];
             const T dc0 = -s0*ub[0], dc1 = -s1*ub[1], ds0 = c0*ub[0], ds1 = c1*ub[1];
             B.HDot_FM[1] = sv(mk3<T>(T(0), dc0, ds0), zero3<T>());

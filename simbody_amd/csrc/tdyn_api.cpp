@@ -33,12 +33,12 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
     for (int b = 1; b <= d.nb; ++b) {
         const int p = d.parent[b-1], ty = d.mobilizer[b-1];
         if (p < 0 || p >= b) { snprintf(buf, sizeof buf, "tree %d body %d: parent %d must be in [0,%d)", t, b, p, b); return fail(TDYN_E_ARG, buf); }
-        if (ty < TDYN_PIN || ty > TDYN_BENDSTRETCH) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
+        if (ty < TDYN_PIN || ty > TDYN_BUSHING) { snprintf(buf, sizeof buf, "tree %d body %d: unknown mobilizer %d", t, b, ty); return fail(TDYN_E_ARG, buf); }
         if (ty == TDYN_SCREW && (!d.screw_pitch || !std::isfinite(d.screw_pitch[b-1]))) {
             snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
         }
-        if (ty == TDYN_FREE && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Free mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
-        if ((ty == TDYN_FREE || ty == TDYN_BENDSTRETCH) && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed %s mobilizer is not supported in this build", t, b, ty == TDYN_FREE ? "Free" : "BendStretch"); return fail(TDYN_E_ARG, buf); }
+        if ((ty == TDYN_FREE || ty == TDYN_BUSHING) && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: %s mobilizer allowed on Ground only in this build", t, b, ty == TDYN_FREE ? "Free" : "Bushing"); return fail(TDYN_E_ARG, buf); }
+        if ((ty == TDYN_FREE || ty == TDYN_BENDSTRETCH || ty == TDYN_BUSHING) && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed %s mobilizer is not supported in this build", t, b, ty == TDYN_FREE ? "Free" : ty == TDYN_BUSHING ? "Bushing" : "BendStretch"); return fail(TDYN_E_ARG, buf); }
         if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
         if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
             !finite_all(d.X_PF + 12*(b-1), 12) || !finite_all(d.X_BM + 12*(b-1), 12)) {
@@ -78,6 +78,7 @@ void pack_body_params(const tdyn_tree_desc& d, int b, double* P) {
     if (d.mobilizer[b-1] == TDYN_SCREW) P[PRM_PITCH] = d.screw_pitch[b-1];
     if (d.reversed && d.reversed[b-1]) P[PRM_REV] = 1.0;
     if (d.use_euler_angles && (d.mobilizer[b-1] == TDYN_BALL || d.mobilizer[b-1] == TDYN_FREE)) P[PRM_EULER] = 1.0;
+    if (d.mobilizer[b-1] == TDYN_BUSHING) P[PRM_EULER] = 2.0;   // runs as a Free body with x-y-z angles, u = qdot
 }
 
 bool any_reversed(const tdyn_tree_desc& d) {
@@ -197,9 +198,12 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     for (int t = 0; t < n_trees; ++t) {
         B->any_rev = B->any_rev || any_reversed(trees[t]);
         B->any_euler = B->any_euler || trees[t].use_euler_angles != 0;
-        for (int b = 0; b < trees[t].nb; ++b) B->any_bendstretch = B->any_bendstretch || trees[t].mobilizer[b] == TDYN_BENDSTRETCH;
+        for (int b = 0; b < trees[t].nb; ++b) {
+            B->any_bendstretch = B->any_bendstretch || trees[t].mobilizer[b] == TDYN_BENDSTRETCH;
+            B->any_bushing = B->any_bushing || trees[t].mobilizer[b] == TDYN_BUSHING;
+        }
     }
-    B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev && !B->any_euler;
+    B->uniform = ((tree_of_instance == nullptr) || n_trees == 1) && !B->any_rev && !B->any_euler && !B->any_bushing;
     std::vector<int4> body_b, pre, bfs, bfs_b; std::vector<int> child_idx, lvl; std::vector<double> prm, grav, prm_bfs;
     B->stream_ok = !B->uniform;
     std::vector<int> tq(n_trees), tu(n_trees), tnb(n_trees);
@@ -213,12 +217,14 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         bool chain = true;
         for (int b = 1; b <= nb; ++b) {
             const int ty = d.mobilizer[b-1];
+            // the kernels carry a Bushing as a Free body flagged PRM_EULER = 2
+            const int dty = ty == TDYN_BUSHING ? TDYN_FREE : ty;
             chain = chain && d.parent[b-1] == b - 1 && !mob_generic_only(ty);
-            B->h_body_a.push_back(make_int4(d.parent[b-1], ty, q, u));
+            B->h_body_a.push_back(make_int4(d.parent[b-1], dty, q, u));
             body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, gslot));
             for (int c : ch[b]) child_idx.push_back(c);
             // batches of different trees carry non-Free bodies in padded 3-dof records
-            const int rd = ((!B->uniform || mob_generic_only(ty)) && ty != TDYN_FREE) ? 3 : mob_dof(ty);
+            const int rd = ((!B->uniform || mob_generic_only(dty)) && dty != TDYN_FREE) ? 3 : mob_dof(dty);
             q += mob_nq(ty); u += mob_dof(ty); rec += rec_size(rd); gslot += 7 * mob_dof(ty);
             if ((ty == TDYN_BALL || ty == TDYN_FREE) && !d.use_euler_angles) ++nquat;
             double P[PRM_STRIDE];
@@ -236,7 +242,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
                 stack.insert(stack.end(), ch[b].rbegin(), ch[b].rend());
             }
             for (int b = 2; b <= nb; ++b)
-                if (d.mobilizer[b-1] == TDYN_FREE) B->stream_ok = false;   // a Free body must come first
+                if (d.mobilizer[b-1] == TDYN_FREE || d.mobilizer[b-1] == TDYN_BUSHING)
+                    B->stream_ok = false;   // a 6-dof body must come first
         }
         for (int k = 0; k < 3; ++k) grav.push_back(d.gravity[k]);
         grav.push_back(0.0);
@@ -321,7 +328,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
     std::vector<int> inst_of_lane(n_instances);
     for (int i = 0; i < n_instances; ++i) inst_of_lane[i] = i;
     if (!B->uniform) {
-        static const double cost[12] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9, 0.6, 1.3};
+        static const double cost[13] = {1.0, 0.9, 1.4, 1.8, 3.3, 1.5, 1.5, 1.2, 1.0, 1.9, 0.6, 1.3, 3.4};
+        static_assert(sizeof cost / sizeof cost[0] == TDYN_BUSHING + 1, "one cost per mobilizer");
         std::vector<double> tw(n_trees, 0.0);
         for (int t = 0; t < n_trees; ++t)
             for (int bdy = 0; bdy < trees[t].nb; ++bdy) tw[t] += cost[trees[t].mobilizer[bdy]];

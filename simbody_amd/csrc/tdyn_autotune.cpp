@@ -34,7 +34,7 @@ std::string shape_key(const tdyn_batch* b) {
     char buf[160];
     snprintf(buf, sizeof buf, "tree-lanes n%d nb%d depth%d width%d trees%d ext%d", ilog2(b->n_inst),
              ilog2((double)b->total_b / b->n_inst - 1.0), ilog2(b->dmax + 1.0), ilog2(w),
-             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch || b->known_p) ? 1 : 0);
+             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch || b->any_bushing || b->known_p) ? 1 : 0);
     return buf;
 }
 

@@ -129,6 +129,7 @@ template <int TYPE, class K> struct Joint {
     int ty = TYPE, d = D;       // run-time type and dof of a GEN body
     bool rv = false;            // reversed (GEN only); Hm: the mobilizer's own angular H columns
     v3<K> Hm[3];
+    bool bu = false;            // a Bushing (FREE, extended builds): u = qdot, Gimbal angle columns
     // Reversed mobilizer (RigidBodyNodeSpec.h:262-277): what set() computed is
     // the mobilizer's X_MF and H_MF; X_FM = ~X_MF and, by
     // calcReverseMobilizerH_FM (RigidBodyNodeSpec.cpp:131-166),
@@ -160,8 +161,11 @@ template <int TYPE, class K> struct Joint {
     // ec keeps (c0, s0, c1, s1) of those angles for their rates.
     K ec[4];
     // EXT: the build for extended models (BendStretch bodies possible).
+    // bush: a Bushing carried as a Free body (RigidBodyNodeSpec_Bushing.h:
+    // 152-191): the Euler-mode Free's X_FM, the Gimbal's angle columns.
     template <bool CACHED, bool EXT = false>
-    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0), bool eul = false) {
+    __device__ __forceinline__ void set(const double* qv, K* tc, int rty = TYPE, K pitch = K(0), bool eul = false,
+                                        bool bush = false) {
         const K z = K(0), o = K(1);
         p_FM = zero3<K>();
         if constexpr (TYPE == FREE) {
@@ -178,6 +182,11 @@ template <int TYPE, class K> struct Joint {
                 H_FM[3] = {zero3<K>(), V3(o, z, z)};
                 H_FM[4] = {zero3<K>(), V3(z, o, z)};
                 H_FM[5] = {zero3<K>(), V3(z, z, o)};
+                bu = bush;
+                if (bush) {
+                    H_FM[1] = {V3(z, c0, s0), zero3<K>()};
+                    H_FM[2] = {V3(s1, -s0*c1, c0*c1), zero3<K>()};
+                }
                 return;
             }
         }
@@ -357,6 +366,14 @@ __device__ __forceinline__ sv<K> coriolis(const Joint<TYPE, K>& J, const m33<K>&
     for (int j = 0; j < D; ++j) {
         v3<K> hdw = zero3<K>();
         if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(J.R_FM, 1)); }
+        if constexpr (TYPE == FREE && REV) {
+            // a Bushing's angle columns turn with the angles before them
+            // (_Bushing.h:195-220): d/dt H_FM(j) = (sum_{k<j} qdot_k H_FM(k).w) x H_FM(j).w
+            if (J.bu && (j == 1 || j == 2)) {
+                const v3<K> wj = j == 1 ? K(uv[0])*J.H_FM[0].w : K(uv[0])*J.H_FM[0].w + K(uv[1])*J.H_FM[1].w;
+                hdw = cross(wj, J.H_FM[j].w);
+            }
+        }
         v3<K> hdv = zero3<K>();
         if constexpr (TYPE == GEN) {
             // Universal and Gimbal (_Universal.h:173-189, _Gimbal.h:180-202):
@@ -489,10 +506,17 @@ __device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 
     }
 }
 
-// Euler-angle mode of a Ball (generic) or Free body, in models that may use it.
+// Euler-angle mode of a Ball (generic) or Free body, in models that may use it
+// (a Bushing's q are such angles too).
 template <int TYPE, class PA>
 __device__ __forceinline__ bool euler_of(const PA& P) {
     if constexpr ((TYPE == GEN || TYPE == FREE) && PrmTraits<PA>::ext) return P[PRM_EULER] != 0.0;
+    return false;
+}
+// A Bushing, carried as a Free body, in models that may hold one.
+template <int TYPE, class PA>
+__device__ __forceinline__ bool bushing_of(const PA& P) {
+    if constexpr (TYPE == FREE && PrmTraits<PA>::ext) return P[PRM_EULER] == 2.0;
     return false;
 }
 // Reverses a generic body's joint when the model may hold reversed
@@ -543,15 +567,22 @@ __device__ __forceinline__ void pass1_body(const PA& P, const Pre<K>& x, int qi,
     }
     Joint<TYPE, K> J;
     K tc[4] = {K(0), K(0), K(0), K(0)};
-    J.template set<false, PrmTraits<PA>::ext>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), eul);
+    J.template set<false, PrmTraits<PA>::ext>(qv, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x), eul,
+                                           bushing_of<TYPE>(P));
     if constexpr (TYPE == GEN || TYPE == FREE) {
         if (eul && (TYPE == FREE || x.ty == BALL)) {
-            // x-y-z angle rates N(q) w (RigidBodyNodeSpec_Ball.h:183-214,
-            // _Free.h:222-257); the fourth q slot is unused, its rate zero
-            const v3<K> ed = euler_dot(J.ec[0], J.ec[1], J.ec[2], J.ec[3], V3(K(uv[0]), K(uv[1]), K(uv[2])));
-            f(qi) = double(ed.x); f(qi + 1) = double(ed.y); f(qi + 2) = double(ed.z);
-            if constexpr (TYPE == FREE) { f(qi+3) = uv[3]; f(qi+4) = uv[4]; f(qi+5) = uv[5]; f(qi+6) = 0.0; }
-            else f(qi + 3) = 0.0;
+            if (TYPE == FREE && J.bu) {
+                // Bushing: qdot = u
+#pragma unroll
+                for (int k = 0; k < 6; ++k) f(qi + k) = uv[k];
+            } else {
+                // x-y-z angle rates N(q) w (RigidBodyNodeSpec_Ball.h:183-214,
+                // _Free.h:222-257); the fourth q slot is unused, its rate zero
+                const v3<K> ed = euler_dot(J.ec[0], J.ec[1], J.ec[2], J.ec[3], V3(K(uv[0]), K(uv[1]), K(uv[2])));
+                f(qi) = double(ed.x); f(qi + 1) = double(ed.y); f(qi + 2) = double(ed.z);
+                if constexpr (TYPE == FREE) { f(qi+3) = uv[3]; f(qi+4) = uv[4]; f(qi+5) = uv[5]; f(qi+6) = 0.0; }
+                else f(qi + 3) = 0.0;
+            }
         }
     }
     reverse_if<TYPE>(P, J);
@@ -630,7 +661,7 @@ __device__ __forceinline__ void pass2_body(const PA& P, const Pre<K>& x, const v
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true, PrmTraits<PA>::ext>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x),
-                                          euler_of<TYPE>(P));
+                                          euler_of<TYPE>(P), bushing_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);
@@ -752,7 +783,7 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
     Joint<TYPE, K> J;
     K tc[4] = {x.tc[0], x.tc[1], x.tc[2], x.tc[3]};
     J.template set<true, PrmTraits<PA>::ext>(x.q, tc, TYPE == GEN ? x.ty : TYPE, screw_pitch<TYPE, K>(P, x),
-                                          euler_of<TYPE>(P));
+                                          euler_of<TYPE>(P), bushing_of<TYPE>(P));
     reverse_if<TYPE>(P, J);
     m33<K> R_PB; v3<K> p_PB;
     body_in_parent<K>(P, J.R_FM, J.p_FM, R_PB, p_PB);

@@ -58,6 +58,7 @@ struct tdyn_batch {
     bool any_rev = false;        // some tree has a reversed mobilizer (the batch runs padded generic bodies)
     bool any_euler = false;      // some tree uses Euler angles for Ball and Free (likewise)
     bool any_bendstretch = false; // some tree has a BendStretch body
+    bool any_bushing = false;     // some tree has a Bushing body
     const int* known_p = nullptr;     // tdyn_batch_set_known_udot: caller's device arrays (rollouts)
     const double* known_u = nullptr;
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
@@ -120,7 +121,7 @@ struct tdyn_batch {
         a.chain = (!known_p && (evaluator == 2 || (evaluator == 0 && is_chain))) ? 1 : 0;
         a.presc = known_p; a.udot_known = known_u;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.ext = (any_rev || any_euler || any_bendstretch || known_p) ? 1 : 0;
+        a.ext = (any_rev || any_euler || any_bendstretch || any_bushing || known_p) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

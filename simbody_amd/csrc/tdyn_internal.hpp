@@ -20,7 +20,7 @@ __device__ __forceinline__ int4 ldi4(const int4* base, int idx) {
 }
 
 enum { PIN = 0, SLIDER = 1, UNIVERSAL = 2, BALL = 3, FREE = 4, TRANSLATION = 5, PLANAR = 6, CYLINDER = 7, SCREW = 8,
-       GIMBAL = 9, WELD = 10, BENDSTRETCH = 11 };
+       GIMBAL = 9, WELD = 10, BENDSTRETCH = 11, BUSHING = 12 };
 // Kernel-side body kind: any non-Free mobilizer carried as a padded 3-dof body.
 enum { GEN = 15 };
 __host__ __device__ inline int mob_dof(int t) {
@@ -39,6 +39,7 @@ __host__ __device__ inline int mob_nq(int t) {
     case TRANSLATION: case PLANAR: case GIMBAL: return 3;
     case BALL: return 4;
     case WELD: return 0;
+    case BUSHING: return 6;
     default: return 7;
     }
 }
@@ -54,7 +55,8 @@ enum {
     PRM_R_PF = 0, PRM_P_PF = 9, PRM_R_MB = 12, PRM_P_MB = 21, PRM_MASS = 24,
     PRM_COM = 25, PRM_G = 28 /* unit inertia about Bo in B: xx yy zz xy xz yz */,
     PRM_P_BM = 34, PRM_PITCH = 37 /* Screw */, PRM_REV = 38 /* 1: reversed mobilizer */,
-    PRM_EULER = 39 /* 1: Ball/Free q are x-y-z angles */, PRM_STRIDE = 40
+    PRM_EULER = 39 /* 1: Ball/Free q are x-y-z angles; 2: a Bushing (a Free body whose q are x-y-z
+                       angles and p_FM, u = qdot, H_FM the Gimbal's angle columns) */, PRM_STRIDE = 40
 };
 
 // Per-body record in the per-lane scratch (doubles, relative to rec_off).

@@ -106,13 +106,16 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
     if (p == 0) { R_GP = ident<double>(); p_GP = zero3<double>(); }
     else { rp = c.rec(p); R_GP = ldm(c, rp + REC_R); p_GP = ld3(c, rp + REC_P); }
 
+    // Ball/Free q as x-y-z angles (Euler-angle mode; PRM_EULER 2: a Bushing,
+    // six q, u = qdot, RigidBodyNodeSpec_Bushing.h:152-220); ec: their
+    // c0 s0 c1 s1
+    const bool eul = (TYPE == BALL || TYPE == FREE) && P[PRM_EULER] != 0.0;
+    const bool bush = TYPE == FREE && P[PRM_EULER] == 2.0;
     double qv[NQ];
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) qv[k] = q(qi + k);
+    for (int k = 0; k < NQ; ++k) qv[k] = (TYPE == FREE && k == 6 && bush) ? 0.0 : q(qi + k);
     dm R_FM; d3 p_FM = zero3<double>();
     dv H_FM[D];
-    // Ball/Free q as x-y-z angles (Euler-angle mode); ec: their c0 s0 c1 s1
-    const bool eul = (TYPE == BALL || TYPE == FREE) && P[PRM_EULER] != 0.0;
     double ec[4] = {1.0, 0.0, 1.0, 0.0};
     if constexpr (TYPE == PIN) {
         double sn, cs; sincos(qv[0], &sn, &cs);
@@ -143,6 +146,10 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
         H_FM[1] = {V3(0.0, 1.0, 0.0), zero3<double>()};
         H_FM[2] = {V3(0.0, 0.0, 1.0), zero3<double>()};
         if constexpr (TYPE == FREE) {
+            if (bush) {   // the Gimbal's angle columns (w_FM = H_FM qdot)
+                H_FM[1] = {V3(0.0, ec[0], ec[1]), zero3<double>()};
+                H_FM[2] = {V3(ec[3], -ec[1]*ec[2], ec[0]*ec[2]), zero3<double>()};
+            }
             p_FM = eul ? V3(qv[3], qv[4], qv[5]) : V3(qv[4], qv[5], qv[6]);
             H_FM[3] = {zero3<double>(), V3(1.0, 0.0, 0.0)};
             H_FM[4] = {zero3<double>(), V3(0.0, 1.0, 0.0)};
@@ -181,7 +188,10 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
 #pragma unroll
         for (int k = 0; k < D; ++k) uv[k] = u(ui + k);
         if constexpr (TYPE == BALL || TYPE == FREE) {
-            if (eul) {
+            if (bush) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) qd(qi + k, uv[k]);
+            } else if (eul) {
                 const d3 ed = euler_dot(ec[0], ec[1], ec[2], ec[3], V3(uv[0], uv[1], uv[2]));
                 qd(qi, ed.x); qd(qi + 1, ed.y); qd(qi + 2, ed.z);
                 if constexpr (TYPE == FREE) { qd(qi+3, uv[3]); qd(qi+4, uv[4]); qd(qi+5, uv[5]); qd(qi+6, 0.0); }
@@ -206,6 +216,14 @@ __device__ __forceinline__ void kin_body(const Lane& c, int b, const int4 ba, co
         for (int j = 0; j < D; ++j) {
             d3 hdw = zero3<double>();
             if constexpr (TYPE == UNIVERSAL) { if (j == 1) hdw = cross(w_FM, col(R_FM, 1)); }
+            if constexpr (TYPE == FREE) {
+                // a Bushing's angle columns turn with the angles before them:
+                // d/dt H_FM(j) = (sum_{k<j} qdot_k H_FM(k).w) x H_FM(j).w
+                if (bush && (j == 1 || j == 2)) {
+                    const d3 wj = j == 1 ? uv[0]*H_FM[0].w : uv[0]*H_FM[0].w + uv[1]*H_FM[1].w;
+                    hdw = cross(wj, H_FM[j].w);
+                }
+            }
             const d3 lin = (zero3<double>() - cross(r_MB_F, hdw)) - cross(wxr, H_FM[j].w);
             const dv HD = {mul(R_GF, hdw) + cross(w_GF, H[j].w), mul(R_GF, lin) + cross(w_GF, H[j].v)};
             VD = VD + uv[j]*HD;
@@ -670,6 +688,11 @@ __device__ __forceinline__ void outward_body(const Lane& c, int b, const int4 ba
         bool quat = TYPE == BALL || TYPE == FREE;
         if constexpr (TYPE == GEN) quat = ba.y == BALL;
         if constexpr (D >= 3) {
+            if (TYPE == FREE && c.prm(b)[PRM_EULER] == 2.0) {      // Bushing: qdotdot = udot
+#pragma unroll
+                for (int k = 0; k < 6; ++k) qdd(ba.z + k, ud[k]);
+                return;
+            }
             if (quat && c.prm(b)[PRM_EULER] != 0.0) {
                 // x-y-z angles: qdotdot = N b + NDot w (RigidBodyNodeSpec_Ball.h:355-392)
                 double s0, c0, s1, c1;
@@ -1455,7 +1478,8 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                 double s2 = 0;
                 for (int b = 0; b < nb; ++b) {
                     const int4 ba = ldi4(body_a, b);
-                    if (ba.y != BALL && ba.y != FREE) continue;
+                    // quaternions only (a Free body flagged PRM_EULER holds angles)
+                    if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                     double n2 = 0;
                     for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
                     const double e = sqrt(n2) - 1.0;
@@ -1467,7 +1491,8 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                     ++projections;
                     for (int b = 0; b < nb; ++b) {
                         const int4 ba = ldi4(body_a, b);
-                        if (ba.y != BALL && ba.y != FREE) continue;
+                        // quaternions only (a Free body flagged PRM_EULER holds angles)
+                        if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                         double qq[4], n2 = 0;
                         for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
                         const double n = sqrt(n2);

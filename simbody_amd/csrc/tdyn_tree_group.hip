@@ -384,7 +384,8 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
                 double s2 = 0;
                 for (int b = l; b < nb; b += G) {
                     const int4 ba = ldi4(body_a, b);
-                    if (ba.y != BALL && ba.y != FREE) continue;
+                    // quaternions only (a Free body flagged PRM_EULER holds angles)
+                    if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                     double n2 = 0;
                     for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
                     const double e = sqrt(n2) - 1.0;
@@ -397,7 +398,8 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
                     ++projections;
                     for (int b = l; b < nb; b += G) {
                         const int4 ba = ldi4(body_a, b);
-                        if (ba.y != BALL && ba.y != FREE) continue;
+                        // quaternions only (a Free body flagged PRM_EULER holds angles)
+                        if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                         double qq[4], n2 = 0;
                         for (int k = 0; k < 4; ++k) { qq[k] = Y1(ba.z + k); n2 += qq[k]*qq[k]; }
                         const double n = sqrt(n2);

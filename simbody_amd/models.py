@@ -18,11 +18,11 @@ from dataclasses import dataclass, field
 import numpy as np
 
 PIN, SLIDER, UNIVERSAL, BALL, FREE = 0, 1, 2, 3, 4
-TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL, WELD, BENDSTRETCH = 5, 6, 7, 8, 9, 10, 11
+TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL, WELD, BENDSTRETCH, BUSHING = 5, 6, 7, 8, 9, 10, 11, 12
 DOF = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 3, FREE: 6, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1, GIMBAL: 3,
-       WELD: 0, BENDSTRETCH: 2}
+       WELD: 0, BENDSTRETCH: 2, BUSHING: 6}
 NQ = {PIN: 1, SLIDER: 1, UNIVERSAL: 2, BALL: 4, FREE: 7, TRANSLATION: 3, PLANAR: 3, CYLINDER: 2, SCREW: 1,
-      GIMBAL: 3, WELD: 0, BENDSTRETCH: 2}
+      GIMBAL: 3, WELD: 0, BENDSTRETCH: 2, BUSHING: 6}
 NAMES = {PIN: "Pin", SLIDER: "Slider", UNIVERSAL: "Universal", BALL: "Ball", FREE: "Free"}
 
 
@@ -333,10 +333,11 @@ def _state_plan(spec):
             plan.append(("trans", qi, c)); qi += 1; c += 1
         elif t == WELD:
             continue
-        elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL, BENDSTRETCH):
+        elif t in (TRANSLATION, PLANAR, CYLINDER, SCREW, GIMBAL, BENDSTRETCH, BUSHING):
             kinds = {TRANSLATION: ("trans",) * 3, PLANAR: ("angle", "trans", "trans"),
                      CYLINDER: ("angle", "trans"), SCREW: ("angle",),
-                     GIMBAL: ("angle", "tilt", "angle"), BENDSTRETCH: ("angle", "trans")}[t]
+                     GIMBAL: ("angle", "tilt", "angle"), BENDSTRETCH: ("angle", "trans"),
+                     BUSHING: ("angle", "tilt", "angle", "trans", "trans", "trans")}[t]
             for k in kinds:
                 plan.append((k, qi, c)); qi += 1; c += 1
         elif getattr(spec, "use_euler_angles", False):

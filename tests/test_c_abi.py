@@ -80,7 +80,10 @@ def test_extended_model_descriptions_are_checked_on_the_host():
 
     spec = M.spatial_chain(4, seed=2)
     cases = [
-        ({"mobilizer": np.array([0, 12, 0, 0], np.int32)}, "unknown mobilizer"),
+        ({"mobilizer": np.array([0, 13, 0, 0], np.int32)}, "unknown mobilizer"),
+        ({"mobilizer": np.array([0, M.BUSHING, 0, 0], np.int32)}, "Bushing mobilizer allowed on Ground only"),
+        ({"mobilizer": np.array([M.BUSHING, 0, 0, 0], np.int32), "reversed": np.array([1, 0, 0, 0], np.int32)},
+         "reversed Bushing"),
         ({"mobilizer": np.array([0, M.SCREW, 0, 0], np.int32), "pitch": None}, "screw_pitch"),
         ({"mobilizer": np.array([M.FREE, 0, 0, 0], np.int32), "reversed": np.array([1, 0, 0, 0], np.int32)},
          "reversed Free"),
