@@ -278,6 +278,16 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
  * (TimeStepper::stepTo(t_end) with Integrator::setFinalTime(t_end),
  * AbstractIntegratorRep.cpp:216-436).  Resumable: calling again continues. */
 int tdyn_rollout_advance(tdyn_batch* b, int max_steps, hipStream_t s);
+/* TimeStepper::stepTo(report time) with Integrator::setAllowInterpolation(
+ * false) (AbstractIntegratorRep.cpp:216-240, takeOneStep :512-530): moves
+ * every instance's final time to t_end (device, per instance; NULL: all
+ * t_end_all).  An instance that had reached its previous final time resumes
+ * from its state, time, step size and statistics when the new time lies
+ * ahead; tdyn_rollout_advance then stops every step at the new time, so
+ * begin(t_1), advance, get, extend(t_2), advance, get, ... reports the
+ * trajectory at t_1, t_2, ... exactly as Simbody's TimeStepper does with
+ * interpolation off. */
+int tdyn_rollout_extend(tdyn_batch* b, const double* t_end, double t_end_all, hipStream_t s);
 /* Copy out state (packed q, u), per-instance time t, status (0 running,
  * 1 reached t_end, <0 failed) and stats (attempts, accepted, rejected,
  * projections; 4 ints per instance).  Any output may be NULL. */

@@ -129,6 +129,8 @@ public:
         check(tdyn_rollout_begin(h_, q, u, appliedBodyForces, t0, tEnd, tEndAll, &integ.cfg, s_));
     }
     void advance(int maxSteps) const { check(tdyn_rollout_advance(h_, maxSteps, s_)); }
+    // stepTo(next report time) with Integrator::setAllowInterpolation(false)
+    void extendTo(const double* tEnd, double tEndAll) const { check(tdyn_rollout_extend(h_, tEnd, tEndAll, s_)); }
     void getState(double* q, double* u, double* t, int32_t* status, int32_t* stats) const {
         check(tdyn_rollout_get(h_, q, u, t, status, stats, s_));
     }

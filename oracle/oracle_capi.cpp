@@ -242,6 +242,20 @@ int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T
     return st.status;
 }
 
+// Integration through report times stops[0..n) (the last is the final time)
+// with interpolation off: every step stops at the next report time; yout
+// [n][nq+nu] gets the state there (Rkm::integrateStops).
+int oracle_integrate_stops(void* p, double* y, const double* Fapp, double t0, const double* stops, int n,
+                           double accuracy, double fixed_h, double* yout, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    RkmStats st = rk.integrateStops(y, t0, stops, n, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, yout);
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
 // Integration with known udot on the bodies flagged in presc ([nb+1]): the
 // values udot_known ([nu]) on their mobilities at every derivative.
 int oracle_integrate_known(void* p, double* y, const double* Fapp, const int* presc, const double* udot_known,

@@ -45,7 +45,7 @@ def lib():
                      "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
                      "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
                      "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
-                     "oracle_integrate_known"]:
+                     "oracle_integrate_known", "oracle_integrate_stops"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -211,6 +211,19 @@ class OracleModel:
                                ctypes.c_long(max_attempts), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
         return y, dict(zip(keys, list(st)))
+
+    def integrate_stops(self, y, stops, accuracy, fixed_h=0.0, Fapp=None, t0=0.0):
+        """Integration through the report times `stops` with interpolation
+        off; returns (states [len(stops), ny] at the report times, stats)."""
+        y = np.array(y, dtype=np.float64, copy=True)
+        stops = np.ascontiguousarray(stops, dtype=np.float64)
+        out = np.zeros((len(stops), y.size))
+        st = (ctypes.c_long * 6)()
+        lib().oracle_integrate_stops(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), _p(stops),
+                                     ctypes.c_int(len(stops)), ctypes.c_double(accuracy),
+                                     ctypes.c_double(fixed_h), _p(out), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return out, dict(zip(keys, list(st)))
 
     def count_flops_integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None):
         y = np.array(y, dtype=np.float64, copy=True)

@@ -1222,7 +1222,20 @@ template <class T> struct Rkm {
 
     // Integrate y (q then u) from t0 to T.  Returns stats.
     RkmStats integrate(T* y, double t0, double Tend, const RkmConfig& cfg, const SV<T>* Fapp) {
+        return integrateStops(y, t0, &Tend, 1, cfg, Fapp, nullptr);
+    }
+
+    // Integrate through the report times stops[0..n) (increasing; the last is
+    // the final time), as TimeStepper::stepTo(stops[k]) in turn with
+    // Integrator::setAllowInterpolation(false) (AbstractIntegratorRep.cpp:
+    // 216-240: tMax = the report time; takeOneStep :512-530 stops each step
+    // there): no step crosses a report time, and the step size carries from
+    // one report interval to the next.  yout (nullable, n rows of nq+nu)
+    // receives the state at each report time.
+    RkmStats integrateStops(T* y, double t0, const double* stops, int nstops, const RkmConfig& cfg,
+                            const SV<T>* Fapp, T* yout) {
         RkmStats st;
+        int k = 0;
         const int ny = M.nq + M.nu;
         const double acc = cfg.accuracy;
         const double consTol = acc/10;
@@ -1237,7 +1250,11 @@ template <class T> struct Rkm {
         if (hmax != -1) hcur = std::min(hcur, hmax);
         const double projectionLimit = std::max(2*consTol, std::sqrt(consTol));
         for (;;) {
-            if (t >= Tend) break;
+            while (k < nstops && t >= stops[k]) {
+                if (yout) for (int i=0;i<ny;++i) yout[(size_t)k*ny + i] = y[i];
+                ++k;
+            }
+            if (k == nstops) break;
             // saveTimeAndStateAsPrevious + realizeStateDerivatives
             for (int i=0;i<ny;++i) y0[i] = y[i];
             for (int i=0;i<M.nu;++i) {
@@ -1246,7 +1263,7 @@ template <class T> struct Rkm {
             }
             if (!derivs(tr, ws, y0.data(), Fapp, f0.data())) { st.status = 2; break; }
             ++st.evals;
-            const double tMax = Tend;
+            const double tMax = stops[k];
             bool succeeded = false;
             double t1 = t;
             do {

@@ -635,6 +635,22 @@ int tdyn_rollout_derivs(tdyn_batch* b, double* qdot, double* udot, hipStream_t s
     return TDYN_OK;
 }
 
+int tdyn_rollout_extend(tdyn_batch* b, const double* t_end, double t_end_all, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_extend: call tdyn_rollout_begin first");
+    if (!t_end && !std::isfinite(t_end_all)) return fail(TDYN_E_ARG, "tdyn_rollout_extend: final time must be finite");
+    const int L = b->n_inst;
+    const double* te = nullptr;
+    if (t_end) {
+        // the new times go through t_end's own buffer (lane order), then the
+        // kernel reopens the lanes that finished before them
+        if (b->permuted) HIPCHK(launch_lane_gather(b->inst.p, L, t_end, b->t_end.p, s));
+        else HIPCHK(hipMemcpyAsync(b->t_end.p, t_end, (size_t)L * sizeof(double), hipMemcpyDeviceToDevice, s));
+        te = b->t_end.p;
+    }
+    HIPCHK(launch_rollout_extend(L, te, t_end_all, b->t.p, b->t_end.p, b->status.p, s));
+    return TDYN_OK;
+}
+
 int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
                  const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s) {
     int rc = tdyn_rollout_begin(b, q, u, fapp, t0, nullptr, t_end, cfg, s);

@@ -242,6 +242,8 @@ hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u,
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s);
 // per-instance <-> per-lane copies for batches whose lanes are permuted
 hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s);
+hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, const double* t, double* t_end,
+                                 int* status, hipStream_t s);
 hipError_t launch_lane_scatter(const int* inst, int n, const double* t_lane, double* t_inst, const int* st_lane,
                                int* st_inst, const int* stats_lane, int* stats_inst, hipStream_t s);
 

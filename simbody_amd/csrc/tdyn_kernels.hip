@@ -1664,6 +1664,22 @@ __global__ void k_lane_scatter(const int* inst, int n, const double* tl, double*
     if (si) si[i] = sl[l];
     if (xi) for (int k = 0; k < 4; ++k) xi[4*i + k] = xl[4*l + k];
 }
+// New final times (te, lane order, or te_all for every lane): a lane that had
+// reached its old final time resumes if the new one lies ahead of it.
+__global__ void k_rollout_extend(int n, const double* te, double te_all, const double* t, double* t_end,
+                                 int* status) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const double e = te ? te[l] : te_all;
+    t_end[l] = e;
+    if (status[l] == 1 && t[l] < e) status[l] = 0;
+}
+hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, const double* t, double* t_end,
+                                 int* status, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_rollout_extend<<<dim3((n + 255)/256), dim3(256), 0, s>>>(n, te_lane, te_all, t, t_end, status);
+    return hipGetLastError();
+}
 hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     k_lane_gather<<<dim3((n + 255)/256), dim3(256), 0, s>>>(inst, n, src_inst, dst_lane);

@@ -38,7 +38,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
            "tdyn_multiply_by_system_jacobian_transpose", "tdyn_calc_body_acceleration_from_udot",
            "tdyn_calc_composite_body_inertias", "tdyn_rollout_begin", "tdyn_rollout_advance",
-           "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_last_error",
+           "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_rollout_extend", "tdyn_last_error",
            "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
            "tdyn_server_advance", "tdyn_server_collect"]
 
@@ -107,6 +107,7 @@ def lib():
         L.tdyn_rollout_begin.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, ctypes.c_double,
                                          ctypes.POINTER(IntegratorCfgC), vp]
         L.tdyn_rollout_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_rollout_extend.argtypes = [vp, vp, ctypes.c_double, vp]
         L.tdyn_rollout_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.tdyn_rollout_derivs.argtypes = [vp, vp, vp, vp]
         L.tdyn_rollout.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
@@ -416,6 +417,41 @@ class MatterBatch:
 
     def rollout_advance(self, max_steps):
         _check(lib().tdyn_rollout_advance(self.h, int(max_steps), _stream()))
+
+    def rollout_extend(self, t_end):
+        """Move every instance's final time to t_end (a float or a device
+        tensor per instance); finished instances resume from their state and
+        step size (stepTo(report time) with interpolation off)."""
+        if hasattr(t_end, "is_cuda"):
+            _check(lib().tdyn_rollout_extend(self.h, _ptr(t_end), ctypes.c_double(0.0), _stream()))
+        else:
+            _check(lib().tdyn_rollout_extend(self.h, None, ctypes.c_double(float(t_end)), _stream()))
+
+    def _run_to_end(self):
+        while True:
+            self.rollout_advance(1 << 20)
+            q, u, t, st, stats = self.rollout_get()
+            if not bool((st == 0).any()):
+                return q, u, t, st, stats
+
+    def stepToReports(self, q, u, integ, report_times, fapp=None, t0=0.0):
+        """TimeStepper::stepTo(t_k) for each report time in turn, with
+        Integrator::setAllowInterpolation(false): no step crosses a report
+        time and the step size carries across them; the last report time is
+        the final time.  Returns (q [k], u [k] per report time, stats[n,4])."""
+        times = [float(x) for x in report_times]
+        if not times or any(b <= a for a, b in zip(times, times[1:])) or times[0] < t0:
+            raise ValueError("report times must increase from t0")
+        self.rollout_begin(q, u, integ, times[0], fapp=fapp, t0=t0)
+        qs, us = [], []
+        for k, tk in enumerate(times):
+            if k:
+                self.rollout_extend(tk)
+            q1, u1, _, st, stats = self._run_to_end()
+            if bool((st < 0).any()):
+                raise TdynError(5, "Integrator::StepFailed for at least one instance")
+            qs.append(q1); us.append(u1)
+        return qs, us, stats
 
     def rollout_get(self):
         torch = self.torch
