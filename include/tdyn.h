@@ -299,6 +299,21 @@ int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* st
  * SimbodyMatterSubsystemRep.cpp:1527-1555).  Overwrites the integrator's
  * stage storage; call between advances. */
 int tdyn_rollout_derivs(tdyn_batch* b, double* qdot, double* udot, hipStream_t s);
+/* TimeStepper::stepTo(report times) ... to t_end with Simbody's default
+ * reporting, interpolation on (AbstractIntegratorRep.cpp:63-93, 216-330):
+ * the steps ignore the report times (the trajectory and stats are those of
+ * tdyn_rollout to t_end); each report time r (report_times: n_reports HOST
+ * doubles, increasing, within [t0, t_end]) gets the end state of the step
+ * that reaches r when it lands on r, else the cubic Hermite interpolant of
+ * that step's two ends (interpolateOrder3, IntegratorRep.h:285-313) with its
+ * quaternions normalized when their RMS norm error exceeds the constraint
+ * tolerance (projectQ of the interpolated state).  q_reports [n][total_q],
+ * u_reports [n][total_u] (device); q, u in/out (state at t_end).  Runs one
+ * accepted step per launch with a derivative evaluation at each step's end,
+ * so it is slower than tdyn_rollout; synchronizes s. */
+int tdyn_rollout_reports(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
+                         const double* report_times, int n_reports, const tdyn_integrator_cfg* cfg,
+                         double* q_reports, double* u_reports, int32_t* stats, hipStream_t s);
 /* begin + advance until every instance finished + get; synchronizes s.
  * Returns TDYN_E_STEP_FAILED if any instance failed. */
 int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0,

@@ -123,6 +123,13 @@ public:
                 const RungeKuttaMerson& integ, int32_t* stats = nullptr) const {
         check(tdyn_rollout(h_, q, u, appliedBodyForces, t0, tEnd, &integ.cfg, stats, s_));
     }
+    // stepTo(tEnd) with reports by interpolation at reportTimes (host array)
+    void stepToWithReports(double* q, double* u, const double* appliedBodyForces, double t0, double tEnd,
+                           const double* reportTimes, int nReports, const RungeKuttaMerson& integ,
+                           double* qReports, double* uReports, int32_t* stats = nullptr) const {
+        check(tdyn_rollout_reports(h_, q, u, appliedBodyForces, t0, tEnd, reportTimes, nReports, &integ.cfg,
+                                   qReports, uReports, stats, s_));
+    }
     // resumable form
     void initialize(const double* q, const double* u, const double* appliedBodyForces, double t0,
                     const double* tEnd, double tEndAll, const RungeKuttaMerson& integ) const {

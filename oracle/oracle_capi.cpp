@@ -256,6 +256,21 @@ int oracle_integrate_stops(void* p, double* y, const double* Fapp, double t0, co
     return st.status;
 }
 
+// Integration to T with reports by interpolation (Simbody's default
+// reporting): rout [n][nq+nu] gets the state at each report time
+// (Rkm::integrateStops with reports); y gets the state at T.
+int oracle_integrate_reports(void* p, double* y, const double* Fapp, double t0, double T, const double* reports,
+                             int n, double accuracy, double fixed_h, double* rout, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    RkmStats st = rk.integrateStops(y, t0, &T, 1, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, nullptr,
+                                    reports, n, rout);
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
 // Integration with known udot on the bodies flagged in presc ([nb+1]): the
 // values udot_known ([nu]) on their mobilities at every derivative.
 int oracle_integrate_known(void* p, double* y, const double* Fapp, const int* presc, const double* udot_known,

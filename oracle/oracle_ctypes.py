@@ -45,7 +45,7 @@ def lib():
                      "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
                      "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
                      "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
-                     "oracle_integrate_known", "oracle_integrate_stops"]:
+                     "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -224,6 +224,19 @@ class OracleModel:
                                      ctypes.c_double(fixed_h), _p(out), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
         return out, dict(zip(keys, list(st)))
+
+    def integrate_reports(self, y, T, reports, accuracy, fixed_h=0.0, Fapp=None, t0=0.0):
+        """Integration to T reporting at `reports` by Hermite interpolation
+        (interpolation on); returns (report states [len, ny], y(T), stats)."""
+        y = np.array(y, dtype=np.float64, copy=True)
+        reports = np.ascontiguousarray(reports, dtype=np.float64)
+        out = np.zeros((len(reports), y.size))
+        st = (ctypes.c_long * 6)()
+        lib().oracle_integrate_reports(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+                                       _p(reports), ctypes.c_int(len(reports)), ctypes.c_double(accuracy),
+                                       ctypes.c_double(fixed_h), _p(out), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return out, y, dict(zip(keys, list(st)))
 
     def count_flops_integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None):
         y = np.array(y, dtype=np.float64, copy=True)

@@ -1232,10 +1232,21 @@ template <class T> struct Rkm {
     // there): no step crosses a report time, and the step size carries from
     // one report interval to the next.  yout (nullable, n rows of nq+nu)
     // receives the state at each report time.
+    //
+    // reports (nullable, nrep increasing times in [t0, last stop]): Simbody's
+    // default reporting with interpolation on (AbstractIntegratorRep.cpp:
+    // 63-93, 310-330): steps ignore the report times; after each accepted
+    // step every report time r in (t_prev, t] gets the step's end state if
+    // r == t, else the cubic Hermite interpolant of (t_prev, y_prev, f_prev)
+    // and (t, y, f(y)) (interpolateOrder3, IntegratorRep.h:285-313), whose
+    // quaternions are then normalized when their RMS norm error exceeds the
+    // constraint tolerance (projectQ on the interpolated state,
+    // SimbodyMatterSubsystemRep.cpp:4151-4160).  rout: nrep rows.
     RkmStats integrateStops(T* y, double t0, const double* stops, int nstops, const RkmConfig& cfg,
-                            const SV<T>* Fapp, T* yout) {
+                            const SV<T>* Fapp, T* yout, const double* reports = nullptr, int nrep = 0,
+                            T* rout = nullptr) {
         RkmStats st;
-        int k = 0;
+        int k = 0, kr = 0;
         const int ny = M.nq + M.nu;
         const double acc = cfg.accuracy;
         const double consTol = acc/10;
@@ -1249,6 +1260,8 @@ template <class T> struct Rkm {
         if (hmin != -1) hcur = std::max(hcur, hmin);
         if (hmax != -1) hcur = std::min(hcur, hmax);
         const double projectionLimit = std::max(2*consTol, std::sqrt(consTol));
+        for (; kr < nrep && reports[kr] <= t; ++kr)
+            for (int i=0;i<ny;++i) rout[(size_t)kr*ny + i] = y[i];
         for (;;) {
             while (k < nstops && t >= stops[k]) {
                 if (yout) for (int i=0;i<ny;++i) yout[(size_t)k*ny + i] = y[i];
@@ -1302,6 +1315,19 @@ template <class T> struct Rkm {
                 hcur = hnew;
                 if (!succeeded) ++st.rejected;
             } while (!succeeded);
+            if (kr < nrep && reports[kr] <= t1) {
+                std::vector<T> f1(ny);
+                derivs(tr, ws, y1.data(), Fapp, f1.data());
+                for (; kr < nrep && reports[kr] <= t1; ++kr) {
+                    T* yt = rout + (size_t)kr*ny;
+                    if (reports[kr] == t1) { for (int i=0;i<ny;++i) yt[i] = y1[i]; continue; }
+                    const T h = T(t1 - t), d = T((reports[kr] - t) / (t1 - t));
+                    const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
+                    const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
+                    for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*f1[i];
+                    if (M.nquat > 0 && double(quatErrRMS(yt)) > consTol) normalizeQuats(yt, (T*)nullptr);
+                }
+            }
             t = t1;
             for (int i=0;i<ny;++i) y[i] = y1[i];
             ++st.accepted;

@@ -651,6 +651,68 @@ int tdyn_rollout_extend(tdyn_batch* b, const double* t_end, double t_end_all, hi
     return TDYN_OK;
 }
 
+int tdyn_rollout_reports(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
+                         const double* report_times, int n_reports, const tdyn_integrator_cfg* cfg,
+                         double* q_reports, double* u_reports, int32_t* stats, hipStream_t s) {
+    if (!b || !q || !u || !cfg || n_reports < 0 || (n_reports > 0 && (!report_times || !q_reports || !u_reports)))
+        return fail(TDYN_E_ARG, "tdyn_rollout_reports: batch, q, u, cfg and (with reports) times and outputs are required");
+    std::vector<double> rep(report_times, report_times + n_reports);
+    for (int k = 0; k < n_reports; ++k)
+        if (!std::isfinite(rep[k]) || rep[k] < t0 || rep[k] > t_end || (k && !(rep[k] > rep[k-1])))
+            return fail(TDYN_E_ARG, "tdyn_rollout_reports: report times must increase within [t0, t_end]");
+    int rc = tdyn_rollout_begin(b, q, u, fapp, t0, nullptr, t_end, cfg, s);
+    if (rc) return rc;
+    const int L = b->n_inst;
+    const size_t TQ = (size_t)b->total_q, TU = (size_t)b->total_u;
+    // the two ends of the current step: packed q, u, qdot, udot and lane times
+    DBuf<double> ends[2][4], tl[2], rt;
+    DBuf<int> next_k;
+    for (int e = 0; e < 2; ++e) {
+        HIPCHK(ends[e][0].alloc(TQ)); HIPCHK(ends[e][1].alloc(TU));
+        HIPCHK(ends[e][2].alloc(TQ)); HIPCHK(ends[e][3].alloc(TU)); HIPCHK(tl[e].alloc(L));
+    }
+    HIPCHK(rt.upload(rep.empty() ? std::vector<double>{0.0} : rep));
+    int cur = 0;
+    if ((rc = tdyn_rollout_get(b, ends[0][0].p, ends[0][1].p, nullptr, nullptr, nullptr, s)) ||
+        (rc = tdyn_rollout_derivs(b, ends[0][2].p, ends[0][3].p, s)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(tl[0].p, b->t.p, (size_t)L * sizeof(double), hipMemcpyDeviceToDevice, s));
+    int k0 = 0;                                    // reports at t0: the initial state
+    for (; k0 < n_reports && rep[k0] <= t0; ++k0) {
+        HIPCHK(hipMemcpyAsync(q_reports + k0 * TQ, ends[0][0].p, TQ * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipMemcpyAsync(u_reports + k0 * TU, ends[0][1].p, TU * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    HIPCHK(next_k.upload(std::vector<int>((size_t)L, k0)));
+    const DevTables T = b->tables();
+    std::vector<int> st((size_t)L);
+    for (long step = 0;; ++step) {
+        if (step > 100000000L) return fail(TDYN_E_STEP_FAILED, "rollout did not finish");
+        if ((rc = tdyn_rollout_advance(b, 1, s))) return rc;
+        const int nx = cur ^ 1;
+        if ((rc = tdyn_rollout_get(b, ends[nx][0].p, ends[nx][1].p, nullptr, nullptr, nullptr, s)) ||
+            (rc = tdyn_rollout_derivs(b, ends[nx][2].p, ends[nx][3].p, s)))
+            return rc;
+        if (n_reports > k0) {
+            const double* prev[4] = {ends[cur][0].p, ends[cur][1].p, ends[cur][2].p, ends[cur][3].p};
+            const double* now[4] = {ends[nx][0].p, ends[nx][1].p, ends[nx][2].p, ends[nx][3].p};
+            HIPCHK(launch_report_interp(T, b->uniform ? nullptr : b->tree_of.p, b->q_off.p, b->u_off.p, L, rt.p,
+                                        n_reports, next_k.p, tl[cur].p, b->t.p, prev, now, b->cfg.cons_tol,
+                                        q_reports, u_reports, (long)TQ, (long)TU, s));
+        }
+        HIPCHK(hipMemcpyAsync(tl[nx].p, b->t.p, (size_t)L * sizeof(double), hipMemcpyDeviceToDevice, s));
+        cur = nx;
+        HIPCHK(hipMemcpyAsync(st.data(), b->status.p, (size_t)L * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool running = false, failed = false;
+        for (int x : st) { running |= x == 0; failed |= x < 0; }
+        if (!running) {
+            if ((rc = tdyn_rollout_get(b, q, u, nullptr, nullptr, stats, s))) return rc;
+            HIPCHK(hipStreamSynchronize(s));
+            return failed ? fail(TDYN_E_STEP_FAILED, "Integrator::StepFailed for at least one instance") : TDYN_OK;
+        }
+    }
+}
+
 int tdyn_rollout(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
                  const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s) {
     int rc = tdyn_rollout_begin(b, q, u, fapp, t0, nullptr, t_end, cfg, s);

@@ -244,6 +244,11 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s);
 hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s);
 hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, const double* t, double* t_end,
                                  int* status, hipStream_t s);
+// prev/now: packed q, u, qdot, udot (instance order) at a step's two ends
+hipError_t launch_report_interp(const DevTables& T, const int* tree_of, const int* q_off, const int* u_off, int L,
+                                const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
+                                const double* const prev[4], const double* const now[4], double cons_tol,
+                                double* q_out, double* u_out, long total_q, long total_u, hipStream_t s);
 hipError_t launch_lane_scatter(const int* inst, int n, const double* t_lane, double* t_inst, const int* st_lane,
                                int* st_inst, const int* stats_lane, int* stats_inst, hipStream_t s);
 

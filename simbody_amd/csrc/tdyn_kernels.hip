@@ -1680,6 +1680,77 @@ hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, co
     k_rollout_extend<<<dim3((n + 255)/256), dim3(256), 0, s>>>(n, te_lane, te_all, t, t_end, status);
     return hipGetLastError();
 }
+// Reports by interpolation after one accepted step per lane (Simbody's
+// default reporting, AbstractIntegratorRep.cpp:63-93, 310-330): every report
+// time r in (t_prev, t_now] of the lane's instance gets the step's end state
+// if r == t_now, else the cubic Hermite interpolant (interpolateOrder3,
+// IntegratorRep.h:285-313) of the step's two ends, then projectQ of the
+// interpolated state: its quaternions normalized when their RMS norm error
+// exceeds cons_tol (SimbodyMatterSubsystemRep.cpp:4151-4160).  Packed
+// instance-order states and derivatives; next_k: the lane's next report.
+__global__ void k_report_interp(DevTables T, const int* tree_of, const int* q_off, const int* u_off, int L,
+                                const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
+                                const double* qp, const double* up, const double* fqp, const double* fup,
+                                const double* qn, const double* un, const double* fqn, const double* fun,
+                                double cons_tol, double* q_out, double* u_out, long total_q, long total_u) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L) return;
+    const double t0 = t_prev[l], t1 = t_now[l];
+    int k = next_k[l];
+    if (!(t1 > t0) || k >= nrep || rt[k] > t1) return;
+    const int tr = tree_of ? tree_of[l] : 0;
+    const int4 ta = ldi4(T.tree_a, tr);
+    const int nb = ta.x, nq = ta.y, nu = ta.z, nquat = ldi4(T.tree_b, tr).y;
+    const int qo = q_off[l], uo = u_off[l];
+    for (; k < nrep && rt[k] <= t1; ++k) {
+        double* oq = q_out + (size_t)k * total_q + qo;
+        double* ou = u_out + (size_t)k * total_u + uo;
+        if (rt[k] == t1) {
+            for (int j = 0; j < nq; ++j) oq[j] = qn[qo + j];
+            for (int j = 0; j < nu; ++j) ou[j] = un[uo + j];
+            continue;
+        }
+        const double h = t1 - t0, d = (rt[k] - t0) / h;
+        const double cy1 = d*d*(3.0 - 2.0*d), cy0 = 1.0 - cy1;
+        const double hdd1 = h*d*(d - 1.0), cf1 = hdd1*d, cf0 = cf1 - hdd1;
+        for (int j = 0; j < nq; ++j)
+            oq[j] = ((cy0*qp[qo + j] + cy1*qn[qo + j]) + cf0*fqp[qo + j]) + cf1*fqn[qo + j];
+        for (int j = 0; j < nu; ++j)
+            ou[j] = ((cy0*up[uo + j] + cy1*un[uo + j]) + cf0*fup[uo + j]) + cf1*fun[uo + j];
+        if (nquat > 0) {
+            auto quat = [&](int b) {
+                const int4 ba = ldi4(T.body_a, ta.w + b);
+                return (ba.y == BALL || ba.y == FREE) && T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] == 0.0;
+            };
+            double s2 = 0;
+            for (int b = 0; b < nb; ++b) {
+                if (!quat(b)) continue;
+                const double* qq = oq + ldi4(T.body_a, ta.w + b).z;
+                const double e = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - 1.0;
+                s2 += e*e;
+            }
+            if (sqrt(s2 / nquat) > cons_tol) {
+                for (int b = 0; b < nb; ++b) {
+                    if (!quat(b)) continue;
+                    double* qq = oq + ldi4(T.body_a, ta.w + b).z;
+                    const double n = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]);
+                    for (int c = 0; c < 4; ++c) qq[c] = qq[c] / n;
+                }
+            }
+        }
+    }
+    next_k[l] = k;
+}
+hipError_t launch_report_interp(const DevTables& T, const int* tree_of, const int* q_off, const int* u_off, int L,
+                                const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
+                                const double* const prev[4], const double* const now[4], double cons_tol,
+                                double* q_out, double* u_out, long total_q, long total_u, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_report_interp<<<dim3((L + 127)/128), dim3(128), 0, s>>>(T, tree_of, q_off, u_off, L, rt, nrep, next_k, t_prev,
+        t_now, prev[0], prev[1], prev[2], prev[3], now[0], now[1], now[2], now[3], cons_tol, q_out, u_out,
+        total_q, total_u);
+    return hipGetLastError();
+}
 hipError_t launch_lane_gather(const int* inst, int n, const double* src_inst, double* dst_lane, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     k_lane_gather<<<dim3((n + 255)/256), dim3(256), 0, s>>>(inst, n, src_inst, dst_lane);

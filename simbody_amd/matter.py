@@ -38,7 +38,8 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
            "tdyn_multiply_by_system_jacobian_transpose", "tdyn_calc_body_acceleration_from_udot",
            "tdyn_calc_composite_body_inertias", "tdyn_rollout_begin", "tdyn_rollout_advance",
-           "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_rollout_extend", "tdyn_last_error",
+           "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_rollout_extend", "tdyn_rollout_reports",
+           "tdyn_last_error",
            "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
            "tdyn_server_advance", "tdyn_server_collect"]
 
@@ -108,6 +109,8 @@ def lib():
                                          ctypes.POINTER(IntegratorCfgC), vp]
         L.tdyn_rollout_advance.argtypes = [vp, ctypes.c_int, vp]
         L.tdyn_rollout_extend.argtypes = [vp, vp, ctypes.c_double, vp]
+        L.tdyn_rollout_reports.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_int,
+                                           ctypes.POINTER(IntegratorCfgC), vp, vp, vp, vp]
         L.tdyn_rollout_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.tdyn_rollout_derivs.argtypes = [vp, vp, vp, vp]
         L.tdyn_rollout.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
@@ -452,6 +455,25 @@ class MatterBatch:
                 raise TdynError(5, "Integrator::StepFailed for at least one instance")
             qs.append(q1); us.append(u1)
         return qs, us, stats
+
+    def stepToWithReports(self, q, u, integ, t_end, report_times, fapp=None, t0=0.0):
+        """stepTo(t_end) with reports by interpolation (Simbody's default):
+        the trajectory of stepTo(t_end), and at each report time the Hermite
+        interpolant of the step that reaches it.  Returns (q(t_end),
+        u(t_end), [q at each report], [u at each report], stats[n,4])."""
+        torch = self.torch
+        q = q.clone(); u = u.clone()
+        times = np.ascontiguousarray(report_times, dtype=np.float64)
+        n = len(times)
+        Qr = self._new(max(n, 1) * self.total_q); Ur = self._new(max(n, 1) * self.total_u)
+        stats = torch.zeros(4 * self.n_instances, dtype=torch.int32, device=self.device)
+        cfg = integ.c()
+        _check(lib().tdyn_rollout_reports(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.c_double(t0),
+                                          ctypes.c_double(t_end), times.ctypes.data_as(ctypes.c_void_p),
+                                          ctypes.c_int(n), ctypes.byref(cfg), _ptr(Qr), _ptr(Ur), _ptr(stats),
+                                          _stream()))
+        Qr = Qr.view(-1, self.total_q); Ur = Ur.view(-1, self.total_u)
+        return q, u, [Qr[k] for k in range(n)], [Ur[k] for k in range(n)], stats.view(-1, 4)
 
     def rollout_get(self):
         torch = self.torch

@@ -5,7 +5,10 @@ integrateStops: the state at every report time within 1e-8 and the final
 accepted, rejected and projection counts identical, on every evaluator
 (tree sweeps, streaming, level-parallel, chain), with one set of report
 times for the batch and with per-instance final times; an earlier final
-time leaves a finished instance where it is."""
+time leaves a finished instance where it is.  Reports by interpolation
+(tdyn_rollout_reports, Simbody's default) against the oracle's
+integrateStops with reports: the plain rollout's trajectory and counts, and
+each Hermite report (quaternions projected) within 1e-8."""
 import numpy as np
 import pytest
 
@@ -91,3 +94,48 @@ def test_per_instance_final_times_and_backward_extend():
     B.rollout_advance(1 << 20)
     qc, uc, tc, sc, _ = B.rollout_get()
     assert bool((sc == 1).all()) and torch.equal(qc, qb) and torch.equal(uc, ub) and torch.equal(tc, tb)
+
+
+RTIMES = [0.0, 0.013, 0.05, 0.0777, 0.1]
+
+
+@pytest.mark.parametrize("mode,lanes,active", [(1, 0, 1), (3, 1, 3), (3, 8, 4)])
+def test_interpolated_reports_on_tree_evaluators(mode, lanes, active):
+    """tdyn_rollout_reports (interpolation on): the trajectory and counts of
+    the plain rollout, and at each report time the oracle's Hermite report
+    (quaternions projected) within 1e-8."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    trees, qs, us, fs = _ragged(441)
+    n = len(trees)
+    B = MatterBatch(trees, tree_of=np.arange(n, dtype=np.int32))
+    B.set_evaluator(mode); B.set_tree_lanes(lanes)
+    q1, u1, Q, U, st = B.stepToWithReports(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-4), 0.1, RTIMES,
+                                           fapp=B.pack(fs))
+    assert B.active_evaluator()[0] == active
+    st = st.cpu().numpy()
+    for i, t in enumerate(trees):
+        R, yT, s = OracleModel(t).integrate_reports(np.concatenate([qs[i], us[i]]), 0.1, RTIMES, 1e-4, Fapp=fs[i])
+        assert (s["accepted"], s["rejected"], s["projections"]) == tuple(st[i, 1:4]), (i, s, st[i])
+        assert rel_inf(np.concatenate([B.split_q(q1)[i], B.split_u(u1)[i]]), yT) < 1e-8, i
+        for k in range(len(RTIMES)):
+            y = np.concatenate([B.split_q(Q[k])[i], B.split_u(U[k])[i]])
+            assert rel_inf(y, R[k]) < 1e-8, (i, k)
+
+
+def test_interpolated_reports_on_chain_evaluator():
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    spec = M.spatial_chain(12, seed=11)
+    n = 128
+    B = MatterBatch([spec], n_instances=n)
+    Y0 = np.stack([np.concatenate(M.random_state(spec, 443, i)) for i in range(n)])
+    q1, u1, Q, U, st = B.stepToWithReports(B.to_device(Y0[:, :spec.nq]), B.to_device(Y0[:, spec.nq:]),
+                                           RungeKuttaMerson(accuracy=1e-4), 0.1, RTIMES)
+    assert B.active_evaluator()[0] == 2
+    st = st.cpu().numpy()
+    o = OracleModel(spec)
+    for i in range(0, n, 16):
+        R, yT, s = o.integrate_reports(Y0[i], 0.1, RTIMES, 1e-4)
+        assert (s["accepted"], s["rejected"], s["projections"]) == tuple(st[i, 1:4]), (i, s, st[i])
+        for k in range(len(RTIMES)):
+            y = np.concatenate([Q[k].cpu().numpy().reshape(n, -1)[i], U[k].cpu().numpy().reshape(n, -1)[i]])
+            assert rel_inf(y, R[k]) < 1e-8, (i, k)
