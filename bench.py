@@ -221,7 +221,7 @@ def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 5000, 20000, 50000), wi
 
 
 def row_operators(torch, M, MatterBatch, n=65536, reps=5, cpu=True, cpu_sample=2000):
-    """Single-shot operators on the config-2 chains (65,536 x 32 bodies, one
+    """Single-shot operators on the chain workload (65,536 x 32 bodies, one
     model for the batch): kernel time per call (HIP events on the launch
     stream, mean of `reps` calls after one warm call), bodies/s, and the
     boundary bytes (inputs and outputs of the call) per second."""
@@ -246,7 +246,7 @@ def row_operators(torch, M, MatterBatch, n=65536, reps=5, cpu=True, cpu_sample=2
                                          nq + 2 * nu + 6 * nbp),
         "calcCompositeBodyInertias": (lambda: B.calcCompositeBodyInertias(q, out=ob), nq + 10 * nbp),
     }
-    out = {"workload": f"{n} chains x {spec.nb} bodies (config-2 model and states), FP64, one call",
+    out = {"workload": f"{n} chains x {spec.nb} bodies (the chain rollout's model and states), FP64, one call",
            "ops": {}}
     for name, (call, dbl) in ops.items():
         call(); torch.cuda.synchronize()

@@ -1,5 +1,5 @@
 // Example caller of the C++ drop-in (include/tdyn_matter.hpp): the README
-// pendulum of the reference extended to 10 Pin links (config 1), written the
+// pendulum of the reference extended to 10 Pin links, written the
 // way a Simbody user writes it -- build the tree, realize(Acceleration), then
 // TimeStepper::stepTo with RungeKuttaMerson at accuracy 1e-8.  Also prints
 // M*udot (multiplyByM) and the composite mass of body 1

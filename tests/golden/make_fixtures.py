@@ -5,7 +5,7 @@ external LAPACK), so they pin the oracle against silent change and give the
 GPU tests fixed expected values; the oracle itself is pinned by
 tests/test_oracle.py.
 
-  pendulum10.json : config 1 (10-link planar Pin chain), seeded state, udot,
+  pendulum10.json : 10-link planar Pin chain, seeded state, udot,
                     and q,u after RK-Merson accuracy 1e-8 to t = 1 with the
                     accept/reject counts.
 """
@@ -30,7 +30,7 @@ def pendulum10():
     q = rng.uniform(-np.pi, np.pi, 10); u = rng.normal(size=10)
     ud, qd, qdd, _ = o.realize_acceleration(q, u)
     y, st = o.integrate(np.concatenate([q, u]), 1.0, 1e-8)
-    return {"what": "config 1: 10-link planar Pin chain (simbody_amd.models.pendulum_chain(10)); "
+    return {"what": "10-link planar Pin chain (simbody_amd.models.pendulum_chain(10)); "
                     "oracle udot at (q,u) and RK-Merson accuracy 1e-8 to t=1",
             "q": q.tolist(), "u": u.tolist(), "udot": ud.tolist(),
             "q1": y[:10].tolist(), "u1": y[10:].tolist(),

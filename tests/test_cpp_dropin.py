@@ -1,5 +1,5 @@
 """The C++ drop-in header (include/tdyn_matter.hpp) used as a Simbody user
-would: the 10-link pendulum of config 1, realize(Acceleration) and
+would: the 10-link pendulum, realize(Acceleration) and
 TimeStepper::stepTo with RK-Merson at accuracy 1e-8, compared with the
 committed fixture (oracle results); multiplyByM and the composite mass of
 body 1 at the initial state."""

@@ -112,7 +112,7 @@ def test_reaction_forces():
 
 
 def test_shared_chain_batch_udot_and_residual():
-    """Config-2 shape (one 32-body Pin/Universal/Ball chain model), many instances."""
+    """One 32-body Pin/Universal/Ball chain model, many instances."""
     spec = M.spatial_chain(32, seed=2)
     n = 256
     B = _batch([spec], n=n)
@@ -153,7 +153,7 @@ def test_fixed_step_rollout_matches_oracle():
 
 
 def test_adaptive_pendulum_rollout_matches_oracle():
-    """Config 1: 10-link planar pendulum, RK-Merson accuracy 1e-8 to t = 1."""
+    """10-link planar pendulum, RK-Merson accuracy 1e-8 to t = 1."""
     from simbody_amd.matter import RungeKuttaMerson
     spec = M.pendulum_chain(10)
     n = 8
@@ -170,7 +170,7 @@ def test_adaptive_pendulum_rollout_matches_oracle():
 
 
 def test_adaptive_tree_rollout_with_forces():
-    """Config-3 shape: uneven trees (Free roots included), applied forces, accuracy 1e-6."""
+    """Uneven trees (Free roots included), applied forces, accuracy 1e-6."""
     from simbody_amd.matter import RungeKuttaMerson
     trees = mixed_trees(31, 6, nmin=8, nmax=24)
     B = _batch(trees, np.arange(6, dtype=np.int32))
@@ -228,7 +228,7 @@ def test_chain_evaluator_free_root_matches_oracle():
 
 
 def test_pendulum10_fixture_on_gpu():
-    """Config 1 fixture (tests/golden/pendulum10.json): udot at 1e-12, the
+    """10-link pendulum fixture (tests/golden/pendulum10.json): udot at 1e-12, the
     adaptive RK-Merson run to t=1 with identical step counts and q(1) at 1e-9."""
     import json, os
     from simbody_amd.matter import RungeKuttaMerson

@@ -59,7 +59,7 @@ def test_sdfast_reaction_forces():
 # Closed forms
 # ---------------------------------------------------------------------------
 def test_single_pendulum_closed_form():
-    """One Pin link of config 1: theta'' = -m g L sin(theta) / (I + m L^2) = -4.9 sin(theta)."""
+    """One Pin link of the planar pendulum: theta'' = -m g L sin(theta) / (I + m L^2) = -4.9 sin(theta)."""
     o = OracleModel(M.pendulum_chain(1))
     for th in np.linspace(-3.0, 3.0, 13):
         for w in (-2.0, 0.0, 1.5):
@@ -270,7 +270,7 @@ def test_quaternion_projection_cadence():
 
 
 def test_pendulum10_regression_fixture():
-    """The oracle still reproduces the committed config-1 fixture."""
+    """The oracle still reproduces the committed 10-link pendulum fixture."""
     g = json.load(open(os.path.join(HERE, "golden", "pendulum10.json")))
     o = OracleModel(M.pendulum_chain(10))
     q, u = np.array(g["q"]), np.array(g["u"])
