@@ -70,7 +70,7 @@ extern "C" {
 typedef struct tdyn_tree_desc {
     int32_t nb;                /* mobilized bodies (Ground excluded) */
     const int32_t* parent;     /* [nb] parent body number, 0 = Ground, parent[b-1] < b */
-    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_BUSHING (Free and Bushing only on Ground) */
+    const int32_t* mobilizer;  /* [nb] TDYN_PIN .. TDYN_BUSHING (Bushing only on Ground) */
     const double* mass;        /* [nb] */
     const double* com_B;       /* [nb][3] mass center in B */
     const double* inertia_B;   /* [nb][6] inertia about Bo in B: xx yy zz xy xz yz */

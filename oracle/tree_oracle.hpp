@@ -249,7 +249,7 @@ struct Model {
             if (p < 0 || p >= b) return false;
             const int t = d.mobilizer[i];
             if (t < PIN || t > BUSHING) return false;
-            if ((t == FREE || t == BUSHING) && p != 0) return false;
+            if (t == BUSHING && p != 0) return false;
             parent[b] = p; type[b] = t; dof[b] = mobDof(t);
             level[b] = level[p] + 1;
             children[p].push_back(b);

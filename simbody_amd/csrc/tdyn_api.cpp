@@ -37,7 +37,7 @@ int validate_tree(const tdyn_tree_desc& d, int t) {
         if (ty == TDYN_SCREW && (!d.screw_pitch || !std::isfinite(d.screw_pitch[b-1]))) {
             snprintf(buf, sizeof buf, "tree %d body %d: Screw needs a finite screw_pitch", t, b); return fail(TDYN_E_ARG, buf);
         }
-        if ((ty == TDYN_FREE || ty == TDYN_BUSHING) && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: %s mobilizer allowed on Ground only in this build", t, b, ty == TDYN_FREE ? "Free" : "Bushing"); return fail(TDYN_E_ARG, buf); }
+        if (ty == TDYN_BUSHING && p != 0) { snprintf(buf, sizeof buf, "tree %d body %d: Bushing mobilizer allowed on Ground only in this build", t, b); return fail(TDYN_E_ARG, buf); }
         if ((ty == TDYN_FREE || ty == TDYN_BENDSTRETCH || ty == TDYN_BUSHING) && d.reversed && d.reversed[b-1]) { snprintf(buf, sizeof buf, "tree %d body %d: a reversed %s mobilizer is not supported in this build", t, b, ty == TDYN_FREE ? "Free" : ty == TDYN_BUSHING ? "Bushing" : "BendStretch"); return fail(TDYN_E_ARG, buf); }
         if (!(d.mass[b-1] > 0) || !std::isfinite(d.mass[b-1])) { snprintf(buf, sizeof buf, "tree %d body %d: mass must be positive and finite", t, b); return fail(TDYN_E_ARG, buf); }
         if (!finite_all(d.com_B + 3*(b-1), 3) || !finite_all(d.inertia_B + 6*(b-1), 6) ||
@@ -219,7 +219,8 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             const int ty = d.mobilizer[b-1];
             // the kernels carry a Bushing as a Free body flagged PRM_EULER = 2
             const int dty = ty == TDYN_BUSHING ? TDYN_FREE : ty;
-            chain = chain && d.parent[b-1] == b - 1 && !mob_generic_only(ty);
+            // (the chain evaluator carries a Free body as the first body only)
+            chain = chain && d.parent[b-1] == b - 1 && !mob_generic_only(ty) && !(ty == TDYN_FREE && b > 1);
             B->h_body_a.push_back(make_int4(d.parent[b-1], dty, q, u));
             body_b.push_back(make_int4((int)child_idx.size(), (int)ch[b].size(), rec, gslot));
             for (int c : ch[b]) child_idx.push_back(c);

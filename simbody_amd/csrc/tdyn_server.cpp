@@ -63,6 +63,8 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
             return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs serial-chain topologies (parent[b-1] == b-1)");
         if (mob_generic_only(topology->mobilizer[b-1]))
             return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs Pin, Slider, Universal, Ball and Free bodies");
+        if (topology->mobilizer[b-1] == TDYN_FREE && b > 1)
+            return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs a Free body as the first body only");
     }
     if (any_reversed(*topology))
         return fail(TDYN_E_ARG, "tdyn_server_create: the serving path runs forward mobilizers only");

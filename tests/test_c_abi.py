@@ -43,8 +43,8 @@ def test_invalid_trees_are_rejected_on_the_host():
     spec = M.spatial_chain(4, seed=1)
     bad_parent = M.TreeSpec(**{**spec.__dict__, "parent": np.array([0, 2, 1, 2], np.int32)})
     bad_mass = M.TreeSpec(**{**spec.__dict__, "mass": np.array([1.0, -1.0, 1.0, 1.0])})
-    bad_free = M.TreeSpec(**{**spec.__dict__, "mobilizer": np.array([0, 4, 0, 0], np.int32)})
-    for s, code in [(bad_parent, 1), (bad_mass, 1), (bad_free, 1)]:
+    bad_bushing = M.TreeSpec(**{**spec.__dict__, "mobilizer": np.array([0, M.BUSHING, 0, 0], np.int32)})
+    for s, code in [(bad_parent, 1), (bad_mass, 1), (bad_bushing, 1)]:
         with pytest.raises(matter.TdynError) as e:
             _create(L, s)
         assert e.value.code == code
