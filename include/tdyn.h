@@ -172,8 +172,10 @@ int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const
 /* Known udot for rollouts (locks: known udot 0 with the locked u at its
  * value, usually 0; prescribed constant udot): the device arrays of
  * tdyn_calc_acceleration_with_known_udot, kept by pointer (the caller keeps
- * them alive) and used by every later tdyn_rollout_begin / advance / derivs
- * of this batch; NULL, NULL clears.  No Simbody counterpart as a call: what
+ * them alive while a rollout begun with them runs).  They take effect at the
+ * next tdyn_rollout_begin: a rollout keeps the arrays (and the evaluator
+ * setting) of its begin for every advance, get and derivs; NULL, NULL
+ * clears.  No Simbody counterpart as a call: what
  * MobilizedBody::lock(state) and Motion prescriptions do to the State.
  * Such rollouts run in FP64 on the tree sweeps or, for a batch of different
  * trees, on the level-parallel tree evaluator (evaluator 0 or 3 with lanes
@@ -270,7 +272,9 @@ int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const d
 /* TimeStepper::initialize (Integrator.cpp:333-435): load q,u (packed),
  * constant applied body forces fapp (NULL: none), start time t0 and
  * per-instance final times t_end (device, NULL: all t_end_all).  The state
- * stays resident on the device until tdyn_rollout_get. */
+ * stays resident on the device until tdyn_rollout_get.  A begin refused for
+ * its arguments or settings (TDYN_E_ARG) leaves the rollout in progress as it
+ * was; one that fails later (TDYN_E_HIP, TDYN_E_NOMEM) ends it. */
 int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const double* fapp,
                        double t0, const double* t_end, double t_end_all,
                        const tdyn_integrator_cfg* cfg, hipStream_t s);

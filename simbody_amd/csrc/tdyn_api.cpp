@@ -151,16 +151,17 @@ int build_tree_plan(tdyn_batch* b, hipStream_t s) {
     return 0;
 }
 
-int ensure_rollout_buffers(tdyn_batch* b) {
+int ensure_rollout_buffers(tdyn_batch* b, int precision) {
     const int ny = b->max_nq + b->max_nu;
     const size_t L = b->n_inst;
     hipError_t e;
-    if (b->precision == 1) {
+    if (precision == 1) {
         const size_t n32 = (size_t)(6 * b->max_nu + 4 * b->max_nb) * L;
         if (b->scratch32.n < n32 && (e = b->scratch32.alloc(n32)) != hipSuccess)
             return hipfail(e, "mixed-precision scratch allocation");
     }
-    if (b->rollout_ready && b->ny_max == ny) return TDYN_OK;
+    if (b->vec.p && b->ny_max == ny) return TDYN_OK;
+    b->ny_max = 0;
     // 8 extra rows: the chain evaluator prefetches fixed-size q/u windows
     if ((e = b->vec.alloc((size_t)11 * ny * L + 8 * L)) != hipSuccess ||
         (e = b->uscale.alloc((size_t)std::max(1, b->max_nu) * L)) != hipSuccess ||
@@ -169,7 +170,6 @@ int ensure_rollout_buffers(tdyn_batch* b) {
         (e = b->stats.alloc(4 * L)) != hipSuccess)
         return hipfail(e, "rollout state allocation");
     b->ny_max = ny;
-    b->rollout_ready = true;
     return TDYN_OK;
 }
 } // namespace host
@@ -567,21 +567,27 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
                        const tdyn_integrator_cfg* cfg, hipStream_t s) {
     if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_rollout_begin: batch, q, u and cfg are required");
     if (!t_end && !(t_end_all > t0)) return fail(TDYN_E_ARG, "tdyn_rollout_begin: final time must exceed t0");
-    { int prec = 0; const int rc = make_rkm_cfg(cfg, b->cfg, prec); if (rc) return rc; b->precision = prec; }
-    if (b->precision == 1 && !(b->is_chain && b->evaluator != 1))
+    // Every check runs on locals first: a refused begin leaves the rollout in
+    // progress (settings, layout, state) as it was.
+    RkmCfg c;
+    int prec = 0;
+    { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }
+    if (prec == 1 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
-    if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || b->precision == 1))
+    if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || prec == 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps or the level-parallel "
                                 "tree evaluator (not one lane per instance), in FP64");
     const size_t L = b->n_inst;
-    b->rollout_aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) && !(b->known_p && b->tree_lanes == 1);
-    b->rollout_grp = 0;
-    if (b->rollout_aos && b->tree_lanes != 1) {
+    const bool aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) && !(b->known_p && b->tree_lanes == 1);
+    const int grp = aos && b->tree_lanes != 1 ? 1 : 0;
+    // From here on the batch changes: a failure (allocation, upload) ends the
+    // rollout in progress.
+    b->rollout_ready = false;
+    if (grp) {
         const int rc = build_tree_plan(b, s);
         if (rc) return rc;
-        b->rollout_grp = 1;
     }
-    { const int rc = ensure_rollout_buffers(b); if (rc) return rc; }
+    { const int rc = ensure_rollout_buffers(b, prec); if (rc) return rc; }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));
         HIPCHK(hipMemcpyAsync(b->fapp.p, fapp, (size_t)b->total_b * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -596,9 +602,18 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         HIPCHK(hipMemcpyAsync(b->t_end.p, te.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
     }
+    // commit: the rollout keeps these settings until the next begin
+    b->cfg = c;
+    b->precision = prec;
+    b->rollout_aos = aos;
+    b->rollout_grp = grp;
+    b->roll_known_p = b->known_p;
+    b->roll_known_u = b->known_u;
+    b->roll_evaluator = b->evaluator;
     RkmArgs a = b->rk();
     HIPCHK(launch_pack_state(a, q, u, 1, nullptr, nullptr, s));
     HIPCHK(launch_rkm_init(a, t0, s));
+    b->rollout_ready = true;
     return TDYN_OK;
 }
 

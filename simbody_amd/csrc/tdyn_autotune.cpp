@@ -26,15 +26,18 @@ struct Candidate { int mode, lanes, lmin, lmax; };
 int ilog2(double x) { int k = 0; while (x >= 2.0) { x *= 0.5; ++k; } return k; }
 
 // Shape of the batch: instances, mean bodies, deepest body and mean widest
-// level (log2 buckets), and whether trees differ.
+// level (log2 buckets), whether trees differ, whether they carry the extended
+// bodies (reversed, Euler-angle, BendStretch, Bushing) and whether known udot
+// is set (which rules out the streaming evaluator).
 std::string shape_key(const tdyn_batch* b) {
     double w = 0;
     for (int t = 0; t < b->n_trees; ++t) w += b->h_tree_maxw.empty() ? 1 : b->h_tree_maxw[t];
     w /= std::max(1, b->n_trees);
-    char buf[160];
-    snprintf(buf, sizeof buf, "tree-lanes n%d nb%d depth%d width%d trees%d ext%d", ilog2(b->n_inst),
+    char buf[192];
+    snprintf(buf, sizeof buf, "tree-lanes n%d nb%d depth%d width%d trees%d ext%d known%d", ilog2(b->n_inst),
              ilog2((double)b->total_b / b->n_inst - 1.0), ilog2(b->dmax + 1.0), ilog2(w),
-             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch || b->any_bushing || b->known_p) ? 1 : 0);
+             b->n_trees > 1 ? 1 : 0, (b->any_rev || b->any_euler || b->any_bendstretch || b->any_bushing) ? 1 : 0,
+             b->known_p ? 1 : 0);
     return buf;
 }
 
@@ -120,7 +123,13 @@ int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const d
     const std::string key = shape_key(b);
     Candidate best = cands[0];
     int from_cache = 0;
-    if (cands.size() > 1 && cache_lookup(cache_path, key, best)) {
+    Candidate hit{};
+    auto same = [&](const Candidate& c) {
+        return c.mode == hit.mode && c.lanes == hit.lanes && c.lmin == hit.lmin && c.lmax == hit.lmax;
+    };
+    // a cached choice is used only if it is one of this batch's candidates
+    if (cands.size() > 1 && cache_lookup(cache_path, key, hit) && std::any_of(cands.begin(), cands.end(), same)) {
+        best = hit;
         from_cache = 1;
     } else if (cands.size() > 1) {
         float best_ms = 0;

@@ -61,6 +61,11 @@ struct tdyn_batch {
     bool any_bushing = false;     // some tree has a Bushing body
     const int* known_p = nullptr;     // tdyn_batch_set_known_udot: caller's device arrays (rollouts)
     const double* known_u = nullptr;
+    // what the current rollout was begun with: advance, get and derivs read
+    // these, never the settings changed since tdyn_rollout_begin
+    const int* roll_known_p = nullptr;
+    const double* roll_known_u = nullptr;
+    int roll_evaluator = 0;
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
     // device tables
@@ -118,10 +123,10 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (!known_p && (evaluator == 2 || (evaluator == 0 && is_chain))) ? 1 : 0;
-        a.presc = known_p; a.udot_known = known_u;
+        a.chain = (!roll_known_p && (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
+        a.presc = roll_known_p; a.udot_known = roll_known_u;
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.ext = (any_rev || any_euler || any_bendstretch || any_bushing || known_p) ? 1 : 0;
+        a.ext = (any_rev || any_euler || any_bendstretch || any_bushing || roll_known_p) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;
@@ -138,7 +143,8 @@ namespace host {
 // IntegratorRep.h:737-740 and the System time scale).
 int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision);
 // Allocates the device-resident integrator state of a batch (state vectors,
-// per-lane time, step, status and statistics) for its current sizes.
-int ensure_rollout_buffers(tdyn_batch* b);
+// per-lane time, step, status and statistics, and for precision 1 the
+// mixed path's hand-over rows) for its current sizes.
+int ensure_rollout_buffers(tdyn_batch* b, int precision);
 } // namespace host
 } // namespace tdyn

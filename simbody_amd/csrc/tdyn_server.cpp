@@ -96,8 +96,10 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
     if (rc) { delete S; return rc; }
     S->B->cfg = c;
     S->B->evaluator = 2;
+    S->B->roll_evaluator = 2;
     S->B->precision = prec;
-    if ((rc = ensure_rollout_buffers(S->B))) { delete S; return rc; }
+    if ((rc = ensure_rollout_buffers(S->B, prec))) { delete S; return rc; }
+    S->B->rollout_ready = true;
     hipError_t e;
     if ((e = S->prm_lane.alloc((size_t)(nb * PRM_STRIDE + 3) * n)) != hipSuccess ||
         (e = S->B->fapp.alloc((size_t)S->B->total_b * 6)) != hipSuccess ||
@@ -138,6 +140,13 @@ int tdyn_server_admit(tdyn_server* sv, int n, const tdyn_tree_desc* trees, const
             !std::equal(sv->parent.begin(), sv->parent.end(), d.parent) ||
             !std::equal(sv->mobilizer.begin(), sv->mobilizer.end(), d.mobilizer))
             return fail(TDYN_E_ARG, "tdyn_server_admit: request " + std::to_string(i) + " does not have the server's topology");
+        // the slot kernels read neither reversal nor Euler-angle flags: refuse them, as tdyn_server_create does
+        if (any_reversed(d))
+            return fail(TDYN_E_ARG, "tdyn_server_admit: request " + std::to_string(i) +
+                                    " has a reversed mobilizer; the serving path runs forward mobilizers only");
+        if (d.use_euler_angles)
+            return fail(TDYN_E_ARG, "tdyn_server_admit: request " + std::to_string(i) +
+                                    " uses Euler angles; the serving path runs quaternion Ball and Free bodies only");
         const int rc = validate_tree(d, i);
         if (rc) return rc;
         if (!std::isfinite(t_end[i])) return fail(TDYN_E_ARG, "tdyn_server_admit: non-finite t_end");
