@@ -242,6 +242,47 @@ int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T
     return st.status;
 }
 
+// The oracle's articulated-inertia shift (abiShift, the P' = Phi P Phi^T of
+// MassProperties.cpp:100-127) on one 21-double ABI (storeABI layout; the
+// symmetric blocks are read from their lower triangle).
+int oracle_abi_shift(const double* P, const double* s, double* out) {
+    ABI<double> A;
+    auto sym = [](const double* o) {
+        M33<double> X;
+        X(0,0)=o[0]; X(1,1)=o[1]; X(2,2)=o[2];
+        X(1,0)=X(0,1)=o[3]; X(2,0)=X(0,2)=o[4]; X(2,1)=X(1,2)=o[5];
+        return X;
+    };
+    A.M = sym(P); A.J = sym(P+6);
+    for (int i=0;i<9;++i) A.F.m[i] = P[12+i];
+    storeABI(abiShift(A, mk3(s[0], s[1], s[2])), out);
+    return 0;
+}
+
+// Integration to T recording every step attempt (Rkm::trace): rec gets up to
+// max_rec rows of 8 doubles (RkmAttempt), *n_rec the number of attempts.
+// init_h > 0 sets Integrator::setInitialStepSize.
+int oracle_integrate_trace(void* p, double* y, const double* Fapp, double t0, double T, double accuracy,
+                           double fixed_h, double init_h, double* rec, int max_rec, int* n_rec, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    std::vector<RkmAttempt> tr;
+    rk.trace = &tr;
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    if (init_h > 0) cfg.init_h = init_h;
+    RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    const int n = (int)std::min<size_t>(tr.size(), (size_t)std::max(0, max_rec));
+    for (int i=0;i<n;++i) {
+        const RkmAttempt& a = tr[i];
+        const double r[8] = {a.t0, a.t1, a.err_pre, a.qerr, a.err, a.projected, a.accepted, a.h_next};
+        std::copy(r, r + 8, rec + 8*(size_t)i);
+    }
+    *n_rec = (int)tr.size();
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
 // Integration through report times stops[0..n) (the last is the final time)
 // with interpolation off: every step stops at the next report time; yout
 // [n][nq+nu] gets the state there (Rkm::integrateStops).

@@ -45,7 +45,8 @@ def lib():
                      "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
                      "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
                      "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
-                     "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports"]:
+                     "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
+                     "oracle_integrate_trace", "oracle_abi_shift"]:
             getattr(L, name).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -212,6 +213,22 @@ class OracleModel:
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
         return y, dict(zip(keys, list(st)))
 
+    def integrate_trace(self, y, T, accuracy, fixed_h=0.0, init_h=0.0, Fapp=None, t0=0.0, max_rec=100000):
+        """Integration to T recording every step attempt; returns (y(T),
+        stats, attempts) with attempts a structured array of t0, t1, err_pre,
+        qerr, err, projected, accepted, h_next (Rkm::trace)."""
+        y = np.array(y, dtype=np.float64, copy=True)
+        rec = np.zeros((max_rec, 8))
+        n = ctypes.c_int(0)
+        st = (ctypes.c_long * 6)()
+        lib().oracle_integrate_trace(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+                                     ctypes.c_double(accuracy), ctypes.c_double(fixed_h), ctypes.c_double(init_h),
+                                     _p(rec), ctypes.c_int(max_rec), ctypes.byref(n), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        names = ["t0", "t1", "err_pre", "qerr", "err", "projected", "accepted", "h_next"]
+        tr = np.rec.fromarrays(rec[:min(n.value, max_rec)].T, names=names)
+        return y, dict(zip(keys, list(st))), tr
+
     def integrate_stops(self, y, stops, accuracy, fixed_h=0.0, Fapp=None, t0=0.0):
         """Integration through the report times `stops` with interpolation
         off; returns (states [len(stops), ny] at the report times, stats)."""
@@ -252,6 +269,15 @@ class OracleModel:
         lib().oracle_count_flops_ops(self.h, _p(q), _p(u), f, g)
         k = ["addsub", "mul", "div", "sqrt", "trig"]
         return dict(zip(k, list(f))), dict(zip(k, list(g)))
+
+
+def abi_shift(P, s):
+    """The oracle's articulated-inertia shift of one 21-double ABI (M, J as
+    xx yy zz xy xz yz, then F row-major) by the vector s."""
+    P = np.ascontiguousarray(P, np.float64); s = np.ascontiguousarray(s, np.float64)
+    out = np.zeros(21)
+    lib().oracle_abi_shift(_p(P), _p(s), _p(out))
+    return out
 
 
 def time_integrate_batch(models, model_of_inst, Y, T, accuracy, fixed_h, n_threads, Fapp=None):

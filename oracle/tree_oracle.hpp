@@ -1140,10 +1140,19 @@ struct RkmConfig {
 };
 struct RkmStats { long attempts=0, accepted=0, rejected=0, projections=0, evals=0; int status=0; };
 
+// One step attempt as the step controller saw it (test hook): the attempt's
+// start and end times, the error norm before the projection gate (NaN when
+// the stages did not converge), the quaternion error RMS(|q|-1) that the gate
+// read (-1 when the gate was skipped), the error norm the controller used
+// (+inf for a failed attempt), whether q was projected, whether the step was
+// accepted, and the step size the controller chose next.
+struct RkmAttempt { double t0, t1, err_pre, qerr, err, projected, accepted, h_next; };
+
 template <class T> struct Rkm {
     const Model& M;
     Tree<T> tr;
     Workspace<T> ws;
+    std::vector<RkmAttempt>* trace = nullptr;   // nullable: records every attempt
     std::vector<T> y0, f0, fa, fb, ysave, y1, ytmp, yerr, uscale;
     explicit Rkm(const Model& m) : M(m), tr(m) {
         const int ny = m.nq + m.nu;
@@ -1289,12 +1298,16 @@ template <class T> struct Rkm {
                 bool converged = attemptODE(t, t1, Fapp);
                 st.evals += 4;
                 double errNorm;
+                double errPre = std::numeric_limits<double>::quiet_NaN(), qErr = -1.0;
+                bool projected = false;
                 if (converged) {
                     const double e1 = double(errorNorm(y1.data(), yerr.data()));
+                    errPre = e1;
                     if (!(e1 > 16.0*acc)) {   // NaN falls through to projection like the reference's comparison
                         const double qe = double(quatErrRMS(y1.data()));
+                        qErr = qe;
                         if (qe > projectionLimit) converged = false;
-                        else if (qe > consTol) { normalizeQuats(y1.data(), yerr.data()); ++st.projections; }
+                        else if (qe > consTol) { normalizeQuats(y1.data(), yerr.data()); ++st.projections; projected = true; }
                     }
                 }
                 errNorm = converged ? double(errorNorm(y1.data(), yerr.data()))
@@ -1314,6 +1327,8 @@ template <class T> struct Rkm {
                 succeeded = (hnew >= hcur);
                 hcur = hnew;
                 if (!succeeded) ++st.rejected;
+                if (trace) trace->push_back({t, t1, errPre, qErr, errNorm, projected ? 1.0 : 0.0,
+                                             succeeded ? 1.0 : 0.0, hnew});
             } while (!succeeded);
             if (kr < nrep && reports[kr] <= t1) {
                 std::vector<T> f1(ny);
