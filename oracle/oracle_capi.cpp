@@ -5,6 +5,9 @@
 
 #include <atomic>
 #include <chrono>
+#include <pthread.h>
+#include <sched.h>
+#include <time.h>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -373,14 +376,35 @@ int oracle_count_flops_ops(void* p, const double* q, const double* u, unsigned l
 // time, std::thread pool) and return wall seconds.  Instance i uses
 // models[model_of_inst[i]]; Y is instance-major with stride ystride, and
 // Fapp (nullable) is instance-major with stride fstride.  out_body_steps =
-// sum over instances of accepted steps * nb.
+// sum over instances of accepted steps * nb.  pin != 0: worker k runs on the
+// k-th CPU of the process's affinity mask (1:1 while n_threads <= its size).
+// out_cpu_s (nullable): the workers' summed thread CPU time
+// (CLOCK_THREAD_CPUTIME_ID), the measure of TestMultibodyPerformance.cpp:
+// 168-198.
 double oracle_time_integrate_batch(void** models, const int* model_of_inst, int n_inst,
                                    double* Y, int ystride, const double* Fapp, int fstride,
                                    double t0, double T, double accuracy, double fixed_h,
-                                   int n_threads, long long* out_body_steps, long long* out_attempts) {
+                                   int n_threads, long long* out_body_steps, long long* out_attempts,
+                                   int pin, double* out_cpu_s) {
     std::atomic<int> next{0};
     std::atomic<long long> bodySteps{0}, attempts{0};
-    auto worker = [&]() {
+    std::vector<int> cpus;
+    if (pin) {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        if (sched_getaffinity(0, sizeof set, &set) == 0)
+            for (int c = 0; c < CPU_SETSIZE; ++c) if (CPU_ISSET(c, &set)) cpus.push_back(c);
+    }
+    std::vector<double> cpuSec(n_threads, 0.0);
+    auto worker = [&](int k) {
+        if (!cpus.empty()) {
+            cpu_set_t one;
+            CPU_ZERO(&one);
+            CPU_SET(cpus[k % cpus.size()], &one);
+            (void)pthread_setaffinity_np(pthread_self(), sizeof one, &one);
+        }
+        timespec c0, c1;
+        clock_gettime(CLOCK_THREAD_CPUTIME_ID, &c0);
         std::vector<std::unique_ptr<Rkm<double>>> cache;
         std::vector<void*> cacheKey;
         for (;;) {
@@ -398,14 +422,17 @@ double oracle_time_integrate_batch(void** models, const int* model_of_inst, int 
             bodySteps += (long long)st.accepted * M.nb;
             attempts += st.attempts;
         }
+        clock_gettime(CLOCK_THREAD_CPUTIME_ID, &c1);
+        cpuSec[k] = (c1.tv_sec - c0.tv_sec) + 1e-9 * (c1.tv_nsec - c0.tv_nsec);
     };
     const auto t_start = std::chrono::steady_clock::now();
     std::vector<std::thread> th;
-    for (int k=0;k<n_threads;++k) th.emplace_back(worker);
+    for (int k=0;k<n_threads;++k) th.emplace_back(worker, k);
     for (auto& x : th) x.join();
     const auto t_end = std::chrono::steady_clock::now();
     if (out_body_steps) *out_body_steps = bodySteps.load();
     if (out_attempts) *out_attempts = attempts.load();
+    if (out_cpu_s) { double c = 0; for (double x : cpuSec) c += x; *out_cpu_s = c; }
     return std::chrono::duration<double>(t_end - t_start).count();
 }
 

@@ -280,16 +280,48 @@ def abi_shift(P, s):
     return out
 
 
-def time_integrate_batch(models, model_of_inst, Y, T, accuracy, fixed_h, n_threads, Fapp=None):
-    """CPU baseline: one instance per thread, std::thread pool.  Y is
-    (n_inst, ny) and is integrated in place.  Returns (seconds, body_steps, attempts)."""
+def time_integrate_batch(models, model_of_inst, Y, T, accuracy, fixed_h, n_threads, Fapp=None, pin=True,
+                         cpu_time=False):
+    """CPU baseline: one instance per thread, std::thread pool, worker k
+    pinned to the k-th CPU of the affinity mask (pin).  Y is (n_inst, ny)
+    and is integrated in place.  Returns (seconds, body_steps, attempts), and
+    with cpu_time the workers' summed thread CPU seconds as a fourth item."""
     arr = (ctypes.c_void_p * len(models))(*[m.h.value for m in models])
     moi = np.ascontiguousarray(model_of_inst, np.int32)
-    bs = ctypes.c_longlong(0); at = ctypes.c_longlong(0)
+    bs = ctypes.c_longlong(0); at = ctypes.c_longlong(0); cs = ctypes.c_double(0.0)
     fstride = 0 if Fapp is None else Fapp.shape[1]
     sec = lib().oracle_time_integrate_batch(
         arr, moi.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(len(moi)),
         _p(Y), ctypes.c_int(Y.shape[1]), _p(Fapp), ctypes.c_int(fstride), ctypes.c_double(0.0),
         ctypes.c_double(T), ctypes.c_double(accuracy), ctypes.c_double(fixed_h),
-        ctypes.c_int(n_threads), ctypes.byref(bs), ctypes.byref(at))
+        ctypes.c_int(n_threads), ctypes.byref(bs), ctypes.byref(at), ctypes.c_int(1 if pin else 0),
+        ctypes.byref(cs))
+    if cpu_time:
+        return sec, bs.value, at.value, cs.value
     return sec, bs.value, at.value
+
+
+def host_cpu_info():
+    """What the CPU baseline ran on: online CPUs, the CPUs this process may
+    run on, the CPU model and its clock (current MHz of the first CPU and,
+    where the kernel exposes it, the maximum)."""
+    info = {"nproc_online": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        info["affinity_cpus"] = os.cpu_count()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and "model" not in info:
+                    info["model"] = line.split(":", 1)[1].strip()
+                elif line.startswith("cpu MHz") and "mhz" not in info:
+                    info["mhz"] = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq") as f:
+            info["max_mhz"] = int(f.read()) / 1000.0
+    except (OSError, ValueError):
+        pass
+    return info
