@@ -26,30 +26,45 @@ class TreeDesc(ctypes.Structure):
 
 
 _lib = None
+_variants = {}
+# The same restatement compiled with a*b+c contracted to fused multiply-adds
+# (-ffp-contract=fast -mfma): the rounding freedom of the GPU build, for
+# tests that bound how far two valid roundings of the algorithm may differ.
+FMA_LIB_PATH = os.path.join(_HERE, "liboracle_fma.so")
 
 
-def lib():
+def _open(path):
+    if not os.path.exists(path):
+        raise RuntimeError(f"oracle not built ({os.path.basename(path)}): run `make -C oracle` "
+                           "(or __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    L.oracle_model_new.restype = vp
+    L.oracle_model_new.argtypes = [ctypes.POINTER(TreeDesc)]
+    L.oracle_model_free.argtypes = [vp]
+    L.oracle_time_integrate_batch.restype = ctypes.c_double
+    for name in ["oracle_model_sizes", "oracle_model_layout", "oracle_kinematics",
+                 "oracle_system_forces", "oracle_forward", "oracle_realize_acceleration",
+                 "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
+                 "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
+                 "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
+                 "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
+                 "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
+                 "oracle_integrate_trace", "oracle_abi_shift"]:
+        getattr(L, name).restype = ctypes.c_int
+    return L
+
+
+def lib(variant=None):
+    """The oracle library; variant "fma" is the FMA-contracted build."""
     global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
-        L = ctypes.CDLL(LIB_PATH)
-        vp = ctypes.c_void_p
-        L.oracle_model_new.restype = vp
-        L.oracle_model_new.argtypes = [ctypes.POINTER(TreeDesc)]
-        L.oracle_model_free.argtypes = [vp]
-        L.oracle_time_integrate_batch.restype = ctypes.c_double
-        for name in ["oracle_model_sizes", "oracle_model_layout", "oracle_kinematics",
-                     "oracle_system_forces", "oracle_forward", "oracle_realize_acceleration",
-                     "oracle_minv", "oracle_residual", "oracle_abi", "oracle_reactions",
-                     "oracle_integrate", "oracle_count_flops_integrate", "oracle_count_flops_ops",
-                     "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
-                     "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
-                     "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
-                     "oracle_integrate_trace", "oracle_abi_shift"]:
-            getattr(L, name).restype = ctypes.c_int
-        _lib = L
-    return _lib
+    if variant is None:
+        if _lib is None:
+            _lib = _open(LIB_PATH)
+        return _lib
+    if variant not in _variants:
+        _variants[variant] = _open({"fma": FMA_LIB_PATH}[variant])
+    return _variants[variant]
 
 
 def _p(a):
@@ -68,7 +83,8 @@ class OracleModel:
     """One tree.  `spec` is any object with numpy fields parent, mobilizer,
     mass, com_B, inertia_B, X_PF, X_BM and gravity (see simbody_amd.models)."""
 
-    def __init__(self, spec):
+    def __init__(self, spec, variant=None):
+        self._L = lib(variant)
         self._keep = [np.ascontiguousarray(spec.parent, np.int32),
                       np.ascontiguousarray(spec.mobilizer, np.int32),
                       np.ascontiguousarray(spec.mass, np.float64),
@@ -93,38 +109,38 @@ class OracleModel:
             self._rev = np.ascontiguousarray(rev, np.int32)
             d.reversed = _pi(self._rev)
         self._desc = d
-        h = lib().oracle_model_new(ctypes.byref(d))
+        h = self._L.oracle_model_new(ctypes.byref(d))
         if not h:
             raise ValueError("invalid tree description")
         self.h = ctypes.c_void_p(h)
         s = (ctypes.c_int * 5)()
-        lib().oracle_model_sizes(self.h, s)
+        self._L.oracle_model_sizes(self.h, s)
         self.nb, self.nq, self.nu, self.nquat, self.nlevels = list(s)
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.oracle_model_free(self.h)
+        if getattr(self, "h", None) and getattr(self, "_L", None) is not None:
+            self._L.oracle_model_free(self.h)
             self.h = None
 
     def layout(self):
         qi = (ctypes.c_int * self.nb)(); ui = (ctypes.c_int * self.nb)(); lv = (ctypes.c_int * self.nb)()
-        lib().oracle_model_layout(self.h, qi, ui, lv)
+        self._L.oracle_model_layout(self.h, qi, ui, lv)
         return np.array(qi), np.array(ui), np.array(lv)
 
     def kinematics(self, q, u):
         X = np.zeros((self.nb + 1, 12)); V = np.zeros((self.nb + 1, 6))
         qd = np.zeros(self.nq); e = np.zeros(2); qe = np.zeros(max(self.nquat, 1))
-        lib().oracle_kinematics(self.h, _p(q), _p(u), _p(X), _p(V), _p(qd), _p(e), _p(qe))
+        self._L.oracle_kinematics(self.h, _p(q), _p(u), _p(X), _p(V), _p(qd), _p(e), _p(qe))
         return dict(X_GB=X, V_GB=V, qdot=qd, ke=e[0], pe=e[1], qerr=qe[:self.nquat])
 
     def system_forces(self, q, Fapp=None):
         F = np.zeros((self.nb + 1, 6))
-        lib().oracle_system_forces(self.h, _p(q), _p(Fapp), _p(F))
+        self._L.oracle_system_forces(self.h, _p(q), _p(Fapp), _p(F))
         return F
 
     def forward(self, q, u, fmob=None, F=None):
         ud = np.zeros(self.nu); qdd = np.zeros(self.nq); A = np.zeros((self.nb + 1, 6))
-        rc = lib().oracle_forward(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(ud), _p(qdd), _p(A))
+        rc = self._L.oracle_forward(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(ud), _p(qdd), _p(A))
         return ud, qdd, A, rc
 
     def forward_prescribed(self, q, u, fmob, F, presc, udot_known):
@@ -134,7 +150,7 @@ class OracleModel:
         ud = np.array(udot_known, np.float64, copy=True); tau = np.zeros(self.nu)
         A = np.zeros((self.nb + 1, 6))
         pr = np.ascontiguousarray(presc, np.int32)
-        rc = lib().oracle_forward_prescribed(self.h, _p(q), _p(u), _p(fmob), _p(F), _pi(pr), _p(ud), _p(tau), _p(A))
+        rc = self._L.oracle_forward_prescribed(self.h, _p(q), _p(u), _p(fmob), _p(F), _pi(pr), _p(ud), _p(tau), _p(A))
         return ud, tau, A, rc
 
     def integrate_known(self, y0, T, accuracy, presc, udot_known, Fapp=None, fixed_h=0.0):
@@ -144,70 +160,70 @@ class OracleModel:
         pr = np.ascontiguousarray(presc, np.int32)
         uk = np.ascontiguousarray(udot_known, np.float64)
         Fa = None if Fapp is None else np.ascontiguousarray(Fapp, np.float64)
-        lib().oracle_integrate_known(self.h, _p(y), _p(Fa), _pi(pr), _p(uk), ctypes.c_double(0.0),
+        self._L.oracle_integrate_known(self.h, _p(y), _p(Fa), _pi(pr), _p(uk), ctypes.c_double(0.0),
                                      ctypes.c_double(T), ctypes.c_double(accuracy), ctypes.c_double(fixed_h), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
         return y, dict(zip(keys, list(st)))
 
     def realize_acceleration(self, q, u, Fapp=None):
         ud = np.zeros(self.nu); qd = np.zeros(self.nq); qdd = np.zeros(self.nq)
-        rc = lib().oracle_realize_acceleration(self.h, _p(q), _p(u), _p(Fapp), _p(ud), _p(qd), _p(qdd))
+        rc = self._L.oracle_realize_acceleration(self.h, _p(q), _p(u), _p(Fapp), _p(ud), _p(qd), _p(qdd))
         return ud, qd, qdd, rc
 
     def minv(self, q, f):
         out = np.zeros(self.nu)
-        lib().oracle_minv(self.h, _p(q), _p(np.ascontiguousarray(f, np.float64)), _p(out))
+        self._L.oracle_minv(self.h, _p(q), _p(np.ascontiguousarray(f, np.float64)), _p(out))
         return out
 
     def residual(self, q, u, fmob=None, F=None, udot=None):
         out = np.zeros(self.nu)
-        lib().oracle_residual(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(udot), _p(out))
+        self._L.oracle_residual(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(udot), _p(out))
         return out
 
     def abi(self, q, dofs):
         n2 = int(sum(d * d for d in dofs))
         P = np.zeros((self.nb, 21)); PP = np.zeros((self.nb, 21)); D = np.zeros(n2); DI = np.zeros(n2)
-        rc = lib().oracle_abi(self.h, _p(q), _p(P), _p(PP), _p(D), _p(DI))
+        rc = self._L.oracle_abi(self.h, _p(q), _p(P), _p(PP), _p(D), _p(DI))
         return P, PP, D, DI, rc
 
     def reactions(self, q, u, fmob=None, F=None):
         out = np.zeros((self.nb + 1, 6))
-        lib().oracle_reactions(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(out))
+        self._L.oracle_reactions(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(out))
         return out
 
     def multiply_by_m(self, q, v):
         out = np.zeros(self.nu)
-        lib().oracle_multiply_by_m(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(out))
+        self._L.oracle_multiply_by_m(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(out))
         return out
 
     def jacobian(self, q, v):
         Jv = np.zeros((self.nb + 1, 6))
-        lib().oracle_jacobian(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(Jv))
+        self._L.oracle_jacobian(self.h, _p(q), _p(np.ascontiguousarray(v, np.float64)), _p(Jv))
         return Jv
 
     def jacobian_transpose(self, q, F):
         out = np.zeros(self.nu)
-        lib().oracle_jacobian_transpose(self.h, _p(q), _p(np.ascontiguousarray(F, np.float64).reshape(-1)),
+        self._L.oracle_jacobian_transpose(self.h, _p(q), _p(np.ascontiguousarray(F, np.float64).reshape(-1)),
                                         _p(out))
         return out
 
     def body_acceleration(self, q, u, udot=None):
         A = np.zeros((self.nb + 1, 6))
         ud = None if udot is None else np.ascontiguousarray(udot, np.float64)
-        lib().oracle_body_acceleration(self.h, _p(q), _p(u), _p(ud), _p(A))
+        self._L.oracle_body_acceleration(self.h, _p(q), _p(u), _p(ud), _p(A))
         return A
 
     def composite_inertias(self, q):
         """(nb+1, 10): mass, mass center (3), unit inertia about the body
         origin (xx yy zz xy xz yz), in G; row 0 is Ground (infinite mass)."""
         R = np.zeros((self.nb + 1, 10))
-        lib().oracle_composite_inertias(self.h, _p(q), _p(R))
+        self._L.oracle_composite_inertias(self.h, _p(q), _p(R))
         return R
 
     def integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None, t0=0.0, max_attempts=0):
         y = np.array(y, dtype=np.float64, copy=True)
         st = (ctypes.c_long * 6)()
-        lib().oracle_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+        self._L.oracle_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
                                ctypes.c_double(accuracy), ctypes.c_double(fixed_h),
                                ctypes.c_long(max_attempts), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
@@ -221,7 +237,7 @@ class OracleModel:
         rec = np.zeros((max_rec, 8))
         n = ctypes.c_int(0)
         st = (ctypes.c_long * 6)()
-        lib().oracle_integrate_trace(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+        self._L.oracle_integrate_trace(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
                                      ctypes.c_double(accuracy), ctypes.c_double(fixed_h), ctypes.c_double(init_h),
                                      _p(rec), ctypes.c_int(max_rec), ctypes.byref(n), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
@@ -236,7 +252,7 @@ class OracleModel:
         stops = np.ascontiguousarray(stops, dtype=np.float64)
         out = np.zeros((len(stops), y.size))
         st = (ctypes.c_long * 6)()
-        lib().oracle_integrate_stops(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), _p(stops),
+        self._L.oracle_integrate_stops(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), _p(stops),
                                      ctypes.c_int(len(stops)), ctypes.c_double(accuracy),
                                      ctypes.c_double(fixed_h), _p(out), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
@@ -249,7 +265,7 @@ class OracleModel:
         reports = np.ascontiguousarray(reports, dtype=np.float64)
         out = np.zeros((len(reports), y.size))
         st = (ctypes.c_long * 6)()
-        lib().oracle_integrate_reports(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
+        self._L.oracle_integrate_reports(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
                                        _p(reports), ctypes.c_int(len(reports)), ctypes.c_double(accuracy),
                                        ctypes.c_double(fixed_h), _p(out), st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
@@ -258,7 +274,7 @@ class OracleModel:
     def count_flops_integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None):
         y = np.array(y, dtype=np.float64, copy=True)
         t = (ctypes.c_ulonglong * 5)(); st = (ctypes.c_long * 6)()
-        lib().oracle_count_flops_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(0.0),
+        self._L.oracle_count_flops_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(0.0),
                                            ctypes.c_double(T), ctypes.c_double(accuracy),
                                            ctypes.c_double(fixed_h), t, st)
         keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
@@ -266,7 +282,7 @@ class OracleModel:
 
     def count_flops_ops(self, q, u):
         f = (ctypes.c_ulonglong * 5)(); g = (ctypes.c_ulonglong * 5)()
-        lib().oracle_count_flops_ops(self.h, _p(q), _p(u), f, g)
+        self._L.oracle_count_flops_ops(self.h, _p(q), _p(u), f, g)
         k = ["addsub", "mul", "div", "sqrt", "trig"]
         return dict(zip(k, list(f))), dict(zip(k, list(g)))
 
