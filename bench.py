@@ -194,45 +194,92 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     return out
 
 
-def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 5000, 20000, 50000), window=1.0, n_cap=25000,
-                seed=5, budget_gib=24, steps_per_advance=16):
-    """Serving loop: Poisson arrivals at each rate; every request is its own
-    32-body chain drawn from the workload's distributions (Pin/Universal/Ball
-    cycling, own masses, inertias and offsets), a seeded initial state,
-    horizon ~ U(0.1, 2) s, RK-Merson accuracy 1e-6.  Slots come from an HBM
-    budget; latency = arrival -> result on the host."""
+def row_serving(torch, M, RungeKuttaMerson, rates=(500, 2000, 8000), mixed_rates=(2000,), window=10.0,
+                warmup=2.0, seed=5, budget_gib=24, steps_per_advance=16, n_models=8192):
+    """Serving loop, steady state: Poisson arrivals for `window` seconds at
+    each rate, the first `warmup` seconds discarded; every request is a
+    32-body chain drawn from the chain workload's distributions (own masses,
+    inertias and offsets; n_models distinct draws reused cyclically), a
+    seeded initial state, horizon ~ U(0.1, 2) s, RK-Merson accuracy 1e-6.
+    Slots come from an HBM budget; latency = arrival -> result on the host.
+    Requests still running at window + 3 s count as not completed."""
     from simbody_amd.serving import Requests, Server, serve
     template = M.spatial_chain(32, seed=2)
-    n_max = min(n_cap, int(max(rates) * window))
-    log(f"[bench] serving: generating {n_max} requests")
-    params = M.chain_params(seed, np.arange(n_max), nb=32)
-    Y = M.random_states(template, seed, np.arange(n_max))
-    horizon = M.Rng(seed, 91).uniform(n_max, 0.1, 2.0)
-    out = {"workload": "Poisson arrivals; each request one 32-body chain with its own constants, horizon "
-                       "U(0.1,2) s, RK-Merson accuracy 1e-6; FP64 and mixed precision",
+    params = M.chain_params(seed, np.arange(n_models), nb=32)
+    out = {"workload": f"Poisson arrivals for {window:g} s (first {warmup:g} s discarded); each request one "
+                       "32-body chain with its own constants, horizon U(0.1,2) s, RK-Merson accuracy 1e-6; "
+                       "FP64 and mixed precision",
            "hbm_budget_bytes": budget_gib << 30, "steps_per_advance": steps_per_advance, "sweep": []}
-    for prec, rate in [(p, r) for p in (0, 1) for r in rates]:
+    for prec, rate in [(0, r) for r in rates] + [(1, r) for r in mixed_rates]:
         sv = Server(template, RungeKuttaMerson(accuracy=1e-6, precision=prec), budget_gib << 30, max_slots=65536)
         out["slots_fp64" if prec == 0 else "slots_mixed"] = sv.slots
-        n = min(n_cap, int(rate * window))
-        req = Requests.from_chain_params(template, tuple(p[:n] for p in params), Y[:n, :template.nq],
-                                         Y[:n, template.nq:], horizon[:n])
+        n = int(rate * window)
+        Y = M.random_states(template, seed + rate, np.arange(n))
+        horizon = M.Rng(seed + rate, 91).uniform(n, 0.1, 2.0)
+        req = Requests.from_chain_params(template, params, Y[:, :template.nq], Y[:, template.nq:], horizon,
+                                         model_of=np.arange(n) % n_models)
         arrivals = np.cumsum(np.random.default_rng(seed + rate).exponential(1.0 / rate, size=n))
-        log(f"[bench] serving: precision {prec}, {rate} req/s, {n} requests")
-        r = serve(sv, req, arrivals, steps_per_advance=steps_per_advance, time_limit=180)
+        log(f"[bench] serving chains: precision {prec}, {rate} req/s, {n} requests")
+        r = serve(sv, req, arrivals, steps_per_advance=steps_per_advance, time_limit=window + 3.0, warmup=warmup)
+        st = r.get("steady", {})
         ok = np.isfinite(r["latency"])
-        lat = r["latency"][ok]
-        body_steps = int((r["stats"][ok, 1].astype(np.int64) * template.nb).sum())
+        win = (r["done_at"] >= warmup) & (r["done_at"] <= window)
+        body_steps = int((r["stats"][win, 1].astype(np.int64) * template.nb).sum())
         out["sweep"].append({"precision": "fp64" if prec == 0 else "mixed", "rate_req_s": rate, "requests": n,
-                             "completed": int(r["completed"]),
-                             "failed": int((r["status"][ok] < 0).sum()), "wall_s": r["wall"],
-                             "throughput_body_steps_s": body_steps / r["wall"],
-                             "completed_req_s": int(r["completed"]) / r["wall"],
-                             "latency_p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
-                             "latency_p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
-                             "accepted_steps_mean": float(r["stats"][ok, 1].mean()) if len(lat) else None,
-                             "max_occupied_slots": int(r["max_occupied"]), "launches": int(r["launches"])})
+                             "offered_req_s": st.get("offered_req_s"),
+                             "completed_req_s": st.get("completed_in_window_req_s"),
+                             "completed": int(r["completed"]), "failed": int((r["status"][ok] < 0).sum()),
+                             "throughput_body_steps_s": body_steps / (window - warmup),
+                             "latency_p50_s": st.get("latency_p50_s"), "latency_p99_s": st.get("latency_p99_s"),
+                             "accepted_steps_mean": float(r["stats"][ok, 1].mean()) if ok.any() else None,
+                             "max_occupied_slots": int(r["max_occupied"]), "launches": int(r["launches"]),
+                             "wall_s": r["wall"]})
         log(f"[bench] serving: {out['sweep'][-1]}")
+        del sv
+    return out
+
+
+def row_tree_serving(torch, M, RungeKuttaMerson, rates=(250, 1000, 4000), window=10.0, warmup=2.0, seed=7,
+                     budget_gib=24, steps_per_advance=16, n_models=2048):
+    """Serving requests of different trees (TreeServer): Poisson arrivals for
+    `window` seconds, the first `warmup` discarded; each request one uneven
+    branched tree (nb ~ U{8..64}, depth <= 20, Pin/Slider/Universal/Ball and
+    Free roots; n_models distinct trees reused cyclically), a seeded initial
+    state, horizon ~ U(0.1, 1) s, RK-Merson accuracy 1e-6; each request's
+    memory is its own tree's, from pools sized by the HBM budget."""
+    from simbody_amd.serving import TreeRequests, TreeServer, serve
+    models = [M.random_tree(seed, i, nmin=8, nmax=64) for i in range(n_models)]
+    out = {"workload": f"Poisson arrivals for {window:g} s (first {warmup:g} s discarded); each request one "
+                       "uneven tree, nb~U{8..64}, horizon U(0.1,1) s, RK-Merson accuracy 1e-6, FP64",
+           "hbm_budget_bytes": budget_gib << 30, "steps_per_advance": steps_per_advance, "sweep": []}
+    for rate in rates:
+        sv = TreeServer(RungeKuttaMerson(accuracy=1e-6), budget_gib << 30, max_slots=65536)
+        n = int(rate * window)
+        specs = [models[i % n_models] for i in range(n)]
+        qs, us = [None] * n, [None] * n
+        for m in range(min(n, n_models)):          # states drawn per model, vectorized over its requests
+            idx = np.arange(m, n, n_models)
+            Ym = M.random_states(models[m], seed + rate, idx)
+            for j, i in enumerate(idx):
+                qs[i] = Ym[j, :models[m].nq]; us[i] = Ym[j, models[m].nq:]
+        horizon = M.Rng(seed + rate, 93).uniform(n, 0.1, 1.0)
+        req = TreeRequests(specs, qs, us, horizon)
+        arrivals = np.cumsum(np.random.default_rng(seed + rate).exponential(1.0 / rate, size=n))
+        log(f"[bench] serving trees: {rate} req/s, {n} requests")
+        r = serve(sv, req, arrivals, steps_per_advance=steps_per_advance, time_limit=window + 3.0, warmup=warmup)
+        st = r.get("steady", {})
+        ok = np.isfinite(r["latency"])
+        win = (r["done_at"] >= warmup) & (r["done_at"] <= window)
+        nbs = np.array([t.nb for t in specs])
+        body_steps = int((r["stats"][win, 1].astype(np.int64) * nbs[win]).sum())
+        out["sweep"].append({"rate_req_s": rate, "requests": n, "offered_req_s": st.get("offered_req_s"),
+                             "completed_req_s": st.get("completed_in_window_req_s"),
+                             "completed": int(r["completed"]), "failed": int((r["status"][ok] < 0).sum()),
+                             "throughput_body_steps_s": body_steps / (window - warmup),
+                             "latency_p50_s": st.get("latency_p50_s"), "latency_p99_s": st.get("latency_p99_s"),
+                             "mean_bodies": float(nbs.mean()), "max_occupied_slots": int(r["max_occupied"]),
+                             "launches": int(r["launches"]), "wall_s": r["wall"]})
+        log(f"[bench] serving trees: {out['sweep'][-1]}")
         del sv
     return out
 
@@ -361,7 +408,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the secondary workloads (pendulum latency, uneven trees)")
-    ap.add_argument("--only", choices=["pendulum", "trees", "serving", "mixed", "operators"], default=None,
+    ap.add_argument("--only", choices=["pendulum", "trees", "serving", "tree-serving", "mixed", "operators"],
+                    default=None,
                     help="run only one secondary workload and print its JSON")
     ap.add_argument("--tree-lanes", type=int, default=0,
                     help="lanes per instance of the tree evaluator (0 automatic, 1, 2, 4, 8, 16)")
@@ -411,6 +459,9 @@ def main():
         return
     if args.only == "serving":
         print(json.dumps(row_serving(torch, M, RungeKuttaMerson)), flush=True)
+        return
+    if args.only == "tree-serving":
+        print(json.dumps(row_tree_serving(torch, M, RungeKuttaMerson)), flush=True)
         return
 
     h = 1e-3
@@ -489,6 +540,7 @@ def main():
         rows["operators"] = row_operators(torch, M, MatterBatch, cpu=not args.no_cpu)
         rows["mixed_precision"] = row_mixed(torch, M, MatterBatch, RungeKuttaMerson)
         rows["serving"] = row_serving(torch, M, RungeKuttaMerson)
+        rows["tree_serving"] = row_tree_serving(torch, M, RungeKuttaMerson)
         out["rows"] = rows
     print(json.dumps(out), flush=True)
 

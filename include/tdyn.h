@@ -124,6 +124,20 @@ int tdyn_batch_set_tree_lane_range(tdyn_batch* b, int lanes_min, int lanes_max);
  * instance). */
 int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes);
 
+/* Per-instance status of the last single-shot operator that formed the
+ * articulated-body inertias (realize acceleration, calcAcceleration*,
+ * calcAccelerationWithKnownUDot, multiplyByMInv, calcArticulatedBodyInertias,
+ * calcMobilizerReactionForces): 0, or TDYN_E_SINGULAR where some body's
+ * D = H^T P H was not invertible (a determinant, or a pivot of the 6x6
+ * elimination, not positive).  The reference throws for a singular 6x6
+ * (lapackInverse, SmallMatrixMixed.h:878-892); here the call itself returns
+ * TDYN_OK (it is asynchronous), the instance's outputs hold inf/NaN, and this
+ * query reports it.  status (device, nullable): n_instances int32 copied on
+ * stream s.  n_singular (host, nullable): synchronizes s, counts the singular
+ * instances and then returns TDYN_E_SINGULAR if there are any.  No reference
+ * counterpart as a call (the reference's check is the throw). */
+int tdyn_batch_op_status(tdyn_batch* b, int32_t* status, int* n_singular, hipStream_t s);
+
 /* info[0..7] = n_instances, total q, total u, total bodies (incl. Ground),
  * max nq, max nu, max nb, n_trees. */
 int tdyn_batch_info(const tdyn_batch* b, int64_t* info);
@@ -248,8 +262,11 @@ typedef struct tdyn_integrator_cfg {
     int32_t max_attempts;        /* per-instance attempt budget (<=0: 1e8) */
     int32_t precision;           /* 0 = FP64; 1 = mixed: FP32 kinematics, bias forces and
                                     articulated inertias, FP64 D = H^T P H and its inverse,
-                                    quaternion normalization and Kahan-compensated state
-                                    (chain evaluator only; TDYN_E_ARG otherwise) */
+                                    quaternion normalization and Kahan-compensated state;
+                                    2 = FP64 arithmetic with the G = P H D^-1 hand-over rows
+                                    stored in FP32; 3 = as 2 with the joint trig cache in
+                                    FP32 too.  1-3: chain evaluator of one model per batch
+                                    only (TDYN_E_ARG otherwise; serving: 0 or 1) */
 } tdyn_integrator_cfg;
 
 /* Autotuner of the tree evaluators' occupancy (no Simbody counterpart: it
@@ -356,6 +373,51 @@ int tdyn_server_advance(tdyn_server* sv, int max_steps, hipStream_t s);
  * Synchronizes s. */
 int tdyn_server_collect(tdyn_server* sv, int max_out, int32_t* slots, double* q, double* u, double* t,
                         int32_t* status, int32_t* stats, int* n_out, hipStream_t s);
+
+/* ---- Serving requests of different trees ---------------------------------
+ * No reference counterpart.  A tree server's slots hold requests of any tree
+ * the tree sweeps run: any size, topology and mobilizer mix (the trees of
+ * tdyn_batch_create), each with its own constants, gravity, initial state,
+ * final time and applied body forces.  A request's memory comes from two
+ * pools sized from the HBM budget at creation -- body tables (nb + 1
+ * entries) and records plus integrator state (its own nq + nu) -- so its
+ * byte cost is its own tree's (tdyn_tree_server_request_bytes).  Requests
+ * are admitted in order while a slot is free and both pools have room for
+ * the next one; collection returns the memory.  All slots advance together,
+ * each with its own step control, and a slot's results do not depend on the
+ * other slots.  FP64; not thread-safe (one host thread). */
+typedef struct tdyn_tree_server tdyn_tree_server;
+
+/* max_slots <= 0: 65,536 slots.  The budget pays the slots' own state (about
+ * 130 bytes each), then a quarter of the rest goes to the body pool and the
+ * remainder to the record and state pool. */
+int tdyn_tree_server_create(const tdyn_integrator_cfg* cfg, size_t hbm_budget_bytes, int max_slots,
+                            tdyn_tree_server** out);
+int tdyn_tree_server_destroy(tdyn_tree_server* sv);
+/* info[0..6] = slots, free slots, occupied slots, budget bytes, bytes in use
+ * (slots' own state plus the pools' occupied parts), body pool capacity
+ * (bodies), record and state pool capacity (doubles). */
+int tdyn_tree_server_info(const tdyn_tree_server* sv, int64_t* info);
+/* What one request of this tree takes from the budget (bytes). */
+int tdyn_tree_server_request_bytes(const tdyn_tree_desc* tree, int64_t* bytes);
+/* Admit requests in order while a slot is free and the pools have room; host
+ * arrays: trees[n], q and u packed request after request (each its own nq,
+ * nu), t_end[n] (> 0; every request starts at t = 0), fapp packed
+ * (nb+1)*6 per request or NULL.  All n requests are validated before any is
+ * admitted.  Writes the slots of the admitted (the first *n_admitted)
+ * requests.  Synchronizes s. */
+int tdyn_tree_server_admit(tdyn_tree_server* sv, int n, const tdyn_tree_desc* trees, const double* q,
+                           const double* u, const double* t_end, const double* fapp, int32_t* slots,
+                           int* n_admitted, hipStream_t s);
+/* Advance every running slot by at most max_steps accepted steps (one launch, async). */
+int tdyn_tree_server_advance(tdyn_tree_server* sv, int max_steps, hipStream_t s);
+/* Retire up to max_out finished slots whose states fit y_cap doubles: host
+ * outputs slot ids, y (each request's q then u, packed), y_off (where each
+ * starts in y), t, status (1 done, <0 failed) and stats [n][4] (t, status,
+ * stats nullable).  The slots and their memory become free.  Synchronizes s. */
+int tdyn_tree_server_collect(tdyn_tree_server* sv, int max_out, int64_t y_cap, int32_t* slots, double* y,
+                             int64_t* y_off, double* t, int32_t* status, int32_t* stats, int* n_out,
+                             hipStream_t s);
 
 const char* tdyn_last_error(void);
 

@@ -55,6 +55,13 @@ public:
 
     tdyn_batch* handle() const { return h_; }
     void setStream(hipStream_t s) { s_ = s; }
+    // After each operator that forms articulated-body inertias, synchronize
+    // the stream and throw tdyn::Error(TDYN_E_SINGULAR) when some instance's
+    // D was singular, as the reference throws (SmallMatrixMixed.h:878-892).
+    // On by default; off keeps the calls asynchronous (query
+    // checkSingular() or tdyn_batch_op_status yourself).
+    void setThrowOnSingular(bool on) { throwOnSingular_ = on; }
+    void checkSingular() const { int n = 0; check(tdyn_batch_op_status(h_, nullptr, &n, s_)); }
 
     // sizes: instances, total q, total u, total bodies incl. Ground, max nq, max nu, max nb, trees
     std::vector<int64_t> info() const { std::vector<int64_t> v(8); check(tdyn_batch_info(h_, v.data())); return v; }
@@ -68,15 +75,18 @@ public:
     void realizeAcceleration(const double* q, const double* u, const double* appliedBodyForces, double* udot,
                              double* qdot = nullptr) const {
         check(tdyn_realize_acceleration(h_, q, u, appliedBodyForces, udot, qdot, s_));
+        if (throwOnSingular_) checkSingular();
     }
     void calcAccelerationIgnoringConstraints(const double* q, const double* u, const double* appliedMobilityForces,
                                              const double* appliedBodyForces, double* udot, double* A_GB,
                                              double* qdotdot = nullptr) const {
         check(tdyn_calc_acceleration_ignoring_constraints(h_, q, u, appliedMobilityForces, appliedBodyForces,
                                                           udot, A_GB, qdotdot, s_));
+        if (throwOnSingular_) checkSingular();
     }
     void multiplyByMInv(const double* q, const double* v, double* MinvV) const {
         check(tdyn_multiply_by_m_inv(h_, q, v, MinvV, s_));
+        if (throwOnSingular_) checkSingular();
     }
     void calcResidualForceIgnoringConstraints(const double* q, const double* u, const double* appliedMobilityForces,
                                               const double* appliedBodyForces, const double* knownUdot,
@@ -87,10 +97,12 @@ public:
     // realizeArticulatedBodyInertias + getArticulatedBodyInertia for every body
     void realizeArticulatedBodyInertias(const double* q, double* P, double* PPlus = nullptr) const {
         check(tdyn_calc_articulated_body_inertias(h_, q, P, PPlus, s_));
+        if (throwOnSingular_) checkSingular();
     }
     void calcMobilizerReactionForces(const double* q, const double* u, const double* appliedMobilityForces,
                                      const double* appliedBodyForces, double* FM_G) const {
         check(tdyn_calc_mobilizer_reaction_forces(h_, q, u, appliedMobilityForces, appliedBodyForces, FM_G, s_));
+        if (throwOnSingular_) checkSingular();
     }
     // Forward dynamics with known udot on the bodies flagged in `known`
     // (prescribed motion at the acceleration level, locks; no single
@@ -100,6 +112,7 @@ public:
                                        const int32_t* known, const double* udotKnown, double* udot, double* tau,
                                        double* A_GB = nullptr) const {
         check(tdyn_calc_acceleration_with_known_udot(h_, q, u, fMob, F, known, udotKnown, udot, tau, A_GB, s_));
+        if (throwOnSingular_) checkSingular();
     }
     void multiplyByM(const double* q, const double* v, double* Mv) const {
         check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
@@ -145,6 +158,7 @@ public:
 private:
     tdyn_batch* h_ = nullptr;
     hipStream_t s_ = nullptr;
+    bool throwOnSingular_ = true;
 };
 
 } // namespace tdyn

@@ -88,8 +88,9 @@ bool any_reversed(const tdyn_tree_desc& d) {
 }
 
 int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
-    if (cfg->precision != 0 && cfg->precision != 1)
-        return fail(TDYN_E_ARG, "integrator precision must be 0 (FP64) or 1 (mixed)");
+    if (cfg->precision < 0 || cfg->precision > 3)
+        return fail(TDYN_E_ARG, "integrator precision must be 0 (FP64), 1 (mixed), 2 or 3 (FP64 with "
+                                "single-precision hand-over rows)");
     precision = cfg->precision;
     const double acc = cfg->accuracy > 0 ? cfg->accuracy : 1e-3;
     if (!(acc < 1.0)) return fail(TDYN_E_ARG, "accuracy must be in (0,1)");
@@ -155,7 +156,7 @@ int ensure_rollout_buffers(tdyn_batch* b, int precision) {
     const int ny = b->max_nq + b->max_nu;
     const size_t L = b->n_inst;
     hipError_t e;
-    if (precision == 1) {
+    if (precision != 0) {
         const size_t n32 = (size_t)(6 * b->max_nu + 4 * b->max_nb) * L;
         if (b->scratch32.n < n32 && (e = b->scratch32.alloc(n32)) != hipSuccess)
             return hipfail(e, "mixed-precision scratch allocation");
@@ -366,7 +367,9 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
         (e = B->prm_bfs.upload(prm_bfs)) != hipSuccess ||
         (e = B->lvl.upload(lvl)) != hipSuccess ||
         (e = B->scratch.alloc((size_t)std::max({B->max_rec + 24, B->stream_ok ? B->stream_stride : 0,
-                                                B->stream_ok ? B->group_stride : 0}) * n_instances)) != hipSuccess) {
+                                                B->stream_ok ? B->group_stride : 0}) * n_instances)) != hipSuccess ||
+        (e = B->op_status.alloc((size_t)n_instances)) != hipSuccess ||
+        (e = hipMemset(B->op_status.p, 0, (size_t)n_instances * sizeof(int))) != hipSuccess) {
         delete B;
         return hipfail(e, "tdyn_batch_create: device allocation/upload");
     }
@@ -419,6 +422,22 @@ int tdyn_batch_get_evaluator(const tdyn_batch* b, int* active, int* lanes) {
     }
     *active = act;
     if (lanes) *lanes = ln;
+    return TDYN_OK;
+}
+
+int tdyn_batch_op_status(tdyn_batch* b, int32_t* status, int* n_singular, hipStream_t s) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    const size_t L = b->n_inst;
+    if (status) HIPCHK(hipMemcpyAsync(status, b->op_status.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (!n_singular) return TDYN_OK;
+    std::vector<int> h(L);
+    HIPCHK(hipMemcpyAsync(h.data(), b->op_status.p, L * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int k = 0, first = -1;
+    for (size_t i = 0; i < L; ++i) if (h[i] != 0) { if (first < 0) first = (int)i; ++k; }
+    *n_singular = k;
+    if (k) return fail(TDYN_E_SINGULAR, "articulated inertia D is singular for " + std::to_string(k) +
+                                        " instance(s), the first being instance " + std::to_string(first));
     return TDYN_OK;
 }
 
@@ -572,9 +591,9 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     RkmCfg c;
     int prec = 0;
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }
-    if (prec == 1 && !(b->is_chain && b->evaluator != 1))
-        return fail(TDYN_E_ARG, "tdyn_rollout_begin: mixed precision runs on the chain evaluator (one serial-chain model per batch)");
-    if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || prec == 1))
+    if (prec != 0 && !(b->is_chain && b->evaluator != 1))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: precision 1-3 runs on the chain evaluator (one serial-chain model per batch)");
+    if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || prec != 0))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps or the level-parallel "
                                 "tree evaluator (not one lane per instance), in FP64");
     const size_t L = b->n_inst;

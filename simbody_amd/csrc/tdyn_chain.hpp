@@ -440,10 +440,12 @@ using Rows = RowsT<double>;
 // body's G slot of the double scratch, trig cache at its own double rows.
 // Mixed path: G and the trig cache in float rows (G at 6*uIndex, trig cache
 // after all G rows), DI*eps at the double G slot.
-template <class K> struct KStore {
-    RowsT<K> k; Rows d; int g, tc, de;
-    __device__ __forceinline__ TD_GLOBAL K& G(int i) const { return k(g + i); }
-    __device__ __forceinline__ TD_GLOBAL K& T(int i) const { return k(tc + i); }
+// GS, TS: the storage types of G and of the trig cache (double, or float
+// where those rows are single precision).
+template <class GS, class TS = GS> struct KStore {
+    RowsT<GS> kg; RowsT<TS> kt; Rows d; int g, tc, de;
+    __device__ __forceinline__ TD_GLOBAL GS& G(int i) const { return kg(g + i); }
+    __device__ __forceinline__ TD_GLOBAL TS& T(int i) const { return kt(tc + i); }
     __device__ __forceinline__ TD_GLOBAL double& DE(int i) const { return d(de + i); }
 };
 
@@ -494,8 +496,8 @@ __device__ __forceinline__ void prefetch_t(Pre<K>& x, const YR& y, int nq, int q
 // AF: the body may be a Free root.  Only body 1 can be Free (the host rejects
 // Free elsewhere), and it is peeled out of every loop, so the loops prefetch
 // with AF = false and never hold a 7-q/6-u body's registers.
-template <bool G, bool AF, class K>
-__device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 ba, const KStore<K>& st, bool tc) {
+template <bool G, bool AF, class K, class ST>
+__device__ __forceinline__ void prefetch(Pre<K>& x, const Rows& y, int nq, int4 ba, const ST& st, bool tc) {
     switch (ba.y) {
     case PIN:       prefetch_t<PIN, G>(x, y, nq, ba.z, ba.w, st, tc); break;
     case SLIDER:    prefetch_t<SLIDER, G>(x, y, nq, ba.z, ba.w, st, tc); break;
@@ -805,7 +807,8 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
             G = {V3(x.g[6*r], x.g[6*r+1], x.g[6*r+2]), V3(x.g[6*r+3], x.g[6*r+4], x.g[6*r+5])};
             ud[r] = x.de[r] - dotd(G, Ap);
         } else {
-            G = {V3(st.G(6*r), st.G(6*r + 1), st.G(6*r + 2)), V3(st.G(6*r + 3), st.G(6*r + 4), st.G(6*r + 5))};
+            G = {V3(K(st.G(6*r)), K(st.G(6*r + 1)), K(st.G(6*r + 2))),
+                 V3(K(st.G(6*r + 3)), K(st.G(6*r + 4)), K(st.G(6*r + 5)))};
             ud[r] = st.DE(r) - dotd(G, Ap);
         }
     }
@@ -824,17 +827,21 @@ __device__ __forceinline__ void pass3_body(const PA& P, const Pre<K>& x, int ui,
 }
 
 // ---------------------------------------------------------------------------
-// The evaluator.  sc: this lane's double scratch rows (G records at
-// body_b.w, FP64 trig cache at tree_b.z + 4(b-1)); sk: the rows of the K
-// hand-over values (the double scratch for K = double, the float scratch of
-// the mixed path otherwise).
+// The evaluator.  K: the arithmetic of kinematics, bias forces and
+// articulated inertias (double, or float on the mixed path).  GS, TS: the
+// storage of the hand-over rows G and trig cache.  sc: this lane's double
+// scratch rows (G records at body_b.w, FP64 trig cache at tree_b.z +
+// 4(b-1)); sf: the float scratch (G at 6 uIndex, then the trig cache at
+// 6 nu + 4(b-1)), used for the rows stored in single precision.
 // ---------------------------------------------------------------------------
-template <class PA, class K>
+template <class PA, class K, class GS = K, class TS = GS>
 struct ChainEval {
     static constexpr bool MIXED = !std::is_same<K, double>::value;
+    static constexpr bool G32 = std::is_same<GS, float>::value;
+    static constexpr bool T32 = std::is_same<TS, float>::value;
     PA prm; const int4* body_a; const int4* body_b;
     int nb, nq, nu, tcbase;
-    Rows sc; RowsT<K> sk; v3<K> grav; const double* fapp;
+    Rows sc; RowsT<float> sf; v3<K> grav; const double* fapp;
 
     __device__ ChainEval(const RkmArgs& a, int lane, int tr) {
         const int4 ta = ldi4(a.T.tree_a, tr);
@@ -842,8 +849,7 @@ struct ChainEval {
         body_a = a.T.body_a + ta.w; body_b = a.T.body_b + ta.w;
         tcbase = ldi4(a.T.tree_b, tr).z;
         sc = {a.scratch, a.L, (unsigned)lane};
-        if constexpr (MIXED) sk = {a.scratch32, a.L, (unsigned)lane};
-        else sk = sc;
+        sf = {a.scratch32, a.L, (unsigned)lane};
         if constexpr (PrmTraits<PA>::lane) {
             // per-lane model: nb*PRM_STRIDE rows, then 3 gravity rows
             const PrmL pl{a.prm_lane, a.L, (unsigned)lane};
@@ -858,9 +864,12 @@ struct ChainEval {
         fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr;
     }
     // Hand-over rows of body b (1-based, table entry ba, G slot gs).
-    __device__ __forceinline__ KStore<K> store(int b, int4 ba, int gs) const {
-        if constexpr (MIXED) return {sk, sc, 6*ba.w, 6*nu + 4*(b-1), gs + 6*mob_dof(ba.y)};
-        else return {sk, sc, gs, tcbase + 4*(b-1), gs + 6*mob_dof(ba.y)};
+    __device__ __forceinline__ KStore<GS, TS> store(int b, int4 ba, int gs) const {
+        KStore<GS, TS> st;
+        if constexpr (G32) { st.kg = sf; st.g = 6*ba.w; } else { st.kg = sc; st.g = gs; }
+        if constexpr (T32) { st.kt = sf; st.tc = 6*nu + 4*(b-1); } else { st.kt = sc; st.tc = tcbase + 4*(b-1); }
+        st.d = sc; st.de = gs + 6*mob_dof(ba.y);
+        return st;
     }
     __device__ __forceinline__ sv<K> F(int b) const {
         if (!fapp) return svzero<K>();
@@ -1032,7 +1041,7 @@ struct ChainEval {
         for (int b = nb; b >= bstop; --b) {
             int4 bn = ba;
             if (b > bstop) { bn = ldi4(body_a, b-2); prefetch<false, false>(nxt, in, nq, bn, store(b-1, bn, 0), true); }
-            const KStore<K> st = store(b, ba, ldi4(body_b, b-1).w);
+            const KStore<GS, TS> st = store(b, ba, ldi4(body_b, b-1).w);
             const PA P = prm + (size_t)(b-1) * PRM_STRIDE;
             const bool hc = b < nb;
             const sv<K> Fb = F(b);
@@ -1046,7 +1055,7 @@ struct ChainEval {
         }
         if (freeRoot) {
             Pre<K> c0;
-            const KStore<K> st = store(1, a0, ldi4(body_b, 0).w);
+            const KStore<GS, TS> st = store(1, a0, ldi4(body_b, 0).w);
             prefetch<false, true>(c0, in, nq, a0, st, true);
             pass2_body<FREE, K>(prm, c0, grav, F(1), st, R, p, V, nb > 1, PP, zp, phic);
         }

@@ -75,6 +75,7 @@ struct tdyn_batch {
     tdyn::host::DBuf<int> inst;          // instance of each lane
     bool permuted = false;               // lanes are not in instance order
     tdyn::host::DBuf<double> prm, gravity, scratch, prm_bfs;
+    tdyn::host::DBuf<int> op_status;     // per instance: status of the last operator that formed D
     // rollout state
     tdyn::host::DBuf<double> vec, uscale, t, h, t_end;
     tdyn::host::DBuf<int> status, stats;
@@ -113,6 +114,7 @@ struct tdyn_batch {
         std::memset(&a, 0, sizeof(a));
         a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
         a.scratch = scratch.p; a.rec_stride = max_rec + 24; a.off = {q_off.p, u_off.p, b_off.p}; a.inst = permuted ? inst.p : nullptr;
+        a.status = op_status.p;
         return a;
     }
     RkmArgs rk() const {

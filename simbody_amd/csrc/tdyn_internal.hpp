@@ -153,7 +153,7 @@ struct RkmArgs {
     int* stats;              // per lane: attempts accepted rejected projections
     double* uscale;          // SoA nu_max*L
     RkmCfg cfg;
-    int precision;           // 0 = FP64, 1 = mixed
+    int precision;           // 0 = FP64, 1 = mixed, 2/3 = FP64 with single-precision G (and trig) rows
     int chain;               // 1: uniform serial-chain model, use the register-resident chain evaluator
     int idf;                 // chain evaluator: every frame is an identity rotation at the joints
                              // (R_PF = I, p_PF = 0, R_MB = I), so the frame products are skipped
@@ -171,6 +171,9 @@ struct RkmArgs {
     int plan_waves;
     const int* presc;        // known udot (tree sweeps only): per body (nb+1 per instance, packed) nonzero = known
     const double* udot_known; // the known udots (packed u layout)
+    const long long* pool;   // tree-server slots (nullable): per lane, where its state vectors start in vec
+                             // (11 lane-contiguous vectors of its own nq+nu) and its records in scratch,
+                             // in doubles; the lane's tree is tree `lane` of the tables
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that
@@ -223,7 +226,10 @@ hipError_t launch_chain_fp64_batch(const RkmArgs& a, hipStream_t s, bool f0_only
 hipError_t launch_chain_fp64_lane(const RkmArgs& a, hipStream_t s, bool f0_only);
 hipError_t launch_chain_mixed_batch(const RkmArgs& a, hipStream_t s, bool f0_only);
 hipError_t launch_chain_mixed_lane(const RkmArgs& a, hipStream_t s, bool f0_only);
+// precision 2, 3: FP64 arithmetic, single-precision G (and, 3, trig cache) rows
+hipError_t launch_chain_split_batch(const RkmArgs& a, hipStream_t s, bool f0_only);
 inline hipError_t launch_chain_variant(const RkmArgs& a, hipStream_t s, bool f0_only) {
+    if (a.precision >= 2 && !a.prm_lane) return launch_chain_split_batch(a, s, f0_only);
     if (a.precision == 1)
         return a.prm_lane ? launch_chain_mixed_lane(a, s, f0_only) : launch_chain_mixed_batch(a, s, f0_only);
     return a.prm_lane ? launch_chain_fp64_lane(a, s, f0_only) : launch_chain_fp64_batch(a, s, f0_only);
@@ -232,6 +238,15 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s);
 // Level-parallel tree evaluator (wavefront plan a.plan): rollout steps, or
 // (f0_only) the derivative of the current state.
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only);
+// Tree-server slots (pooled tables and state, RkmArgs::pool): admission of n
+// staged requests (per request i: ints at ioff[i], doubles at doff[i], laid
+// out by tdyn_tree_server.cpp) into their slots, and collection of finished
+// slots (q then u at qoff[i] of out, t, status and statistics), which marks
+// the slots empty.
+hipError_t launch_tsrv_admit(const RkmArgs& a, const int* slots, int n, const long long* ioff, const int* ints,
+                             const long long* doff, const double* dbl, hipStream_t s);
+hipError_t launch_tsrv_collect(const RkmArgs& a, const int* slots, int n, const long long* qoff, double* out,
+                               double* t_out, int* outi, hipStream_t s);
 hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
                             double t0, hipStream_t s);
 hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double* out, int out_stride, int* outi,

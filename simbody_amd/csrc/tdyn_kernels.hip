@@ -456,15 +456,27 @@ __device__ void sweep_kin(const Lane& c, const QG& q, const UG& u, const QD& qd)
 // reference's SmallMatrixMixed.h:895-960); dof=6: Gauss-Jordan elimination
 // without pivoting (D = H^T P H is symmetric positive definite for positive
 // mass; the reference uses LAPACK LU with partial pivoting, agreement is at
-// rounding level).
-template <int D> __device__ __forceinline__ void inv_small(const double* A, double* Ai) {
-    if constexpr (D == 1) { Ai[0] = 1.0 / A[0]; }
+// rounding level).  Returns true when D is not invertible as a symmetric
+// positive definite matrix: a determinant (dof<=3) that is not positive, or
+// (dof=6) an elimination pivot that is not above 1e-13 of its diagonal
+// entry (for a positive definite D the ratio is in (0, 1]; below 1e-13, D is
+// singular to working precision), NaN included.  The reference throws for
+// the LAPACK case when a pivot is exactly zero (SmallMatrixMixed.h:878-892)
+// and divides silently for dof<=3; both are reported here, and the inverse
+// is formed anyway (inf/NaN or meaningless values).
+template <int D> __device__ __forceinline__ bool inv_small(const double* A, double* Ai) {
+    bool bad = false;
+    if constexpr (D == 1) { bad = !(A[0] > 0.0); Ai[0] = 1.0 / A[0]; }
     else if constexpr (D == 2) {
-        const double o = 1.0 / (A[0]*A[3] - A[1]*A[2]);
+        const double det = A[0]*A[3] - A[1]*A[2];
+        bad = !(det > 0.0);
+        const double o = 1.0 / det;
         Ai[0] = o*A[3]; Ai[1] = -(o*A[1]); Ai[2] = -(o*A[2]); Ai[3] = o*A[0];
     } else if constexpr (D == 3) {
         const double d00 = A[4]*A[8]-A[5]*A[7], nd01 = A[5]*A[6]-A[3]*A[8], d02 = A[3]*A[7]-A[4]*A[6];
-        const double o = 1.0 / (A[0]*d00 + A[1]*nd01 + A[2]*d02);
+        const double det = A[0]*d00 + A[1]*nd01 + A[2]*d02;
+        bad = !(det > 0.0);
+        const double o = 1.0 / det;
         const double nd10 = A[2]*A[7]-A[1]*A[8], d11 = A[0]*A[8]-A[2]*A[6], nd12 = A[1]*A[6]-A[0]*A[7],
                      d20 = A[1]*A[5]-A[2]*A[4], nd21 = A[2]*A[3]-A[0]*A[5], d22 = A[0]*A[4]-A[1]*A[3];
         Ai[0]=o*d00;  Ai[1]=o*nd10; Ai[2]=o*d20;
@@ -476,6 +488,7 @@ template <int D> __device__ __forceinline__ void inv_small(const double* A, doub
         for (int i = 0; i < D*D; ++i) { M[i] = A[i]; Ai[i] = (i % (D+1) == 0) ? 1.0 : 0.0; }
 #pragma unroll
         for (int k = 0; k < D; ++k) {
+            bad = bad || !(M[k*D+k] > 1e-13 * A[k*D+k]);
             const double ip = 1.0 / M[k*D+k];
 #pragma unroll
             for (int j = 0; j < D; ++j) { M[k*D+j] *= ip; Ai[k*D+j] *= ip; }
@@ -488,6 +501,7 @@ template <int D> __device__ __forceinline__ void inv_small(const double* A, doub
             }
         }
     }
+    return bad;
 }
 
 enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8, IN_PRES = 16 };
@@ -505,7 +519,7 @@ struct Pres { const int* m; const double* up; double* tau; };
 // TYPE = GEN: the padded generic body (see kin_body_gen); dof from ba.y.
 template <int TYPE> struct MobD { static constexpr int D = TYPE == GEN ? 3 : Mob<TYPE>::D; };
 template <int TYPE, int MODE, class FB, class FM>
-__device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
+__device__ __forceinline__ bool inward_body(const Lane& c, int b, const int4 ba, const int4 bb,
                                             const FB& fbody, const FM& fmob,
                                             double* outP, double* outPP, const double* grav, Pres pr = {}) {
     // (IN_PRES) a body with known udot: P+ = P, z+ = z and z += P H udot; its
@@ -540,7 +554,7 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
 #pragma unroll
         for (int j = 0; j < D; ++j) if (j >= d) Dm[j*D+j] = 1.0;
     }
-    inv_small<D>(Dm, DI);
+    const bool singular = inv_small<D>(Dm, DI);
     dv G[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -629,23 +643,27 @@ __device__ __forceinline__ void inward_body(const Lane& c, int b, const int4 ba,
             c.s(rb + REC_DIEPS + r) = pres ? eps[r] : s;
         }
     }
+    return singular;
 }
 
+// Returns true when some body's D was not invertible (inv_small).
 template <int MODE, bool PAD = false, class FB, class FM>
-__device__ void sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, double* outP, double* outPP,
+__device__ bool sweep_inward(const Lane& c, const FB& fbody, const FM& fmob, double* outP, double* outPP,
                              const double* grav, Pres pr = {}) {
+    bool sing = false;
     for (int b = c.nb; b >= 1; --b) {
         const int4 ba = c.ba(b), bb = c.bb(b);
         if (PAD || mob_generic_only(ba.y)) {
-            if (ba.y != FREE) { inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); continue; }
+            if (ba.y != FREE) { sing |= inward_body<GEN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); continue; }
         }
         switch (ba.y) {
-        case PIN: case SLIDER: inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
-        case UNIVERSAL:        inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
-        case BALL:             inward_body<BALL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
-        default:               inward_body<FREE, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        case PIN: case SLIDER: sing |= inward_body<PIN, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        case UNIVERSAL:        sing |= inward_body<UNIVERSAL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        case BALL:             sing |= inward_body<BALL, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
+        default:               sing |= inward_body<FREE, MODE>(c, b, ba, bb, fbody, fmob, outP, outPP, grav, pr); break;
         }
     }
+    return sing;
 }
 
 // ---------------------------------------------------------------------------
@@ -881,6 +899,11 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 // ---------------------------------------------------------------------------
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
+// Per-instance status of the operators that form articulated inertias:
+// 0, or TDYN_E_SINGULAR (4) when some body's D was not invertible.
+__device__ __forceinline__ void op_status(const OpArgs& a, int lane, bool singular) {
+    if (a.status) a.status[a.inst ? a.inst[lane] : lane] = singular ? 4 : 0;
+}
 // D of a record: padded generic bodies are 3-dof in batches of different trees.
 #define TD_RDOF(ba) (((PAD || mob_generic_only((ba).y)) && (ba).y != FREE) ? 3 : mob_dof((ba).y))
 template <int OP, bool UNI>
@@ -929,7 +952,7 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
         const PackedGetOpt fm{a.fmob ? a.fmob + uo : nullptr};
         const Pres pr{a.presc + bo, a.udot_in + uo, a.out_tau + uo};
-        sweep_inward<IN_FWD | IN_PRES, PAD>(c, fb, fm, nullptr, nullptr, grav, pr);
+        op_status(a, lane, sweep_inward<IN_FWD | IN_PRES, PAD>(c, fb, fm, nullptr, nullptr, grav, pr));
         sweep_outward2<true, PAD, true>(c, q, u, PackedPut{a.out_u + uo}, NoPut{}, pr);
         if (a.out_body) {
             for (int b = 0; b <= c.nb; ++b) {
@@ -942,8 +965,8 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
         sweep_kin<true, PAD>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
         const PackedGetOpt fm{(OP != OP_FWD_SYS && a.fmob) ? a.fmob + uo : nullptr};
-        if constexpr (OP == OP_FWD_SYS) sweep_inward<IN_FWD | IN_SYSF, PAD>(c, fb, fm, nullptr, nullptr, grav);
-        else sweep_inward<IN_FWD, PAD>(c, fb, fm, nullptr, nullptr, grav);
+        if constexpr (OP == OP_FWD_SYS) op_status(a, lane, sweep_inward<IN_FWD | IN_SYSF, PAD>(c, fb, fm, nullptr, nullptr, grav));
+        else op_status(a, lane, sweep_inward<IN_FWD, PAD>(c, fb, fm, nullptr, nullptr, grav));
         const PackedPut uout{a.out_u ? a.out_u + uo : nullptr};
         if constexpr (OP == OP_FWD_GIVEN) {
             sweep_outward2<true, PAD>(c, q, u, uout, PackedPut{a.out_q ? a.out_q + qo : nullptr});
@@ -977,12 +1000,12 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     } else if constexpr (OP == OP_MINV) {
         sweep_kin<false, PAD>(c, q, u, NoPut{});
         const PackedGet f{a.fmob + uo};
-        sweep_inward<IN_MINV, PAD>(c, BodyForce{nullptr}, f, nullptr, nullptr, grav);
+        op_status(a, lane, sweep_inward<IN_MINV, PAD>(c, BodyForce{nullptr}, f, nullptr, nullptr, grav));
         sweep_outward2<false, PAD>(c, q, u, PackedPut{a.out_u + uo}, NoPut{});
     } else if constexpr (OP == OP_ABI) {
         sweep_kin<false, PAD>(c, q, u, NoPut{});
-        sweep_inward<IN_ABI_OUT, PAD>(c, BodyForce{nullptr}, Zero{}, a.out_abi + 21*(size_t)bo,
-                                 a.out_abi2 ? a.out_abi2 + 21*(size_t)bo : nullptr, grav);
+        op_status(a, lane, sweep_inward<IN_ABI_OUT, PAD>(c, BodyForce{nullptr}, Zero{}, a.out_abi + 21*(size_t)bo,
+                                                         a.out_abi2 ? a.out_abi2 + 21*(size_t)bo : nullptr, grav));
         for (int i = 0; i < 21; ++i) {
             a.out_abi[21*(size_t)bo + i] = 0.0;
             if (a.out_abi2) a.out_abi2[21*(size_t)bo + i] = 0.0;
@@ -1152,6 +1175,28 @@ struct TreeEval {
             sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
             sweep_outward2<true, PAD>(c, y, u, fu, NoPut{});
         }
+    }
+};
+
+// Tree-server slots: the tree sweeps on a slot's own pooled regions (its
+// records, lane-contiguous, and its state vectors of its own size), so that
+// slots of different trees take only the memory their trees need.
+struct TreePoolEval {
+    using VR = tstream::LaneVec;
+    static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
+        const int4 ta = ldi4(a.T.tree_a, lane);
+        return VR{a.vec + a.pool[2*lane] + (size_t)v * (ta.y + ta.z)};
+    }
+    Lane c; const double* grav; const double* fapp;
+    __device__ TreePoolEval(const RkmArgs& a, int lane, int tr)
+        : c(make_lane_t(a.T, tr, 0, 1, a.scratch + a.pool[2*lane + 1], 1)), grav(a.T.gravity + 4*tr),
+          fapp(a.fapp + 6*(size_t)a.off.b_off[lane]) {}
+    __device__ __forceinline__ void derivs(const VR& yr, const VR& fr) const {
+        const PackedGet y{yr.p}, u{yr.p + c.nq};
+        const PackedPut f{fr.p}, fu{fr.p + c.nq};
+        sweep_kin<true, true>(c, y, u, f);
+        sweep_inward<IN_FWD | IN_SYSF, true>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+        sweep_outward2<true, true>(c, y, u, fu, NoPut{});
     }
 };
 
@@ -1622,6 +1667,102 @@ __global__ void k_srv_collect(RkmArgs a, const int* slots, int n, double* out, i
     for (int k = 0; k < 4; ++k) outi[5*i + 1 + k] = a.stats[4*slot + k];
     a.status[slot] = 2;
 }
+// Tree-server admission: one 64-thread block per request.  ints: nb, nq, nu,
+// body_off, b_off, nquat, rec, 0, then body_a (4 nb), body_b (4 nb; child
+// begin already offset into the pool), child_idx (nb).  doubles: prm (nb
+// PRM_STRIDE), gravity (3), q (nq), u (nu), t_end, applied forces (6 (nb+1)),
+// the pool offsets of the slot's vectors and records (2).
+__global__ void __launch_bounds__(64) k_tsrv_admit(RkmArgs a, const int* slots, int n, const long long* ioff,
+                                                   const int* ints, const long long* doff, const double* dbl) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int slot = slots[i];
+    const int* I = ints + ioff[i];
+    const double* D = dbl + doff[i];
+    const int nb = I[0], nq = I[1], nu = I[2], body_off = I[3], b_off = I[4];
+    int4* body_a = const_cast<int4*>(a.T.body_a) + body_off;
+    int4* body_b = const_cast<int4*>(a.T.body_b) + body_off;
+    int* child = const_cast<int*>(a.T.child_idx) + body_off;
+    double* prm = const_cast<double*>(a.T.prm) + (size_t)body_off * PRM_STRIDE;
+    const int* ba = I + 8;
+    const int* bb = ba + 4*nb;
+    const int* ci = bb + 4*nb;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        body_a[k] = make_int4(ba[4*k], ba[4*k+1], ba[4*k+2], ba[4*k+3]);
+        body_b[k] = make_int4(bb[4*k], bb[4*k+1], bb[4*k+2], bb[4*k+3]);
+        child[k] = ci[k];
+    }
+    for (int k = threadIdx.x; k < nb * PRM_STRIDE; k += blockDim.x) prm[k] = D[k];
+    const double* g = D + nb * PRM_STRIDE;
+    const double* q = g + 3;
+    const double* u = q + nq;
+    const double te = u[nu];
+    const double* fa = u + nu + 1;
+    const double* po = fa + 6 * (nb + 1);
+    const long long vbase = (long long)po[0], rbase = (long long)po[1];
+    double* y = a.vec + vbase;
+    for (int k = threadIdx.x; k < nq; k += blockDim.x) y[k] = q[k];
+    for (int k = threadIdx.x; k < nu; k += blockDim.x) y[nq + k] = u[k];
+    double* fapp = const_cast<double*>(a.fapp) + 6 * (size_t)b_off;
+    for (int k = threadIdx.x; k < 6 * (nb + 1); k += blockDim.x) fapp[k] = fa[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const_cast<int4*>(a.T.tree_a)[slot] = make_int4(nb, nq, nu, body_off);
+        const_cast<int4*>(a.T.tree_b)[slot] = make_int4(I[6], I[5], 0, 0);
+        double* gr = const_cast<double*>(a.T.gravity) + 4 * slot;
+        gr[0] = g[0]; gr[1] = g[1]; gr[2] = g[2]; gr[3] = 0.0;
+        const_cast<int*>(a.off.b_off)[slot] = b_off;
+        long long* pool = const_cast<long long*>(a.pool);
+        pool[2*slot] = vbase; pool[2*slot + 1] = rbase;
+        // initialize (as rkm_init_lane): quaternions normalized, t, h, stats
+        for (int b = 0; b < nb; ++b) {
+            const int4 x = body_a[b];
+            if (x.y != BALL && x.y != FREE) continue;
+            if (prm[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0) continue;
+            double qq[4], n2 = 0;
+            for (int k = 0; k < 4; ++k) { qq[k] = y[x.z + k]; n2 += qq[k]*qq[k]; }
+            const double nn = sqrt(n2);
+            for (int k = 0; k < 4; ++k) y[x.z + k] = qq[k] / nn;
+        }
+        const RkmCfg& cfg = a.cfg;
+        double hmin = cfg.min_h, hmax = cfg.max_h, hinit = cfg.init_h;
+        if (cfg.fixed_h > 0) hmin = hmax = hinit = cfg.fixed_h;
+        double h = hinit != -1 ? hinit : cfg.timescale / 10;
+        if (hmin != -1) h = fmax(h, hmin);
+        if (hmax != -1) h = fmin(h, hmax);
+        const_cast<double*>(a.t_end)[slot] = te;
+        a.t[slot] = 0.0; a.h[slot] = h;
+        for (int k = 0; k < 4; ++k) a.stats[4*slot + k] = 0;
+        a.status[slot] = 0;
+    }
+}
+__global__ void k_tsrv_collect(RkmArgs a, const int* slots, int n, const long long* qoff, double* out,
+                               double* t_out, int* outi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int slot = slots[i];
+    const int4 ta = a.T.tree_a[slot];
+    const int ny = ta.y + ta.z;
+    const double* y = a.vec + a.pool[2*slot];
+    double* o = out + qoff[i];
+    for (int k = 0; k < ny; ++k) o[k] = y[k];
+    t_out[i] = a.t[slot];
+    outi[5*i] = a.status[slot];
+    for (int k = 0; k < 4; ++k) outi[5*i + 1 + k] = a.stats[4*slot + k];
+    a.status[slot] = 2;
+}
+hipError_t launch_tsrv_admit(const RkmArgs& a, const int* slots, int n, const long long* ioff, const int* ints,
+                             const long long* doff, const double* dbl, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_tsrv_admit<<<dim3(n), dim3(64), 0, s>>>(a, slots, n, ioff, ints, doff, dbl);
+    return hipGetLastError();
+}
+hipError_t launch_tsrv_collect(const RkmArgs& a, const int* slots, int n, const long long* qoff, double* out,
+                               double* t_out, int* outi, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_tsrv_collect<<<dim3((n + 63)/64), dim3(64), 0, s>>>(a, slots, n, qoff, out, t_out, outi);
+    return hipGetLastError();
+}
 hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slots, int n, int rec_stride,
                             double t0, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -1643,6 +1784,7 @@ hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     const dim3 grd((a.n_inst + 63)/64), blk(64);
+    if (a.pool) { k_rkm<TreePoolEval, false><<<grd, blk, 0, s>>>(a); return hipGetLastError(); }
     if (a.chain) return launch_chain_variant(a, s, false);
     if (a.grp) return launch_tree_group(a, s, false);
     if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);

@@ -30,13 +30,13 @@ __device__ __forceinline__ bool adjust_step(double err, double acc, bool limited
 // the chain evaluator's sweeps (no separate vector passes in the common
 // path).  The state ping-pongs between VY and VY1; a lane whose current state
 // sits in VY1 when its launch ends copies it back to VY.
-template <class PA, class K>
+template <class PA, class K, class GS = K, class TS = GS>
 __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
     using namespace chain;
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     if (a.status[lane] != 0) return;
-    const ChainEval<PA, K> ev(a, lane, 0);
+    const ChainEval<PA, K, GS, TS> ev(a, lane, 0);
     const int4 ta = ldi4(a.T.tree_a, 0);
     const int nb = ta.x, nq = ta.y, nu = ta.z, ny = nq + nu;
     const int4* body_a = a.T.body_a + ta.w;
@@ -137,12 +137,12 @@ __global__ void __launch_bounds__(64) k_rkm_chain(RkmArgs a) {
 
 // f = f(y) of the current rollout state (vector VY) into VF0, by the
 // rollout's evaluator at its precision: the derivative the integrator uses.
-template <class PA, class K>
+template <class PA, class K, class GS = K, class TS = GS>
 __global__ void __launch_bounds__(64) k_chain_f0(RkmArgs a) {
     using namespace chain;
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
-    const ChainEval<PA, K> ev(a, lane, 0);
+    const ChainEval<PA, K, GS, TS> ev(a, lane, 0);
     const size_t NV = (size_t)a.ny_max * a.L;
     auto rows = [&](int v) { return Rows{a.vec + v*NV, a.L, (unsigned)lane}; };
     const RkRows r{rows(VY), rows(VF0), rows(VFA), rows(VFB), rows(VYT), rows(VF0), rows(VY1), rows(VYE),
@@ -151,12 +151,12 @@ __global__ void __launch_bounds__(64) k_chain_f0(RkmArgs a) {
     ev.template derivs_rk<0>(r, 0.0, ns);
 }
 
-// Launches k_rkm_chain<PA, K> (f0_only: k_chain_f0<PA, K>).
-template <class PA, class K>
+// Launches k_rkm_chain<PA, K, GS, TS> (f0_only: k_chain_f0).
+template <class PA, class K, class GS = K, class TS = GS>
 hipError_t launch_chain_t(const RkmArgs& a, hipStream_t s, bool f0_only) {
     const dim3 grd((a.n_inst + 63)/64), blk(64);
-    if (f0_only) k_chain_f0<PA, K><<<grd, blk, 0, s>>>(a);
-    else k_rkm_chain<PA, K><<<grd, blk, 0, s>>>(a);
+    if (f0_only) k_chain_f0<PA, K, GS, TS><<<grd, blk, 0, s>>>(a);
+    else k_rkm_chain<PA, K, GS, TS><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ TDYN_ERRORS = {1: "TDYN_E_ARG", 2: "TDYN_E_SIZE", 3: "TDYN_E_STAGE", 4: "TDYN_E_
 
 EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator", "tdyn_batch_get_evaluator",
            "tdyn_batch_set_tree_lanes", "tdyn_batch_set_tree_lane_range",
-           "tdyn_batch_info",
+           "tdyn_batch_info", "tdyn_batch_op_status",
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
@@ -41,7 +41,10 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_rollout_extend", "tdyn_rollout_reports",
            "tdyn_last_error",
            "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
-           "tdyn_server_advance", "tdyn_server_collect"]
+           "tdyn_server_advance", "tdyn_server_collect",
+           "tdyn_tree_server_create", "tdyn_tree_server_destroy", "tdyn_tree_server_info",
+           "tdyn_tree_server_request_bytes", "tdyn_tree_server_admit", "tdyn_tree_server_advance",
+           "tdyn_tree_server_collect"]
 
 
 class TdynError(RuntimeError):
@@ -88,6 +91,7 @@ def lib():
         L.tdyn_batch_set_tree_lanes.argtypes = [vp, ctypes.c_int]
         L.tdyn_batch_set_tree_lane_range.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.tdyn_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.tdyn_batch_op_status.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int), vp]
         L.tdyn_batch_offsets.argtypes = [vp, _i32, _i32, _i32]
         L.tdyn_realize_kinematics.argtypes = [vp] + [vp] * 6 + [vp]
         L.tdyn_realize_acceleration.argtypes = [vp] + [vp] * 5 + [vp]
@@ -122,6 +126,17 @@ def lib():
         L.tdyn_server_admit.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TreeDescC), _d, _d, _d, _d, _i32,
                                         ctypes.POINTER(ctypes.c_int), vp]
         L.tdyn_server_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_tree_server_create.argtypes = [ctypes.POINTER(IntegratorCfgC), ctypes.c_size_t, ctypes.c_int,
+                                              ctypes.POINTER(vp)]
+        L.tdyn_tree_server_destroy.argtypes = [vp]
+        L.tdyn_tree_server_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+        L.tdyn_tree_server_request_bytes.argtypes = [ctypes.POINTER(TreeDescC), ctypes.POINTER(ctypes.c_int64)]
+        L.tdyn_tree_server_admit.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TreeDescC), _d, _d, _d, _d, _i32,
+                                             ctypes.POINTER(ctypes.c_int), vp]
+        L.tdyn_tree_server_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_tree_server_collect.argtypes = [vp, ctypes.c_int, ctypes.c_int64, _i32, _d,
+                                               ctypes.POINTER(ctypes.c_int64), _d, _i32, _i32,
+                                               ctypes.POINTER(ctypes.c_int), vp]
         L.tdyn_server_collect.argtypes = [vp, ctypes.c_int, _i32, _d, _d, _d, _i32, _i32,
                                           ctypes.POINTER(ctypes.c_int), vp]
         _lib = L
@@ -295,6 +310,22 @@ class MatterBatch:
         _check(lib().tdyn_realize_kinematics(self.h, _ptr(q), _ptr(u), _ptr(X), _ptr(V), _ptr(qd),
                                              _ptr(E), _stream()))
         return X, V, qd, E
+
+    def op_status(self):
+        """Per-instance status of the last operator that formed the
+        articulated-body inertias (int32 device tensor): 0, or 4
+        (TDYN_E_SINGULAR) where some body's D was not invertible."""
+        import torch
+        st = torch.empty(self.n_instances, dtype=torch.int32, device=self.device)
+        _check(lib().tdyn_batch_op_status(self.h, _ptr(st), None, _stream()))
+        return st
+
+    def check_singular(self):
+        """Raises TdynError(TDYN_E_SINGULAR) when the last operator that formed
+        the articulated-body inertias met a singular D in some instance (what
+        the reference throws, SmallMatrixMixed.h:878-892)."""
+        n = ctypes.c_int(0)
+        _check(lib().tdyn_batch_op_status(self.h, None, ctypes.byref(n), _stream()))
 
     def calcKineticEnergy(self, q, u):
         _, _, _, E = self.realizeKinematics(q, u, False, False)

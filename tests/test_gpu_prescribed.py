@@ -134,3 +134,23 @@ def test_autotune_with_known_udot():
                                               Fapp=fs[i])
         assert (s["accepted"], s["rejected"]) == tuple(st[i, 1:3]), (i, s, st[i])
         assert rel_inf(np.concatenate([q1s[i], u1s[i]]), y) < 1e-8, i
+
+
+def test_autotune_cache_hit_outside_the_candidates_is_reprobed(tmp_path):
+    """A cached choice that this batch cannot run (the one-lane streaming
+    evaluator, stored for a batch of the same shape without known udot) is
+    not applied: the autotuner probes again, and the rollout it sets up
+    begins."""
+    from simbody_amd.matter import RungeKuttaMerson
+    per, B, qs, us, fs, masks, known = _known_case("ragged")
+    B.set_known_udot(B.pack_body_flags(masks), B.pack(known))
+    rkm = RungeKuttaMerson(accuracy=1e-6)
+    cache = str(tmp_path / "tune.txt")
+    B.autotune(B.pack(qs), B.pack(us), rkm, 0.02, fapp=B.pack(fs), cache=cache)
+    key = open(cache).read().splitlines()[-1].split("\t")[0]
+    assert "known1" in key
+    with open(cache, "a") as f:
+        f.write(key + "\t3 1 0 32\n")
+    chosen = B.autotune(B.pack(qs), B.pack(us), rkm, 0.02, fapp=B.pack(fs), cache=cache)
+    assert chosen[1] != 1 and chosen[4] == 0, chosen
+    B.stepTo(B.pack(qs), B.pack(us), rkm, 0.05, fapp=B.pack(fs))
