@@ -309,6 +309,28 @@ int tdyn_rollout_advance(tdyn_batch* b, int max_steps, hipStream_t s);
  * trajectory at t_1, t_2, ... exactly as Simbody's TimeStepper does with
  * interpolation off. */
 int tdyn_rollout_extend(tdyn_batch* b, const double* t_end, double t_end_all, hipStream_t s);
+/* Size of the device array that tdyn_rollout_noalloc works in (one copy per
+ * GPU the library runs on, held until the process ends). */
+#define TDYN_NOALLOC_SCRATCH_BYTES (32ull << 20)
+
+/* tdyn_rollout without allocating: no reference counterpart (the reference
+ * is a CPU library); for callers that cannot allocate device memory when
+ * they call (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results as tdyn_rollout (q, u
+ * in and out, device; fapp, stats nullable; one final time t_end for every
+ * instance), run by the chain evaluator or the tree sweeps as the batch's
+ * evaluator setting selects (not the level-parallel evaluator: evaluator 3
+ * is TDYN_E_ARG, and batches of different trees on the automatic setting run
+ * the tree sweeps), FP64 only.  The per-lane records and the whole
+ * integrator state live in a device array of TDYN_NOALLOC_SCRATCH_BYTES
+ * instead of the batch's buffers (about 38 KB per instance of a 32-body
+ * chain, so some 850 such instances); a batch that needs more is refused
+ * with TDYN_E_SIZE.  Calls that overlap
+ * on one GPU share the array and race; the batch's own rollout (begin /
+ * advance / get) is not touched.  Synchronizes s before returning. */
+int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const double* fapp, double t0, double t_end,
+                         const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s);
+
 /* Copy out state (packed q, u), per-instance time t, status (0 running,
  * 1 reached t_end, <0 failed) and stats (attempts, accepted, rejected,
  * projections; 4 ints per instance).  Any output may be NULL. */

@@ -39,6 +39,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_multiply_by_system_jacobian_transpose", "tdyn_calc_body_acceleration_from_udot",
            "tdyn_calc_composite_body_inertias", "tdyn_rollout_begin", "tdyn_rollout_advance",
            "tdyn_rollout_get", "tdyn_rollout_derivs", "tdyn_rollout", "tdyn_rollout_extend", "tdyn_rollout_reports",
+           "tdyn_rollout_noalloc",
            "tdyn_last_error",
            "tdyn_server_create", "tdyn_server_destroy", "tdyn_server_info", "tdyn_server_admit",
            "tdyn_server_advance", "tdyn_server_collect",
@@ -116,6 +117,8 @@ def lib():
         L.tdyn_rollout_reports.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double, vp, ctypes.c_int,
                                            ctypes.POINTER(IntegratorCfgC), vp, vp, vp, vp]
         L.tdyn_rollout_get.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.tdyn_rollout_noalloc.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
+                                           ctypes.POINTER(IntegratorCfgC), vp, vp]
         L.tdyn_rollout_derivs.argtypes = [vp, vp, vp, vp]
         L.tdyn_rollout.argtypes = [vp, vp, vp, vp, ctypes.c_double, ctypes.c_double,
                                    ctypes.POINTER(IntegratorCfgC), vp, vp]
@@ -520,6 +523,16 @@ class MatterBatch:
         qd = self._new(self.total_q); ud = self._new(self.total_u)
         _check(lib().tdyn_rollout_derivs(self.h, _ptr(qd), _ptr(ud), _stream()))
         return qd, ud
+
+    def stepTo_noalloc(self, q, u, integ, t_end, fapp=None, t0=0.0, stats=None):
+        """stepTo through tdyn_rollout_noalloc: no device allocation in the
+        call (the integrator state lives in the library's fixed device
+        array); q and u are advanced in place and stats (int32 device tensor
+        of 4 per instance, or None) receives the statistics."""
+        cfg = integ.c()
+        _check(lib().tdyn_rollout_noalloc(self.h, _ptr(q), _ptr(u), _ptr(fapp), ctypes.c_double(t0),
+                                          ctypes.c_double(t_end), ctypes.byref(cfg), _ptr(stats), _stream()))
+        return q, u, stats
 
     def stepTo(self, q, u, integ, t_end, fapp=None, t0=0.0):
         """TimeStepper(system, integ).initialize(state); stepTo(t_end), with
