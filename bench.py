@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE per dispatch (difference of a 10-step and a 1-step
 # launch, over 9 steps x 2,097,152 bodies), each scaled by what the same
 # counter reads for a plain 2 GiB device copy (FETCH x2.0, WRITE x1.0).
-TRAFFIC_FETCH_PER_BODY_STEP = 1596.3
-TRAFFIC_WRITE_PER_BODY_STEP = 1048.0
+TRAFFIC_FETCH_PER_BODY_STEP = 1602.2
+TRAFFIC_WRITE_PER_BODY_STEP = 1048.6
 
 
 def log(msg):
@@ -194,7 +194,7 @@ def row_trees(torch, M, MatterBatch, RungeKuttaMerson, n=16384, cpu=True, cpu_bu
     return out
 
 
-def row_serving(torch, M, RungeKuttaMerson, rates=(500, 2000, 8000), mixed_rates=(2000,), window=10.0,
+def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 8000, 32000), mixed_rates=(8000,), window=10.0,
                 warmup=2.0, seed=5, budget_gib=24, steps_per_advance=16, n_models=8192):
     """Serving loop, steady state: Poisson arrivals for `window` seconds at
     each rate, the first `warmup` seconds discarded; every request is a
@@ -239,7 +239,7 @@ def row_serving(torch, M, RungeKuttaMerson, rates=(500, 2000, 8000), mixed_rates
     return out
 
 
-def row_tree_serving(torch, M, RungeKuttaMerson, rates=(250, 1000, 4000), window=10.0, warmup=2.0, seed=7,
+def row_tree_serving(torch, M, RungeKuttaMerson, rates=(1000, 4000, 16000), window=10.0, warmup=2.0, seed=7,
                      budget_gib=24, steps_per_advance=16, n_models=2048):
     """Serving requests of different trees (TreeServer): Poisson arrivals for
     `window` seconds, the first `warmup` discarded; each request one uneven

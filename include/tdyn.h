@@ -196,6 +196,24 @@ int tdyn_calc_acceleration_with_known_udot(tdyn_batch* b, const double* q, const
  * other than 1). */
 int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double* udot_known);
 
+/* Prescribed motion for rollouts: Motion::Sinusoid and Motion::Steady
+ * (Simbody/include/simbody/internal/Motion.h:240-300; MotionImpl.h:182-300)
+ * on any bodies.  motion: device, 4 doubles per body (nb+1 per instance,
+ * Ground first, the body layout of F): level, amplitude a, rate w, phase
+ * phi, with level 0 none; 1 position, every q = a sin(w t + phi) and u =
+ * qdot (mobilizers with qdot = u: not Ball or Free); 2
+ * velocity, every u = a sin(w t + phi); 3 acceleration, every udot = a
+ * sin(w t + phi); 4 steady velocity, every u = a.  As in the reference's
+ * integrator (IntegratorRep.h:901-934), every stage is evaluated with the
+ * prescribed q, u and udot at its own time and each step's end state gets
+ * the prescribed q and u before its error estimate.  Checked and copied
+ * once here (the array is read again by later rollouts: keep it alive);
+ * takes effect at the next tdyn_rollout_begin; NULL clears.  Rollouts with
+ * motion run on the tree sweeps in FP64 (evaluator 0 or 1), without known
+ * udot.  The reference's counterpart: adding Motion objects to mobilized
+ * bodies. */
+int tdyn_batch_set_motion(tdyn_batch* b, const double* motion);
+
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
  * .cpp:431-471). */
 int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV,

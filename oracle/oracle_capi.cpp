@@ -286,6 +286,20 @@ int oracle_integrate_trace(void* p, double* y, const double* Fapp, double t0, do
     return st.status;
 }
 
+// Integration to T with prescribed motion: motion [nb+1][4] (Ground row
+// first: level, amplitude, rate, phase; levels as MotionLevel).
+int oracle_integrate_motion(void* p, double* y, const double* Fapp, const double* motion, double t0, double T,
+                            double accuracy, double fixed_h, long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    rk.motion = motion;
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
+    RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
 // Integration through report times stops[0..n) (the last is the final time)
 // with interpolation off: every step stops at the next report time; yout
 // [n][nq+nu] gets the state there (Rkm::integrateStops).

@@ -50,7 +50,7 @@ def _open(path):
                  "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
                  "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
                  "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
-                 "oracle_integrate_trace", "oracle_abi_shift"]:
+                 "oracle_integrate_trace", "oracle_abi_shift", "oracle_integrate_motion"]:
         getattr(L, name).restype = ctypes.c_int
     return L
 
@@ -244,6 +244,19 @@ class OracleModel:
         names = ["t0", "t1", "err_pre", "qerr", "err", "projected", "accepted", "h_next"]
         tr = np.rec.fromarrays(rec[:min(n.value, max_rec)].T, names=names)
         return y, dict(zip(keys, list(st))), tr
+
+    def integrate_motion(self, y, T, accuracy, motion, fixed_h=0.0, Fapp=None, t0=0.0):
+        """integrate() with prescribed motion: motion (nb+1, 4) rows, Ground
+        first: level (0 none, 1 position sinusoid, 2 velocity sinusoid, 3
+        acceleration sinusoid, 4 steady velocity), amplitude, rate, phase."""
+        y = np.array(y, dtype=np.float64, copy=True)
+        m = np.ascontiguousarray(motion, np.float64).reshape(-1)
+        Fa = None if Fapp is None else np.ascontiguousarray(Fapp, np.float64)
+        st = (ctypes.c_long * 6)()
+        self._L.oracle_integrate_motion(self.h, _p(y), _p(Fa), _p(m), ctypes.c_double(t0), ctypes.c_double(T),
+                                        ctypes.c_double(accuracy), ctypes.c_double(fixed_h), st)
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        return y, dict(zip(keys, list(st)))
 
     def integrate_stops(self, y, stops, accuracy, fixed_h=0.0, Fapp=None, t0=0.0):
         """Integration through the report times `stops` with interpolation

@@ -1148,16 +1148,71 @@ struct RkmStats { long attempts=0, accepted=0, rejected=0, projections=0, evals=
 // accepted, and the step size the controller chose next.
 struct RkmAttempt { double t0, t1, err_pre, qerr, err, projected, accepted, h_next; };
 
+// Prescribed motion of a body's mobilities (Motion.h:240-300, MotionImpl.h:
+// 182-300): Motion::Sinusoid m(t) = a sin(w t + phi) applied to every q
+// (position level: u = qdot = m', udot = m''; mobilizers with qdot = u
+// only), to every u (velocity level: udot = m') or to every udot
+// (acceleration level); Motion::Steady: every u = a, udot = 0.  Four doubles
+// per body, Ground first: level, a, w, phi.
+enum MotionLevel { MOTION_NONE = 0, MOTION_POSITION = 1, MOTION_VELOCITY = 2, MOTION_ACCELERATION = 3,
+                   MOTION_STEADY = 4 };
+
 template <class T> struct Rkm {
     const Model& M;
     Tree<T> tr;
     Workspace<T> ws;
     std::vector<RkmAttempt>* trace = nullptr;   // nullable: records every attempt
+    const double* motion = nullptr;             // nullable: prescribed motion, 4 per body (Ground first)
+    std::vector<int> mPresc;                    // bodies with prescribed udot (from motion)
+    std::vector<double> mUdot;                  // their udot at the evaluation's time
+    std::vector<T> yp;                          // the evaluated state with q, u prescribed
     std::vector<T> y0, f0, fa, fb, ysave, y1, ytmp, yerr, uscale;
     explicit Rkm(const Model& m) : M(m), tr(m) {
         const int ny = m.nq + m.nu;
         y0.resize(ny); f0.resize(ny); fa.resize(ny); fb.resize(ny);
         ysave.resize(ny); y1.resize(ny); ytmp.resize(ny); yerr.resize(ny); uscale.resize(m.nu);
+    }
+
+    // prescribeQ, prescribeU (SimbodyMatterSubsystemRep.cpp:3776-3830) and
+    // the prescribed udot at time t: q and u of y overwritten where the
+    // motion prescribes them, udot into mUdot.
+    void prescribe(double t, T* y) {
+        mPresc.assign(M.nb + 1, 0);
+        mUdot.assign(M.nu, 0.0);
+        for (int b = 1; b <= M.nb; ++b) {
+            const double* m = motion + 4*b;
+            const int level = (int)m[0];
+            if (level == MOTION_NONE) continue;
+            mPresc[b] = 1;
+            const double a = m[1], w = m[2], ph = m[3];
+            const double sn = std::sin(w*t + ph), cs = std::cos(w*t + ph);
+            const double f = a*sn, fd = a*w*cs, fdd = -a*w*w*sn;
+            const int qi = M.qIndex[b], ui = M.uIndex[b];
+            for (int i = 0; i < M.dof[b]; ++i) {
+                if (level == MOTION_POSITION) {
+                    y[qi + i] = T(f); y[M.nq + ui + i] = T(fd); mUdot[ui + i] = fdd;
+                } else if (level == MOTION_VELOCITY) {
+                    y[M.nq + ui + i] = T(f); mUdot[ui + i] = fd;
+                } else if (level == MOTION_STEADY) {
+                    y[M.nq + ui + i] = T(a); mUdot[ui + i] = 0.0;
+                } else {
+                    mUdot[ui + i] = f;
+                }
+            }
+        }
+    }
+    // The derivative at time t (setAdvancedStateAndRealizeDerivatives,
+    // IntegratorRep.h:901-915): the prescribed q and u replace the state's,
+    // the prescribed udot are known udot.  y itself is not changed.
+    bool derivsAt(double t, const T* y, const SV<T>* Fapp, T* ydot) {
+        if (!motion) return derivs(tr, ws, y, Fapp, ydot);
+        const int ny = M.nq + M.nu;
+        yp.assign(y, y + ny);
+        prescribe(t, yp.data());
+        ws.presc = mPresc.data(); ws.udotKnown = mUdot.data();
+        const bool ok = derivs(tr, ws, yp.data(), Fapp, ydot);
+        ws.presc = nullptr; ws.udotKnown = nullptr;
+        return ok;
     }
 
     // Quaternion normalization of q (and, if given, removal of the radial
@@ -1210,19 +1265,25 @@ template <class T> struct Rkm {
     }
 
     // One RK-Merson attempt from (t0,y0,f0) to t1.  Fills y1, yerr, ysave.
+    // Stage times t0 + h/3, t0 + h/3, t0 + h/2, t1; y1 gets the prescribed
+    // q and u at t1 before the error estimate (setAdvancedStateAndRealize-
+    // Kinematics, IntegratorRep.h:920-934, RungeKuttaMersonIntegrator.cpp:
+    // 125-137).
     bool attemptODE(double t0, double t1, const SV<T>* Fapp) {
         const int ny = M.nq + M.nu;
         const T h = T(t1 - t0);
+        const double hd = t1 - t0;
         bool ok = true;
         for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(3))*f0[i];
-        ok &= derivs(tr, ws, ytmp.data(), Fapp, fa.data());
+        ok &= derivsAt(t0 + hd/3, ytmp.data(), Fapp, fa.data());
         for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(6))*(f0[i] + fa[i]);
-        ok &= derivs(tr, ws, ytmp.data(), Fapp, fa.data());
+        ok &= derivsAt(t0 + hd/3, ytmp.data(), Fapp, fa.data());
         for (int i=0;i<ny;++i) ytmp[i] = y0[i] + (h/T(8))*(f0[i] + T(3)*fa[i]);
-        ok &= derivs(tr, ws, ytmp.data(), Fapp, fb.data());
+        ok &= derivsAt(t0 + hd/2, ytmp.data(), Fapp, fb.data());
         for (int i=0;i<ny;++i) ysave[i] = y0[i] + (h/T(2))*((f0[i] - T(3)*fa[i]) + T(4)*fb[i]);
-        ok &= derivs(tr, ws, ysave.data(), Fapp, fa.data());
+        ok &= derivsAt(t1, ysave.data(), Fapp, fa.data());
         for (int i=0;i<ny;++i) y1[i] = y0[i] + (h/T(6))*((f0[i] + T(4)*fb[i]) + fa[i]);
+        if (motion) prescribe(t1, y1.data());
         for (int i=0;i<ny;++i) yerr[i] = T(0.2)*T(fabs(double(y1[i]-ysave[i])));
         return ok;
     }
@@ -1261,7 +1322,9 @@ template <class T> struct Rkm {
         const double consTol = acc/10;
         double hmin = cfg.min_h, hmax = cfg.max_h, hinit = cfg.init_h;
         if (cfg.fixed_h > 0) hmin = hmax = hinit = cfg.fixed_h;
-        // initialize(): forced projection of the quaternions, then derivatives.
+        // initialize(): prescribed q and u at t0, forced projection of the
+        // quaternions, then derivatives.
+        if (motion) prescribe(t0, y);
         normalizeQuats(y, (T*)nullptr);
         for (int i=0;i<ny;++i) y0[i] = y[i];
         double t = t0;
@@ -1283,7 +1346,7 @@ template <class T> struct Rkm {
                 const double vi = fabs(double(y0[M.nq+i]));
                 uscale[i] = vi*1.0 > 1.0 ? T(1.0/vi) : T(1.0);
             }
-            if (!derivs(tr, ws, y0.data(), Fapp, f0.data())) { st.status = 2; break; }
+            if (!derivsAt(t, y0.data(), Fapp, f0.data())) { st.status = 2; break; }
             ++st.evals;
             const double tMax = stops[k];
             bool succeeded = false;
@@ -1332,7 +1395,7 @@ template <class T> struct Rkm {
             } while (!succeeded);
             if (kr < nrep && reports[kr] <= t1) {
                 std::vector<T> f1(ny);
-                derivs(tr, ws, y1.data(), Fapp, f1.data());
+                derivsAt(t1, y1.data(), Fapp, f1.data());
                 for (; kr < nrep && reports[kr] <= t1; ++kr) {
                     T* yt = rout + (size_t)kr*ny;
                     if (reports[kr] == t1) { for (int i=0;i<ny;++i) yt[i] = y1[i]; continue; }

@@ -395,6 +395,40 @@ int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double*
     return TDYN_OK;
 }
 
+int tdyn_batch_set_motion(tdyn_batch* b, const double* motion) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (!motion) { b->motion = nullptr; return TDYN_OK; }
+    // the description is checked once, here: levels, finite values, and
+    // position level only on mobilizers whose qdot is u
+    const size_t nbody = (size_t)b->total_b;
+    std::vector<double> m(4 * nbody);
+    HIPCHK(hipMemcpy(m.data(), motion, m.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int> flags(nbody, 0);
+    for (int i = 0; i < b->n_inst; ++i) {
+        const int tr = b->h_tree_of.empty() ? 0 : b->h_tree_of[i];
+        const int4 ta = b->h_tree_a[tr];
+        for (int k = 1; k <= ta.x; ++k) {
+            const double* r = m.data() + 4 * ((size_t)b->h_boff[i] + k);
+            const int level = (int)r[0];
+            if (r[0] != (double)level || level < 0 || level > 4 || !std::isfinite(r[1]) || !std::isfinite(r[2]) ||
+                !std::isfinite(r[3]))
+                return fail(TDYN_E_ARG, "tdyn_batch_set_motion: instance " + std::to_string(i) + " body " +
+                                        std::to_string(k) + ": level must be 0-4 and a, w, phi finite");
+            const int ty = b->h_body_a[ta.w + k - 1].y;
+            if (level == 1 && (ty == TDYN_BALL || ty == TDYN_FREE))
+                return fail(TDYN_E_ARG, "tdyn_batch_set_motion: position-level motion needs a mobilizer with "
+                                        "qdot = u (not Ball or Free)");
+            flags[b->h_boff[i] + k] = level != 0;
+        }
+    }
+    HIPCHK(b->motion_flags.upload(flags));
+    if (b->motion_udot.n != (size_t)std::max<int64_t>(1, b->total_u))
+        HIPCHK(b->motion_udot.alloc((size_t)std::max<int64_t>(1, b->total_u)));
+    HIPCHK(hipMemset(b->motion_udot.p, 0, b->motion_udot.n * sizeof(double)));
+    b->motion = motion;
+    return TDYN_OK;
+}
+
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
     if (!(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64))
@@ -596,8 +630,12 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || prec != 0))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps or the level-parallel "
                                 "tree evaluator (not one lane per instance), in FP64");
+    if (b->motion && (b->known_p || b->evaluator == 2 || b->evaluator == 3 || prec != 0))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: prescribed motion runs on the tree sweeps (evaluator 0 or 1), "
+                                "in FP64, without known udot");
     const size_t L = b->n_inst;
-    const bool aos = b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) && !(b->known_p && b->tree_lanes == 1);
+    const bool aos = !b->motion && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) &&
+                     !(b->known_p && b->tree_lanes == 1);
     const int grp = aos && b->tree_lanes != 1 ? 1 : 0;
     // From here on the batch changes: a failure (allocation, upload) ends the
     // rollout in progress.
@@ -628,6 +666,7 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     b->rollout_grp = grp;
     b->roll_known_p = b->known_p;
     b->roll_known_u = b->known_u;
+    b->roll_motion = b->motion;
     b->roll_evaluator = b->evaluator;
     RkmArgs a = b->rk();
     HIPCHK(launch_pack_state(a, q, u, 1, nullptr, nullptr, s));

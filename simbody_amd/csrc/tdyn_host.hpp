@@ -61,10 +61,16 @@ struct tdyn_batch {
     bool any_bushing = false;     // some tree has a Bushing body
     const int* known_p = nullptr;     // tdyn_batch_set_known_udot: caller's device arrays (rollouts)
     const double* known_u = nullptr;
+    // tdyn_batch_set_motion: the caller's device array, its per-body flags
+    // and the per-instance prescribed udot the rollout kernels write
+    const double* motion = nullptr;
+    tdyn::host::DBuf<int> motion_flags;
+    tdyn::host::DBuf<double> motion_udot;
     // what the current rollout was begun with: advance, get and derivs read
     // these, never the settings changed since tdyn_rollout_begin
     const int* roll_known_p = nullptr;
     const double* roll_known_u = nullptr;
+    const double* roll_motion = nullptr;
     int roll_evaluator = 0;
     std::vector<int> h_qoff, h_uoff, h_boff, h_tree_of;
     std::vector<int4> h_tree_a, h_tree_b, h_body_a;
@@ -125,10 +131,11 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (!roll_known_p && (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
+        a.chain = (!roll_known_p && !roll_motion && (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
         a.presc = roll_known_p; a.udot_known = roll_known_u;
+        if (roll_motion) { a.motion = roll_motion; a.presc = motion_flags.p; a.udot_known = motion_udot.p; }
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
-        a.ext = (any_rev || any_euler || any_bendstretch || any_bushing || roll_known_p) ? 1 : 0;
+        a.ext = (any_rev || any_euler || any_bendstretch || any_bushing || a.presc) ? 1 : 0;
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;

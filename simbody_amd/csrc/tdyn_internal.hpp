@@ -171,6 +171,10 @@ struct RkmArgs {
     int plan_waves;
     const int* presc;        // known udot (tree sweeps only): per body (nb+1 per instance, packed) nonzero = known
     const double* udot_known; // the known udots (packed u layout)
+    const double* motion;    // prescribed motion (tree sweeps, nullable): 4 per body (nb+1 per instance,
+                             // packed body layout): level (1 position, 2 velocity, 3 acceleration sinusoid,
+                             // 4 steady velocity), amplitude, rate, phase; presc holds its flags and
+                             // udot_known receives the prescribed udot of each evaluation
     const long long* pool;   // tree-server slots (nullable): per lane, where its state vectors start in vec
                              // (11 lane-contiguous vectors of its own nq+nu) and its records in scratch,
                              // in doubles; the lane's tree is tree `lane` of the tables

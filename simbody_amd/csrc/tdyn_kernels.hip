@@ -1153,6 +1153,7 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 template <bool PAD>
 struct TreeEval {
+    static constexpr bool kMotion = true;     // carries prescribed motion
     using VR = chain::RowsT<double, false>;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
         return VR{a.vec + (size_t)v * a.ny_max * a.L, a.L, (unsigned)lane};
@@ -1182,6 +1183,7 @@ struct TreeEval {
 // records, lane-contiguous, and its state vectors of its own size), so that
 // slots of different trees take only the memory their trees need.
 struct TreePoolEval {
+    static constexpr bool kMotion = false;
     using VR = tstream::LaneVec;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
         const int4 ta = ldi4(a.T.tree_a, lane);
@@ -1219,6 +1221,7 @@ struct TreePoolEval {
 // ---------------------------------------------------------------------------
 
 struct TreeStreamEval {
+    static constexpr bool kMotion = false;
     using VR = tstream::LaneVec;
     DevTables T;
     int nb, nq, body_off, acc0, pos0;
@@ -1411,6 +1414,44 @@ struct TreeStreamEval {
 // vector VY; a step evaluates f0 = f(y0) once and then, per attempt, the four
 // Merson stages (RungeKuttaMersonIntegrator.cpp:86-140) through ONE inlined
 // evaluator inside a stage loop.
+// Prescribed motion at time t (prescribeQ/prescribeU, SimbodyMatterSubsystemRep.cpp:
+// 3776-3830, and the prescribed udot; Motion::Sinusoid m(t) = a sin(w t + phi)
+// and Motion::Steady, MotionImpl.h:182-300): q and u of the state rows y
+// overwritten where the motion prescribes them (if qu), the prescribed udot
+// into the lane's known-udot row.  Position level only on mobilizers with
+// qdot = u (checked by the host).
+template <class YR>
+__device__ void prescribe_lane(const RkmArgs& a, int lane, int tr, double t, const YR& y, bool qu) {
+    const int4 ta = ldi4(a.T.tree_a, tr);
+    const int nb = ta.x, nq = ta.y;
+    const double* m0 = a.motion + 4*(size_t)a.off.b_off[lane];
+    double* ud = const_cast<double*>(a.udot_known) + a.off.u_off[lane];
+    for (int b = 1; b <= nb; ++b) {
+        const double* m = m0 + 4*b;
+        const int level = (int)m[0];
+        if (level == 0) continue;
+        const int4 ba = ldi4(a.T.body_a, ta.w + b - 1);
+        const int d = mob_dof(ba.y);
+        double sn, cs;
+        sincos(m[2]*t + m[3], &sn, &cs);
+        const double f = m[1]*sn, fd = m[1]*m[2]*cs, fdd = -m[1]*m[2]*m[2]*sn;
+        for (int i = 0; i < d; ++i) {
+            if (level == 1) {
+                if (qu) { y(ba.z + i) = f; y(nq + ba.w + i) = fd; }
+                ud[ba.w + i] = fdd;
+            } else if (level == 2) {
+                if (qu) y(nq + ba.w + i) = f;
+                ud[ba.w + i] = fd;
+            } else if (level == 4) {
+                if (qu) y(nq + ba.w + i) = m[1];
+                ud[ba.w + i] = 0.0;
+            } else {
+                ud[ba.w + i] = f;
+            }
+        }
+    }
+}
+
 // UNI: every instance uses tree 0, so the tree tables are read with
 // wave-uniform (scalar) loads.
 template <class EV, bool UNI>
@@ -1491,31 +1532,53 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             const double h = t1 - t;
             for (int st = s0; st < 5; ++st) {
                 TRows in = Y, out = F0;
+                double ts = t;             // the stage's time
                 if (st == 1) {
 #pragma unroll 8
                     for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/3)*F0(k);
-                    in = YT; out = FA;
+                    in = YT; out = FA; ts = t + h/3;
                 } else if (st == 2) {
 #pragma unroll 8
                     for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/6)*(F0(k) + FA(k));
-                    in = YT; out = FA;
+                    in = YT; out = FA; ts = t + h/3;
                 } else if (st == 3) {
 #pragma unroll 8
                     for (int k = 0; k < ny; ++k) YT(k) = Y(k) + (h/8)*(F0(k) + 3*FA(k));
-                    in = YT; out = FB;
+                    in = YT; out = FB; ts = t + h/2;
                 } else if (st == 4) {
 #pragma unroll 8
                     for (int k = 0; k < ny; ++k) YS(k) = Y(k) + (h/2)*((F0(k) - 3*FA(k)) + 4*FB(k));
-                    in = YS; out = FA;
+                    in = YS; out = FA; ts = t1;
+                }
+                if constexpr (EV::kMotion) {
+                    if (a.motion) {
+                        // the evaluated state gets the prescribed q and u; ysave
+                        // keeps its own for the error estimate
+                        if (st == 4) {
+                            for (int k = 0; k < ny; ++k) YT(k) = YS(k);
+                            in = YT;
+                        }
+                        prescribe_lane(a, lane, tr, ts, in, st != 0);
+                    }
                 }
                 ev.derivs(in, out);
             }
             s0 = 1;
+            bool motion = false;
+            if constexpr (EV::kMotion) motion = a.motion != nullptr;
+            if (motion) {
+                // the step's end state gets the prescribed q and u at t1 before
+                // the error estimate (setAdvancedStateAndRealizeKinematics)
+                for (int k = 0; k < ny; ++k) Y1(k) = Y(k) + (h/6)*((F0(k) + 4*FB(k)) + FA(k));
+                if constexpr (EV::kMotion) prescribe_lane(a, lane, tr, t1, Y1, true);
+                for (int k = 0; k < ny; ++k) YE(k) = 0.2 * fabs(Y1(k) - YS(k));
+            } else {
 #pragma unroll 8
-            for (int k = 0; k < ny; ++k) {
-                const double y1 = Y(k) + (h/6)*((F0(k) + 4*FB(k)) + FA(k));
-                Y1(k) = y1;
-                YE(k) = 0.2 * fabs(y1 - YS(k));
+                for (int k = 0; k < ny; ++k) {
+                    const double y1 = Y(k) + (h/6)*((F0(k) + 4*FB(k)) + FA(k));
+                    Y1(k) = y1;
+                    YE(k) = 0.2 * fabs(y1 - YS(k));
+                }
             }
             bool converged = true;
             const double e1 = errNorm();
@@ -1581,6 +1644,7 @@ __global__ void __launch_bounds__(64) k_tree_f0(RkmArgs a) {
     if (lane >= a.n_inst) return;
     const int tr = UNI ? 0 : a.tree_of[lane];
     const EV ev(a, lane, tr);
+    if constexpr (EV::kMotion) { if (a.motion) prescribe_lane(a, lane, tr, a.t[lane], EV::vrow(a, VY, lane), true); }
     ev.derivs(EV::vrow(a, VY, lane), EV::vrow(a, VF0, lane));
 }
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
@@ -1599,6 +1663,11 @@ hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s) {
 // initialize(): forced quaternion normalization, t and h set.
 __device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
     const Lane c = make_lane(a.T, a.tree_of, lane, a.L, a.scratch);
+    if (a.motion && !a.vec_aos) {
+        // prescribed q and u at t0 (initialize: prescribeQ, prescribeU)
+        const chain::RowsT<double, false> y{a.vec + (size_t)VY * a.ny_max * a.L, a.L, (unsigned)lane};
+        prescribe_lane(a, lane, a.tree_of ? a.tree_of[lane] : 0, t0, y, true);
+    }
     for (int b = 1; b <= c.nb; ++b) {
         const int4 ba = c.ba(b);
         if (ba.y != BALL && ba.y != FREE) continue;

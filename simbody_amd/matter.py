@@ -32,7 +32,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_offsets",
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
-           "tdyn_batch_set_known_udot", "tdyn_batch_autotune",
+           "tdyn_batch_set_known_udot", "tdyn_batch_set_motion", "tdyn_batch_autotune",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -99,6 +99,7 @@ def lib():
         L.tdyn_calc_acceleration_ignoring_constraints.argtypes = [vp] + [vp] * 7 + [vp]
         L.tdyn_calc_acceleration_with_known_udot.argtypes = [vp] + [vp] * 9 + [vp]
         L.tdyn_batch_set_known_udot.argtypes = [vp, vp, vp]
+        L.tdyn_batch_set_motion.argtypes = [vp, vp]
         L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
                                           ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
@@ -379,6 +380,15 @@ class MatterBatch:
         by this object; None, None clears."""
         self._known = (known, udot_known)
         _check(lib().tdyn_batch_set_known_udot(self.h, _ptr(known), _ptr(udot_known)))
+
+    def set_motion(self, motion=None):
+        """Prescribed motion for the following rollouts (Motion::Sinusoid,
+        Motion::Steady): a float64 device tensor of 4 per body (nb+1 per
+        instance, Ground first): level (1 position, 2 velocity, 3
+        acceleration sinusoid, 4 steady velocity), amplitude, rate, phase;
+        None clears."""
+        self._motion = motion
+        _check(lib().tdyn_batch_set_motion(self.h, _ptr(motion)))
 
     def pack_body_flags(self, per_instance):
         """Concatenate per-instance body flags (nb+1 each, Ground first) into
