@@ -108,45 +108,62 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
 
 // Wavefront plan of the level-parallel tree evaluator for the next rollout
 // (returns 0 and leaves the plan empty for the one-lane streaming
-// evaluator).  Lanes per instance G: the tree_lanes setting, or automatic:
-// the widest level of the instance's tree rounded up to a power of two,
-// clamped to [lanes_min, lanes_max].  A rollout lasts as long as its slowest
-// instance, and the largest trees are the slowest; wide trees get lanes for
-// all the bodies of a level, and their wavefronts come first in the grid.
-// lanes_min 0: the most lanes (up to 16) that keep the grid within about
-// four wavefronts per SIMD (the kernel's registers allow one resident
-// wavefront per SIMD).  Measured on 16,384 trees of 8..128 bodies (body-
-// steps/s): 4, 8, 16, 32, 64 lanes for every instance 0.56, 1.06, 1.32,
-// 1.27, 0.95 x 1e8; per instance in [16, 32] 1.31e8, [8, 64] 1.23e8.
+// evaluator).  Lanes per instance: G, the tree_lanes setting, or automatic:
+// lanes_min (0: the most lanes, up to 16, that keep the batch within about
+// four groups per resident group slot).  The instances are dealt to groups
+// of G lanes by a work queue: persistent wavefronts (as many as the device
+// keeps resident) each run 64/G groups, and a group that finishes an
+// instance takes the next from the queue, so a wavefront's lanes stay busy
+// while its instances' step counts differ.  The queue is ordered by
+// decreasing cost estimate, the level steps of one derivative (sum over the
+// tree's levels of ceil(width / G)); a rollout lasts at least as long as
+// its slowest instance, so with the automatic setting an instance whose
+// estimate exceeds the mean load of a group slot runs first, in a wavefront
+// of its own class with lanes_max lanes.  Measured on 16,384 trees of
+// 8..128 bodies, before the queue (static wavefronts of instances ordered
+// by size; body-steps/s): 4, 8, 16, 32, 64 lanes for every instance 0.56,
+// 1.06, 1.32, 1.27, 0.95 x 1e8; per instance its widest level in [16, 32]
+// 1.31e8.
 int build_tree_plan(tdyn_batch* b, hipStream_t s) {
     if (b->tree_lanes == 1) return 0;
-    int gmin = b->lanes_min, gmax = b->lanes_max;
-    if (gmin <= 0) {
-        const int64_t lanes_target = 4 * 64 * (int64_t)b->n_simd;
-        gmin = 4;
-        while (gmin < 16 && (int64_t)b->n_inst * 2 * gmin <= lanes_target) gmin *= 2;
-    }
-    gmax = std::max(gmax, gmin);
     const int L = b->n_inst;
-    std::vector<int> g(L);
-    for (int l = 0; l < L; ++l) {
-        int w = 4;
-        while (w < b->h_tree_maxw[b->h_lane_tree[l]]) w *= 2;
-        g[l] = b->tree_lanes > 1 ? b->tree_lanes : std::min(gmax, std::max(gmin, w));
+    const int64_t resident = std::max(1, tree_group_resident_waves());
+    int gq = b->tree_lanes > 1 ? b->tree_lanes : b->lanes_min;
+    if (gq <= 0) {
+        gq = 4;
+        while (gq < 16 && (int64_t)L * 2 * gq <= 4 * 64 * resident) gq *= 2;
     }
+    const int gbig = b->tree_lanes > 1 ? gq : std::max(gq, b->lanes_max);
+    auto steps = [&](int l, int G) {
+        int64_t c = 0;
+        for (int w : b->h_tree_lw[b->h_lane_tree[l]]) c += (w + G - 1) / G;
+        return c;
+    };
+    std::vector<int64_t> cost(L);
+    int64_t total = 0;
+    for (int l = 0; l < L; ++l) { cost[l] = steps(l, gq); total += cost[l]; }
+    std::vector<int> ord(L);
+    for (int l = 0; l < L; ++l) ord[l] = l;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    // the mean load of one group slot of the resident wavefronts
+    const double mean_load = (double)total / (double)(resident * (64 / gq));
+    std::vector<int> big, rest;
+    for (int l : ord) ((gbig > gq && cost[l] > mean_load) ? big : rest).push_back(l);
     std::vector<int4> plan; std::vector<int> idx;
-    for (int G = 64; G >= 4; G /= 2) {
-        std::vector<int> mine;
-        for (int l = 0; l < L; ++l) if (g[l] == G) mine.push_back(l);
-        for (size_t k = 0; k < mine.size(); k += 64 / G) {
-            const int cnt = (int)std::min(mine.size() - k, (size_t)(64 / G));
-            plan.push_back(make_int4(G, (int)idx.size(), cnt, 0));
-            idx.insert(idx.end(), mine.begin() + k, mine.begin() + k + cnt);
-        }
+    for (size_t k = 0; k < big.size(); k += 64 / gbig) {
+        const int cnt = (int)std::min(big.size() - k, (size_t)(64 / gbig));
+        plan.push_back(make_int4(gbig, (int)idx.size(), cnt, 0));
+        idx.insert(idx.end(), big.begin() + k, big.begin() + k + cnt);
+    }
+    if (!rest.empty()) {
+        const int64_t waves = std::min<int64_t>(resident, ((int64_t)rest.size() + 64 / gq - 1) / (64 / gq));
+        for (int64_t w = 0; w < waves; ++w) plan.push_back(make_int4(gq, (int)idx.size(), (int)rest.size(), 1));
+        idx.insert(idx.end(), rest.begin(), rest.end());
     }
     hipError_t e;
     if ((e = b->plan.upload(plan)) != hipSuccess || (e = b->plan_idx.upload(idx)) != hipSuccess)
         return hipfail(e, "tree evaluator plan upload");
+    if (!b->plan_ctr.p && (e = b->plan_ctr.alloc(1)) != hipSuccess) return hipfail(e, "tree evaluator queue counter");
     b->plan_waves = (int)plan.size();
     (void)s;
     return 0;
@@ -286,6 +303,9 @@ int tdyn_batch_create(const tdyn_tree_desc* trees, int n_trees, const int32_t* t
             int maxw = 0;
             for (size_t k = 0; k + 1 < starts.size(); ++k) maxw = std::max(maxw, starts[k + 1] - starts[k]);
             B->h_tree_maxw.push_back(maxw);
+            std::vector<int> lw;
+            for (size_t k = 0; k + 1 < starts.size(); ++k) lw.push_back(starts[k + 1] - starts[k]);
+            B->h_tree_lw.push_back(lw);
             lvl.push_back((int)starts.size() - 1);
             lvl.push_back((int)prm_base);
             lvl.insert(lvl.end(), starts.begin(), starts.end());

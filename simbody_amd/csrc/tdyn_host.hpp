@@ -104,7 +104,9 @@ struct tdyn_batch {
     int group_stride = 0;        // scratch doubles per instance of the level-parallel evaluator
     int n_simd = 1024;           // SIMDs of the device (4 per compute unit)
     std::vector<int> h_lane_tree, h_tree_maxw;   // tree of each lane; widest level of each tree
+    std::vector<std::vector<int>> h_tree_lw;     // the width of every level of each tree
     tdyn::host::DBuf<int4> plan; tdyn::host::DBuf<int> plan_idx;   // wavefront plan of the level-parallel evaluator
+    tdyn::host::DBuf<int> plan_ctr;              // its work queue's counter
     int plan_waves = 0;
     RkmCfg cfg{};
 
@@ -139,7 +141,7 @@ struct tdyn_batch {
         a.vec_aos = rollout_aos ? 1 : 0;
         a.stream_dmax = dmax;
         a.grp = rollout_aos ? rollout_grp : 0;
-        a.plan = plan.p; a.plan_idx = plan_idx.p; a.plan_waves = plan_waves;
+        a.plan = plan.p; a.plan_idx = plan_idx.p; a.plan_waves = plan_waves; a.plan_ctr = plan_ctr.p;
         if (rollout_aos) a.rec_stride = a.grp ? group_stride : stream_stride;
         return a;
     }

@@ -166,9 +166,12 @@ struct RkmArgs {
     int stream_dmax;         // streaming tree evaluator: deepest body of any tree
     int grp;                 // >1: level-parallel tree evaluator (vec_aos layout), run by the plan below
     const int4* plan;        // level-parallel evaluator, per wavefront: lanes per instance G, first entry of
-                             // plan_idx, instance count (<= 64/G)
+                             // plan_idx, instance count, kind: 0 the wavefront runs those instances
+                             // (count <= 64/G), 1 its groups take instances from that range as a work
+                             // queue through plan_ctr
     const int* plan_idx;     // instances (lane slots) in plan order
     int plan_waves;
+    int* plan_ctr;           // the work queue's counter (zeroed before each launch)
     const int* presc;        // known udot (tree sweeps only): per body (nb+1 per instance, packed) nonzero = known
     const double* udot_known; // the known udots (packed u layout)
     const double* motion;    // prescribed motion (tree sweeps, nullable): 4 per body (nb+1 per instance,
@@ -225,6 +228,8 @@ __device__ __forceinline__ double& vec_at(const RkmArgs& a, int v, int lane, int
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
 
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
+// Wavefronts of the level-parallel tree evaluator the device keeps resident.
+int tree_group_resident_waves();
 // Chain-evaluator rollouts (f0_only: only the derivative of the current state)
 hipError_t launch_chain_fp64_batch(const RkmArgs& a, hipStream_t s, bool f0_only);
 hipError_t launch_chain_fp64_lane(const RkmArgs& a, hipStream_t s, bool f0_only);

@@ -114,7 +114,7 @@ extern "C" int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const d
     a.idf = (a.chain ? b->idf : b->idf_all) ? 1 : 0;
     a.ext = (b->any_rev || b->any_euler || b->any_bendstretch || b->any_bushing || b->known_p) ? 1 : 0;
     a.vec_aos = 0; a.grp = 0;
-    a.plan = nullptr; a.plan_idx = nullptr; a.plan_waves = 0;
+    a.plan = nullptr; a.plan_idx = nullptr; a.plan_waves = 0; a.plan_ctr = nullptr;
     a.rec_stride = rec;
     a.scratch = reinterpret_cast<double*>(base + o_rec);
     a.vec = reinterpret_cast<double*>(base + o_vec);

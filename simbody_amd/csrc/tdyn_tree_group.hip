@@ -429,19 +429,21 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
     }
 }
 
-// Work distribution: a plan of wavefronts, each running up to 64/G
-// instances with G lanes each (G chosen per wavefront by the host from the
-// width of the instances' tree levels, so that the largest trees, which
-// bound a rollout's duration, get the most lanes); the wavefronts of the
-// widest trees come first in the grid.
+// Work distribution: a plan of wavefronts.  Kind 0 runs up to 64/G given
+// instances with G lanes each; kind 1 (persistent wavefronts, as many as
+// the device keeps resident) has each of its 64/G groups take instances
+// from a work queue (a range of plan_idx in decreasing order of cost) until
+// the queue is empty.  The lanes of a wavefront reconverge after an
+// instance, so its groups take their next instances together; the queue
+// balances the wavefronts against each other (a wavefront whose instances
+// needed few steps takes more of them) and starts the costliest first.
+// (Switching instances per step instead, so that a group whose instance is
+// done does not wait for the others of its wavefront, measured 3-4 % slower
+// on 16,384 trees: the groups' steps fall out of phase and the wavefront
+// runs the union of their paths.)  An instance's arithmetic does not depend
+// on the group or wavefront that runs it.
 template <int IDF, bool F0>
-__device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, int count) {
-    extern __shared__ double lds_frontier[];
-    const int lg = __builtin_ctz((unsigned)G);
-    const int grp = threadIdx.x >> lg, l = threadIdx.x & (G - 1);
-    if (grp >= count) return;
-    const int inst = a.plan_idx[start + grp];
-    double* frontier = lds_frontier + (size_t)grp * 2 * kFrontierFields * G;
+__device__ __forceinline__ void run_instance(const RkmArgs& a, int inst, int l, int G, double* frontier) {
     if constexpr (F0) {
         const GroupEval<IDF> ev(a, inst, a.tree_of[inst], l, G, frontier);
         ev.derivs(GroupEval<IDF>::vrow(a, VY, inst), GroupEval<IDF>::vrow(a, VF0, inst));
@@ -450,15 +452,35 @@ __device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, in
         grp_instance<IDF>(a, inst, l, G, frontier);
     }
 }
+template <int IDF, bool F0>
+__device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, int count, int kind) {
+    extern __shared__ double lds_frontier[];
+    const int lg = __builtin_ctz((unsigned)G);
+    const int grp = threadIdx.x >> lg, l = threadIdx.x & (G - 1);
+    double* frontier = lds_frontier + (size_t)grp * 2 * kFrontierFields * G;
+    const int leader = threadIdx.x & ~(G - 1);
+    // (one call site of the evaluator for both kinds)
+    int k = grp;
+    for (;;) {
+        if (kind != 0) {
+            if (l == 0) k = atomicAdd(a.plan_ctr, 1);
+            k = __shfl(k, leader, 64);
+        }
+        if (k >= count) break;
+        run_instance<IDF, F0>(a, a.plan_idx[start + k], l, G, frontier);
+        if (kind == 0) break;
+    }
+}
 // One build for every G (4 to 64, a power of two, the same in the whole
 // wavefront): one copy of the evaluator's code serves all wavefronts.
 template <int IDF, bool F0>
 __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
     const int4 w = a.plan[blockIdx.x];
     const int G = __builtin_amdgcn_readfirstlane(w.x);
-    plan_wave<IDF, F0>(a, G, w.y, w.z);
+    plan_wave<IDF, F0>(a, G, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
+                       __builtin_amdgcn_readfirstlane(w.w));
 }
-
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 } // namespace
 
@@ -470,6 +492,7 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);
+    k_zero_int<<<1, 1, 0, s>>>(a.plan_ctr);
     if (a.ext) {
         if (f0_only) k_rkm_plan<2, true><<<grd, blk, kFrontierBytes, s>>>(a);
         else k_rkm_plan<2, false><<<grd, blk, kFrontierBytes, s>>>(a);
@@ -481,6 +504,15 @@ hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
         else k_rkm_plan<0, false><<<grd, blk, kFrontierBytes, s>>>(a);
     }
     return hipGetLastError();
+}
+
+int tree_group_resident_waves() {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rkm_plan<1, false>, 64, kFrontierBytes) != hipSuccess)
+        return 1024;
+    return std::max(1, cus * per_cu);
 }
 
 } // namespace tdyn
