@@ -18,6 +18,7 @@ namespace {
 struct Handle {
     Model m;
     std::unique_ptr<Tree<double>> tr;
+    std::vector<ConsDef> cons;   // holonomic constraints of the model
 };
 inline Handle* H(void* p) { return static_cast<Handle*>(p); }
 
@@ -49,6 +50,61 @@ void* oracle_model_new(const TreeDesc* d) {
     return h;
 }
 void oracle_model_free(void* p) { delete H(p); }
+
+// Holonomic constraints of the model (Constraint::Ball, Constraint::Rod):
+// per constraint ib[3] = type (0 Ball, 1 Rod), body 1, body 2 (0 = Ground)
+// and dd[7] = station on body 1, station on body 2 (body frames), length.
+// Returns -1 for an invalid description (bodies out of range or equal, an
+// unknown type, a non-positive rod length, Euler-angle models).
+int oracle_set_constraints(void* p, int n, const int* ib, const double* dd) {
+    Handle* h = H(p); const Model& M = h->m;
+    std::vector<ConsDef> cs;
+    for (int i = 0; i < n; ++i) {
+        ConsDef c;
+        c.type = ib[3*i]; c.b1 = ib[3*i+1]; c.b2 = ib[3*i+2];
+        for (int k = 0; k < 3; ++k) { c.s1[k] = dd[7*i+k]; c.s2[k] = dd[7*i+3+k]; }
+        c.len = dd[7*i+6];
+        if ((c.type != CONS_BALL && c.type != CONS_ROD) || c.b1 < 0 || c.b2 < 0 || c.b1 > M.nb || c.b2 > M.nb ||
+            c.b1 == c.b2 || (c.type == CONS_ROD && !(c.len > 0)) || M.euler)
+            return -1;
+        cs.push_back(c);
+    }
+    h->cons = cs;
+    return 0;
+}
+// Loop forward dynamics (gravity + Fapp, (nb+1)*6 nullable): udot [nu] and
+// the multipliers lam [m] (forces in G).  Returns 1 when G M^-1 G^T is
+// singular.
+int oracle_loop_forward(void* p, const double* q, const double* u, const double* Fapp, double* udot, double* lam) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    realizeVelocity(tr, q, u, (double*)nullptr);
+    std::vector<SV<double>> F(M.nb+1, svzero<double>());
+    addGravity(tr, F.data());
+    if (Fapp) for (int b=1;b<=M.nb;++b) F[b] = F[b] + sv(mk3(Fapp[6*b], Fapp[6*b+1], Fapp[6*b+2]),
+                                                        mk3(Fapp[6*b+3], Fapp[6*b+4], Fapp[6*b+5]));
+    realizeABI(tr);
+    realizeABVelocity(tr);
+    return loopForward(tr, h->cons, q, u, F.data(), udot, lam) ? 0 : 1;
+}
+// Constraint errors at (q, u): perr [m], verr [m] (either nullable), and the
+// acceleration errors aerr [m] (nullable) of the body accelerations of udot.
+int oracle_constraint_errors(void* p, const double* q, const double* u, const double* udot, double* perr,
+                             double* verr, double* aerr) {
+    Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
+    realizePosition(tr, q);
+    realizeVelocity(tr, q, u, (double*)nullptr);
+    if (perr) consPerr(tr, h->cons, perr);
+    std::vector<SV<double>> V(M.nb+1);
+    for (int b=0;b<=M.nb;++b) V[b] = tr.B[b].V_GB;
+    if (verr) consRowsOf(tr, h->cons, V.data(), verr);
+    if (aerr) {
+        std::vector<SV<double>> A(M.nb+1);
+        bodyAccelerationFromUDot(tr, udot, A.data());
+        consAerr(tr, h->cons, A.data(), aerr);
+    }
+    return 0;
+}
 
 int oracle_model_sizes(void* p, int* out /*nb nq nu nquat nlevels*/) {
     const Model& M = H(p)->m;
@@ -237,6 +293,7 @@ int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T
                      double accuracy, double fixed_h, long max_attempts, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     if (max_attempts > 0) cfg.max_attempts = max_attempts;
@@ -269,6 +326,7 @@ int oracle_integrate_trace(void* p, double* y, const double* Fapp, double t0, do
                            double fixed_h, double init_h, double* rec, int max_rec, int* n_rec, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     std::vector<RkmAttempt> tr;
     rk.trace = &tr;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
@@ -292,6 +350,7 @@ int oracle_integrate_motion(void* p, double* y, const double* Fapp, const double
                             double accuracy, double fixed_h, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     rk.motion = motion;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
@@ -307,6 +366,7 @@ int oracle_integrate_stops(void* p, double* y, const double* Fapp, double t0, co
                            double accuracy, double fixed_h, double* yout, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrateStops(y, t0, stops, n, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, yout);
@@ -321,6 +381,7 @@ int oracle_integrate_reports(void* p, double* y, const double* Fapp, double t0, 
                              int n, double accuracy, double fixed_h, double* rout, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrateStops(y, t0, &T, 1, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, nullptr,
@@ -335,6 +396,7 @@ int oracle_integrate_known(void* p, double* y, const double* Fapp, const int* pr
                            double t0, double T, double accuracy, double fixed_h, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
+    if (!h->cons.empty()) rk.cons = &h->cons;
     rk.ws.presc = presc; rk.ws.udotKnown = udot_known;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;

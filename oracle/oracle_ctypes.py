@@ -50,7 +50,8 @@ def _open(path):
                  "oracle_multiply_by_m", "oracle_jacobian", "oracle_jacobian_transpose",
                  "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
                  "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
-                 "oracle_integrate_trace", "oracle_abi_shift", "oracle_integrate_motion"]:
+                 "oracle_integrate_trace", "oracle_abi_shift", "oracle_integrate_motion",
+                 "oracle_set_constraints", "oracle_loop_forward", "oracle_constraint_errors"]:
         getattr(L, name).restype = ctypes.c_int
     return L
 
@@ -116,6 +117,7 @@ class OracleModel:
         s = (ctypes.c_int * 5)()
         self._L.oracle_model_sizes(self.h, s)
         self.nb, self.nq, self.nu, self.nquat, self.nlevels = list(s)
+        self._cons_rows = 0
 
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "_L", None) is not None:
@@ -142,6 +144,39 @@ class OracleModel:
         ud = np.zeros(self.nu); qdd = np.zeros(self.nq); A = np.zeros((self.nb + 1, 6))
         rc = self._L.oracle_forward(self.h, _p(q), _p(u), _p(fmob), _p(F), _p(ud), _p(qdd), _p(A))
         return ud, qdd, A, rc
+
+    def set_constraints(self, cons):
+        """Holonomic constraints: a list of (type, body1, body2, station1,
+        station2, length) with type 0 Constraint::Ball, 1 Constraint::Rod
+        (bodies 0 = Ground, stations in the body frames).  Later
+        integrations and loop_forward obey them.  Returns the oracle's code
+        (-1: invalid)."""
+        n = len(cons)
+        ib = np.zeros(3 * max(n, 1), np.int32); dd = np.zeros(7 * max(n, 1))
+        for i, (ty, b1, b2, s1, s2, ln) in enumerate(cons):
+            ib[3*i:3*i+3] = (ty, b1, b2)
+            dd[7*i:7*i+3] = s1; dd[7*i+3:7*i+6] = s2; dd[7*i+6] = ln
+        self._cons_rows = sum(3 if c[0] == 0 else 1 for c in cons)
+        return self._L.oracle_set_constraints(self.h, n, _pi(ib), _p(dd))
+
+    def loop_forward(self, q, u, Fapp=None):
+        """Constrained forward dynamics: (udot, multipliers, rc)."""
+        ud = np.zeros(self.nu); lam = np.zeros(max(self._cons_rows, 1))
+        Fa = None if Fapp is None else np.ascontiguousarray(Fapp, np.float64)
+        rc = self._L.oracle_loop_forward(self.h, _p(np.ascontiguousarray(q, np.float64)),
+                                         _p(np.ascontiguousarray(u, np.float64)), _p(Fa), _p(ud), _p(lam))
+        return ud, lam[:self._cons_rows], rc
+
+    def constraint_errors(self, q, u, udot=None):
+        """(perr, verr, aerr) at (q, u) (aerr None without udot)."""
+        m = max(self._cons_rows, 1)
+        pe, ve, ae = np.zeros(m), np.zeros(m), np.zeros(m)
+        ud = None if udot is None else np.ascontiguousarray(udot, np.float64)
+        self._L.oracle_constraint_errors(self.h, _p(np.ascontiguousarray(q, np.float64)),
+                                         _p(np.ascontiguousarray(u, np.float64)), _p(ud), _p(pe), _p(ve),
+                                         _p(ae) if ud is not None else None)
+        k = self._cons_rows
+        return pe[:k], ve[:k], (ae[:k] if ud is not None else None)
 
     def forward_prescribed(self, q, u, fmob, F, presc, udot_known):
         """Forward dynamics with known udot on the bodies where presc[b] != 0

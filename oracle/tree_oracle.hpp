@@ -1022,6 +1022,162 @@ template <class T> void bodyAccelerationFromUDot(Tree<T>& tr, const T* udot, SV<
     }
 }
 
+// ---------------------------------------------------------------------------
+// Holonomic constraints, with Ground as the ancestor frame A in which the
+// reference states them (its multipliers are forces in A; with A = Ground
+// they are forces in G):
+//   Constraint::Ball (ConstraintImpl.h:1835-2000): station P on body 1,
+//     station S on body 2; perr = p_S - p_P; verr = v_S - v_C and aerr =
+//     a_S - a_C with C the material point of body 1 at S (lever p_S - p_B1);
+//     +lambda at S on body 2, -lambda at C on body 1;
+//   Constraint::Rod (Constraint_RodImpl.h:193-300): stations Sf on body 1, Sb
+//     on body 2, length d; perr = |p_Sb - p_Sf| - d; verr = (v_Sb - v_Sf).Cz;
+//     aerr = (a_Sb - a_Sf).Cz + (v_Sb - v_Sf).Czd, Cz = (p_Sb - p_Sf)/r,
+//     Czd = (pd - Cz (pd.Cz))/r; lambda Cz at Sb on body 2, -lambda Cz at Sf
+//     on body 1.
+// Station velocity v = V.lin + V.ang x r; station acceleration a = A.lin +
+// A.ang x r + w x (w x r).
+// ---------------------------------------------------------------------------
+enum ConsType { CONS_BALL = 0, CONS_ROD = 1 };
+struct ConsDef { int type, b1, b2; double s1[3], s2[3], len; };
+inline int consRows(const ConsDef& c) { return c.type == CONS_BALL ? 3 : 1; }
+inline int consRowsTotal(const std::vector<ConsDef>& cs) {
+    int m = 0;
+    for (const ConsDef& c : cs) m += consRows(c);
+    return m;
+}
+
+template <class T> struct ConsGeom { V3<T> p1, p2, r1, r2, Cz; T r; };
+template <class T> ConsGeom<T> consGeom(const Tree<T>& tr, const ConsDef& c) {
+    const Xform<T>& X1 = tr.B[c.b1].X_GB;
+    const Xform<T>& X2 = tr.B[c.b2].X_GB;
+    ConsGeom<T> g;
+    const V3<T> l1 = X1.R * mk3<T>(T(c.s1[0]), T(c.s1[1]), T(c.s1[2]));
+    const V3<T> l2 = X2.R * mk3<T>(T(c.s2[0]), T(c.s2[1]), T(c.s2[2]));
+    g.p1 = X1.p + l1; g.p2 = X2.p + l2;
+    g.r2 = l2;
+    g.r1 = c.type == CONS_BALL ? g.p2 - X1.p : l1;
+    g.r = T(0); g.Cz = zero3<T>();
+    if (c.type == CONS_ROD) {
+        const V3<T> d = g.p2 - g.p1;
+        g.r = sqrt(dot(d, d));
+        g.Cz = (T(1)/g.r) * d;
+    }
+    return g;
+}
+template <class T> inline V3<T> stationVel(const SV<T>& V, const V3<T>& r) { return V.b + cross(V.a, r); }
+template <class T> inline V3<T> stationAcc(const SV<T>& A, const SV<T>& V, const V3<T>& r) {
+    return (A.b + cross(A.a, r)) + cross(V.a, cross(V.a, r));
+}
+// Position errors (needs realizePosition).
+template <class T> void consPerr(const Tree<T>& tr, const std::vector<ConsDef>& cs, T* perr) {
+    int k = 0;
+    for (const ConsDef& c : cs) {
+        const ConsGeom<T> g = consGeom(tr, c);
+        if (c.type == CONS_BALL) { const V3<T> d = g.p2 - g.p1; perr[k] = d[0]; perr[k+1] = d[1]; perr[k+2] = d[2]; k += 3; }
+        else perr[k++] = g.r - T(c.len);
+    }
+}
+// Rows of the constraint Jacobian applied to per-body spatial velocities V
+// (no centripetal terms): the velocity errors when V = V_GB, the rows of
+// P v when V = J v.
+template <class T> void consRowsOf(const Tree<T>& tr, const std::vector<ConsDef>& cs, const SV<T>* V, T* out) {
+    int k = 0;
+    for (const ConsDef& c : cs) {
+        const ConsGeom<T> g = consGeom(tr, c);
+        const V3<T> d = stationVel(V[c.b2], g.r2) - stationVel(V[c.b1], g.r1);
+        if (c.type == CONS_BALL) { out[k] = d[0]; out[k+1] = d[1]; out[k+2] = d[2]; k += 3; }
+        else out[k++] = dot(d, g.Cz);
+    }
+}
+// Acceleration errors from body accelerations A (with their velocity terms)
+// at the velocities of the tree.
+template <class T> void consAerr(const Tree<T>& tr, const std::vector<ConsDef>& cs, const SV<T>* A, T* aerr) {
+    int k = 0;
+    for (const ConsDef& c : cs) {
+        const ConsGeom<T> g = consGeom(tr, c);
+        const SV<T>& V1 = tr.B[c.b1].V_GB; const SV<T>& V2 = tr.B[c.b2].V_GB;
+        const V3<T> dd = stationAcc(A[c.b2], V2, g.r2) - stationAcc(A[c.b1], V1, g.r1);
+        if (c.type == CONS_BALL) { aerr[k] = dd[0]; aerr[k+1] = dd[1]; aerr[k+2] = dd[2]; k += 3; }
+        else {
+            const V3<T> pd = stationVel(V2, g.r2) - stationVel(V1, g.r1);
+            const V3<T> Czd = (T(1)/g.r) * (pd - dot(pd, g.Cz) * g.Cz);
+            aerr[k++] = dot(dd, g.Cz) + dot(pd, Czd);
+        }
+    }
+}
+// Body forces (in G, about the body origins) of multipliers lambda, added
+// to F with sign sgn.
+template <class T> void consForces(const Tree<T>& tr, const std::vector<ConsDef>& cs, const T* lam, T sgn,
+                                   SV<T>* F) {
+    int k = 0;
+    for (const ConsDef& c : cs) {
+        const ConsGeom<T> g = consGeom(tr, c);
+        V3<T> f;
+        if (c.type == CONS_BALL) { f = mk3<T>(lam[k], lam[k+1], lam[k+2]); k += 3; }
+        else f = lam[k++] * g.Cz;
+        f = sgn * f;
+        F[c.b2] = F[c.b2] + sv(cross(g.r2, f), f);
+        F[c.b1] = F[c.b1] - sv(cross(g.r1, f), f);
+    }
+}
+// Gaussian elimination with partial pivoting: solves A x = b in place (A n x
+// n row-major, b -> x).  False when a pivot is zero.
+template <class T> bool solveDense(int n, std::vector<T>& A, T* b) {
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        for (int r = k + 1; r < n; ++r) if (fabs(double(A[r*n+k])) > fabs(double(A[piv*n+k]))) piv = r;
+        if (A[piv*n+k] == T(0)) return false;
+        if (piv != k) { for (int j = 0; j < n; ++j) std::swap(A[k*n+j], A[piv*n+j]); std::swap(b[k], b[piv]); }
+        for (int r = k + 1; r < n; ++r) {
+            const T f = A[r*n+k] / A[k*n+k];
+            for (int j = k; j < n; ++j) A[r*n+j] = A[r*n+j] - f*A[k*n+j];
+            b[r] = b[r] - f*b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        T s = b[k];
+        for (int j = k + 1; j < n; ++j) s = s - A[k*n+j]*b[j];
+        b[k] = s / A[k*n+k];
+    }
+    return true;
+}
+// Loop forward dynamics (calcLoopForwardDynamicsOperator,
+// SimbodyMatterSubsystemRep.cpp:5076-5152): udot of the tree under F, the
+// acceleration errors aerr it leaves, W = G M^-1 G^T one column per
+// multiplier (M^-1 of the generalized force of a unit multiplier; the rows
+// of the resulting body accelerations, which carry no velocity terms), W
+// lambda = aerr, and udot again under F minus the constraint forces.  Needs
+// realizeABI and realizeABVelocity.  lam (nullable) receives the
+// multipliers.  False when W is singular.
+template <class T> bool loopForward(Tree<T>& tr, const std::vector<ConsDef>& cs, const T* q, const T* u,
+                                    const SV<T>* F, T* udot, T* lam) {
+    const Model& M = *tr.M;
+    const int m = consRowsTotal(cs), nu = M.nu;
+    forwardDynamics(tr, q, u, (const T*)nullptr, F, udot, (T*)nullptr);
+    std::vector<SV<T>> A(M.nb + 1);
+    for (int b = 0; b <= M.nb; ++b) A[b] = tr.B[b].A_GB;
+    std::vector<T> aerr(m), W((size_t)m*m), e(m), f(nu), x(nu), col(m);
+    consAerr(tr, cs, A.data(), aerr.data());
+    std::vector<SV<T>> Fk(M.nb + 1);
+    for (int k = 0; k < m; ++k) {
+        for (int i = 0; i < m; ++i) e[i] = T(i == k ? 1 : 0);
+        for (auto& v : Fk) v = svzero<T>();
+        consForces(tr, cs, e.data(), T(1), Fk.data());
+        systemJacobianTranspose(tr, Fk.data(), f.data());
+        multiplyByMInv(tr, f.data(), x.data());
+        for (int b = 0; b <= M.nb; ++b) A[b] = tr.B[b].A_GB;
+        consRowsOf(tr, cs, A.data(), col.data());
+        for (int i = 0; i < m; ++i) W[(size_t)i*m + k] = col[i];
+    }
+    if (!solveDense(m, W, aerr.data())) return false;
+    std::vector<SV<T>> Fc(F, F + M.nb + 1);
+    consForces(tr, cs, aerr.data(), T(-1), Fc.data());
+    forwardDynamics(tr, q, u, (const T*)nullptr, Fc.data(), udot, (T*)nullptr);
+    if (lam) for (int i = 0; i < m; ++i) lam[i] = aerr[i];
+    return true;
+}
+
 // Unit inertia of a unit point mass at p, taken about the origin:
 // |p|^2 I - p p^T (UnitInertia::pointMassAt, MassProperties.h:866-867).
 template <class T> inline M33<T> pointMassUnitInertia(const V3<T>& p) {
@@ -1091,6 +1247,10 @@ template <class T> void reactionForces(const Tree<T>& tr, SV<T>* FM) {
 template <class T> struct Workspace {
     std::vector<SV<T>> F;
     std::vector<T> tmp;
+    // holonomic constraints (null: none) and the multipliers of the last
+    // evaluation
+    const std::vector<ConsDef>* cons = nullptr;
+    std::vector<T> lam;
     // known udot (locks, constant prescribed udot): presc [nb+1], udotKnown
     // [nu]; null when none
     const int* presc = nullptr;
@@ -1112,6 +1272,9 @@ template <class T> bool derivs(Tree<T>& tr, Workspace<T>& ws, const T* y, const 
         for (int i=0;i<M.nu;++i) ud[i] = T(ws.udotKnown[i]);
         ws.tmp.resize(M.nu);
         forwardDynamicsPrescribed(tr, ws.presc, (const T*)nullptr, ws.F.data(), ud, ws.tmp.data());
+    } else if (ws.cons && !ws.cons->empty()) {
+        ws.lam.resize(consRowsTotal(*ws.cons));
+        if (!loopForward(tr, *ws.cons, q, u, ws.F.data(), ydot + M.nq, ws.lam.data())) return false;
     } else {
         forwardDynamics(tr, q, u, (const T*)nullptr, ws.F.data(), ydot + M.nq, (T*)nullptr);
     }
@@ -1213,6 +1376,232 @@ template <class T> struct Rkm {
         const bool ok = derivs(tr, ws, yp.data(), Fapp, ydot);
         ws.presc = nullptr; ws.udotKnown = nullptr;
         return ok;
+    }
+
+    // ---- holonomic constraints (null: none) ----------------------------
+    // Projection with the reference's default weights (every constraint
+    // error and u weight 1, StateImpl.h:508-566): projectQ
+    // (SimbodyMatterSubsystemRep.cpp:4096-4440) and projectU (:4666-4934).
+    const std::vector<ConsDef>* cons = nullptr;
+    std::vector<T> cP;
+    // P (m x nu, row-major) at the current position kinematics: column j
+    // holds the constraint rows of J e_j.
+    void consP() {
+        const int m = consRowsTotal(*cons), nu = M.nu;
+        cP.assign((size_t)m*nu, T(0));
+        std::vector<T> e(nu, T(0)), col(m);
+        std::vector<SV<T>> Jv(M.nb + 1);
+        for (int j = 0; j < nu; ++j) {
+            e[j] = T(1);
+            systemJacobian(tr, e.data(), Jv.data());
+            e[j] = T(0);
+            consRowsOf(tr, *cons, Jv.data(), col.data());
+            for (int i = 0; i < m; ++i) cP[(size_t)i*nu + j] = col[i];
+        }
+    }
+    // du = NInv(q) dq and dq = N(q) du: the unnormalized quaternion forms
+    // (Rotation.h:712-748) on Ball and Free rotations, identity elsewhere
+    // (every other mobilizer here has qdot = u).
+    void mulNInv(const T* q, const T* dq, T* du) const {
+        for (int b = 1; b <= M.nb; ++b) {
+            const int qi = M.qIndex[b], ui = M.uIndex[b];
+            if (M.isQuat(b)) {
+                const V3<T> w = quatDotToW(q + qi, dq + qi);
+                du[ui] = w[0]; du[ui+1] = w[1]; du[ui+2] = w[2];
+                if (M.type[b] == FREE) for (int k = 0; k < 3; ++k) du[ui+3+k] = dq[qi+4+k];
+            } else for (int k = 0; k < M.dof[b]; ++k) du[ui+k] = dq[qi+k];
+        }
+    }
+    void mulN(const T* q, const T* du, T* dq) const {
+        for (int b = 1; b <= M.nb; ++b) {
+            const int qi = M.qIndex[b], ui = M.uIndex[b];
+            if (M.isQuat(b)) {
+                quatDot(q + qi, mk3<T>(du[ui], du[ui+1], du[ui+2]), dq + qi);
+                if (M.type[b] == FREE) for (int k = 0; k < 3; ++k) dq[qi+4+k] = du[ui+3+k];
+            } else for (int k = 0; k < M.dof[b]; ++k) dq[qi+k] = du[ui+k];
+        }
+    }
+    T rmsOf(const std::vector<T>& v) const {
+        if (v.empty()) return T(0);
+        T s = T(0);
+        for (const T& x : v) s = s + x*x;
+        return sqrt(s/T(v.size()));
+    }
+    // The least-norm Newton update in q (Pq = P NInv, dq = N NInv Pq^T
+    // (Pq Pq^T)^-1 r) for the residual r at the current kinematics.
+    bool qUpdate(const T* q, const std::vector<T>& r, std::vector<T>& dq) {
+        const int m = (int)r.size(), nq = M.nq, nu = M.nu;
+        consP();
+        std::vector<T> Pq((size_t)m*nq, T(0)), rowu(nu), rowq(nq);
+        for (int i = 0; i < m; ++i) {
+            // row i of P NInv: (NInv^T P_i^T)^T, NInv^T from the transposed forms
+            for (int j = 0; j < nu; ++j) rowu[j] = cP[(size_t)i*nu + j];
+            for (int b = 1; b <= M.nb; ++b) {
+                const int qi = M.qIndex[b], ui = M.uIndex[b];
+                if (M.isQuat(b)) {
+                    const T* qq = q + qi;
+                    const T e0 = T(2)*qq[0], e1 = T(2)*qq[1], e2 = T(2)*qq[2], e3 = T(2)*qq[3];
+                    const T a = rowu[ui], bb = rowu[ui+1], c = rowu[ui+2];
+                    rowq[qi]   = -e1*a - e2*bb - e3*c;
+                    rowq[qi+1] =  e0*a + e3*bb - e2*c;
+                    rowq[qi+2] = -e3*a + e0*bb + e1*c;
+                    rowq[qi+3] =  e2*a - e1*bb + e0*c;
+                    if (M.type[b] == FREE) for (int k = 0; k < 3; ++k) rowq[qi+4+k] = rowu[ui+3+k];
+                } else for (int k = 0; k < M.dof[b]; ++k) rowq[qi+k] = rowu[ui+k];
+            }
+            for (int j = 0; j < nq; ++j) Pq[(size_t)i*nq + j] = rowq[j];
+        }
+        std::vector<T> A((size_t)m*m), y(r);
+        for (int i = 0; i < m; ++i)
+            for (int k = 0; k < m; ++k) {
+                T sacc = T(0);
+                for (int j = 0; j < nq; ++j) sacc = sacc + Pq[(size_t)i*nq + j]*Pq[(size_t)k*nq + j];
+                A[(size_t)i*m + k] = sacc;
+            }
+        if (!solveDense(m, A, y.data())) return false;
+        std::vector<T> dqw(nq, T(0)), du(nu);
+        for (int j = 0; j < nq; ++j) {
+            T sacc = T(0);
+            for (int i = 0; i < m; ++i) sacc = sacc + Pq[(size_t)i*nq + j]*y[i];
+            dqw[j] = sacc;
+        }
+        mulNInv(q, dqw.data(), du.data());
+        dq.assign(nq, T(0));
+        mulN(q, du.data(), dq.data());
+        return true;
+    }
+    // projectQ with the integrator's options: LocalOnly (stop when the
+    // error grows from the second iteration on) unless initializing,
+    // ForceProjection and ForceFullNewton when initializing.  q and its
+    // error estimate qerr (nullable) are updated; false on failure.  The
+    // error-estimate projection uses Pq at the final q (the reference solves
+    // with the factorization of the last iteration's starting q; the two
+    // differ by the last Newton step, below the constraint tolerance).
+    bool projectQ(T* q, T* qerr, double consTol, double projLimit, bool init, bool* changed) {
+        const int m = consRowsTotal(*cons), nq = M.nq;
+        const double Significant = std::pow(2.220446049250313e-16, 0.875);
+        *changed = false;
+        std::vector<T> perr(m);
+        realizePosition(tr, q);
+        consPerr(tr, *cons, perr.data());
+        const double pn0 = double(rmsOf(perr)), qn0 = double(quatErrRMS(q));
+        if (std::max(pn0, qn0) > projLimit) return false;
+        if (pn0 == 0 || (pn0 <= consTol && !init)) {
+            if (qn0 > consTol || init) {
+                if (M.nquat) { normalizeQuats(q, qerr); *changed = true; }
+                if (double(quatErrRMS(q)) > consTol) return false;
+            }
+            return true;
+        }
+        const double target = std::max(0.1*consTol, Significant);
+        std::vector<T> save(q, q + nq), dq;
+        double pn = pn0, prev = pn0;
+        int its = 0;
+        do {
+            if (!qUpdate(q, perr, dq)) return false;
+            for (int j = 0; j < nq; ++j) q[j] = q[j] - dq[j];
+            *changed = true; ++its;
+            realizePosition(tr, q);
+            consPerr(tr, *cons, perr.data());
+            pn = double(rmsOf(perr));
+            if (!init && its >= 2 && pn > prev) {
+                for (int j = 0; j < nq; ++j) q[j] = q[j] + dq[j];
+                realizePosition(tr, q);
+                consPerr(tr, *cons, perr.data());
+                pn = double(rmsOf(perr));
+                break;
+            }
+            prev = pn;
+        } while (pn > target && its < 20);
+        if (pn > consTol) {
+            if (pn >= pn0) { for (int j = 0; j < nq; ++j) q[j] = save[j]; realizePosition(tr, q); }
+            return false;
+        }
+        if (qerr) {
+            // remove the error estimate's component off the constraint manifold
+            std::vector<T> du(M.nu), r(m);
+            mulNInv(q, qerr, du.data());
+            std::vector<SV<T>> Jv(M.nb + 1);
+            systemJacobian(tr, du.data(), Jv.data());
+            consRowsOf(tr, *cons, Jv.data(), r.data());
+            if (!qUpdate(q, r, dq)) return false;
+            for (int j = 0; j < nq; ++j) qerr[j] = qerr[j] - dq[j];
+        }
+        if (M.nquat) {
+            normalizeQuats(q, qerr);
+            if (double(quatErrRMS(q)) > consTol) return false;
+        }
+        return true;
+    }
+    // projectU with the integrator's options (as projectQ): u and its error
+    // estimate uerr (nullable) updated at the kinematics of q; relative u
+    // scaling max(|u_i|, 1) from the u on entry; one factorization of
+    // P S^2 P^T for every iteration.
+    bool projectU(const T* q, T* u, T* uerr, double consTol, double projLimit, bool init, bool* changed) {
+        const int m = consRowsTotal(*cons), nu = M.nu;
+        const double Significant = std::pow(2.220446049250313e-16, 0.875);
+        *changed = false;
+        std::vector<T> verr(m);
+        std::vector<SV<T>> V(M.nb + 1);
+        auto velErr = [&]() {
+            realizeVelocity(tr, q, u, (T*)nullptr);
+            for (int b = 0; b <= M.nb; ++b) V[b] = tr.B[b].V_GB;
+            consRowsOf(tr, *cons, V.data(), verr.data());
+            return double(rmsOf(verr));
+        };
+        realizePosition(tr, q);
+        const double vn0 = velErr();
+        if (vn0 > projLimit) return false;
+        if (vn0 == 0 || (vn0 <= consTol && !init)) return true;
+        const double target = std::max(0.1*consTol, Significant);
+        std::vector<T> sc(nu), save(u, u + nu), du(nu), y(m);
+        for (int i = 0; i < nu; ++i) { const T a = T(fabs(double(u[i]))); sc[i] = a > T(1) ? a : T(1); }
+        consP();
+        std::vector<T> A0((size_t)m*m);
+        for (int i = 0; i < m; ++i)
+            for (int k = 0; k < m; ++k) {
+                T sacc = T(0);
+                for (int j = 0; j < nu; ++j) sacc = sacc + cP[(size_t)i*nu + j]*(sc[j]*sc[j])*cP[(size_t)k*nu + j];
+                A0[(size_t)i*m + k] = sacc;
+            }
+        auto solveU = [&](const std::vector<T>& r) {
+            std::vector<T> A(A0);
+            y = r;
+            if (!solveDense(m, A, y.data())) return false;
+            for (int j = 0; j < nu; ++j) {
+                T sacc = T(0);
+                for (int i = 0; i < m; ++i) sacc = sacc + cP[(size_t)i*nu + j]*y[i];
+                du[j] = (sc[j]*sc[j])*sacc;
+            }
+            return true;
+        };
+        double vn = vn0, prev = vn0;
+        int its = 0;
+        do {
+            if (!solveU(verr)) return false;
+            for (int j = 0; j < nu; ++j) u[j] = u[j] - du[j];
+            *changed = true; ++its;
+            vn = velErr();
+            if (!init && its >= 2 && vn > prev) {
+                for (int j = 0; j < nu; ++j) u[j] = u[j] + du[j];
+                vn = velErr();
+                break;
+            }
+            prev = vn;
+        } while (vn > target && its < 7);
+        if (vn > consTol) {
+            if (vn >= vn0) for (int j = 0; j < nu; ++j) u[j] = save[j];
+            return false;
+        }
+        if (uerr) {
+            std::vector<SV<T>> Jv(M.nb + 1);
+            std::vector<T> r(m);
+            systemJacobian(tr, uerr, Jv.data());
+            consRowsOf(tr, *cons, Jv.data(), r.data());
+            if (!solveU(r)) return false;
+            for (int j = 0; j < nu; ++j) uerr[j] = uerr[j] - du[j];
+        }
+        return true;
     }
 
     // Quaternion normalization of q (and, if given, removal of the radial
@@ -1325,7 +1714,20 @@ template <class T> struct Rkm {
         // initialize(): prescribed q and u at t0, forced projection of the
         // quaternions, then derivatives.
         if (motion) prescribe(t0, y);
-        normalizeQuats(y, (T*)nullptr);
+        const bool hasCons = cons && !cons->empty();
+        ws.cons = hasCons ? cons : nullptr;
+        if (hasCons) {
+            // initialize(): projectQ and projectU with ForceProjection and
+            // ForceFullNewton (Integrator.cpp:376-377)
+            bool ch;
+            if (!projectQ(y, (T*)nullptr, consTol, std::numeric_limits<double>::infinity(), true, &ch) ||
+                !projectU(y, y + M.nq, (T*)nullptr, consTol, std::numeric_limits<double>::infinity(), true, &ch)) {
+                st.status = 5;
+                return st;
+            }
+        } else {
+            normalizeQuats(y, (T*)nullptr);
+        }
         for (int i=0;i<ny;++i) y0[i] = y[i];
         double t = t0;
         double hcur = (hinit != -1) ? hinit : cfg.timescale/10;
@@ -1369,7 +1771,15 @@ template <class T> struct Rkm {
                     if (!(e1 > 16.0*acc)) {   // NaN falls through to projection like the reference's comparison
                         const double qe = double(quatErrRMS(y1.data()));
                         qErr = qe;
-                        if (qe > projectionLimit) converged = false;
+                        if (hasCons) {
+                            // attemptDAEStep (AbstractIntegratorRep.cpp:183-207)
+                            bool chq = false, chu = false;
+                            if (!projectQ(y1.data(), yerr.data(), consTol, projectionLimit, false, &chq) ||
+                                !projectU(y1.data(), y1.data() + M.nq, yerr.data() + M.nq, consTol, projectionLimit,
+                                          false, &chu))
+                                converged = false;
+                            if (chq) { ++st.projections; projected = true; }
+                        } else if (qe > projectionLimit) converged = false;
                         else if (qe > consTol) { normalizeQuats(y1.data(), yerr.data()); ++st.projections; projected = true; }
                     }
                 }
