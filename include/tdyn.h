@@ -156,7 +156,9 @@ int tdyn_realize_kinematics(tdyn_batch* b, const double* q, const double* u, dou
 
 /* realize(state, Stage::Acceleration) of a MultibodySystem with
  * Force::Gravity and constant applied body forces fapp (NULL: none)
- * (SimbodyMatterSubsystemRep.cpp:1527-1555): udot, qdot (NULL ok). */
+ * (SimbodyMatterSubsystemRep.cpp:1527-1555): udot, qdot (NULL ok).  With
+ * constraints set (tdyn_batch_set_constraints), the loop forward dynamics,
+ * as realizeLoopForwardDynamics (:5160-5200) makes it. */
 int tdyn_realize_acceleration(tdyn_batch* b, const double* q, const double* u, const double* fapp,
                               double* udot, double* qdot, hipStream_t s);
 
@@ -213,6 +215,51 @@ int tdyn_batch_set_known_udot(tdyn_batch* b, const int32_t* known, const double*
  * udot.  The reference's counterpart: adding Motion objects to mobilized
  * bodies. */
 int tdyn_batch_set_motion(tdyn_batch* b, const double* motion);
+
+/* Holonomic constraints of a tree: Constraint::Ball (station1 on body1 and
+ * station2 on body2 held together; ConstraintImpl.h:1835-2000) and
+ * Constraint::Rod (the two stations held at distance `length`;
+ * Constraint_RodImpl.h:193-300), stations in the body frames, body 0 =
+ * Ground.  The multipliers are in Ground (the reference's ancestor frame A
+ * is Ground for these) and, as the reference's, are subtracted from the
+ * applied forces: a Ball's lambda is minus the force on body 2 at the
+ * station; a Rod's, minus the tension along station1 -> station2. */
+#define TDYN_CONS_BALL 0
+#define TDYN_CONS_ROD  1
+#define TDYN_MAX_CONSTRAINT_ROWS 12   /* constraint equations per tree (Ball 3, Rod 1) */
+typedef struct tdyn_constraint_desc {
+    int32_t type;              /* TDYN_CONS_BALL or TDYN_CONS_ROD */
+    int32_t body1, body2;      /* two different bodies of the tree, 0 = Ground */
+    double station1[3];        /* on body1, in its frame */
+    double station2[3];        /* on body2, in its frame */
+    double length;             /* Rod: the distance (> 0); Ball: unused */
+} tdyn_constraint_desc;
+/* Sets the constraints of tree `tree` of the batch (replacing earlier ones;
+ * n = 0 clears), as adding Constraint::Ball / Constraint::Rod objects to the
+ * matter subsystem does (Constraint.cpp).  Not with Euler-angle Ball or Free
+ * models.  Rollouts with constraints need a batch of one tree (a batch of
+ * different trees takes the single-shot operators only) and run the tree
+ * sweeps in FP64 (evaluator 0 or 1), without known udot, prescribed motion,
+ * reports or tdyn_rollout_noalloc: every derivative is the loop forward dynamics
+ * (calcLoopForwardDynamicsOperator, SimbodyMatterSubsystemRep.cpp:
+ * 5076-5152), the initial state is projected onto the constraints
+ * (initialize(), Integrator.cpp:376-377; status -4 where that fails), and
+ * each step's end state and error estimate are projected as attemptDAEStep
+ * does (AbstractIntegratorRep.cpp:137-208: projectQ, projectU,
+ * SimbodyMatterSubsystemRep.cpp:4096-4440, 4666-4934), with the reference's
+ * default unit weights.  A rollout in progress ends (begin again). */
+int tdyn_batch_set_constraints(tdyn_batch* b, int tree, int n, const tdyn_constraint_desc* c);
+/* realize(Acceleration) of a constrained system (SimbodyMatterSubsystemRep.
+ * cpp:5076-5152 via realizeLoopForwardDynamics): gravity and the applied
+ * body forces F ((nb+1)*6 per instance, nullable), udot (packed u layout)
+ * and the multipliers lambda (TDYN_MAX_CONSTRAINT_ROWS per instance, zero
+ * past the tree's rows; nullable) -- the reference's
+ * getConstraintMultipliers.  Instances of trees without constraints get the
+ * tree's forward dynamics.  tdyn_batch_op_status reports TDYN_E_SINGULAR
+ * where G M^-1 G^T is singular (redundant constraints are not reduced, as
+ * the reference's FactorQTZ does). */
+int tdyn_calc_constrained_acceleration(tdyn_batch* b, const double* q, const double* u, const double* F,
+                                       double* udot, double* lambda, hipStream_t s);
 
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
  * .cpp:431-471). */

@@ -169,6 +169,32 @@ int build_tree_plan(tdyn_batch* b, hipStream_t s) {
     return 0;
 }
 
+int ensure_constraints(tdyn_batch* b) {
+    if (!b->cons_dirty) return TDYN_OK;
+    std::vector<int4> ct, ca;
+    std::vector<double> cp;
+    for (size_t t = 0; t < b->h_cons.size(); ++t) {
+        int rows = 0;
+        for (const tdyn_constraint_desc& c : b->h_cons[t]) {
+            ca.push_back(make_int4(c.type, c.body1, c.body2, 0));
+            for (int k = 0; k < 3; ++k) cp.push_back(c.station1[k]);
+            for (int k = 0; k < 3; ++k) cp.push_back(c.station2[k]);
+            cp.push_back(c.length);
+            rows += c.type == TDYN_CONS_BALL ? 3 : 1;
+        }
+        ct.push_back(make_int4((int)ca.size() - (int)b->h_cons[t].size(), (int)b->h_cons[t].size(), rows, 0));
+    }
+    if (ca.empty()) { ca.push_back(make_int4(0, 0, 0, 0)); cp.assign(7, 0.0); }
+    hipError_t e;
+    if ((e = b->cons_t.upload(ct)) != hipSuccess || (e = b->cons_a.upload(ca)) != hipSuccess ||
+        (e = b->cons_p.upload(cp)) != hipSuccess)
+        return hipfail(e, "constraint tables upload");
+    const size_t ws = (size_t)tdyn::CWS_N * b->n_inst;
+    if (b->cons_ws.n != ws && (e = b->cons_ws.alloc(ws)) != hipSuccess) return hipfail(e, "constraint workspace");
+    b->cons_dirty = false;
+    return TDYN_OK;
+}
+
 int ensure_rollout_buffers(tdyn_batch* b, int precision) {
     const int ny = b->max_nq + b->max_nu;
     const size_t L = b->n_inst;
@@ -449,6 +475,48 @@ int tdyn_batch_set_motion(tdyn_batch* b, const double* motion) {
     return TDYN_OK;
 }
 
+int tdyn_batch_set_constraints(tdyn_batch* b, int tree, int n, const tdyn_constraint_desc* c) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (tree < 0 || tree >= b->n_trees) return fail(TDYN_E_ARG, "tdyn_batch_set_constraints: tree out of range");
+    if (n < 0 || (n > 0 && !c)) return fail(TDYN_E_ARG, "tdyn_batch_set_constraints: n >= 0 descriptions required");
+    if (n > 0 && b->any_euler)
+        return fail(TDYN_E_ARG, "tdyn_batch_set_constraints: not with Euler-angle Ball or Free models");
+    const int nb = b->h_tree_a[tree].x;
+    int rows = 0;
+    for (int i = 0; i < n; ++i) {
+        const tdyn_constraint_desc& d = c[i];
+        const std::string at = "tdyn_batch_set_constraints: constraint " + std::to_string(i) + ": ";
+        if (d.type != TDYN_CONS_BALL && d.type != TDYN_CONS_ROD) return fail(TDYN_E_ARG, at + "unknown type");
+        if (d.body1 < 0 || d.body1 > nb || d.body2 < 0 || d.body2 > nb || d.body1 == d.body2)
+            return fail(TDYN_E_ARG, at + "bodies must be two different bodies of the tree (0 = Ground)");
+        for (int k = 0; k < 3; ++k)
+            if (!std::isfinite(d.station1[k]) || !std::isfinite(d.station2[k]))
+                return fail(TDYN_E_ARG, at + "stations must be finite");
+        if (d.type == TDYN_CONS_ROD && !(d.length > 0 && std::isfinite(d.length)))
+            return fail(TDYN_E_ARG, at + "a rod's length must be positive");
+        rows += d.type == TDYN_CONS_BALL ? 3 : 1;
+    }
+    if (rows > TDYN_MAX_CONSTRAINT_ROWS)
+        return fail(TDYN_E_SIZE, "tdyn_batch_set_constraints: more than TDYN_MAX_CONSTRAINT_ROWS constraint equations");
+    if (b->h_cons.size() != (size_t)b->n_trees) b->h_cons.resize(b->n_trees);
+    b->h_cons[tree].assign(c, c + n);
+    b->any_cons = false;
+    for (const auto& v : b->h_cons) b->any_cons = b->any_cons || !v.empty();
+    b->cons_dirty = true;
+    b->rollout_ready = false;          // a rollout in progress was begun with the old model
+    return ensure_constraints(b);
+}
+
+int tdyn_calc_constrained_acceleration(tdyn_batch* b, const double* q, const double* u, const double* F,
+                                       double* udot, double* lambda, hipStream_t s) {
+    if (!b || !q || !u || !udot) return fail(TDYN_E_ARG, "realize(Acceleration): batch, q, u and udot are required");
+    { const int rc = ensure_constraints(b); if (rc) return rc; }
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.fbody = F; a.out_u = udot; a.out_lam = lambda;
+    HIPCHK(launch_op(OP_FWD_CONS, a, s));
+    return TDYN_OK;
+}
+
 int tdyn_batch_set_tree_lanes(tdyn_batch* b, int lanes) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
     if (!(lanes == 0 || lanes == 1 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64))
@@ -524,6 +592,14 @@ int tdyn_realize_kinematics(tdyn_batch* b, const double* q, const double* u, dou
 int tdyn_realize_acceleration(tdyn_batch* b, const double* q, const double* u, const double* fapp,
                               double* udot, double* qdot, hipStream_t s) {
     if (!b || !q || !u || !udot) return fail(TDYN_E_ARG, "tdyn_realize_acceleration: batch, q, u and udot are required");
+    if (b->any_cons) {
+        // with constraints, realize(Acceleration) is the loop forward dynamics
+        { const int rc = ensure_constraints(b); if (rc) return rc; }
+        OpArgs a = b->op();
+        a.q = q; a.u = u; a.fbody = fapp; a.out_u = udot; a.out_q = qdot;
+        HIPCHK(launch_op(OP_FWD_CONS, a, s));
+        return TDYN_OK;
+    }
     OpArgs a = b->op();
     a.q = q; a.u = u; a.fbody = fapp; a.out_u = udot; a.out_q = qdot;
     HIPCHK(launch_op(OP_FWD_SYS, a, s));
@@ -653,8 +729,18 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     if (b->motion && (b->known_p || b->evaluator == 2 || b->evaluator == 3 || prec != 0))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: prescribed motion runs on the tree sweeps (evaluator 0 or 1), "
                                 "in FP64, without known udot");
+    if (b->any_cons && (b->known_p || b->motion || b->evaluator == 2 || b->evaluator == 3 || prec != 0))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: constraints run on the tree sweeps (evaluator 0 or 1), in FP64, "
+                                "without known udot or prescribed motion");
+    // Rollouts with constraints of a batch of different trees (padded
+    // records) are switched off: their one GPU run faulted, and the cause is
+    // not found yet.  Batches of one tree run them.
+    if (b->any_cons && !b->uniform)
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: constraints in rollouts need a batch of one tree (a batch of "
+                                "different trees takes tdyn_calc_constrained_acceleration only)");
+    { const int rc = ensure_constraints(b); if (rc) return rc; }
     const size_t L = b->n_inst;
-    const bool aos = !b->motion && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) &&
+    const bool aos = !b->motion && !b->any_cons && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) &&
                      !(b->known_p && b->tree_lanes == 1);
     const int grp = aos && b->tree_lanes != 1 ? 1 : 0;
     // From here on the batch changes: a failure (allocation, upload) ends the
@@ -691,6 +777,7 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     RkmArgs a = b->rk();
     HIPCHK(launch_pack_state(a, q, u, 1, nullptr, nullptr, s));
     HIPCHK(launch_rkm_init(a, t0, s));
+    HIPCHK(launch_cons_init(a, s));
     b->rollout_ready = true;
     return TDYN_OK;
 }
@@ -750,6 +837,9 @@ int tdyn_rollout_reports(tdyn_batch* b, double* q, double* u, const double* fapp
                          double* q_reports, double* u_reports, int32_t* stats, hipStream_t s) {
     if (!b || !q || !u || !cfg || n_reports < 0 || (n_reports > 0 && (!report_times || !q_reports || !u_reports)))
         return fail(TDYN_E_ARG, "tdyn_rollout_reports: batch, q, u, cfg and (with reports) times and outputs are required");
+    if (b->any_cons)
+        return fail(TDYN_E_ARG, "tdyn_rollout_reports: not with constraints (interpolated states would need "
+                                "their own projection)");
     std::vector<double> rep(report_times, report_times + n_reports);
     for (int k = 0; k < n_reports; ++k)
         if (!std::isfinite(rep[k]) || rep[k] < t0 || rep[k] > t_end || (k && !(rep[k] > rep[k-1])))

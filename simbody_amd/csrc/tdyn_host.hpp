@@ -107,6 +107,12 @@ struct tdyn_batch {
     std::vector<std::vector<int>> h_tree_lw;     // the width of every level of each tree
     tdyn::host::DBuf<int4> plan; tdyn::host::DBuf<int> plan_idx;   // wavefront plan of the level-parallel evaluator
     tdyn::host::DBuf<int> plan_ctr;              // its work queue's counter
+    // holonomic constraints (tdyn_batch_set_constraints): per tree, and their
+    // device tables and per-lane workspace (uploaded when changed)
+    std::vector<std::vector<tdyn_constraint_desc>> h_cons;
+    tdyn::host::DBuf<int4> cons_t, cons_a;
+    tdyn::host::DBuf<double> cons_p, cons_ws;
+    bool any_cons = false, cons_dirty = false;
     int plan_waves = 0;
     RkmCfg cfg{};
 
@@ -115,6 +121,7 @@ struct tdyn_batch {
         T.body_a = body_a.p; T.body_b = body_b.p; T.child_idx = child_idx.p; T.prm = prm.p;
         T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p; T.bfs = bfs.p; T.lvl = lvl.p;
         T.bfs_b = bfs_b.p; T.prm_bfs = prm_bfs.p;
+        T.cons_t = any_cons ? cons_t.p : nullptr; T.cons_a = cons_a.p; T.cons_p = cons_p.p;
         return T;
     }
     OpArgs op() const {
@@ -123,6 +130,7 @@ struct tdyn_batch {
         a.T = tables(); a.tree_of = uniform ? nullptr : tree_of.p; a.n_inst = n_inst; a.L = n_inst;
         a.scratch = scratch.p; a.rec_stride = max_rec + 24; a.off = {q_off.p, u_off.p, b_off.p}; a.inst = permuted ? inst.p : nullptr;
         a.status = op_status.p;
+        a.cons_ws = any_cons ? cons_ws.p : nullptr;
         return a;
     }
     RkmArgs rk() const {
@@ -133,7 +141,9 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (!roll_known_p && !roll_motion && (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
+        a.chain = (!roll_known_p && !roll_motion && !any_cons &&
+                   (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
+        a.cons_ws = any_cons ? cons_ws.p : nullptr;
         a.presc = roll_known_p; a.udot_known = roll_known_u;
         if (roll_motion) { a.motion = roll_motion; a.presc = motion_flags.p; a.udot_known = motion_udot.p; }
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
@@ -157,5 +167,8 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision);
 // per-lane time, step, status and statistics, and for precision 1 the
 // mixed path's hand-over rows) for its current sizes.
 int ensure_rollout_buffers(tdyn_batch* b, int precision);
+// Uploads the constraint tables and allocates the per-lane workspace when
+// the constraints changed.
+int ensure_constraints(tdyn_batch* b);
 } // namespace host
 } // namespace tdyn

@@ -68,6 +68,15 @@ enum {
 };
 __host__ __device__ inline int rec_size(int dof) { return REC_DIEPS + 13 * dof; }
 
+// Holonomic constraints: rows of one tree at most (the public limit
+// TDYN_MAX_CONSTRAINT_ROWS), their kinds, and the per-lane workspace
+// (doubles, lane-strided): W and a factor of it (CONS_MAXR^2 each), then the
+// residual, solution and multiplier vectors (CONS_MAXR each).
+constexpr int CONS_MAXR = 12;
+enum { CONS_BALL = 0, CONS_ROD = 1 };
+enum { CWS_W = 0, CWS_F = CONS_MAXR * CONS_MAXR, CWS_R = 2 * CONS_MAXR * CONS_MAXR, CWS_Y = CWS_R + CONS_MAXR,
+       CWS_LAM = CWS_Y + CONS_MAXR, CWS_N = CWS_LAM + CONS_MAXR };
+
 struct DevTables {
     const int4* body_a;      // per global body: parent(local), type, qIndex, uIndex
     const int4* body_b;      // per global body: child_begin, child_count, rec_off, -
@@ -84,6 +93,9 @@ struct DevTables {
     const int4* bfs_b;       // per global body, breadth-first order: mobilizer type, qIndex, uIndex, -
     const double* prm_bfs;   // per tree at lvl[1]: breadth-first positions in blocks of 16, constant k of
                              // position j at [(j/16)*16*PRM_STRIDE + k*16 + j%16]
+    const int4* cons_t;      // per tree (nullable: no constraints): first constraint, count, rows, -
+    const int4* cons_a;      // per constraint: kind, body 1, body 2, -
+    const double* cons_p;    // per constraint: station on body 1, station on body 2, rod length (7)
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
@@ -105,7 +117,7 @@ struct RkmCfg {
 
 // Launch entry points (tdyn_kernels.hip).
 enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6,
-              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11, OP_FWD_PRES = 12 };
+              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11, OP_FWD_PRES = 12, OP_FWD_CONS = 13 };
 
 struct OpArgs {
     DevTables T;
@@ -132,6 +144,8 @@ struct OpArgs {
     const int* inst;         // instance of each lane (nullable: lane = instance)
     const int* presc;        // OP_FWD_PRES: per body (nb+1 per instance) nonzero = udot known (udot_in)
     double* out_tau;         // OP_FWD_PRES: tau on the known mobilities, 0 elsewhere (packed u layout)
+    double* cons_ws;         // OP_FWD_CONS: per-lane constraint workspace (CWS_N doubles, lane-strided)
+    double* out_lam;         // OP_FWD_CONS: multipliers, CONS_MAXR per instance (nullable)
 };
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s);
 
@@ -181,6 +195,8 @@ struct RkmArgs {
     const long long* pool;   // tree-server slots (nullable): per lane, where its state vectors start in vec
                              // (11 lane-contiguous vectors of its own nq+nu) and its records in scratch,
                              // in doubles; the lane's tree is tree `lane` of the tables
+    double* cons_ws;         // holonomic constraints (nullable: none in the batch): per-lane workspace
+                             // (CWS_N doubles, lane-strided); the tree sweeps only
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that
@@ -228,6 +244,10 @@ __device__ __forceinline__ double& vec_at(const RkmArgs& a, int v, int lane, int
 enum { VY = 0, VY0 = 1, VF0 = 2, VFA = 3, VFB = 4, VYS = 5, VY1 = 6, VYT = 7, VYE = 8, VC0 = 9, VC1 = 10, NVEC = 11 };
 
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s);
+// The initial projection of the lanes whose trees carry constraints
+// (initialize(): projectQ, projectU with ForceProjection), after
+// launch_rkm_init; status -4 where it fails.
+hipError_t launch_cons_init(const RkmArgs& a, hipStream_t s);
 // Wavefronts of the level-parallel tree evaluator the device keeps resident.
 int tree_group_resident_waves();
 // Chain-evaluator rollouts (f0_only: only the derivative of the current state)

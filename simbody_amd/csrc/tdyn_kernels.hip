@@ -504,7 +504,9 @@ template <int D> __device__ __forceinline__ bool inv_small(const double* A, doub
     return bad;
 }
 
-enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8, IN_PRES = 16 };
+// IN_BF (with IN_MINV): the body forces enter the first pass as applied
+// forces, so the pass computes M^-1 J^T F (the multipliers' accelerations).
+enum { IN_ABI_OUT = 1, IN_FWD = 2, IN_MINV = 4, IN_SYSF = 8, IN_PRES = 16, IN_BF = 32 };
 // Known udot (prescribed motion at the acceleration level, locks): per-body
 // flags m (nb+1 per instance, Ground first), the known udot up and the tau
 // output (packed u layout), all offset to the instance.
@@ -621,6 +623,7 @@ __device__ __forceinline__ bool inward_body(const Lane& c, int b, const int4 ba,
             }
         } else {
             z = svzero<double>();
+            if constexpr ((MODE & IN_BF) != 0) z = z - fbody(b);
         }
         for (int k = 0; k < bb.y; ++k) {
             const int rc = c.rec(cptr(c.T.child_idx)[bb.x + k]);
@@ -897,6 +900,434 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 };
 
 // ---------------------------------------------------------------------------
+// Holonomic constraints, Ground as the ancestor frame (the frame of the
+// multipliers): Constraint::Ball (ConstraintImpl.h:1835-2000: station P on
+// body 1, S on body 2; perr = p_S - p_P; velocity and acceleration rows at
+// S and at C, the material point of body 1 at S; +lambda at S on body 2,
+// -lambda at C on body 1) and Constraint::Rod (Constraint_RodImpl.h:193-300:
+// perr = |p_Sb - p_Sf| - d; rows along Cz = (p_Sb - p_Sf)/r plus the Czd
+// term; lambda Cz at Sb, -lambda Cz at Sf).  Everything is read from the
+// records of the current kinematics; the workspace (CWS_*) holds W, its
+// factor and the vectors.  Per-lane, on the tree sweeps.
+// ---------------------------------------------------------------------------
+// D of a record: padded generic bodies are 3-dof in batches of different trees.
+#define TD_RDOF(ba) (((PAD || mob_generic_only((ba).y)) && (ba).y != FREE) ? 3 : mob_dof((ba).y))
+struct Cons {
+    const int4* ca; const double* cp; int n, m;   // the lane's tree's constraints and rows
+    double* ws; int L;
+    __device__ __forceinline__ double& w(int k) const { return ws[(size_t)k * L]; }
+};
+__device__ __forceinline__ Cons make_cons(const DevTables& T, int tr, double* ws, int lane, int L) {
+    Cons k{nullptr, nullptr, 0, 0, nullptr, L};
+    if (!T.cons_t || !ws) return k;
+    const int4 t = ldi4(T.cons_t, tr);
+    k.ca = T.cons_a + t.x; k.cp = T.cons_p + 7*(size_t)t.x; k.n = t.y; k.m = t.z; k.ws = ws + lane;
+    return k;
+}
+struct CGeom { d3 p1, p2, r1, r2, Cz; double r; };
+__device__ __forceinline__ CGeom cons_geom(const Lane& c, const int4 ca, const double* cp) {
+    dm R1 = ident<double>(), R2 = ident<double>();
+    d3 o1 = zero3<double>(), o2 = zero3<double>();
+    if (ca.y > 0) { const int rb = c.rec(ca.y); R1 = ldm(c, rb + REC_R); o1 = ld3(c, rb + REC_P); }
+    if (ca.z > 0) { const int rb = c.rec(ca.z); R2 = ldm(c, rb + REC_R); o2 = ld3(c, rb + REC_P); }
+    const d3 l1 = mul(R1, V3(cp[0], cp[1], cp[2])), l2 = mul(R2, V3(cp[3], cp[4], cp[5]));
+    CGeom g;
+    g.p1 = o1 + l1; g.p2 = o2 + l2;
+    g.r2 = l2;
+    g.r1 = ca.x == CONS_BALL ? g.p2 - o1 : l1;
+    g.r = 0.0; g.Cz = zero3<double>();
+    if (ca.x == CONS_ROD) {
+        const d3 d = g.p2 - g.p1;
+        g.r = sqrt(dot(d, d));
+        g.Cz = (1.0/g.r) * d;
+    }
+    return g;
+}
+__device__ __forceinline__ dv rec_sv(const Lane& c, int b, int field) {
+    return b > 0 ? ldsv(c, c.rec(b) + field) : svzero<double>();
+}
+__device__ __forceinline__ d3 st_vel(const dv& V, const d3& r) { return V.v + cross(V.w, r); }
+__device__ __forceinline__ d3 st_acc(const dv& A, const dv& V, const d3& r) {
+    return (A.v + cross(A.w, r)) + cross(V.w, cross(V.w, r));
+}
+// Position errors into ws[out..].
+__device__ void cons_perr(const Lane& c, const Cons& K, int out) {
+    int row = 0;
+    for (int i = 0; i < K.n; ++i) {
+        const int4 ca = K.ca[i];
+        const CGeom g = cons_geom(c, ca, K.cp + 7*i);
+        if (ca.x == CONS_BALL) {
+            const d3 d = g.p2 - g.p1;
+            K.w(out + row) = d.x; K.w(out + row + 1) = d.y; K.w(out + row + 2) = d.z; row += 3;
+        } else {
+            K.w(out + row) = g.r - K.cp[7*i + 6]; row += 1;
+        }
+    }
+}
+// The constraint rows of the per-body spatial vectors in record field
+// `field` into ws[out + row*stride]: velocity errors (REC_V), P v (the J v
+// of the Jacobian sweeps), or with accel the acceleration errors of the
+// body accelerations in REC_AGB (their velocity terms and the rod's Czd
+// term from REC_V).
+__device__ void cons_rows(const Lane& c, const Cons& K, int field, bool accel, int out, int stride) {
+    int row = 0;
+    for (int i = 0; i < K.n; ++i) {
+        const int4 ca = K.ca[i];
+        const CGeom g = cons_geom(c, ca, K.cp + 7*i);
+        const dv X1 = rec_sv(c, ca.y, field), X2 = rec_sv(c, ca.z, field);
+        d3 d, pd = zero3<double>();
+        if (accel) {
+            const dv V1 = rec_sv(c, ca.y, REC_V), V2 = rec_sv(c, ca.z, REC_V);
+            d = st_acc(X2, V2, g.r2) - st_acc(X1, V1, g.r1);
+            pd = st_vel(V2, g.r2) - st_vel(V1, g.r1);
+        } else {
+            d = st_vel(X2, g.r2) - st_vel(X1, g.r1);
+        }
+        if (ca.x == CONS_BALL) {
+            K.w(out + row*stride) = d.x; K.w(out + (row + 1)*stride) = d.y; K.w(out + (row + 2)*stride) = d.z;
+            row += 3;
+        } else {
+            double v = dot(d, g.Cz);
+            if (accel) {
+                const d3 Czd = (1.0/g.r) * (pd - dot(pd, g.Cz) * g.Cz);
+                v = v + dot(pd, Czd);
+            }
+            K.w(out + row*stride) = v; row += 1;
+        }
+    }
+}
+// Body forces of the multipliers ws[lam..] with sign sgn, plus the applied
+// forces fapp (nullable), for the inward sweeps.
+struct ConsForce {
+    const Lane* c; Cons K; int lam; double sgn; const double* fapp;
+    __device__ dv operator()(int b) const {
+        dv F = BodyForce{fapp}(b);
+        int row = 0;
+        for (int i = 0; i < K.n; ++i) {
+            const int4 ca = K.ca[i];
+            const int nr = ca.x == CONS_BALL ? 3 : 1;
+            if (ca.y == b || ca.z == b) {
+                const CGeom g = cons_geom(*c, ca, K.cp + 7*i);
+                d3 f = ca.x == CONS_BALL ? V3(K.w(lam + row), K.w(lam + row + 1), K.w(lam + row + 2))
+                                         : K.w(lam + row) * g.Cz;
+                f = sgn * f;
+                if (ca.z == b) F = F + dv{cross(g.r2, f), f};
+                if (ca.y == b) F = F - dv{cross(g.r1, f), f};
+            }
+            row += nr;
+        }
+        return F;
+    }
+};
+// Solves A y = r for the symmetric positive definite m x m A at ws[a]
+// (row-major) by Cholesky in ws[CWS_F] (A is kept); r at ws[r], y at ws[y].
+// False when a pivot is not positive.
+__device__ bool cons_solve(const Cons& K, int a, int r, int y) {
+    const int m = K.m;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double sacc = K.w(a + i*m + j);
+            for (int k = 0; k < j; ++k) sacc -= K.w(CWS_F + i*m + k) * K.w(CWS_F + j*m + k);
+            if (i == j) {
+                if (!(sacc > 0.0)) return false;
+                K.w(CWS_F + i*m + i) = sqrt(sacc);
+            } else {
+                K.w(CWS_F + i*m + j) = sacc / K.w(CWS_F + j*m + j);
+            }
+        }
+    for (int i = 0; i < m; ++i) {
+        double sacc = K.w(r + i);
+        for (int k = 0; k < i; ++k) sacc -= K.w(CWS_F + i*m + k) * K.w(y + k);
+        K.w(y + i) = sacc / K.w(CWS_F + i*m + i);
+    }
+    for (int i = m - 1; i >= 0; --i) {
+        double sacc = K.w(y + i);
+        for (int k = i + 1; k < m; ++k) sacc -= K.w(CWS_F + k*m + i) * K.w(y + k);
+        K.w(y + i) = sacc / K.w(CWS_F + i*m + i);
+    }
+    return true;
+}
+__device__ __forceinline__ void cons_unit(const Cons& K, int off, int k) {
+    for (int i = 0; i < K.m; ++i) K.w(off + i) = i == k ? 1.0 : 0.0;
+}
+// J^T F (tau, into the u functor) and J v (into REC_AGB) on the records.
+template <bool PAD, class FB, class TO>
+__device__ void cons_sweep_jt(const Lane& c, const FB& fb, const TO& tau) {
+    for (int b = c.nb; b >= 1; --b) {
+        const int4 ba = c.ba(b), bb = c.bb(b);
+        switch (TD_RDOF(ba)) {
+        case 1: jact_in_body<1, false>(c, b, ba, bb, fb, tau); break;
+        case 2: jact_in_body<2, false>(c, b, ba, bb, fb, tau); break;
+        case 3: jact_in_body<3, false>(c, b, ba, bb, fb, tau); break;
+        default: jact_in_body<6, false>(c, b, ba, bb, fb, tau); break;
+        }
+    }
+}
+template <bool PAD, class VG>
+__device__ void cons_sweep_j(const Lane& c, const VG& v) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        switch (TD_RDOF(ba)) {
+        case 1: jac_out_body<1>(c, b, ba, v); break;
+        case 2: jac_out_body<2>(c, b, ba, v); break;
+        case 3: jac_out_body<3>(c, b, ba, v); break;
+        default: jac_out_body<6>(c, b, ba, v); break;
+        }
+    }
+}
+// Loop forward dynamics (calcLoopForwardDynamicsOperator, SimbodyMatter-
+// SubsystemRep.cpp:5076-5152): the tree's udot under gravity and fapp, the
+// acceleration errors it leaves, W = G M^-1 G^T one column per multiplier
+// (M^-1 J^T of the unit multiplier's body forces; the rows of the resulting
+// body accelerations, which carry no velocity terms), W lambda = aerr
+// (multipliers at ws[CWS_LAM]), and udot again under fapp minus the
+// constraint forces.  qd gets qdot, ud udot.  False when W is not positive
+// definite.
+template <bool PAD, class QG, class UG, class QD, class UO>
+__device__ bool cons_derivs(const Lane& c, const Cons& K, const double* fapp, const double* grav, const QG& q,
+                            const UG& u, const QD& qd, const UO& ud) {
+    sweep_kin<true, PAD>(c, q, u, qd);
+    sweep_inward<IN_FWD | IN_SYSF, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav);
+    sweep_outward2<true, PAD>(c, q, u, NoPut{}, NoPut{});
+    cons_rows(c, K, REC_AGB, true, CWS_R, 1);
+    for (int k = 0; k < K.m; ++k) {
+        cons_unit(K, CWS_Y, k);
+        sweep_inward<IN_MINV | IN_BF, PAD>(c, ConsForce{&c, K, CWS_Y, 1.0, nullptr}, Zero{}, nullptr, nullptr, grav);
+        sweep_outward2<false, PAD>(c, q, u, NoPut{}, NoPut{});
+        cons_rows(c, K, REC_AGB, false, CWS_W + k, K.m);
+    }
+    if (!cons_solve(K, CWS_W, CWS_R, CWS_LAM)) return false;
+    sweep_inward<IN_FWD | IN_SYSF, PAD>(c, ConsForce{&c, K, CWS_LAM, -1.0, fapp}, Zero{}, nullptr, nullptr, grav);
+    sweep_outward2<true, PAD>(c, q, u, ud, NoPut{});
+    return true;
+}
+
+// ---- projections (unit weights: the reference's default constraint-error
+// and u weights, StateImpl.h:508-566) --------------------------------------
+// Per-body scale of J^T F: NInv NInv^T, 4|q|^2 on a quaternion's rotation
+// rates (Rotation.h:742-748), 1 elsewhere; the batch has no Euler-angle
+// models here.
+__device__ __forceinline__ bool body_quat(const Lane& c, int4 ba, int b) {
+    return (ba.y == BALL || ba.y == FREE) && c.prm(b)[PRM_EULER] == 0.0;
+}
+template <class QG, class UR>
+__device__ void cons_scale_d(const Lane& c, const QG& q, const UR& t) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        if (!body_quat(c, ba, b)) continue;
+        double n2 = 0;
+        for (int k = 0; k < 4; ++k) { const double x = q(ba.z + k); n2 += x*x; }
+        for (int k = 0; k < 3; ++k) t(ba.w + k) = (4.0*n2) * t(ba.w + k);
+    }
+}
+// dq = N(q) du: the unnormalized quaternion form (Rotation.h:712-719) on Ball
+// and Free rotations, identity elsewhere (qdot = u).
+template <class QG, class UR, class QR>
+__device__ void cons_mul_n(const Lane& c, const QG& q, const UR& du, const QR& dq) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        if (body_quat(c, ba, b)) {
+            double qq[4], d4[4];
+            for (int k = 0; k < 4; ++k) qq[k] = q(ba.z + k);
+            quat_dot(qq, V3(du(ba.w), du(ba.w + 1), du(ba.w + 2)), d4);
+            for (int k = 0; k < 4; ++k) dq(ba.z + k) = d4[k];
+            if (ba.y == FREE) for (int k = 0; k < 3; ++k) dq(ba.z + 4 + k) = du(ba.w + 3 + k);
+        } else {
+            const int d = mob_dof(ba.y);
+            for (int k = 0; k < d; ++k) dq(ba.z + k) = du(ba.w + k);
+        }
+    }
+}
+__device__ __forceinline__ double cons_rms(const Cons& K, int off) {
+    double sacc = 0;
+    for (int i = 0; i < K.m; ++i) sacc += K.w(off + i) * K.w(off + i);
+    return K.m ? sqrt(sacc / K.m) : 0.0;
+}
+template <class QG>
+__device__ double cons_quat_rms(const Lane& c, const QG& q, int nquat) {
+    if (nquat == 0) return 0.0;
+    double s2 = 0;
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        if (!body_quat(c, ba, b)) continue;
+        double n2 = 0;
+        for (int k = 0; k < 4; ++k) { const double x = q(ba.z + k); n2 += x*x; }
+        const double e = sqrt(n2) - 1.0;
+        s2 += e*e;
+    }
+    return sqrt(s2 / nquat);
+}
+template <class QR, class ER>
+__device__ void cons_normalize(const Lane& c, const QR& q, const ER* e) {
+    for (int b = 1; b <= c.nb; ++b) {
+        const int4 ba = c.ba(b);
+        if (!body_quat(c, ba, b)) continue;
+        double qq[4], n2 = 0;
+        for (int k = 0; k < 4; ++k) { qq[k] = q(ba.z + k); n2 += qq[k]*qq[k]; }
+        const double n = sqrt(n2);
+        for (int k = 0; k < 4; ++k) { qq[k] = qq[k] / n; q(ba.z + k) = qq[k]; }
+        if (e) {
+            double d = 0;
+            for (int k = 0; k < 4; ++k) d += (*e)(ba.z + k) * qq[k];
+            for (int k = 0; k < 4; ++k) (*e)(ba.z + k) = (*e)(ba.z + k) - d*qq[k];
+        }
+    }
+}
+// The least-norm Newton update dq = N NInv Pq^T (Pq Pq^T)^-1 r (Pq = P NInv,
+// so Pq Pq^T = P D P^T and NInv Pq^T = D P^T) for the residual ws[r] at the
+// current kinematics of q; du (u rows) and dq (q rows) are scratch rows.
+template <bool PAD, class QG, class UR, class QR>
+__device__ bool cons_q_update(const Lane& c, const Cons& K, const QG& q, int r, const UR& du, const QR& dq) {
+    for (int k = 0; k < K.m; ++k) {
+        cons_unit(K, CWS_Y, k);
+        cons_sweep_jt<PAD>(c, ConsForce{&c, K, CWS_Y, 1.0, nullptr}, [&](int i, double v) { du(i) = v; });
+        cons_scale_d(c, q, du);
+        cons_sweep_j<PAD>(c, [&](int i) { return du(i); });
+        cons_rows(c, K, REC_AGB, false, CWS_W + k, K.m);
+    }
+    if (!cons_solve(K, CWS_W, r, CWS_LAM)) return false;
+    cons_sweep_jt<PAD>(c, ConsForce{&c, K, CWS_LAM, 1.0, nullptr}, [&](int i, double v) { du(i) = v; });
+    cons_scale_d(c, q, du);
+    cons_mul_n(c, q, du, dq);
+    return true;
+}
+// projectQ (SimbodyMatterSubsystemRep.cpp:4096-4440) with the integrator's
+// options: LocalOnly unless init; ForceProjection and ForceFullNewton when
+// init.  q, u: the state rows (u only feeds the kinematics); e (nullable):
+// the q error estimate; sv (q rows), du (u rows), dq (q rows): scratch.  The
+// error-estimate projection uses Pq at the final q (the reference keeps the
+// factorization of the last iteration's starting q).
+template <bool PAD, class QR, class UR, class ER, class SR, class DUR, class DQR>
+__device__ bool cons_project_q(const Lane& c, const Cons& K, const QR& q, const UR& u, const ER* e, const SR& sv,
+                               const DUR& du, const DQR& dq, int nq, int nquat, double consTol, double projLimit,
+                               bool init, bool* changed) {
+    *changed = false;
+    sweep_kin<false, PAD>(c, q, u, NoPut{});
+    cons_perr(c, K, CWS_R);
+    const double pn0 = cons_rms(K, CWS_R), qn0 = cons_quat_rms(c, q, nquat);
+    if (fmax(pn0, qn0) > projLimit || pn0 != pn0) return false;
+    if (pn0 == 0 || (pn0 <= consTol && !init)) {
+        if (qn0 > consTol || init) {
+            if (nquat) { cons_normalize(c, q, e); *changed = true; }
+            if (cons_quat_rms(c, q, nquat) > consTol) return false;
+        }
+        return true;
+    }
+    const double target = fmax(0.1*consTol, 1.9323057602939027e-14);   // SignificantReal = eps^(7/8)
+    for (int j = 0; j < nq; ++j) sv(j) = q(j);
+    double pn = pn0, prev = pn0;
+    int its = 0;
+    do {
+        if (!cons_q_update<PAD>(c, K, q, CWS_R, du, dq)) return false;
+        for (int j = 0; j < nq; ++j) q(j) = q(j) - dq(j);
+        *changed = true; ++its;
+        sweep_kin<false, PAD>(c, q, u, NoPut{});
+        cons_perr(c, K, CWS_R);
+        pn = cons_rms(K, CWS_R);
+        if (!init && its >= 2 && pn > prev) {
+            for (int j = 0; j < nq; ++j) q(j) = q(j) + dq(j);
+            sweep_kin<false, PAD>(c, q, u, NoPut{});
+            cons_perr(c, K, CWS_R);
+            pn = cons_rms(K, CWS_R);
+            break;
+        }
+        prev = pn;
+    } while (pn > target && its < 20);
+    if (!(pn <= consTol)) {
+        if (!(pn < pn0)) {
+            for (int j = 0; j < nq; ++j) q(j) = sv(j);
+            sweep_kin<false, PAD>(c, q, u, NoPut{});
+        }
+        return false;
+    }
+    if (e) {
+        // remove the error estimate's component off the constraint manifold:
+        // r = Pq e = P NInv e, then e -= dq
+        for (int b = 1; b <= c.nb; ++b) {
+            const int4 ba = c.ba(b);
+            if (body_quat(c, ba, b)) {
+                double qq[4], ee[4];
+                for (int k = 0; k < 4; ++k) { qq[k] = q(ba.z + k); ee[k] = (*e)(ba.z + k); }
+                const d3 w = quat_dot_to_w(qq, ee);
+                du(ba.w) = w.x; du(ba.w + 1) = w.y; du(ba.w + 2) = w.z;
+                if (ba.y == FREE) for (int k = 0; k < 3; ++k) du(ba.w + 3 + k) = (*e)(ba.z + 4 + k);
+            } else {
+                const int d = mob_dof(ba.y);
+                for (int k = 0; k < d; ++k) du(ba.w + k) = (*e)(ba.z + k);
+            }
+        }
+        cons_sweep_j<PAD>(c, [&](int i) { return du(i); });
+        cons_rows(c, K, REC_AGB, false, CWS_R, 1);
+        if (!cons_q_update<PAD>(c, K, q, CWS_R, du, dq)) return false;
+        for (int j = 0; j < nq; ++j) (*e)(j) = (*e)(j) - dq(j);
+    }
+    if (nquat) {
+        cons_normalize(c, q, e);
+        if (cons_quat_rms(c, q, nquat) > consTol) return false;
+    }
+    return true;
+}
+// projectU (SimbodyMatterSubsystemRep.cpp:4666-4934) with the integrator's
+// options (as projectQ): relative u scaling s = max(|u_i|, 1) from the u on
+// entry, one factorization of P S^2 P^T for every iteration.  e (nullable):
+// the u error estimate; sv, du (u rows): scratch.
+template <bool PAD, class QR, class UR, class ER, class SR, class DUR>
+__device__ bool cons_project_u(const Lane& c, const Cons& K, const QR& q, const UR& u, const ER* e, const SR& sv,
+                               const DUR& du, int nu, double consTol, double projLimit, bool init, bool* changed) {
+    *changed = false;
+    sweep_kin<true, PAD>(c, q, u, NoPut{});
+    cons_rows(c, K, REC_V, false, CWS_R, 1);
+    const double vn0 = cons_rms(K, CWS_R);
+    if (vn0 > projLimit || vn0 != vn0) return false;
+    if (vn0 == 0 || (vn0 <= consTol && !init)) return true;
+    const double target = fmax(0.1*consTol, 1.9323057602939027e-14);
+    for (int j = 0; j < nu; ++j) sv(j) = u(j);
+    auto s2 = [&](int j) { const double a = fabs(sv(j)); const double sc = a > 1.0 ? a : 1.0; return sc*sc; };
+    for (int k = 0; k < K.m; ++k) {
+        cons_unit(K, CWS_Y, k);
+        cons_sweep_jt<PAD>(c, ConsForce{&c, K, CWS_Y, 1.0, nullptr}, [&](int i, double v) { du(i) = v; });
+        for (int j = 0; j < nu; ++j) du(j) = s2(j) * du(j);
+        cons_sweep_j<PAD>(c, [&](int i) { return du(i); });
+        cons_rows(c, K, REC_AGB, false, CWS_W + k, K.m);
+    }
+    auto solve_du = [&](int r) {
+        if (!cons_solve(K, CWS_W, r, CWS_LAM)) return false;
+        cons_sweep_jt<PAD>(c, ConsForce{&c, K, CWS_LAM, 1.0, nullptr}, [&](int i, double v) { du(i) = v; });
+        for (int j = 0; j < nu; ++j) du(j) = s2(j) * du(j);
+        return true;
+    };
+    double vn = vn0, prev = vn0;
+    int its = 0;
+    do {
+        if (!solve_du(CWS_R)) return false;
+        for (int j = 0; j < nu; ++j) u(j) = u(j) - du(j);
+        *changed = true; ++its;
+        sweep_kin<true, PAD>(c, q, u, NoPut{});
+        cons_rows(c, K, REC_V, false, CWS_R, 1);
+        vn = cons_rms(K, CWS_R);
+        if (!init && its >= 2 && vn > prev) {
+            for (int j = 0; j < nu; ++j) u(j) = u(j) + du(j);
+            sweep_kin<true, PAD>(c, q, u, NoPut{});
+            cons_rows(c, K, REC_V, false, CWS_R, 1);
+            vn = cons_rms(K, CWS_R);
+            break;
+        }
+        prev = vn;
+    } while (vn > target && its < 7);
+    if (!(vn <= consTol)) {
+        if (!(vn < vn0)) for (int j = 0; j < nu; ++j) u(j) = sv(j);
+        return false;
+    }
+    if (e) {
+        cons_sweep_j<PAD>(c, [&](int i) { return (*e)(i); });
+        cons_rows(c, K, REC_AGB, false, CWS_R, 1);
+        if (!solve_du(CWS_R)) return false;
+        for (int j = 0; j < nu; ++j) (*e)(j) = (*e)(j) - du(j);
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
 // Per-instance status of the operators that form articulated inertias:
@@ -904,8 +1335,6 @@ struct BodyForce {   // packed (nb+1)*6 per instance; nullptr = zero
 __device__ __forceinline__ void op_status(const OpArgs& a, int lane, bool singular) {
     if (a.status) a.status[a.inst ? a.inst[lane] : lane] = singular ? 4 : 0;
 }
-// D of a record: padded generic bodies are 3-dof in batches of different trees.
-#define TD_RDOF(ba) (((PAD || mob_generic_only((ba).y)) && (ba).y != FREE) ? 3 : mob_dof((ba).y))
 template <int OP, bool UNI>
 __global__ void __launch_bounds__(64) k_op(OpArgs a) {
     constexpr bool PAD = !UNI;      // batches of different trees use padded records
@@ -961,6 +1390,26 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
                 o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
             }
         }
+    } else if constexpr (OP == OP_FWD_CONS) {
+        // realize(Acceleration) with constraints: loop forward dynamics
+        // (gravity and the applied body forces), udot and the multipliers
+        const Cons K = make_cons(a.T, tr, a.cons_ws, lane, a.L);
+        const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
+        const PackedPut uout{a.out_u ? a.out_u + uo : nullptr};
+        bool ok = true;
+        const PackedPut qout{a.out_q ? a.out_q + qo : nullptr};
+        if (K.m) {
+            ok = cons_derivs<PAD>(c, K, a.fbody ? a.fbody + 6*(size_t)bo : nullptr, grav, q, u, qout, uout);
+        } else {
+            sweep_kin<true, PAD>(c, q, u, qout);
+            ok = !sweep_inward<IN_FWD | IN_SYSF, PAD>(c, fb, Zero{}, nullptr, nullptr, grav);
+            sweep_outward2<true, PAD>(c, q, u, uout, NoPut{});
+        }
+        if (a.out_lam) {
+            double* lo = a.out_lam + (size_t)CONS_MAXR * (a.inst ? a.inst[lane] : lane);
+            for (int i = 0; i < CONS_MAXR; ++i) lo[i] = (ok && i < K.m) ? K.w(CWS_LAM + i) : 0.0;
+        }
+        op_status(a, lane, !ok);
     } else if constexpr (OP == OP_FWD_SYS || OP == OP_FWD_GIVEN || OP == OP_REACT) {
         sweep_kin<true, PAD>(c, q, u, PackedPut{(OP == OP_FWD_SYS && a.out_q) ? a.out_q + qo : nullptr});
         const BodyForce fb{a.fbody ? a.fbody + 6*(size_t)bo : nullptr};
@@ -1132,6 +1581,7 @@ static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     case OP_BACC:      k_op<OP_BACC, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_CBI:       k_op<OP_CBI, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_FWD_PRES:  k_op<OP_FWD_PRES, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_FWD_CONS:  k_op<OP_FWD_CONS, UNI><<<grd, blk, 0, s>>>(a); break;
     }
 }
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
@@ -1154,20 +1604,30 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 template <bool PAD>
 struct TreeEval {
     static constexpr bool kMotion = true;     // carries prescribed motion
+    static constexpr bool kCons = true;       // carries holonomic constraints
+    static constexpr bool kPad = PAD;
     using VR = chain::RowsT<double, false>;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
         return VR{a.vec + (size_t)v * a.ny_max * a.L, a.L, (unsigned)lane};
     }
     Lane c; const double* grav; const double* fapp;
     Pres pr;   // known udot (locks, constant prescribed udot): pr.m null when none
+    Cons K;    // the lane's constraints (K.m = 0: none)
     __device__ TreeEval(const RkmArgs& a, int lane, int tr)
         : c(make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0)), grav(a.T.gravity + 4*tr),
           fapp(a.fapp ? a.fapp + 6*(size_t)a.off.b_off[lane] : nullptr),
           pr{a.presc ? a.presc + a.off.b_off[lane] : nullptr, a.presc ? a.udot_known + a.off.u_off[lane] : nullptr,
-             nullptr} {}
+             nullptr},
+          K(make_cons(a.T, tr, a.cons_ws, lane, a.L)) {}
     __device__ __forceinline__ void derivs(const chain::RowsT<double, false>& yr, const chain::RowsT<double, false>& fr) const {
         const SoAGet y{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
         const SoAPut f{fr.base, fr.L, fr.lane}, fu{fr.base + (size_t)c.nq * fr.L, fr.L, fr.lane};
+        if (K.m) {
+            // loop forward dynamics; a singular G M^-1 G^T fails the step (NaN udot)
+            if (!cons_derivs<PAD>(c, K, fapp, grav, y, u, f, fu))
+                for (int k = 0; k < c.nu; ++k) fu(k, __builtin_nan(""));
+            return;
+        }
         sweep_kin<true, PAD>(c, y, u, f);
         if (pr.m) {
             sweep_inward<IN_FWD | IN_SYSF | IN_PRES, PAD>(c, BodyForce{fapp}, Zero{}, nullptr, nullptr, grav, pr);
@@ -1184,6 +1644,8 @@ struct TreeEval {
 // slots of different trees take only the memory their trees need.
 struct TreePoolEval {
     static constexpr bool kMotion = false;
+    static constexpr bool kCons = false;
+    static constexpr bool kPad = true;
     using VR = tstream::LaneVec;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int lane) {
         const int4 ta = ldi4(a.T.tree_a, lane);
@@ -1222,6 +1684,8 @@ struct TreePoolEval {
 
 struct TreeStreamEval {
     static constexpr bool kMotion = false;
+    static constexpr bool kCons = false;
+    static constexpr bool kPad = true;
     using VR = tstream::LaneVec;
     DevTables T;
     int nb, nq, body_off, acc0, pos0;
@@ -1582,7 +2046,26 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             }
             bool converged = true;
             const double e1 = errNorm();
-            if (!(e1 > 16.0*acc) && nquat > 0) {
+            bool cons = false;
+            if constexpr (EV::kCons) cons = ev.K.m > 0;
+            if (cons) {
+                if constexpr (EV::kCons) {
+                    // attemptDAEStep (AbstractIntegratorRep.cpp:165-207): projectQ
+                    // then projectU of y1 and its error estimate, if the error
+                    // is worth it
+                    if (!(e1 > 16.0*acc)) {
+                        const TRows Y1u{Y1.base + (size_t)nq * L, L, Y1.lane}, YEu{YE.base + (size_t)nq * L, L, YE.lane};
+                        const TRows YSu{YS.base + (size_t)nq * L, L, YS.lane}, YTu{YT.base + (size_t)nq * L, L, YT.lane};
+                        bool chq = false, chu = false;
+                        if (!cons_project_q<EV::kPad>(ev.c, ev.K, Y1, Y1u, &YE, YS, YTu, YT, nq, nquat, consTol,
+                                                      projLimit, false, &chq) ||
+                            !cons_project_u<EV::kPad>(ev.c, ev.K, Y1, Y1u, &YEu, YSu, YTu, nu, consTol, projLimit,
+                                                      false, &chu))
+                            converged = false;
+                        if (chq) ++projections;
+                    }
+                }
+            } else if (!(e1 > 16.0*acc) && nquat > 0) {
                 double s2 = 0;
                 for (int b = 0; b < nb; ++b) {
                     const int4 ba = ldi4(body_a, b);
@@ -1668,7 +2151,10 @@ __device__ void rkm_init_lane(const RkmArgs& a, int lane, double t0) {
         const chain::RowsT<double, false> y{a.vec + (size_t)VY * a.ny_max * a.L, a.L, (unsigned)lane};
         prescribe_lane(a, lane, a.tree_of ? a.tree_of[lane] : 0, t0, y, true);
     }
-    for (int b = 1; b <= c.nb; ++b) {
+    // (a lane with constraints is projected by k_cons_init instead:
+    // projectQ normalizes the quaternions as it projects)
+    const bool constrained = a.cons_ws && a.T.cons_t && ldi4(a.T.cons_t, a.tree_of ? a.tree_of[lane] : 0).z > 0;
+    for (int b = 1; b <= c.nb && !constrained; ++b) {
         const int4 ba = c.ba(b);
         if (ba.y != BALL && ba.y != FREE) continue;
         if (c.prm(b)[PRM_EULER] != 0.0) continue;      // x-y-z angles, not a quaternion
@@ -1848,6 +2334,38 @@ hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double*
 hipError_t launch_rkm_init(const RkmArgs& a, double t0, hipStream_t s) {
     if (a.n_inst <= 0) return hipSuccess;
     k_rkm_init<<<dim3((a.n_inst + 63)/64), dim3(64), 0, s>>>(a, t0);
+    return hipGetLastError();
+}
+// initialize() of a lane with constraints (Integrator.cpp:376-377): projectQ
+// and projectU with ForceProjection and ForceFullNewton (no LocalOnly), with
+// VYS and VYT as scratch rows; status -4 when either fails.
+template <bool PAD>
+__global__ void __launch_bounds__(64) k_cons_init(RkmArgs a) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= a.n_inst) return;
+    const int tr = PAD ? a.tree_of[lane] : 0;
+    const Cons K = make_cons(a.T, tr, a.cons_ws, lane, a.L);
+    if (K.m == 0) return;
+    const Lane c = make_lane_t(a.T, tr, lane, a.L, a.scratch, PAD ? a.rec_stride : 0);
+    using TRows = typename TreeEval<PAD>::VR;
+    const int4 ta = ldi4(a.T.tree_a, tr);
+    const int nq = ta.y, nu = ta.z, nquat = ldi4(a.T.tree_b, tr).y;
+    const TRows Y = TreeEval<PAD>::vrow(a, VY, lane), YS = TreeEval<PAD>::vrow(a, VYS, lane),
+                YT = TreeEval<PAD>::vrow(a, VYT, lane);
+    const TRows Yu{Y.base + (size_t)nq * a.L, a.L, Y.lane}, YSu{YS.base + (size_t)nq * a.L, a.L, YS.lane},
+                YTu{YT.base + (size_t)nq * a.L, a.L, YT.lane};
+    const double inf = __builtin_inf();
+    bool ch = false;
+    if (!cons_project_q<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YS, YTu, YT, nq, nquat, a.cfg.cons_tol, inf, true,
+                             &ch) ||
+        !cons_project_u<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YSu, YTu, nu, a.cfg.cons_tol, inf, true, &ch))
+        a.status[lane] = -4;
+}
+hipError_t launch_cons_init(const RkmArgs& a, hipStream_t s) {
+    if (a.n_inst <= 0 || !a.cons_ws) return hipSuccess;
+    const dim3 grd((a.n_inst + 63)/64), blk(64);
+    if (a.tree_of) k_cons_init<true><<<grd, blk, 0, s>>>(a);
+    else k_cons_init<false><<<grd, blk, 0, s>>>(a);
     return hipGetLastError();
 }
 hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {

@@ -82,6 +82,7 @@ extern "C" int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const d
     if (b->evaluator == 3) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: runs the chain evaluator or the tree "
                                                    "sweeps (evaluator 0, 1 or 2), not the level-parallel evaluator");
     if (b->motion) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: prescribed motion runs on tdyn_rollout_begin only");
+    if (b->any_cons) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: constraints run on tdyn_rollout_begin only");
     const int L = b->n_inst;
     const int ny = b->max_nq + b->max_nu;
     const int rec = b->max_rec + 24;
@@ -110,6 +111,7 @@ extern "C" int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const d
     a.scratch32 = nullptr;
     a.presc = b->known_p; a.udot_known = b->known_u;
     a.motion = nullptr;
+    a.cons_ws = nullptr;
     a.chain = (!b->known_p && (b->evaluator == 2 || (b->evaluator == 0 && b->is_chain))) ? 1 : 0;
     a.idf = (a.chain ? b->idf : b->idf_all) ? 1 : 0;
     a.ext = (b->any_rev || b->any_euler || b->any_bendstretch || b->any_bushing || b->known_p) ? 1 : 0;

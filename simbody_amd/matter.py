@@ -33,6 +33,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_realize_kinematics", "tdyn_realize_acceleration",
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
            "tdyn_batch_set_known_udot", "tdyn_batch_set_motion", "tdyn_batch_autotune",
+           "tdyn_batch_set_constraints", "tdyn_calc_constrained_acceleration",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -59,6 +60,16 @@ class TreeDescC(ctypes.Structure):
                 ("mass", _d), ("com_B", _d), ("inertia_B", _d),
                 ("X_PF", _d), ("X_BM", _d), ("gravity", ctypes.c_double * 3), ("screw_pitch", _d),
                 ("reversed", _i32), ("use_euler_angles", ctypes.c_int32)]
+
+
+class ConstraintDescC(ctypes.Structure):
+    """tdyn_constraint_desc."""
+    _fields_ = [("type", ctypes.c_int32), ("body1", ctypes.c_int32), ("body2", ctypes.c_int32),
+                ("station1", ctypes.c_double * 3), ("station2", ctypes.c_double * 3), ("length", ctypes.c_double)]
+
+
+CONS_BALL, CONS_ROD = 0, 1
+MAX_CONSTRAINT_ROWS = 12
 
 
 class IntegratorCfgC(ctypes.Structure):
@@ -100,6 +111,8 @@ def lib():
         L.tdyn_calc_acceleration_with_known_udot.argtypes = [vp] + [vp] * 9 + [vp]
         L.tdyn_batch_set_known_udot.argtypes = [vp, vp, vp]
         L.tdyn_batch_set_motion.argtypes = [vp, vp]
+        L.tdyn_batch_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ConstraintDescC)]
+        L.tdyn_calc_constrained_acceleration.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
                                           ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
@@ -380,6 +393,27 @@ class MatterBatch:
         by this object; None, None clears."""
         self._known = (known, udot_known)
         _check(lib().tdyn_batch_set_known_udot(self.h, _ptr(known), _ptr(udot_known)))
+
+    def set_constraints(self, tree, constraints):
+        """Constraint::Ball / Constraint::Rod on tree `tree`: a list of
+        (type, body1, body2, station1, station2, length), type CONS_BALL or
+        CONS_ROD, bodies 0 = Ground, stations in the body frames; [] clears."""
+        n = len(constraints)
+        arr = (ConstraintDescC * max(n, 1))()
+        for i, (ty, b1, b2, s1, s2, ln) in enumerate(constraints):
+            arr[i].type, arr[i].body1, arr[i].body2 = int(ty), int(b1), int(b2)
+            for k in range(3):
+                arr[i].station1[k] = float(s1[k]); arr[i].station2[k] = float(s2[k])
+            arr[i].length = float(ln)
+        _check(lib().tdyn_batch_set_constraints(self.h, int(tree), n, arr))
+
+    def calcConstrainedAcceleration(self, q, u, F=None):
+        """realize(Acceleration) with constraints: (udot, multipliers
+        [n_instances, MAX_CONSTRAINT_ROWS])."""
+        ud = self._new(self.total_u); lam = self._new(MAX_CONSTRAINT_ROWS * self.n_instances)
+        _check(lib().tdyn_calc_constrained_acceleration(self.h, _ptr(q), _ptr(u), _ptr(F), _ptr(ud), _ptr(lam),
+                                                       _stream()))
+        return ud, lam.view(self.n_instances, MAX_CONSTRAINT_ROWS)
 
     def set_motion(self, motion=None):
         """Prescribed motion for the following rollouts (Motion::Sinusoid,
