@@ -114,6 +114,17 @@ public:
         check(tdyn_calc_acceleration_with_known_udot(h_, q, u, fMob, F, known, udotKnown, udot, tau, A_GB, s_));
         if (throwOnSingular_) checkSingular();
     }
+    // Constraint::Ball / Constraint::Rod on tree `tree` (tdyn_batch_set_constraints).
+    void setConstraints(int tree, int n, const tdyn_constraint_desc* c) const {
+        check(tdyn_batch_set_constraints(h_, tree, n, c));
+    }
+    // realize(Acceleration) with constraints and getConstraintMultipliers
+    // (TDYN_MAX_CONSTRAINT_ROWS per instance).
+    void calcConstrainedAcceleration(const double* q, const double* u, const double* appliedBodyForces,
+                                     double* udot, double* multipliers) const {
+        check(tdyn_calc_constrained_acceleration(h_, q, u, appliedBodyForces, udot, multipliers, s_));
+        if (throwOnSingular_) checkSingular();
+    }
     void multiplyByM(const double* q, const double* v, double* Mv) const {
         check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
     }
