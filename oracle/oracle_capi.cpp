@@ -19,7 +19,16 @@ struct Handle {
     Model m;
     std::unique_ptr<Tree<double>> tr;
     std::vector<ConsDef> cons;   // holonomic constraints of the model
+    std::vector<WitnessDef> wit; // event witnesses of the model
+    std::vector<EventRecord> events;   // the events of the last integration
+    RkmStats last;                     // and its statistics (final time, termination)
 };
+// Attaches the model's constraints and witnesses to an integrator.
+void attach(Handle* h, Rkm<double>& rk) {
+    if (!h->cons.empty()) rk.cons = &h->cons;
+    h->events.clear();
+    if (!h->wit.empty()) { rk.wit = &h->wit; rk.events = &h->events; }
+}
 inline Handle* H(void* p) { return static_cast<Handle*>(p); }
 
 void loadForces(const Model& M, const double* F, std::vector<SV<double>>& out) {
@@ -89,6 +98,60 @@ int oracle_loop_forward(void* p, const double* q, const double* u, const double*
 }
 // Constraint errors at (q, u): perr [m], verr [m] (either nullable), and the
 // acceleration errors aerr [m] (nullable) of the body accelerations of udot.
+// Event witnesses of the model (WitnessDef): per witness ib[6] = kind,
+// index, body, trigger mask, action, action index and dd[9] = station (3),
+// direction (3), offset, factor, window.  Later integrations (not the one
+// with reports by interpolation) localize and handle their events.  Returns
+// -1 for an invalid description.
+int oracle_set_witnesses(void* p, int n, const int* ib, const double* dd) {
+    Handle* h = H(p); const Model& M = h->m;
+    std::vector<WitnessDef> ws;
+    for (int i = 0; i < n; ++i) {
+        WitnessDef w;
+        w.kind = ib[6*i]; w.index = ib[6*i+1]; w.body = ib[6*i+2]; w.trigger = ib[6*i+3];
+        w.action = ib[6*i+4]; w.aindex = ib[6*i+5];
+        for (int k = 0; k < 3; ++k) { w.station[k] = dd[9*i+k]; w.dir[k] = dd[9*i+3+k]; }
+        w.offset = dd[9*i+6]; w.factor = dd[9*i+7]; w.window = dd[9*i+8];
+        for (int k = 0; k < 9; ++k) if (!std::isfinite(dd[9*i+k])) return -1;
+        if (w.kind < WIT_Q || w.kind > WIT_TIME || w.trigger < 1 || w.trigger > 3 ||
+            w.action < EV_REPORT || w.action > EV_TERMINATE || !(w.window > 0)) return -1;
+        if (w.kind == WIT_Q && (w.index < 0 || w.index >= M.nq)) return -1;
+        if (w.kind == WIT_U && (w.index < 0 || w.index >= M.nu)) return -1;
+        if ((w.kind == WIT_STATION_POS || w.kind == WIT_STATION_VEL) && (w.body < 0 || w.body > M.nb)) return -1;
+        if (w.action == EV_SCALE_U && w.aindex >= M.nu) return -1;
+        ws.push_back(w);
+    }
+    h->wit = ws;
+    return 0;
+}
+// The witness values at (t, y).
+int oracle_witness_values(void* p, double t, const double* y, double* e) {
+    Handle* h = H(p);
+    Rkm<double> rk(h->m);
+    rk.wit = &h->wit;
+    rk.witnessValues(t, y, e);
+    return 0;
+}
+// The last integration's events: up to cap rows of (tLow, tHigh) into t and
+// (witness, transition) into ids; returns how many it had.  *t_final and
+// *terminated (nullable) get its final time and whether a handler ended it.
+int oracle_last_events(void* p, double* t, int* ids, int cap, double* t_final, int* terminated) {
+    Handle* h = H(p);
+    const int n = (int)h->events.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        t[2*i] = h->events[i].tLow; t[2*i+1] = h->events[i].tHigh;
+        ids[2*i] = h->events[i].witness; ids[2*i+1] = h->events[i].transition;
+    }
+    if (t_final) *t_final = h->last.t;
+    if (terminated) *terminated = h->last.terminated;
+    return n;
+}
+// estimateRootTime of the localization (IntegratorRep.h:351-376).
+double oracle_estimate_root_time(double tLow, double fLow, double tHigh, double fHigh, double bias,
+                                 double minWindow) {
+    return estimateRootTime(tLow, fLow, tHigh, fHigh, bias, minWindow);
+}
+
 int oracle_constraint_errors(void* p, const double* q, const double* u, const double* udot, double* perr,
                              double* verr, double* aerr) {
     Handle* h = H(p); const Model& M = h->m; Tree<double>& tr = *h->tr;
@@ -293,11 +356,30 @@ int oracle_integrate(void* p, double* y, const double* Fapp, double t0, double T
                      double accuracy, double fixed_h, long max_attempts, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     if (max_attempts > 0) cfg.max_attempts = max_attempts;
     RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    h->last = st;
+    if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
+    return st.status;
+}
+
+// Integration to T with the integrator settings c[6] = accuracy, fixed
+// step, initial, minimum and maximum step (-1: unset) and time scale
+// (Integrator::setInitialStepSize, setMinimumStepSize, setMaximumStepSize;
+// System::setDefaultTimeScale).
+int oracle_integrate_cfg(void* p, double* y, const double* Fapp, double t0, double T, const double* c,
+                         long* stats) {
+    Handle* h = H(p); const Model& M = h->m;
+    Rkm<double> rk(M);
+    attach(h, rk);
+    std::vector<SV<double>> F; loadForces(M, Fapp, F);
+    RkmConfig cfg; cfg.accuracy = c[0]; cfg.fixed_h = c[1]; cfg.init_h = c[2]; cfg.min_h = c[3]; cfg.max_h = c[4];
+    cfg.timescale = c[5];
+    RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -326,7 +408,7 @@ int oracle_integrate_trace(void* p, double* y, const double* Fapp, double t0, do
                            double fixed_h, double init_h, double* rec, int max_rec, int* n_rec, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     std::vector<RkmAttempt> tr;
     rk.trace = &tr;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
@@ -340,6 +422,7 @@ int oracle_integrate_trace(void* p, double* y, const double* Fapp, double t0, do
         std::copy(r, r + 8, rec + 8*(size_t)i);
     }
     *n_rec = (int)tr.size();
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -350,11 +433,12 @@ int oracle_integrate_motion(void* p, double* y, const double* Fapp, const double
                             double accuracy, double fixed_h, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     rk.motion = motion;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -366,10 +450,11 @@ int oracle_integrate_stops(void* p, double* y, const double* Fapp, double t0, co
                            double accuracy, double fixed_h, double* yout, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrateStops(y, t0, stops, n, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, yout);
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -380,12 +465,16 @@ int oracle_integrate_stops(void* p, double* y, const double* Fapp, double t0, co
 int oracle_integrate_reports(void* p, double* y, const double* Fapp, double t0, double T, const double* reports,
                              int n, double accuracy, double fixed_h, double* rout, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
+    // (a report time inside an event's window changes the localization
+    // trials, AbstractIntegratorRep.cpp:640-680; not restated)
+    if (!h->wit.empty()) return -1;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrateStops(y, t0, &T, 1, cfg, Fapp ? F.data() : (const SV<double>*)nullptr, nullptr,
                                     reports, n, rout);
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -396,11 +485,12 @@ int oracle_integrate_known(void* p, double* y, const double* Fapp, const int* pr
                            double t0, double T, double accuracy, double fixed_h, long* stats) {
     Handle* h = H(p); const Model& M = h->m;
     Rkm<double> rk(M);
-    if (!h->cons.empty()) rk.cons = &h->cons;
+    attach(h, rk);
     rk.ws.presc = presc; rk.ws.udotKnown = udot_known;
     std::vector<SV<double>> F; loadForces(M, Fapp, F);
     RkmConfig cfg; cfg.accuracy = accuracy; cfg.fixed_h = fixed_h;
     RkmStats st = rk.integrate(y, t0, T, cfg, Fapp ? F.data() : (const SV<double>*)nullptr);
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }
@@ -421,6 +511,7 @@ int oracle_count_flops_integrate(void* p, double* y, const double* Fapp, double 
     const FlopTally t = tally();
     tallies[0]=t.addsub; tallies[1]=t.mul; tallies[2]=t.div; tallies[3]=t.sqrt; tallies[4]=t.trig;
     for (int i=0;i<ny;++i) y[i] = yc[i].v;
+    h->last = st;
     if (stats) { stats[0]=st.attempts; stats[1]=st.accepted; stats[2]=st.rejected; stats[3]=st.projections; stats[4]=st.evals; stats[5]=st.status; }
     return st.status;
 }

@@ -51,9 +51,25 @@ def _open(path):
                  "oracle_body_acceleration", "oracle_composite_inertias", "oracle_forward_prescribed",
                  "oracle_integrate_known", "oracle_integrate_stops", "oracle_integrate_reports",
                  "oracle_integrate_trace", "oracle_abi_shift", "oracle_integrate_motion",
-                 "oracle_set_constraints", "oracle_loop_forward", "oracle_constraint_errors"]:
+                 "oracle_set_constraints", "oracle_loop_forward", "oracle_constraint_errors",
+                 "oracle_set_witnesses", "oracle_witness_values", "oracle_last_events",
+                 "oracle_integrate_cfg"]:
         getattr(L, name).restype = ctypes.c_int
+    L.oracle_estimate_root_time.restype = ctypes.c_double
+    L.oracle_estimate_root_time.argtypes = [ctypes.c_double] * 6
     return L
+
+
+# Witness kinds, trigger masks and handler actions (tree_oracle.hpp WitnessDef).
+WIT_Q, WIT_U, WIT_STATION_POS, WIT_STATION_VEL, WIT_TIME = 0, 1, 2, 3, 4
+TRIG_FALLING, TRIG_RISING, TRIG_BOTH = 1, 2, 3
+EV_REPORT, EV_SCALE_U, EV_TERMINATE = 0, 1, 2
+_WIT_DEFAULTS = dict(index=0, body=0, trigger=TRIG_BOTH, action=EV_REPORT, action_index=-1,
+                     station=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 0.0), offset=0.0, factor=1.0, window=0.1)
+
+
+def estimate_root_time(tLow, fLow, tHigh, fHigh, bias=1.0, min_window=1e-14):
+    return lib().oracle_estimate_root_time(tLow, fLow, tHigh, fHigh, bias, min_window)
 
 
 def lib(variant=None):
@@ -248,6 +264,36 @@ class OracleModel:
         self._L.oracle_body_acceleration(self.h, _p(q), _p(u), _p(ud), _p(A))
         return A
 
+    def set_witnesses(self, witnesses):
+        """Event witnesses: a list of dicts with `kind` and any of index,
+        body, trigger, action, action_index, station, direction, offset,
+        factor, window (defaults: both transitions, report, window 0.1).
+        Later integrations localize and handle their events.  Returns the
+        oracle's code (-1: invalid)."""
+        n = len(witnesses)
+        ib = np.zeros(6 * max(n, 1), np.int32); dd = np.zeros(9 * max(n, 1))
+        for i, w in enumerate(witnesses):
+            w = dict(_WIT_DEFAULTS, **w)
+            ib[6*i:6*i+6] = (w["kind"], w["index"], w["body"], w["trigger"], w["action"], w["action_index"])
+            dd[9*i:9*i+3] = w["station"]; dd[9*i+3:9*i+6] = w["direction"]
+            dd[9*i+6:9*i+9] = (w["offset"], w["factor"], w["window"])
+        return self._L.oracle_set_witnesses(self.h, n, _pi(ib), _p(dd))
+
+    def witness_values(self, t, y, n):
+        e = np.zeros(max(n, 1))
+        self._L.oracle_witness_values(self.h, ctypes.c_double(t), _p(np.ascontiguousarray(y, np.float64)), _p(e))
+        return e[:n]
+
+    def last_events(self, cap=100000):
+        """The last integration's events: (windows [n, 2] of (tLow, tHigh),
+        ids [n, 2] of (witness, transition), final time, terminated)."""
+        t = np.zeros(2 * cap); ids = np.zeros(2 * cap, np.int32)
+        tf = ctypes.c_double(0.0); term = ctypes.c_int(0)
+        n = self._L.oracle_last_events(self.h, _p(t), _pi(ids), ctypes.c_int(cap), ctypes.byref(tf),
+                                       ctypes.byref(term))
+        m = min(n, cap)
+        return t[:2*m].reshape(m, 2), ids[:2*m].reshape(m, 2), tf.value, bool(term.value)
+
     def composite_inertias(self, q):
         """(nb+1, 10): mass, mass center (3), unit inertia about the body
         origin (xx yy zz xy xz yz), in G; row 0 is Ground (infinite mass)."""
@@ -255,9 +301,16 @@ class OracleModel:
         self._L.oracle_composite_inertias(self.h, _p(q), _p(R))
         return R
 
-    def integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None, t0=0.0, max_attempts=0):
+    def integrate(self, y, T, accuracy, fixed_h=0.0, Fapp=None, t0=0.0, max_attempts=0, init_h=None,
+                  min_h=None, max_h=None, timescale=0.1):
         y = np.array(y, dtype=np.float64, copy=True)
         st = (ctypes.c_long * 6)()
+        keys = ["attempts", "accepted", "rejected", "projections", "evals", "status"]
+        if init_h is not None or min_h is not None or max_h is not None or timescale != 0.1:
+            c = np.array([accuracy, fixed_h, -1.0 if init_h is None else init_h, -1.0 if min_h is None else min_h,
+                          -1.0 if max_h is None else max_h, timescale])
+            self._L.oracle_integrate_cfg(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T), _p(c), st)
+            return y, dict(zip(keys, list(st)))
         self._L.oracle_integrate(self.h, _p(y), _p(Fapp), ctypes.c_double(t0), ctypes.c_double(T),
                                ctypes.c_double(accuracy), ctypes.c_double(fixed_h),
                                ctypes.c_long(max_attempts), st)

@@ -1301,7 +1301,13 @@ struct RkmConfig {
     double timescale = 0.1;     // System default time scale
     long   max_attempts = 100000000;
 };
-struct RkmStats { long attempts=0, accepted=0, rejected=0, projections=0, evals=0; int status=0; };
+// status: 0 success, 2 a derivative failed, 3 the step fell to zero, 4 too
+// many attempts, 5 the initial projection failed, 6 event localization
+// failed; terminated: an event handler ended the integration at time t.
+struct RkmStats {
+    long attempts=0, accepted=0, rejected=0, projections=0, evals=0; int status=0;
+    int terminated = 0; double t = 0;
+};
 
 // One step attempt as the step controller saw it (test hook): the attempt's
 // start and end times, the error norm before the projection gate (NaN when
@@ -1319,6 +1325,85 @@ struct RkmAttempt { double t0, t1, err_pre, qerr, err, projected, accepted, h_ne
 // per body, Ground first: level, a, w, phi.
 enum MotionLevel { MOTION_NONE = 0, MOTION_POSITION = 1, MOTION_VELOCITY = 2, MOTION_ACCELERATION = 3,
                    MOTION_STEADY = 4 };
+
+// Event witnesses: the trigger functions of TriggeredEventHandler and
+// TriggeredEventReporter (EventHandler.h, EventReporter.h) over a fixed set
+// of kinds, with EventTriggerInfo's settings (Event.h:196-258: rising
+// and/or falling sign transitions, the localization window in units of the
+// time scale, default 0.1) and a fixed set of handler actions:
+//   WIT_Q            q[index] - offset                       (Stage::Position)
+//   WIT_U            u[index] - offset                       (Stage::Velocity)
+//   WIT_STATION_POS  dir . p_G(station of body) - offset     (Stage::Position)
+//   WIT_STATION_VEL  dir . v_G(station of body) - offset     (Stage::Velocity)
+//   WIT_TIME         t - offset                              (Stage::Time)
+//   EV_REPORT     a reporter: the event is recorded, the state is not changed
+//   EV_SCALE_U    u[aindex] *= factor (every u when aindex < 0): impacts with
+//                 restitution (factor -e), energy injection
+//   EV_TERMINATE  the handler sets shouldTerminate
+enum WitnessKind { WIT_Q = 0, WIT_U = 1, WIT_STATION_POS = 2, WIT_STATION_VEL = 3, WIT_TIME = 4 };
+enum { TRIG_FALLING = 1, TRIG_RISING = 2 };    // Event::Trigger, Event.h:159-167
+enum EventAction { EV_REPORT = 0, EV_SCALE_U = 1, EV_TERMINATE = 2 };
+struct WitnessDef {
+    int kind, index, body, trigger, action, aindex;
+    double station[3], dir[3], offset, factor, window;
+};
+// One triggered event: the localization window (tLow, tHigh], the witness
+// and the transition reported (TRIG_FALLING or TRIG_RISING,
+// EventTriggerInfo::calcTransitionToReport).
+struct EventRecord { double tLow, tHigh; int witness, transition; };
+constexpr double kSignificantReal = 1.9323057602939027e-14;    // eps^(7/8)
+constexpr int kMaxLocalizeIters = 400;
+
+// SimTK::sign and Event::classifyTransition (Event.h:177-189): no event
+// when the sign is unchanged or leaves zero; a transition to zero counts
+// as a full sign change.
+inline int signOf(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+inline int classifyTransition(int before, int after) {
+    if (before == after || before == 0) return 0;
+    return before == 1 ? TRIG_FALLING : TRIG_RISING;
+}
+// estimateRootTime (IntegratorRep.h:351-376): the secant estimate with the
+// caller's bias, bisection when an end value is exactly zero or the
+// interval is at most minWindow, kept a buffer zone of max(10% of the
+// interval, minWindow/2) away from either end.
+inline double estimateRootTime(double tLow, double fLow, double tHigh, double fHigh, double bias,
+                               double minWindow) {
+    const double h = tHigh - tLow;
+    if (fLow == 0 || fHigh == 0 || h <= minWindow) return tLow + h/2;
+    const double x = fHigh/(fHigh - bias*fLow);
+    double tRoot = tHigh - x*h;
+    const double buf = std::max(0.1*h, minWindow/2);
+    tRoot = std::max(tRoot, tLow + buf);
+    tRoot = std::min(tRoot, tHigh - buf);
+    return tRoot;
+}
+// findEventCandidates (IntegratorRep.h:390-443): the witnesses (all, or
+// those of `viable`) whose sign transition over (tLow, tHigh] passes their
+// mask; each one's root estimate and reported transition, the earliest
+// estimate and the narrowest required window max(min(acc * timescale *
+// window), minWindow).
+struct EventCandidates {
+    std::vector<int> idx, trans;
+    std::vector<double> est;
+    double earliest = 0, narrowest = 0;
+};
+inline void findEventCandidates(const std::vector<WitnessDef>& W, const std::vector<int>* viable, double tLow,
+                                const double* eLow, double tHigh, const double* eHigh, double bias,
+                                double minWindow, double acc, double timescale, EventCandidates& out) {
+    out.idx.clear(); out.trans.clear(); out.est.clear();
+    out.earliest = out.narrowest = std::numeric_limits<double>::infinity();
+    const int n = viable ? (int)viable->size() : (int)W.size();
+    for (int i = 0; i < n; ++i) {
+        const int e = viable ? (*viable)[i] : i;
+        const int seen = classifyTransition(signOf(eLow[e]), signOf(eHigh[e])) & W[e].trigger;
+        if (!seen) continue;
+        out.idx.push_back(e);
+        out.trans.push_back((seen & TRIG_RISING) ? TRIG_RISING : TRIG_FALLING);
+        out.narrowest = std::max(std::min(out.narrowest, acc*timescale*W[e].window), minWindow);
+        out.est.push_back(estimateRootTime(tLow, eLow[e], tHigh, eHigh[e], bias, minWindow));
+        out.earliest = std::min(out.earliest, out.est.back());
+    }
+}
 
 template <class T> struct Rkm {
     const Model& M;
@@ -1677,6 +1762,126 @@ template <class T> struct Rkm {
         return ok;
     }
 
+    // ---- event witnesses (null or empty: none) --------------------------
+    const std::vector<WitnessDef>* wit = nullptr;
+    std::vector<EventRecord>* events = nullptr;      // nullable: receives every triggered event
+    std::vector<double> e0, eLow, eHigh, eMid;       // witness values: step start, window ends, trial
+    std::vector<T> evF1, evY, wqd;
+    // The witness values at (t, y); the station kinds realize the position
+    // (and velocity) kinematics of y.
+    void witnessValues(double t, const T* y, double* e) {
+        bool pos = false, vel = false;
+        for (const WitnessDef& w : *wit) {
+            pos |= w.kind == WIT_STATION_POS || w.kind == WIT_STATION_VEL;
+            vel |= w.kind == WIT_STATION_VEL;
+        }
+        if (pos) realizePosition(tr, y);
+        if (vel) { wqd.resize(M.nq); realizeVelocity(tr, y, y + M.nq, wqd.data()); }
+        for (size_t i = 0; i < wit->size(); ++i) {
+            const WitnessDef& w = (*wit)[i];
+            double v;
+            if (w.kind == WIT_Q) v = double(y[w.index]);
+            else if (w.kind == WIT_U) v = double(y[M.nq + w.index]);
+            else if (w.kind == WIT_TIME) v = t;
+            else {
+                const Xform<T>& X = tr.B[w.body].X_GB;
+                const V3<T> r = X.R * mk3<T>(T(w.station[0]), T(w.station[1]), T(w.station[2]));
+                const V3<T> p = w.kind == WIT_STATION_POS ? X.p + r : stationVel(tr.B[w.body].V_GB, r);
+                v = double((T(w.dir[0])*p[0] + T(w.dir[1])*p[1]) + T(w.dir[2])*p[2]);
+            }
+            e[i] = v - w.offset;
+        }
+    }
+    // createInterpolatedState (AbstractIntegratorRep.cpp:63-90) between the
+    // step's start (t0, y0, f0) and its end (t1, y1, evF1): the cubic Hermite
+    // interpolant (interpolateOrder3, IntegratorRep.h:299-313) at t, then the
+    // LocalOnly projection (quaternions normalized when their RMS norm error
+    // exceeds the constraint tolerance).
+    void interpLocal(double t0, double t1, double t, double consTol, T* yt) {
+        const int ny = M.nq + M.nu;
+        const T h = T(t1 - t0), d = T((t - t0) / (t1 - t0));
+        const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
+        const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
+        for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*evF1[i];
+        if (M.nquat > 0 && double(quatErrRMS(yt)) > consTol) normalizeQuats(yt, (T*)nullptr);
+    }
+    // Events of the step just accepted from (t, y0, f0) to (t1, y1)
+    // (takeOneStep's trigger search and localization, AbstractIntegratorRep.
+    // cpp:583-756; the handlers as TimeStepper::stepTo calls them,
+    // TimeStepper.cpp:178-205; Integrator.cpp:437-447).  e0 holds the
+    // witnesses at the step's start.  The candidates over the whole step
+    // decide which events can trigger; the window (tLow, tHigh] shrinks
+    // around the earliest by trial states interpolated inside the step until
+    // it is at most the narrowest required window; the advanced state is
+    // then moved back to tHigh by interpolation, every remaining candidate is
+    // recorded and its handler acts on that state.  A terminating handler
+    // that changes nothing leaves the state interpolated at tLow (the state
+    // ReachedEventTrigger reports); one that changes the state leaves it at
+    // tHigh.  e0 then holds the witnesses of the state the next step starts
+    // from.  Returns 0 (no event), 1 (events handled), 2 (terminate) or -1
+    // (localization failed: no candidate in either half, or no convergence in
+    // kMaxLocalizeIters trials).
+    int stepEvents(double t, double& t1, const SV<T>* Fapp, const RkmConfig& cfg, double consTol) {
+        const std::vector<WitnessDef>& W = *wit;
+        const int nw = (int)W.size(), ny = M.nq + M.nu;
+        const double acc = cfg.accuracy, ts = cfg.timescale;
+        eHigh.resize(nw); eLow.resize(nw); eMid.resize(nw);
+        witnessValues(t1, y1.data(), eHigh.data());
+        const double minWindow = kSignificantReal * std::max(1.0, t1);
+        EventCandidates C, N;
+        findEventCandidates(W, nullptr, t, e0.data(), t1, eHigh.data(), 1.0, minWindow, acc, ts, C);
+        if (C.idx.empty()) { e0 = eHigh; return 0; }
+        double tLow = t, tHigh = t1;
+        evF1.resize(ny); evY.resize(ny);
+        if (tHigh - tLow > C.narrowest) {
+            derivsAt(t1, y1.data(), Fapp, evF1.data());
+            eLow = e0;
+            double bias = 1;
+            int side2 = 0, side1 = 0, it = 0;      // the half chosen two trials ago and at the last one
+            do {
+                if (++it > kMaxLocalizeIters) return -1;
+                if (side2 != 0 && side1 != 0) bias = side2 != side1 ? 1.0 : (side1 < 0 ? bias/2 : bias*2);
+                const double tMid = C.earliest;
+                interpLocal(t, t1, tMid, consTol, evY.data());
+                witnessValues(tMid, evY.data(), eMid.data());
+                findEventCandidates(W, &C.idx, tLow, eLow.data(), tMid, eMid.data(), bias, minWindow, acc, ts, N);
+                if (!N.idx.empty()) {
+                    side2 = side1; side1 = -1;
+                    tHigh = tMid; eHigh = eMid; C = N;
+                    continue;
+                }
+                findEventCandidates(W, &C.idx, tMid, eMid.data(), tHigh, eHigh.data(), bias, minWindow, acc, ts, N);
+                if (N.idx.empty()) return -1;
+                side2 = side1; side1 = 1;
+                tLow = tMid; eLow = eMid; C = N;
+            } while (tHigh - tLow > C.narrowest);
+            if (tHigh < t1) {
+                interpLocal(t, t1, tHigh, consTol, evY.data());
+                for (int i=0;i<ny;++i) y1[i] = evY[i];
+                t1 = tHigh;
+            }
+        }
+        bool modified = false, term = false;
+        for (size_t k = 0; k < C.idx.size(); ++k) {
+            const WitnessDef& w = W[C.idx[k]];
+            if (events) events->push_back({tLow, tHigh, C.idx[k], C.trans[k]});
+            if (w.action == EV_SCALE_U) {
+                for (int i = 0; i < M.nu; ++i)
+                    if (w.aindex < 0 || w.aindex == i) y1[M.nq + i] = T(w.factor) * y1[M.nq + i];
+                modified = true;
+            } else if (w.action == EV_TERMINATE) term = true;
+        }
+        if (term && !modified) {
+            derivsAt(t1, y1.data(), Fapp, evF1.data());
+            interpLocal(t, t1, tLow, consTol, evY.data());
+            for (int i=0;i<ny;++i) y1[i] = evY[i];
+            t1 = tLow;
+        }
+        if (term) return 2;
+        witnessValues(t1, y1.data(), e0.data());
+        return 1;
+    }
+
     static bool isFiniteD(double x) { return std::isfinite(x); }
 
     // Integrate y (q then u) from t0 to T.  Returns stats.
@@ -1730,6 +1935,9 @@ template <class T> struct Rkm {
         }
         for (int i=0;i<ny;++i) y0[i] = y[i];
         double t = t0;
+        st.t = t0;
+        const bool hasWit = wit && !wit->empty();
+        if (hasWit) { e0.resize(wit->size()); witnessValues(t0, y, e0.data()); }
         double hcur = (hinit != -1) ? hinit : cfg.timescale/10;
         if (hmin != -1) hcur = std::max(hcur, hmin);
         if (hmax != -1) hcur = std::min(hcur, hmax);
@@ -1803,6 +2011,17 @@ template <class T> struct Rkm {
                 if (trace) trace->push_back({t, t1, errPre, qErr, errNorm, projected ? 1.0 : 0.0,
                                              succeeded ? 1.0 : 0.0, hnew});
             } while (!succeeded);
+            if (hasWit) {
+                const int r = stepEvents(t, t1, Fapp, cfg, consTol);
+                if (r < 0) { st.status = 6; return st; }
+                if (r == 2) {
+                    t = t1;
+                    for (int i=0;i<ny;++i) y[i] = y1[i];
+                    ++st.accepted;
+                    st.terminated = 1; st.t = t;
+                    return st;
+                }
+            }
             if (kr < nrep && reports[kr] <= t1) {
                 std::vector<T> f1(ny);
                 derivsAt(t1, y1.data(), Fapp, f1.data());
@@ -1820,6 +2039,7 @@ template <class T> struct Rkm {
             for (int i=0;i<ny;++i) y[i] = y1[i];
             ++st.accepted;
         }
+        st.t = t;
         return st;
     }
 };

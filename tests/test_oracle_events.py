@@ -1,0 +1,242 @@
+"""Event witnesses in the oracle (CPU): trigger search, localization and
+handlers of takeOneStep (AbstractIntegratorRep.cpp:583-756) with
+estimateRootTime and findEventCandidates (IntegratorRep.h:351-443), the
+trigger classification of Event.h:177-258 and the handler protocol of
+TimeStepper::stepTo (TimeStepper.cpp:178-205, Integrator.cpp:437-447),
+pinned by what the mechanics and the algorithm's own bounds fix:
+- a ball on a slider under gravity moves on parabolas that RK-Merson and
+  the Hermite interpolant reproduce exactly, so every bounce's window
+  (tLow, tHigh] must hold the exact crossing of the parabola that the
+  previous bounce (restitution at that event's tHigh) started, be no wider
+  than accuracy * time scale * window, and the final state must be the
+  exact parabola's;
+- a pendulum released from the horizontal crosses the vertical at odd
+  multiples of a quarter period and turns at multiples of a half period,
+  with the period of the elliptic integral 4 sqrt(I/(m g L)) K(1/2); the
+  transitions alternate and the trigger masks keep the wanted ones;
+- multiplying every u by sqrt(1.2) at each bottom crossing multiplies the
+  energy above the bottom by 1.2 each time;
+- a terminating handler that changes nothing leaves the state at tLow (the
+  exact parabola's state there, still above the floor), one that also
+  scales u leaves it at tHigh;
+- a witness that starts at exactly zero is no event (a transition away
+  from zero), and estimateRootTime's secant, bisection and buffer zone
+  follow from the formula's cases."""
+import numpy as np
+import pytest
+from scipy.special import ellipk
+
+from simbody_amd import models as M
+import oracle_ctypes as O
+from oracle_ctypes import OracleModel
+
+G = 9.8
+
+
+def slider(m=2.0):
+    """One slider body along x, gravity along -x: x'' = -G."""
+    return M.TreeSpec(parent=np.array([0], np.int32), mobilizer=np.array([M.SLIDER], np.int32),
+                      mass=np.array([m]), com_B=np.zeros((1, 3)), inertia_B=np.array([[0.1, 0.1, 0.1, 0, 0, 0]]),
+                      X_PF=np.array([M._xform()]), X_BM=np.array([M._xform()]), gravity=np.array([-G, 0.0, 0.0]))
+
+
+def pendulum(m=10.0, L=1.0, Ic=1e-3):
+    """A Pin about z with its mass center at (L, 0, 0) of the body, gravity
+    along -y; q = 0 is horizontal.  Returns (spec, I about the pin)."""
+    I = m * L * L + Ic
+    spec = M.TreeSpec(parent=np.array([0], np.int32), mobilizer=np.array([M.PIN], np.int32),
+                      mass=np.array([m]), com_B=np.array([[L, 0.0, 0.0]]),
+                      inertia_B=np.array([[Ic, I, I, 0, 0, 0]]),
+                      X_PF=np.array([M._xform()]), X_BM=np.array([M._xform()]), gravity=np.array([0.0, -G, 0.0]))
+    return spec, I
+
+
+def period(I, m=10.0, L=1.0):
+    """Period of the pendulum released at rest from the horizontal."""
+    return 4.0 * np.sqrt(I / (m * G * L)) * ellipk(0.5)
+
+
+def _parabola(x0, v0, dt):
+    return x0 + v0 * dt - 0.5 * G * dt * dt, v0 - G * dt
+
+
+def _crossing(x0, v0):
+    """First time after 0 at which x0 + v0 t - G t^2/2 falls through zero (x0 <= 0 or x0 > 0)."""
+    return (v0 + np.sqrt(v0 * v0 + 2 * G * x0)) / G
+
+
+@pytest.mark.parametrize("acc", [1e-3, 1e-6])
+def test_bouncing_ball_windows_hold_exact_crossings(acc):
+    e, T = 0.8, 2.5
+    o = OracleModel(slider())
+    assert o.set_witnesses([dict(kind=O.WIT_Q, index=0, trigger=O.TRIG_FALLING, action=O.EV_SCALE_U,
+                                 action_index=0, factor=-e)]) == 0
+    y, st = o.integrate(np.array([1.0, 0.0]), T, acc, max_h=0.05)
+    assert st["status"] == 0
+    win, ids, tf, term = o.last_events()
+    assert tf == T and not term
+    assert len(win) >= 4
+    assert np.all(ids[:, 0] == 0) and np.all(ids[:, 1] == O.TRIG_FALLING)
+    x, v, t = 1.0, 0.0, 0.0
+    for (lo, hi) in win:
+        tc = t + _crossing(x, v)
+        assert lo - 1e-12 < tc <= hi + 1e-12, (lo, tc, hi)
+        assert 0 < hi - lo <= acc * 0.1 * 0.1
+        x, v = _parabola(x, v, hi - t)       # the state the handler saw, at tHigh
+        assert x <= 0
+        v, t = -e * v, hi
+    xT, vT = _parabola(x, v, T - t)
+    assert abs(y[0] - xT) < 1e-9 and abs(y[1] - vT) < 1e-9
+
+
+def test_estimate_root_time_cases():
+    f = O.estimate_root_time
+    # linear function with the root inside the buffer zone: the secant is exact
+    assert f(0.0, -1.0, 1.0, 3.0) == pytest.approx(0.25, abs=1e-15)
+    # bias moves it: x = fH/(fH - bias fL)
+    assert f(0.0, -1.0, 1.0, 3.0, bias=2.0) == pytest.approx(1.0 - 3.0 / 5.0, abs=1e-15)
+    # a root near an end is pushed 10% of the interval away from it
+    assert f(0.0, -1.0, 1.0, 1000.0) == pytest.approx(0.1, abs=1e-15)
+    assert f(2.0, -1000.0, 3.0, 1.0) == pytest.approx(2.9, abs=1e-15)
+    # an end value of exactly zero, or an interval at most minWindow: bisect
+    assert f(0.0, 0.0, 1.0, 3.0) == 0.5
+    assert f(0.0, -1.0, 1.0, 0.0) == 0.5
+    assert f(0.0, -1.0, 1e-15, 3.0, min_window=1e-14) == 0.5e-15
+
+
+def _pend_events(witnesses, T, acc=1e-8, q0=0.0, u0=0.0):
+    spec, I = pendulum()
+    o = OracleModel(spec)
+    assert o.set_witnesses(witnesses) == 0
+    y, st = o.integrate(np.array([q0, u0]), T, acc)
+    assert st["status"] == 0
+    win, ids, tf, term = o.last_events()
+    return o, y, win, ids, period(I)
+
+
+def test_pendulum_crossings_at_quarter_periods():
+    spec, I = pendulum()
+    P = period(I)
+    wits = [dict(kind=O.WIT_STATION_POS, body=1, station=(1.0, 0, 0), direction=(1.0, 0, 0)),     # x of the bob
+            dict(kind=O.WIT_STATION_VEL, body=1, station=(1.0, 0, 0), direction=(0, 1.0, 0)),     # its y velocity
+            dict(kind=O.WIT_U, index=0)]                                                           # the pin's rate
+    o, y, win, ids, _ = _pend_events(wits, 2.2 * P)
+    tc = win.mean(axis=1)
+    for w, times, first in ((0, [P/4, 3*P/4, 5*P/4, 7*P/4], O.TRIG_FALLING),
+                            (1, [P/4, P/2, 3*P/4, P, 5*P/4, 3*P/2, 7*P/4, 2*P], O.TRIG_RISING),
+                            (2, [P/2, P, 3*P/2, 2*P], O.TRIG_RISING)):
+        sel = ids[:, 0] == w
+        assert np.allclose(tc[sel], times, atol=2e-7), (w, tc[sel] - times)
+        # transitions alternate, starting from the physical direction
+        tr = ids[sel, 1]
+        assert tr[0] == first and np.all(tr[1:] != tr[:-1])
+    assert np.all(win[:, 1] - win[:, 0] <= 1e-8 * 0.1 * 0.1)
+    assert np.all(np.diff(win[:, 0]) >= 0)
+    # the zero velocity at the start (u = 0, v_y = 0) is no event
+    assert win[0, 0] > 0.1 * P
+
+
+def test_trigger_masks_keep_the_wanted_transitions():
+    spec, I = pendulum()
+    P = period(I)
+    base = dict(kind=O.WIT_STATION_POS, body=1, station=(1.0, 0, 0), direction=(1.0, 0, 0))
+    _, _, w_f, i_f, _ = _pend_events([dict(base, trigger=O.TRIG_FALLING)], 2.2 * P)
+    _, _, w_r, i_r, _ = _pend_events([dict(base, trigger=O.TRIG_RISING)], 2.2 * P)
+    assert np.allclose(w_f.mean(axis=1), [P/4, 5*P/4], atol=2e-7) and np.all(i_f[:, 1] == O.TRIG_FALLING)
+    assert np.allclose(w_r.mean(axis=1), [3*P/4, 7*P/4], atol=2e-7) and np.all(i_r[:, 1] == O.TRIG_RISING)
+
+
+def test_energy_injection_at_bottom_crossings():
+    """u *= sqrt(1.2) at every bottom crossing: the energy above the bottom
+    (all kinetic there) grows by 1.2 per event (ZeroPositionHandler of the
+    reference's integrator tests, with every crossing accelerated)."""
+    spec, I = pendulum()
+    P = period(I)
+    m, L = 10.0, 1.0
+    w = dict(kind=O.WIT_STATION_POS, body=1, station=(1.0, 0, 0), direction=(1.0, 0, 0),
+             action=O.EV_SCALE_U, action_index=-1, factor=np.sqrt(1.2))
+    o, y, win, ids, _ = _pend_events([w], 1.6 * P)
+    n = len(win)
+    assert n >= 3
+    E = 0.5 * I * y[1] ** 2 + m * G * L * (np.sin(y[0]) + 1.0)
+    assert E == pytest.approx(m * G * L * 1.2 ** n, rel=1e-5)
+
+
+def test_terminate_leaves_state_at_tlow():
+    acc, v0 = 1e-6, 1.0
+    tc = (v0 + np.sqrt(v0 * v0 + 2 * G)) / G
+    o = OracleModel(slider())
+    assert o.set_witnesses([dict(kind=O.WIT_Q, index=0, trigger=O.TRIG_FALLING, action=O.EV_TERMINATE)]) == 0
+    y, st = o.integrate(np.array([1.0, v0]), 5.0, acc, max_h=0.05)
+    win, ids, tf, term = o.last_events()
+    assert st["status"] == 0 and term and len(win) == 1
+    lo, hi = win[0]
+    assert lo - 1e-12 < tc <= hi + 1e-12 and hi - lo <= acc * 0.01
+    assert tf == lo
+    x, v = _parabola(1.0, v0, lo)
+    assert x >= 0 and abs(y[0] - x) < 1e-12 and abs(y[1] - v) < 1e-12
+    # with a handler that also scales u in the same window: the state at tHigh
+    assert o.set_witnesses([dict(kind=O.WIT_Q, index=0, trigger=O.TRIG_FALLING, action=O.EV_SCALE_U,
+                                 action_index=0, factor=-0.5),
+                            dict(kind=O.WIT_Q, index=0, trigger=O.TRIG_FALLING, action=O.EV_TERMINATE)]) == 0
+    y, st = o.integrate(np.array([1.0, v0]), 5.0, acc, max_h=0.05)
+    win, ids, tf, term = o.last_events()
+    assert term and len(win) == 2 and np.all(win[0] == win[1]) and list(ids[:, 0]) == [0, 1]
+    lo, hi = win[0]
+    assert tf == hi
+    x, v = _parabola(1.0, v0, hi)
+    assert x <= 0 and abs(y[0] - x) < 1e-12 and abs(y[1] + 0.5 * v) < 1e-12
+
+
+def test_time_witness_and_window_setting():
+    spec, I = pendulum()
+    for window in (0.1, 0.003):
+        o = OracleModel(spec)
+        assert o.set_witnesses([dict(kind=O.WIT_TIME, offset=0.3217, window=window)]) == 0
+        y, st = o.integrate(np.array([0.0, 0.0]), 0.6, 1e-4)
+        win, ids, tf, term = o.last_events()
+        assert len(win) == 1 and ids[0, 1] == O.TRIG_RISING
+        assert win[0, 0] < 0.3217 <= win[0, 1] and win[0, 1] - win[0, 0] <= 1e-4 * 0.1 * window
+    # the integration went on from tHigh: the end state is the event-free one to the accuracy
+    o2 = OracleModel(spec)
+    y2, _ = o2.integrate(np.array([0.0, 0.0]), 0.6, 1e-4)
+    assert np.allclose(y, y2, atol=1e-3)
+
+
+def test_invalid_witnesses_refused():
+    o = OracleModel(slider())
+    for bad in (dict(kind=7), dict(kind=O.WIT_Q, index=1), dict(kind=O.WIT_U, index=-1),
+                dict(kind=O.WIT_STATION_POS, body=2), dict(kind=O.WIT_Q, trigger=0),
+                dict(kind=O.WIT_Q, action=5), dict(kind=O.WIT_Q, window=0.0),
+                dict(kind=O.WIT_Q, action=O.EV_SCALE_U, action_index=1)):
+        assert o.set_witnesses([bad]) == -1, bad
+
+
+def test_two_valid_roundings_agree_to_the_window():
+    """The oracle and its FMA-contracted build (the rounding freedom a GPU
+    build has) find the same events, step counts and states to rounding on
+    a spatial chain with quaternions and three kinds of witness; a window
+    moves by at most its required width (a trial within rounding of a root
+    may choose the other half)."""
+    spec = M.spatial_chain(nb=5, seed=7, types=(M.BALL, M.PIN, M.UNIVERSAL))
+    spec.gravity = np.array([0.0, -G, 0.0])
+    wits = [dict(kind=O.WIT_STATION_POS, body=5, station=(0.1, 0.2, 0.0), direction=(0, 1.0, 0), offset=-1.0),
+            dict(kind=O.WIT_STATION_VEL, body=3, station=(0.0, 0.3, 0.1), direction=(0.6, 0.0, 0.8),
+                 trigger=O.TRIG_FALLING),
+            dict(kind=O.WIT_U, index=1, action=O.EV_SCALE_U, action_index=2, factor=0.9)]
+    runs = []
+    for variant in (None, "fma"):
+        o = OracleModel(spec, variant=variant)
+        assert o.set_witnesses(wits) == 0
+        out = []
+        for i in range(8):
+            q, u = M.random_state(spec, 11, i, qscale=0.5, uscale=0.5)
+            F = np.ascontiguousarray(M.random_body_forces(spec, 11, i))
+            y, st = o.integrate(np.concatenate([q, u]), 1.5, 1e-6, Fapp=F)
+            out.append((y, st) + o.last_events()[:2])
+        runs.append(out)
+    for (y0, s0, w0, i0), (y1, s1, w1, i1) in zip(*runs):
+        assert len(w0) == len(w1) > 0 and np.array_equal(i0, i1)
+        assert (s0["accepted"], s0["rejected"]) == (s1["accepted"], s1["rejected"])
+        assert np.abs(w0 - w1).max() <= 1e-6 * 0.1 * 0.1
+        assert np.abs(y0 - y1).max() < 1e-10 * np.abs(y0).max()
