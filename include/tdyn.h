@@ -261,6 +261,87 @@ int tdyn_batch_set_constraints(tdyn_batch* b, int tree, int n, const tdyn_constr
 int tdyn_calc_constrained_acceleration(tdyn_batch* b, const double* q, const double* u, const double* F,
                                        double* udot, double* lambda, hipStream_t s);
 
+/* Event witnesses of a tree: the trigger functions of TriggeredEventHandler
+ * and TriggeredEventReporter (EventHandler.h, EventReporter.h; getValue)
+ * over a fixed set of kinds, with EventTriggerInfo's settings (Event.h:
+ * 196-258) and a fixed set of handler actions (handleEvent):
+ *   TDYN_WIT_Q            q[index] - offset (Stage::Position)
+ *   TDYN_WIT_U            u[index] - offset (Stage::Velocity)
+ *   TDYN_WIT_STATION_POS  direction . (position in Ground of `station` on
+ *                         `body`) - offset (Stage::Position)
+ *   TDYN_WIT_STATION_VEL  direction . (its velocity in Ground) - offset
+ *                         (Stage::Velocity)
+ *   TDYN_WIT_TIME         t - offset (Stage::Time)
+ *   TDYN_EVENT_REPORT     a reporter: the event is recorded, the state is
+ *                         not changed
+ *   TDYN_EVENT_SCALE_U    u[action_index] *= factor (every u of the
+ *                         instance when action_index < 0): impacts with
+ *                         restitution (factor -e), energy injection
+ *   TDYN_EVENT_TERMINATE  the handler sets shouldTerminate: the instance
+ *                         stops with status 3
+ * index is the coordinate within the tree; bodies are 1..nb (0 = Ground). */
+#define TDYN_WIT_Q            0
+#define TDYN_WIT_U            1
+#define TDYN_WIT_STATION_POS  2
+#define TDYN_WIT_STATION_VEL  3
+#define TDYN_WIT_TIME         4
+#define TDYN_TRIGGER_FALLING  1   /* Event::Falling: positive to negative or zero */
+#define TDYN_TRIGGER_RISING   2   /* Event::Rising: negative to positive or zero */
+#define TDYN_EVENT_REPORT     0
+#define TDYN_EVENT_SCALE_U    1
+#define TDYN_EVENT_TERMINATE  2
+#define TDYN_MAX_WITNESSES    8   /* per tree */
+typedef struct tdyn_witness_desc {
+    int32_t kind;              /* TDYN_WIT_* */
+    int32_t index;             /* TDYN_WIT_Q, _U: the coordinate */
+    int32_t body;              /* TDYN_WIT_STATION_*: the body */
+    int32_t trigger;           /* TDYN_TRIGGER_FALLING | TDYN_TRIGGER_RISING (both: the reference's default) */
+    int32_t action;            /* TDYN_EVENT_* */
+    int32_t action_index;      /* TDYN_EVENT_SCALE_U: the u scaled, < 0 every u */
+    double station[3];         /* in the body frame */
+    double direction[3];       /* in Ground */
+    double offset;             /* subtracted from the value */
+    double factor;             /* TDYN_EVENT_SCALE_U */
+    double window;             /* required localization window in units of accuracy * time scale
+                                  (EventTriggerInfo::setRequiredLocalizationTimeWindow; the
+                                  reference's default is 0.1) */
+} tdyn_witness_desc;
+/* Sets the witnesses of tree `tree` (replacing earlier ones; n = 0 clears),
+ * as adding TriggeredEventHandler / TriggeredEventReporter objects to the
+ * System does.  Rollouts then watch them after every accepted step as
+ * takeOneStep does (AbstractIntegratorRep.cpp:583-756): the witnesses that
+ * change sign across the step under their trigger masks are the candidates
+ * (a sign change away from zero is not one, Event.h:177-189); the window
+ * (tLow, tHigh] shrinks around the earliest by trial states interpolated
+ * inside the step (cubic Hermite, then the quaternion projection) and the
+ * secant / bisection estimates of estimateRootTime (IntegratorRep.h:
+ * 351-376) until it is at most the narrowest required window (accuracy *
+ * time scale * window, at least SignificantReal * max(1, t)); the state moves
+ * back to tHigh by interpolation, every remaining candidate is recorded
+ * (tdyn_rollout_get_events) and its handler acts on that state, in witness
+ * order; integration continues from it with the step size it had
+ * (TimeStepper.cpp:178-205, Integrator.cpp:437-447).  A terminating handler
+ * that changes nothing leaves the instance at tLow, in the state
+ * interpolated there (the state ReachedEventTrigger reports); one that
+ * changes the state leaves it at tHigh.  As in the reference, a witness
+ * that changes sign twice within one step is not seen (bound the step size
+ * for such witnesses).  Rollouts with witnesses need a batch of one tree
+ * and run the tree sweeps in FP64 (evaluator 0 or 1), without known udot,
+ * prescribed motion, constraints, reports or tdyn_rollout_noalloc; status
+ * -5 marks an instance whose localization failed.  A rollout in progress
+ * ends (begin again). */
+int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w);
+/* Events recorded per instance per rollout (default 64; later events are
+ * counted, not recorded).  A rollout in progress ends (begin again). */
+int tdyn_batch_set_event_capacity(tdyn_batch* b, int capacity);
+/* The events of the current rollout (device outputs, each nullable):
+ * windows [n_instances][capacity][2] (tLow, tHigh), ids
+ * [n_instances][capacity][2] (witness, transition: TDYN_TRIGGER_FALLING or
+ * TDYN_TRIGGER_RISING as EventTriggerInfo::calcTransitionToReport gives
+ * it), counts [n_instances] (events seen; entries past the capacity are not
+ * recorded), in the order they happened. */
+int tdyn_rollout_get_events(tdyn_batch* b, double* windows, int32_t* ids, int32_t* counts, hipStream_t s);
+
 /* multiplyByMInv(state, v, MinvV) (SimbodyMatterSubsystem.h:1343-1345,
  * .cpp:431-471). */
 int tdyn_multiply_by_m_inv(tdyn_batch* b, const double* q, const double* v, double* MinvV,
@@ -397,7 +478,7 @@ int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const double* fapp
                          const tdyn_integrator_cfg* cfg, int32_t* stats, hipStream_t s);
 
 /* Copy out state (packed q, u), per-instance time t, status (0 running,
- * 1 reached t_end, <0 failed) and stats (attempts, accepted, rejected,
+ * 1 reached t_end, 3 an event handler terminated it, <0 failed) and stats (attempts, accepted, rejected,
  * projections; 4 ints per instance).  Any output may be NULL. */
 int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* status,
                      int32_t* stats, hipStream_t s);

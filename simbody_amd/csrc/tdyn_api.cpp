@@ -195,6 +195,31 @@ int ensure_constraints(tdyn_batch* b) {
     return TDYN_OK;
 }
 
+int ensure_witnesses(tdyn_batch* b) {
+    if (!b->wit_dirty) return TDYN_OK;
+    std::vector<int4> wt, wa;
+    std::vector<double> wp;
+    for (size_t t = 0; t < b->h_wit.size(); ++t) {
+        int need = 0;
+        for (const tdyn_witness_desc& w : b->h_wit[t]) {
+            wa.push_back(make_int4(w.kind | (w.trigger << 4) | (w.action << 8), w.index, w.body, w.action_index));
+            for (int k = 0; k < 3; ++k) wp.push_back(w.station[k]);
+            for (int k = 0; k < 3; ++k) wp.push_back(w.direction[k]);
+            wp.push_back(w.offset); wp.push_back(w.factor); wp.push_back(w.window);
+            if (w.kind == TDYN_WIT_STATION_POS) need = std::max(need, 1);
+            if (w.kind == TDYN_WIT_STATION_VEL) need = 2;
+        }
+        wt.push_back(make_int4((int)wa.size() - (int)b->h_wit[t].size(), (int)b->h_wit[t].size(), need, 0));
+    }
+    if (wa.empty()) { wa.push_back(make_int4(0, 0, 0, 0)); wp.assign(9, 0.0); }
+    hipError_t e;
+    if ((e = b->wit_t.upload(wt)) != hipSuccess || (e = b->wit_a.upload(wa)) != hipSuccess ||
+        (e = b->wit_p.upload(wp)) != hipSuccess)
+        return hipfail(e, "witness tables upload");
+    b->wit_dirty = false;
+    return TDYN_OK;
+}
+
 int ensure_rollout_buffers(tdyn_batch* b, int precision) {
     const int ny = b->max_nq + b->max_nu;
     const size_t L = b->n_inst;
@@ -507,6 +532,59 @@ int tdyn_batch_set_constraints(tdyn_batch* b, int tree, int n, const tdyn_constr
     return ensure_constraints(b);
 }
 
+int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (tree < 0 || tree >= b->n_trees) return fail(TDYN_E_ARG, "tdyn_batch_set_witnesses: tree out of range");
+    if (n < 0 || (n > 0 && !w)) return fail(TDYN_E_ARG, "tdyn_batch_set_witnesses: n >= 0 descriptions required");
+    if (n > TDYN_MAX_WITNESSES) return fail(TDYN_E_SIZE, "tdyn_batch_set_witnesses: more than TDYN_MAX_WITNESSES");
+    const int4 ta = b->h_tree_a[tree];
+    const int nb = ta.x, nq = ta.y, nu = ta.z;
+    for (int i = 0; i < n; ++i) {
+        const tdyn_witness_desc& d = w[i];
+        const std::string at = "tdyn_batch_set_witnesses: witness " + std::to_string(i) + ": ";
+        if (d.kind < TDYN_WIT_Q || d.kind > TDYN_WIT_TIME) return fail(TDYN_E_ARG, at + "unknown kind");
+        if (d.kind == TDYN_WIT_Q && (d.index < 0 || d.index >= nq)) return fail(TDYN_E_ARG, at + "no such q");
+        if (d.kind == TDYN_WIT_U && (d.index < 0 || d.index >= nu)) return fail(TDYN_E_ARG, at + "no such u");
+        if ((d.kind == TDYN_WIT_STATION_POS || d.kind == TDYN_WIT_STATION_VEL) && (d.body < 0 || d.body > nb))
+            return fail(TDYN_E_ARG, at + "no such body (0 = Ground)");
+        if (d.trigger < 1 || d.trigger > 3)
+            return fail(TDYN_E_ARG, at + "trigger must be TDYN_TRIGGER_FALLING and/or TDYN_TRIGGER_RISING");
+        if (d.action < TDYN_EVENT_REPORT || d.action > TDYN_EVENT_TERMINATE)
+            return fail(TDYN_E_ARG, at + "unknown action");
+        if (d.action == TDYN_EVENT_SCALE_U && d.action_index >= nu) return fail(TDYN_E_ARG, at + "no such u to scale");
+        const double v[11] = {d.station[0], d.station[1], d.station[2], d.direction[0], d.direction[1],
+                              d.direction[2], d.offset, d.factor, d.window, 0.0, 0.0};
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(v[k])) return fail(TDYN_E_ARG, at + "values must be finite");
+        if (!(d.window > 0)) return fail(TDYN_E_ARG, at + "the localization window must be positive");
+    }
+    if (b->h_wit.size() != (size_t)b->n_trees) b->h_wit.resize(b->n_trees);
+    b->h_wit[tree].assign(w, w + n);
+    b->any_wit = false;
+    for (const auto& v : b->h_wit) b->any_wit = b->any_wit || !v.empty();
+    b->wit_dirty = true;
+    b->rollout_ready = false;          // a rollout in progress was begun without them
+    return ensure_witnesses(b);
+}
+
+int tdyn_batch_set_event_capacity(tdyn_batch* b, int capacity) {
+    if (!b) return fail(TDYN_E_ARG, "null batch");
+    if (capacity < 1 || capacity > (1 << 20)) return fail(TDYN_E_ARG, "tdyn_batch_set_event_capacity: 1 to 2^20");
+    b->ev_cap = capacity;
+    b->rollout_ready = false;
+    return TDYN_OK;
+}
+
+int tdyn_rollout_get_events(tdyn_batch* b, double* windows, int32_t* ids, int32_t* counts, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_get_events: no rollout");
+    if (!b->any_wit || !b->ev_count.p) return fail(TDYN_E_STAGE, "tdyn_rollout_get_events: the rollout has no witnesses");
+    const size_t L = b->n_inst, E = L * (size_t)b->ev_cap * 2;
+    if (windows) HIPCHK(hipMemcpyAsync(windows, b->ev_win.p, E * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (ids) HIPCHK(hipMemcpyAsync(ids, b->ev_ids.p, E * sizeof(int), hipMemcpyDeviceToDevice, s));
+    if (counts) HIPCHK(hipMemcpyAsync(counts, b->ev_count.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
+    return TDYN_OK;
+}
+
 int tdyn_calc_constrained_acceleration(tdyn_batch* b, const double* q, const double* u, const double* F,
                                        double* udot, double* lambda, hipStream_t s) {
     if (!b || !q || !u || !udot) return fail(TDYN_E_ARG, "realize(Acceleration): batch, q, u and udot are required");
@@ -738,9 +816,14 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     if (b->any_cons && !b->uniform)
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: constraints in rollouts need a batch of one tree (a batch of "
                                 "different trees takes tdyn_calc_constrained_acceleration only)");
+    if (b->any_wit && (b->known_p || b->motion || b->any_cons || b->evaluator == 2 || b->evaluator == 3 ||
+                       prec != 0 || !b->uniform))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: event witnesses run on the tree sweeps (evaluator 0 or 1) of a "
+                                "batch of one tree, in FP64, without known udot, prescribed motion or constraints");
     { const int rc = ensure_constraints(b); if (rc) return rc; }
+    { const int rc = ensure_witnesses(b); if (rc) return rc; }
     const size_t L = b->n_inst;
-    const bool aos = !b->motion && !b->any_cons && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) &&
+    const bool aos = !b->motion && !b->any_cons && !b->any_wit && b->stream_ok && (b->evaluator == 0 || b->evaluator >= 3) &&
                      !(b->known_p && b->tree_lanes == 1);
     const int grp = aos && b->tree_lanes != 1 ? 1 : 0;
     // From here on the batch changes: a failure (allocation, upload) ends the
@@ -751,6 +834,14 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         if (rc) return rc;
     }
     { const int rc = ensure_rollout_buffers(b, prec); if (rc) return rc; }
+    if (b->any_wit) {
+        const size_t E = L * (size_t)b->ev_cap * 2;
+        if (b->ev_ws.n != (size_t)tdyn::EW_N * L) HIPCHK(b->ev_ws.alloc((size_t)tdyn::EW_N * L));
+        if (b->ev_win.n != E) HIPCHK(b->ev_win.alloc(E));
+        if (b->ev_ids.n != E) HIPCHK(b->ev_ids.alloc(E));
+        if (b->ev_count.n != L) HIPCHK(b->ev_count.alloc(L));
+        HIPCHK(hipMemsetAsync(b->ev_count.p, 0, L * sizeof(int), s));
+    }
     if (fapp) {
         if (b->fapp.n != (size_t)b->total_b * 6) HIPCHK(b->fapp.alloc((size_t)b->total_b * 6));
         HIPCHK(hipMemcpyAsync(b->fapp.p, fapp, (size_t)b->total_b * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -840,6 +931,9 @@ int tdyn_rollout_reports(tdyn_batch* b, double* q, double* u, const double* fapp
     if (b->any_cons)
         return fail(TDYN_E_ARG, "tdyn_rollout_reports: not with constraints (interpolated states would need "
                                 "their own projection)");
+    if (b->any_wit)
+        return fail(TDYN_E_ARG, "tdyn_rollout_reports: not with event witnesses (a report time inside an event's "
+                                "window changes the localization)");
     std::vector<double> rep(report_times, report_times + n_reports);
     for (int k = 0; k < n_reports; ++k)
         if (!std::isfinite(rep[k]) || rep[k] < t0 || rep[k] > t_end || (k && !(rep[k] > rep[k-1])))

@@ -109,8 +109,9 @@ int tdyn_batch_autotune(tdyn_batch* b, const double* q, const double* u, const d
     if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_batch_autotune: batch, q, u and cfg are required");
     if (!(t_probe > 0)) return fail(TDYN_E_ARG, "tdyn_batch_autotune: t_probe must be positive");
     std::vector<Candidate> cands;
-    if (b->motion || b->any_cons) {
-        // prescribed motion and constraints run on the tree sweeps only
+    if (b->motion || b->any_cons || b->any_wit) {
+        // prescribed motion, constraints and event witnesses run on the tree
+        // sweeps only
         cands.push_back({1, 0, 0, 32});
     } else if (!b->stream_ok) {
         // one model for the batch (chain evaluator or tree sweeps): nothing

@@ -113,6 +113,15 @@ struct tdyn_batch {
     tdyn::host::DBuf<int4> cons_t, cons_a;
     tdyn::host::DBuf<double> cons_p, cons_ws;
     bool any_cons = false, cons_dirty = false;
+    // event witnesses (tdyn_batch_set_witnesses): per tree, their device
+    // tables (uploaded when changed), and the rollout's event workspace and
+    // log (allocated by tdyn_rollout_begin)
+    std::vector<std::vector<tdyn_witness_desc>> h_wit;
+    tdyn::host::DBuf<int4> wit_t, wit_a;
+    tdyn::host::DBuf<double> wit_p, ev_ws, ev_win;
+    tdyn::host::DBuf<int> ev_ids, ev_count;
+    int ev_cap = 64;
+    bool any_wit = false, wit_dirty = false;
     int plan_waves = 0;
     RkmCfg cfg{};
 
@@ -122,6 +131,7 @@ struct tdyn_batch {
         T.tree_a = tree_a.p; T.tree_b = tree_b.p; T.gravity = gravity.p; T.pre = pre.p; T.bfs = bfs.p; T.lvl = lvl.p;
         T.bfs_b = bfs_b.p; T.prm_bfs = prm_bfs.p;
         T.cons_t = any_cons ? cons_t.p : nullptr; T.cons_a = cons_a.p; T.cons_p = cons_p.p;
+        T.wit_t = any_wit ? wit_t.p : nullptr; T.wit_a = wit_a.p; T.wit_p = wit_p.p;
         return T;
     }
     OpArgs op() const {
@@ -141,9 +151,11 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (!roll_known_p && !roll_motion && !any_cons &&
+        a.chain = (!roll_known_p && !roll_motion && !any_cons && !any_wit &&
                    (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
         a.cons_ws = any_cons ? cons_ws.p : nullptr;
+        if (any_wit) { a.ev_ws = ev_ws.p; a.ev_win = ev_win.p; a.ev_ids = ev_ids.p; a.ev_count = ev_count.p; }
+        a.ev_cap = ev_cap;
         a.presc = roll_known_p; a.udot_known = roll_known_u;
         if (roll_motion) { a.motion = roll_motion; a.presc = motion_flags.p; a.udot_known = motion_udot.p; }
         a.idf = (a.chain ? idf : idf_all) ? 1 : 0;
@@ -170,5 +182,7 @@ int ensure_rollout_buffers(tdyn_batch* b, int precision);
 // Uploads the constraint tables and allocates the per-lane workspace when
 // the constraints changed.
 int ensure_constraints(tdyn_batch* b);
+// Uploads the witness tables when the witnesses changed.
+int ensure_witnesses(tdyn_batch* b);
 } // namespace host
 } // namespace tdyn

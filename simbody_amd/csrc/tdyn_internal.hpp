@@ -76,6 +76,17 @@ constexpr int CONS_MAXR = 12;
 enum { CONS_BALL = 0, CONS_ROD = 1 };
 enum { CWS_W = 0, CWS_F = CONS_MAXR * CONS_MAXR, CWS_R = 2 * CONS_MAXR * CONS_MAXR, CWS_Y = CWS_R + CONS_MAXR,
        CWS_LAM = CWS_Y + CONS_MAXR, CWS_N = CWS_LAM + CONS_MAXR };
+// Event witnesses (at most WIT_MAX per tree): kinds, trigger masks (bit 0
+// falling, bit 1 rising) and handler actions as in include/tdyn.h.  The
+// per-lane event workspace (doubles, lane-strided) holds the witness values
+// at the step's start, at the two ends of the localization window and at
+// the trial time.
+constexpr int WIT_MAX = 8;
+enum { WIT_Q = 0, WIT_U = 1, WIT_STATION_POS = 2, WIT_STATION_VEL = 3, WIT_TIME = 4 };
+enum { EVA_REPORT = 0, EVA_SCALE_U = 1, EVA_TERMINATE = 2 };
+enum { EW_E0 = 0, EW_LOW = WIT_MAX, EW_HIGH = 2 * WIT_MAX, EW_MID = 3 * WIT_MAX, EW_N = 4 * WIT_MAX };
+// Rollout status of a lane whose event handler asked to terminate.
+constexpr int ST_TERMINATED = 3;
 
 struct DevTables {
     const int4* body_a;      // per global body: parent(local), type, qIndex, uIndex
@@ -96,6 +107,11 @@ struct DevTables {
     const int4* cons_t;      // per tree (nullable: no constraints): first constraint, count, rows, -
     const int4* cons_a;      // per constraint: kind, body 1, body 2, -
     const double* cons_p;    // per constraint: station on body 1, station on body 2, rod length (7)
+    const int4* wit_t;       // per tree (nullable: no witnesses): first witness, count, kinematics needed
+                             // (0 none, 1 positions, 2 velocities), -
+    const int4* wit_a;       // per witness: kind | trigger << 4 | action << 8, coordinate index, body,
+                             // action index
+    const double* wit_p;     // per witness: station (3), direction (3), offset, factor, window (9)
 };
 
 // Boundary layout offsets of each instance (instance-major packing).
@@ -163,7 +179,8 @@ struct RkmArgs {
     double* t;               // per lane
     double* h;               // per lane: current step size
     const double* t_end;     // per lane
-    int* status;             // per lane: 0 running, 1 done, 2 empty serving slot, <0 failed
+    int* status;             // per lane: 0 running, 1 done, 2 empty serving slot, 3 terminated by an event
+                             // handler, <0 failed
     int* stats;              // per lane: attempts accepted rejected projections
     double* uscale;          // SoA nu_max*L
     RkmCfg cfg;
@@ -197,6 +214,12 @@ struct RkmArgs {
                              // in doubles; the lane's tree is tree `lane` of the tables
     double* cons_ws;         // holonomic constraints (nullable: none in the batch): per-lane workspace
                              // (CWS_N doubles, lane-strided); the tree sweeps only
+    double* ev_ws;           // event witnesses (nullable: none): per-lane workspace (EW_N doubles,
+                             // lane-strided); the tree sweeps of batches of one tree only
+    double* ev_win;          // per lane, ev_cap events: localization window (tLow, tHigh)
+    int* ev_ids;             // per lane, ev_cap events: witness, transition (1 falling, 2 rising)
+    int* ev_count;           // per lane: events seen since the rollout began (may exceed ev_cap)
+    int ev_cap;
 };
 // Per-lane scratch of the streaming tree evaluator (doubles): ST (24 per
 // depth: R p V A of the last body with two or more children seen at that

@@ -1600,6 +1600,92 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // be resumed by the next launch (the serving loop relies on this).
 // ---------------------------------------------------------------------------
 
+// ---- event witnesses ----------------------------------------------------
+// The trigger functions of TriggeredEventHandler / TriggeredEventReporter
+// (EventHandler.h, EventReporter.h) over a fixed set of kinds, their
+// EventTriggerInfo masks and windows (Event.h:196-258), and the handler
+// actions (tdyn.h, TDYN_WIT_* and TDYN_EVENT_*).  Witness i's values live in
+// the lane's event workspace at EW_E0 + i (the step's start) and in three
+// rotating slots (window ends and trial).
+struct Wit {
+    const int4* wa; const double* wp; int n, need;
+    double* ws; int L;
+    __device__ __forceinline__ double& w(int k) const { return ws[(size_t)k * L]; }
+};
+__device__ __forceinline__ Wit make_wit(const DevTables& T, int tr, double* ws, int lane, int L) {
+    Wit W{nullptr, nullptr, 0, 0, nullptr, L};
+    if (!T.wit_t || !ws) return W;
+    const int4 t = ldi4(T.wit_t, tr);
+    W.wa = T.wit_a + t.x; W.wp = T.wit_p + 9*(size_t)t.x; W.n = t.y; W.need = t.z; W.ws = ws + lane;
+    return W;
+}
+// Witness values at (t, y) into ws[out + i]; the station kinds read the
+// records of y's kinematics (positions; velocities too when a velocity
+// witness needs them).
+template <bool PAD, class YR>
+__device__ void wit_values(const Lane& c, const Wit& W, double t, const YR& yr, int out) {
+    const SoAGet q{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
+    if (W.need == 2) sweep_kin<true, PAD>(c, q, u, NoPut{});
+    else if (W.need == 1) sweep_kin<false, PAD>(c, q, u, NoPut{});
+    for (int i = 0; i < W.n; ++i) {
+        const int4 wa = W.wa[i];
+        const double* p = W.wp + 9*i;
+        const int kind = wa.x & 15;
+        double v;
+        if (kind == WIT_Q) v = q(wa.y);
+        else if (kind == WIT_U) v = u(wa.y);
+        else if (kind == WIT_TIME) v = t;
+        else {
+            dm R = ident<double>();
+            d3 o = zero3<double>();
+            if (wa.z > 0) { const int rb = c.rec(wa.z); R = ldm(c, rb + REC_R); o = ld3(c, rb + REC_P); }
+            const d3 r = mul(R, V3(p[0], p[1], p[2]));
+            const d3 x = kind == WIT_STATION_POS ? o + r : st_vel(rec_sv(c, wa.z, REC_V), r);
+            v = (p[3]*x.x + p[4]*x.y) + p[5]*x.z;
+        }
+        W.w(out + i) = v - p[6];
+    }
+}
+__device__ __forceinline__ int sign_of(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+// estimateRootTime (IntegratorRep.h:351-376): secant with bias, bisection
+// when an end value is exactly zero or the interval is at most minWindow,
+// kept max(10% of the interval, minWindow/2) away from either end.
+__device__ __forceinline__ double root_time(double tLow, double fLow, double tHigh, double fHigh, double bias,
+                                            double minWindow) {
+    const double h = tHigh - tLow;
+    if (fLow == 0 || fHigh == 0 || h <= minWindow) return tLow + h/2;
+    const double x = fHigh/(fHigh - bias*fLow);
+    double tr = tHigh - x*h;
+    const double buf = fmax(0.1*h, minWindow/2);
+    tr = fmax(tr, tLow + buf);
+    tr = fmin(tr, tHigh - buf);
+    return tr;
+}
+// findEventCandidates (IntegratorRep.h:390-443) over the witnesses of the
+// mask `viable`, window ends at workspace slots lo and hi: the candidate
+// mask (Event::classifyTransition, Event.h:177-189, under each witness's
+// trigger mask), the candidates reported rising, the earliest root estimate
+// and the narrowest required window max(min(acc*timescale*window), minWindow).
+__device__ unsigned find_cands(const Wit& W, unsigned viable, double tLow, int lo, double tHigh, int hi,
+                               double bias, double minWindow, double accTs, unsigned& rising, double& earliest,
+                               double& narrowest) {
+    unsigned cand = 0;
+    rising = 0;
+    earliest = narrowest = __builtin_inf();
+    for (int i = 0; i < W.n; ++i) {
+        if (!((viable >> i) & 1u)) continue;
+        const double el = W.w(lo + i), eh = W.w(hi + i);
+        const int sl = sign_of(el), sh = sign_of(eh);
+        const int seen = ((sl == sh || sl == 0) ? 0 : (sl == 1 ? 1 : 2)) & ((W.wa[i].x >> 4) & 3);
+        if (!seen) continue;
+        cand |= 1u << i;
+        if (seen & 2) rising |= 1u << i;
+        narrowest = fmax(fmin(narrowest, accTs * W.wp[9*i + 8]), minWindow);
+        earliest = fmin(earliest, root_time(tLow, el, tHigh, eh, bias, minWindow));
+    }
+    return cand;
+}
+
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 template <bool PAD>
 struct TreeEval {
@@ -1916,9 +2002,136 @@ __device__ void prescribe_lane(const RkmArgs& a, int lane, int tr, double t, con
     }
 }
 
+// Events of the step just accepted from (t, Y, F0) to (t1, Y1)
+// (AbstractIntegratorRep.cpp:583-756; handlers as TimeStepper::stepTo calls
+// them, TimeStepper.cpp:178-205, Integrator.cpp:437-447).  The candidates
+// over the whole step decide which events can trigger; the window (tLow,
+// tHigh] shrinks around the earliest estimate by trial states interpolated
+// inside the step (createInterpolatedState: the cubic Hermite interpolant,
+// IntegratorRep.h:299-313, then the LocalOnly projection, quaternions
+// normalized when their RMS norm error exceeds the constraint tolerance)
+// until it is at most the narrowest required window; the advanced state
+// moves back to tHigh by interpolation; every remaining candidate is
+// recorded and its handler acts on that state.  A terminating handler that
+// changes nothing leaves the state interpolated at tLow (the state
+// ReachedEventTrigger reports), one that changes the state leaves it at
+// tHigh.  The start-of-step values at EW_E0 then belong to the state the
+// next step starts from.  F1 and YT are free rows.  Returns 0 (no event), 1
+// (handled), 2 (terminate) or -1 (no candidate in either half of a trial, or
+// no convergence within 400 trials).
+template <class EV, class TRows>
+__device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lane, int nq, int nu, int nb,
+                           int nquat, const int4* body_a, const double* prm_tree, double t, double& t1,
+                           const TRows& Y, const TRows& F0, const TRows& Y1, const TRows& F1, const TRows& YT) {
+    const int ny = nq + nu;
+    const double consTol = a.cfg.cons_tol, accTs = a.cfg.accuracy * a.cfg.timescale;
+    int lo = EW_E0, hi = EW_HIGH, mid = EW_LOW;
+    wit_values<EV::kPad>(ev.c, W, t1, Y1, hi);
+    const double minWindow = 1.9323057602939027e-14 * fmax(1.0, t1);   // SignificantReal * max(1, t)
+    unsigned rising;
+    double earliest, narrowest;
+    unsigned cand = find_cands(W, ~0u, t, lo, t1, hi, 1.0, minWindow, accTs, rising, earliest, narrowest);
+    if (!cand) {
+        for (int i = 0; i < W.n; ++i) W.w(EW_E0 + i) = W.w(hi + i);
+        return 0;
+    }
+    // the interpolant at tm of (t, Y, F0) and (te, Y1, F1) into out, projected
+    auto interp = [&](double te, double tm, const TRows& out) {
+        const double h = te - t, d = (tm - t) / (te - t);
+        const double cy1 = d*d*(3.0 - 2.0*d), cy0 = 1.0 - cy1;
+        const double hdd1 = h*d*(d - 1.0), cf1 = hdd1*d, cf0 = cf1 - hdd1;
+        for (int k = 0; k < ny; ++k) out(k) = ((cy0*Y(k) + cy1*Y1(k)) + cf0*F0(k)) + cf1*F1(k);
+        if (nquat == 0) return;
+        double s2 = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int4 ba = ldi4(body_a, b);
+            if (ba.y != BALL && (ba.y != FREE || prm_tree[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
+            double n2 = 0;
+            for (int k = 0; k < 4; ++k) { const double x = out(ba.z + k); n2 += x*x; }
+            const double e = sqrt(n2) - 1.0;
+            s2 += e*e;
+        }
+        if (!(sqrt(s2 / nquat) > consTol)) return;
+        for (int b = 0; b < nb; ++b) {
+            const int4 ba = ldi4(body_a, b);
+            if (ba.y != BALL && (ba.y != FREE || prm_tree[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
+            double n2 = 0;
+            for (int k = 0; k < 4; ++k) { const double x = out(ba.z + k); n2 += x*x; }
+            const double n = sqrt(n2);
+            for (int k = 0; k < 4; ++k) out(ba.z + k) = out(ba.z + k) / n;
+        }
+    };
+    double tLow = t, tHigh = t1;
+    const double t1o = t1;
+    if (tHigh - tLow > narrowest) {
+        ev.derivs(Y1, F1);
+        double bias = 1.0;
+        int side2 = 0, side1 = 0, it = 0;     // the half chosen two trials ago and at the last one
+        do {
+            if (++it > 400) return -1;
+            if (side2 != 0 && side1 != 0) bias = side2 != side1 ? 1.0 : (side1 < 0 ? bias/2 : bias*2);
+            const double tMid = earliest;
+            interp(t1o, tMid, YT);
+            wit_values<EV::kPad>(ev.c, W, tMid, YT, mid);
+            unsigned r2;
+            double e2, n2;
+            unsigned nc = find_cands(W, cand, tLow, lo, tMid, mid, bias, minWindow, accTs, r2, e2, n2);
+            if (nc) {
+                side2 = side1; side1 = -1;
+                tHigh = tMid;
+                const int x = hi; hi = mid; mid = x;
+            } else {
+                nc = find_cands(W, cand, tMid, mid, tHigh, hi, bias, minWindow, accTs, r2, e2, n2);
+                if (!nc) return -1;
+                side2 = side1; side1 = 1;
+                tLow = tMid;
+                const int x = lo; lo = mid; mid = x;
+            }
+            cand = nc; rising = r2; earliest = e2; narrowest = n2;
+        } while (tHigh - tLow > narrowest);
+        if (tHigh < t1o) {
+            interp(t1o, tHigh, YT);
+            for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
+            t1 = tHigh;
+        }
+    }
+    bool modified = false, term = false;
+    int cnt = a.ev_count[lane];
+    for (int i = 0; i < W.n; ++i) {
+        if (!((cand >> i) & 1u)) continue;
+        if (cnt < a.ev_cap) {
+            const size_t e = (size_t)lane * a.ev_cap + cnt;
+            a.ev_win[2*e] = tLow; a.ev_win[2*e + 1] = tHigh;
+            a.ev_ids[2*e] = i; a.ev_ids[2*e + 1] = ((rising >> i) & 1u) ? 2 : 1;
+        }
+        ++cnt;
+        const int act = (W.wa[i].x >> 8) & 15;
+        if (act == EVA_SCALE_U) {
+            const int ai = W.wa[i].w;
+            const double f = W.wp[9*i + 7];
+            for (int k = 0; k < nu; ++k)
+                if (ai < 0 || ai == k) Y1(nq + k) = f * Y1(nq + k);
+            modified = true;
+        } else if (act == EVA_TERMINATE) {
+            term = true;
+        }
+    }
+    a.ev_count[lane] = cnt;
+    if (term && !modified) {
+        ev.derivs(Y1, F1);
+        interp(t1, tLow, YT);
+        for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
+        t1 = tLow;
+    }
+    if (term) return 2;
+    wit_values<EV::kPad>(ev.c, W, t1, Y1, EW_E0);
+    return 1;
+}
+
 // UNI: every instance uses tree 0, so the tree tables are read with
-// wave-uniform (scalar) loads.
-template <class EV, bool UNI>
+// wave-uniform (scalar) loads.  EVT: the tree's event witnesses are watched
+// (a batch of one tree on the tree sweeps).
+template <class EV, bool UNI, bool EVT = false>
 __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
@@ -1946,6 +2159,13 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     int attempts = a.stats[4*lane], accepted = a.stats[4*lane+1], rejected = a.stats[4*lane+2],
         projections = a.stats[4*lane+3];
     int status = 0, steps = 0;
+    Wit W{nullptr, nullptr, 0, 0, nullptr, L};
+    if constexpr (EVT) {
+        // the witnesses at the state the launch starts from (the values the
+        // previous launch left are the same numbers)
+        W = make_wit(a.T, tr, a.ev_ws, lane, L);
+        if (W.n) wit_values<EV::kPad>(ev.c, W, t, Y, EW_E0);
+    }
 
     // error norm at y1: q part weighted by N NInv for quaternions (IntegratorRep.h:
     // 454-523), u part relative to y0 (calcRelativeScaling, :644-655)
@@ -2111,6 +2331,20 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
             if (!ok) ++rejected;
         } while (!ok);
         if (status < 0) break;
+        if constexpr (EVT) {
+            if (W.n) {
+                const int r = step_events(a, ev, W, lane, nq, nu, nb, nquat, body_a,
+                                          a.T.prm + (size_t)ta.w * PRM_STRIDE, t, t1, Y, F0, Y1, FB, YT);
+                if (r < 0) { status = -5; break; }
+                if (r == 2) {
+                    t = t1;
+                    for (int k = 0; k < ny; ++k) Y(k) = Y1(k);
+                    ++accepted; ++steps;
+                    status = ST_TERMINATED;
+                    break;
+                }
+            }
+        }
         t = t1;
 #pragma unroll 8
         for (int k = 0; k < ny; ++k) Y(k) = Y1(k);
@@ -2374,6 +2608,12 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.pool) { k_rkm<TreePoolEval, false><<<grd, blk, 0, s>>>(a); return hipGetLastError(); }
     if (a.chain) return launch_chain_variant(a, s, false);
     if (a.grp) return launch_tree_group(a, s, false);
+    if (a.ev_ws) {
+        // event witnesses: the tree sweeps of a batch of one tree
+        if (a.vec_aos || a.tree_of) return hipErrorInvalidValue;
+        k_rkm<TreeEval<false>, true, true><<<grd, blk, 0, s>>>(a);
+        return hipGetLastError();
+    }
     if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
     else if (a.tree_of) k_rkm<TreeEval<true>, false><<<grd, blk, 0, s>>>(a);
     else k_rkm<TreeEval<false>, true><<<grd, blk, 0, s>>>(a);

@@ -83,6 +83,7 @@ extern "C" int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const d
                                                    "sweeps (evaluator 0, 1 or 2), not the level-parallel evaluator");
     if (b->motion) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: prescribed motion runs on tdyn_rollout_begin only");
     if (b->any_cons) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: constraints run on tdyn_rollout_begin only");
+    if (b->any_wit) return fail(TDYN_E_ARG, "tdyn_rollout_noalloc: event witnesses run on tdyn_rollout_begin only");
     const int L = b->n_inst;
     const int ny = b->max_nq + b->max_nu;
     const int rec = b->max_rec + 24;

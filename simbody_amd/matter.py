@@ -34,6 +34,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_calc_acceleration_ignoring_constraints", "tdyn_calc_acceleration_with_known_udot",
            "tdyn_batch_set_known_udot", "tdyn_batch_set_motion", "tdyn_batch_autotune",
            "tdyn_batch_set_constraints", "tdyn_calc_constrained_acceleration",
+           "tdyn_batch_set_witnesses", "tdyn_batch_set_event_capacity", "tdyn_rollout_get_events",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -70,6 +71,26 @@ class ConstraintDescC(ctypes.Structure):
 
 CONS_BALL, CONS_ROD = 0, 1
 MAX_CONSTRAINT_ROWS = 12
+
+
+class WitnessDescC(ctypes.Structure):
+    """tdyn_witness_desc."""
+    _fields_ = [("kind", ctypes.c_int32), ("index", ctypes.c_int32), ("body", ctypes.c_int32),
+                ("trigger", ctypes.c_int32), ("action", ctypes.c_int32), ("action_index", ctypes.c_int32),
+                ("station", ctypes.c_double * 3), ("direction", ctypes.c_double * 3), ("offset", ctypes.c_double),
+                ("factor", ctypes.c_double), ("window", ctypes.c_double)]
+
+
+# Event witnesses: kinds, trigger masks, handler actions (tdyn.h TDYN_WIT_*,
+# TDYN_TRIGGER_*, TDYN_EVENT_*), and the rollout status of an instance whose
+# handler terminated it.
+WIT_Q, WIT_U, WIT_STATION_POS, WIT_STATION_VEL, WIT_TIME = 0, 1, 2, 3, 4
+TRIGGER_FALLING, TRIGGER_RISING, TRIGGER_BOTH = 1, 2, 3
+EVENT_REPORT, EVENT_SCALE_U, EVENT_TERMINATE = 0, 1, 2
+MAX_WITNESSES = 8
+STATUS_TERMINATED = 3
+WITNESS_DEFAULTS = dict(index=0, body=0, trigger=TRIGGER_BOTH, action=EVENT_REPORT, action_index=-1,
+                        station=(0.0, 0.0, 0.0), direction=(0.0, 0.0, 0.0), offset=0.0, factor=1.0, window=0.1)
 
 
 class IntegratorCfgC(ctypes.Structure):
@@ -113,6 +134,9 @@ def lib():
         L.tdyn_batch_set_motion.argtypes = [vp, vp]
         L.tdyn_batch_set_constraints.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ConstraintDescC)]
         L.tdyn_calc_constrained_acceleration.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.tdyn_batch_set_witnesses.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WitnessDescC)]
+        L.tdyn_batch_set_event_capacity.argtypes = [vp, ctypes.c_int]
+        L.tdyn_rollout_get_events.argtypes = [vp, vp, vp, vp, vp]
         L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
                                           ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
@@ -414,6 +438,54 @@ class MatterBatch:
         _check(lib().tdyn_calc_constrained_acceleration(self.h, _ptr(q), _ptr(u), _ptr(F), _ptr(ud), _ptr(lam),
                                                        _stream()))
         return ud, lam.view(self.n_instances, MAX_CONSTRAINT_ROWS)
+
+    def set_witnesses(self, tree, witnesses):
+        """Event witnesses of tree `tree` (TriggeredEventHandler /
+        TriggeredEventReporter): a list of dicts with `kind` (WIT_*) and any
+        of index, body, trigger (TRIGGER_*), action (EVENT_*), action_index,
+        station, direction, offset, factor, window (WITNESS_DEFAULTS); []
+        clears."""
+        n = len(witnesses)
+        arr = (WitnessDescC * max(n, 1))()
+        for i, w in enumerate(witnesses):
+            w = dict(WITNESS_DEFAULTS, **w)
+            d = arr[i]
+            d.kind, d.index, d.body = int(w["kind"]), int(w["index"]), int(w["body"])
+            d.trigger, d.action, d.action_index = int(w["trigger"]), int(w["action"]), int(w["action_index"])
+            for k in range(3):
+                d.station[k] = float(w["station"][k]); d.direction[k] = float(w["direction"][k])
+            d.offset, d.factor, d.window = float(w["offset"]), float(w["factor"]), float(w["window"])
+        _check(lib().tdyn_batch_set_witnesses(self.h, int(tree), n, arr))
+
+    def set_event_capacity(self, capacity):
+        """Events recorded per instance per rollout (later ones are only counted)."""
+        self._ev_cap = int(capacity)
+        _check(lib().tdyn_batch_set_event_capacity(self.h, self._ev_cap))
+
+    def rollout_events(self):
+        """The current rollout's events: windows [n, capacity, 2] (tLow,
+        tHigh), ids [n, capacity, 2] (witness, transition: TRIGGER_FALLING
+        or TRIGGER_RISING) and counts [n] (events seen)."""
+        torch = self.torch
+        cap = getattr(self, "_ev_cap", 64)
+        n = self.n_instances
+        w = self._new(n * cap * 2)
+        ids = torch.zeros(n * cap * 2, dtype=torch.int32, device=self.device)
+        cnt = torch.zeros(n, dtype=torch.int32, device=self.device)
+        _check(lib().tdyn_rollout_get_events(self.h, _ptr(w), _ptr(ids), _ptr(cnt), _stream()))
+        return w.view(n, cap, 2), ids.view(n, cap, 2), cnt
+
+    def stepToWithEvents(self, q, u, integ, t_end, fapp=None, t0=0.0):
+        """TimeStepper::stepTo(t_end) with the batch's event witnesses:
+        returns a dict of q, u, t, status (1 reached t_end, STATUS_TERMINATED
+        a handler ended it), stats [n, 4], windows, ids, counts
+        (rollout_events)."""
+        self.rollout_begin(q, u, integ, t_end, fapp=fapp, t0=t0)
+        q1, u1, t1, st, stats = self._run_to_end()
+        if bool((st < 0).any()):
+            raise TdynError(5, "Integrator::StepFailed for at least one instance")
+        w, ids, cnt = self.rollout_events()
+        return dict(q=q1, u=u1, t=t1, status=st, stats=stats, windows=w, ids=ids, counts=cnt)
 
     def set_motion(self, motion=None):
         """Prescribed motion for the following rollouts (Motion::Sinusoid,
