@@ -125,6 +125,12 @@ public:
         check(tdyn_calc_constrained_acceleration(h_, q, u, appliedBodyForces, udot, multipliers, s_));
         if (throwOnSingular_) checkSingular();
     }
+    // TriggeredEventHandler / TriggeredEventReporter witnesses of tree `tree`
+    // (tdyn_batch_set_witnesses) and the events recorded per instance.
+    void setWitnesses(int tree, int n, const tdyn_witness_desc* w) const {
+        check(tdyn_batch_set_witnesses(h_, tree, n, w));
+    }
+    void setEventCapacity(int capacity) const { check(tdyn_batch_set_event_capacity(h_, capacity)); }
     void multiplyByM(const double* q, const double* v, double* Mv) const {
         check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
     }
@@ -164,6 +170,11 @@ public:
     void extendTo(const double* tEnd, double tEndAll) const { check(tdyn_rollout_extend(h_, tEnd, tEndAll, s_)); }
     void getState(double* q, double* u, double* t, int32_t* status, int32_t* stats) const {
         check(tdyn_rollout_get(h_, q, u, t, status, stats, s_));
+    }
+    // the events of the resumable form's rollout: windows (tLow, tHigh), ids
+    // (witness, transition) and counts per instance
+    void getEvents(double* windows, int32_t* ids, int32_t* counts) const {
+        check(tdyn_rollout_get_events(h_, windows, ids, counts, s_));
     }
 
 private:
