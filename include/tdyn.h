@@ -331,6 +331,12 @@ typedef struct tdyn_witness_desc {
  * -5 marks an instance whose localization failed.  A rollout in progress
  * ends (begin again). */
 int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w);
+/* The witnesses' values at a state (realize(state) to their stages, then
+ * State::getEventTriggers, State.h): q, u (packed), t (one time per
+ * instance, nullable: 0) -> values [n_instances][TDYN_MAX_WITNESSES] (device;
+ * zero past the tree's witnesses).  Any batch (different trees too). */
+int tdyn_calc_event_triggers(tdyn_batch* b, const double* q, const double* u, const double* t, double* values,
+                             hipStream_t s);
 /* Events recorded per instance per rollout (default 64; later events are
  * counted, not recorded).  A rollout in progress ends (begin again). */
 int tdyn_batch_set_event_capacity(tdyn_batch* b, int capacity);

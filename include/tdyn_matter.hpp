@@ -131,6 +131,10 @@ public:
         check(tdyn_batch_set_witnesses(h_, tree, n, w));
     }
     void setEventCapacity(int capacity) const { check(tdyn_batch_set_event_capacity(h_, capacity)); }
+    // State::getEventTriggers after realize: TDYN_MAX_WITNESSES per instance
+    void calcEventTriggers(const double* q, const double* u, const double* t, double* values) const {
+        check(tdyn_calc_event_triggers(h_, q, u, t, values, s_));
+    }
     void multiplyByM(const double* q, const double* v, double* Mv) const {
         check(tdyn_multiply_by_m(h_, q, v, Mv, s_));
     }

@@ -567,6 +567,21 @@ int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_
     return ensure_witnesses(b);
 }
 
+int tdyn_calc_event_triggers(tdyn_batch* b, const double* q, const double* u, const double* t, double* values,
+                             hipStream_t s) {
+    if (!b || !q || !u || !values) return fail(TDYN_E_ARG, "getEventTriggers: batch, q, u and values are required");
+    { const int rc = ensure_witnesses(b); if (rc) return rc; }
+    const size_t L = b->n_inst;
+    if (!b->any_wit) {
+        HIPCHK(hipMemsetAsync(values, 0, L * TDYN_MAX_WITNESSES * sizeof(double), s));
+        return TDYN_OK;
+    }
+    OpArgs a = b->op();
+    a.q = q; a.u = u; a.times = t; a.out_wit = values;
+    HIPCHK(launch_op(OP_WIT, a, s));
+    return TDYN_OK;
+}
+
 int tdyn_batch_set_event_capacity(tdyn_batch* b, int capacity) {
     if (!b) return fail(TDYN_E_ARG, "null batch");
     if (capacity < 1 || capacity > (1 << 20)) return fail(TDYN_E_ARG, "tdyn_batch_set_event_capacity: 1 to 2^20");

@@ -133,7 +133,8 @@ struct RkmCfg {
 
 // Launch entry points (tdyn_kernels.hip).
 enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6,
-              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11, OP_FWD_PRES = 12, OP_FWD_CONS = 13 };
+              OP_MULM = 7, OP_JAC = 8, OP_JACT = 9, OP_BACC = 10, OP_CBI = 11, OP_FWD_PRES = 12, OP_FWD_CONS = 13,
+              OP_WIT = 14 };
 
 struct OpArgs {
     DevTables T;
@@ -162,6 +163,8 @@ struct OpArgs {
     double* out_tau;         // OP_FWD_PRES: tau on the known mobilities, 0 elsewhere (packed u layout)
     double* cons_ws;         // OP_FWD_CONS: per-lane constraint workspace (CWS_N doubles, lane-strided)
     double* out_lam;         // OP_FWD_CONS: multipliers, CONS_MAXR per instance (nullable)
+    const double* times;     // OP_WIT: time per instance (nullable: 0)
+    double* out_wit;         // OP_WIT: witness values, WIT_MAX per instance
 };
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s);
 

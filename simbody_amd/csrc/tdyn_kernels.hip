@@ -1327,6 +1327,98 @@ __device__ bool cons_project_u(const Lane& c, const Cons& K, const QR& q, const 
     return true;
 }
 
+// ---- event witnesses ----------------------------------------------------
+// The trigger functions of TriggeredEventHandler / TriggeredEventReporter
+// (EventHandler.h, EventReporter.h) over a fixed set of kinds, their
+// EventTriggerInfo masks and windows (Event.h:196-258), and the handler
+// actions (tdyn.h, TDYN_WIT_* and TDYN_EVENT_*).  Witness i's values live in
+// the lane's event workspace at EW_E0 + i (the step's start) and in three
+// rotating slots (window ends and trial).
+struct Wit {
+    const int4* wa; const double* wp; int n, need;
+    double* ws; int L;
+    __device__ __forceinline__ double& w(int k) const { return ws[(size_t)k * L]; }
+};
+// ws (nullable: the single-shot operator, which keeps no workspace).
+__device__ __forceinline__ Wit make_wit(const DevTables& T, int tr, double* ws, int lane, int L) {
+    Wit W{nullptr, nullptr, 0, 0, nullptr, L};
+    if (!T.wit_t) return W;
+    const int4 t = ldi4(T.wit_t, tr);
+    W.wa = T.wit_a + t.x; W.wp = T.wit_p + 9*(size_t)t.x; W.n = t.y; W.need = t.z; W.ws = ws ? ws + lane : nullptr;
+    return W;
+}
+// Witness values at (t, q, u) to out(i, value); the station kinds read the
+// records of the kinematics of q, u (positions; velocities too when a
+// velocity witness needs them).
+template <bool PAD, class QG, class UG, class OUT>
+__device__ void wit_eval(const Lane& c, const Wit& W, double t, const QG& q, const UG& u, const OUT& out) {
+    if (W.need == 2) sweep_kin<true, PAD>(c, q, u, NoPut{});
+    else if (W.need == 1) sweep_kin<false, PAD>(c, q, u, NoPut{});
+    for (int i = 0; i < W.n; ++i) {
+        const int4 wa = W.wa[i];
+        const double* p = W.wp + 9*i;
+        const int kind = wa.x & 15;
+        double v;
+        if (kind == WIT_Q) v = q(wa.y);
+        else if (kind == WIT_U) v = u(wa.y);
+        else if (kind == WIT_TIME) v = t;
+        else {
+            dm R = ident<double>();
+            d3 o = zero3<double>();
+            if (wa.z > 0) { const int rb = c.rec(wa.z); R = ldm(c, rb + REC_R); o = ld3(c, rb + REC_P); }
+            const d3 r = mul(R, V3(p[0], p[1], p[2]));
+            const d3 x = kind == WIT_STATION_POS ? o + r : st_vel(rec_sv(c, wa.z, REC_V), r);
+            v = (p[3]*x.x + p[4]*x.y) + p[5]*x.z;
+        }
+        out(i, v - p[6]);
+    }
+}
+// The same at the state rows yr, into the lane's workspace slots out + i.
+template <bool PAD, class YR>
+__device__ void wit_values(const Lane& c, const Wit& W, double t, const YR& yr, int out) {
+    const SoAGet q{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
+    wit_eval<PAD>(c, W, t, q, u, [&](int i, double v) { W.w(out + i) = v; });
+}
+__device__ __forceinline__ int sign_of(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+// estimateRootTime (IntegratorRep.h:351-376): secant with bias, bisection
+// when an end value is exactly zero or the interval is at most minWindow,
+// kept max(10% of the interval, minWindow/2) away from either end.
+__device__ __forceinline__ double root_time(double tLow, double fLow, double tHigh, double fHigh, double bias,
+                                            double minWindow) {
+    const double h = tHigh - tLow;
+    if (fLow == 0 || fHigh == 0 || h <= minWindow) return tLow + h/2;
+    const double x = fHigh/(fHigh - bias*fLow);
+    double tr = tHigh - x*h;
+    const double buf = fmax(0.1*h, minWindow/2);
+    tr = fmax(tr, tLow + buf);
+    tr = fmin(tr, tHigh - buf);
+    return tr;
+}
+// findEventCandidates (IntegratorRep.h:390-443) over the witnesses of the
+// mask `viable`, window ends at workspace slots lo and hi: the candidate
+// mask (Event::classifyTransition, Event.h:177-189, under each witness's
+// trigger mask), the candidates reported rising, the earliest root estimate
+// and the narrowest required window max(min(acc*timescale*window), minWindow).
+__device__ unsigned find_cands(const Wit& W, unsigned viable, double tLow, int lo, double tHigh, int hi,
+                               double bias, double minWindow, double accTs, unsigned& rising, double& earliest,
+                               double& narrowest) {
+    unsigned cand = 0;
+    rising = 0;
+    earliest = narrowest = __builtin_inf();
+    for (int i = 0; i < W.n; ++i) {
+        if (!((viable >> i) & 1u)) continue;
+        const double el = W.w(lo + i), eh = W.w(hi + i);
+        const int sl = sign_of(el), sh = sign_of(eh);
+        const int seen = ((sl == sh || sl == 0) ? 0 : (sl == 1 ? 1 : 2)) & ((W.wa[i].x >> 4) & 3);
+        if (!seen) continue;
+        cand |= 1u << i;
+        if (seen & 2) rising |= 1u << i;
+        narrowest = fmax(fmin(narrowest, accTs * W.wp[9*i + 8]), minWindow);
+        earliest = fmin(earliest, root_time(tLow, el, tHigh, eh, bias, minWindow));
+    }
+    return cand;
+}
+
 // ---------------------------------------------------------------------------
 // Operator kernels (the batched SimbodyMatterSubsystem operators).
 // ---------------------------------------------------------------------------
@@ -1390,6 +1482,14 @@ __global__ void __launch_bounds__(64) k_op(OpArgs a) {
                 o[0]=A.w.x; o[1]=A.w.y; o[2]=A.w.z; o[3]=A.v.x; o[4]=A.v.y; o[5]=A.v.z;
             }
         }
+    } else if constexpr (OP == OP_WIT) {
+        // realize to the witnesses' stages and read the event triggers
+        // (State::getEventTriggers); zero past the tree's witnesses
+        const Wit W = make_wit(a.T, tr, nullptr, lane, a.L);
+        double* o = a.out_wit + (size_t)WIT_MAX * (a.inst ? a.inst[lane] : lane);
+        const double t = a.times ? a.times[a.inst ? a.inst[lane] : lane] : 0.0;
+        for (int i = 0; i < WIT_MAX; ++i) o[i] = 0.0;
+        wit_eval<PAD>(c, W, t, q, u, [&](int i, double v) { o[i] = v; });
     } else if constexpr (OP == OP_FWD_CONS) {
         // realize(Acceleration) with constraints: loop forward dynamics
         // (gravity and the applied body forces), udot and the multipliers
@@ -1582,6 +1682,7 @@ static void launch_op_t(OpKind k, const OpArgs& a, hipStream_t s) {
     case OP_CBI:       k_op<OP_CBI, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_FWD_PRES:  k_op<OP_FWD_PRES, UNI><<<grd, blk, 0, s>>>(a); break;
     case OP_FWD_CONS:  k_op<OP_FWD_CONS, UNI><<<grd, blk, 0, s>>>(a); break;
+    case OP_WIT:       k_op<OP_WIT, UNI><<<grd, blk, 0, s>>>(a); break;
     }
 }
 hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
@@ -1599,92 +1700,6 @@ hipError_t launch_op(OpKind k, const OpArgs& a, hipStream_t s) {
 // projection).  Lanes stop after cfg.max_steps accepted steps so a rollout can
 // be resumed by the next launch (the serving loop relies on this).
 // ---------------------------------------------------------------------------
-
-// ---- event witnesses ----------------------------------------------------
-// The trigger functions of TriggeredEventHandler / TriggeredEventReporter
-// (EventHandler.h, EventReporter.h) over a fixed set of kinds, their
-// EventTriggerInfo masks and windows (Event.h:196-258), and the handler
-// actions (tdyn.h, TDYN_WIT_* and TDYN_EVENT_*).  Witness i's values live in
-// the lane's event workspace at EW_E0 + i (the step's start) and in three
-// rotating slots (window ends and trial).
-struct Wit {
-    const int4* wa; const double* wp; int n, need;
-    double* ws; int L;
-    __device__ __forceinline__ double& w(int k) const { return ws[(size_t)k * L]; }
-};
-__device__ __forceinline__ Wit make_wit(const DevTables& T, int tr, double* ws, int lane, int L) {
-    Wit W{nullptr, nullptr, 0, 0, nullptr, L};
-    if (!T.wit_t || !ws) return W;
-    const int4 t = ldi4(T.wit_t, tr);
-    W.wa = T.wit_a + t.x; W.wp = T.wit_p + 9*(size_t)t.x; W.n = t.y; W.need = t.z; W.ws = ws + lane;
-    return W;
-}
-// Witness values at (t, y) into ws[out + i]; the station kinds read the
-// records of y's kinematics (positions; velocities too when a velocity
-// witness needs them).
-template <bool PAD, class YR>
-__device__ void wit_values(const Lane& c, const Wit& W, double t, const YR& yr, int out) {
-    const SoAGet q{yr.base, yr.L, yr.lane}, u{yr.base + (size_t)c.nq * yr.L, yr.L, yr.lane};
-    if (W.need == 2) sweep_kin<true, PAD>(c, q, u, NoPut{});
-    else if (W.need == 1) sweep_kin<false, PAD>(c, q, u, NoPut{});
-    for (int i = 0; i < W.n; ++i) {
-        const int4 wa = W.wa[i];
-        const double* p = W.wp + 9*i;
-        const int kind = wa.x & 15;
-        double v;
-        if (kind == WIT_Q) v = q(wa.y);
-        else if (kind == WIT_U) v = u(wa.y);
-        else if (kind == WIT_TIME) v = t;
-        else {
-            dm R = ident<double>();
-            d3 o = zero3<double>();
-            if (wa.z > 0) { const int rb = c.rec(wa.z); R = ldm(c, rb + REC_R); o = ld3(c, rb + REC_P); }
-            const d3 r = mul(R, V3(p[0], p[1], p[2]));
-            const d3 x = kind == WIT_STATION_POS ? o + r : st_vel(rec_sv(c, wa.z, REC_V), r);
-            v = (p[3]*x.x + p[4]*x.y) + p[5]*x.z;
-        }
-        W.w(out + i) = v - p[6];
-    }
-}
-__device__ __forceinline__ int sign_of(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
-// estimateRootTime (IntegratorRep.h:351-376): secant with bias, bisection
-// when an end value is exactly zero or the interval is at most minWindow,
-// kept max(10% of the interval, minWindow/2) away from either end.
-__device__ __forceinline__ double root_time(double tLow, double fLow, double tHigh, double fHigh, double bias,
-                                            double minWindow) {
-    const double h = tHigh - tLow;
-    if (fLow == 0 || fHigh == 0 || h <= minWindow) return tLow + h/2;
-    const double x = fHigh/(fHigh - bias*fLow);
-    double tr = tHigh - x*h;
-    const double buf = fmax(0.1*h, minWindow/2);
-    tr = fmax(tr, tLow + buf);
-    tr = fmin(tr, tHigh - buf);
-    return tr;
-}
-// findEventCandidates (IntegratorRep.h:390-443) over the witnesses of the
-// mask `viable`, window ends at workspace slots lo and hi: the candidate
-// mask (Event::classifyTransition, Event.h:177-189, under each witness's
-// trigger mask), the candidates reported rising, the earliest root estimate
-// and the narrowest required window max(min(acc*timescale*window), minWindow).
-__device__ unsigned find_cands(const Wit& W, unsigned viable, double tLow, int lo, double tHigh, int hi,
-                               double bias, double minWindow, double accTs, unsigned& rising, double& earliest,
-                               double& narrowest) {
-    unsigned cand = 0;
-    rising = 0;
-    earliest = narrowest = __builtin_inf();
-    for (int i = 0; i < W.n; ++i) {
-        if (!((viable >> i) & 1u)) continue;
-        const double el = W.w(lo + i), eh = W.w(hi + i);
-        const int sl = sign_of(el), sh = sign_of(eh);
-        const int seen = ((sl == sh || sl == 0) ? 0 : (sl == 1 ? 1 : 2)) & ((W.wa[i].x >> 4) & 3);
-        if (!seen) continue;
-        cand |= 1u << i;
-        if (seen & 2) rising |= 1u << i;
-        narrowest = fmax(fmin(narrowest, accTs * W.wp[9*i + 8]), minWindow);
-        earliest = fmin(earliest, root_time(tLow, el, tHigh, eh, bias, minWindow));
-    }
-    return cand;
-}
 
 // Derivative evaluators: y (this lane's rows: q then u) -> f (qdot then udot).
 template <bool PAD>

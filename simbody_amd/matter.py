@@ -35,6 +35,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_set_known_udot", "tdyn_batch_set_motion", "tdyn_batch_autotune",
            "tdyn_batch_set_constraints", "tdyn_calc_constrained_acceleration",
            "tdyn_batch_set_witnesses", "tdyn_batch_set_event_capacity", "tdyn_rollout_get_events",
+           "tdyn_calc_event_triggers",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -137,6 +138,7 @@ def lib():
         L.tdyn_batch_set_witnesses.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(WitnessDescC)]
         L.tdyn_batch_set_event_capacity.argtypes = [vp, ctypes.c_int]
         L.tdyn_rollout_get_events.argtypes = [vp, vp, vp, vp, vp]
+        L.tdyn_calc_event_triggers.argtypes = [vp, vp, vp, vp, vp, vp]
         L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
                                           ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
@@ -456,6 +458,13 @@ class MatterBatch:
                 d.station[k] = float(w["station"][k]); d.direction[k] = float(w["direction"][k])
             d.offset, d.factor, d.window = float(w["offset"]), float(w["factor"]), float(w["window"])
         _check(lib().tdyn_batch_set_witnesses(self.h, int(tree), n, arr))
+
+    def calcEventTriggers(self, q, u, t=None):
+        """The witnesses' values at (q, u, t) (State::getEventTriggers after
+        realize): [n_instances, MAX_WITNESSES], zero past each tree's."""
+        out = self._new(MAX_WITNESSES * self.n_instances)
+        _check(lib().tdyn_calc_event_triggers(self.h, _ptr(q), _ptr(u), _ptr(t), _ptr(out), _stream()))
+        return out.view(self.n_instances, MAX_WITNESSES)
 
     def set_event_capacity(self, capacity):
         """Events recorded per instance per rollout (later ones are only counted)."""

@@ -205,3 +205,48 @@ def test_settings_witnesses_cannot_run_with_are_refused():
                 [dict(kind=O.WIT_TIME)] * 9):
         with pytest.raises(TdynError):
             B.set_witnesses(0, bad)
+
+
+def _scaled_diff(a, b):
+    """SimTK-style relative difference: |a - b| / max(|a|, |b|, 1)."""
+    return float(np.max(np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1.0)))
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_event_triggers_match_oracle(shared):
+    """State::getEventTriggers after realize (tdyn_calc_event_triggers) on a
+    batch of different random trees with general joint frames (padded
+    records), and on one shared tree: every witness kind at each instance's
+    own time, 1e-12 against the oracle's witness values."""
+    from simbody_amd.matter import MatterBatch
+    n = 12
+    specs = [M.random_tree(21, k % (1 if shared else 4), nmin=6, nmax=24, general_frames=True)
+             for k in range(1 if shared else 4)]
+    tree_of = np.array([k % len(specs) for k in range(n)], np.int32)
+    r = np.random.default_rng(8)
+    wits = []
+    for s in specs:
+        w = [dict(kind=O.WIT_Q, index=int(r.integers(s.nq)), offset=0.1),
+             dict(kind=O.WIT_U, index=int(r.integers(s.nu)), offset=-0.2),
+             dict(kind=O.WIT_TIME, offset=0.3)]
+        for _ in range(3):
+            w.append(dict(kind=O.WIT_STATION_POS, body=int(r.integers(s.nb + 1)), station=tuple(r.normal(size=3)),
+                          direction=tuple(r.normal(size=3)), offset=float(r.normal())))
+            w.append(dict(kind=O.WIT_STATION_VEL, body=int(r.integers(1, s.nb + 1)),
+                          station=tuple(r.normal(size=3)), direction=tuple(r.normal(size=3))))
+        wits.append(w[:8])
+    B = MatterBatch(specs, tree_of=tree_of)
+    for k, w in enumerate(wits):
+        B.set_witnesses(k, w)
+    qs, us = [], []
+    for i in range(n):
+        q, u = M.random_state(specs[tree_of[i]], 5, i)
+        qs.append(q); us.append(u)
+    ts = r.uniform(0, 1, n)
+    vals = B.calcEventTriggers(B.pack(qs), B.pack(us), B.to_device(ts)).cpu().numpy()
+    for i in range(n):
+        o = OracleModel(specs[tree_of[i]])
+        assert o.set_witnesses(wits[tree_of[i]]) == 0
+        ref = o.witness_values(ts[i], np.concatenate([qs[i], us[i]]), len(wits[tree_of[i]]))
+        assert _scaled_diff(vals[i, :len(ref)], ref) < 1e-12, i
+        assert np.all(vals[i, len(ref):] == 0)
