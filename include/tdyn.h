@@ -325,11 +325,11 @@ typedef struct tdyn_witness_desc {
  * interpolated there (the state ReachedEventTrigger reports); one that
  * changes the state leaves it at tHigh.  As in the reference, a witness
  * that changes sign twice within one step is not seen (bound the step size
- * for such witnesses).  Rollouts with witnesses need a batch of one tree
- * and run the tree sweeps in FP64 (evaluator 0 or 1), without known udot,
- * prescribed motion, constraints, reports or tdyn_rollout_noalloc; status
- * -5 marks an instance whose localization failed.  A rollout in progress
- * ends (begin again). */
+ * for such witnesses).  Rollouts with witnesses run the tree sweeps in
+ * FP64 (evaluator 0 or 1), on batches of one tree or of different trees,
+ * without known udot, prescribed motion, constraints, reports or
+ * tdyn_rollout_noalloc; status -5 marks an instance whose localization
+ * failed.  A rollout in progress ends (begin again). */
 int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w);
 /* The witnesses' values at a state (realize(state) to their stages, then
  * State::getEventTriggers, State.h): q, u (packed), t (one time per

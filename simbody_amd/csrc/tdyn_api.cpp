@@ -594,6 +594,11 @@ int tdyn_rollout_get_events(tdyn_batch* b, double* windows, int32_t* ids, int32_
     if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_get_events: no rollout");
     if (!b->any_wit || !b->ev_count.p) return fail(TDYN_E_STAGE, "tdyn_rollout_get_events: the rollout has no witnesses");
     const size_t L = b->n_inst, E = L * (size_t)b->ev_cap * 2;
+    if (b->permuted) {
+        HIPCHK(launch_event_scatter(b->inst.p, (int)L, b->ev_cap, b->ev_win.p, b->ev_ids.p, b->ev_count.p, windows,
+                                    ids, counts, s));
+        return TDYN_OK;
+    }
     if (windows) HIPCHK(hipMemcpyAsync(windows, b->ev_win.p, E * sizeof(double), hipMemcpyDeviceToDevice, s));
     if (ids) HIPCHK(hipMemcpyAsync(ids, b->ev_ids.p, E * sizeof(int), hipMemcpyDeviceToDevice, s));
     if (counts) HIPCHK(hipMemcpyAsync(counts, b->ev_count.p, L * sizeof(int), hipMemcpyDeviceToDevice, s));
@@ -832,9 +837,9 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: constraints in rollouts need a batch of one tree (a batch of "
                                 "different trees takes tdyn_calc_constrained_acceleration only)");
     if (b->any_wit && (b->known_p || b->motion || b->any_cons || b->evaluator == 2 || b->evaluator == 3 ||
-                       prec != 0 || !b->uniform))
-        return fail(TDYN_E_ARG, "tdyn_rollout_begin: event witnesses run on the tree sweeps (evaluator 0 or 1) of a "
-                                "batch of one tree, in FP64, without known udot, prescribed motion or constraints");
+                       prec != 0))
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: event witnesses run on the tree sweeps (evaluator 0 or 1), in "
+                                "FP64, without known udot, prescribed motion or constraints");
     { const int rc = ensure_constraints(b); if (rc) return rc; }
     { const int rc = ensure_witnesses(b); if (rc) return rc; }
     const size_t L = b->n_inst;

@@ -319,6 +319,9 @@ hipError_t launch_report_interp(const DevTables& T, const int* tree_of, const in
                                 const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
                                 const double* const prev[4], const double* const now[4], double cons_tol,
                                 double* q_out, double* u_out, long total_q, long total_u, hipStream_t s);
+// the event log (per lane) to instance order
+hipError_t launch_event_scatter(const int* inst, int n, int cap, const double* win, const int* ids, const int* cnt,
+                                double* win_out, int* ids_out, int* cnt_out, hipStream_t s);
 hipError_t launch_lane_scatter(const int* inst, int n, const double* t_lane, double* t_inst, const int* st_lane,
                                int* st_inst, const int* stats_lane, int* stats_inst, hipStream_t s);
 

@@ -2624,9 +2624,10 @@ hipError_t launch_rkm(const RkmArgs& a, hipStream_t s) {
     if (a.chain) return launch_chain_variant(a, s, false);
     if (a.grp) return launch_tree_group(a, s, false);
     if (a.ev_ws) {
-        // event witnesses: the tree sweeps of a batch of one tree
-        if (a.vec_aos || a.tree_of) return hipErrorInvalidValue;
-        k_rkm<TreeEval<false>, true, true><<<grd, blk, 0, s>>>(a);
+        // event witnesses: the tree sweeps
+        if (a.vec_aos) return hipErrorInvalidValue;
+        if (a.tree_of) k_rkm<TreeEval<true>, false, true><<<grd, blk, 0, s>>>(a);
+        else k_rkm<TreeEval<false>, true, true><<<grd, blk, 0, s>>>(a);
         return hipGetLastError();
     }
     if (a.vec_aos) k_rkm<TreeStreamEval, false><<<grd, blk, 0, s>>>(a);
@@ -2650,6 +2651,23 @@ __global__ void k_lane_scatter(const int* inst, int n, const double* tl, double*
 }
 // New final times (te, lane order, or te_all for every lane): a lane that had
 // reached its old final time resumes if the new one lies ahead of it.
+// The event log of each lane to its instance's rows (lanes permuted).
+__global__ void k_event_scatter(const int* inst, int n, int cap, const double* win, const int* ids, const int* cnt,
+                                double* win_out, int* ids_out, int* cnt_out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const size_t i = inst[l];
+    const size_t so = (size_t)l * cap * 2, d = i * cap * 2;
+    if (win_out) for (int k = 0; k < 2*cap; ++k) win_out[d + k] = win[so + k];
+    if (ids_out) for (int k = 0; k < 2*cap; ++k) ids_out[d + k] = ids[so + k];
+    if (cnt_out) cnt_out[i] = cnt[l];
+}
+hipError_t launch_event_scatter(const int* inst, int n, int cap, const double* win, const int* ids, const int* cnt,
+                                double* win_out, int* ids_out, int* cnt_out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_event_scatter<<<dim3((n + 63)/64), dim3(64), 0, s>>>(inst, n, cap, win, ids, cnt, win_out, ids_out, cnt_out);
+    return hipGetLastError();
+}
 __global__ void k_rollout_extend(int n, const double* te, double te_all, const double* t, double* t_end,
                                  int* status) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;

@@ -7,8 +7,9 @@ counts and final states (1e-9) on a bouncing ball (whose windows must also
 hold the exact parabola crossings), a pendulum with position, velocity and
 rate witnesses and energy injection, a spatial chain with quaternions, and
 terminating handlers (status 3, time tLow); the event log's capacity, a
-rollout resumed launch by launch, and the settings witnesses cannot run
-with are refused on the host."""
+rollout resumed launch by launch, a batch of different trees, the
+witnesses' values at a state (getEventTriggers), and the settings witnesses
+cannot run with are refused on the host."""
 import numpy as np
 import pytest
 
@@ -194,11 +195,11 @@ def test_settings_witnesses_cannot_run_with_are_refused():
     with pytest.raises(TdynError):
         B.stepToWithReports(q, u, rkm, 0.2, [0.1])
     B.stepTo(q, u, rkm, 0.1)
-    # a batch of different trees
-    C = MatterBatch([spec, slider()], tree_of=np.array([0, 1], np.int32))
-    C.set_witnesses(0, wits)
+    # with constraints
+    B.set_constraints(0, [(1, 0, 1, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), 1.0)])
     with pytest.raises(TdynError):
-        C.stepTo(C.pack([qs[0], np.array([1.0])]), C.pack([us[0], np.array([0.0])]), rkm, 0.1)
+        B.stepTo(q, u, rkm, 0.1)
+    B.set_constraints(0, [])
     for bad in ([dict(kind=O.WIT_Q, index=1)], [dict(kind=O.WIT_STATION_POS, body=2)],
                 [dict(kind=O.WIT_Q, trigger=0)], [dict(kind=O.WIT_Q, window=0.0)],
                 [dict(kind=9)], [dict(kind=O.WIT_Q, action=O.EV_SCALE_U, action_index=1)],
@@ -250,3 +251,45 @@ def test_event_triggers_match_oracle(shared):
         ref = o.witness_values(ts[i], np.concatenate([qs[i], us[i]]), len(wits[tree_of[i]]))
         assert _scaled_diff(vals[i, :len(ref)], ref) < 1e-12, i
         assert np.all(vals[i, len(ref):] == 0)
+
+
+def test_batch_of_different_trees_matches_oracle():
+    """Witnesses on a batch of different trees (padded records, lanes sorted
+    by tree size): a bouncing ball, the pendulum and the spatial chain, each
+    with its own witnesses; every instance's events, step counts and state
+    as the oracle's."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    sl = slider()
+    spec_p, wits_p, _, _, _ = _pendulum_case(n=3)
+    ch = M.spatial_chain(nb=5, seed=7, types=(M.BALL, M.PIN, M.UNIVERSAL))
+    ch.gravity = np.array([0.0, -G, 0.0])
+    wits_c = [dict(kind=O.WIT_STATION_POS, body=5, station=(0.1, 0.2, 0.0), direction=(0, 1.0, 0), offset=-1.0),
+              dict(kind=O.WIT_STATION_VEL, body=3, station=(0.0, 0.3, 0.1), direction=(0.6, 0.0, 0.8),
+                   trigger=O.TRIG_FALLING),
+              dict(kind=O.WIT_U, index=1, action=O.EV_SCALE_U, action_index=2, factor=0.9)]
+    wits_s = [dict(kind=O.WIT_Q, index=0, trigger=O.TRIG_FALLING, action=O.EV_SCALE_U, action_index=0, factor=-0.8)]
+    specs, wits = [sl, spec_p, ch], [wits_s, wits_p, wits_c]
+    tree_of = np.array([0, 1, 2, 1, 0, 2, 2, 0, 1, 2, 0, 1], np.int32)
+    r = np.random.default_rng(2)
+    qs, us = [], []
+    for i, t in enumerate(tree_of):
+        if t == 0:
+            qs.append(np.array([0.5 + r.uniform()])); us.append(np.array([r.uniform(-1, 2)]))
+        elif t == 1:
+            qs.append(np.array([r.uniform(-0.3, 0.3)])); us.append(np.array([r.uniform(-1, 1)]))
+        else:
+            q, u = M.random_state(ch, 11, i, qscale=0.5, uscale=0.5)
+            qs.append(q); us.append(u)
+    B = MatterBatch(specs, tree_of=tree_of)
+    for k, w in enumerate(wits):
+        B.set_witnesses(k, w)
+    res = B.stepToWithEvents(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-6, max_step=0.05), 1.5)
+    assert B.active_evaluator()[0] == 1
+    g = dict(q=B.split_q(res["q"]), u=B.split_u(res["u"]))
+    for k in ("t", "status", "stats", "windows", "ids", "counts"):
+        g[k] = res[k].cpu().numpy()
+    ref = []
+    for i, t in enumerate(tree_of):
+        ref.append(_oracle(specs[t], wits[t], [qs[i]], [us[i]], 1e-6, 1.5, max_h=0.05)[0])
+    assert sum(len(x["win"]) for x in ref) >= 2 * len(tree_of)
+    _compare(g, ref, wtol=1e-6 * 0.1 * 0.1)
