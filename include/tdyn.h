@@ -488,6 +488,33 @@ int tdyn_rollout_noalloc(tdyn_batch* b, double* q, double* u, const double* fapp
  * projections; 4 ints per instance).  Any output may be NULL. */
 int tdyn_rollout_get(tdyn_batch* b, double* q, double* u, double* t, int32_t* status,
                      int32_t* stats, hipStream_t s);
+/* Rollout snapshot (checkpoint / resume).  The reference's checkpoint is a
+ * copy of the State (State.h:1206-1222; no serialization) and its resume is
+ * Integrator::initialize(state) (Integrator.cpp:333-435), which projects and
+ * restarts the step control.  Here a snapshot is the whole integrator state
+ * of every instance in a documented byte layout (host memory), and a
+ * restore continues each instance exactly where it was: same state, time,
+ * step size, status and statistics, so the resumed rollout's results equal
+ * the uninterrupted one's bit for bit.  Layout (little-endian):
+ *   0   char[8] "TDYNSNP1"; 8 uint32 version (1); 12 int32 n_instances;
+ *   16  int64 total_q; 24 int64 total_u; 32 uint64 model hash (FNV-1a of
+ *       the batch's tree, body and constant tables, gravity and instance
+ *       trees); 40 int32 precision; 44 int32 evaluator setting;
+ *   48  double accuracy, constraint tolerance, fixed step, initial, minimum
+ *       and maximum step (-1: unset), time scale; 104 int32 max attempts;
+ *   112 per instance, instance order: double t[n], h[n], t_end[n]; int32
+ *       status[n], stats[4n]; double q[total_q], u[total_u] (packed); with
+ *       precision 1 also the Kahan compensation rows of q and u.
+ * The event log is not part of a snapshot (a restored rollout's counts start
+ * at zero).  tdyn_rollout_snapshot: bytes (nullable) gets the size; out
+ * (host, nullable: size query only) of capacity bytes; synchronizes s. */
+int tdyn_rollout_snapshot(tdyn_batch* b, void* out, int64_t capacity, int64_t* bytes, hipStream_t s);
+/* Begins a rollout from a snapshot taken on a batch of the same model
+ * (TDYN_E_ARG for another model or format, TDYN_E_SIZE for other sizes),
+ * with the batch's own evaluator setting and constraints, witnesses, known
+ * udot and motion as set now; fapp as in tdyn_rollout_begin.  No initial
+ * projection.  Synchronizes s. */
+int tdyn_rollout_restore(tdyn_batch* b, const void* snapshot, int64_t bytes, const double* fapp, hipStream_t s);
 /* The state derivative (qdot, udot; packed) of the current rollout state,
  * evaluated by the rollout's evaluator at the rollout's precision: the f0
  * the integrator uses (realize(state, Stage::Acceleration) + getQDot/getUDot,

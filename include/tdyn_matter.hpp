@@ -175,6 +175,18 @@ public:
     void getState(double* q, double* u, double* t, int32_t* status, int32_t* stats) const {
         check(tdyn_rollout_get(h_, q, u, t, status, stats, s_));
     }
+    // checkpoint / resume: the rollout's integrator state as bytes
+    // (tdyn_rollout_snapshot layout), and a rollout begun from such bytes
+    std::vector<unsigned char> snapshot() const {
+        int64_t n = 0;
+        check(tdyn_rollout_snapshot(h_, nullptr, 0, &n, s_));
+        std::vector<unsigned char> out((size_t)n);
+        check(tdyn_rollout_snapshot(h_, out.data(), n, &n, s_));
+        return out;
+    }
+    void restore(const std::vector<unsigned char>& snap, const double* appliedBodyForces = nullptr) const {
+        check(tdyn_rollout_restore(h_, snap.data(), (int64_t)snap.size(), appliedBodyForces, s_));
+    }
     // the events of the resumable form's rollout: windows (tLow, tHigh), ids
     // (witness, transition) and counts per instance
     void getEvents(double* windows, int32_t* ids, int32_t* counts) const {

@@ -809,9 +809,22 @@ int tdyn_calc_mobilizer_reaction_forces(tdyn_batch* b, const double* q, const do
     return TDYN_OK;
 }
 
+// tdyn_rollout_begin; init false (tdyn_rollout_restore): the state is taken
+// as it is, without initialize()'s projection and prescription (the caller
+// then sets every lane's t, h, status and statistics).
+static int rollout_begin_impl(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                              double t0, const double* t_end, double t_end_all,
+                              const tdyn_integrator_cfg* cfg, bool init, hipStream_t s);
+
 int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const double* fapp,
                        double t0, const double* t_end, double t_end_all,
                        const tdyn_integrator_cfg* cfg, hipStream_t s) {
+    return rollout_begin_impl(b, q, u, fapp, t0, t_end, t_end_all, cfg, true, s);
+}
+
+static int rollout_begin_impl(tdyn_batch* b, const double* q, const double* u, const double* fapp,
+                              double t0, const double* t_end, double t_end_all,
+                              const tdyn_integrator_cfg* cfg, bool init, hipStream_t s) {
     if (!b || !q || !u || !cfg) return fail(TDYN_E_ARG, "tdyn_rollout_begin: batch, q, u and cfg are required");
     if (!t_end && !(t_end_all > t0)) return fail(TDYN_E_ARG, "tdyn_rollout_begin: final time must exceed t0");
     // Every check runs on locals first: a refused begin leaves the rollout in
@@ -887,8 +900,172 @@ int tdyn_rollout_begin(tdyn_batch* b, const double* q, const double* u, const do
     b->roll_evaluator = b->evaluator;
     RkmArgs a = b->rk();
     HIPCHK(launch_pack_state(a, q, u, 1, nullptr, nullptr, s));
-    HIPCHK(launch_rkm_init(a, t0, s));
-    HIPCHK(launch_cons_init(a, s));
+    if (init) {
+        HIPCHK(launch_rkm_init(a, t0, s));
+        HIPCHK(launch_cons_init(a, s));
+    }
+    b->rollout_ready = true;
+    return TDYN_OK;
+}
+
+// ---- rollout snapshots -------------------------------------------------
+extern "C++" {
+namespace {
+constexpr char kSnapMagic[8] = {'T', 'D', 'Y', 'N', 'S', 'N', 'P', '1'};
+constexpr int kSnapHeader = 112;
+// FNV-1a over the batch's model: tree and body tables, constants, gravity
+// and the tree of every instance (a snapshot resumes only on the same model).
+uint64_t model_hash(tdyn_batch* b, int& rc) {
+    rc = 0;
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    mix(&b->n_inst, sizeof b->n_inst); mix(&b->n_trees, sizeof b->n_trees);
+    mix(b->h_tree_of.data(), b->h_tree_of.size() * sizeof(int));
+    mix(b->h_tree_a.data(), b->h_tree_a.size() * sizeof(int4));
+    mix(b->h_body_a.data(), b->h_body_a.size() * sizeof(int4));
+    std::vector<double> prm(b->prm.n), grav(b->gravity.n);
+    if ((!prm.empty() && hipMemcpy(prm.data(), b->prm.p, prm.size() * sizeof(double), hipMemcpyDeviceToHost)) ||
+        (!grav.empty() && hipMemcpy(grav.data(), b->gravity.p, grav.size() * sizeof(double), hipMemcpyDeviceToHost)))
+        rc = TDYN_E_HIP;
+    mix(prm.data(), prm.size() * sizeof(double)); mix(grav.data(), grav.size() * sizeof(double));
+    return h;
+}
+int64_t snapshot_bytes(const tdyn_batch* b, int precision) {
+    const int64_t n = b->n_inst, ny = b->total_q + b->total_u;
+    return kSnapHeader + n * (3 * 8 + 5 * 4) + 8 * ny * (precision == 1 ? 2 : 1);
+}
+template <class T> void put(std::vector<unsigned char>& o, size_t at, const T& v) { std::memcpy(o.data() + at, &v, sizeof v); }
+template <class T> T get(const unsigned char* p, size_t at) { T v; std::memcpy(&v, p + at, sizeof v); return v; }
+} // namespace
+} // extern "C++"
+
+int tdyn_rollout_snapshot(tdyn_batch* b, void* out, int64_t capacity, int64_t* bytes, hipStream_t s) {
+    if (!b || !b->rollout_ready) return fail(TDYN_E_STAGE, "tdyn_rollout_snapshot: no rollout");
+    const int64_t need = snapshot_bytes(b, b->precision);
+    if (bytes) *bytes = need;
+    if (!out) return TDYN_OK;
+    if (capacity < need) return fail(TDYN_E_SIZE, "tdyn_rollout_snapshot: the buffer is too small");
+    const size_t L = b->n_inst, TQ = b->total_q, TU = b->total_u;
+    int rc = 0;
+    const uint64_t mh = model_hash(b, rc);
+    if (rc) return fail(rc, "tdyn_rollout_snapshot: model tables copy failed");
+    // per-lane arrays to host, then to instance order
+    std::vector<double> t(L), h(L), te(L);
+    std::vector<int> st(L), sx(4 * L), inst(L);
+    HIPCHK(hipMemcpyAsync(t.data(), b->t.p, L * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h.data(), b->h.p, L * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(te.data(), b->t_end.p, L * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(st.data(), b->status.p, L * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(sx.data(), b->stats.p, 4 * L * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (b->permuted) HIPCHK(hipMemcpyAsync(inst.data(), b->inst.p, L * sizeof(int), hipMemcpyDeviceToHost, s));
+    else for (size_t l = 0; l < L; ++l) inst[l] = (int)l;
+    // packed q, u (and the compensation rows of the mixed path), instance order
+    const int nrow = b->precision == 1 ? 2 : 1;
+    DBuf<double> dq, du;
+    HIPCHK(dq.alloc(TQ * nrow)); HIPCHK(du.alloc(TU * nrow));
+    RkmArgs a = b->rk();
+    HIPCHK(launch_pack_state(a, nullptr, nullptr, 0, dq.p, du.p, s, VY));
+    if (nrow == 2) HIPCHK(launch_pack_state(a, nullptr, nullptr, 0, dq.p + TQ, du.p + TU, s, VC0));
+    std::vector<double> hq(TQ * nrow), hu(TU * nrow);
+    HIPCHK(hipMemcpyAsync(hq.data(), dq.p, hq.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hu.data(), du.p, hu.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<unsigned char> o((size_t)need, 0);
+    std::memcpy(o.data(), kSnapMagic, 8);
+    put<uint32_t>(o, 8, 1u); put<int32_t>(o, 12, (int32_t)L);
+    put<int64_t>(o, 16, (int64_t)TQ); put<int64_t>(o, 24, (int64_t)TU); put<uint64_t>(o, 32, mh);
+    put<int32_t>(o, 40, b->precision); put<int32_t>(o, 44, b->roll_evaluator);
+    const RkmCfg& c = b->cfg;
+    const double cd[7] = {c.accuracy, c.cons_tol, c.fixed_h, c.init_h, c.min_h, c.max_h, c.timescale};
+    for (int k = 0; k < 7; ++k) put<double>(o, 48 + 8*k, cd[k]);
+    put<int32_t>(o, 104, c.max_attempts);
+    size_t at = kSnapHeader;
+    std::vector<double> ti(L), hi(L), tei(L);
+    std::vector<int> sti(L), sxi(4 * L);
+    for (size_t l = 0; l < L; ++l) {
+        const int i = inst[l];
+        ti[i] = t[l]; hi[i] = h[l]; tei[i] = te[l]; sti[i] = st[l];
+        for (int k = 0; k < 4; ++k) sxi[4*i + k] = sx[4*l + k];
+    }
+    std::memcpy(o.data() + at, ti.data(), L * 8); at += L * 8;
+    std::memcpy(o.data() + at, hi.data(), L * 8); at += L * 8;
+    std::memcpy(o.data() + at, tei.data(), L * 8); at += L * 8;
+    std::memcpy(o.data() + at, sti.data(), L * 4); at += L * 4;
+    std::memcpy(o.data() + at, sxi.data(), 4 * L * 4); at += 4 * L * 4;
+    std::memcpy(o.data() + at, hq.data(), TQ * 8); at += TQ * 8;
+    std::memcpy(o.data() + at, hu.data(), TU * 8); at += TU * 8;
+    if (nrow == 2) {
+        std::memcpy(o.data() + at, hq.data() + TQ, TQ * 8); at += TQ * 8;
+        std::memcpy(o.data() + at, hu.data() + TU, TU * 8); at += TU * 8;
+    }
+    std::memcpy(out, o.data(), o.size());
+    return TDYN_OK;
+}
+
+int tdyn_rollout_restore(tdyn_batch* b, const void* in, int64_t bytes, const double* fapp, hipStream_t s) {
+    if (!b || !in) return fail(TDYN_E_ARG, "tdyn_rollout_restore: batch and snapshot are required");
+    const unsigned char* p = (const unsigned char*)in;
+    if (bytes < kSnapHeader || std::memcmp(p, kSnapMagic, 8) != 0 || get<uint32_t>(p, 8) != 1u)
+        return fail(TDYN_E_ARG, "tdyn_rollout_restore: not a snapshot of this format");
+    const size_t L = b->n_inst, TQ = b->total_q, TU = b->total_u;
+    const int prec = get<int32_t>(p, 40);
+    if (get<int32_t>(p, 12) != (int32_t)L || get<int64_t>(p, 16) != (int64_t)TQ || get<int64_t>(p, 24) != (int64_t)TU ||
+        prec < 0 || prec > 3 || bytes != snapshot_bytes(b, prec))
+        return fail(TDYN_E_SIZE, "tdyn_rollout_restore: the snapshot's sizes are not this batch's");
+    int rc = 0;
+    const uint64_t mh = model_hash(b, rc);
+    if (rc) return fail(rc, "tdyn_rollout_restore: model tables copy failed");
+    if (mh != get<uint64_t>(p, 32)) return fail(TDYN_E_ARG, "tdyn_rollout_restore: the snapshot is of another model");
+    tdyn_integrator_cfg cfg;
+    std::memset(&cfg, 0, sizeof cfg);
+    const double acc = get<double>(p, 48), ct = get<double>(p, 56), fh = get<double>(p, 64), ih = get<double>(p, 72),
+                 mnh = get<double>(p, 80), mxh = get<double>(p, 88), ts = get<double>(p, 96);
+    cfg.accuracy = acc; cfg.constraint_tolerance = ct; cfg.fixed_step = fh > 0 ? fh : 0.0;
+    cfg.initial_step = ih > 0 ? ih : 0.0; cfg.min_step = mnh > 0 ? mnh : 0.0; cfg.max_step = mxh > 0 ? mxh : 0.0;
+    cfg.time_scale = ts; cfg.max_attempts = get<int32_t>(p, 104); cfg.precision = prec;
+    size_t at = kSnapHeader;
+    std::vector<double> ti(L), hi(L), tei(L);
+    std::vector<int> sti(L), sxi(4 * L);
+    std::memcpy(ti.data(), p + at, L * 8); at += L * 8;
+    std::memcpy(hi.data(), p + at, L * 8); at += L * 8;
+    std::memcpy(tei.data(), p + at, L * 8); at += L * 8;
+    std::memcpy(sti.data(), p + at, L * 4); at += L * 4;
+    std::memcpy(sxi.data(), p + at, 4 * L * 4); at += 4 * L * 4;
+    const size_t q_at = at;
+    // the state rows go through the device (packed layout, instance order)
+    const int nrow = prec == 1 ? 2 : 1;
+    DBuf<double> dq, du, dte;
+    HIPCHK(dq.alloc(TQ * nrow)); HIPCHK(du.alloc(TU * nrow)); HIPCHK(dte.alloc(L));
+    HIPCHK(hipMemcpy(dq.p, p + q_at, TQ * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(du.p, p + q_at + TQ * 8, TU * 8, hipMemcpyHostToDevice));
+    if (nrow == 2) {
+        HIPCHK(hipMemcpy(dq.p + TQ, p + q_at + (TQ + TU) * 8, TQ * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(du.p + TU, p + q_at + (2*TQ + TU) * 8, TU * 8, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(dte.p, tei.data(), L * 8, hipMemcpyHostToDevice));
+    if ((rc = rollout_begin_impl(b, dq.p, du.p, fapp, 0.0, dte.p, 0.0, &cfg, false, s))) return rc;
+    b->rollout_ready = false;
+    RkmArgs a = b->rk();
+    if (nrow == 2) HIPCHK(launch_pack_state(a, dq.p + TQ, du.p + TU, 1, nullptr, nullptr, s, VC0));
+    // per-instance t, h, status, statistics to lane order
+    std::vector<int> inst(L);
+    if (b->permuted) HIPCHK(hipMemcpy(inst.data(), b->inst.p, L * sizeof(int), hipMemcpyDeviceToHost));
+    else for (size_t l = 0; l < L; ++l) inst[l] = (int)l;
+    std::vector<double> t(L), h(L);
+    std::vector<int> st(L), sx(4 * L);
+    for (size_t l = 0; l < L; ++l) {
+        const int i = inst[l];
+        t[l] = ti[i]; h[l] = hi[i]; st[l] = sti[i];
+        for (int k = 0; k < 4; ++k) sx[4*l + k] = sxi[4*i + k];
+    }
+    HIPCHK(hipMemcpyAsync(b->t.p, t.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->h.p, h.data(), L * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->status.p, st.data(), L * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->stats.p, sx.data(), 4 * L * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
     b->rollout_ready = true;
     return TDYN_OK;
 }

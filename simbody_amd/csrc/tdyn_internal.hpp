@@ -306,7 +306,8 @@ hipError_t launch_srv_admit(const RkmArgs& a, const double* rec, const int* slot
                             double t0, hipStream_t s);
 hipError_t launch_srv_collect(const RkmArgs& a, const int* slots, int n, double* out, int out_stride, int* outi,
                               hipStream_t s);
-// vec: which state vector to read when unpacking (0 = the state y, 2 = f0)
+// vec: which state vector to read when unpacking, or write when packing
+// (0 = the state y, 2 = f0, VC0 = the mixed path's compensation)
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
                              double* q_out, double* u_out, hipStream_t s, int vec = 0);
 hipError_t launch_rollout_f0(const RkmArgs& a, hipStream_t s);

@@ -2785,7 +2785,7 @@ __global__ void k_pack(RkmArgs a, const double* q, const double* u, int to_soa, 
 hipError_t launch_pack_state(const RkmArgs& a, const double* q, const double* u, int to_soa,
                              double* q_out, double* u_out, hipStream_t s, int vec) {
     if (a.n_inst <= 0) return hipSuccess;
-    k_pack<<<dim3((a.n_inst + 255)/256), dim3(256), 0, s>>>(a, q, u, to_soa, q_out, u_out, to_soa ? 0 : vec);
+    k_pack<<<dim3((a.n_inst + 255)/256), dim3(256), 0, s>>>(a, q, u, to_soa, q_out, u_out, vec);
     return hipGetLastError();
 }
 

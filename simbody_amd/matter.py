@@ -35,7 +35,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_batch_set_known_udot", "tdyn_batch_set_motion", "tdyn_batch_autotune",
            "tdyn_batch_set_constraints", "tdyn_calc_constrained_acceleration",
            "tdyn_batch_set_witnesses", "tdyn_batch_set_event_capacity", "tdyn_rollout_get_events",
-           "tdyn_calc_event_triggers",
+           "tdyn_calc_event_triggers", "tdyn_rollout_snapshot", "tdyn_rollout_restore",
            "tdyn_multiply_by_m_inv",
            "tdyn_calc_residual_force_ignoring_constraints", "tdyn_calc_articulated_body_inertias",
            "tdyn_calc_mobilizer_reaction_forces", "tdyn_multiply_by_m", "tdyn_multiply_by_system_jacobian",
@@ -139,6 +139,8 @@ def lib():
         L.tdyn_batch_set_event_capacity.argtypes = [vp, ctypes.c_int]
         L.tdyn_rollout_get_events.argtypes = [vp, vp, vp, vp, vp]
         L.tdyn_calc_event_triggers.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.tdyn_rollout_snapshot.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64), vp]
+        L.tdyn_rollout_restore.argtypes = [vp, vp, ctypes.c_int64, vp, vp]
         L.tdyn_batch_autotune.argtypes = [vp, vp, vp, vp, ctypes.POINTER(IntegratorCfgC), ctypes.c_double,
                                           ctypes.c_char_p, vp, vp]
         L.tdyn_multiply_by_m_inv.argtypes = [vp] + [vp] * 3 + [vp]
@@ -641,6 +643,21 @@ class MatterBatch:
         stats = torch.zeros(4 * self.n_instances, dtype=torch.int32, device=self.device)
         _check(lib().tdyn_rollout_get(self.h, _ptr(q), _ptr(u), _ptr(t), _ptr(st), _ptr(stats), _stream()))
         return q, u, t, st, stats.view(-1, 4)
+
+    def rollout_snapshot(self):
+        """The current rollout's integrator state as bytes (the layout of
+        tdyn_rollout_snapshot): a checkpoint to store or send."""
+        n = ctypes.c_int64(0)
+        _check(lib().tdyn_rollout_snapshot(self.h, None, 0, ctypes.byref(n), _stream()))
+        buf = ctypes.create_string_buffer(n.value)
+        _check(lib().tdyn_rollout_snapshot(self.h, buf, n, ctypes.byref(n), _stream()))
+        return buf.raw
+
+    def rollout_restore(self, snapshot, fapp=None):
+        """Resume a rollout from rollout_snapshot() bytes taken on a batch of
+        the same model (advance / get / events continue from there)."""
+        buf = ctypes.create_string_buffer(bytes(snapshot), len(snapshot))
+        _check(lib().tdyn_rollout_restore(self.h, buf, len(snapshot), _ptr(fapp), _stream()))
 
     def rollout_derivs(self):
         """(qdot, udot) of the current rollout state by the rollout's
