@@ -327,9 +327,13 @@ typedef struct tdyn_witness_desc {
  * that changes sign twice within one step is not seen (bound the step size
  * for such witnesses).  Rollouts with witnesses run the tree sweeps in
  * FP64 (evaluator 0 or 1), on batches of one tree or of different trees,
- * without known udot, prescribed motion, constraints, reports or
- * tdyn_rollout_noalloc; status -5 marks an instance whose localization
- * failed.  A rollout in progress ends (begin again). */
+ * with or without constraints (whose rollouts need a batch of one tree;
+ * trial and backed-up states are then projected onto them, projectQ and
+ * projectU at the tolerance, as realizeAndProjectKinematicsWithThrow does,
+ * IntegratorRep.h:858-897), without known udot, prescribed motion, reports
+ * or tdyn_rollout_noalloc; status -5 marks an instance whose localization
+ * or whose projection of an interpolated state failed.  A rollout in
+ * progress ends (begin again). */
 int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w);
 /* The witnesses' values at a state (realize(state) to their stages, then
  * State::getEventTriggers, State.h): q, u (packed), t (one time per

@@ -1795,15 +1795,26 @@ template <class T> struct Rkm {
     // createInterpolatedState (AbstractIntegratorRep.cpp:63-90) between the
     // step's start (t0, y0, f0) and its end (t1, y1, evF1): the cubic Hermite
     // interpolant (interpolateOrder3, IntegratorRep.h:299-313) at t, then the
-    // LocalOnly projection (quaternions normalized when their RMS norm error
-    // exceeds the constraint tolerance).
-    void interpLocal(double t0, double t1, double t, double consTol, T* yt) {
+    // LocalOnly projection at the constraint tolerance
+    // (realizeAndProjectKinematicsWithThrow, IntegratorRep.h:858-897: with
+    // constraints projectQ then projectU, no projection limit, no error
+    // estimate; without, the quaternions normalized when their RMS norm
+    // error exceeds the tolerance).  False when the projection fails (the
+    // reference throws).
+    bool interpLocal(double t0, double t1, double t, double consTol, T* yt) {
         const int ny = M.nq + M.nu;
         const T h = T(t1 - t0), d = T((t - t0) / (t1 - t0));
         const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
         const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*evF1[i];
+        if (ws.cons) {
+            bool ch;
+            const double inf = std::numeric_limits<double>::infinity();
+            return projectQ(yt, (T*)nullptr, consTol, inf, false, &ch) &&
+                   projectU(yt, yt + M.nq, (T*)nullptr, consTol, inf, false, &ch);
+        }
         if (M.nquat > 0 && double(quatErrRMS(yt)) > consTol) normalizeQuats(yt, (T*)nullptr);
+        return true;
     }
     // Events of the step just accepted from (t, y0, f0) to (t1, y1)
     // (takeOneStep's trigger search and localization, AbstractIntegratorRep.
@@ -1819,8 +1830,9 @@ template <class T> struct Rkm {
     // ReachedEventTrigger reports); one that changes the state leaves it at
     // tHigh.  e0 then holds the witnesses of the state the next step starts
     // from.  Returns 0 (no event), 1 (events handled), 2 (terminate) or -1
-    // (localization failed: no candidate in either half, or no convergence in
-    // kMaxLocalizeIters trials).
+    // (localization failed: no candidate in either half, no convergence in
+    // kMaxLocalizeIters trials, or the projection of an interpolated state
+    // failed).
     int stepEvents(double t, double& t1, const SV<T>* Fapp, const RkmConfig& cfg, double consTol) {
         const std::vector<WitnessDef>& W = *wit;
         const int nw = (int)W.size(), ny = M.nq + M.nu;
@@ -1842,7 +1854,7 @@ template <class T> struct Rkm {
                 if (++it > kMaxLocalizeIters) return -1;
                 if (side2 != 0 && side1 != 0) bias = side2 != side1 ? 1.0 : (side1 < 0 ? bias/2 : bias*2);
                 const double tMid = C.earliest;
-                interpLocal(t, t1, tMid, consTol, evY.data());
+                if (!interpLocal(t, t1, tMid, consTol, evY.data())) return -1;
                 witnessValues(tMid, evY.data(), eMid.data());
                 findEventCandidates(W, &C.idx, tLow, eLow.data(), tMid, eMid.data(), bias, minWindow, acc, ts, N);
                 if (!N.idx.empty()) {
@@ -1856,7 +1868,7 @@ template <class T> struct Rkm {
                 tLow = tMid; eLow = eMid; C = N;
             } while (tHigh - tLow > C.narrowest);
             if (tHigh < t1) {
-                interpLocal(t, t1, tHigh, consTol, evY.data());
+                if (!interpLocal(t, t1, tHigh, consTol, evY.data())) return -1;
                 for (int i=0;i<ny;++i) y1[i] = evY[i];
                 t1 = tHigh;
             }
@@ -1873,7 +1885,7 @@ template <class T> struct Rkm {
         }
         if (term && !modified) {
             derivsAt(t1, y1.data(), Fapp, evF1.data());
-            interpLocal(t, t1, tLow, consTol, evY.data());
+            if (!interpLocal(t, t1, tLow, consTol, evY.data())) return -1;
             for (int i=0;i<ny;++i) y1[i] = evY[i];
             t1 = tLow;
         }

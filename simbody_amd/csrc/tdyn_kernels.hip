@@ -2031,13 +2031,15 @@ __device__ void prescribe_lane(const RkmArgs& a, int lane, int tr, double t, con
 // changes nothing leaves the state interpolated at tLow (the state
 // ReachedEventTrigger reports), one that changes the state leaves it at
 // tHigh.  The start-of-step values at EW_E0 then belong to the state the
-// next step starts from.  F1 and YT are free rows.  Returns 0 (no event), 1
-// (handled), 2 (terminate) or -1 (no candidate in either half of a trial, or
-// no convergence within 400 trials).
+// next step starts from.  F1, YT, FA, YS and YE are free rows.  Returns 0 (no
+// event), 1 (handled), 2 (terminate) or -1 (no candidate in either half of a
+// trial, no convergence within 400 trials, or a failed projection of an
+// interpolated state).
 template <class EV, class TRows>
 __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lane, int nq, int nu, int nb,
                            int nquat, const int4* body_a, const double* prm_tree, double t, double& t1,
-                           const TRows& Y, const TRows& F0, const TRows& Y1, const TRows& F1, const TRows& YT) {
+                           const TRows& Y, const TRows& F0, const TRows& Y1, const TRows& F1, const TRows& YT,
+                           const TRows& FA, const TRows& YS, const TRows& YE) {
     const int ny = nq + nu;
     const double consTol = a.cfg.cons_tol, accTs = a.cfg.accuracy * a.cfg.timescale;
     int lo = EW_E0, hi = EW_HIGH, mid = EW_LOW;
@@ -2051,12 +2053,28 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
         return 0;
     }
     // the interpolant at tm of (t, Y, F0) and (te, Y1, F1) into out, projected
-    auto interp = [&](double te, double tm, const TRows& out) {
+    // (realizeAndProjectKinematicsWithThrow, IntegratorRep.h:858-897: with
+    // constraints projectQ then projectU at the tolerance, no limit, no
+    // error estimate, FA, YS and YE as scratch rows); false when that fails
+    auto interp = [&](double te, double tm, const TRows& out) -> bool {
         const double h = te - t, d = (tm - t) / (te - t);
         const double cy1 = d*d*(3.0 - 2.0*d), cy0 = 1.0 - cy1;
         const double hdd1 = h*d*(d - 1.0), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int k = 0; k < ny; ++k) out(k) = ((cy0*Y(k) + cy1*Y1(k)) + cf0*F0(k)) + cf1*F1(k);
-        if (nquat == 0) return;
+        if constexpr (EV::kCons) {
+            if (ev.K.m > 0) {
+                const TRows ou{out.base + (size_t)nq * out.L, out.L, out.lane};
+                const TRows su{YS.base + (size_t)nq * YS.L, YS.L, YS.lane};
+                const TRows du{FA.base + (size_t)nq * FA.L, FA.L, FA.lane};
+                const double inf = __builtin_inf();
+                bool ch;
+                return cons_project_q<EV::kPad>(ev.c, ev.K, out, ou, (const TRows*)nullptr, YS, du, YE, nq, nquat,
+                                                consTol, inf, false, &ch) &&
+                       cons_project_u<EV::kPad>(ev.c, ev.K, out, ou, (const TRows*)nullptr, su, du, nu, consTol, inf,
+                                                false, &ch);
+            }
+        }
+        if (nquat == 0) return true;
         double s2 = 0;
         for (int b = 0; b < nb; ++b) {
             const int4 ba = ldi4(body_a, b);
@@ -2066,7 +2084,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             const double e = sqrt(n2) - 1.0;
             s2 += e*e;
         }
-        if (!(sqrt(s2 / nquat) > consTol)) return;
+        if (!(sqrt(s2 / nquat) > consTol)) return true;
         for (int b = 0; b < nb; ++b) {
             const int4 ba = ldi4(body_a, b);
             if (ba.y != BALL && (ba.y != FREE || prm_tree[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
@@ -2075,6 +2093,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             const double n = sqrt(n2);
             for (int k = 0; k < 4; ++k) out(ba.z + k) = out(ba.z + k) / n;
         }
+        return true;
     };
     double tLow = t, tHigh = t1;
     const double t1o = t1;
@@ -2086,7 +2105,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             if (++it > 400) return -1;
             if (side2 != 0 && side1 != 0) bias = side2 != side1 ? 1.0 : (side1 < 0 ? bias/2 : bias*2);
             const double tMid = earliest;
-            interp(t1o, tMid, YT);
+            if (!interp(t1o, tMid, YT)) return -1;
             wit_values<EV::kPad>(ev.c, W, tMid, YT, mid);
             unsigned r2;
             double e2, n2;
@@ -2105,7 +2124,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             cand = nc; rising = r2; earliest = e2; narrowest = n2;
         } while (tHigh - tLow > narrowest);
         if (tHigh < t1o) {
-            interp(t1o, tHigh, YT);
+            if (!interp(t1o, tHigh, YT)) return -1;
             for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
             t1 = tHigh;
         }
@@ -2134,7 +2153,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
     a.ev_count[lane] = cnt;
     if (term && !modified) {
         ev.derivs(Y1, F1);
-        interp(t1, tLow, YT);
+        if (!interp(t1, tLow, YT)) return -1;
         for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
         t1 = tLow;
     }
@@ -2349,7 +2368,7 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         if constexpr (EVT) {
             if (W.n) {
                 const int r = step_events(a, ev, W, lane, nq, nu, nb, nquat, body_a,
-                                          a.T.prm + (size_t)ta.w * PRM_STRIDE, t, t1, Y, F0, Y1, FB, YT);
+                                          a.T.prm + (size_t)ta.w * PRM_STRIDE, t, t1, Y, F0, Y1, FB, YT, FA, YS, YE);
                 if (r < 0) { status = -5; break; }
                 if (r == 2) {
                     t = t1;

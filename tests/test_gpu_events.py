@@ -195,11 +195,11 @@ def test_settings_witnesses_cannot_run_with_are_refused():
     with pytest.raises(TdynError):
         B.stepToWithReports(q, u, rkm, 0.2, [0.1])
     B.stepTo(q, u, rkm, 0.1)
-    # with constraints
-    B.set_constraints(0, [(1, 0, 1, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), 1.0)])
+    # with prescribed motion
+    B.set_motion(B.to_device(np.zeros(4 * (spec.nb + 1) * 4)))
     with pytest.raises(TdynError):
         B.stepTo(q, u, rkm, 0.1)
-    B.set_constraints(0, [])
+    B.set_motion(None)
     for bad in ([dict(kind=O.WIT_Q, index=1)], [dict(kind=O.WIT_STATION_POS, body=2)],
                 [dict(kind=O.WIT_Q, trigger=0)], [dict(kind=O.WIT_Q, window=0.0)],
                 [dict(kind=9)], [dict(kind=O.WIT_Q, action=O.EV_SCALE_U, action_index=1)],
@@ -293,3 +293,47 @@ def test_batch_of_different_trees_matches_oracle():
         ref.append(_oracle(specs[t], wits[t], [qs[i]], [us[i]], 1e-6, 1.5, max_h=0.05)[0])
     assert sum(len(x["win"]) for x in ref) >= 2 * len(tree_of)
     _compare(g, ref, wtol=1e-6 * 0.1 * 0.1)
+
+
+def test_constrained_point_pendulum_matches_oracle():
+    """Witnesses with constraints (the reference's integrator-test pendulum:
+    a point mass held by a Rod, IntegratorTestFramework.h): trial and
+    backed-up states are projected onto the constraints (projectQ then
+    projectU at the tolerance), and the events, step counts and states
+    follow the oracle's.  The projections' rounding differs between the
+    two builds (sweeps against dense products), so windows agree to their
+    required width and states to 1e-8."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    from test_oracle_events import point_pendulum
+    spec, cons = point_pendulum()
+    wits = [dict(kind=O.WIT_Q, index=0, action=O.EV_SCALE_U, action_index=-1, factor=1.1),
+            dict(kind=O.WIT_TIME, offset=0.77),
+            dict(kind=O.WIT_STATION_POS, body=1, direction=(0, 1.0, 0), offset=-0.9)]
+    qs, us = [], []
+    for k in range(8):
+        a = -0.3 + 0.09 * k
+        qs.append(np.array([np.cos(a), np.sin(a), 0.0]))
+        us.append(np.array([-np.sin(a), np.cos(a), 0.0]) * (0.4 * k - 1.0))
+    B = MatterBatch([spec], n_instances=8)
+    B.set_constraints(0, cons)
+    B.set_witnesses(0, wits)
+    res = B.stepToWithEvents(B.pack(qs), B.pack(us), RungeKuttaMerson(accuracy=1e-7), 2.0)
+    g = dict(q=B.split_q(res["q"]), u=B.split_u(res["u"]))
+    for k in ("t", "status", "stats", "windows", "ids", "counts"):
+        g[k] = res[k].cpu().numpy()
+    o = OracleModel(spec)
+    assert o.set_constraints(cons) == 0
+    ref = []
+    for i in range(8):
+        assert o.set_witnesses(wits) == 0
+        y, st = o.integrate(np.concatenate([qs[i], us[i]]), 2.0, 1e-7)
+        win, ids, tf, term = o.last_events()
+        ref.append(dict(y=y, st=st, win=win, ids=ids, t=tf, term=term))
+    assert sum(len(r["win"]) for r in ref) >= 24
+    for i, r in enumerate(ref):
+        assert g["counts"][i] == len(r["win"]) and np.array_equal(g["ids"][i, :len(r["win"])], r["ids"]), i
+        assert np.abs(g["windows"][i, :len(r["win"])] - r["win"]).max() <= 1e-7 * 0.1 * 0.1 * 2, i
+        assert (r["st"]["accepted"], r["st"]["rejected"]) == tuple(g["stats"][i, 1:3]), (i, r["st"], g["stats"][i])
+        assert rel_inf(np.concatenate([g["q"][i], g["u"][i]]), r["y"]) < 1e-8, i
+        pe, ve, _ = o.constraint_errors(g["q"][i], g["u"][i])
+        assert abs(pe[0]) < 1e-8 and abs(ve[0]) < 1e-8

@@ -240,3 +240,53 @@ def test_two_valid_roundings_agree_to_the_window():
         assert (s0["accepted"], s0["rejected"]) == (s1["accepted"], s1["rejected"])
         assert np.abs(w0 - w1).max() <= 1e-6 * 0.1 * 0.1
         assert np.abs(y0 - y1).max() < 1e-10 * np.abs(y0).max()
+
+
+def point_pendulum(m=10.0):
+    """The reference's integrator-test pendulum (PendulumSystem.h): a point
+    mass with x, y (and z) coordinates held at distance 1 from the origin by
+    a constraint -- here a Translation body and a Constraint::Rod to Ground,
+    gravity along -y.  Returns (spec, constraints)."""
+    spec = M.TreeSpec(parent=np.array([0], np.int32), mobilizer=np.array([M.TRANSLATION], np.int32),
+                      mass=np.array([m]), com_B=np.zeros((1, 3)), inertia_B=np.array([[1e-3, 1e-3, 1e-3, 0, 0, 0]]),
+                      X_PF=np.array([M._xform()]), X_BM=np.array([M._xform()]), gravity=np.array([0.0, -G, 0.0]))
+    return spec, [(1, 0, 1, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), 1.0)]
+
+
+def test_constrained_pendulum_of_the_reference_integrator_tests():
+    """IntegratorTestFramework.h:67-170 on the constrained point pendulum:
+    the zero-position witness (x, both directions) at odd quarter periods,
+    the zero-velocity witness (x velocity, rising only) at the far left, the
+    y velocity rising through zero at the bottom,
+    the period of the elliptic integral 4 sqrt(L/g) K(1/2); multiplying the
+    velocity by sqrt(1.5) at each crossing multiplies the energy above the
+    bottom by 1.5; every interpolated and handled state stays on the
+    constraint."""
+    spec, cons = point_pendulum()
+    P = 4.0 * np.sqrt(1.0 / G) * ellipk(0.5)
+    o = OracleModel(spec)
+    assert o.set_constraints(cons) == 0
+    assert o.set_witnesses([dict(kind=O.WIT_Q, index=0), dict(kind=O.WIT_U, index=0, trigger=O.TRIG_RISING),
+                            dict(kind=O.WIT_U, index=1, trigger=O.TRIG_RISING)]) == 0
+    y, st = o.integrate(np.array([1.0, 0, 0, 0, 0, 0]), 2.2 * P, 1e-8)
+    assert st["status"] == 0
+    win, ids, tf, term = o.last_events()
+    tc = win.mean(axis=1)
+    assert np.allclose(tc[ids[:, 0] == 0], [P/4, 3*P/4, 5*P/4, 7*P/4], atol=1e-6)
+    # the x velocity has a triple zero at the far left (the velocity is
+    # vertical there), so its crossing is known only to about the cube root
+    # of the integration error; the y velocity's simple zero at the bottom
+    # is sharp
+    assert np.allclose(tc[ids[:, 0] == 1], [P/2, 3*P/2], atol=1e-3)
+    assert np.allclose(tc[ids[:, 0] == 2], [P/4, 3*P/4, 5*P/4, 7*P/4], atol=1e-6)
+    assert list(ids[ids[:, 0] == 0, 1]) == [O.TRIG_FALLING, O.TRIG_RISING] * 2
+    pe, ve, _ = o.constraint_errors(y[:3], y[3:])
+    assert abs(pe[0]) < 1e-9 and abs(ve[0]) < 1e-9
+    # energy injection at every crossing
+    assert o.set_witnesses([dict(kind=O.WIT_Q, index=0, action=O.EV_SCALE_U, action_index=-1,
+                                 factor=np.sqrt(1.5))]) == 0
+    y, st = o.integrate(np.array([1.0, 0, 0, 0, 0, 0]), 0.9 * P, 1e-8)
+    win, ids, tf, term = o.last_events()
+    assert st["status"] == 0 and len(win) == 2
+    E = 0.5 * 10.0 * (y[3]**2 + y[4]**2 + y[5]**2) + 10.0 * G * (y[1] + 1.0)
+    assert E == pytest.approx(10.0 * G * 1.5 ** 2, rel=1e-6)
