@@ -2,7 +2,8 @@
 would: the 10-link pendulum, realize(Acceleration) and
 TimeStepper::stepTo with RK-Merson at accuracy 1e-8, compared with the
 committed fixture (oracle results); multiplyByM and the composite mass of
-body 1 at the initial state."""
+body 1 at the initial state; the events of a reporter on the tip (against
+the oracle's) and a checkpoint resumed in a second batch."""
 import json
 import os
 import subprocess
@@ -39,3 +40,14 @@ def test_cpp_dropin_pendulum10():
     assert r["mass1"] == 10.0
     assert (r["attempts"], r["accepted"], r["rejected"]) == (g["attempts"], g["accepted"], g["rejected"])
     assert rel_inf(r["q1"] + r["u1"], g["q1"] + g["u1"]) < 1e-9
+    # events of a reporter on the x of body 10's origin, against the oracle
+    o = OracleModel(spec)
+    assert o.set_witnesses([dict(kind=2, body=10, direction=(1.0, 0, 0))]) == 0
+    y, st = o.integrate(np.concatenate([g["q"], g["u"]]), 1.0, 1e-8)
+    win, ids, tf, term = o.last_events()
+    assert r["events"] == len(win) > 0
+    assert np.abs(np.array(r["windows"]).reshape(-1, 2) - win).max() < 1e-9
+    assert r["transitions"] == list(ids[:, 1].astype(float))
+    assert rel_inf(r["q_events"], y[:10]) < 1e-8
+    # the checkpointed run resumed in a second batch ends in the same state
+    assert r["resume_same"] is True
