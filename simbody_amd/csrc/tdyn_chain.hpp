@@ -1010,7 +1010,9 @@ struct ChainEval {
         // pass 1: base to tip (stage input formed here)
         m33<K> R = ident<K>(); v3<K> p = zero3<K>(); sv<K> V = svzero<K>();
         {
-            Pre1 c1, n1;
+            // the raw sources two bodies ahead: a body's pass-1 work is
+            // shorter than a memory round trip
+            Pre1 c1, n1, n2;
             int bstart = 1;
             if (freeRoot) {
                 Pre1 c0;
@@ -1018,18 +1020,19 @@ struct ChainEval {
                 p1<FREE, STAGE>(c0, r, a0, 1, h, R, p, V);
                 bstart = 2;
             }
-            if (bstart <= nb) pre1_any<STAGE, false>(c1, r, ldi4(body_a, bstart-1));
-            int4 ba = ldi4(body_a, bstart-1);
+            int4 ba = ldi4(body_a, bstart-1), bn1 = ba;
+            if (bstart <= nb) pre1_any<STAGE, false>(c1, r, ba);
+            if (bstart < nb) { bn1 = ldi4(body_a, bstart); pre1_any<STAGE, false>(n1, r, bn1); }
             for (int b = bstart; b <= nb; ++b) {
-                int4 bn = ba;
-                if (b < nb) { bn = ldi4(body_a, b); pre1_any<STAGE, false>(n1, r, bn); }
+                int4 bn2 = bn1;
+                if (b + 1 < nb) { bn2 = ldi4(body_a, b + 1); pre1_any<STAGE, false>(n2, r, bn2); }
                 switch (ba.y) {
                 case PIN:       p1<PIN, STAGE>(c1, r, ba, b, h, R, p, V); break;
                 case SLIDER:    p1<SLIDER, STAGE>(c1, r, ba, b, h, R, p, V); break;
                 case UNIVERSAL: p1<UNIVERSAL, STAGE>(c1, r, ba, b, h, R, p, V); break;
                 default:        p1<BALL, STAGE>(c1, r, ba, b, h, R, p, V); break;
                 }
-                c1 = n1; ba = bn;
+                c1 = n1; n1 = n2; ba = bn1; bn1 = bn2;
             }
         }
         // pass 2: tip to base
