@@ -34,8 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE per dispatch (difference of a 10-step and a 1-step
 # launch, over 9 steps x 2,097,152 bodies), each scaled by what the same
 # counter reads for a plain 2 GiB device copy (FETCH x2.0, WRITE x1.0).
-TRAFFIC_FETCH_PER_BODY_STEP = 1602.2
-TRAFFIC_WRITE_PER_BODY_STEP = 1048.6
+TRAFFIC_FETCH_PER_BODY_STEP = 1592.9
+TRAFFIC_WRITE_PER_BODY_STEP = 1040.3
 
 
 def log(msg):
