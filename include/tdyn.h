@@ -330,10 +330,11 @@ typedef struct tdyn_witness_desc {
  * with or without constraints (whose rollouts need a batch of one tree;
  * trial and backed-up states are then projected onto them, projectQ and
  * projectU at the tolerance, as realizeAndProjectKinematicsWithThrow does,
- * IntegratorRep.h:858-897), without known udot, prescribed motion, reports
- * or tdyn_rollout_noalloc; status -5 marks an instance whose localization
- * or whose projection of an interpolated state failed.  A rollout in
- * progress ends (begin again). */
+ * IntegratorRep.h:858-897), with or without known udot and prescribed
+ * motion (trial and backed-up states then carry the prescribed q and u at
+ * their own times), not with reports or tdyn_rollout_noalloc; status -5
+ * marks an instance whose localization or whose projection of an
+ * interpolated state failed.  A rollout in progress ends (begin again). */
 int tdyn_batch_set_witnesses(tdyn_batch* b, int tree, int n, const tdyn_witness_desc* w);
 /* The witnesses' values at a state (realize(state) to their stages, then
  * State::getEventTriggers, State.h): q, u (packed), t (one time per

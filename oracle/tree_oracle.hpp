@@ -1795,8 +1795,9 @@ template <class T> struct Rkm {
     // createInterpolatedState (AbstractIntegratorRep.cpp:63-90) between the
     // step's start (t0, y0, f0) and its end (t1, y1, evF1): the cubic Hermite
     // interpolant (interpolateOrder3, IntegratorRep.h:299-313) at t, then the
-    // LocalOnly projection at the constraint tolerance
-    // (realizeAndProjectKinematicsWithThrow, IntegratorRep.h:858-897: with
+    // prescribed q and u at t and the LocalOnly projection at the constraint
+    // tolerance (realizeAndProjectKinematicsWithThrow, IntegratorRep.h:
+    // 858-897: prescribeQ, prescribeU; with
     // constraints projectQ then projectU, no projection limit, no error
     // estimate; without, the quaternions normalized when their RMS norm
     // error exceeds the tolerance).  False when the projection fails (the
@@ -1807,6 +1808,7 @@ template <class T> struct Rkm {
         const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
         const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*evF1[i];
+        if (motion) prescribe(t, yt);       // prescribeQ, prescribeU of the interpolated state
         if (ws.cons) {
             bool ch;
             const double inf = std::numeric_limits<double>::infinity();

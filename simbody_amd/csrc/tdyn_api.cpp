@@ -849,9 +849,9 @@ static int rollout_begin_impl(tdyn_batch* b, const double* q, const double* u, c
     if (b->any_cons && !b->uniform)
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: constraints in rollouts need a batch of one tree (a batch of "
                                 "different trees takes tdyn_calc_constrained_acceleration only)");
-    if (b->any_wit && (b->known_p || b->motion || b->evaluator == 2 || b->evaluator == 3 || prec != 0))
+    if (b->any_wit && (b->evaluator == 2 || b->evaluator == 3 || prec != 0))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: event witnesses run on the tree sweeps (evaluator 0 or 1), in "
-                                "FP64, without known udot or prescribed motion");
+                                "FP64");
     { const int rc = ensure_constraints(b); if (rc) return rc; }
     { const int rc = ensure_witnesses(b); if (rc) return rc; }
     const size_t L = b->n_inst;

@@ -2036,7 +2036,7 @@ __device__ void prescribe_lane(const RkmArgs& a, int lane, int tr, double t, con
 // trial, no convergence within 400 trials, or a failed projection of an
 // interpolated state).
 template <class EV, class TRows>
-__device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lane, int nq, int nu, int nb,
+__device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lane, int tr, int nq, int nu, int nb,
                            int nquat, const int4* body_a, const double* prm_tree, double t, double& t1,
                            const TRows& Y, const TRows& F0, const TRows& Y1, const TRows& F1, const TRows& YT,
                            const TRows& FA, const TRows& YS, const TRows& YE) {
@@ -2061,6 +2061,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
         const double cy1 = d*d*(3.0 - 2.0*d), cy0 = 1.0 - cy1;
         const double hdd1 = h*d*(d - 1.0), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int k = 0; k < ny; ++k) out(k) = ((cy0*Y(k) + cy1*Y1(k)) + cf0*F0(k)) + cf1*F1(k);
+        if constexpr (EV::kMotion) { if (a.motion) prescribe_lane(a, lane, tr, tm, out, true); }
         if constexpr (EV::kCons) {
             if (ev.K.m > 0) {
                 const TRows ou{out.base + (size_t)nq * out.L, out.L, out.lane};
@@ -2097,8 +2098,13 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
     };
     double tLow = t, tHigh = t1;
     const double t1o = t1;
-    if (tHigh - tLow > narrowest) {
+    // f(y1): with prescribed motion the known udot at the time of y1
+    auto end_derivs = [&](double te) {
+        if constexpr (EV::kMotion) { if (a.motion) prescribe_lane(a, lane, tr, te, Y1, true); }
         ev.derivs(Y1, F1);
+    };
+    if (tHigh - tLow > narrowest) {
+        end_derivs(t1o);
         double bias = 1.0;
         int side2 = 0, side1 = 0, it = 0;     // the half chosen two trials ago and at the last one
         do {
@@ -2152,7 +2158,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
     }
     a.ev_count[lane] = cnt;
     if (term && !modified) {
-        ev.derivs(Y1, F1);
+        end_derivs(t1);
         if (!interp(t1, tLow, YT)) return -1;
         for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
         t1 = tLow;
@@ -2367,7 +2373,7 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
         if (status < 0) break;
         if constexpr (EVT) {
             if (W.n) {
-                const int r = step_events(a, ev, W, lane, nq, nu, nb, nquat, body_a,
+                const int r = step_events(a, ev, W, lane, tr, nq, nu, nb, nquat, body_a,
                                           a.T.prm + (size_t)ta.w * PRM_STRIDE, t, t1, Y, F0, Y1, FB, YT, FA, YS, YE);
                 if (r < 0) { status = -5; break; }
                 if (r == 2) {

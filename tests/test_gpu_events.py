@@ -195,11 +195,6 @@ def test_settings_witnesses_cannot_run_with_are_refused():
     with pytest.raises(TdynError):
         B.stepToWithReports(q, u, rkm, 0.2, [0.1])
     B.stepTo(q, u, rkm, 0.1)
-    # with prescribed motion
-    B.set_motion(B.to_device(np.zeros(4 * (spec.nb + 1) * 4)))
-    with pytest.raises(TdynError):
-        B.stepTo(q, u, rkm, 0.1)
-    B.set_motion(None)
     for bad in ([dict(kind=O.WIT_Q, index=1)], [dict(kind=O.WIT_STATION_POS, body=2)],
                 [dict(kind=O.WIT_Q, trigger=0)], [dict(kind=O.WIT_Q, window=0.0)],
                 [dict(kind=9)], [dict(kind=O.WIT_Q, action=O.EV_SCALE_U, action_index=1)],
@@ -337,3 +332,81 @@ def test_constrained_point_pendulum_matches_oracle():
         assert rel_inf(np.concatenate([g["q"][i], g["u"][i]]), r["y"]) < 1e-8, i
         pe, ve, _ = o.constraint_errors(g["q"][i], g["u"][i])
         assert abs(pe[0]) < 1e-8 and abs(ve[0]) < 1e-8
+
+
+def _station_wits(spec, seed, n=3):
+    r = np.random.default_rng(seed)
+    w = [dict(kind=O.WIT_TIME, offset=0.037)]
+    for _ in range(n):
+        w.append(dict(kind=O.WIT_STATION_POS, body=int(r.integers(1, spec.nb + 1)), station=tuple(r.normal(size=3)),
+                      direction=tuple(r.normal(size=3)), offset=float(0.3 * r.normal())))
+    return w
+
+
+def _compare_runs(g, ref, n):
+    for i in range(n):
+        y, s, win, ids = ref[i]
+        assert s["status"] == 0
+        assert g["counts"][i] == len(win) and np.array_equal(g["ids"][i, :len(win)], ids), i
+        assert np.abs(g["windows"][i, :len(win)] - win).max(initial=0) < 1e-8, i
+        assert (s["accepted"], s["rejected"]) == tuple(g["stats"][i, 1:3]), (i, s, g["stats"][i])
+        assert rel_inf(np.concatenate([g["q"][i], g["u"][i]]), y) < 1e-8, i
+
+
+def _gpu_events(B, qs, us, fs, integ, T):
+    res = B.stepToWithEvents(B.pack(qs), B.pack(us), integ, T, fapp=B.pack(fs))
+    assert B.active_evaluator()[0] == 1
+    g = dict(q=B.split_q(res["q"]), u=B.split_u(res["u"]))
+    for k in ("stats", "windows", "ids", "counts"):
+        g[k] = res[k].cpu().numpy()
+    return g
+
+
+def test_witnesses_with_prescribed_motion_match_oracle():
+    """Witnesses on a chain whose root follows Motion::Sinusoid at position
+    level: trial and backed-up states carry the prescribed q and u at their
+    own times (createInterpolatedState's prescribeQ, prescribeU), the
+    derivative at a step's end its known udot."""
+    from simbody_amd.matter import MatterBatch, RungeKuttaMerson
+    from test_gpu_motion import _random_motion
+    spec = M.spatial_chain(8, seed=13, types=(M.PIN, M.UNIVERSAL, M.BALL))
+    n = 12
+    wits = _station_wits(spec, 5)
+    B = MatterBatch([spec], n_instances=n)
+    B.set_witnesses(0, wits)
+    qs, us, fs, ms = [], [], [], []
+    for i in range(n):
+        q, u = M.random_state(spec, 51, i)
+        m = _random_motion(spec, 5100 + i)
+        m[1] = (1, 0.4, 3.0, 0.1 * i)           # the root Pin at position level
+        qs.append(q); us.append(u); fs.append(M.random_body_forces(spec, 52, i)); ms.append(m)
+    B.set_motion(B.pack(ms))
+    g = _gpu_events(B, qs, us, fs, RungeKuttaMerson(accuracy=1e-6), 0.6)
+    o = OracleModel(spec)
+    assert o.set_witnesses(wits) == 0
+    ref = []
+    for i in range(n):
+        y, s = o.integrate_motion(np.concatenate([qs[i], us[i]]), 0.6, 1e-6, ms[i], Fapp=fs[i])
+        ref.append((y, s) + o.last_events()[:2])
+    assert sum(len(r[2]) for r in ref) >= 2 * n
+    _compare_runs(g, ref, n)
+
+
+def test_witnesses_with_known_udot_match_oracle():
+    """Witnesses on rollouts with locks and constant known udot."""
+    from simbody_amd.matter import RungeKuttaMerson
+    from test_gpu_prescribed import _known_case
+    per, B, qs, us, fs, masks, known = _known_case("shared")
+    spec = per[0]
+    wits = _station_wits(spec, 6)
+    B.set_witnesses(0, wits)
+    B.set_known_udot(B.pack_body_flags(masks), B.pack(known))
+    g = _gpu_events(B, qs, us, fs, RungeKuttaMerson(accuracy=1e-6), 0.3)
+    o = OracleModel(spec)
+    assert o.set_witnesses(wits) == 0
+    ref = []
+    for i in range(len(per)):
+        y, s = o.integrate_known(np.concatenate([qs[i], us[i]]), 0.3, 1e-6, masks[i], known[i], Fapp=fs[i])
+        ref.append((y, s) + o.last_events()[:2])
+    assert sum(len(r[2]) for r in ref) >= len(per)
+    _compare_runs(g, ref, len(per))
