@@ -239,21 +239,26 @@ def row_serving(torch, M, RungeKuttaMerson, rates=(1000, 8000, 32000), mixed_rat
     return out
 
 
-def row_tree_serving(torch, M, RungeKuttaMerson, rates=(1000, 4000, 16000), window=10.0, warmup=2.0, seed=7,
-                     budget_gib=24, steps_per_advance=16, n_models=2048):
+def row_tree_serving(torch, M, RungeKuttaMerson, rates=(1000, 8000, 32000), window=10.0, warmup=2.0, seed=7,
+                     budget_gib=24, steps_per_advance=16, n_models=2048, lanes=0):
     """Serving requests of different trees (TreeServer): Poisson arrivals for
     `window` seconds, the first `warmup` discarded; each request one uneven
     branched tree (nb ~ U{8..64}, depth <= 20, Pin/Slider/Universal/Ball and
     Free roots; n_models distinct trees reused cyclically), a seeded initial
     state, horizon ~ U(0.1, 1) s, RK-Merson accuracy 1e-6; each request's
-    memory is its own tree's, from pools sized by the HBM budget."""
+    memory is its own tree's, from pools sized by the HBM budget.  lanes: the
+    server's evaluator setting (0 level-parallel with lanes from each tree's
+    widest level, 1 the one-lane tree sweeps)."""
     from simbody_amd.serving import TreeRequests, TreeServer, serve
     models = [M.random_tree(seed, i, nmin=8, nmax=64) for i in range(n_models)]
     out = {"workload": f"Poisson arrivals for {window:g} s (first {warmup:g} s discarded); each request one "
                        "uneven tree, nb~U{8..64}, horizon U(0.1,1) s, RK-Merson accuracy 1e-6, FP64",
-           "hbm_budget_bytes": budget_gib << 30, "steps_per_advance": steps_per_advance, "sweep": []}
+           "hbm_budget_bytes": budget_gib << 30, "steps_per_advance": steps_per_advance,
+           "evaluator": "one-lane tree sweeps" if lanes == 1 else
+                        f"level-parallel, {'lanes from the widest level' if lanes == 0 else f'{lanes} lanes'}",
+           "sweep": []}
     for rate in rates:
-        sv = TreeServer(RungeKuttaMerson(accuracy=1e-6), budget_gib << 30, max_slots=65536)
+        sv = TreeServer(RungeKuttaMerson(accuracy=1e-6), budget_gib << 30, max_slots=65536, lanes=lanes)
         n = int(rate * window)
         specs = [models[i % n_models] for i in range(n)]
         qs, us = [None] * n, [None] * n
@@ -278,7 +283,8 @@ def row_tree_serving(torch, M, RungeKuttaMerson, rates=(1000, 4000, 16000), wind
                              "throughput_body_steps_s": body_steps / (window - warmup),
                              "latency_p50_s": st.get("latency_p50_s"), "latency_p99_s": st.get("latency_p99_s"),
                              "mean_bodies": float(nbs.mean()), "max_occupied_slots": int(r["max_occupied"]),
-                             "launches": int(r["launches"]), "wall_s": r["wall"]})
+                             "launches": int(r["launches"]), "wall_s": r["wall"],
+                             "admit_s": r["admit_s"], "advance_collect_s": r["advance_collect_s"]})
         log(f"[bench] serving trees: {out['sweep'][-1]}")
         del sv
     return out
@@ -419,6 +425,8 @@ def main():
                     help="automatic lanes per instance: min (0: from the batch size) and max")
     ap.add_argument("--no-tree-autotune", action="store_true",
                     help="uneven-trees row: keep the given lane settings instead of tdyn_batch_autotune's choice")
+    ap.add_argument("--rates", type=int, nargs="+", default=None,
+                    help="--only tree-serving: arrival rates (req/s) to sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 2 GiB (counter calibration for HBM traffic)")
     args = ap.parse_args()
@@ -461,7 +469,8 @@ def main():
         print(json.dumps(row_serving(torch, M, RungeKuttaMerson)), flush=True)
         return
     if args.only == "tree-serving":
-        print(json.dumps(row_tree_serving(torch, M, RungeKuttaMerson)), flush=True)
+        kw = {"rates": tuple(args.rates)} if args.rates else {}
+        print(json.dumps(row_tree_serving(torch, M, RungeKuttaMerson, lanes=args.tree_lanes, **kw)), flush=True)
         return
 
     h = 1e-3

@@ -582,20 +582,24 @@ int tdyn_server_collect(tdyn_server* sv, int max_out, int32_t* slots, double* q,
 
 /* ---- Serving requests of different trees ---------------------------------
  * No reference counterpart.  A tree server's slots hold requests of any tree
- * the tree sweeps run: any size, topology and mobilizer mix (the trees of
- * tdyn_batch_create), each with its own constants, gravity, initial state,
- * final time and applied body forces.  A request's memory comes from two
- * pools sized from the HBM budget at creation -- body tables (nb + 1
- * entries) and records plus integrator state (its own nq + nu) -- so its
- * byte cost is its own tree's (tdyn_tree_server_request_bytes).  Requests
- * are admitted in order while a slot is free and both pools have room for
- * the next one; collection returns the memory.  All slots advance together,
- * each with its own step control, and a slot's results do not depend on the
- * other slots.  FP64; not thread-safe (one host thread). */
+ * of tdyn_batch_create (any size, topology and mobilizer mix), each with its
+ * own constants, gravity, initial state, final time and applied body
+ * forces.  A request's memory comes from two pools sized from the HBM budget
+ * at creation -- body tables (nb + 1 entries, rounded up to whole blocks of
+ * four) and records plus integrator state (its own nq + nu) -- so its byte
+ * cost is its own tree's (tdyn_tree_server_request_bytes).  Requests are
+ * admitted in order while a slot is free and both pools have room for the
+ * next one; collection returns the memory.  All slots advance together, each
+ * with its own step control.  A request runs on the level-parallel tree
+ * evaluator (a group of lanes per request) when its tree allows it (no Free
+ * or Bushing body below body 1), else on the one-lane tree sweeps; the
+ * evaluator and its lane count are fixed at admission from the request's
+ * own tree, so a slot's results do not depend on the other slots.  FP64;
+ * not thread-safe (one host thread). */
 typedef struct tdyn_tree_server tdyn_tree_server;
 
 /* max_slots <= 0: 65,536 slots.  The budget pays the slots' own state (about
- * 130 bytes each), then a quarter of the rest goes to the body pool and the
+ * 160 bytes each), then a quarter of the rest goes to the body pool and the
  * remainder to the record and state pool. */
 int tdyn_tree_server_create(const tdyn_integrator_cfg* cfg, size_t hbm_budget_bytes, int max_slots,
                             tdyn_tree_server** out);
@@ -604,8 +608,15 @@ int tdyn_tree_server_destroy(tdyn_tree_server* sv);
  * (slots' own state plus the pools' occupied parts), body pool capacity
  * (bodies), record and state pool capacity (doubles). */
 int tdyn_tree_server_info(const tdyn_tree_server* sv, int64_t* info);
-/* What one request of this tree takes from the budget (bytes). */
+/* What one request of this tree takes from the budget (bytes), at the
+ * automatic lane setting. */
 int tdyn_tree_server_request_bytes(const tdyn_tree_desc* tree, int64_t* bytes);
+/* Evaluator of the requests admitted from now on: 0 (default) automatic --
+ * the level-parallel evaluator where the tree allows it, with lanes from the
+ * tree's widest level (8, 16 or 32); 1 the one-lane tree sweeps for every
+ * request; 4..64 (a power of two) the level-parallel evaluator with that
+ * many lanes where the tree allows it. */
+int tdyn_tree_server_set_lanes(tdyn_tree_server* sv, int lanes);
 /* Admit requests in order while a slot is free and the pools have room; host
  * arrays: trees[n], q and u packed request after request (each its own nq,
  * nu), t_end[n] (> 0; every request starts at t = 0), fapp packed

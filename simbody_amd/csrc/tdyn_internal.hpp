@@ -201,11 +201,12 @@ struct RkmArgs {
     int grp;                 // >1: level-parallel tree evaluator (vec_aos layout), run by the plan below
     const int4* plan;        // level-parallel evaluator, per wavefront: lanes per instance G, first entry of
                              // plan_idx, instance count, kind: 0 the wavefront runs those instances
-                             // (count <= 64/G), 1 its groups take instances from that range as a work
-                             // queue through plan_ctr
+                             // (count <= 64/G), k >= 1 its groups take instances from that range as a
+                             // work queue through plan_ctr[k - 1]
     const int* plan_idx;     // instances (lane slots) in plan order
     int plan_waves;
-    int* plan_ctr;           // the work queue's counter (zeroed before each launch)
+    int* plan_ctr;           // the work queues' counters (zeroed before each launch)
+    int plan_nctr;           // how many counters plan_ctr holds (0 counts as 1)
     const int* presc;        // known udot (tree sweeps only): per body (nb+1 per instance, packed) nonzero = known
     const double* udot_known; // the known udots (packed u layout)
     const double* motion;    // prescribed motion (tree sweeps, nullable): 4 per body (nb+1 per instance,
@@ -215,6 +216,8 @@ struct RkmArgs {
     const long long* pool;   // tree-server slots (nullable): per lane, where its state vectors start in vec
                              // (11 lane-contiguous vectors of its own nq+nu) and its records in scratch,
                              // in doubles; the lane's tree is tree `lane` of the tables
+    const unsigned char* pool_grp; // tree-server slots: 1 where the slot runs the level-parallel evaluator
+                             // (its records and tables in that evaluator's layout; the tree sweeps skip it)
     double* cons_ws;         // holonomic constraints (nullable: none in the batch): per-lane workspace
                              // (CWS_N doubles, lane-strided); the tree sweeps only
     double* ev_ws;           // event witnesses (nullable: none): per-lane workspace (EW_N doubles,

@@ -2176,6 +2176,8 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     const int lane = blockIdx.x * blockDim.x + threadIdx.x;
     if (lane >= a.n_inst) return;
     if (a.status[lane] != 0) return;
+    // a tree-server slot run by the level-parallel evaluator
+    if constexpr (std::is_same_v<EV, TreePoolEval>) { if (a.pool_grp && a.pool_grp[lane]) return; }
     const int tr = UNI ? 0 : a.tree_of[lane];
     const EV ev(a, lane, tr);
     const int4 ta = ldi4(a.T.tree_a, tr);
@@ -2497,10 +2499,13 @@ __global__ void k_srv_collect(RkmArgs a, const int* slots, int n, double* out, i
     a.status[slot] = 2;
 }
 // Tree-server admission: one 64-thread block per request.  ints: nb, nq, nu,
-// body_off, b_off, nquat, rec, 0, then body_a (4 nb), body_b (4 nb; child
-// begin already offset into the pool), child_idx (nb).  doubles: prm (nb
-// PRM_STRIDE), gravity (3), q (nq), u (nu), t_end, applied forces (6 (nb+1)),
-// the pool offsets of the slot's vectors and records (2).
+// body_off, b_off, nquat, rec, grp (1: the level-parallel evaluator), lvl
+// offset, prm_bfs offset, level count, 0, then body_a (4 nb), body_b (4 nb;
+// child begin already offset into the pool), child_idx (nb), and for grp
+// bfs (4 nb), bfs_b (4 nb) and the level list (level count + 3).  doubles:
+// prm (nb PRM_STRIDE), gravity (3), q (nq), u (nu), t_end, applied forces
+// (6 (nb+1)), the pool offsets of the slot's vectors and records (2); for
+// grp the breadth-first constants are built here from prm and bfs.
 __global__ void __launch_bounds__(64) k_tsrv_admit(RkmArgs a, const int* slots, int n, const long long* ioff,
                                                    const int* ints, const long long* doff, const double* dbl) {
     const int i = blockIdx.x;
@@ -2513,13 +2518,27 @@ __global__ void __launch_bounds__(64) k_tsrv_admit(RkmArgs a, const int* slots, 
     int4* body_b = const_cast<int4*>(a.T.body_b) + body_off;
     int* child = const_cast<int*>(a.T.child_idx) + body_off;
     double* prm = const_cast<double*>(a.T.prm) + (size_t)body_off * PRM_STRIDE;
-    const int* ba = I + 8;
+    const int grp = I[7], lvl_off = I[8], pbfs_off = I[9], nlev = I[10];
+    const int* ba = I + 12;
     const int* bb = ba + 4*nb;
     const int* ci = bb + 4*nb;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         body_a[k] = make_int4(ba[4*k], ba[4*k+1], ba[4*k+2], ba[4*k+3]);
         body_b[k] = make_int4(bb[4*k], bb[4*k+1], bb[4*k+2], bb[4*k+3]);
         child[k] = ci[k];
+    }
+    if (grp) {
+        const int* bf = ci + nb;
+        const int* bfb = bf + 4*nb;
+        const int* lv = bfb + 4*nb;
+        int4* bfs = const_cast<int4*>(a.T.bfs) + body_off;
+        int4* bfs_b = const_cast<int4*>(a.T.bfs_b) + body_off;
+        for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+            bfs[k] = make_int4(bf[4*k], bf[4*k+1], bf[4*k+2], bf[4*k+3]);
+            bfs_b[k] = make_int4(bfb[4*k], bfb[4*k+1], bfb[4*k+2], bfb[4*k+3]);
+        }
+        int* lvl = const_cast<int*>(a.T.lvl) + lvl_off;
+        for (int k = threadIdx.x; k < nlev + 3; k += blockDim.x) lvl[k] = lv[k];
     }
     for (int k = threadIdx.x; k < nb * PRM_STRIDE; k += blockDim.x) prm[k] = D[k];
     const double* g = D + nb * PRM_STRIDE;
@@ -2529,6 +2548,17 @@ __global__ void __launch_bounds__(64) k_tsrv_admit(RkmArgs a, const int* slots, 
     const double* fa = u + nu + 1;
     const double* po = fa + 6 * (nb + 1);
     const long long vbase = (long long)po[0], rbase = (long long)po[1];
+    if (grp) {
+        // the constants in breadth-first order, blocks of BW positions stored
+        // constant-major (padding positions zero), from the staged ones
+        const int BW = tgroup::BW, nd = tgroup::blocks(nb) * BW * PRM_STRIDE;
+        const int* bf = ci + nb;
+        double* pb = const_cast<double*>(a.T.prm_bfs) + pbfs_off;
+        for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+            const int blk = e / (PRM_STRIDE * BW), r = e % (PRM_STRIDE * BW), c = r / BW, k = blk * BW + r % BW;
+            pb[e] = k < nb ? D[(size_t)(bf[4*k] - 1) * PRM_STRIDE + c] : 0.0;
+        }
+    }
     double* y = a.vec + vbase;
     for (int k = threadIdx.x; k < nq; k += blockDim.x) y[k] = q[k];
     for (int k = threadIdx.x; k < nu; k += blockDim.x) y[nq + k] = u[k];
@@ -2537,7 +2567,8 @@ __global__ void __launch_bounds__(64) k_tsrv_admit(RkmArgs a, const int* slots, 
     __syncthreads();
     if (threadIdx.x == 0) {
         const_cast<int4*>(a.T.tree_a)[slot] = make_int4(nb, nq, nu, body_off);
-        const_cast<int4*>(a.T.tree_b)[slot] = make_int4(I[6], I[5], 0, 0);
+        const_cast<int4*>(a.T.tree_b)[slot] = make_int4(I[6], I[5], 0, grp ? lvl_off : 0);
+        const_cast<unsigned char*>(a.pool_grp)[slot] = (unsigned char)(grp ? 1 : 0);
         double* gr = const_cast<double*>(a.T.gravity) + 4 * slot;
         gr[0] = g[0]; gr[1] = g[1]; gr[2] = g[2]; gr[3] = 0.0;
         const_cast<int*>(a.off.b_off)[slot] = b_off;

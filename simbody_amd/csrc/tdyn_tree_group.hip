@@ -101,6 +101,12 @@ struct GroupEval {
     const int* pm; const double* uk;   // known udot: per-body flags (Ground first) and values, or null
     bool free0;
     static __device__ __forceinline__ VR vrow(const RkmArgs& a, int v, int inst) {
+        if (a.pool) {
+            // a tree-server slot: its 11 vectors of its own nq + nu, where
+            // the slot's pool entry puts them (the slot's tree is tree inst)
+            const int4 ta = ldi4(a.T.tree_a, inst);
+            return VR{a.vec + a.pool[2*inst] + (size_t)v * (ta.y + ta.z)};
+        }
         return VR{a.vec + ((size_t)v * a.L + inst) * a.ny_max};
     }
     __device__ GroupEval(const RkmArgs& a, int inst, int tr, int lane_in_group, int lanes, double* frontier)
@@ -111,7 +117,7 @@ struct GroupEval {
         bfs = a.T.bfs + body_off; bfs_b = a.T.bfs_b + body_off;
         const int* lv = a.T.lvl + ldi4(a.T.tree_b, tr).w;
         nlev = lv[0]; prm = a.T.prm_bfs + lv[1]; ls = lv + 2;
-        S = a.scratch + (size_t)inst * a.rec_stride;
+        S = a.scratch + (a.pool ? a.pool[2*inst + 1] : (long long)inst * a.rec_stride);
         const double* g = a.T.gravity + 4*tr;
         grav = V3(g[0], g[1], g[2]);
         fapp = a.fapp ? a.fapp + 6*(size_t)a.off.b_off[inst] : nullptr;
@@ -430,10 +436,11 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
 }
 
 // Work distribution: a plan of wavefronts.  Kind 0 runs up to 64/G given
-// instances with G lanes each; kind 1 (persistent wavefronts, as many as
-// the device keeps resident) has each of its 64/G groups take instances
-// from a work queue (a range of plan_idx in decreasing order of cost) until
-// the queue is empty.  The lanes of a wavefront reconverge after an
+// instances with G lanes each; kind k >= 1 (persistent wavefronts, as many
+// as the device keeps resident) has each of its 64/G groups take instances
+// from work queue k (a range of plan_idx in decreasing order of cost, its
+// counter plan_ctr[k - 1]; the tree server keeps one queue per lane count)
+// until the queue is empty.  The lanes of a wavefront reconverge after an
 // instance, so its groups take their next instances together; the queue
 // balances the wavefronts against each other (a wavefront whose instances
 // needed few steps takes more of them) and starts the costliest first.
@@ -463,7 +470,7 @@ __device__ __forceinline__ void plan_wave(const RkmArgs& a, int G, int start, in
     int k = grp;
     for (;;) {
         if (kind != 0) {
-            if (l == 0) k = atomicAdd(a.plan_ctr, 1);
+            if (l == 0) k = atomicAdd(a.plan_ctr + (kind - 1), 1);
             k = __shfl(k, leader, 64);
         }
         if (k >= count) break;
@@ -480,7 +487,9 @@ __global__ void __launch_bounds__(64) k_rkm_plan(RkmArgs a) {
     plan_wave<IDF, F0>(a, G, __builtin_amdgcn_readfirstlane(w.y), __builtin_amdgcn_readfirstlane(w.z),
                        __builtin_amdgcn_readfirstlane(w.w));
 }
-__global__ void k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_int(int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
 
 } // namespace
 
@@ -492,7 +501,7 @@ __global__ void k_zero_int(int* p) { *p = 0; }
 hipError_t launch_tree_group(const RkmArgs& a, hipStream_t s, bool f0_only) {
     if (a.n_inst <= 0 || a.plan_waves <= 0) return hipSuccess;
     const dim3 grd((unsigned)a.plan_waves), blk(64);
-    k_zero_int<<<1, 1, 0, s>>>(a.plan_ctr);
+    k_zero_int<<<1, 64, 0, s>>>(a.plan_ctr, a.plan_nctr > 0 ? a.plan_nctr : 1);
     if (a.ext) {
         if (f0_only) k_rkm_plan<2, true><<<grd, blk, kFrontierBytes, s>>>(a);
         else k_rkm_plan<2, false><<<grd, blk, kFrontierBytes, s>>>(a);

@@ -48,7 +48,7 @@ EXPORTS = ["tdyn_batch_create", "tdyn_batch_destroy", "tdyn_batch_set_evaluator"
            "tdyn_server_advance", "tdyn_server_collect",
            "tdyn_tree_server_create", "tdyn_tree_server_destroy", "tdyn_tree_server_info",
            "tdyn_tree_server_request_bytes", "tdyn_tree_server_admit", "tdyn_tree_server_advance",
-           "tdyn_tree_server_collect"]
+           "tdyn_tree_server_collect", "tdyn_tree_server_set_lanes"]
 
 
 class TdynError(RuntimeError):
@@ -179,6 +179,7 @@ def lib():
         L.tdyn_tree_server_admit.argtypes = [vp, ctypes.c_int, ctypes.POINTER(TreeDescC), _d, _d, _d, _d, _i32,
                                              ctypes.POINTER(ctypes.c_int), vp]
         L.tdyn_tree_server_advance.argtypes = [vp, ctypes.c_int, vp]
+        L.tdyn_tree_server_set_lanes.argtypes = [vp, ctypes.c_int]
         L.tdyn_tree_server_collect.argtypes = [vp, ctypes.c_int, ctypes.c_int64, _i32, _d,
                                                ctypes.POINTER(ctypes.c_int64), _d, _i32, _i32,
                                                ctypes.POINTER(ctypes.c_int), vp]

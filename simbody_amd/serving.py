@@ -102,9 +102,11 @@ class Server:
 class TreeServer:
     """Slots for requests of different trees (tdyn_tree_server_*): any size,
     topology and mobilizer mix; each request's memory comes from the
-    server's pools, sized from the HBM budget, by its own tree's needs."""
+    server's pools, sized from the HBM budget, by its own tree's needs.
+    lanes: 0 the level-parallel evaluator where a tree allows it (lanes from
+    its widest level), 1 the one-lane tree sweeps, 4..64 fixed lanes."""
 
-    def __init__(self, integ, hbm_budget_bytes, max_slots=0, device="cuda:0"):
+    def __init__(self, integ, hbm_budget_bytes, max_slots=0, device="cuda:0", lanes=0):
         import torch
         self.device = torch.device(device)
         cfg = integ.c()
@@ -113,6 +115,7 @@ class TreeServer:
             _check(lib().tdyn_tree_server_create(ctypes.byref(cfg), ctypes.c_size_t(int(hbm_budget_bytes)),
                                                  int(max_slots), ctypes.byref(h)))
         self.h = h
+        self.set_lanes(lanes)
         self.slots = self.info()["slots"]
         self._ny = np.zeros(self.slots, np.int64)     # state size of the request in each slot
         self._y = np.zeros(1 << 12)
@@ -137,6 +140,10 @@ class TreeServer:
         _check(lib().tdyn_tree_server_info(self.h, a))
         return dict(zip(["slots", "free", "occupied", "budget_bytes", "bytes_in_use", "body_pool", "state_pool"],
                         list(a)))
+
+    def set_lanes(self, lanes):
+        """Evaluator of the requests admitted from now on (see the class)."""
+        _check(lib().tdyn_tree_server_set_lanes(self.h, int(lanes)))
 
     @staticmethod
     def request_bytes(spec):
@@ -335,6 +342,7 @@ def serve(server, req, arrivals, steps_per_advance=16, time_limit=None, warmup=0
     head = 0            # next request to admit (FIFO); requests head..nxt-1 are queued
     nxt = 0; n_done = 0; occupied = 0
     launches = 0; max_occupied = 0
+    t_admit = 0.0; t_step = 0.0       # host wall time in admission / in advance + collect
     t0 = time.perf_counter()
     while n_done < n:
         now = time.perf_counter() - t0
@@ -344,7 +352,9 @@ def serve(server, req, arrivals, steps_per_advance=16, time_limit=None, warmup=0
         free = server.slots - occupied
         if nxt > head and free > 0:
             k = min(free, nxt - head)
+            ta = time.perf_counter()
             slots = server.admit_requests(req, head, k)
+            t_admit += time.perf_counter() - ta
             kk = len(slots)
             slot_req[slots] = np.arange(head, head + kk)
             admitted_at[head:head + kk] = time.perf_counter() - t0
@@ -357,10 +367,12 @@ def serve(server, req, arrivals, steps_per_advance=16, time_limit=None, warmup=0
                 if wait > 0:
                     time.sleep(min(wait, 0.01))
             continue
+        ta = time.perf_counter()
         server.advance(steps_per_advance)
         launches += 1
         res = server.collect()
         tdone = time.perf_counter() - t0
+        t_step += time.perf_counter() - ta
         ids = slot_req[res["slots"]]
         slot_req[res["slots"]] = -1
         if trees:
@@ -375,7 +387,8 @@ def serve(server, req, arrivals, steps_per_advance=16, time_limit=None, warmup=0
         n_done += k; occupied -= k
     wall = time.perf_counter() - t0
     out = dict(status=status, stats=stats, latency=latency, admitted_at=admitted_at, done_at=done_at,
-               wall=wall, launches=launches, max_occupied=max_occupied, completed=n_done)
+               wall=wall, launches=launches, max_occupied=max_occupied, completed=n_done,
+               admit_s=t_admit, advance_collect_s=t_step)
     if trees:
         out["y"] = out_y
     else:

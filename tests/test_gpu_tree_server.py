@@ -3,19 +3,22 @@ simbody_amd.serving.TreeServer).
 
 Requests of different sizes, topologies and mobilizers (serial chains,
 branched trees with Free roots, trees mixing the ten mobilizers, reversed
-mobilizers) share one server.  Each slot runs the tree sweeps on its own
-pooled memory, so a request's result is the same alone and in the mix, to
-the bit; against the oracle the trajectory bar of the rollout parity tests
-holds (identical accepted and rejected counts, states within 1e-8
-relative).  Memory is the request's own: a larger tree costs more bytes,
-the pools admit requests until they are full, and give the memory back when
-the requests are collected."""
+mobilizers, Free bodies below body 1) share one server.  Each slot runs on
+its own pooled memory, on the level-parallel evaluator with lanes fixed by
+its own tree or (a Free body below body 1, or lanes 1) on the one-lane tree
+sweeps, so a request's result is the same alone and in the mix, to the bit;
+against the oracle the trajectory bar of the rollout parity tests holds
+(identical accepted and rejected counts, states within 1e-8 relative) under
+every lane setting.  Memory is the request's own: a larger tree costs more
+bytes, the pools admit requests until they are full, and give the memory
+back when the requests are collected."""
 import numpy as np
 import pytest
 
 from common import ext_trees, mixed_trees, rel_inf, rev_trees
 from simbody_amd import models as M
 from oracle_ctypes import OracleModel
+from test_oracle_free_offground import free_mid_trees
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -23,14 +26,17 @@ pytestmark = pytest.mark.gpu
 ACC = 1e-6
 
 
-def _requests(n, seed):
+def _requests(n, seed, free_mid=False):
     """A mix of trees: 12-body chains, random trees with Free roots, trees of
-    the ten mobilizers, trees with reversed mobilizers; sizes 4 to 40."""
+    the ten mobilizers, trees with reversed mobilizers; sizes 4 to 40; with
+    free_mid, also trees with Free bodies below body 1 (tree sweeps)."""
     specs = []
     gens = [lambda i: M.chain_spec(M.chain_params(seed, [i], nb=12), 0),
             lambda i: mixed_trees(seed * 3 + i, 1, nmin=4, nmax=40)[0],
             lambda i: ext_trees(seed * 5 + i, 1, nmin=4, nmax=24)[0],
             lambda i: rev_trees(seed * 7 + i, 1, nmin=4, nmax=24)[0]]
+    if free_mid:
+        gens.append(lambda i: free_mid_trees(seed * 11 + i, 1, nmin=4, nmax=24)[0])
     for i in range(n):
         specs.append(gens[i % len(gens)](i))
     ys = [M.random_state(t, seed, i) for i, t in enumerate(specs)]
@@ -39,10 +45,10 @@ def _requests(n, seed):
     return specs, [y[0] for y in ys], [y[1] for y in ys], t_end, F
 
 
-def _server(slots=64, budget=1 << 28):
+def _server(slots=64, budget=1 << 28, lanes=0):
     from simbody_amd.matter import RungeKuttaMerson
     from simbody_amd.serving import TreeServer
-    return TreeServer(RungeKuttaMerson(accuracy=ACC), budget, max_slots=slots)
+    return TreeServer(RungeKuttaMerson(accuracy=ACC), budget, max_slots=slots, lanes=lanes)
 
 
 def _run_all(sv, specs, q, u, t_end, F=None, steps=3):
@@ -66,9 +72,14 @@ def _run_all(sv, specs, q, u, t_end, F=None, steps=3):
     return out
 
 
-def test_mixed_trees_match_the_oracle():
-    specs, q, u, t_end, F = _requests(16, 81)
-    out = _run_all(_server(), specs, q, u, t_end, F)
+@pytest.mark.parametrize("lanes", [0, 1, 4, 32])
+def test_mixed_trees_match_the_oracle(lanes):
+    """Every lane setting: automatic (level-parallel, lanes from each tree's
+    widest level), 1 (every request on the tree sweeps), 4 and 32 lanes; the
+    mix includes trees with Free bodies below body 1, which take the tree
+    sweeps under every setting."""
+    specs, q, u, t_end, F = _requests(20, 81, free_mid=True)
+    out = _run_all(_server(lanes=lanes), specs, q, u, t_end, F)
     for i, t in enumerate(specs):
         y, ts, st, stats = out[i]
         assert st == 1 and ts == t_end[i], (i, st, ts)
@@ -81,8 +92,8 @@ def test_result_same_alone_and_in_the_mix():
     """Probes served alone, then in a mix of 90 requests of different trees
     admitted in waves into 32 slots (probes wait for slots and pool memory):
     bitwise the same."""
-    specs, q, u, t_end, F = _requests(90, 83)
-    probes = [1, 6, 23, 46, 70, 89]
+    specs, q, u, t_end, F = _requests(90, 83, free_mid=True)
+    probes = [1, 6, 12, 23, 46, 69, 70, 89]
     alone = {i: _run_all(_server(slots=64), [specs[i]], [q[i]], [u[i]], t_end[[i]], [F[i]])[0] for i in probes}
     mix = _run_all(_server(slots=32, budget=24 << 20), specs, q, u, t_end, F, steps=2)
     for i in probes:
