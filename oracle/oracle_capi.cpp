@@ -22,9 +22,11 @@ struct Handle {
     std::vector<WitnessDef> wit; // event witnesses of the model
     std::vector<EventRecord> events;   // the events of the last integration
     RkmStats last;                     // and its statistics (final time, termination)
+    bool inf_norm = false, project_every_step = false, project_interpolated = true;   // integrator options
 };
-// Attaches the model's constraints and witnesses to an integrator.
+// Attaches the model's constraints, witnesses and integrator options to an integrator.
 void attach(Handle* h, Rkm<double>& rk) {
+    rk.infNorm = h->inf_norm; rk.forceProj = h->project_every_step; rk.projInterp = h->project_interpolated;
     if (!h->cons.empty()) rk.cons = &h->cons;
     h->events.clear();
     if (!h->wit.empty()) { rk.wit = &h->wit; rk.events = &h->events; }
@@ -103,6 +105,16 @@ int oracle_loop_forward(void* p, const double* q, const double* u, const double*
 // direction (3), offset, factor, window.  Later integrations (not the one
 // with reports by interpolation) localize and handle their events.  Returns
 // -1 for an invalid description.
+// Integrator options of the model's later integrations: infinity norm
+// (Integrator::setUseInfinityNorm), forced projection on every step
+// (setProjectEveryStep), projection of interpolated states
+// (setProjectInterpolatedStates, default on).
+int oracle_set_integrator_options(void* p, int inf_norm, int project_every_step, int project_interpolated) {
+    Handle* h = H(p);
+    h->inf_norm = inf_norm != 0; h->project_every_step = project_every_step != 0;
+    h->project_interpolated = project_interpolated != 0;
+    return 0;
+}
 int oracle_set_witnesses(void* p, int n, const int* ib, const double* dd) {
     Handle* h = H(p); const Model& M = h->m;
     std::vector<WitnessDef> ws;

@@ -53,7 +53,7 @@ def _open(path):
                  "oracle_integrate_trace", "oracle_abi_shift", "oracle_integrate_motion",
                  "oracle_set_constraints", "oracle_loop_forward", "oracle_constraint_errors",
                  "oracle_set_witnesses", "oracle_witness_values", "oracle_last_events",
-                 "oracle_integrate_cfg"]:
+                 "oracle_integrate_cfg", "oracle_set_integrator_options"]:
         getattr(L, name).restype = ctypes.c_int
     L.oracle_estimate_root_time.restype = ctypes.c_double
     L.oracle_estimate_root_time.argtypes = [ctypes.c_double] * 6
@@ -263,6 +263,12 @@ class OracleModel:
         ud = None if udot is None else np.ascontiguousarray(udot, np.float64)
         self._L.oracle_body_acceleration(self.h, _p(q), _p(u), _p(ud), _p(A))
         return A
+
+    def set_integrator_options(self, inf_norm=False, project_every_step=False, project_interpolated=True):
+        """Options of later integrations: Integrator::setUseInfinityNorm,
+        setProjectEveryStep, setProjectInterpolatedStates."""
+        return self._L.oracle_set_integrator_options(self.h, int(bool(inf_norm)), int(bool(project_every_step)),
+                                                     int(bool(project_interpolated)))
 
     def set_witnesses(self, witnesses):
         """Event witnesses: a list of dicts with `kind` and any of index,

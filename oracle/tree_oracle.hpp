@@ -1468,6 +1468,13 @@ template <class T> struct Rkm {
     // error and u weight 1, StateImpl.h:508-566): projectQ
     // (SimbodyMatterSubsystemRep.cpp:4096-4440) and projectU (:4666-4934).
     const std::vector<ConsDef>* cons = nullptr;
+    // Integrator options (Integrator::setUseInfinityNorm, setProjectEveryStep,
+    // setProjectInterpolatedStates; IntegratorRep.h:708-712, 780-790,
+    // 858-875): every error and constraint norm the largest magnitude
+    // instead of the RMS; ForceProjection on every projection (quaternions
+    // normalized, one Newton iteration, even within tolerance); interpolated
+    // report and localization states left unprojected.
+    bool infNorm = false, forceProj = false, projInterp = true;
     std::vector<T> cP;
     // P (m x nu, row-major) at the current position kinematics: column j
     // holds the constraint rows of J e_j.
@@ -1506,8 +1513,16 @@ template <class T> struct Rkm {
             } else for (int k = 0; k < M.dof[b]; ++k) dq[qi+k] = du[ui+k];
         }
     }
-    T rmsOf(const std::vector<T>& v) const {
+    // Norm of constraint errors: RMS, or the largest magnitude under
+    // setUseInfinityNorm (ProjectOptions::UseInfinityNorm, SimbodyMatter-
+    // SubsystemRep.cpp:4108, 4129, 4700).
+    T consNorm(const std::vector<T>& v) const {
         if (v.empty()) return T(0);
+        if (infNorm) {
+            T m = T(0);
+            for (const T& x : v) { const T a = T(fabs(double(x))); if (a > m) m = a; }
+            return m;
+        }
         T s = T(0);
         for (const T& x : v) s = s + x*x;
         return sqrt(s/T(v.size()));
@@ -1569,12 +1584,14 @@ template <class T> struct Rkm {
         std::vector<T> perr(m);
         realizePosition(tr, q);
         consPerr(tr, *cons, perr.data());
-        const double pn0 = double(rmsOf(perr)), qn0 = double(quatErrRMS(q));
+        const double pn0 = double(consNorm(perr)), qn0 = double(quatErrNorm(q));
+        // ForceProjection: at initialize, or every step (setProjectEveryStep)
+        const bool force = init || forceProj;
         if (std::max(pn0, qn0) > projLimit) return false;
-        if (pn0 == 0 || (pn0 <= consTol && !init)) {
-            if (qn0 > consTol || init) {
+        if (pn0 == 0 || (pn0 <= consTol && !force)) {
+            if (qn0 > consTol || force) {
                 if (M.nquat) { normalizeQuats(q, qerr); *changed = true; }
-                if (double(quatErrRMS(q)) > consTol) return false;
+                if (double(quatErrNorm(q)) > consTol) return false;
             }
             return true;
         }
@@ -1588,12 +1605,12 @@ template <class T> struct Rkm {
             *changed = true; ++its;
             realizePosition(tr, q);
             consPerr(tr, *cons, perr.data());
-            pn = double(rmsOf(perr));
+            pn = double(consNorm(perr));
             if (!init && its >= 2 && pn > prev) {
                 for (int j = 0; j < nq; ++j) q[j] = q[j] + dq[j];
                 realizePosition(tr, q);
                 consPerr(tr, *cons, perr.data());
-                pn = double(rmsOf(perr));
+                pn = double(consNorm(perr));
                 break;
             }
             prev = pn;
@@ -1614,7 +1631,7 @@ template <class T> struct Rkm {
         }
         if (M.nquat) {
             normalizeQuats(q, qerr);
-            if (double(quatErrRMS(q)) > consTol) return false;
+            if (double(quatErrNorm(q)) > consTol) return false;
         }
         return true;
     }
@@ -1632,12 +1649,12 @@ template <class T> struct Rkm {
             realizeVelocity(tr, q, u, (T*)nullptr);
             for (int b = 0; b <= M.nb; ++b) V[b] = tr.B[b].V_GB;
             consRowsOf(tr, *cons, V.data(), verr.data());
-            return double(rmsOf(verr));
+            return double(consNorm(verr));
         };
         realizePosition(tr, q);
         const double vn0 = velErr();
         if (vn0 > projLimit) return false;
-        if (vn0 == 0 || (vn0 <= consTol && !init)) return true;
+        if (vn0 == 0 || (vn0 <= consTol && !(init || forceProj))) return true;
         const double target = std::max(0.1*consTol, Significant);
         std::vector<T> sc(nu), save(u, u + nu), du(nu), y(m);
         for (int i = 0; i < nu; ++i) { const T a = T(fabs(double(u[i]))); sc[i] = a > T(1) ? a : T(1); }
@@ -1704,21 +1721,29 @@ template <class T> struct Rkm {
             }
         }
     }
-    // RMS of (|quat|-1) over the quaternions of q; NaN-safe.
-    T quatErrRMS(const T* q) const {
+    // RMS of (|quat|-1) over the quaternions of q, or its largest magnitude
+    // under the infinity norm; NaN-safe (a NaN error propagates).
+    T quatErrNorm(const T* q) const {
         if (M.nquat == 0) return T(0);
         T s = T(0);
         for (int b=1;b<=M.nb;++b) {
             if (!M.isQuat(b)) continue;
             const T* qq = q + M.qIndex[b];
             const T e = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - T(1);
-            s = s + e*e;
+            if (infNorm) { const T a = T(fabs(double(e))); s = (a > s || std::isnan(double(a))) ? a : s; }
+            else s = s + e*e;
         }
-        return sqrt(s/T(M.nquat));
+        return infNorm ? s : sqrt(s/T(M.nquat));
     }
-    // max(RMS(N Wu NInv dq), RMS(uScale*du)), Wu = 1 (SimbodyMatterSubsystemRep.cpp:1630-1634).
+    // max(RMS(N Wu NInv dq), RMS(uScale*du)), Wu = 1 (SimbodyMatterSubsystemRep.cpp:1630-1634);
+    // under setUseInfinityNorm the largest magnitudes instead of the RMS
+    // (calcWeightedInfNormQ, calcWeightedInfNorm, IntegratorRep.h:459-465).
     T errorNorm(const T* q, const T* err) const {
         T sq = T(0);
+        auto acc = [&](T& s, T x) {
+            if (infNorm) { const T a = T(fabs(double(x))); s = (a > s || std::isnan(double(a))) ? a : s; }
+            else s = s + x*x;
+        };
         for (int b=1;b<=M.nb;++b) {
             const int t = M.type[b];
             const T* e = err + M.qIndex[b];
@@ -1726,15 +1751,15 @@ template <class T> struct Rkm {
                 const T* qq = q + M.qIndex[b];
                 const V3<T> w = quatDotToW(qq, e);
                 T d[4]; quatDot(qq, w, d);
-                for (int i=0;i<4;++i) sq = sq + d[i]*d[i];
-                if (t == FREE) for (int i=4;i<7;++i) sq = sq + e[i]*e[i];
-            } else for (int i=0;i<M.dof[b];++i) sq = sq + e[i]*e[i];
+                for (int i=0;i<4;++i) acc(sq, d[i]);
+                if (t == FREE) for (int i=4;i<7;++i) acc(sq, e[i]);
+            } else for (int i=0;i<M.dof[b];++i) acc(sq, e[i]);
         }
-        const T qn = M.nq ? sqrt(sq/T(M.nq)) : T(0);
+        const T qn = M.nq ? (infNorm ? sq : sqrt(sq/T(M.nq))) : T(0);
         T su = T(0);
         const T* eu = err + M.nq;
-        for (int i=0;i<M.nu;++i) { const T w = uscale[i]*eu[i]; su = su + w*w; }
-        const T un = M.nu ? sqrt(su/T(M.nu)) : T(0);
+        for (int i=0;i<M.nu;++i) acc(su, uscale[i]*eu[i]);
+        const T un = M.nu ? (infNorm ? su : sqrt(su/T(M.nu))) : T(0);
         return qn >= un ? qn : un;
     }
 
@@ -1802,20 +1827,24 @@ template <class T> struct Rkm {
     // estimate; without, the quaternions normalized when their RMS norm
     // error exceeds the tolerance).  False when the projection fails (the
     // reference throws).
-    bool interpLocal(double t0, double t1, double t, double consTol, T* yt) {
+    bool interpLocal(double t0, double t1, double t, double consTol, T* yt, bool backUp = false) {
         const int ny = M.nq + M.nu;
         const T h = T(t1 - t0), d = T((t - t0) / (t1 - t0));
         const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
         const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*evF1[i];
         if (motion) prescribe(t, yt);       // prescribeQ, prescribeU of the interpolated state
+        // setProjectInterpolatedStates(false) leaves interpolated states
+        // unprojected, but not the advanced state backed up to an event
+        // (AbstractIntegratorRep.cpp:78-85, 116-126)
+        if (!backUp && !projInterp) return true;
         if (ws.cons) {
             bool ch;
             const double inf = std::numeric_limits<double>::infinity();
             return projectQ(yt, (T*)nullptr, consTol, inf, false, &ch) &&
                    projectU(yt, yt + M.nq, (T*)nullptr, consTol, inf, false, &ch);
         }
-        if (M.nquat > 0 && double(quatErrRMS(yt)) > consTol) normalizeQuats(yt, (T*)nullptr);
+        if (M.nquat > 0 && (double(quatErrNorm(yt)) > consTol || forceProj)) normalizeQuats(yt, (T*)nullptr);
         return true;
     }
     // Events of the step just accepted from (t, y0, f0) to (t1, y1)
@@ -1870,7 +1899,7 @@ template <class T> struct Rkm {
                 tLow = tMid; eLow = eMid; C = N;
             } while (tHigh - tLow > C.narrowest);
             if (tHigh < t1) {
-                if (!interpLocal(t, t1, tHigh, consTol, evY.data())) return -1;
+                if (!interpLocal(t, t1, tHigh, consTol, evY.data(), true)) return -1;
                 for (int i=0;i<ny;++i) y1[i] = evY[i];
                 t1 = tHigh;
             }
@@ -1991,7 +2020,7 @@ template <class T> struct Rkm {
                     const double e1 = double(errorNorm(y1.data(), yerr.data()));
                     errPre = e1;
                     if (!(e1 > 16.0*acc)) {   // NaN falls through to projection like the reference's comparison
-                        const double qe = double(quatErrRMS(y1.data()));
+                        const double qe = double(quatErrNorm(y1.data()));
                         qErr = qe;
                         if (hasCons) {
                             // attemptDAEStep (AbstractIntegratorRep.cpp:183-207)
@@ -2002,7 +2031,9 @@ template <class T> struct Rkm {
                                 converged = false;
                             if (chq) { ++st.projections; projected = true; }
                         } else if (qe > projectionLimit) converged = false;
-                        else if (qe > consTol) { normalizeQuats(y1.data(), yerr.data()); ++st.projections; projected = true; }
+                        else if (M.nquat > 0 && (qe > consTol || forceProj)) {
+                            normalizeQuats(y1.data(), yerr.data()); ++st.projections; projected = true;
+                        }
                     }
                 }
                 errNorm = converged ? double(errorNorm(y1.data(), yerr.data()))
@@ -2046,7 +2077,8 @@ template <class T> struct Rkm {
                     const T cy1 = d*d*(T(3) - T(2)*d), cy0 = T(1) - cy1;
                     const T hdd1 = h*d*(d - T(1)), cf1 = hdd1*d, cf0 = cf1 - hdd1;
                     for (int i=0;i<ny;++i) yt[i] = ((cy0*y0[i] + cy1*y1[i]) + cf0*f0[i]) + cf1*f1[i];
-                    if (M.nquat > 0 && double(quatErrRMS(yt)) > consTol) normalizeQuats(yt, (T*)nullptr);
+                    if (projInterp && M.nquat > 0 && (double(quatErrNorm(yt)) > consTol || forceProj))
+                        normalizeQuats(yt, (T*)nullptr);
                 }
             }
             t = t1;
