@@ -424,6 +424,17 @@ typedef struct tdyn_integrator_cfg {
                                     stored in FP32; 3 = as 2 with the joint trig cache in
                                     FP32 too.  1-3: chain evaluator of one model per batch
                                     only (TDYN_E_ARG otherwise; serving: 0 or 1) */
+    int32_t use_infinity_norm;   /* Integrator::setUseInfinityNorm (Integrator.cpp:276-280): nonzero makes
+                                    every error and constraint norm the largest magnitude instead of
+                                    the RMS.  FP64 only; runs on the tree evaluators (a chain batch then
+                                    takes the tree sweeps); the chain server refuses it */
+    int32_t project_every_step;  /* setProjectEveryStep (Integrator.cpp:288-290): nonzero forces the
+                                    projection (quaternion normalization, one Newton iteration) on every
+                                    attempt that reaches it.  Same limits as use_infinity_norm */
+    int32_t project_interpolated_states;   /* setProjectInterpolatedStates (Integrator.cpp:294-296):
+                                    0 (default) projects interpolated report and event-localization
+                                    states, -1 leaves them unprojected (the state backed up to an event
+                                    is projected either way) */
 } tdyn_integrator_cfg;
 
 /* Autotuner of the tree evaluators' occupancy (no Simbody counterpart: it

@@ -37,6 +37,10 @@ struct RungeKuttaMerson {
     RungeKuttaMerson& setInitialStepSize(double h) { cfg.initial_step = h; return *this; }
     RungeKuttaMerson& setMinimumStepSize(double h) { cfg.min_step = h; return *this; }
     RungeKuttaMerson& setMaximumStepSize(double h) { cfg.max_step = h; return *this; }
+    // Integrator.cpp:276-296 (FP64; the chain evaluator and chain server refuse the first two)
+    RungeKuttaMerson& setUseInfinityNorm(bool on) { cfg.use_infinity_norm = on ? 1 : 0; return *this; }
+    RungeKuttaMerson& setProjectEveryStep(bool on) { cfg.project_every_step = on ? 1 : 0; return *this; }
+    RungeKuttaMerson& setProjectInterpolatedStates(bool on) { cfg.project_interpolated_states = on ? 0 : -1; return *this; }
 };
 
 class MatterBatch {

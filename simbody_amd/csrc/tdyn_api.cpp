@@ -103,6 +103,11 @@ int make_rkm_cfg(const tdyn_integrator_cfg* cfg, RkmCfg& c, int& precision) {
     c.timescale = cfg->time_scale > 0 ? cfg->time_scale : 0.1;
     c.max_attempts = cfg->max_attempts > 0 ? cfg->max_attempts : 100000000;
     c.max_steps = 0;
+    c.inf_norm = cfg->use_infinity_norm ? 1 : 0;
+    c.force_proj = cfg->project_every_step ? 1 : 0;
+    c.proj_interp = cfg->project_interpolated_states < 0 ? 0 : 1;
+    if ((c.inf_norm || c.force_proj) && precision != 0)
+        return fail(TDYN_E_ARG, "the infinity norm and projection on every step run in FP64 (precision 0)");
     return TDYN_OK;
 }
 
@@ -834,6 +839,9 @@ static int rollout_begin_impl(tdyn_batch* b, const double* q, const double* u, c
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }
     if (prec != 0 && !(b->is_chain && b->evaluator != 1))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: precision 1-3 runs on the chain evaluator (one serial-chain model per batch)");
+    if ((c.inf_norm || c.force_proj) && b->evaluator == 2)
+        return fail(TDYN_E_ARG, "tdyn_rollout_begin: the infinity norm and projection on every step run on the tree "
+                                "evaluators (evaluator 0, 1 or 3), not the chain evaluator");
     if (b->known_p && (b->evaluator == 2 || (b->evaluator == 3 && b->tree_lanes == 1) || prec != 0))
         return fail(TDYN_E_ARG, "tdyn_rollout_begin: known udot runs on the tree sweeps or the level-parallel "
                                 "tree evaluator (not one lane per instance), in FP64");
@@ -1171,6 +1179,7 @@ int tdyn_rollout_reports(tdyn_batch* b, double* q, double* u, const double* fapp
             const double* now[4] = {ends[nx][0].p, ends[nx][1].p, ends[nx][2].p, ends[nx][3].p};
             HIPCHK(launch_report_interp(T, b->uniform ? nullptr : b->tree_of.p, b->q_off.p, b->u_off.p, L, rt.p,
                                         n_reports, next_k.p, tl[cur].p, b->t.p, prev, now, b->cfg.cons_tol,
+                                        proj_opts(b->cfg) | (b->cfg.proj_interp ? 0 : PJ_NONE),
                                         q_reports, u_reports, (long)TQ, (long)TU, s));
         }
         HIPCHK(hipMemcpyAsync(tl[nx].p, b->t.p, (size_t)L * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -151,7 +151,7 @@ struct tdyn_batch {
         a.fapp = fapp.p; a.vec = vec.p; a.ny_max = ny_max; a.t = t.p; a.h = h.p; a.t_end = t_end.p;
         a.status = status.p; a.stats = stats.p; a.uscale = uscale.p; a.cfg = cfg; a.precision = precision;
         a.scratch32 = scratch32.p;
-        a.chain = (!roll_known_p && !roll_motion && !any_cons && !any_wit &&
+        a.chain = (!roll_known_p && !roll_motion && !any_cons && !any_wit && !cfg.inf_norm && !cfg.force_proj &&
                    (roll_evaluator == 2 || (roll_evaluator == 0 && is_chain))) ? 1 : 0;
         a.cons_ws = any_cons ? cons_ws.p : nullptr;
         if (any_wit) { a.ev_ws = ev_ws.p; a.ev_win = ev_win.p; a.ev_ids = ev_ids.p; a.ev_count = ev_count.p; }

@@ -129,7 +129,26 @@ struct RkmCfg {
     double timescale;   // System default time scale (0.1)
     int max_steps;      // accepted-step budget for this call (per lane)
     int max_attempts;   // attempt budget per lane for the whole rollout
+    int inf_norm;       // Integrator::setUseInfinityNorm: every error and constraint norm the largest magnitude
+    int force_proj;     // setProjectEveryStep: ForceProjection on every projection
+    int proj_interp;    // setProjectInterpolatedStates (1, the default): project interpolated states
 };
+// Options of the device projections (cons_project_q/u): ForceProjection and
+// ForceFullNewton at initialize; ForceProjection (setProjectEveryStep); the
+// infinity norm (setUseInfinityNorm).
+enum { PJ_INIT = 1, PJ_FORCE = 2, PJ_INF = 4, PJ_NONE = 8 /* reports: interpolated states not projected */ };
+__host__ __device__ inline int proj_opts(const RkmCfg& c) {
+    return (c.inf_norm ? PJ_INF : 0) | (c.force_proj ? PJ_FORCE : 0);
+}
+// Accumulates x into a norm: sum of squares, or (inf) the largest magnitude,
+// a NaN kept.
+__host__ __device__ inline void norm_acc(double& s, double x, bool inf) {
+    if (inf) { const double a = fabs(x); s = (a > s || a != a) ? a : s; }
+    else s += x*x;
+}
+__host__ __device__ inline double norm_end(double s, int n, bool inf) {
+    return n ? (inf ? s : sqrt(s / n)) : 0.0;
+}
 
 // Launch entry points (tdyn_kernels.hip).
 enum OpKind { OP_KIN = 0, OP_FWD_SYS = 1, OP_FWD_GIVEN = 2, OP_MINV = 3, OP_RESID = 4, OP_ABI = 5, OP_REACT = 6,
@@ -322,7 +341,7 @@ hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, co
 hipError_t launch_report_interp(const DevTables& T, const int* tree_of, const int* q_off, const int* u_off, int L,
                                 const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
                                 const double* const prev[4], const double* const now[4], double cons_tol,
-                                double* q_out, double* u_out, long total_q, long total_u, hipStream_t s);
+                                int popt, double* q_out, double* u_out, long total_q, long total_u, hipStream_t s);
 // the event log (per lane) to instance order
 hipError_t launch_event_scatter(const int* inst, int n, int cap, const double* win, const int* ids, const int* cnt,
                                 double* win_out, int* ids_out, int* cnt_out, hipStream_t s);

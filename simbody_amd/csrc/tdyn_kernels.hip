@@ -1138,13 +1138,15 @@ __device__ void cons_mul_n(const Lane& c, const QG& q, const UR& du, const QR& d
         }
     }
 }
-__device__ __forceinline__ double cons_rms(const Cons& K, int off) {
+// Norm of the constraint errors at off: RMS, or the largest magnitude (inf).
+__device__ __forceinline__ double cons_norm(const Cons& K, int off, bool inf) {
     double sacc = 0;
-    for (int i = 0; i < K.m; ++i) sacc += K.w(off + i) * K.w(off + i);
-    return K.m ? sqrt(sacc / K.m) : 0.0;
+    for (int i = 0; i < K.m; ++i) norm_acc(sacc, K.w(off + i), inf);
+    return norm_end(sacc, K.m, inf);
 }
+// Norm of |quaternion| - 1 over the quaternions of q.
 template <class QG>
-__device__ double cons_quat_rms(const Lane& c, const QG& q, int nquat) {
+__device__ double cons_quat_norm(const Lane& c, const QG& q, int nquat, bool inf) {
     if (nquat == 0) return 0.0;
     double s2 = 0;
     for (int b = 1; b <= c.nb; ++b) {
@@ -1152,10 +1154,9 @@ __device__ double cons_quat_rms(const Lane& c, const QG& q, int nquat) {
         if (!body_quat(c, ba, b)) continue;
         double n2 = 0;
         for (int k = 0; k < 4; ++k) { const double x = q(ba.z + k); n2 += x*x; }
-        const double e = sqrt(n2) - 1.0;
-        s2 += e*e;
+        norm_acc(s2, sqrt(n2) - 1.0, inf);
     }
-    return sqrt(s2 / nquat);
+    return norm_end(s2, nquat, inf);
 }
 template <class QR, class ER>
 __device__ void cons_normalize(const Lane& c, const QR& q, const ER* e) {
@@ -1192,24 +1193,26 @@ __device__ bool cons_q_update(const Lane& c, const Cons& K, const QG& q, int r, 
     return true;
 }
 // projectQ (SimbodyMatterSubsystemRep.cpp:4096-4440) with the integrator's
-// options: LocalOnly unless init; ForceProjection and ForceFullNewton when
-// init.  q, u: the state rows (u only feeds the kinematics); e (nullable):
+// options (opt, PJ_* bits): LocalOnly unless PJ_INIT; ForceProjection and
+// ForceFullNewton with PJ_INIT; ForceProjection with PJ_FORCE; every norm
+// the infinity norm with PJ_INF.  q, u: the state rows (u only feeds the kinematics); e (nullable):
 // the q error estimate; sv (q rows), du (u rows), dq (q rows): scratch.  The
 // error-estimate projection uses Pq at the final q (the reference keeps the
 // factorization of the last iteration's starting q).
 template <bool PAD, class QR, class UR, class ER, class SR, class DUR, class DQR>
 __device__ bool cons_project_q(const Lane& c, const Cons& K, const QR& q, const UR& u, const ER* e, const SR& sv,
                                const DUR& du, const DQR& dq, int nq, int nquat, double consTol, double projLimit,
-                               bool init, bool* changed) {
+                               int opt, bool* changed) {
+    const bool init = opt & PJ_INIT, force = opt & (PJ_INIT | PJ_FORCE), inf = opt & PJ_INF;
     *changed = false;
     sweep_kin<false, PAD>(c, q, u, NoPut{});
     cons_perr(c, K, CWS_R);
-    const double pn0 = cons_rms(K, CWS_R), qn0 = cons_quat_rms(c, q, nquat);
+    const double pn0 = cons_norm(K, CWS_R, inf), qn0 = cons_quat_norm(c, q, nquat, inf);
     if (fmax(pn0, qn0) > projLimit || pn0 != pn0) return false;
-    if (pn0 == 0 || (pn0 <= consTol && !init)) {
-        if (qn0 > consTol || init) {
+    if (pn0 == 0 || (pn0 <= consTol && !force)) {
+        if (qn0 > consTol || force) {
             if (nquat) { cons_normalize(c, q, e); *changed = true; }
-            if (cons_quat_rms(c, q, nquat) > consTol) return false;
+            if (cons_quat_norm(c, q, nquat, inf) > consTol) return false;
         }
         return true;
     }
@@ -1223,12 +1226,12 @@ __device__ bool cons_project_q(const Lane& c, const Cons& K, const QR& q, const 
         *changed = true; ++its;
         sweep_kin<false, PAD>(c, q, u, NoPut{});
         cons_perr(c, K, CWS_R);
-        pn = cons_rms(K, CWS_R);
+        pn = cons_norm(K, CWS_R, inf);
         if (!init && its >= 2 && pn > prev) {
             for (int j = 0; j < nq; ++j) q(j) = q(j) + dq(j);
             sweep_kin<false, PAD>(c, q, u, NoPut{});
             cons_perr(c, K, CWS_R);
-            pn = cons_rms(K, CWS_R);
+            pn = cons_norm(K, CWS_R, inf);
             break;
         }
         prev = pn;
@@ -1263,7 +1266,7 @@ __device__ bool cons_project_q(const Lane& c, const Cons& K, const QR& q, const 
     }
     if (nquat) {
         cons_normalize(c, q, e);
-        if (cons_quat_rms(c, q, nquat) > consTol) return false;
+        if (cons_quat_norm(c, q, nquat, inf) > consTol) return false;
     }
     return true;
 }
@@ -1273,13 +1276,14 @@ __device__ bool cons_project_q(const Lane& c, const Cons& K, const QR& q, const 
 // the u error estimate; sv, du (u rows): scratch.
 template <bool PAD, class QR, class UR, class ER, class SR, class DUR>
 __device__ bool cons_project_u(const Lane& c, const Cons& K, const QR& q, const UR& u, const ER* e, const SR& sv,
-                               const DUR& du, int nu, double consTol, double projLimit, bool init, bool* changed) {
+                               const DUR& du, int nu, double consTol, double projLimit, int opt, bool* changed) {
+    const bool init = opt & PJ_INIT, force = opt & (PJ_INIT | PJ_FORCE), inf = opt & PJ_INF;
     *changed = false;
     sweep_kin<true, PAD>(c, q, u, NoPut{});
     cons_rows(c, K, REC_V, false, CWS_R, 1);
-    const double vn0 = cons_rms(K, CWS_R);
+    const double vn0 = cons_norm(K, CWS_R, inf);
     if (vn0 > projLimit || vn0 != vn0) return false;
-    if (vn0 == 0 || (vn0 <= consTol && !init)) return true;
+    if (vn0 == 0 || (vn0 <= consTol && !force)) return true;
     const double target = fmax(0.1*consTol, 1.9323057602939027e-14);
     for (int j = 0; j < nu; ++j) sv(j) = u(j);
     auto s2 = [&](int j) { const double a = fabs(sv(j)); const double sc = a > 1.0 ? a : 1.0; return sc*sc; };
@@ -1304,12 +1308,12 @@ __device__ bool cons_project_u(const Lane& c, const Cons& K, const QR& q, const 
         *changed = true; ++its;
         sweep_kin<true, PAD>(c, q, u, NoPut{});
         cons_rows(c, K, REC_V, false, CWS_R, 1);
-        vn = cons_rms(K, CWS_R);
+        vn = cons_norm(K, CWS_R, inf);
         if (!init && its >= 2 && vn > prev) {
             for (int j = 0; j < nu; ++j) u(j) = u(j) + du(j);
             sweep_kin<true, PAD>(c, q, u, NoPut{});
             cons_rows(c, K, REC_V, false, CWS_R, 1);
-            vn = cons_rms(K, CWS_R);
+            vn = cons_norm(K, CWS_R, inf);
             break;
         }
         prev = vn;
@@ -2055,24 +2059,29 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
     // the interpolant at tm of (t, Y, F0) and (te, Y1, F1) into out, projected
     // (realizeAndProjectKinematicsWithThrow, IntegratorRep.h:858-897: with
     // constraints projectQ then projectU at the tolerance, no limit, no
-    // error estimate, FA, YS and YE as scratch rows); false when that fails
-    auto interp = [&](double te, double tm, const TRows& out) -> bool {
+    // error estimate, FA, YS and YE as scratch rows); false when that fails.
+    // Under setProjectInterpolatedStates(false) only the state backed up to
+    // the event (back_up) is projected (AbstractIntegratorRep.cpp:78-85,
+    // 116-126).
+    const bool inf = a.cfg.inf_norm, force = a.cfg.force_proj;
+    auto interp = [&](double te, double tm, const TRows& out, bool back_up) -> bool {
         const double h = te - t, d = (tm - t) / (te - t);
         const double cy1 = d*d*(3.0 - 2.0*d), cy0 = 1.0 - cy1;
         const double hdd1 = h*d*(d - 1.0), cf1 = hdd1*d, cf0 = cf1 - hdd1;
         for (int k = 0; k < ny; ++k) out(k) = ((cy0*Y(k) + cy1*Y1(k)) + cf0*F0(k)) + cf1*F1(k);
         if constexpr (EV::kMotion) { if (a.motion) prescribe_lane(a, lane, tr, tm, out, true); }
+        if (!back_up && !a.cfg.proj_interp) return true;
         if constexpr (EV::kCons) {
             if (ev.K.m > 0) {
                 const TRows ou{out.base + (size_t)nq * out.L, out.L, out.lane};
                 const TRows su{YS.base + (size_t)nq * YS.L, YS.L, YS.lane};
                 const TRows du{FA.base + (size_t)nq * FA.L, FA.L, FA.lane};
-                const double inf = __builtin_inf();
+                const double nolimit = __builtin_inf();
                 bool ch;
                 return cons_project_q<EV::kPad>(ev.c, ev.K, out, ou, (const TRows*)nullptr, YS, du, YE, nq, nquat,
-                                                consTol, inf, false, &ch) &&
-                       cons_project_u<EV::kPad>(ev.c, ev.K, out, ou, (const TRows*)nullptr, su, du, nu, consTol, inf,
-                                                false, &ch);
+                                                consTol, nolimit, proj_opts(a.cfg), &ch) &&
+                       cons_project_u<EV::kPad>(ev.c, ev.K, out, ou, (const TRows*)nullptr, su, du, nu, consTol,
+                                                nolimit, proj_opts(a.cfg), &ch);
             }
         }
         if (nquat == 0) return true;
@@ -2082,10 +2091,9 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             if (ba.y != BALL && (ba.y != FREE || prm_tree[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
             double n2 = 0;
             for (int k = 0; k < 4; ++k) { const double x = out(ba.z + k); n2 += x*x; }
-            const double e = sqrt(n2) - 1.0;
-            s2 += e*e;
+            norm_acc(s2, sqrt(n2) - 1.0, inf);
         }
-        if (!(sqrt(s2 / nquat) > consTol)) return true;
+        if (!force && !(norm_end(s2, nquat, inf) > consTol)) return true;
         for (int b = 0; b < nb; ++b) {
             const int4 ba = ldi4(body_a, b);
             if (ba.y != BALL && (ba.y != FREE || prm_tree[(size_t)b * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
@@ -2111,7 +2119,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             if (++it > 400) return -1;
             if (side2 != 0 && side1 != 0) bias = side2 != side1 ? 1.0 : (side1 < 0 ? bias/2 : bias*2);
             const double tMid = earliest;
-            if (!interp(t1o, tMid, YT)) return -1;
+            if (!interp(t1o, tMid, YT, false)) return -1;
             wit_values<EV::kPad>(ev.c, W, tMid, YT, mid);
             unsigned r2;
             double e2, n2;
@@ -2130,7 +2138,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
             cand = nc; rising = r2; earliest = e2; narrowest = n2;
         } while (tHigh - tLow > narrowest);
         if (tHigh < t1o) {
-            if (!interp(t1o, tHigh, YT)) return -1;
+            if (!interp(t1o, tHigh, YT, true)) return -1;
             for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
             t1 = tHigh;
         }
@@ -2159,7 +2167,7 @@ __device__ int step_events(const RkmArgs& a, const EV& ev, const Wit& W, int lan
     a.ev_count[lane] = cnt;
     if (term && !modified) {
         end_derivs(t1);
-        if (!interp(t1, tLow, YT)) return -1;
+        if (!interp(t1, tLow, YT, false)) return -1;
         for (int k = 0; k < ny; ++k) Y1(k) = YT(k);
         t1 = tLow;
     }
@@ -2210,7 +2218,9 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
     }
 
     // error norm at y1: q part weighted by N NInv for quaternions (IntegratorRep.h:
-    // 454-523), u part relative to y0 (calcRelativeScaling, :644-655)
+    // 454-523), u part relative to y0 (calcRelativeScaling, :644-655); RMS,
+    // or the largest magnitude under setUseInfinityNorm
+    const bool infn = cfg.inf_norm;
     auto errNorm = [&]() -> double {
         double sq = 0;
         for (int b = 0; b < nb; ++b) {
@@ -2224,22 +2234,22 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                 for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
                 const d3 w = quat_dot_to_w(qq, e);
                 quat_dot(qq, w, d4);
-                sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
-                if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = YE(qi + k); sq += x*x; }
+                if (infn) for (int k = 0; k < 4; ++k) norm_acc(sq, d4[k], true);
+                else sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                if (ba.y == FREE) for (int k = 4; k < 7; ++k) norm_acc(sq, YE(qi + k), infn);
             } else {
-                for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = YE(qi + k); sq += x*x; }
+                for (int k = 0; k < mob_dof(ba.y); ++k) norm_acc(sq, YE(qi + k), infn);
             }
         }
-        const double qn = nq ? sqrt(sq / nq) : 0.0;
+        const double qn = norm_end(sq, nq, infn);
         double su = 0;
 #pragma unroll 8
         for (int k = 0; k < nu; ++k) {
             const double vi = fabs(Y(nq + k));
             const double us = vi > 1.0 ? 1.0/vi : 1.0;
-            const double x = us * YE(nq + k);
-            su += x*x;
+            norm_acc(su, us * YE(nq + k), infn);
         }
-        const double un = nu ? sqrt(su / nu) : 0.0;
+        const double un = norm_end(su, nu, infn);
         return qn >= un ? qn : un;
     };
 
@@ -2320,9 +2330,9 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                         const TRows YSu{YS.base + (size_t)nq * L, L, YS.lane}, YTu{YT.base + (size_t)nq * L, L, YT.lane};
                         bool chq = false, chu = false;
                         if (!cons_project_q<EV::kPad>(ev.c, ev.K, Y1, Y1u, &YE, YS, YTu, YT, nq, nquat, consTol,
-                                                      projLimit, false, &chq) ||
+                                                      projLimit, proj_opts(cfg), &chq) ||
                             !cons_project_u<EV::kPad>(ev.c, ev.K, Y1, Y1u, &YEu, YSu, YTu, nu, consTol, projLimit,
-                                                      false, &chu))
+                                                      proj_opts(cfg), &chu))
                             converged = false;
                         if (chq) ++projections;
                     }
@@ -2335,12 +2345,11 @@ __global__ void __launch_bounds__(64) k_rkm(RkmArgs a) {
                     if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                     double n2 = 0;
                     for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
-                    const double e = sqrt(n2) - 1.0;
-                    s2 += e*e;
+                    norm_acc(s2, sqrt(n2) - 1.0, infn);
                 }
-                const double qe = sqrt(s2 / nquat);
+                const double qe = norm_end(s2, nquat, infn);
                 if (qe > projLimit) converged = false;
-                else if (qe > consTol) {
+                else if (qe > consTol || cfg.force_proj) {
                     ++projections;
                     for (int b = 0; b < nb; ++b) {
                         const int4 ba = ldi4(body_a, b);
@@ -2661,9 +2670,11 @@ __global__ void __launch_bounds__(64) k_cons_init(RkmArgs a) {
                 YTu{YT.base + (size_t)nq * a.L, a.L, YT.lane};
     const double inf = __builtin_inf();
     bool ch = false;
-    if (!cons_project_q<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YS, YTu, YT, nq, nquat, a.cfg.cons_tol, inf, true,
+    if (!cons_project_q<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YS, YTu, YT, nq, nquat, a.cfg.cons_tol, inf,
+                             PJ_INIT | proj_opts(a.cfg),
                              &ch) ||
-        !cons_project_u<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YSu, YTu, nu, a.cfg.cons_tol, inf, true, &ch))
+        !cons_project_u<PAD>(c, K, Y, Yu, (const TRows*)nullptr, YSu, YTu, nu, a.cfg.cons_tol, inf,
+                             PJ_INIT | proj_opts(a.cfg), &ch))
         a.status[lane] = -4;
 }
 hipError_t launch_cons_init(const RkmArgs& a, hipStream_t s) {
@@ -2744,15 +2755,19 @@ hipError_t launch_rollout_extend(int n, const double* te_lane, double te_all, co
 // if r == t_now, else the cubic Hermite interpolant (interpolateOrder3,
 // IntegratorRep.h:285-313) of the step's two ends, then projectQ of the
 // interpolated state: its quaternions normalized when their RMS norm error
-// exceeds cons_tol (SimbodyMatterSubsystemRep.cpp:4151-4160).  Packed
+// exceeds cons_tol (SimbodyMatterSubsystemRep.cpp:4151-4160); popt: PJ_INF
+// (the infinity norm), PJ_FORCE (always normalized), PJ_NONE (interpolated
+// states not projected, setProjectInterpolatedStates(false)).  Packed
 // instance-order states and derivatives; next_k: the lane's next report.
 __global__ void k_report_interp(DevTables T, const int* tree_of, const int* q_off, const int* u_off, int L,
                                 const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
                                 const double* qp, const double* up, const double* fqp, const double* fup,
                                 const double* qn, const double* un, const double* fqn, const double* fun,
-                                double cons_tol, double* q_out, double* u_out, long total_q, long total_u) {
+                                double cons_tol, int popt, double* q_out, double* u_out, long total_q,
+                                long total_u) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= L) return;
+    const bool inf = popt & PJ_INF;
     const double t0 = t_prev[l], t1 = t_now[l];
     int k = next_k[l];
     if (!(t1 > t0) || k >= nrep || rt[k] > t1) return;
@@ -2775,7 +2790,7 @@ __global__ void k_report_interp(DevTables T, const int* tree_of, const int* q_of
             oq[j] = ((cy0*qp[qo + j] + cy1*qn[qo + j]) + cf0*fqp[qo + j]) + cf1*fqn[qo + j];
         for (int j = 0; j < nu; ++j)
             ou[j] = ((cy0*up[uo + j] + cy1*un[uo + j]) + cf0*fup[uo + j]) + cf1*fun[uo + j];
-        if (nquat > 0) {
+        if (nquat > 0 && !(popt & PJ_NONE)) {
             auto quat = [&](int b) {
                 const int4 ba = ldi4(T.body_a, ta.w + b);
                 return (ba.y == BALL || ba.y == FREE) && T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] == 0.0;
@@ -2784,10 +2799,9 @@ __global__ void k_report_interp(DevTables T, const int* tree_of, const int* q_of
             for (int b = 0; b < nb; ++b) {
                 if (!quat(b)) continue;
                 const double* qq = oq + ldi4(T.body_a, ta.w + b).z;
-                const double e = sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - 1.0;
-                s2 += e*e;
+                norm_acc(s2, sqrt(qq[0]*qq[0] + qq[1]*qq[1] + qq[2]*qq[2] + qq[3]*qq[3]) - 1.0, inf);
             }
-            if (sqrt(s2 / nquat) > cons_tol) {
+            if ((popt & PJ_FORCE) || norm_end(s2, nquat, inf) > cons_tol) {
                 for (int b = 0; b < nb; ++b) {
                     if (!quat(b)) continue;
                     double* qq = oq + ldi4(T.body_a, ta.w + b).z;
@@ -2802,10 +2816,10 @@ __global__ void k_report_interp(DevTables T, const int* tree_of, const int* q_of
 hipError_t launch_report_interp(const DevTables& T, const int* tree_of, const int* q_off, const int* u_off, int L,
                                 const double* rt, int nrep, int* next_k, const double* t_prev, const double* t_now,
                                 const double* const prev[4], const double* const now[4], double cons_tol,
-                                double* q_out, double* u_out, long total_q, long total_u, hipStream_t s) {
+                                int popt, double* q_out, double* u_out, long total_q, long total_u, hipStream_t s) {
     if (L <= 0) return hipSuccess;
     k_report_interp<<<dim3((L + 127)/128), dim3(128), 0, s>>>(T, tree_of, q_off, u_off, L, rt, nrep, next_k, t_prev,
-        t_now, prev[0], prev[1], prev[2], prev[3], now[0], now[1], now[2], now[3], cons_tol, q_out, u_out,
+        t_now, prev[0], prev[1], prev[2], prev[3], now[0], now[1], now[2], now[3], cons_tol, popt, q_out, u_out,
         total_q, total_u);
     return hipGetLastError();
 }

@@ -74,6 +74,9 @@ int tdyn_server_create(const tdyn_tree_desc* topology, const tdyn_integrator_cfg
     int prec = 0;
     { const int rc = make_rkm_cfg(cfg, c, prec); if (rc) return rc; }
     if (prec > 1) return fail(TDYN_E_ARG, "tdyn_server_create: serving runs precision 0 (FP64) or 1 (mixed)");
+    if (c.inf_norm || c.force_proj)
+        return fail(TDYN_E_ARG, "tdyn_server_create: the chain slots run the RMS norm and the default projection "
+                                "(the tree server takes the infinity norm and projection on every step)");
     int nq = 0, nu = 0, rec = 0;
     for (int b = 1; b <= nb; ++b) {
         const int ty = topology->mobilizer[b-1];

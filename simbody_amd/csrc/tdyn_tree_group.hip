@@ -50,6 +50,13 @@ __device__ __forceinline__ double grp_sum(int G, double x) {
     for (int m = 1; m < G; m <<= 1) x += __shfl_xor(x, m, 64);
     return x;
 }
+// The group's combined norm accumulator: the sum of squares, or (inf) the
+// largest magnitude with a NaN kept (norm_acc's rule, lane against lane).
+__device__ __forceinline__ double grp_norm(int G, double x, bool inf) {
+    if (!inf) return grp_sum(G, x);
+    for (int m = 1; m < G; m <<= 1) { const double y = __shfl_xor(x, m, 64); x = (y > x || y != y) ? y : x; }
+    return x;
+}
 
 template <int V> using TyTag = std::integral_constant<int, V>;
 
@@ -313,7 +320,9 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
         projections = a.stats[4*inst+3];
     int status = 0, steps = 0;
 
-    // error norm at y1 (IntegratorRep.h:454-523, calcRelativeScaling :644-655)
+    // error norm at y1 (IntegratorRep.h:454-523, calcRelativeScaling :644-655),
+    // RMS or (setUseInfinityNorm) the largest magnitude
+    const bool infn = cfg.inf_norm;
     auto errNorm = [&]() -> double {
         double sq = 0;
         for (int b = l; b < nb; b += G) {
@@ -327,22 +336,22 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
                 for (int k = 0; k < 4; ++k) { qq[k] = Y1(qi + k); e[k] = YE(qi + k); }
                 const d3 w = quat_dot_to_w(qq, e);
                 quat_dot(qq, w, d4);
-                sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
-                if (ba.y == FREE) for (int k = 4; k < 7; ++k) { const double x = YE(qi + k); sq += x*x; }
+                if (infn) for (int k = 0; k < 4; ++k) norm_acc(sq, d4[k], true);
+                else sq += d4[0]*d4[0] + d4[1]*d4[1] + d4[2]*d4[2] + d4[3]*d4[3];
+                if (ba.y == FREE) for (int k = 4; k < 7; ++k) norm_acc(sq, YE(qi + k), infn);
             } else {
-                for (int k = 0; k < mob_dof(ba.y); ++k) { const double x = YE(qi + k); sq += x*x; }
+                for (int k = 0; k < mob_dof(ba.y); ++k) norm_acc(sq, YE(qi + k), infn);
             }
         }
         double su = 0;
         for (int k = l; k < nu; k += G) {
             const double vi = fabs(Y(nq + k));
             const double us = vi > 1.0 ? 1.0/vi : 1.0;
-            const double x = us * YE(nq + k);
-            su += x*x;
+            norm_acc(su, us * YE(nq + k), infn);
         }
-        sq = grp_sum(G, sq); su = grp_sum(G, su);
-        const double qn = nq ? sqrt(sq / nq) : 0.0;
-        const double un = nu ? sqrt(su / nu) : 0.0;
+        sq = grp_norm(G, sq, infn); su = grp_norm(G, su, infn);
+        const double qn = norm_end(sq, nq, infn);
+        const double un = norm_end(su, nu, infn);
         return qn >= un ? qn : un;
     };
 
@@ -394,13 +403,12 @@ __device__ __forceinline__ void grp_instance(const RkmArgs& a, int inst, int l, 
                     if (ba.y != BALL && (ba.y != FREE || a.T.prm[(size_t)(ta.w + b) * PRM_STRIDE + PRM_EULER] != 0.0)) continue;
                     double n2 = 0;
                     for (int k = 0; k < 4; ++k) { const double x = Y1(ba.z + k); n2 += x*x; }
-                    const double e = sqrt(n2) - 1.0;
-                    s2 += e*e;
+                    norm_acc(s2, sqrt(n2) - 1.0, infn);
                 }
-                s2 = grp_sum(G, s2);
-                const double qe = sqrt(s2 / nquat);
+                s2 = grp_norm(G, s2, infn);
+                const double qe = norm_end(s2, nquat, infn);
                 if (qe > projLimit) converged = false;
-                else if (qe > consTol) {
+                else if (qe > consTol || cfg.force_proj) {
                     ++projections;
                     for (int b = l; b < nb; b += G) {
                         const int4 ba = ldi4(body_a, b);

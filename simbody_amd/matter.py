@@ -99,7 +99,8 @@ class IntegratorCfgC(ctypes.Structure):
                 ("fixed_step", ctypes.c_double), ("initial_step", ctypes.c_double),
                 ("min_step", ctypes.c_double), ("max_step", ctypes.c_double),
                 ("time_scale", ctypes.c_double), ("max_attempts", ctypes.c_int32),
-                ("precision", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("use_infinity_norm", ctypes.c_int32),
+                ("project_every_step", ctypes.c_int32), ("project_interpolated_states", ctypes.c_int32)]
 
 
 _lib = None
@@ -239,17 +240,32 @@ class RungeKuttaMerson:
     FP64, MIXED = 0, 1
 
     def __init__(self, accuracy=1e-3, fixed_step=0.0, constraint_tolerance=0.0, initial_step=0.0,
-                 min_step=0.0, max_step=0.0, time_scale=0.1, max_attempts=0, precision=0):
+                 min_step=0.0, max_step=0.0, time_scale=0.1, max_attempts=0, precision=0,
+                 use_infinity_norm=False, project_every_step=False, project_interpolated_states=True):
         self.accuracy, self.fixed_step = accuracy, fixed_step
         self.constraint_tolerance, self.initial_step = constraint_tolerance, initial_step
         self.min_step, self.max_step, self.time_scale = min_step, max_step, time_scale
         self.max_attempts = max_attempts
         self.precision = precision   # 0 FP64, 1 mixed (chain evaluator)
+        # Integrator::setUseInfinityNorm, setProjectEveryStep, setProjectInterpolatedStates
+        self.use_infinity_norm = use_infinity_norm
+        self.project_every_step = project_every_step
+        self.project_interpolated_states = project_interpolated_states
+
+    def setUseInfinityNorm(self, on):
+        self.use_infinity_norm = bool(on)
+
+    def setProjectEveryStep(self, on):
+        self.project_every_step = bool(on)
+
+    def setProjectInterpolatedStates(self, on):
+        self.project_interpolated_states = bool(on)
 
     def c(self):
         return IntegratorCfgC(self.accuracy, self.constraint_tolerance, self.fixed_step,
                               self.initial_step, self.min_step, self.max_step, self.time_scale,
-                              self.max_attempts, self.precision)
+                              self.max_attempts, self.precision, int(bool(self.use_infinity_norm)),
+                              int(bool(self.project_every_step)), 0 if self.project_interpolated_states else -1)
 
 
 class MatterBatch:
